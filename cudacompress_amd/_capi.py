@@ -1,0 +1,95 @@
+"""ctypes binding of the C ABI in include/cudacompress_amd.h.
+
+Loads cudacompress_amd/lib/libcudacompress_amd.so built in-tree; there is no fallback: if the
+library is missing the import fails with an error naming the build command.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libcudacompress_amd.so")
+
+c_u32 = ctypes.c_uint32
+c_u64 = ctypes.c_uint64
+c_int = ctypes.c_int
+c_float = ctypes.c_float
+c_vp = ctypes.c_void_p
+P_u32 = ctypes.POINTER(c_u32)
+P_u64 = ctypes.POINTER(c_u64)
+PP_u32 = ctypes.POINTER(P_u32)
+PP_u64 = ctypes.POINTER(P_u64)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "cc_status_string": (ctypes.c_char_p, [c_int]),
+    "cc_last_error": (ctypes.c_char_p, []),
+    "cc_create_instance": (c_int, [c_int, c_u32, c_u32, c_u32, c_u32, ctypes.POINTER(c_vp)]),
+    "cc_destroy_instance": (None, [c_vp]),
+    "cc_get_instance_device": (c_int, [c_vp]),
+    "cc_get_instance_stream_count_max": (c_u32, [c_vp]),
+    "cc_get_instance_elem_count_per_stream_max": (c_u32, [c_vp]),
+    "cc_get_instance_coding_block_size": (c_u32, [c_vp]),
+    "cc_get_instance_log2_huffman_distinct_symbol_count_max": (c_u32, [c_vp]),
+    "cc_get_instance_use_long_symbols": (c_int, [c_vp]),
+    "cc_set_instance_stream": (None, [c_vp, c_vp]),
+    "cc_get_instance_stream": (c_vp, [c_vp]),
+    "cc_set_instance_dialect": (c_int, [c_vp, c_int]),
+    "cc_encode_rlhuff": (c_int, [c_vp, c_int, c_vp, c_u32, c_u32, c_vp, PP_u32, PP_u64, PP_u64]),
+    "cc_encode_huff": (c_int, [c_vp, c_int, c_vp, c_u32, c_u32, c_vp, PP_u32, PP_u64, PP_u64]),
+    "cc_decode_rlhuff": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp, c_u32, c_vp, c_vp, c_u32, c_u32]),
+    "cc_decode_huff": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp, c_u32, c_vp, c_vp, c_u32, c_u32]),
+    "cc_dwt_float_forward": (c_int, [c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_vp]),
+    "cc_dwt_float_inverse": (c_int, [c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_vp]),
+    "cc_dwt_float_inverse_from_symbols": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_float, c_int, c_int, c_int,
+                                                   c_int, c_int, c_int, c_int, c_int, c_vp]),
+    "cc_dwt_int_forward": (c_int, [c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_vp]),
+    "cc_dwt_int_inverse": (c_int, [c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_vp]),
+    "cc_symbolize": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, c_u32, c_vp]),
+    "cc_unsymbolize": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, c_u32, c_vp]),
+    "cc_quantize_to_symbols": (c_int, [c_vp, c_int, c_vp, c_u32, c_u32, c_u32, c_float, c_u32, c_u32, c_int, c_vp]),
+    "cc_quantize_to_symbols_roundtrip": (c_int, [c_vp, c_int, c_vp, c_u32, c_u32, c_u32, c_float, c_u32, c_u32, c_int, c_vp]),
+    "cc_unquantize_from_symbols": (c_int, [c_vp, c_vp, c_int, c_u32, c_u32, c_u32, c_float, c_u32, c_u32, c_int, c_vp]),
+    "cc_quantize_to_short": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_float, c_u32, c_u32, c_int, c_vp]),
+    "cc_unquantize_from_short": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_float, c_u32, c_u32, c_int, c_vp]),
+    "cc_get_max_abs": (c_int, [c_vp, c_vp, c_u32, c_vp]),
+    "cc_compress_volume_float": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, c_float, c_int, c_int, PP_u32, P_u64]),
+    "cc_decompress_volume_float": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, c_vp, c_u64, c_float, c_int, c_int]),
+    "cc_compress_volume_lossless": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, PP_u32, P_u64]),
+    "cc_decompress_volume_lossless": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, c_vp, c_u64]),
+}
+
+_lib = None
+
+
+def _share_hip_runtime_with_torch():
+    """One HIP runtime per process.  PyTorch-ROCm wheels bundle their own libamdhip64.so (same
+    SONAME, libamdhip64.so.7, as /opt/rocm's).  Loading that file first, by path, makes this
+    library bind to it through the SONAME and lets a later `import torch` find it by inode, so
+    torch tensors and this library share one runtime whichever is imported first.  Two runtimes
+    in one process do not see the GPU.  No-op when torch is not installed."""
+    import importlib.util
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.origin:
+        return
+    path = os.path.join(os.path.dirname(spec.origin), "lib", "libamdhip64.so")
+    if os.path.exists(path):
+        ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+
+
+def load():
+    """Loads the library (once).  Raises ImportError if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"cudacompress_amd native library not found at {LIB_PATH}; build it with "
+            "`python -m cudacompress_amd.build` (hipcc, gfx950)")
+    _share_hip_runtime_with_torch()
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib

@@ -1,0 +1,521 @@
+"""Python mirror of the cudaCompress C++ API over the C ABI (include/cudacompress_amd.h).
+
+Names, argument order and meaning follow the reference headers (include/cudaCompress/Instance.h,
+Encode.h, BitStream.h, util/DWT.h, util/Quantize.h).  Device arrays are torch tensors (or raw
+device addresses as ints); streams are torch.cuda.Stream objects, raw hipStream_t ints, or None
+(the null stream).  Encode/decode return False on symbol overflow, as the reference does, and
+raise CCError for every other failure.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _capi
+
+CC_OK = 0
+CC_ERR_SYMBOL_OVERFLOW = 1
+CC_ERR_OUT_OF_MEMORY = 2
+CC_ERR_INVALID_ARG = 3
+CC_ERR_DEVICE = 4
+CC_ERR_CODEWORD_TOO_LONG = 5
+CC_ERR_CORRUPT_STREAM = 6
+
+DIALECT_GPU = 0
+DIALECT_CPU = 1
+
+QUANTIZE_DEADZONE = 0
+QUANTIZE_UNIFORM = 1
+
+
+class CCError(RuntimeError):
+    def __init__(self, status, where=""):
+        lib = _capi.load()
+        msg = lib.cc_status_string(status).decode()
+        detail = lib.cc_last_error().decode()
+        super().__init__(f"{where}: {msg}" + (f" ({detail})" if detail else ""))
+        self.status = status
+
+
+def _check(status, where):
+    if status != CC_OK:
+        raise CCError(status, where)
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+def _stream(s):
+    if s is None:
+        return None
+    if isinstance(s, int):
+        return s
+    return s.cuda_stream
+
+
+def _elem_size(t):
+    return t.element_size() if hasattr(t, "element_size") else None
+
+
+# ------------------------------------------------------------------------------------ Instance
+class Instance:
+    """cudaCompress::Instance (include/cudaCompress/Instance.h:14-45)."""
+
+    def __init__(self, handle):
+        self._h = ctypes.c_void_p(handle)
+        self._stream = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            _capi.load().cc_destroy_instance(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # getInstance* (Instance.h:36-41)
+    @property
+    def cudaDevice(self):
+        return _capi.load().cc_get_instance_device(self._h)
+
+    @property
+    def streamCountMax(self):
+        return _capi.load().cc_get_instance_stream_count_max(self._h)
+
+    @property
+    def elemCountPerStreamMax(self):
+        return _capi.load().cc_get_instance_elem_count_per_stream_max(self._h)
+
+    @property
+    def codingBlockSize(self):
+        return _capi.load().cc_get_instance_coding_block_size(self._h)
+
+    @property
+    def log2HuffmanDistinctSymbolCountMax(self):
+        return _capi.load().cc_get_instance_log2_huffman_distinct_symbol_count_max(self._h)
+
+    @property
+    def useLongSymbols(self):
+        return bool(_capi.load().cc_get_instance_use_long_symbols(self._h))
+
+    @property
+    def symbolSize(self):
+        return 4 if self.useLongSymbols else 2
+
+    def setStream(self, stream):
+        """setInstanceCudaStream (Instance.h:44)."""
+        self._stream = stream
+        _capi.load().cc_set_instance_stream(self._h, _stream(stream))
+
+    def getStream(self):
+        return _capi.load().cc_get_instance_stream(self._h)
+
+    def setDialect(self, dialect):
+        _check(_capi.load().cc_set_instance_dialect(self._h, dialect), "setDialect")
+
+
+def createInstance(cudaDevice=-1, streamCountMax=1, elemCountPerStreamMax=1, codingBlockSize=0,
+                   log2HuffmanDistinctSymbolCountMax=0):
+    """createInstance (Instance.h:26-31).  Raises CCError instead of returning nullptr."""
+    lib = _capi.load()
+    h = ctypes.c_void_p()
+    _check(lib.cc_create_instance(cudaDevice, streamCountMax, elemCountPerStreamMax, codingBlockSize,
+                                  log2HuffmanDistinctSymbolCountMax, ctypes.byref(h)), "createInstance")
+    return Instance(h.value)
+
+
+def destroyInstance(inst):
+    inst.close()
+
+
+# ------------------------------------------------------------------------------------ BitStream
+class BitStream:
+    """Host bit stream with the semantics of include/cudaCompress/BitStream.h.
+
+    Words are uint32, bits are written MSB-first; aligned reads/writes round the position up to
+    a 32-bit word first and afterwards.
+    """
+
+    def __init__(self, words=None, bit_size=None):
+        if words is None:
+            self._w = np.zeros(0, dtype=np.uint32)
+            self._size = 0
+        else:
+            self._w = np.ascontiguousarray(words, dtype=np.uint32).copy()
+            self._size = int(self._w.size * 32 if bit_size is None else bit_size)
+        self._pos = 0
+
+    # -- queries
+    def getBitPosition(self):
+        return self._pos
+
+    def setBitPosition(self, pos):
+        if pos > self._size:
+            self._pos = self._size
+            return False
+        self._pos = int(pos)
+        return True
+
+    def getBitSize(self):
+        return self._size
+
+    def getRaw(self):
+        n = (self._size + 31) // 32
+        return self._w[:n]
+
+    def getRawSizeUInts(self):
+        return (self._size + 31) // 32
+
+    def words(self):
+        return self.getRaw().copy()
+
+    # -- writing
+    def _grow(self, bits):
+        need = (bits + 31) // 32
+        if need > self._w.size:
+            nw = np.zeros(max(need, 2 * self._w.size), dtype=np.uint32)
+            nw[: self._w.size] = self._w
+            self._w = nw
+        if bits > self._size:
+            self._size = bits
+
+    def writeBits(self, data, count):
+        data = int(data) & ((1 << count) - 1) if count < 32 else int(data) & 0xFFFFFFFF
+        self._grow(self._pos + count)
+        for i in range(count):
+            bit = (data >> (count - 1 - i)) & 1
+            p = self._pos + i
+            sh = 31 - (p % 32)
+            w = int(self._w[p // 32])
+            self._w[p // 32] = (w & ~(1 << sh) & 0xFFFFFFFF) | (bit << sh)
+        self._pos += count
+
+    def writeAlign(self, nbytes=4):
+        bits = nbytes * 8
+        off = self._pos % bits
+        if off:
+            self._pos += bits - off
+        self._grow(self._pos)
+
+    def writeAligned(self, arr):
+        arr = np.ascontiguousarray(arr)
+        self.writeAlign(4)
+        self.writeAlign(arr.itemsize)
+        if arr.size == 0:
+            return
+        nbytes = arr.nbytes
+        start = self._pos // 8
+        self._grow(self._pos + nbytes * 8)
+        self._w.view(np.uint8)[start:start + nbytes] = arr.view(np.uint8)
+        self._pos += nbytes * 8
+        self.writeAlign(4)
+
+    def _append_segment(self, words, nbits, start_bit):
+        """Places an encoded segment (nbits counted from bit 0 of words[0], the first start_bit of
+        which are placeholders) at the current position, which must be start_bit mod 32."""
+        assert self._pos % 32 == start_bit
+        nw = (nbits + 31) // 32
+        w0 = self._pos // 32
+        self._grow(self._pos - start_bit + nbits)
+        if nw:
+            seg = np.asarray(words[:nw], dtype=np.uint32)
+            if start_bit:
+                keep = np.uint32((0xFFFFFFFF << (32 - start_bit)) & 0xFFFFFFFF)
+                self._w[w0] = (self._w[w0] & keep) | seg[0]
+                self._w[w0 + 1:w0 + nw] = seg[1:]
+            else:
+                self._w[w0:w0 + nw] = seg
+        self._pos += nbits - start_bit
+
+    # -- reading
+    def readBits(self, count):
+        v = 0
+        for _ in range(count):
+            if self._pos >= self._size:
+                break
+            v = (v << 1) | ((int(self._w[self._pos // 32]) >> (31 - self._pos % 32)) & 1)
+            self._pos += 1
+        return v
+
+    def align(self, nbytes=4):
+        bits = nbytes * 8
+        off = self._pos % bits
+        if off:
+            self._pos = min(self._pos + bits - off, self._size)
+        return True
+
+    def readAligned(self, dtype, count):
+        dtype = np.dtype(dtype)
+        self.align(4)
+        self.align(dtype.itemsize)
+        start = self._pos // 8
+        avail = (self._size - self._pos) // 8 // dtype.itemsize
+        count = min(count, avail)
+        out = self._w.view(np.uint8)[start:start + count * dtype.itemsize].view(dtype).copy()
+        self._pos += count * dtype.itemsize * 8
+        self.align(4)
+        return out
+
+
+# ------------------------------------------------------------------------------------ coding
+def _encode(fn_name, inst, bitStreams, streams, count):
+    lib = _capi.load()
+    ns = len(streams)
+    single = isinstance(bitStreams, BitStream)
+    targets = [bitStreams] * ns if single else list(bitStreams)
+    if len(targets) != ns:
+        raise ValueError("one bit stream per symbol stream is required")
+    size = inst.symbolSize
+    for s in streams:
+        es = _elem_size(s)
+        if es is not None and es != size:
+            raise ValueError(f"symbol streams must have {size}-byte elements for this instance")
+    ptrs = (ctypes.c_void_p * max(ns, 1))(*[_ptr(s) for s in streams])
+    seen = set()
+    start = (ctypes.c_uint32 * max(ns, 1))()
+    for i, b in enumerate(targets):
+        if id(b) not in seen:
+            start[i] = b.getBitPosition() % 32
+            seen.add(id(b))
+        else:
+            start[i] = 0
+    words = _capi.P_u32()
+    beg = _capi.P_u64()
+    bits = _capi.P_u64()
+    st = getattr(lib, fn_name)(inst.handle, size, ptrs, ns, count, start, ctypes.byref(words), ctypes.byref(beg),
+                               ctypes.byref(bits))
+    if st == CC_ERR_SYMBOL_OVERFLOW:
+        return False
+    _check(st, fn_name)
+    if ns == 0:
+        return True
+    total = int(beg[ns - 1]) + (int(bits[ns - 1]) + 31) // 32
+    all_words = np.ctypeslib.as_array(words, shape=(max(total, 1),))[:total]
+    for i, b in enumerate(targets):
+        s0 = int(beg[i])
+        nb = int(bits[i])
+        b._append_segment(all_words[s0:s0 + (nb + 31) // 32], nb, int(start[i]))
+    return True
+
+
+def _decode(fn_name, inst, bitStreams, streams, count):
+    lib = _capi.load()
+    ns = len(streams)
+    single = isinstance(bitStreams, BitStream)
+    sources = [bitStreams] if single else []
+    s2b = None
+    if not single:
+        index = {}
+        mapping = []
+        for b in bitStreams:
+            if id(b) not in index:
+                index[id(b)] = len(sources)
+                sources.append(b)
+            mapping.append(index[id(b)])
+        s2b = (ctypes.c_uint32 * max(ns, 1))(*mapping)
+    nbs = len(sources)
+    raws = [np.ascontiguousarray(b._w[: max(b.getRawSizeUInts(), 1)]) for b in sources]
+    wptrs = (ctypes.c_void_p * nbs)(*[r.ctypes.data for r in raws])
+    bits = (ctypes.c_uint64 * nbs)(*[b.getBitSize() for b in sources])
+    pos = (ctypes.c_uint64 * nbs)(*[b.getBitPosition() for b in sources])
+    outs = (ctypes.c_void_p * max(ns, 1))(*[_ptr(s) for s in streams])
+    st = getattr(lib, fn_name)(inst.handle, inst.symbolSize, wptrs, bits, pos, nbs, s2b, outs, ns, count)
+    _check(st, fn_name)
+    for b, p in zip(sources, pos):
+        b._pos = int(p)
+    return True
+
+
+def encodeRLHuff(inst, bitStream, symbolStreams, symbolCountPerStream):
+    """encodeRLHuff (Encode.h:28, :42; per-stream bit streams :48, :51)."""
+    return _encode("cc_encode_rlhuff", inst, bitStream, symbolStreams, symbolCountPerStream)
+
+
+def decodeRLHuff(inst, bitStream, symbolStreams, symbolCountPerStream):
+    """decodeRLHuff (Encode.h:29, :43).  Outputs need not be zeroed."""
+    return _decode("cc_decode_rlhuff", inst, bitStream, symbolStreams, symbolCountPerStream)
+
+
+def encodeHuff(inst, bitStream, symbolStreams, symbolCountPerStream):
+    """encodeHuff (Encode.h:31, :45).  Does not write padding into the inputs."""
+    return _encode("cc_encode_huff", inst, bitStream, symbolStreams, symbolCountPerStream)
+
+
+def decodeHuff(inst, bitStream, symbolStreams, symbolCountPerStream):
+    """decodeHuff (Encode.h:32, :46).  Outputs need no padding."""
+    return _decode("cc_decode_huff", inst, bitStream, symbolStreams, symbolCountPerStream)
+
+
+# ------------------------------------------------------------------------------------ util
+class util:
+    """cudaCompress::util (include/cudaCompress/util/DWT.h, Quantize.h)."""
+
+    @staticmethod
+    def dwtFloat3DForward(dpDest, dpBuffer2, dpBuffer1, dpSource, sizeX, sizeY, sizeZ, srcChannelCount=1,
+                          dstRowPitch=0, dstSlicePitch=0, srcRowPitch=0, srcSlicePitch=0, stream=None):
+        _check(_capi.load().cc_dwt_float_forward(_ptr(dpDest), _ptr(dpBuffer2), _ptr(dpBuffer1), _ptr(dpSource),
+                                                 sizeX, sizeY, sizeZ, srcChannelCount, dstRowPitch, dstSlicePitch,
+                                                 srcRowPitch, srcSlicePitch, _stream(stream)), "dwtFloat3DForward")
+
+    @staticmethod
+    def dwtFloat2DForward(dpDest, dpBuffer, dpSource, sizeX, sizeY, srcChannelCount=1, dstRowPitch=0, srcRowPitch=0,
+                          stream=None):
+        _check(_capi.load().cc_dwt_float_forward(_ptr(dpDest), None, _ptr(dpBuffer), _ptr(dpSource), sizeX, sizeY, 1,
+                                                 srcChannelCount, dstRowPitch, 0, srcRowPitch, 0, _stream(stream)),
+               "dwtFloat2DForward")
+
+    @staticmethod
+    def dwtFloat3DInverse(dpDest, dpBuffer2, dpBuffer1, dpSource, sizeX, sizeY, sizeZ, dstChannelCount=1,
+                          dstRowPitch=0, dstSlicePitch=0, srcRowPitch=0, srcSlicePitch=0, stream=None):
+        _check(_capi.load().cc_dwt_float_inverse(_ptr(dpDest), _ptr(dpBuffer2), _ptr(dpBuffer1), _ptr(dpSource),
+                                                 sizeX, sizeY, sizeZ, dstChannelCount, dstRowPitch, dstSlicePitch,
+                                                 srcRowPitch, srcSlicePitch, _stream(stream)), "dwtFloat3DInverse")
+
+    @staticmethod
+    def dwtFloat2DInverse(dpDest, dpBuffer, dpSource, sizeX, sizeY, dstChannelCount=1, dstRowPitch=0, srcRowPitch=0,
+                          stream=None):
+        _check(_capi.load().cc_dwt_float_inverse(_ptr(dpDest), None, _ptr(dpBuffer), _ptr(dpSource), sizeX, sizeY, 1,
+                                                 dstChannelCount, dstRowPitch, 0, srcRowPitch, 0, _stream(stream)),
+               "dwtFloat2DInverse")
+
+    @staticmethod
+    def dwtFloat3DInverseFromSymbols(dpDest, dpBuffer2, dpBuffer1, dpLowpass, dppHighpass, quantStep, sizeX, sizeY,
+                                     sizeZ, dstChannelCount=1, dstRowPitch=0, dstSlicePitch=0, lowpassRowPitch=0,
+                                     lowpassSlicePitch=0, stream=None, symbolSize=2):
+        """dppHighpass: device array (tensor of int64 addresses) of 7 band pointers."""
+        _check(_capi.load().cc_dwt_float_inverse_from_symbols(
+            _ptr(dpDest), _ptr(dpBuffer2), _ptr(dpBuffer1), _ptr(dpLowpass), _ptr(dppHighpass), symbolSize,
+            quantStep, sizeX, sizeY, sizeZ, dstChannelCount, dstRowPitch, dstSlicePitch, lowpassRowPitch,
+            lowpassSlicePitch, _stream(stream)), "dwtFloat3DInverseFromSymbols")
+
+    @staticmethod
+    def dwtFloat2DInverseFromSymbols(dpDest, dpBuffer, dpLowpass, dppHighpass, quantStep, sizeX, sizeY,
+                                     dstChannelCount=1, dstRowPitch=0, lowpassRowPitch=0, stream=None, symbolSize=2):
+        _check(_capi.load().cc_dwt_float_inverse_from_symbols(
+            _ptr(dpDest), None, _ptr(dpBuffer), _ptr(dpLowpass), _ptr(dppHighpass), symbolSize, quantStep, sizeX,
+            sizeY, 1, dstChannelCount, dstRowPitch, 0, lowpassRowPitch, 0, _stream(stream)),
+            "dwtFloat2DInverseFromSymbols")
+
+    @staticmethod
+    def dwtIntForward(dpDest, dpBuffer, dpSource, sizeX, sizeY, sizeZ, dstRowPitch=0, dstSlicePitch=0, srcRowPitch=0,
+                      srcSlicePitch=0, stream=None):
+        _check(_capi.load().cc_dwt_int_forward(_ptr(dpDest), _ptr(dpBuffer), _ptr(dpSource), sizeX, sizeY, sizeZ,
+                                               dstRowPitch, dstSlicePitch, srcRowPitch, srcSlicePitch,
+                                               _stream(stream)), "dwtIntForward")
+
+    @staticmethod
+    def dwtIntInverse(dpDest, dpBuffer, dpSource, sizeX, sizeY, sizeZ, dstRowPitch=0, dstSlicePitch=0, srcRowPitch=0,
+                      srcSlicePitch=0, stream=None):
+        _check(_capi.load().cc_dwt_int_inverse(_ptr(dpDest), _ptr(dpBuffer), _ptr(dpSource), sizeX, sizeY, sizeZ,
+                                               dstRowPitch, dstSlicePitch, srcRowPitch, srcSlicePitch,
+                                               _stream(stream)), "dwtIntInverse")
+
+    @staticmethod
+    def symbolize(dpSymbols, dpData, sizeX, sizeY, sizeZ, rowPitchSrc=0, slicePitchSrc=0, stream=None):
+        _check(_capi.load().cc_symbolize(_ptr(dpSymbols), _ptr(dpData), sizeX, sizeY, sizeZ, rowPitchSrc,
+                                         slicePitchSrc, _stream(stream)), "symbolize")
+
+    @staticmethod
+    def unsymbolize(dpData, dpSymbols, sizeX, sizeY, sizeZ, rowPitchDst=0, slicePitchDst=0, stream=None):
+        _check(_capi.load().cc_unsymbolize(_ptr(dpData), _ptr(dpSymbols), sizeX, sizeY, sizeZ, rowPitchDst,
+                                           slicePitchDst, _stream(stream)), "unsymbolize")
+
+    @staticmethod
+    def quantizeToSymbols(dpSymbols, dpData, sizeX, sizeY, sizeZ, quantizationStep, rowPitchSrc=0, slicePitchSrc=0,
+                          quantType=QUANTIZE_DEADZONE, stream=None):
+        _check(_capi.load().cc_quantize_to_symbols(_ptr(dpSymbols), dpSymbols.element_size(), _ptr(dpData), sizeX,
+                                                   sizeY, sizeZ, quantizationStep, rowPitchSrc, slicePitchSrc,
+                                                   quantType, _stream(stream)), "quantizeToSymbols")
+
+    @staticmethod
+    def quantizeToSymbolsRoundtrip(dpSymbols, dpData, sizeX, sizeY, sizeZ, quantizationStep, rowPitchSrc=0,
+                                   slicePitchSrc=0, quantType=QUANTIZE_DEADZONE, stream=None):
+        _check(_capi.load().cc_quantize_to_symbols_roundtrip(_ptr(dpSymbols), dpSymbols.element_size(), _ptr(dpData),
+                                                             sizeX, sizeY, sizeZ, quantizationStep, rowPitchSrc,
+                                                             slicePitchSrc, quantType, _stream(stream)),
+               "quantizeToSymbolsRoundtrip")
+
+    @staticmethod
+    def unquantizeFromSymbols(dpData, dpSymbols, sizeX, sizeY, sizeZ, quantizationStep, rowPitchDst=0,
+                              slicePitchDst=0, quantType=QUANTIZE_DEADZONE, stream=None):
+        _check(_capi.load().cc_unquantize_from_symbols(_ptr(dpData), _ptr(dpSymbols), dpSymbols.element_size(),
+                                                       sizeX, sizeY, sizeZ, quantizationStep, rowPitchDst,
+                                                       slicePitchDst, quantType, _stream(stream)),
+               "unquantizeFromSymbols")
+
+    @staticmethod
+    def quantizeToShort(dpQuant, dpData, sizeX, sizeY, sizeZ, quantizationStep, rowPitchSrc=0, slicePitchSrc=0,
+                        quantType=QUANTIZE_DEADZONE, stream=None):
+        _check(_capi.load().cc_quantize_to_short(_ptr(dpQuant), _ptr(dpData), sizeX, sizeY, sizeZ, quantizationStep,
+                                                 rowPitchSrc, slicePitchSrc, quantType, _stream(stream)),
+               "quantizeToShort")
+
+    @staticmethod
+    def unquantizeFromShort(dpData, dpQuant, sizeX, sizeY, sizeZ, quantizationStep, rowPitchDst=0, slicePitchDst=0,
+                            quantType=QUANTIZE_DEADZONE, stream=None):
+        _check(_capi.load().cc_unquantize_from_short(_ptr(dpData), _ptr(dpQuant), sizeX, sizeY, sizeZ,
+                                                     quantizationStep, rowPitchDst, slicePitchDst, quantType,
+                                                     _stream(stream)), "unquantizeFromShort")
+
+    @staticmethod
+    def getMaxAbs(inst, dpImage, elemCount, dpValMax):
+        _check(_capi.load().cc_get_max_abs(inst.handle, _ptr(dpImage), elemCount, _ptr(dpValMax)), "getMaxAbs")
+
+
+# ------------------------------------------------------------------------------------ volume codecs
+def compressVolumeFloat(inst, dpImage, sizeX, sizeY, sizeZ, numLevels, quantizationStepLevel0,
+                        doRLEOnlyOnLvl0=False):
+    """compressVolumeFloat (src/examples/CompressVolume.cpp:313); returns the bitstream words,
+    or None on symbol overflow (the reference returns false)."""
+    lib = _capi.load()
+    words = _capi.P_u32()
+    nbits = ctypes.c_uint64()
+    st = lib.cc_compress_volume_float(inst.handle, _ptr(dpImage), sizeX, sizeY, sizeZ, numLevels,
+                                      quantizationStepLevel0, int(doRLEOnlyOnLvl0), inst.symbolSize,
+                                      ctypes.byref(words), ctypes.byref(nbits))
+    if st == CC_ERR_SYMBOL_OVERFLOW:
+        return None
+    _check(st, "compressVolumeFloat")
+    n = nbits.value // 32
+    return np.ctypeslib.as_array(words, shape=(max(n, 1),))[:n].copy()
+
+
+def decompressVolumeFloat(inst, dpImage, sizeX, sizeY, sizeZ, numLevels, bits, quantizationStepLevel0,
+                          doRLEOnlyOnLvl0=False):
+    """decompressVolumeFloat (src/examples/CompressVolume.cpp:412)."""
+    bits = np.ascontiguousarray(bits, dtype=np.uint32)
+    _check(_capi.load().cc_decompress_volume_float(inst.handle, _ptr(dpImage), sizeX, sizeY, sizeZ, numLevels,
+                                                   bits.ctypes.data, bits.size * 32, quantizationStepLevel0,
+                                                   int(doRLEOnlyOnLvl0), inst.symbolSize), "decompressVolumeFloat")
+
+
+def compressVolumeLossless(inst, dpImage, sizeX, sizeY, sizeZ, numLevels):
+    """compressVolumeLossless (src/examples/CompressVolume.cpp:234); sizeZ == 1 for 2D fields."""
+    lib = _capi.load()
+    words = _capi.P_u32()
+    nbits = ctypes.c_uint64()
+    st = lib.cc_compress_volume_lossless(inst.handle, _ptr(dpImage), sizeX, sizeY, sizeZ, numLevels,
+                                         ctypes.byref(words), ctypes.byref(nbits))
+    if st == CC_ERR_SYMBOL_OVERFLOW:
+        return None
+    _check(st, "compressVolumeLossless")
+    n = nbits.value // 32
+    return np.ctypeslib.as_array(words, shape=(max(n, 1),))[:n].copy()
+
+
+def decompressVolumeLossless(inst, dpImage, sizeX, sizeY, sizeZ, numLevels, bits):
+    """decompressVolumeLossless (src/examples/CompressVolume.cpp:279)."""
+    bits = np.ascontiguousarray(bits, dtype=np.uint32)
+    _check(_capi.load().cc_decompress_volume_lossless(inst.handle, _ptr(dpImage), sizeX, sizeY, sizeZ, numLevels,
+                                                      bits.ctypes.data, bits.size * 32), "decompressVolumeLossless")

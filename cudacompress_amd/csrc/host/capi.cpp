@@ -1,0 +1,313 @@
+// extern "C" entry points declared in include/cudacompress_amd.h.  Every function converts
+// cca::Error / std::bad_alloc into a status code and keeps the text for cc_last_error().
+#include <new>
+#include <string>
+
+#include "../kernels.hpp"
+#include "codec.hpp"
+#include "instance.hpp"
+#include "volume.hpp"
+
+namespace cca {
+namespace {
+thread_local std::string g_error;
+}
+void set_error(const std::string& msg) { g_error = msg; }
+void clear_error() { g_error.clear(); }
+}  // namespace cca
+
+namespace {
+
+template <class F>
+cc_status guarded(F&& f) {
+    try {
+        f();
+        cca::clear_error();
+        return CC_OK;
+    } catch (const cca::Error& e) {
+        cca::set_error(e.msg);
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        cca::set_error("host allocation failed");
+        return CC_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        cca::set_error(e.what());
+        return CC_ERR_DEVICE;
+    }
+}
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+cc_status encode_entry(cc_instance* inst, bool rl, int symbolSize, const void* const* streams, uint32_t ns,
+                       uint32_t n, const uint32_t* startBits, const uint32_t** outWords,
+                       const uint64_t** outSegBegin, const uint64_t** outSegBits) {
+    return guarded([&] {
+        CCA_REQUIRE(inst && outWords && outSegBegin && outSegBits, CC_ERR_INVALID_ARG, "null argument");
+        cca::encode_streams(inst, rl, symbolSize, streams, ns, n, startBits);
+        *outWords = inst->out_words.data();
+        *outSegBegin = inst->seg_begin.data();
+        *outSegBits = inst->seg_bits.data();
+    });
+}
+
+cc_status decode_entry(cc_instance* inst, bool rl, int symbolSize, const uint32_t* const* words, const uint64_t* bits,
+                       uint64_t* pos, uint32_t nbs, const uint32_t* s2b, void* const* outs, uint32_t ns, uint32_t n) {
+    return guarded([&] {
+        CCA_REQUIRE(inst, CC_ERR_INVALID_ARG, "null instance");
+        cca::decode_streams(inst, rl, symbolSize, words, bits, pos, nbs, s2b, outs, ns, n);
+    });
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cc_status_string(cc_status s) {
+    switch (s) {
+        case CC_OK: return "ok";
+        case CC_ERR_SYMBOL_OVERFLOW: return "symbol exceeds the Huffman alphabet";
+        case CC_ERR_OUT_OF_MEMORY: return "out of memory";
+        case CC_ERR_INVALID_ARG: return "invalid argument";
+        case CC_ERR_DEVICE: return "device error";
+        case CC_ERR_CODEWORD_TOO_LONG: return "Huffman codeword longer than 31 bits";
+        case CC_ERR_CORRUPT_STREAM: return "corrupt bitstream";
+    }
+    return "unknown status";
+}
+
+const char* cc_last_error(void) { return cca::g_error.c_str(); }
+
+cc_status cc_create_instance(int device, uint32_t streamCountMax, uint32_t elemCountPerStreamMax,
+                             uint32_t codingBlockSize, uint32_t log2HuffmanDistinctSymbolCountMax, cc_instance** out) {
+    return guarded([&] {
+        CCA_REQUIRE(out, CC_ERR_INVALID_ARG, "null output");
+        *out = nullptr;
+        if (codingBlockSize == 0) codingBlockSize = 128;
+        if (log2HuffmanDistinctSymbolCountMax == 0) log2HuffmanDistinctSymbolCountMax = 14;
+        CCA_REQUIRE(log2HuffmanDistinctSymbolCountMax <= 24, CC_ERR_INVALID_ARG,
+                    "log2HuffmanDistinctSymbolCountMax must be <= 24");
+        CCA_REQUIRE(codingBlockSize >= 32 && codingBlockSize <= 256 && (codingBlockSize & (codingBlockSize - 1)) == 0,
+                    CC_ERR_INVALID_ARG, "codingBlockSize must be a power of two in [32, 256]");
+        if (device < 0) CCA_HIP(hipGetDevice(&device));
+        int count = 0;
+        CCA_HIP(hipGetDeviceCount(&count));
+        CCA_REQUIRE(device < count, CC_ERR_INVALID_ARG, "no such device");
+        auto* I = new cc_instance();
+        I->device = device;
+        I->stream_count_max = streamCountMax;
+        I->elem_count_per_stream_max = elemCountPerStreamMax;
+        I->cbs = codingBlockSize;
+        I->log2 = log2HuffmanDistinctSymbolCountMax;
+        try {
+            cca::DeviceGuard g(device);
+            for (auto& e : I->dec_event) {
+                CCA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                CCA_HIP(hipEventRecord(e, nullptr));
+            }
+            // reserve the per-stream buffers up front, as the reference sizes its arena
+            // (InstanceImpl.cpp:85-96), so running out of memory shows up here
+            const uint64_t symSize = I->log2 > 16 ? 4 : 2;
+            const uint64_t per = (uint64_t)streamCountMax * cca::align_up(std::max<uint32_t>(elemCountPerStreamMax, 1), 64) * symSize;
+            if (per > 0) {
+                I->d_rle_sym.get(per);
+                I->d_rle_zc.get(per);
+                I->d_dec_sym.get(per);
+                I->d_dec_zc.get(per);
+            }
+            CCA_HIP(hipDeviceSynchronize());
+        } catch (...) {
+            delete I;
+            throw;
+        }
+        *out = I;
+    });
+}
+
+void cc_destroy_instance(cc_instance* inst) { delete inst; }
+
+int cc_get_instance_device(const cc_instance* inst) { return inst ? inst->device : -1; }
+uint32_t cc_get_instance_stream_count_max(const cc_instance* inst) { return inst ? inst->stream_count_max : 0; }
+uint32_t cc_get_instance_elem_count_per_stream_max(const cc_instance* inst) {
+    return inst ? inst->elem_count_per_stream_max : 0;
+}
+uint32_t cc_get_instance_coding_block_size(const cc_instance* inst) { return inst ? inst->cbs : 0; }
+uint32_t cc_get_instance_log2_huffman_distinct_symbol_count_max(const cc_instance* inst) {
+    return inst ? inst->log2 : 0;
+}
+int cc_get_instance_use_long_symbols(const cc_instance* inst) { return inst && inst->log2 > 16 ? 1 : 0; }
+void cc_set_instance_stream(cc_instance* inst, void* stream) {
+    if (inst) inst->stream = as_stream(stream);
+}
+void* cc_get_instance_stream(const cc_instance* inst) { return inst ? (void*)inst->stream : nullptr; }
+cc_status cc_set_instance_dialect(cc_instance* inst, int dialect) {
+    return guarded([&] {
+        CCA_REQUIRE(inst && (dialect == CC_DIALECT_GPU || dialect == CC_DIALECT_CPU), CC_ERR_INVALID_ARG,
+                    "bad dialect");
+        inst->dialect = dialect;
+    });
+}
+
+cc_status cc_encode_rlhuff(cc_instance* inst, int symbolSize, const void* const* d_symbolStreams, uint32_t streamCount,
+                           uint32_t symbolCountPerStream, const uint32_t* startBitOffsets, const uint32_t** outWords,
+                           const uint64_t** outSegmentWordBegin, const uint64_t** outSegmentBits) {
+    return encode_entry(inst, true, symbolSize, d_symbolStreams, streamCount, symbolCountPerStream, startBitOffsets,
+                        outWords, outSegmentWordBegin, outSegmentBits);
+}
+cc_status cc_encode_huff(cc_instance* inst, int symbolSize, const void* const* d_symbolStreams, uint32_t streamCount,
+                         uint32_t symbolCountPerStream, const uint32_t* startBitOffsets, const uint32_t** outWords,
+                         const uint64_t** outSegmentWordBegin, const uint64_t** outSegmentBits) {
+    return encode_entry(inst, false, symbolSize, d_symbolStreams, streamCount, symbolCountPerStream, startBitOffsets,
+                        outWords, outSegmentWordBegin, outSegmentBits);
+}
+cc_status cc_decode_rlhuff(cc_instance* inst, int symbolSize, const uint32_t* const* bitstreamWords,
+                           const uint64_t* bitstreamBits, uint64_t* bitstreamPos, uint32_t bitstreamCount,
+                           const uint32_t* streamToBitstream, void* const* d_symbolStreams, uint32_t streamCount,
+                           uint32_t symbolCountPerStream) {
+    return decode_entry(inst, true, symbolSize, bitstreamWords, bitstreamBits, bitstreamPos, bitstreamCount,
+                        streamToBitstream, d_symbolStreams, streamCount, symbolCountPerStream);
+}
+cc_status cc_decode_huff(cc_instance* inst, int symbolSize, const uint32_t* const* bitstreamWords,
+                         const uint64_t* bitstreamBits, uint64_t* bitstreamPos, uint32_t bitstreamCount,
+                         const uint32_t* streamToBitstream, void* const* d_symbolStreams, uint32_t streamCount,
+                         uint32_t symbolCountPerStream) {
+    return decode_entry(inst, false, symbolSize, bitstreamWords, bitstreamBits, bitstreamPos, bitstreamCount,
+                        streamToBitstream, d_symbolStreams, streamCount, symbolCountPerStream);
+}
+
+cc_status cc_dwt_float_forward(float* d_dst, float* d_buffer2, float* d_buffer1, const float* d_src, int sizeX,
+                               int sizeY, int sizeZ, int srcChannelCount, int dstRowPitch, int dstSlicePitch,
+                               int srcRowPitch, int srcSlicePitch, void* stream) {
+    return guarded([&] {
+        cca::dwt97_forward(d_dst, d_buffer2, d_buffer1, d_src, sizeX, sizeY, sizeZ, srcChannelCount, dstRowPitch,
+                           dstSlicePitch, srcRowPitch, srcSlicePitch, as_stream(stream));
+    });
+}
+cc_status cc_dwt_float_inverse(float* d_dst, float* d_buffer2, float* d_buffer1, const float* d_src, int sizeX,
+                               int sizeY, int sizeZ, int dstChannelCount, int dstRowPitch, int dstSlicePitch,
+                               int srcRowPitch, int srcSlicePitch, void* stream) {
+    return guarded([&] {
+        cca::dwt97_inverse(d_dst, d_buffer2, d_buffer1, d_src, sizeX, sizeY, sizeZ, dstChannelCount, dstRowPitch,
+                           dstSlicePitch, srcRowPitch, srcSlicePitch, as_stream(stream));
+    });
+}
+cc_status cc_dwt_float_inverse_from_symbols(float* d_dst, float* d_buffer2, float* d_buffer1, const float* d_lowpass,
+                                            const void* const* dd_highpass, int symbolSize, float quantStep,
+                                            int sizeX, int sizeY, int sizeZ, int dstChannelCount, int dstRowPitch,
+                                            int dstSlicePitch, int lowpassRowPitch, int lowpassSlicePitch,
+                                            void* stream) {
+    return guarded([&] {
+        cca::dwt97_inverse_from_symbols(d_dst, d_buffer2, d_buffer1, d_lowpass, dd_highpass, symbolSize, quantStep,
+                                        sizeX, sizeY, sizeZ, dstChannelCount, dstRowPitch, dstSlicePitch,
+                                        lowpassRowPitch, lowpassSlicePitch, as_stream(stream));
+    });
+}
+cc_status cc_dwt_int_forward(int16_t* d_dst, int16_t* d_buffer, const int16_t* d_src, int sizeX, int sizeY, int sizeZ,
+                             int dstRowPitch, int dstSlicePitch, int srcRowPitch, int srcSlicePitch, void* stream) {
+    return guarded([&] {
+        cca::dwt53_forward(d_dst, d_buffer, d_src, sizeX, sizeY, sizeZ, dstRowPitch, dstSlicePitch, srcRowPitch,
+                           srcSlicePitch, as_stream(stream));
+    });
+}
+cc_status cc_dwt_int_inverse(int16_t* d_dst, int16_t* d_buffer, const int16_t* d_src, int sizeX, int sizeY, int sizeZ,
+                             int dstRowPitch, int dstSlicePitch, int srcRowPitch, int srcSlicePitch, void* stream) {
+    return guarded([&] {
+        cca::dwt53_inverse(d_dst, d_buffer, d_src, sizeX, sizeY, sizeZ, dstRowPitch, dstSlicePitch, srcRowPitch,
+                           srcSlicePitch, as_stream(stream));
+    });
+}
+
+cc_status cc_symbolize(uint16_t* d_symbols, const int16_t* d_data, uint32_t sizeX, uint32_t sizeY, uint32_t sizeZ,
+                       uint32_t rowPitchSrc, uint32_t slicePitchSrc, void* stream) {
+    return guarded([&] { cca::symbolize(d_symbols, d_data, sizeX, sizeY, sizeZ, rowPitchSrc, slicePitchSrc, as_stream(stream)); });
+}
+cc_status cc_unsymbolize(int16_t* d_data, const uint16_t* d_symbols, uint32_t sizeX, uint32_t sizeY, uint32_t sizeZ,
+                         uint32_t rowPitchDst, uint32_t slicePitchDst, void* stream) {
+    return guarded([&] { cca::unsymbolize(d_data, d_symbols, sizeX, sizeY, sizeZ, rowPitchDst, slicePitchDst, as_stream(stream)); });
+}
+cc_status cc_quantize_to_symbols(void* d_symbols, int symbolSize, const float* d_data, uint32_t sizeX, uint32_t sizeY,
+                                 uint32_t sizeZ, float quantizationStep, uint32_t rowPitchSrc, uint32_t slicePitchSrc,
+                                 int quantType, void* stream) {
+    return guarded([&] {
+        cca::quantize_to_symbols(d_symbols, symbolSize, const_cast<float*>(d_data), false, sizeX, sizeY, sizeZ,
+                                 quantizationStep, rowPitchSrc, slicePitchSrc, quantType, as_stream(stream));
+    });
+}
+cc_status cc_quantize_to_symbols_roundtrip(void* d_symbols, int symbolSize, float* d_data, uint32_t sizeX,
+                                           uint32_t sizeY, uint32_t sizeZ, float quantizationStep,
+                                           uint32_t rowPitchSrc, uint32_t slicePitchSrc, int quantType, void* stream) {
+    return guarded([&] {
+        cca::quantize_to_symbols(d_symbols, symbolSize, d_data, true, sizeX, sizeY, sizeZ, quantizationStep,
+                                 rowPitchSrc, slicePitchSrc, quantType, as_stream(stream));
+    });
+}
+cc_status cc_unquantize_from_symbols(float* d_data, const void* d_symbols, int symbolSize, uint32_t sizeX,
+                                     uint32_t sizeY, uint32_t sizeZ, float quantizationStep, uint32_t rowPitchDst,
+                                     uint32_t slicePitchDst, int quantType, void* stream) {
+    return guarded([&] {
+        cca::unquantize_from_symbols(d_data, d_symbols, symbolSize, sizeX, sizeY, sizeZ, quantizationStep,
+                                     rowPitchDst, slicePitchDst, quantType, as_stream(stream));
+    });
+}
+cc_status cc_quantize_to_short(int16_t* d_quant, const float* d_data, uint32_t sizeX, uint32_t sizeY, uint32_t sizeZ,
+                               float quantizationStep, uint32_t rowPitchSrc, uint32_t slicePitchSrc, int quantType,
+                               void* stream) {
+    return guarded([&] {
+        cca::quantize_to_short(d_quant, d_data, sizeX, sizeY, sizeZ, quantizationStep, rowPitchSrc, slicePitchSrc,
+                               quantType, as_stream(stream));
+    });
+}
+cc_status cc_unquantize_from_short(float* d_data, const int16_t* d_quant, uint32_t sizeX, uint32_t sizeY,
+                                   uint32_t sizeZ, float quantizationStep, uint32_t rowPitchDst, uint32_t slicePitchDst,
+                                   int quantType, void* stream) {
+    return guarded([&] {
+        cca::unquantize_from_short(d_data, d_quant, sizeX, sizeY, sizeZ, quantizationStep, rowPitchDst, slicePitchDst,
+                                   quantType, as_stream(stream));
+    });
+}
+cc_status cc_get_max_abs(cc_instance* inst, const float* d_image, uint32_t elemCount, float* d_valMax) {
+    return guarded([&] {
+        CCA_REQUIRE(inst && d_valMax, CC_ERR_INVALID_ARG, "null argument");
+        cca::DeviceGuard g(inst->device);
+        cca::max_abs(d_image, elemCount, d_valMax, inst->stream);
+    });
+}
+
+cc_status cc_compress_volume_float(cc_instance* inst, const float* d_image, uint32_t sizeX, uint32_t sizeY,
+                                   uint32_t sizeZ, uint32_t numLevels, float quantStepLevel0, int doRLEOnlyOnLvl0,
+                                   int symbolSize, const uint32_t** outWords, uint64_t* outBits) {
+    return guarded([&] {
+        CCA_REQUIRE(inst && outWords && outBits, CC_ERR_INVALID_ARG, "null argument");
+        cca::compress_volume_float(inst, d_image, sizeX, sizeY, sizeZ, numLevels, quantStepLevel0,
+                                   doRLEOnlyOnLvl0 != 0, symbolSize);
+        *outWords = inst->vol_bits.data();
+        *outBits = 32ull * inst->vol_bits.size();
+    });
+}
+cc_status cc_decompress_volume_float(cc_instance* inst, float* d_image, uint32_t sizeX, uint32_t sizeY, uint32_t sizeZ,
+                                     uint32_t numLevels, const uint32_t* bits, uint64_t bitCount,
+                                     float quantStepLevel0, int doRLEOnlyOnLvl0, int symbolSize) {
+    return guarded([&] {
+        CCA_REQUIRE(inst && bits, CC_ERR_INVALID_ARG, "null argument");
+        cca::decompress_volume_float(inst, d_image, sizeX, sizeY, sizeZ, numLevels, bits, bitCount, quantStepLevel0,
+                                     doRLEOnlyOnLvl0 != 0, symbolSize);
+    });
+}
+cc_status cc_compress_volume_lossless(cc_instance* inst, const int16_t* d_image, uint32_t sizeX, uint32_t sizeY,
+                                      uint32_t sizeZ, uint32_t numLevels, const uint32_t** outWords,
+                                      uint64_t* outBits) {
+    return guarded([&] {
+        CCA_REQUIRE(inst && outWords && outBits, CC_ERR_INVALID_ARG, "null argument");
+        cca::compress_volume_lossless(inst, d_image, sizeX, sizeY, sizeZ, numLevels);
+        *outWords = inst->vol_bits.data();
+        *outBits = 32ull * inst->vol_bits.size();
+    });
+}
+cc_status cc_decompress_volume_lossless(cc_instance* inst, int16_t* d_image, uint32_t sizeX, uint32_t sizeY,
+                                        uint32_t sizeZ, uint32_t numLevels, const uint32_t* bits, uint64_t bitCount) {
+    return guarded([&] {
+        CCA_REQUIRE(inst && bits, CC_ERR_INVALID_ARG, "null argument");
+        cca::decompress_volume_lossless(inst, d_image, sizeX, sizeY, sizeZ, numLevels, bits, bitCount);
+    });
+}
+
+}  // extern "C"

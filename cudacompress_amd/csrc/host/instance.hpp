@@ -1,0 +1,75 @@
+// cc_instance: device binding, parameters and the reusable device/host buffers of the coder
+// (the role of src/cudaCompress/InstanceImpl.h:27-189).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "../common.hpp"
+
+namespace cca {
+
+// Grow-only device buffer (contents are not preserved across growth).
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    void* get(size_t bytes);
+    template <class T>
+    T* as(size_t count) { return static_cast<T*>(get(count * sizeof(T))); }
+    void release();
+};
+
+// Grow-only pinned host buffer.
+struct PinBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    void* get(size_t bytes);
+    template <class T>
+    T* as(size_t count) { return static_cast<T*>(get(count * sizeof(T))); }
+    void release();
+};
+
+// Scoped device selection: makes `dev` current and restores the previous device.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev);
+    ~DeviceGuard();
+};
+
+constexpr int kDecodeRing = 4;  // decode staging slots (the reference keeps 8, InstanceImpl.h:108)
+
+}  // namespace cca
+
+struct cc_instance {
+    int device = 0;
+    uint32_t stream_count_max = 0;
+    uint32_t elem_count_per_stream_max = 0;
+    uint32_t cbs = 128;
+    uint32_t log2 = 14;
+    hipStream_t stream = nullptr;
+    int dialect = CC_DIALECT_GPU;
+
+    // encode scratch
+    cca::DevBuf d_rle_sym, d_rle_zc;            // compacted streams, [stream][elem capacity]
+    cca::DevBuf d_desc, d_tiles, d_map, d_map2, d_results;
+    cca::DevBuf d_hist, d_tables, d_words, d_offsets, d_block_bits, d_stream_bits;
+    cca::PinBuf h_up, h_down, h_hist, h_words;
+    std::vector<uint32_t> out_words;
+    std::vector<uint64_t> seg_begin, seg_bits;
+
+    // decode scratch
+    cca::DevBuf d_dec_stage, d_dec_sym, d_dec_zc, d_dec_sums, d_dec_desc;
+    cca::PinBuf h_dec_stage[cca::kDecodeRing];
+    hipEvent_t dec_event[cca::kDecodeRing] = {};
+    int dec_slot = 0;
+
+    // caller-level (volume codec) scratch
+    cca::DevBuf d_vol[4];
+    cca::DevBuf d_vol_ptrs;
+    cca::PinBuf h_vol_ptrs;
+    std::vector<uint32_t> vol_bits;
+
+    ~cc_instance();
+};

@@ -1,0 +1,17 @@
+// Volume codecs (caller layouts of src/examples/CompressVolume.cpp); output in inst->vol_bits.
+#pragma once
+
+#include "instance.hpp"
+
+namespace cca {
+
+void compress_volume_float(cc_instance* I, const float* image, uint32_t sx, uint32_t sy, uint32_t sz,
+                           uint32_t levels, float q0, bool rle_only_lvl0, int symSize);
+void decompress_volume_float(cc_instance* I, float* image, uint32_t sx, uint32_t sy, uint32_t sz, uint32_t levels,
+                             const uint32_t* words, uint64_t nbits, float q0, bool rle_only_lvl0, int symSize);
+void compress_volume_lossless(cc_instance* I, const int16_t* image, uint32_t sx, uint32_t sy, uint32_t sz,
+                              uint32_t levels);
+void decompress_volume_lossless(cc_instance* I, int16_t* image, uint32_t sx, uint32_t sy, uint32_t sz,
+                                uint32_t levels, const uint32_t* words, uint64_t nbits);
+
+}  // namespace cca

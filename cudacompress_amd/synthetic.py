@@ -1,0 +1,78 @@
+"""Seeded synthetic inputs for the benchmark configurations (numpy, CPU).
+
+* gaussian_field(shape, seed): isotropic Gaussian random field with energy spectrum E(k) ~ k^-5/3
+  (per-mode amplitude ~ k^-(5/3 + d - 1)/2), random phases, zero mean, unit standard deviation.
+* fbm_heightfield(n, hurst, seed): spectral fractional Brownian motion (2D amplitude ~ k^-(H+1)),
+  min-max mapped to [0, 65535] and returned as int16 = u - 32768.
+* velocity_field(n, seed): divergence-free u = curl A with |A_hat| ~ k^-17/6 (E_u ~ k^-5/3),
+  unit RMS, shape (3, n, n, n).
+"""
+import hashlib
+
+import numpy as np
+
+
+def _kmag(shape):
+    ks = [np.fft.fftfreq(n) * n for n in shape[:-1]] + [np.fft.rfftfreq(shape[-1]) * shape[-1]]
+    grids = np.meshgrid(*ks, indexing="ij", sparse=True)
+    k2 = sum(g.astype(np.float64) ** 2 for g in grids)
+    return np.sqrt(k2), grids
+
+
+def gaussian_field(shape, seed, slope=-5.0 / 3.0):
+    rng = np.random.default_rng(seed)
+    d = len(shape)
+    white = rng.standard_normal(shape)
+    spec = np.fft.rfftn(white)
+    k, _ = _kmag(shape)
+    with np.errstate(divide="ignore"):
+        amp = np.where(k > 0, k ** ((slope - (d - 1)) / 2.0), 0.0)
+    field = np.fft.irfftn(spec * amp, s=shape, axes=tuple(range(d)))
+    field -= field.mean()
+    field /= field.std()
+    return field.astype(np.float32)
+
+
+def fbm_heightfield(n, hurst, seed):
+    rng = np.random.default_rng(seed)
+    white = rng.standard_normal((n, n))
+    spec = np.fft.rfftn(white)
+    k, _ = _kmag((n, n))
+    with np.errstate(divide="ignore"):
+        amp = np.where(k > 0, k ** (-(hurst + 1.0)), 0.0)
+    h = np.fft.irfftn(spec * amp, s=(n, n), axes=(0, 1))
+    lo, hi = h.min(), h.max()
+    u = np.rint((h - lo) / (hi - lo) * 65535.0).astype(np.int64)
+    return (u - 32768).astype(np.int16)
+
+
+def velocity_field(n, seed):
+    rng = np.random.default_rng(seed)
+    shape = (n, n, n)
+    k, grids = _kmag(shape)
+    with np.errstate(divide="ignore"):
+        amp = np.where(k > 0, k ** (-17.0 / 6.0), 0.0)
+    a_hat = [np.fft.rfftn(rng.standard_normal(shape)) * amp for _ in range(3)]
+    kz, ky, kx = [2j * np.pi * g / n for g in grids]
+    # u = curl A with components ordered (x, y, z) on a (z, y, x) array
+    ax, ay, az = a_hat
+    ux = ky * az - kz * ay
+    uy = kz * ax - kx * az
+    uz = kx * ay - ky * ax
+    u = np.stack([np.fft.irfftn(c, s=shape, axes=(0, 1, 2)) for c in (ux, uy, uz)])
+    u /= np.sqrt((u ** 2).sum(axis=0).mean())
+    return u.astype(np.float32)
+
+
+def sha256(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def psnr(orig, rec):
+    """PSNR = 20 log10(range / RMSE), range = max - min of the original
+    (src/examples/tools/imgtools.h:137-143)."""
+    orig = np.asarray(orig, dtype=np.float64)
+    rec = np.asarray(rec, dtype=np.float64)
+    rmse = np.sqrt(np.mean((orig - rec) ** 2))
+    rng = orig.max() - orig.min()
+    return float("inf") if rmse == 0 else float(20.0 * np.log10(rng / rmse))

@@ -1,0 +1,191 @@
+/*
+ * cudacompress_amd -- C ABI of the MI355X (gfx950) implementation of the cudaCompress
+ * wavelet compression path.
+ *
+ * Every entry point below replaces one reference interface (m0bl0/cudaCompress, paths relative
+ * to that repository); the cited lines are the declaration being replaced.  Arguments keep the
+ * reference's meaning and order; device pointers are plain pointers, HIP streams are passed as
+ * `void*` (hipStream_t), and every function returns a status code instead of printing and
+ * continuing.  The C++ drop-in API in include/cudaCompress/ is a set of inline wrappers over
+ * these functions.
+ *
+ * Threading: an instance is not thread-safe (as in the reference); use one per host thread.
+ */
+#ifndef CUDACOMPRESS_AMD_H
+#define CUDACOMPRESS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CC_API __attribute__((visibility("default")))
+
+typedef enum cc_status {
+    CC_OK = 0,
+    CC_ERR_SYMBOL_OVERFLOW = 1,    /* a symbol >= 2^log2HuffmanDistinctSymbolCountMax (ref: encode* returns false) */
+    CC_ERR_OUT_OF_MEMORY = 2,
+    CC_ERR_INVALID_ARG = 3,
+    CC_ERR_DEVICE = 4,             /* a HIP runtime call failed */
+    CC_ERR_CODEWORD_TOO_LONG = 5,  /* a Huffman codeword > 31 bits: not representable in the 5-bit header field */
+    CC_ERR_CORRUPT_STREAM = 6      /* bitstream inconsistent with the requested decode */
+} cc_status;
+
+/* Bitstream dialects.  GPU: the reference library's layout (src/cudaCompress/HuffmanTable.cu:470-485).
+ * CPU: the reference CPU coder's layout (src/CPU/HuffmanTableCPU.cpp:108-135; Symbol16 only). */
+enum { CC_DIALECT_GPU = 0, CC_DIALECT_CPU = 1 };
+
+typedef struct cc_instance cc_instance;
+
+CC_API const char* cc_status_string(cc_status s);
+/* Text of the last error raised on the calling thread (empty when none). */
+CC_API const char* cc_last_error(void);
+
+/* ---- Instance: include/cudaCompress/Instance.h:26-45 -------------------------------------- */
+
+/* createInstance, include/cudaCompress/Instance.h:26-31.  codingBlockSize 0 -> 128,
+ * log2HuffmanDistinctSymbolCountMax 0 -> 14 (src/cudaCompress/InstanceImpl.cpp:65-75). */
+CC_API cc_status cc_create_instance(int device, uint32_t streamCountMax, uint32_t elemCountPerStreamMax,
+                                    uint32_t codingBlockSize, uint32_t log2HuffmanDistinctSymbolCountMax,
+                                    cc_instance** out);
+/* destroyInstance, include/cudaCompress/Instance.h:33 */
+CC_API void cc_destroy_instance(cc_instance* inst);
+/* getInstance*, include/cudaCompress/Instance.h:36-41 */
+CC_API int cc_get_instance_device(const cc_instance* inst);
+CC_API uint32_t cc_get_instance_stream_count_max(const cc_instance* inst);
+CC_API uint32_t cc_get_instance_elem_count_per_stream_max(const cc_instance* inst);
+CC_API uint32_t cc_get_instance_coding_block_size(const cc_instance* inst);
+CC_API uint32_t cc_get_instance_log2_huffman_distinct_symbol_count_max(const cc_instance* inst);
+CC_API int cc_get_instance_use_long_symbols(const cc_instance* inst);
+/* setInstanceCudaStream/getInstanceCudaStream, include/cudaCompress/Instance.h:44-45
+ * (implemented as setInstanceStream/getInstanceStream, src/cudaCompress/Instance.cpp:59-67) */
+CC_API void cc_set_instance_stream(cc_instance* inst, void* stream);
+CC_API void* cc_get_instance_stream(const cc_instance* inst);
+/* Bitstream dialect of this instance (CC_DIALECT_GPU by default).  No reference counterpart as a
+ * switch: the CPU dialect is what src/CPU/EncodeCPU.cpp:98-354 reads and writes. */
+CC_API cc_status cc_set_instance_dialect(cc_instance* inst, int dialect);
+
+/* ---- Entropy coding: include/cudaCompress/Encode.h:28-57 ---------------------------------- */
+
+/* encodeRLHuff / encodeHuff (all four overloads each), include/cudaCompress/Encode.h:28,31,34,37,42,45,48,51.
+ * symbolSize: 2 (Symbol16) or 4 (Symbol32); must match the instance (log2 > 16 <=> 4).
+ * d_symbolStreams: host array of streamCount device pointers, symbolCountPerStream symbols each.
+ * startBitOffsets: per stream, the bit offset (0..31) inside the first word at which that stream's
+ *   data will be placed by the caller (NULL = all 0); bits before it in the first word are zero.
+ * On success *outWords points at instance-owned host memory valid until the next encode call on
+ * this instance; stream i occupies words [outSegmentWordBegin[i], ...) and outSegmentBits[i] bits
+ * counted from bit 0 of its first word (so they include startBitOffsets[i] leading zero bits).
+ * Every segment ends 32-bit aligned.  Unlike the reference, encodeHuff does not write padding
+ * into the caller's symbol arrays. */
+CC_API cc_status cc_encode_rlhuff(cc_instance* inst, int symbolSize, const void* const* d_symbolStreams,
+                                  uint32_t streamCount, uint32_t symbolCountPerStream,
+                                  const uint32_t* startBitOffsets, const uint32_t** outWords,
+                                  const uint64_t** outSegmentWordBegin, const uint64_t** outSegmentBits);
+CC_API cc_status cc_encode_huff(cc_instance* inst, int symbolSize, const void* const* d_symbolStreams,
+                                uint32_t streamCount, uint32_t symbolCountPerStream,
+                                const uint32_t* startBitOffsets, const uint32_t** outWords,
+                                const uint64_t** outSegmentWordBegin, const uint64_t** outSegmentBits);
+
+/* decodeRLHuff / decodeHuff, include/cudaCompress/Encode.h:29,32,35,38,43,46,49,52.
+ * bitstreamCount host bitstreams (words, size in bits, current position in bits -- advanced on
+ * return).  streamToBitstream[i] names the bitstream stream i is read from (NULL: all streams
+ * from bitstream 0, back to back, as the single-BitStream overloads do).
+ * decodeRLHuff outputs need not be zeroed by the caller (the reference requires it; this
+ * implementation writes every element).  decodeHuff outputs need no padding.
+ * Returns with the work enqueued on the instance stream (as the reference). */
+CC_API cc_status cc_decode_rlhuff(cc_instance* inst, int symbolSize, const uint32_t* const* bitstreamWords,
+                                  const uint64_t* bitstreamBits, uint64_t* bitstreamPos, uint32_t bitstreamCount,
+                                  const uint32_t* streamToBitstream, void* const* d_symbolStreams,
+                                  uint32_t streamCount, uint32_t symbolCountPerStream);
+CC_API cc_status cc_decode_huff(cc_instance* inst, int symbolSize, const uint32_t* const* bitstreamWords,
+                                const uint64_t* bitstreamBits, uint64_t* bitstreamPos, uint32_t bitstreamCount,
+                                const uint32_t* streamToBitstream, void* const* d_symbolStreams,
+                                uint32_t streamCount, uint32_t symbolCountPerStream);
+
+/* ---- DWT: include/cudaCompress/util/DWT.h:21-155 ------------------------------------------ */
+
+/* dwtFloat3DForward / dwtFloat2DForward (sizeZ == 1, dpBuffer2 unused), util/DWT.h:21-31.
+ * Pitches in elements; 0 selects the reference defaults (src/cudaCompress/util/DWTFloat.cu:474-477). */
+CC_API cc_status cc_dwt_float_forward(float* d_dst, float* d_buffer2, float* d_buffer1, const float* d_src,
+                                      int sizeX, int sizeY, int sizeZ, int srcChannelCount,
+                                      int dstRowPitch, int dstSlicePitch, int srcRowPitch, int srcSlicePitch,
+                                      void* stream);
+/* dwtFloat3DInverse / dwtFloat2DInverse, util/DWT.h:35-45 */
+CC_API cc_status cc_dwt_float_inverse(float* d_dst, float* d_buffer2, float* d_buffer1, const float* d_src,
+                                      int sizeX, int sizeY, int sizeZ, int dstChannelCount,
+                                      int dstRowPitch, int dstSlicePitch, int srcRowPitch, int srcSlicePitch,
+                                      void* stream);
+/* dwtFloat{2D,3D}InverseFromSymbols (ushort and uint highpass), util/DWT.h:48-79.
+ * dd_highpass: DEVICE array of 3 (2D) or 7 (3D) device pointers to dense highpass symbol bands. */
+CC_API cc_status cc_dwt_float_inverse_from_symbols(float* d_dst, float* d_buffer2, float* d_buffer1,
+                                                   const float* d_lowpass, const void* const* dd_highpass,
+                                                   int symbolSize, float quantStep, int sizeX, int sizeY, int sizeZ,
+                                                   int dstChannelCount, int dstRowPitch, int dstSlicePitch,
+                                                   int lowpassRowPitch, int lowpassSlicePitch, void* stream);
+/* dwtIntForward / dwtIntInverse, util/DWT.h:139-151 (sizeZ == 1: 2D) */
+CC_API cc_status cc_dwt_int_forward(int16_t* d_dst, int16_t* d_buffer, const int16_t* d_src,
+                                    int sizeX, int sizeY, int sizeZ, int dstRowPitch, int dstSlicePitch,
+                                    int srcRowPitch, int srcSlicePitch, void* stream);
+CC_API cc_status cc_dwt_int_inverse(int16_t* d_dst, int16_t* d_buffer, const int16_t* d_src,
+                                    int sizeX, int sizeY, int sizeZ, int dstRowPitch, int dstSlicePitch,
+                                    int srcRowPitch, int srcSlicePitch, void* stream);
+
+/* ---- Quantization: include/cudaCompress/util/Quantize.h:29-65 ----------------------------- */
+
+/* symbolize / unsymbolize, util/Quantize.h:29-30 */
+CC_API cc_status cc_symbolize(uint16_t* d_symbols, const int16_t* d_data, uint32_t sizeX, uint32_t sizeY,
+                              uint32_t sizeZ, uint32_t rowPitchSrc, uint32_t slicePitchSrc, void* stream);
+CC_API cc_status cc_unsymbolize(int16_t* d_data, const uint16_t* d_symbols, uint32_t sizeX, uint32_t sizeY,
+                                uint32_t sizeZ, uint32_t rowPitchDst, uint32_t slicePitchDst, void* stream);
+/* quantizeToSymbols / quantizeToSymbolsRoundtrip / unquantizeFromSymbols (ushort and uint,
+ * 3D and 2D), util/Quantize.h:34-50.  quantType: 0 deadzone, 1 uniform. */
+CC_API cc_status cc_quantize_to_symbols(void* d_symbols, int symbolSize, const float* d_data, uint32_t sizeX,
+                                        uint32_t sizeY, uint32_t sizeZ, float quantizationStep, uint32_t rowPitchSrc,
+                                        uint32_t slicePitchSrc, int quantType, void* stream);
+CC_API cc_status cc_quantize_to_symbols_roundtrip(void* d_symbols, int symbolSize, float* d_data, uint32_t sizeX,
+                                                  uint32_t sizeY, uint32_t sizeZ, float quantizationStep,
+                                                  uint32_t rowPitchSrc, uint32_t slicePitchSrc, int quantType,
+                                                  void* stream);
+CC_API cc_status cc_unquantize_from_symbols(float* d_data, const void* d_symbols, int symbolSize, uint32_t sizeX,
+                                            uint32_t sizeY, uint32_t sizeZ, float quantizationStep,
+                                            uint32_t rowPitchDst, uint32_t slicePitchDst, int quantType, void* stream);
+/* quantizeToShort / unquantizeFromShort, util/Quantize.h:54-55 */
+CC_API cc_status cc_quantize_to_short(int16_t* d_quant, const float* d_data, uint32_t sizeX, uint32_t sizeY,
+                                      uint32_t sizeZ, float quantizationStep, uint32_t rowPitchSrc,
+                                      uint32_t slicePitchSrc, int quantType, void* stream);
+CC_API cc_status cc_unquantize_from_short(float* d_data, const int16_t* d_quant, uint32_t sizeX, uint32_t sizeY,
+                                          uint32_t sizeZ, float quantizationStep, uint32_t rowPitchDst,
+                                          uint32_t slicePitchDst, int quantType, void* stream);
+/* getMaxAbs, util/Quantize.h:64 (result written to device memory, instance stream) */
+CC_API cc_status cc_get_max_abs(cc_instance* inst, const float* d_image, uint32_t elemCount, float* d_valMax);
+
+/* ---- Volume codec callers: src/examples/CompressVolume.cpp ------------------------------- */
+
+/* compressVolumeFloat, src/examples/CompressVolume.cpp:313-409 (Symbol16 when symbolSize == 2,
+ * Symbol32 when 4).  Output held by the instance like cc_encode_*. */
+CC_API cc_status cc_compress_volume_float(cc_instance* inst, const float* d_image, uint32_t sizeX, uint32_t sizeY,
+                                          uint32_t sizeZ, uint32_t numLevels, float quantStepLevel0,
+                                          int doRLEOnlyOnLvl0, int symbolSize, const uint32_t** outWords,
+                                          uint64_t* outBits);
+/* decompressVolumeFloat, src/examples/CompressVolume.cpp:412-516 */
+CC_API cc_status cc_decompress_volume_float(cc_instance* inst, float* d_image, uint32_t sizeX, uint32_t sizeY,
+                                            uint32_t sizeZ, uint32_t numLevels, const uint32_t* bits,
+                                            uint64_t bitCount, float quantStepLevel0, int doRLEOnlyOnLvl0,
+                                            int symbolSize);
+/* compressVolumeLossless, src/examples/CompressVolume.cpp:234-277.  sizeZ == 1 uses the
+ * heightfield's three-band layout per level (src/examples/CompressHeightfield.cpp:116-126). */
+CC_API cc_status cc_compress_volume_lossless(cc_instance* inst, const int16_t* d_image, uint32_t sizeX,
+                                             uint32_t sizeY, uint32_t sizeZ, uint32_t numLevels,
+                                             const uint32_t** outWords, uint64_t* outBits);
+/* decompressVolumeLossless, src/examples/CompressVolume.cpp:279-309 */
+CC_API cc_status cc_decompress_volume_lossless(cc_instance* inst, int16_t* d_image, uint32_t sizeX, uint32_t sizeY,
+                                               uint32_t sizeZ, uint32_t numLevels, const uint32_t* bits,
+                                               uint64_t bitCount);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,746 @@
+// CPU oracle for the cudaCompress wavelet-compression path (test infrastructure only).
+//
+// This file restates the reference algorithms on the CPU so the product's HIP path can be
+// checked against them.  Only tests/, the smoke entry point and the CPU-baseline leg of
+// bench.py use it; the product library never links it.
+//
+// Pinning (what checks each part, beyond this restatement itself):
+//   bitstream layout, RLE, CPU-dialect tables, Huffman enc/dec  -- byte-compared against the
+//       reference's own src/CPU coder compiled into oracle/_ref (tests/test_oracle_ref.py) and
+//       against committed golden vectors generated from it (tests/golden/).
+//   GPU-dialect table design  -- libstdc++ std::priority_queue(range ctor)/std::sort, the same
+//       library routines the reference calls; cross-dialect differential test on tie-free
+//       histograms (identical lengths), equal total bits on all histograms (both optimal),
+//       Kraft equality, brute-force optimality on tiny alphabets.
+//   histogram pad quirk -- derived from src/cudaCompress/Histogram.cu:90-105 and the large
+//       histogram kernels' word loop; parity unpinned by execution (no GPU reference here),
+//       pinned by the read of the code only.
+//   CDF 5/3 integer DWT -- exact invertibility (random and extreme inputs, with wrap),
+//       constant input -> zero highpass, ramps -> zero highpass away from the borders,
+//       brute-force recomputation of the lifting steps on tiny arrays by direct formula.
+//   CDF 9/7 float DWT -- fp64 twin agreement to 1e-5, constant input -> lowpass == c*sum(aL),
+//       perfect reconstruction to the 7-digit coefficient precision, linearity.
+//   quantizer -- closed forms (symbol zig-zag bijection, bin edges).
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <climits>
+#include <queue>
+#include <vector>
+
+namespace {
+
+// ---------------------------------------------------------------------------------------
+// Bit writer/reader: MSB-first inside 32-bit words, include/cudaCompress/BitStream.h:377-456
+// (writeBits, writeAligned, writeAlign) and :169-244 (readBits, readAligned, align).
+// ---------------------------------------------------------------------------------------
+struct Writer {
+    std::vector<uint32_t> w;
+    uint64_t pos = 0;  // bit position
+    void ensure(uint64_t bits) {
+        size_t need = (size_t)((bits + 31) / 32);
+        if (w.size() < need) w.resize(need, 0u);
+    }
+    void bits(uint32_t data, uint32_t n) {
+        ensure(pos + n);
+        for (uint32_t i = 0; i < n; i++) {
+            uint32_t bit = (data >> (n - 1 - i)) & 1u;
+            uint64_t p = pos + i;
+            uint32_t& word = w[p / 32];
+            uint32_t sh = 31 - (uint32_t)(p % 32);
+            word = (word & ~(1u << sh)) | (bit << sh);
+        }
+        pos += n;
+    }
+    void align32() {
+        pos = (pos + 31) / 32 * 32;
+        ensure(pos);
+    }
+    void aligned_u32(const uint32_t* v, uint64_t count) {
+        align32();
+        ensure(pos + count * 32);
+        for (uint64_t i = 0; i < count; i++) w[pos / 32 + i] = v[i];
+        pos += count * 32;
+    }
+    void aligned_u16(const uint16_t* v, uint64_t count) {
+        align32();
+        ensure(pos + count * 16);
+        unsigned char* base = reinterpret_cast<unsigned char*>(w.data()) + pos / 8;
+        memcpy(base, v, count * 2);  // little-endian bytes, BitStream.h:421-456
+        pos += count * 16;
+        align32();
+    }
+};
+
+struct Reader {
+    const uint32_t* w;
+    uint64_t size;
+    uint64_t pos;
+    uint32_t bits(uint32_t n) {
+        uint32_t v = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            uint32_t bit = 0;
+            if (pos < size) bit = (w[pos / 32] >> (31 - (uint32_t)(pos % 32))) & 1u;
+            v = (v << 1) | bit;
+            pos++;
+        }
+        return v;
+    }
+    void align32() { pos = (pos + 31) / 32 * 32; }
+    uint32_t aligned_u32() {
+        align32();
+        uint32_t v = w[pos / 32];
+        pos += 32;
+        return v;
+    }
+    const uint32_t* word_ptr() { return w + pos / 32; }
+    void skip_bits(uint64_t n) { pos += n; }
+};
+
+uint32_t required_bits(uint32_t v) {  // src/cudaCompress/util.h:23-32
+    uint32_t b = 0;
+    while (v > 0) { b++; v >>= 1; }
+    return b;
+}
+
+uint32_t load_sym(const void* p, int sz, uint64_t i) {
+    return sz == 2 ? ((const uint16_t*)p)[i] : ((const uint32_t*)p)[i];
+}
+void store_sym(void* p, int sz, uint64_t i, uint32_t v) {
+    if (sz == 2) ((uint16_t*)p)[i] = (uint16_t)v; else ((uint32_t*)p)[i] = v;
+}
+
+// ---------------------------------------------------------------------------------------
+// Huffman table design, src/cudaCompress/HuffmanTable.cu:487-586 and HuffmanDesign.h:13-65.
+// ---------------------------------------------------------------------------------------
+struct Node {
+    uint32_t symbol;
+    uint32_t prob;
+    int len;
+    Node* left;
+    Node* right;
+    void init(uint32_t s, uint32_t p) { symbol = s; prob = p; len = -1; left = right = nullptr; }
+    int assign(int l) {
+        len = l;
+        int r = l;
+        if (left) r = std::max(r, left->assign(l + 1));
+        if (right) r = std::max(r, right->assign(l + 1));
+        return r;
+    }
+};
+struct InvProbCmp {  // HuffmanDesign.h:47-55 (no tie break)
+    bool operator()(const Node* a, const Node* b) const { return b->prob < a->prob; }
+};
+struct LenCmp {  // HuffmanDesign.h:57-63
+    bool operator()(const Node* a, const Node* b) const { return a->len < b->len; }
+};
+
+struct Table {
+    std::vector<uint32_t> count_per_length;  // index = length-1
+    std::vector<uint32_t> symbols;           // by codeword index
+    uint32_t symbol_max = 0;                 // written as symbolBits source
+    // encode tables by symbol
+    std::vector<uint32_t> code, len;
+};
+
+int design(int dialect, const uint32_t* hist, uint32_t symbol_max, Table& t) {
+    t = Table();
+    t.symbol_max = symbol_max;
+    uint32_t distinct = symbol_max + 1;
+    std::vector<Node> nodes(2 * (size_t)distinct - 1);
+    uint32_t next = 0;
+    std::vector<Node*> leaves;
+    for (uint32_t s = 0; s < distinct; s++) {
+        if (hist[s] > 0) {
+            nodes[next].init(s, hist[s]);
+            leaves.push_back(&nodes[next]);
+            next++;
+        }
+    }
+    t.code.assign(distinct, 0u);
+    t.len.assign(distinct, 0xFFFFFFFFu);
+    if (leaves.empty()) return ORC_OK;
+
+    typedef std::priority_queue<Node*, std::vector<Node*>, InvProbCmp> PQ;
+    PQ pq;
+    if (dialect == ORC_DIALECT_GPU) {
+        pq = PQ(leaves.begin(), leaves.end());  // range ctor = make_heap, HuffmanTable.cu:507
+    } else {
+        for (size_t i = 0; i < leaves.size(); i++) pq.push(leaves[i]);  // HuffmanTableCPU.cpp:60-62
+    }
+    while (pq.size() > 1) {
+        Node& nn = nodes[next++];
+        nn.init(0xFFFFFFFFu, 0);
+        Node* l = pq.top(); pq.pop();
+        Node* r = pq.top(); pq.pop();
+        nn.left = l; nn.right = r;
+        nn.prob = l->prob + r->prob;
+        pq.push(&nn);
+    }
+    int lmax = pq.top()->assign(0);
+    std::sort(leaves.begin(), leaves.end(), LenCmp());
+    if (lmax > 31) return ORC_ERR_CODEWORD_TOO_LONG;
+    t.count_per_length.assign((size_t)lmax, 0u);
+    t.symbols.resize(leaves.size());
+    for (size_t i = 0; i < leaves.size(); i++) {
+        if (leaves[i]->len > 0) t.count_per_length[leaves[i]->len - 1]++;
+        t.symbols[i] = leaves[i]->symbol;
+    }
+    // canonical codewords by index, HuffmanTable.cu:553-580
+    size_t ncw = leaves.size();
+    std::vector<uint32_t> lbi(ncw), cbi(ncw);
+    if (lmax > 0) {
+        size_t idx = 0;
+        for (int l = 1; l <= lmax; l++)
+            for (uint32_t i = 0; i < t.count_per_length[l - 1]; i++) lbi[idx++] = (uint32_t)l;
+    } else {
+        lbi[0] = 0;
+    }
+    cbi[0] = 0;
+    for (size_t i = 1; i < ncw; i++) cbi[i] = (cbi[i - 1] + 1) << (lbi[i] - lbi[i - 1]);
+    for (size_t i = 0; i < ncw; i++) {
+        t.code[t.symbols[i]] = cbi[i];
+        t.len[t.symbols[i]] = lbi[i];
+    }
+    return ORC_OK;
+}
+
+// Header, GPU dialect HuffmanTable.cu:470-485; CPU dialect HuffmanTableCPU.cpp:108-135.
+void write_table(const orc_params* p, const Table& t, Writer& w) {
+    bool cpu = p->dialect == ORC_DIALECT_CPU;
+    uint32_t fld = cpu ? 14u : (uint32_t)p->log2;
+    w.bits((uint32_t)t.count_per_length.size(), 5);
+    for (size_t i = 0; i < t.count_per_length.size(); i++) w.bits(t.count_per_length[i], fld);
+    w.bits((uint32_t)t.symbols.size(), fld);
+    uint32_t symmax = t.symbol_max;
+    if (cpu) {  // CPU dialect takes the max over the table's symbols
+        symmax = 0;
+        for (size_t i = 0; i < t.symbols.size(); i++) symmax = std::max(symmax, t.symbols[i]);
+    }
+    uint32_t sb = required_bits(symmax);
+    bool longsym = !cpu && p->log2 > 16;
+    w.bits(sb, cpu ? 14u : (longsym ? 5u : 4u));
+    for (size_t i = 0; i < t.symbols.size(); i++) w.bits(t.symbols[i], sb);
+}
+
+struct DecTable {
+    int first[32], mn[32], mx[32];
+    std::vector<uint32_t> symbols;
+    int nlen = 0;
+};
+
+void read_table(const orc_params* p, Reader& r, DecTable& d) {
+    bool cpu = p->dialect == ORC_DIALECT_CPU;
+    uint32_t fld = cpu ? 14u : (uint32_t)p->log2;
+    bool longsym = !cpu && p->log2 > 16;
+    for (int i = 0; i < 32; i++) d.first[i] = d.mn[i] = d.mx[i] = -1;
+    uint32_t L = r.bits(5);
+    std::vector<uint32_t> cnt(L);
+    for (uint32_t i = 0; i < L; i++) cnt[i] = r.bits(fld);
+    uint32_t nsym = r.bits(fld);
+    uint32_t sb = r.bits(cpu ? 14u : (longsym ? 5u : 4u));
+    // HuffmanTable.cu:138-139: a stored 0 with more than one symbol means 16
+    if (!cpu && !longsym && sb == 0 && nsym > 1) sb = 16;
+    d.symbols.resize(nsym);
+    for (uint32_t i = 0; i < nsym; i++) d.symbols[i] = r.bits(sb);
+    d.nlen = (int)L;
+    if (nsym == 0 || L == 0) return;
+    // HuffmanTable.cu:205-266
+    std::vector<uint32_t> lens;
+    for (uint32_t i = 0; i < L; i++) lens.insert(lens.end(), cnt[i], i + 1);
+    std::vector<int> cw(lens.size());
+    if (!lens.empty()) {
+        cw[0] = 0;
+        for (size_t i = 1; i < lens.size(); i++) cw[i] = (cw[i - 1] + 1) << (lens[i] - lens[i - 1]);
+    }
+    for (uint32_t l = 0, e = 0; l < L; l++) {
+        if (cnt[l] > 0) {
+            d.first[l] = (int)e;
+            d.mn[l] = cw[e];
+            e += cnt[l] - 1;
+            d.mx[l] = cw[e];
+            e++;
+        }
+    }
+}
+
+// Bit-serial canonical decode of one symbol, src/CPU/HuffmanCPU.cpp:23-49.
+uint32_t decode_one(const DecTable& d, Reader& r, int* err) {
+    if (d.symbols.size() <= 1 || d.nlen == 0) {
+        if (d.symbols.empty()) { *err = 1; return 0; }
+        return d.symbols[0];
+    }
+    int code = 0;
+    int len = 0;
+    for (;;) {
+        code = (code << 1) | (int)r.bits(1);
+        len++;
+        if (len > d.nlen) { *err = 1; return 0; }
+        if (code <= d.mx[len - 1]) break;
+    }
+    int idx = d.first[len - 1] + code - d.mn[len - 1];
+    if (idx < 0 || (size_t)idx >= d.symbols.size()) { *err = 1; return 0; }
+    return d.symbols[(size_t)idx];
+}
+
+// RLE, RunLengthKernels.cui:5-44 (fillers (0,255) for every 256 zeros, trailing zeros dropped).
+uint32_t rle(int sz, const void* s, uint32_t n, std::vector<uint32_t>& sym, std::vector<uint32_t>& zc) {
+    sym.clear(); zc.clear();
+    uint32_t run = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t v = load_sym(s, sz, i);
+        if (v == 0) { run++; continue; }
+        for (uint32_t e = 0; e < run / 256; e++) { sym.push_back(0); zc.push_back(255); }
+        sym.push_back(v);
+        zc.push_back(run % 256);
+        run = 0;
+    }
+    return (uint32_t)sym.size();
+}
+
+void hist_with_quirk(int sz, const std::vector<uint32_t>& v, uint32_t symbol_max, std::vector<uint32_t>& h) {
+    h.assign((size_t)symbol_max + 1, 0u);
+    for (size_t i = 0; i < v.size(); i++) h[v[i]]++;
+    // Symbol16 streams are padded with 0xFFFF to a multiple of 8 (Histogram.cu:90-96); the
+    // large-bin kernels read whole 32-bit words, so with 65536 bins an odd count adds one
+    // 0xFFFF to bin 65535.
+    if (sz == 2 && symbol_max == 65535u && (v.size() & 1u)) h[65535]++;
+}
+
+void huff_encode_stream(const Table& t, const std::vector<uint32_t>& v, uint32_t cbs,
+                        std::vector<uint32_t>& words, uint32_t& nbits, std::vector<uint32_t>& offsets) {
+    Writer w;
+    offsets.clear();
+    for (size_t i = 0; i < v.size(); i++) {
+        if (i % cbs == 0) offsets.push_back((uint32_t)w.pos);
+        w.bits(t.code[v[i]], t.len[v[i]]);
+    }
+    nbits = (uint32_t)w.pos;
+    words = w.w;
+    words.resize((nbits + 31) / 32);
+}
+
+void write_coded(const std::vector<uint32_t>& words, uint32_t nbits, const std::vector<uint32_t>& offsets, Writer& w) {
+    w.aligned_u32(&nbits, 1);
+    w.aligned_u32(words.data(), (nbits + 31) / 32);
+    std::vector<uint16_t> inc(offsets.size());
+    if (!offsets.empty()) {
+        inc[0] = (uint16_t)offsets[0];
+        for (size_t i = 1; i < offsets.size(); i++) inc[i] = (uint16_t)(offsets[i] - offsets[i - 1]);
+    }
+    w.aligned_u16(inc.data(), inc.size());
+}
+
+// Reads nbits, codeword words and offsets; decodes `count` symbols coding block by coding block
+// from the stored offsets (the way huffmanDecodeKernel starts each block).
+int read_and_decode(const orc_params* p, const DecTable& d, Reader& r, uint32_t count, std::vector<uint32_t>& out) {
+    uint32_t nbits = r.aligned_u32();
+    const uint32_t* cw = r.word_ptr();
+    uint64_t nwords = (nbits + 31) / 32;
+    r.skip_bits(nwords * 32);
+    uint32_t noff = (count + (uint32_t)p->cbs - 1) / (uint32_t)p->cbs;
+    r.align32();
+    const uint16_t* inc = reinterpret_cast<const uint16_t*>(r.word_ptr());
+    std::vector<uint32_t> off(noff);
+    uint32_t total = 0;
+    for (uint32_t i = 0; i < noff; i++) { total += inc[i]; off[i] = total; }
+    r.skip_bits((uint64_t)noff * 16);
+    r.align32();
+    out.assign(count, 0u);
+    int err = 0;
+    for (uint32_t b = 0; b < noff; b++) {
+        Reader br{cw, nwords * 32, off[b]};
+        uint32_t end = std::min(count, (b + 1) * (uint32_t)p->cbs);
+        for (uint32_t i = b * (uint32_t)p->cbs; i < end; i++) {
+            out[i] = decode_one(d, br, &err);
+            if (err) return ORC_ERR_CORRUPT;
+        }
+    }
+    return ORC_OK;
+}
+
+// ----------------------------- transforms --------------------------------------------
+const float kAL[9] = {0.0267488f, -0.0168641f, -0.0782233f, 0.2668641f, 0.6029490f,
+                      0.2668641f, -0.0782233f, -0.0168641f, 0.0267488f};
+const float kAH[9] = {0.0f, 0.0456359f, -0.0287718f, -0.2956359f, 0.5575435f,
+                      -0.2956359f, -0.0287718f, 0.0456359f, 0.0f};
+const float kGE[9] = {0.0f, 0.0337282f, -0.0575435f, -0.5337281f, 1.1150871f,
+                      -0.5337281f, -0.0575435f, 0.0337282f, 0.0f};
+const float kGO[9] = {0.0534975f, -0.0912718f, -0.1564465f, 0.5912718f, 1.2058980f,
+                      0.5912718f, -0.1564465f, -0.0912718f, 0.0534975f};
+const double kALd[9] = {0.0267488, -0.0168641, -0.0782233, 0.2668641, 0.6029490,
+                        0.2668641, -0.0782233, -0.0168641, 0.0267488};
+const double kAHd[9] = {0.0, 0.0456359, -0.0287718, -0.2956359, 0.5575435,
+                        -0.2956359, -0.0287718, 0.0456359, 0.0};
+const double kGEd[9] = {0.0, 0.0337282, -0.0575435, -0.5337281, 1.1150871,
+                        -0.5337281, -0.0575435, 0.0337282, 0.0};
+const double kGOd[9] = {0.0534975, -0.0912718, -0.1564465, 0.5912718, 1.2058980,
+                        0.5912718, -0.1564465, -0.0912718, 0.0534975};
+
+// whole-sample symmetric extension, DWTCommon.cuh:55-56 (mirrorLeft/mirrorRight), applied
+// until in range so lines shorter than the filter stay defined
+int mirror(int j, int n) {
+    if (n == 1) return 0;
+    for (;;) {
+        if (j < 0) j = -j;
+        else if (j > n - 1) j = 2 * (n - 1) - j;
+        else return j;
+    }
+}
+// inverse extensions, DWTCommon.cuh:58-77 as used at DWTFloatKernels.cui:456-476
+int ext_low(int g, int m) {
+    for (;;) {
+        if (g < 0) g = -g;
+        else if (g >= m) g = 2 * m - 1 - g;
+        else return g;
+    }
+}
+int ext_high(int g, int m) {
+    for (;;) {
+        if (g < 0) g = -g - 1;
+        else if (g >= m) g = 2 * m - 2 - g;
+        else return g;
+    }
+}
+
+template <typename T> T fmac(T a, T b, T c);
+template <> float fmac<float>(float a, float b, float c) { return fmaf(a, b, c); }
+template <> double fmac<double>(double a, double b, double c) { return fma(a, b, c); }
+
+template <typename T>
+void fwd97_line(const T* x, T* y, int n, const T* aL, const T* aH) {
+    int m = n / 2;
+    for (int k = 0; k < m; k++) {
+        T s = T(0);
+        for (int i = 0; i < 9; i++) s = fmac<T>(aL[i], x[mirror(2 * k - 4 + i, n)], s);
+        y[k] = s;
+        s = T(0);
+        for (int i = 0; i < 9; i++) s = fmac<T>(aH[i], x[mirror(2 * k - 3 + i, n)], s);
+        y[m + k] = s;
+    }
+}
+
+template <typename T>
+void inv97_line(const T* c, T* y, int n, const T* gE, const T* gO) {
+    int m = n / 2;
+    for (int j = 0; j < n; j++) {
+        const T* g = (j % 2 == 0) ? gE : gO;
+        T s = T(0);
+        for (int i = 0; i < 9; i++) {
+            int v = j - 4 + i;
+            int gi = (v >= 0) ? v / 2 : -((-v + 1) / 2);  // floor(v/2)
+            T val = (v - 2 * gi == 0) ? c[ext_low(gi, m)] : c[m + ext_high(gi, m)];
+            s = fmac<T>(g[i], val, s);
+        }
+        y[j] = s;
+    }
+}
+
+// CDF 5/3 helpers with int16 truncation, DWTCommon.cuh:133-142
+int16_t div2(int16_t v) { v = (int16_t)(v + ((v >= 0) ? 1 : -1)); return (int16_t)(v / 2); }
+int16_t div4(int16_t v) { v = (int16_t)(v + ((v >= 0) ? 2 : -2)); return (int16_t)(v / 4); }
+
+void fwd53_line(const int16_t* x, int16_t* y, int n) {
+    int m = n / 2;
+    std::vector<int16_t> d(m);
+    for (int k = 0; k < m; k++)
+        d[k] = (int16_t)(x[mirror(2 * k + 1, n)] - div2((int16_t)(x[2 * k] + x[mirror(2 * k + 2, n)])));
+    for (int k = 0; k < m; k++) {
+        int16_t dl = d[k > 0 ? k - 1 : 0];
+        y[k] = (int16_t)(x[2 * k] + div4((int16_t)(dl + d[k])));
+        y[m + k] = d[k];
+    }
+}
+
+void inv53_line(const int16_t* c, int16_t* y, int n) {
+    int m = n / 2;
+    std::vector<int16_t> s(m);
+    for (int k = 0; k < m; k++) {
+        int16_t dl = c[m + (k > 0 ? k - 1 : 0)];
+        s[k] = (int16_t)(c[k] - div4((int16_t)(dl + c[m + k])));
+    }
+    for (int k = 0; k < m; k++) {
+        int16_t sr = s[k + 1 < m ? k + 1 : m - 1];
+        y[2 * k] = s[k];
+        y[2 * k + 1] = (int16_t)(c[m + k] + div2((int16_t)(s[k] + sr)));
+    }
+}
+
+template <typename T, typename F>
+void apply_axis(T* data, int sx, int sy, int sz, int axis, F fn) {
+    int dims[3] = {sx, sy, sz};
+    long long strides[3] = {1, sx, (long long)sx * sy};
+    int n = dims[axis];
+    long long st = strides[axis];
+    std::vector<T> a(n), b(n);
+    int o1 = axis == 0 ? 1 : 0, o2 = axis == 2 ? 1 : 2;
+    for (int j2 = 0; j2 < dims[o2]; j2++)
+        for (int j1 = 0; j1 < dims[o1]; j1++) {
+            long long base = j1 * strides[o1] + j2 * strides[o2];
+            for (int i = 0; i < n; i++) a[i] = data[base + i * st];
+            fn(a.data(), b.data(), n);
+            for (int i = 0; i < n; i++) data[base + i * st] = b[i];
+        }
+}
+
+int encode_impl(int rl, const orc_params* p, const void* const* streams, uint32_t ns, uint32_t count,
+                uint32_t start_bit, Writer& w) {
+    bool cpu = p->dialect == ORC_DIALECT_CPU;
+    int sz = p->symbol_size;
+    uint32_t cbs = (uint32_t)p->cbs;
+    w.pos = start_bit;
+    w.ensure(start_bit);
+    std::vector<std::vector<uint32_t>> syms(ns), zcs(ns);
+    std::vector<uint32_t> smax(ns, 0);
+    uint32_t callmax = 0;
+    for (uint32_t s = 0; s < ns; s++) {
+        if (rl) {
+            rle(sz, streams[s], count, syms[s], zcs[s]);
+        } else {
+            syms[s].resize(count);
+            for (uint32_t i = 0; i < count; i++) syms[s][i] = load_sym(streams[s], sz, i);
+        }
+        for (size_t i = 0; i < syms[s].size(); i++) smax[s] = std::max(smax[s], syms[s][i]);
+        callmax = std::max(callmax, smax[s]);
+    }
+    uint32_t log2 = cpu ? 14u : (uint32_t)p->log2;
+    if (!cpu && (uint64_t)callmax + 1 > (1ull << log2)) return ORC_ERR_SYMBOL_OVERFLOW;  // HuffmanTable.cu:409-419
+    for (uint32_t s = 0; s < ns; s++) {
+        const std::vector<uint32_t>& v = syms[s];
+        uint32_t M = (uint32_t)v.size();
+        if (rl) w.aligned_u32(&M, 1);
+        std::vector<uint32_t> h;
+        if (cpu) { h.assign((size_t)smax[s] + 1, 0u); for (size_t i = 0; i < v.size(); i++) h[v[i]]++; }
+        else hist_with_quirk(sz, v, smax[s], h);
+        Table t;
+        int rc = design(p->dialect, h.data(), smax[s], t);
+        if (rc) return rc;
+        write_table(p, t, w);
+        std::vector<uint32_t> words, offs;
+        uint32_t nb;
+        huff_encode_stream(t, v, cbs, words, nb, offs);
+        write_coded(words, nb, offs, w);
+        if (rl) {
+            const std::vector<uint32_t>& z = zcs[s];
+            uint32_t zmax = 0;
+            for (size_t i = 0; i < z.size(); i++) zmax = std::max(zmax, z[i]);
+            std::vector<uint32_t> hz((size_t)zmax + 1, 0u);
+            for (size_t i = 0; i < z.size(); i++) hz[z[i]]++;
+            Table tz;
+            rc = design(p->dialect, hz.data(), zmax, tz);
+            if (rc) return rc;
+            write_table(p, tz, w);
+            huff_encode_stream(tz, z, cbs, words, nb, offs);
+            write_coded(words, nb, offs, w);
+        }
+    }
+    return ORC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_free(void* p) { free(p); }
+
+int orc_rle_encode(int symbol_size, const void* syms, uint32_t n, void* out_syms, void* out_zc, uint32_t* out_m) {
+    std::vector<uint32_t> s, z;
+    uint32_t m = rle(symbol_size, syms, n, s, z);
+    for (uint32_t i = 0; i < m; i++) { store_sym(out_syms, symbol_size, i, s[i]); store_sym(out_zc, symbol_size, i, z[i]); }
+    *out_m = m;
+    return ORC_OK;
+}
+
+int orc_rle_decode(int symbol_size, const void* syms, const void* zc, uint32_t m, void* out, uint32_t n) {
+    uint64_t idx = 0;
+    for (uint32_t i = 0; i < m; i++) {
+        idx += load_sym(zc, symbol_size, i);
+        if (idx >= n) return ORC_ERR_CORRUPT;
+        store_sym(out, symbol_size, idx, load_sym(syms, symbol_size, i));
+        idx++;
+    }
+    return ORC_OK;
+}
+
+void orc_histogram(int symbol_size, const void* syms, uint32_t n, uint32_t call_symbol_max, uint32_t* hist, uint32_t nbins) {
+    memset(hist, 0, sizeof(uint32_t) * nbins);
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t v = load_sym(syms, symbol_size, i);
+        if (v < nbins) hist[v]++;
+    }
+    if (symbol_size == 2 && call_symbol_max == 65535u && (n & 1u) && nbins > 65535u) hist[65535]++;
+}
+
+int orc_huff_design(int dialect, const uint32_t* hist, uint32_t stream_symbol_max, uint32_t* lengths, uint32_t* codewords,
+                    uint32_t* num_lengths, uint32_t* count_per_length, uint32_t* symbols, uint32_t* nsyms) {
+    Table t;
+    int rc = design(dialect, hist, stream_symbol_max, t);
+    if (rc) return rc;
+    for (uint32_t s = 0; s <= stream_symbol_max; s++) { lengths[s] = t.len[s]; codewords[s] = t.code[s]; }
+    *num_lengths = (uint32_t)t.count_per_length.size();
+    for (int i = 0; i < 32; i++) count_per_length[i] = i < (int)t.count_per_length.size() ? t.count_per_length[i] : 0;
+    for (size_t i = 0; i < t.symbols.size(); i++) symbols[i] = t.symbols[i];
+    *nsyms = (uint32_t)t.symbols.size();
+    return ORC_OK;
+}
+
+int orc_encode(int rl, const orc_params* p, const void* const* streams, uint32_t nstreams, uint32_t count,
+               uint32_t start_bit, uint32_t** out_words, uint64_t* out_words_len, uint64_t* out_bits) {
+    if (!p || (p->symbol_size != 2 && p->symbol_size != 4) || start_bit > 31 || p->cbs <= 0) return ORC_ERR_INVALID;
+    if (p->dialect == ORC_DIALECT_CPU && p->symbol_size != 2) return ORC_ERR_INVALID;
+    Writer w;
+    int rc = encode_impl(rl, p, streams, nstreams, count, start_bit, w);
+    if (rc) return rc;
+    size_t nw = (size_t)((w.pos + 31) / 32);
+    w.w.resize(nw, 0u);
+    uint32_t* o = (uint32_t*)malloc(sizeof(uint32_t) * (nw ? nw : 1));
+    if (nw) memcpy(o, w.w.data(), nw * 4);
+    *out_words = o;
+    *out_words_len = nw;
+    *out_bits = w.pos;
+    return ORC_OK;
+}
+
+int orc_decode(int rl, const orc_params* p, const uint32_t* words, uint64_t nbits, uint64_t* bitpos,
+               void* const* out_streams, uint32_t nstreams, uint32_t count) {
+    if (!p || (p->symbol_size != 2 && p->symbol_size != 4)) return ORC_ERR_INVALID;
+    Reader r{words, nbits, *bitpos};
+    int sz = p->symbol_size;
+    for (uint32_t s = 0; s < nstreams; s++) {
+        if (rl) {
+            uint32_t M = r.aligned_u32();
+            DecTable d;
+            read_table(p, r, d);
+            std::vector<uint32_t> sym, zc;
+            int rc = read_and_decode(p, d, r, M, sym);
+            if (rc) return rc;
+            DecTable dz;
+            read_table(p, r, dz);
+            rc = read_and_decode(p, dz, r, M, zc);
+            if (rc) return rc;
+            for (uint32_t i = 0; i < count; i++) store_sym(out_streams[s], sz, i, 0);
+            uint64_t idx = 0;
+            for (uint32_t i = 0; i < M; i++) {
+                idx += zc[i];
+                if (idx >= count) return ORC_ERR_CORRUPT;
+                store_sym(out_streams[s], sz, idx, sym[i]);
+                idx++;
+            }
+        } else {
+            DecTable d;
+            read_table(p, r, d);
+            std::vector<uint32_t> sym;
+            int rc = read_and_decode(p, d, r, count, sym);
+            if (rc) return rc;
+            for (uint32_t i = 0; i < count; i++) store_sym(out_streams[s], sz, i, sym[i]);
+        }
+        if (r.pos > nbits) return ORC_ERR_CORRUPT;
+    }
+    *bitpos = r.pos;
+    return ORC_OK;
+}
+
+void orc_pack_inc16(const uint32_t* values, uint16_t* incs, uint32_t n) {
+    uint32_t prev = n ? values[0] : 0;
+    if (n) incs[0] = (uint16_t)prev;
+    for (uint32_t i = 1; i < n; i++) { incs[i] = (uint16_t)(values[i] - prev); prev = values[i]; }
+}
+
+void orc_unpack_inc16(const uint16_t* incs, uint32_t* values, uint32_t n) {
+    uint32_t t = 0;
+    for (uint32_t i = 0; i < n; i++) { t += incs[i]; values[i] = t; }
+}
+
+void orc_dwt97_forward_f32(float* data, int sx, int sy, int sz) {
+    auto f = [](const float* a, float* b, int n) { fwd97_line<float>(a, b, n, kAL, kAH); };
+    apply_axis<float>(data, sx, sy, sz, 0, f);
+    apply_axis<float>(data, sx, sy, sz, 1, f);
+    if (sz > 1) apply_axis<float>(data, sx, sy, sz, 2, f);
+}
+void orc_dwt97_inverse_f32(float* data, int sx, int sy, int sz) {
+    auto f = [](const float* a, float* b, int n) { inv97_line<float>(a, b, n, kGE, kGO); };
+    if (sz > 1) apply_axis<float>(data, sx, sy, sz, 2, f);
+    apply_axis<float>(data, sx, sy, sz, 1, f);
+    apply_axis<float>(data, sx, sy, sz, 0, f);
+}
+void orc_dwt97_forward_f64(double* data, int sx, int sy, int sz) {
+    auto f = [](const double* a, double* b, int n) { fwd97_line<double>(a, b, n, kALd, kAHd); };
+    apply_axis<double>(data, sx, sy, sz, 0, f);
+    apply_axis<double>(data, sx, sy, sz, 1, f);
+    if (sz > 1) apply_axis<double>(data, sx, sy, sz, 2, f);
+}
+void orc_dwt97_inverse_f64(double* data, int sx, int sy, int sz) {
+    auto f = [](const double* a, double* b, int n) { inv97_line<double>(a, b, n, kGEd, kGOd); };
+    if (sz > 1) apply_axis<double>(data, sx, sy, sz, 2, f);
+    apply_axis<double>(data, sx, sy, sz, 1, f);
+    apply_axis<double>(data, sx, sy, sz, 0, f);
+}
+void orc_dwt53_forward(int16_t* data, int sx, int sy, int sz) {
+    apply_axis<int16_t>(data, sx, sy, sz, 0, fwd53_line);
+    apply_axis<int16_t>(data, sx, sy, sz, 1, fwd53_line);
+    if (sz > 1) apply_axis<int16_t>(data, sx, sy, sz, 2, fwd53_line);
+}
+void orc_dwt53_inverse(int16_t* data, int sx, int sy, int sz) {
+    if (sz > 1) apply_axis<int16_t>(data, sx, sy, sz, 2, inv53_line);
+    apply_axis<int16_t>(data, sx, sy, sz, 1, inv53_line);
+    apply_axis<int16_t>(data, sx, sy, sz, 0, inv53_line);
+}
+
+static int sat_f2i(float v) {  // saturating truncation, as v_cvt_i32_f32 / cvt.rzi.s32.f32
+    if (v != v) return 0;
+    if (v >= 2147483648.0f) return INT_MAX;
+    if (v <= -2147483648.0f) return INT_MIN;
+    return (int)v;
+}
+
+void orc_quantize(const float* data, uint32_t n, float step, int qtype, int symbol_size, void* symbols) {
+    volatile float inv = 1.0f / step;
+    for (uint32_t i = 0; i < n; i++) {
+        volatile float prod = data[i] * inv;
+        int q;
+        if (qtype == 0) {
+            q = sat_f2i(prod);
+        } else {
+            float sg = data[i] > 0.0f ? 1.0f : (data[i] < 0.0f ? -1.0f : 0.0f);
+            volatile float t = prod + sg * 0.5f;
+            q = sat_f2i(t);
+        }
+        uint32_t s = 2u * (uint32_t)std::abs((long long)q) - (q < 0 ? 1u : 0u);
+        store_sym(symbols, symbol_size, i, s);
+    }
+}
+
+void orc_unquantize(const void* symbols, uint32_t n, float step, int qtype, int symbol_size, float* data) {
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t s = load_sym(symbols, symbol_size, i);
+        uint32_t neg = s % 2u;
+        int q = (int)((1u - 2u * neg) * ((s + neg) / 2u));
+        if (qtype == 0) {
+            float sg = q < 0 ? -1.0f : (q > 0 ? 1.0f : 0.0f);
+            volatile float t = (float)q + sg * 0.5f;
+            data[i] = t * step;
+        } else {
+            data[i] = (float)q * step;
+        }
+    }
+}
+
+void orc_symbolize(const int16_t* data, uint32_t n, uint16_t* symbols) {
+    for (uint32_t i = 0; i < n; i++) {
+        int v = data[i];
+        symbols[i] = (uint16_t)(2u * (uint32_t)std::abs(v) - (v < 0 ? 1u : 0u));
+    }
+}
+
+void orc_unsymbolize(const uint16_t* symbols, uint32_t n, int16_t* data) {
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t s = symbols[i];
+        uint32_t neg = s % 2u;
+        data[i] = (int16_t)(int)((1u - 2u * neg) * ((s + neg) / 2u));
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,116 @@
+"""CPU composition of the volume codecs out of oracle primitives (test infrastructure only).
+
+Restates the caller layouts of src/examples/CompressVolume.cpp (compressVolumeFloat :313-409,
+decompressVolumeFloat :412-516, compressVolumeLossless :234-309) and, for 2D lossless fields, the
+band layout of src/examples/CompressHeightfield.cpp:116-126, on top of oracle/liboracle.so.
+Used by tests/ as the checker and by bench.py as the timed CPU baseline.
+"""
+import numpy as np
+
+from . import pyoracle as o
+
+
+def _bands3d(level_shape):
+    z, y, x = level_shape
+    bz, by, bx = z // 2, y // 2, x // 2
+    out = []
+    for b in range(1, 8):  # CompressVolume.cpp:375-379
+        ox, oy, oz = b % 2, b // 2 % 2, b // 4
+        out.append((slice(oz * bz, (oz + 1) * bz), slice(oy * by, (oy + 1) * by), slice(ox * bx, (ox + 1) * bx)))
+    return out
+
+
+def _bands2d(level_shape):
+    y, x = level_shape
+    by, bx = y // 2, x // 2
+    out = []
+    for b in range(1, 4):  # CompressHeightfield.cpp:119-121
+        ox, oy = b % 2, b // 2
+        out.append((slice(oy * by, (oy + 1) * by), slice(ox * bx, (ox + 1) * bx)))
+    return out
+
+
+def _step(q0, divisor):
+    return np.float32(np.float32(q0) / np.float32(divisor))
+
+
+def compress_volume_float(vol, levels, q0, log2=14, cbs=128, dtype=np.uint16, rle_only_lvl0=False, coef_out=None):
+    vol = np.ascontiguousarray(vol, dtype=np.float32)
+    Z, Y, X = vol.shape
+    coef = vol.copy()
+    for lv in range(levels):
+        s = (slice(0, Z >> lv), slice(0, Y >> lv), slice(0, X >> lv))
+        coef[s] = o.dwt97_forward(coef[s])
+    if coef_out is not None:
+        coef_out.append(coef.copy())
+    words = []
+    f = 1 << levels
+    lp = coef[: Z // f, : Y // f, : X // f]
+    q = _step(q0, f // 2)
+    w, _ = o.encode([o.quantize(lp.ravel(), q, dtype=dtype)], rl=False, log2=log2, cbs=cbs)
+    words.append(w)
+    for level in range(levels - 1, -1, -1):
+        q = _step(q0, f // 2)
+        region = coef[: Z // (f // 2), : Y // (f // 2), : X // (f // 2)]
+        streams = [o.quantize(np.ascontiguousarray(region[b]).ravel(), q, dtype=dtype) for b in _bands3d(region.shape)]
+        rl = not (level > 0 and rle_only_lvl0)
+        w, _ = o.encode(streams, rl=rl, log2=log2, cbs=cbs)
+        words.append(w)
+        f //= 2
+    return np.concatenate(words)
+
+
+def decompress_volume_float(words, shape, levels, q0, log2=14, cbs=128, dtype=np.uint16, rle_only_lvl0=False):
+    Z, Y, X = shape
+    nbits = words.size * 32
+    vol = np.zeros(shape, dtype=np.float32)
+    f = 1 << levels
+    (lp,), pos = o.decode(words, nbits, 1, (Z // f) * (Y // f) * (X // f), dtype=dtype, rl=False, log2=log2, cbs=cbs)
+    vol[: Z // f, : Y // f, : X // f] = o.unquantize(lp, _step(q0, f // 2)).reshape(Z // f, Y // f, X // f)
+    for level in range(levels - 1, -1, -1):
+        count = (Z // f) * (Y // f) * (X // f)
+        rl = not (level > 0 and rle_only_lvl0)
+        streams, pos = o.decode(words, nbits, 7, count, dtype=dtype, rl=rl, log2=log2, cbs=cbs, bitpos=pos)
+        f //= 2
+        q = _step(q0, f)
+        region = vol[: Z // f, : Y // f, : X // f]
+        for b, s in zip(_bands3d(region.shape), streams):
+            region[b] = o.unquantize(s, q).reshape(region[b].shape)
+        vol[: Z // f, : Y // f, : X // f] = o.dwt97_inverse(region)
+    return vol
+
+
+def compress_lossless(img, levels, log2=16, cbs=128):
+    img = np.ascontiguousarray(img, dtype=np.int16)
+    is3d = img.ndim == 3
+    bands = _bands3d if is3d else _bands2d
+    cur = img
+    level_words = []
+    for level in range(levels):
+        coef = o.dwt53_forward(cur)
+        streams = [o.symbolize(np.ascontiguousarray(coef[b]).ravel()) for b in bands(coef.shape)]
+        w, _ = o.encode(streams, rl=True, log2=log2, cbs=cbs)
+        level_words.append(w)
+        cur = np.ascontiguousarray(coef[tuple(slice(0, n // 2) for n in coef.shape)])
+    w, _ = o.encode([o.symbolize(cur.ravel())], rl=False, log2=log2, cbs=cbs)
+    return np.concatenate([w] + level_words[::-1])
+
+
+def decompress_lossless(words, shape, levels, log2=16, cbs=128):
+    is3d = len(shape) == 3
+    bands = _bands3d if is3d else _bands2d
+    nbits = words.size * 32
+    lshape = tuple(n >> levels for n in shape)
+    (ll,), pos = o.decode(words, nbits, 1, int(np.prod(lshape)), rl=False, log2=log2, cbs=cbs)
+    cur = o.unsymbolize(ll).reshape(lshape)
+    nb = 7 if is3d else 3
+    for level in range(levels - 1, -1, -1):
+        cshape = tuple(n >> level for n in shape)
+        coef = np.zeros(cshape, dtype=np.int16)
+        coef[tuple(slice(0, n // 2) for n in cshape)] = cur
+        count = int(np.prod([n // 2 for n in cshape]))
+        streams, pos = o.decode(words, nbits, nb, count, rl=True, log2=log2, cbs=cbs, bitpos=pos)
+        for b, s in zip(bands(cshape), streams):
+            coef[b] = o.unsymbolize(s).reshape(coef[b].shape)
+        cur = o.dwt53_inverse(coef)
+    return cur

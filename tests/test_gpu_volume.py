@@ -1,0 +1,58 @@
+"""GPU parity of the volume codecs (compressVolumeFloat / compressVolumeLossless callers of
+src/examples/CompressVolume.cpp) against the oracle composition in oracle/pipeline.py."""
+import numpy as np
+import pytest
+
+from cudacompress_amd import synthetic as S
+from oracle import pipeline as P
+from tests.gpu_util import log, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("shape,levels", [((32, 32, 32), 2), ((64, 48, 32), 2), ((64, 64, 64), 3)],
+                         ids=["32^3L2", "64x48x32L2", "64^3L3"])
+@pytest.mark.parametrize("rle_lvl0", [False, True], ids=["rle_all", "rle_lvl0"])
+def test_volume_float(cc, torch_gpu, shape, levels, rle_lvl0):
+    torch = torch_gpu
+    log("volume float", shape, levels, rle_lvl0)
+    vol = S.gaussian_field(shape, 1)
+    q0 = 0.01
+    inst = cc.createInstance(0, 7, int(np.prod(shape)) // 8, 128, 14)
+    src = to_dev(torch, vol)
+    words = cc.compressVolumeFloat(inst, src, shape[2], shape[1], shape[0], levels, q0, rle_lvl0)
+    want = P.compress_volume_float(vol, levels, q0, rle_only_lvl0=rle_lvl0)
+    assert words is not None
+    assert words.size == want.size and np.array_equal(words, want)
+    out = torch.full(shape, 123.0, dtype=torch.float32, device="cuda")
+    cc.decompressVolumeFloat(inst, out, shape[2], shape[1], shape[0], levels, words, q0, rle_lvl0)
+    torch.cuda.synchronize()
+    rec = to_host(out)
+    ref = P.decompress_volume_float(want, shape, levels, q0, rle_only_lvl0=rle_lvl0)
+    err = float(np.abs(rec - ref).max())
+    log(f"   words={words.size} psnr={S.psnr(vol, rec):.2f} max|gpu-oracle|={err:.3e}")
+    assert err <= 1e-6 * max(1.0, float(np.abs(ref).max()))
+    inst.close()
+
+
+@pytest.mark.parametrize("shape,levels", [((256, 256), 3), ((128, 64), 2), ((32, 32, 32), 2)],
+                         ids=["2d256L3", "2d128x64L2", "3d32L2"])
+def test_volume_lossless(cc, torch_gpu, shape, levels):
+    torch = torch_gpu
+    log("lossless", shape, levels)
+    if len(shape) == 2:
+        img = S.fbm_heightfield(shape[0], 0.8, 2)[:, : shape[1]].copy()
+        sx, sy, sz = shape[1], shape[0], 1
+    else:
+        img = (S.gaussian_field(shape, 3) * 2000).astype(np.int16)
+        sx, sy, sz = shape[2], shape[1], shape[0]
+    inst = cc.createInstance(0, 7, int(np.prod(shape)) // 4, 128, 16)
+    src = to_dev(torch, img)
+    words = cc.compressVolumeLossless(inst, src, sx, sy, sz, levels)
+    want = P.compress_lossless(img, levels)
+    assert np.array_equal(words, want)
+    out = torch.full(img.shape, -1, dtype=torch.int16, device="cuda")
+    cc.decompressVolumeLossless(inst, out, sx, sy, sz, levels, words)
+    torch.cuda.synchronize()
+    assert np.array_equal(to_host(out), img)
+    inst.close()
