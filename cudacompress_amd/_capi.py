@@ -56,7 +56,27 @@ SIGNATURES = {
     "cc_decompress_volume_float": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, c_vp, c_u64, c_float, c_int, c_int]),
     "cc_compress_volume_lossless": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, PP_u32, P_u64]),
     "cc_decompress_volume_lossless": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, c_vp, c_u64]),
+    "cc_profile_enable": (None, [c_int]),
+    "cc_profile_reset": (None, []),
+    "cc_profile_collect": (c_u32, []),
+    "cc_profile_entry": (c_int, [c_u32, ctypes.POINTER(ctypes.c_char_p), P_u64, ctypes.POINTER(ctypes.c_double), P_u64]),
 }
+
+
+def profile_entries():
+    """Per-kernel (name, launches, total_ms, algorithmic_bytes) since the last reset."""
+    lib = load()
+    n = lib.cc_profile_collect()
+    out = []
+    for i in range(n):
+        name = ctypes.c_char_p()
+        launches = c_u64()
+        ms = ctypes.c_double()
+        nbytes = c_u64()
+        if lib.cc_profile_entry(i, ctypes.byref(name), ctypes.byref(launches), ctypes.byref(ms),
+                                ctypes.byref(nbytes)) == 0:
+            out.append((name.value.decode(), launches.value, ms.value, nbytes.value))
+    return out
 
 _lib = None
 

@@ -36,6 +36,13 @@ struct Error {
 
 #define CCA_LAUNCH_CHECK() CCA_HIP(hipGetLastError())
 
+// Launch bracketed by the kernel timer (see profile.hpp): name and algorithmic bytes of the launch.
+#define CCA_LAUNCH(name, bytes, kern, grid, block, shm, st, ...)                 \
+    do {                                                                         \
+        ::cca::KernelTimer kt_((name), (st), (uint64_t)(bytes));                 \
+        hipLaunchKernelGGL(kern, grid, block, shm, st, __VA_ARGS__);             \
+    } while (0)
+
 __host__ __device__ inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 __host__ __device__ inline uint32_t div_up_dev(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 inline uint64_t align_up(uint64_t a, uint64_t b) { return (a + b - 1) / b * b; }

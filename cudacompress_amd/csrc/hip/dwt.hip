@@ -331,13 +331,13 @@ void check_dims(int sx, int sy, int sz) {
 template <int CH>
 void launch_fwd97_x(float* dst, const float* src, int sx, int sy, int sz, int dRow, int dSlice, int sRow, int sSlice,
                     hipStream_t st) {
-    hipLaunchKernelGGL((fwd_x<Fwd97, CH>), x_grid(sy * sz), dim3(256), 0, st, dst, src, sx, sy * sz, sy, dRow,
+    CCA_LAUNCH("dwt97_fwd_x", (uint64_t)sx * sy * sz * 8, (fwd_x<Fwd97, CH>), x_grid(sy * sz), dim3(256), 0, st, dst, src, sx, sy * sz, sy, dRow,
                        dSlice, sRow, sSlice);
 }
 template <int CH>
 void launch_inv97_x(float* dst, const float* src, int sx, int sy, int sz, int dRow, int dSlice, int sRow, int sSlice,
                     hipStream_t st) {
-    hipLaunchKernelGGL((inv_x<Inv97, CH>), x_grid(sy * sz), dim3(256), 0, st, dst, src, sx, sy * sz, sy, dRow,
+    CCA_LAUNCH("dwt97_inv_x", (uint64_t)sx * sy * sz * 8, (inv_x<Inv97, CH>), x_grid(sy * sz), dim3(256), 0, st, dst, src, sx, sy * sz, sy, dRow,
                        dSlice, sRow, sSlice);
 }
 
@@ -362,11 +362,11 @@ void dwt97_forward(float* dst, float* buf2, float* buf1, const float* src, int s
     }
     CCA_LAUNCH_CHECK();
     float* ydst = do3d ? buf2 : dst;
-    hipLaunchKernelGGL((fwd_s<Fwd97>), s_grid(sx, sy, sz), dim3(256), 0, st, ydst, buf1, sx, sy, (long long)dRow,
+    CCA_LAUNCH("dwt97_fwd_y", (uint64_t)sx * sy * sz * 8, (fwd_s<Fwd97>), s_grid(sx, sy, sz), dim3(256), 0, st, ydst, buf1, sx, sy, (long long)dRow,
                        (long long)dSlice);
     CCA_LAUNCH_CHECK();
     if (do3d) {
-        hipLaunchKernelGGL((fwd_s<Fwd97>), s_grid(sx, sz, sy), dim3(256), 0, st, dst, buf2, sx, sz,
+        CCA_LAUNCH("dwt97_fwd_z", (uint64_t)sx * sy * sz * 8, (fwd_s<Fwd97>), s_grid(sx, sz, sy), dim3(256), 0, st, dst, buf2, sx, sz,
                            (long long)dSlice, (long long)dRow);
         CCA_LAUNCH_CHECK();
     }
@@ -384,12 +384,12 @@ void dwt97_inverse(float* dst, float* buf2, float* buf1, const float* src, int s
     CCA_REQUIRE(dst && buf1 && src && (!do3d || buf2), CC_ERR_INVALID_ARG, "null DWT buffer");
     // Z: src -> buf2, Y: (3D ? buf2 : src) -> buf1, X: buf1 -> dst (DWTFloat.cu:233-322)
     if (do3d) {
-        hipLaunchKernelGGL((inv_s<Inv97, PlainFetch>), s_grid(sx, sz, sy), dim3(256), 0, st, buf2, src, sx, sz,
+        CCA_LAUNCH("dwt97_inv_z", (uint64_t)sx * sy * sz * 8, (inv_s<Inv97, PlainFetch>), s_grid(sx, sz, sy), dim3(256), 0, st, buf2, src, sx, sz,
                            (long long)sSlice, (long long)sRow, (long long)sSlice, (long long)sRow, PlainFetch{});
         CCA_LAUNCH_CHECK();
     }
     const float* ysrc = do3d ? buf2 : src;
-    hipLaunchKernelGGL((inv_s<Inv97, PlainFetch>), s_grid(sx, sy, sz), dim3(256), 0, st, buf1, ysrc, sx, sy,
+    CCA_LAUNCH("dwt97_inv_y", (uint64_t)sx * sy * sz * 8, (inv_s<Inv97, PlainFetch>), s_grid(sx, sy, sz), dim3(256), 0, st, buf1, ysrc, sx, sy,
                        (long long)sRow, (long long)sSlice, (long long)sRow, (long long)sSlice, PlainFetch{});
     CCA_LAUNCH_CHECK();
     switch (ch) {
@@ -417,10 +417,10 @@ void dwt97_inverse_from_symbols(float* dst, float* buf2, float* buf1, const floa
     const long long bRow = sx, bSlice = (long long)sx * sy;
     auto first_pass = [&](auto fetch) {
         if (do3d) {
-            hipLaunchKernelGGL((inv_s<Inv97, decltype(fetch)>), s_grid(sx, sz, sy), dim3(256), 0, st, buf2,
+            CCA_LAUNCH("dwt97_inv_z_from_symbols", (uint64_t)sx * sy * sz * 4 / 8 + (uint64_t)sx * sy * sz * 7 / 8 * symbolSize + (uint64_t)sx * sy * sz * 4, (inv_s<Inv97, decltype(fetch)>), s_grid(sx, sz, sy), dim3(256), 0, st, buf2,
                                (const float*)nullptr, sx, sz, bSlice, bRow, bSlice, bRow, fetch);
         } else {
-            hipLaunchKernelGGL((inv_s<Inv97, decltype(fetch)>), s_grid(sx, sy, 1), dim3(256), 0, st, buf1,
+            CCA_LAUNCH("dwt97_inv_y_from_symbols", (uint64_t)sx * sy * sz / 4 * 4 + (uint64_t)sx * sy * sz * 3 / 4 * symbolSize + (uint64_t)sx * sy * sz * 4, (inv_s<Inv97, decltype(fetch)>), s_grid(sx, sy, 1), dim3(256), 0, st, buf1,
                                (const float*)nullptr, sx, sy, bRow, bSlice, bRow, bSlice, fetch);
         }
     };
@@ -435,7 +435,7 @@ void dwt97_inverse_from_symbols(float* dst, float* buf2, float* buf1, const floa
     }
     CCA_LAUNCH_CHECK();
     if (do3d) {
-        hipLaunchKernelGGL((inv_s<Inv97, PlainFetch>), s_grid(sx, sy, sz), dim3(256), 0, st, buf1, buf2, sx, sy, bRow,
+        CCA_LAUNCH("dwt97_inv_y", (uint64_t)sx * sy * sz * 8, (inv_s<Inv97, PlainFetch>), s_grid(sx, sy, sz), dim3(256), 0, st, buf1, buf2, sx, sy, bRow,
                            bSlice, bRow, bSlice, PlainFetch{});
         CCA_LAUNCH_CHECK();
     }
@@ -459,15 +459,15 @@ void dwt53_forward(int16_t* dst, int16_t* buf, const int16_t* src, int sx, int s
     const bool do3d = sz > 1;
     // X: src -> (3D ? dst : buf), Y: -> (3D ? buf : dst), Z: buf -> dst (DWTInt.cu:106-186)
     int16_t* xdst = do3d ? dst : buf;
-    hipLaunchKernelGGL((fwd_x<Fwd53, 1>), x_grid(sy * sz), dim3(256), 0, st, xdst, src, sx, sy * sz, sy, dRow, dSlice,
+    CCA_LAUNCH("dwt53_fwd_x", (uint64_t)sx * sy * sz * 4, (fwd_x<Fwd53, 1>), x_grid(sy * sz), dim3(256), 0, st, xdst, src, sx, sy * sz, sy, dRow, dSlice,
                        sRow, sSlice);
     CCA_LAUNCH_CHECK();
     int16_t* ydst = do3d ? buf : dst;
-    hipLaunchKernelGGL((fwd_s<Fwd53>), s_grid(sx, sy, sz), dim3(256), 0, st, ydst, xdst, sx, sy, (long long)dRow,
+    CCA_LAUNCH("dwt53_fwd_y", (uint64_t)sx * sy * sz * 4, (fwd_s<Fwd53>), s_grid(sx, sy, sz), dim3(256), 0, st, ydst, xdst, sx, sy, (long long)dRow,
                        (long long)dSlice);
     CCA_LAUNCH_CHECK();
     if (do3d) {
-        hipLaunchKernelGGL((fwd_s<Fwd53>), s_grid(sx, sz, sy), dim3(256), 0, st, dst, buf, sx, sz, (long long)dSlice,
+        CCA_LAUNCH("dwt53_fwd_z", (uint64_t)sx * sy * sz * 4, (fwd_s<Fwd53>), s_grid(sx, sz, sy), dim3(256), 0, st, dst, buf, sx, sz, (long long)dSlice,
                            (long long)dRow);
         CCA_LAUNCH_CHECK();
     }
@@ -485,15 +485,15 @@ void dwt53_inverse(int16_t* dst, int16_t* buf, const int16_t* src, int sx, int s
     // Z: src -> dst, Y: (3D ? dst : src) -> buf, X: buf -> dst (DWTInt.cu:218-316), src pitches
     // for the intermediate buffers
     if (do3d) {
-        hipLaunchKernelGGL((inv_s<Inv53, PlainFetch>), s_grid(sx, sz, sy), dim3(256), 0, st, dst, src, sx, sz,
+        CCA_LAUNCH("dwt53_inv_z", (uint64_t)sx * sy * sz * 4, (inv_s<Inv53, PlainFetch>), s_grid(sx, sz, sy), dim3(256), 0, st, dst, src, sx, sz,
                            (long long)sSlice, (long long)sRow, (long long)sSlice, (long long)sRow, PlainFetch{});
         CCA_LAUNCH_CHECK();
     }
     const int16_t* ysrc = do3d ? dst : src;
-    hipLaunchKernelGGL((inv_s<Inv53, PlainFetch>), s_grid(sx, sy, sz), dim3(256), 0, st, buf, ysrc, sx, sy,
+    CCA_LAUNCH("dwt53_inv_y", (uint64_t)sx * sy * sz * 4, (inv_s<Inv53, PlainFetch>), s_grid(sx, sy, sz), dim3(256), 0, st, buf, ysrc, sx, sy,
                        (long long)sRow, (long long)sSlice, (long long)sRow, (long long)sSlice, PlainFetch{});
     CCA_LAUNCH_CHECK();
-    hipLaunchKernelGGL((inv_x<Inv53, 1>), x_grid(sy * sz), dim3(256), 0, st, dst, buf, sx, sy * sz, sy, dRow, dSlice,
+    CCA_LAUNCH("dwt53_inv_x", (uint64_t)sx * sy * sz * 4, (inv_x<Inv53, 1>), x_grid(sy * sz), dim3(256), 0, st, dst, buf, sx, sy * sz, sy, dRow, dSlice,
                        sRow, sSlice);
     CCA_LAUNCH_CHECK();
 }

@@ -252,39 +252,45 @@ void histogram(const HistStream* d_streams, const HistStream* h_streams, uint32_
     uint32_t maxchunks = 1;
     for (uint32_t i = 0; i < nstreams; i++) maxchunks = std::max(maxchunks, div_up(h_streams[i].n, kHistChunk));
     dim3 grid(maxchunks, nstreams);
-    if (symSize == 2) hipLaunchKernelGGL(k_hist<uint16_t>, grid, dim3(256), 0, st, d_streams);
-    else hipLaunchKernelGGL(k_hist<uint32_t>, grid, dim3(256), 0, st, d_streams);
+    uint64_t bytes = 0;
+    for (uint32_t i = 0; i < nstreams; i++) bytes += (uint64_t)h_streams[i].n * symSize + 4ull * h_streams[i].nbins;
+    if (symSize == 2) CCA_LAUNCH("histogram", bytes, k_hist<uint16_t>, grid, dim3(256), 0, st, d_streams);
+    else CCA_LAUNCH("histogram", bytes, k_hist<uint32_t>, grid, dim3(256), 0, st, d_streams);
     CCA_LAUNCH_CHECK();
 }
 
 void huffman_encode(const HuffEncStream* d_streams, const HuffEncStream* h_streams, uint32_t nstreams, uint32_t cbs,
                     int symSize, uint32_t* d_block_bits, uint32_t nblocks, const uint32_t* d_block_stream,
-                    uint32_t* d_stream_bits, const uint32_t* d_chunk_stream, uint32_t nchunks, hipStream_t st) {
-    (void)h_streams;
+                    uint32_t* d_stream_bits, const uint32_t* d_chunk_stream, uint32_t nchunks, uint64_t total_bits,
+                    hipStream_t st) {
     if (nstreams == 0) return;
+    uint64_t syms = 0;
+    for (uint32_t i = 0; i < nstreams; i++) syms += h_streams[i].n;
+    const uint64_t inb = syms * symSize;
     CCA_REQUIRE(cbs >= 32 && cbs <= 256 && (cbs & (cbs - 1)) == 0, CC_ERR_INVALID_ARG,
                 "codingBlockSize must be a power of two in [32, 256]");
     if (nblocks > 0) {
-        if (symSize == 2) hipLaunchKernelGGL(k_block_bits<uint16_t>, dim3(div_up(nblocks, 4)), dim3(256), 0, st, d_streams, d_block_bits, d_block_stream, nblocks, cbs);
-        else hipLaunchKernelGGL(k_block_bits<uint32_t>, dim3(div_up(nblocks, 4)), dim3(256), 0, st, d_streams, d_block_bits, d_block_stream, nblocks, cbs);
+        if (symSize == 2) CCA_LAUNCH("huff_block_bits", inb + 4ull * nblocks, k_block_bits<uint16_t>, dim3(div_up(nblocks, 4)), dim3(256), 0, st, d_streams, d_block_bits, d_block_stream, nblocks, cbs);
+        else CCA_LAUNCH("huff_block_bits", inb + 4ull * nblocks, k_block_bits<uint32_t>, dim3(div_up(nblocks, 4)), dim3(256), 0, st, d_streams, d_block_bits, d_block_stream, nblocks, cbs);
         CCA_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_block_scan, dim3(nstreams), dim3(1024), 0, st, d_streams, d_block_bits, d_stream_bits, cbs);
+    CCA_LAUNCH("huff_block_scan", 8ull * nblocks, k_block_scan, dim3(nstreams), dim3(1024), 0, st, d_streams, d_block_bits, d_stream_bits, cbs);
     CCA_LAUNCH_CHECK();
     if (nchunks > 0) {
-        if (symSize == 2) hipLaunchKernelGGL(k_pack<uint16_t>, dim3(nchunks), dim3(256), 0, st, d_streams, d_chunk_stream, cbs);
-        else hipLaunchKernelGGL(k_pack<uint32_t>, dim3(nchunks), dim3(256), 0, st, d_streams, d_chunk_stream, cbs);
+        if (symSize == 2) CCA_LAUNCH("huff_pack", inb + total_bits / 8, k_pack<uint16_t>, dim3(nchunks), dim3(256), 0, st, d_streams, d_chunk_stream, cbs);
+        else CCA_LAUNCH("huff_pack", inb + total_bits / 8, k_pack<uint32_t>, dim3(nchunks), dim3(256), 0, st, d_streams, d_chunk_stream, cbs);
         CCA_LAUNCH_CHECK();
     }
 }
 
 void huffman_decode(const HuffDecStream* d_streams, uint32_t nstreams, const uint32_t* d_wave_stream, uint32_t nwaves,
-                    uint32_t cbs, int symSize, hipStream_t st) {
+                    uint32_t cbs, int symSize, uint64_t total_syms, uint64_t total_bits, hipStream_t st) {
+    const uint64_t bytes = total_syms * symSize + total_bits / 8;
     if (nstreams == 0 || nwaves == 0) return;
     CCA_REQUIRE(cbs >= 32 && cbs <= 256 && (cbs & (cbs - 1)) == 0, CC_ERR_INVALID_ARG,
                 "codingBlockSize must be a power of two in [32, 256]");
-    if (symSize == 2) hipLaunchKernelGGL(k_decode<uint16_t>, dim3(nwaves), dim3(64), 0, st, d_streams, d_wave_stream, cbs);
-    else hipLaunchKernelGGL(k_decode<uint32_t>, dim3(nwaves), dim3(64), 0, st, d_streams, d_wave_stream, cbs);
+    if (symSize == 2) CCA_LAUNCH("huff_decode", bytes, k_decode<uint16_t>, dim3(nwaves), dim3(64), 0, st, d_streams, d_wave_stream, cbs);
+    else CCA_LAUNCH("huff_decode", bytes, k_decode<uint32_t>, dim3(nwaves), dim3(64), 0, st, d_streams, d_wave_stream, cbs);
     CCA_LAUNCH_CHECK();
 }
 

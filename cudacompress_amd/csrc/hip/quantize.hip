@@ -124,7 +124,7 @@ void symbolize(uint16_t* sym, const int16_t* data, uint32_t sx, uint32_t sy, uin
     if (slice == 0) slice = row * sy;
     if ((uint64_t)sx * sy * sz == 0) return;
     check3(sx, sy, sz);
-    hipLaunchKernelGGL(k_symbolize, grid3(sx, sy, sz), dim3(256), 0, st, sym, data, sx, sy, sz, row, slice);
+    CCA_LAUNCH("symbolize", (uint64_t)sx * sy * sz * 4, k_symbolize, grid3(sx, sy, sz), dim3(256), 0, st, sym, data, sx, sy, sz, row, slice);
     CCA_LAUNCH_CHECK();
 }
 
@@ -136,7 +136,7 @@ void unsymbolize(int16_t* data, const uint16_t* sym, uint32_t sx, uint32_t sy, u
     if (slice == 0) slice = row * sy;
     if ((uint64_t)sx * sy * sz == 0) return;
     check3(sx, sy, sz);
-    hipLaunchKernelGGL(k_unsymbolize, grid3(sx, sy, sz), dim3(256), 0, st, data, sym, sx, sy, sz, row, slice);
+    CCA_LAUNCH("unsymbolize", (uint64_t)sx * sy * sz * 4, k_unsymbolize, grid3(sx, sy, sz), dim3(256), 0, st, data, sym, sx, sy, sz, row, slice);
     CCA_LAUNCH_CHECK();
 }
 
@@ -150,10 +150,10 @@ void quantize_to_symbols(void* sym, int symSize, float* data, bool roundtrip, ui
     check3(sx, sy, sz);
     const float inv = 1.0f / step;  // host fp32 division, Quantize.cu:94
     if (symSize == 2)
-        hipLaunchKernelGGL(k_quantize<uint16_t>, grid3(sx, sy, sz), dim3(256), 0, st, (uint16_t*)sym, data, step, inv,
+        CCA_LAUNCH("quantize", (uint64_t)sx * sy * sz * (roundtrip ? 10 : 6), k_quantize<uint16_t>, grid3(sx, sy, sz), dim3(256), 0, st, (uint16_t*)sym, data, step, inv,
                            sx, sy, sz, row, slice, qtype, roundtrip ? 1 : 0);
     else
-        hipLaunchKernelGGL(k_quantize<uint32_t>, grid3(sx, sy, sz), dim3(256), 0, st, (uint32_t*)sym, data, step, inv,
+        CCA_LAUNCH("quantize", (uint64_t)sx * sy * sz * (roundtrip ? 12 : 8), k_quantize<uint32_t>, grid3(sx, sy, sz), dim3(256), 0, st, (uint32_t*)sym, data, step, inv,
                            sx, sy, sz, row, slice, qtype, roundtrip ? 1 : 0);
     CCA_LAUNCH_CHECK();
 }
@@ -167,10 +167,10 @@ void unquantize_from_symbols(float* data, const void* sym, int symSize, uint32_t
     if ((uint64_t)sx * sy * sz == 0) return;
     check3(sx, sy, sz);
     if (symSize == 2)
-        hipLaunchKernelGGL(k_unquantize<uint16_t>, grid3(sx, sy, sz), dim3(256), 0, st, data, (const uint16_t*)sym,
+        CCA_LAUNCH("unquantize", (uint64_t)sx * sy * sz * 6, k_unquantize<uint16_t>, grid3(sx, sy, sz), dim3(256), 0, st, data, (const uint16_t*)sym,
                            step, sx, sy, sz, row, slice, qtype);
     else
-        hipLaunchKernelGGL(k_unquantize<uint32_t>, grid3(sx, sy, sz), dim3(256), 0, st, data, (const uint32_t*)sym,
+        CCA_LAUNCH("unquantize", (uint64_t)sx * sy * sz * 8, k_unquantize<uint32_t>, grid3(sx, sy, sz), dim3(256), 0, st, data, (const uint32_t*)sym,
                            step, sx, sy, sz, row, slice, qtype);
     CCA_LAUNCH_CHECK();
 }
@@ -182,7 +182,7 @@ void quantize_to_short(int16_t* q, const float* data, uint32_t sx, uint32_t sy, 
     CCA_REQUIRE(qtype == 0 || qtype == 1, CC_ERR_INVALID_ARG, "unknown quantizer type");
     if ((uint64_t)sx * sy * sz == 0) return;
     check3(sx, sy, sz);
-    hipLaunchKernelGGL(k_quantize_short, grid3(sx, sy, sz), dim3(256), 0, st, q, data, 1.0f / step, sx, sy, sz, row,
+    CCA_LAUNCH("quantize_short", (uint64_t)sx * sy * sz * 6, k_quantize_short, grid3(sx, sy, sz), dim3(256), 0, st, q, data, 1.0f / step, sx, sy, sz, row,
                        slice, qtype);
     CCA_LAUNCH_CHECK();
 }
@@ -194,17 +194,17 @@ void unquantize_from_short(float* data, const int16_t* q, uint32_t sx, uint32_t 
     CCA_REQUIRE(qtype == 0 || qtype == 1, CC_ERR_INVALID_ARG, "unknown quantizer type");
     if ((uint64_t)sx * sy * sz == 0) return;
     check3(sx, sy, sz);
-    hipLaunchKernelGGL(k_unquantize_short, grid3(sx, sy, sz), dim3(256), 0, st, data, q, step, sx, sy, sz, row, slice,
+    CCA_LAUNCH("unquantize_short", (uint64_t)sx * sy * sz * 6, k_unquantize_short, grid3(sx, sy, sz), dim3(256), 0, st, data, q, step, sx, sy, sz, row, slice,
                        qtype);
     CCA_LAUNCH_CHECK();
 }
 
 void max_abs(const float* data, uint32_t n, float* out, hipStream_t st) {
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, (uint32_t*)out);
+    CCA_LAUNCH("max_abs_init", 4, k_zero_u32, dim3(1), dim3(1), 0, st, (uint32_t*)out);
     CCA_LAUNCH_CHECK();
     if (n == 0) return;
     uint32_t blocks = std::min<uint32_t>(div_up(n, 256 * 8), 2048u);
-    hipLaunchKernelGGL(k_max_abs, dim3(blocks), dim3(256), 0, st, data, n, (uint32_t*)out);
+    CCA_LAUNCH("max_abs", (uint64_t)n * 4, k_max_abs, dim3(blocks), dim3(256), 0, st, data, n, (uint32_t*)out);
     CCA_LAUNCH_CHECK();
 }
 

@@ -337,53 +337,56 @@ __global__ __launch_bounds__(256) void k_rld_scatter(const RleDecStream* __restr
 }  // namespace
 
 void rle_encode(const RleStream* d_streams, uint32_t nstreams, RleTile* d_tiles, uint32_t ntiles,
-                const uint32_t* d_tile_stream, RleResult* d_results, int symSize, hipStream_t st) {
+                const uint32_t* d_tile_stream, RleResult* d_results, int symSize, uint64_t in_elems, hipStream_t st) {
+    const uint64_t inb = in_elems * symSize;
     if (nstreams == 0) return;
     if (ntiles > 0) {
-        if (symSize == 2) hipLaunchKernelGGL(k_rle_stats<uint16_t>, dim3(ntiles), dim3(256), 0, st, d_streams, d_tiles, d_tile_stream);
-        else hipLaunchKernelGGL(k_rle_stats<uint32_t>, dim3(ntiles), dim3(256), 0, st, d_streams, d_tiles, d_tile_stream);
+        if (symSize == 2) CCA_LAUNCH("rle_stats", inb, k_rle_stats<uint16_t>, dim3(ntiles), dim3(256), 0, st, d_streams, d_tiles, d_tile_stream);
+        else CCA_LAUNCH("rle_stats", inb, k_rle_stats<uint32_t>, dim3(ntiles), dim3(256), 0, st, d_streams, d_tiles, d_tile_stream);
         CCA_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_rle_scan, dim3(nstreams), dim3(256), 0, st, d_streams, d_tiles, d_results);
+    CCA_LAUNCH("rle_scan", (uint64_t)ntiles * 2 * sizeof(RleTile), k_rle_scan, dim3(nstreams), dim3(256), 0, st, d_streams, d_tiles, d_results);
     CCA_LAUNCH_CHECK();
     if (ntiles > 0) {
-        if (symSize == 2) hipLaunchKernelGGL(k_rle_emit<uint16_t>, dim3(ntiles), dim3(256), 0, st, d_streams, d_tiles, d_tile_stream);
-        else hipLaunchKernelGGL(k_rle_emit<uint32_t>, dim3(ntiles), dim3(256), 0, st, d_streams, d_tiles, d_tile_stream);
+        if (symSize == 2) CCA_LAUNCH("rle_emit", inb, k_rle_emit<uint16_t>, dim3(ntiles), dim3(256), 0, st, d_streams, d_tiles, d_tile_stream);
+        else CCA_LAUNCH("rle_emit", inb, k_rle_emit<uint32_t>, dim3(ntiles), dim3(256), 0, st, d_streams, d_tiles, d_tile_stream);
         CCA_LAUNCH_CHECK();
     }
 }
 
 void stream_max(const RleStream* d_streams, uint32_t nstreams, uint32_t nmax, RleResult* d_results, int symSize,
                 hipStream_t st) {
+    const uint64_t inb = (uint64_t)nmax * nstreams * symSize;
     if (nstreams == 0) return;
-    hipLaunchKernelGGL(k_init_results, dim3(div_up(nstreams, 256)), dim3(256), 0, st, d_results, nstreams, d_streams);
+    CCA_LAUNCH("stream_max_init", (uint64_t)nstreams * 16, k_init_results, dim3(div_up(nstreams, 256)), dim3(256), 0, st, d_results, nstreams, d_streams);
     CCA_LAUNCH_CHECK();
     if (nmax == 0) return;
     CCA_REQUIRE(nstreams <= 65535u, CC_ERR_INVALID_ARG, "too many streams for one launch");
     dim3 grid(std::min<uint32_t>(div_up(nmax, 256 * 16), 1024u), nstreams);
-    if (symSize == 2) hipLaunchKernelGGL(k_stream_max<uint16_t>, grid, dim3(256), 0, st, d_streams, d_results);
-    else hipLaunchKernelGGL(k_stream_max<uint32_t>, grid, dim3(256), 0, st, d_streams, d_results);
+    if (symSize == 2) CCA_LAUNCH("stream_max", inb, k_stream_max<uint16_t>, grid, dim3(256), 0, st, d_streams, d_results);
+    else CCA_LAUNCH("stream_max", inb, k_stream_max<uint32_t>, grid, dim3(256), 0, st, d_streams, d_results);
     CCA_LAUNCH_CHECK();
 }
 
 void rle_decode(const RleDecStream* d_streams, uint32_t nstreams, uint32_t* d_tile_sums, uint32_t ntiles,
-                const uint32_t* d_tile_stream, uint32_t nmax_out, int symSize, hipStream_t st) {
+                const uint32_t* d_tile_stream, uint32_t nmax_out, int symSize, uint64_t total_pairs, uint64_t total_out,
+                hipStream_t st) {
     if (nstreams == 0) return;
     CCA_REQUIRE(nstreams <= 65535u, CC_ERR_INVALID_ARG, "too many streams for one launch");
     if (nmax_out > 0) {
         dim3 grid(std::min<uint32_t>(div_up(nmax_out, 256 * 4), 4096u), nstreams);
-        if (symSize == 2) hipLaunchKernelGGL(k_rld_zero<uint16_t>, grid, dim3(256), 0, st, d_streams);
-        else hipLaunchKernelGGL(k_rld_zero<uint32_t>, grid, dim3(256), 0, st, d_streams);
+        if (symSize == 2) CCA_LAUNCH("rld_zero", total_out * symSize, k_rld_zero<uint16_t>, grid, dim3(256), 0, st, d_streams);
+        else CCA_LAUNCH("rld_zero", total_out * symSize, k_rld_zero<uint32_t>, grid, dim3(256), 0, st, d_streams);
         CCA_LAUNCH_CHECK();
     }
     if (ntiles == 0) return;
-    if (symSize == 2) hipLaunchKernelGGL(k_rld_sums<uint16_t>, dim3(ntiles), dim3(256), 0, st, d_streams, d_tile_sums, d_tile_stream);
-    else hipLaunchKernelGGL(k_rld_sums<uint32_t>, dim3(ntiles), dim3(256), 0, st, d_streams, d_tile_sums, d_tile_stream);
+    if (symSize == 2) CCA_LAUNCH("rld_sums", total_pairs * symSize, k_rld_sums<uint16_t>, dim3(ntiles), dim3(256), 0, st, d_streams, d_tile_sums, d_tile_stream);
+    else CCA_LAUNCH("rld_sums", total_pairs * symSize, k_rld_sums<uint32_t>, dim3(ntiles), dim3(256), 0, st, d_streams, d_tile_sums, d_tile_stream);
     CCA_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_rld_scan, dim3(nstreams), dim3(256), 0, st, d_streams, d_tile_sums);
+    CCA_LAUNCH("rld_scan", (uint64_t)ntiles * 8, k_rld_scan, dim3(nstreams), dim3(256), 0, st, d_streams, d_tile_sums);
     CCA_LAUNCH_CHECK();
-    if (symSize == 2) hipLaunchKernelGGL(k_rld_scatter<uint16_t>, dim3(ntiles), dim3(256), 0, st, d_streams, d_tile_sums, d_tile_stream);
-    else hipLaunchKernelGGL(k_rld_scatter<uint32_t>, dim3(ntiles), dim3(256), 0, st, d_streams, d_tile_sums, d_tile_stream);
+    if (symSize == 2) CCA_LAUNCH("rld_scatter", total_pairs * 3 * symSize, k_rld_scatter<uint16_t>, dim3(ntiles), dim3(256), 0, st, d_streams, d_tile_sums, d_tile_stream);
+    else CCA_LAUNCH("rld_scatter", total_pairs * 3 * symSize, k_rld_scatter<uint32_t>, dim3(ntiles), dim3(256), 0, st, d_streams, d_tile_sums, d_tile_stream);
     CCA_LAUNCH_CHECK();
 }
 

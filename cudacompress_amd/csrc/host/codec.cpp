@@ -105,7 +105,8 @@ void encode_streams(cc_instance* I, bool rl, int symSize, const void* const* str
         RleResult* dres = I->d_results.as<RleResult>(ns);
         if (rl) {
             RleTile* dt = I->d_tiles.as<RleTile>(std::max<uint32_t>(ntiles, 1));
-            rle_encode((const RleStream*)(d + o_desc), ns, dt, ntiles, (const uint32_t*)(d + o_map), dres, symSize, st);
+            rle_encode((const RleStream*)(d + o_desc), ns, dt, ntiles, (const uint32_t*)(d + o_map), dres, symSize,
+                       (uint64_t)n * ns, st);
         } else {
             stream_max((const RleStream*)(d + o_desc), ns, n, dres, symSize, st);
         }
@@ -246,8 +247,10 @@ void encode_streams(cc_instance* I, bool rl, int symSize, const void* const* str
         CCA_HIP(hipMemcpyAsync(dtab, h + o_tab, 4 * tab_total, hipMemcpyHostToDevice, st));
         CCA_HIP(hipMemcpyAsync(d, h + o_desc, s.off - o_desc, hipMemcpyHostToDevice, st));
         const HuffEncStream* dd = (const HuffEncStream*)d;
+        uint64_t total_bits = 0;
+        for (uint32_t j = 0; j < nt; j++) total_bits += bits[j];
         huffman_encode(dd, hd, nt, cbs, symSize, dbb, nblocks, (const uint32_t*)(d + (o_bmap - o_desc)), dsb,
-                       (const uint32_t*)(d + (o_cmap - o_desc)), nchunks, st);
+                       (const uint32_t*)(d + (o_cmap - o_desc)), nchunks, total_bits, st);
         // download codewords and offsets (exact sizes are known from the histograms)
         uint64_t down_words = 0;
         for (uint32_t j = 0; j < nt; j++) down_words += (bits[j] + 31) / 32 + nblk[j];
@@ -434,10 +437,18 @@ void decode_streams(cc_instance* I, bool rl, int symSize, const uint32_t* const*
     }
     CCA_HIP(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, st));
     CCA_HIP(hipEventRecord(I->dec_event[slot], st));
-    huffman_decode((const HuffDecStream*)(d + o_desc), nc, (const uint32_t*)(d + o_wmap), nwaves, cbs, symSize, st);
+    uint64_t total_syms = 0, total_bits = 0, total_pairs = 0;
+    for (uint32_t j = 0; j < nc; j++) {
+        total_syms += coded[j].count;
+        total_bits += 32ull * coded[j].nwords;
+    }
+    for (uint32_t i = 0; i < ns && rl; i++) total_pairs += m[i];
+    huffman_decode((const HuffDecStream*)(d + o_desc), nc, (const uint32_t*)(d + o_wmap), nwaves, cbs, symSize,
+                   total_syms, total_bits, st);
     if (rl) {
         uint32_t* sums = I->d_dec_sums.as<uint32_t>(std::max<uint32_t>(ntiles, 1));
-        rle_decode((const RleDecStream*)(d + o_rdesc), ns, sums, ntiles, (const uint32_t*)(d + o_tmap), n, symSize, st);
+        rle_decode((const RleDecStream*)(d + o_rdesc), ns, sums, ntiles, (const uint32_t*)(d + o_tmap), n, symSize,
+                   total_pairs, (uint64_t)n * ns, st);
     }
 }
 

@@ -3,6 +3,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "profile.hpp"
 
 namespace cca {
 
@@ -59,7 +60,7 @@ struct RleResult {        // per stream
 };
 constexpr int kRleTile = 2048;
 void rle_encode(const RleStream* d_streams, uint32_t nstreams, RleTile* d_tiles, uint32_t ntiles,
-                const uint32_t* d_tile_stream, RleResult* d_results, int symSize, hipStream_t st);
+                const uint32_t* d_tile_stream, RleResult* d_results, int symSize, uint64_t in_elems, hipStream_t st);
 
 struct RleDecStream {
     const void* sym;     // m compacted symbols
@@ -72,7 +73,8 @@ struct RleDecStream {
 };
 // Zero-fills every output element, then scatters the pairs.
 void rle_decode(const RleDecStream* d_streams, uint32_t nstreams, uint32_t* d_tile_sums, uint32_t ntiles,
-                const uint32_t* d_tile_stream, uint32_t nmax_out, int symSize, hipStream_t st);
+                const uint32_t* d_tile_stream, uint32_t nmax_out, int symSize, uint64_t total_pairs, uint64_t total_out,
+                hipStream_t st);
 // Per-stream max symbol into results[i].sym_max (results[i].m = n) for streams coded without RLE.
 void stream_max(const RleStream* d_streams, uint32_t nstreams, uint32_t nmax, RleResult* d_results, int symSize,
                 hipStream_t st);
@@ -101,7 +103,7 @@ struct HuffEncStream {
 void huffman_encode(const HuffEncStream* d_streams, const HuffEncStream* h_streams, uint32_t nstreams,
                     uint32_t cbs, int symSize, uint32_t* d_block_bits, uint32_t nblocks,
                     const uint32_t* d_block_stream, uint32_t* d_stream_bits, const uint32_t* d_chunk_stream,
-                    uint32_t nchunks, hipStream_t st);
+                    uint32_t nchunks, uint64_t total_bits, hipStream_t st);
 
 struct HuffDecStream {
     const uint32_t* words;     // codeword words (+1 word of padding)
@@ -115,6 +117,7 @@ struct HuffDecStream {
     uint32_t nwords;           // codeword words (readable: nwords + 2)
 };
 void huffman_decode(const HuffDecStream* d_streams, uint32_t nstreams, const uint32_t* d_wave_stream,
-                    uint32_t nwaves, uint32_t cbs, int symSize, hipStream_t st);
+                    uint32_t nwaves, uint32_t cbs, int symSize, uint64_t total_syms, uint64_t total_bits,
+                    hipStream_t st);
 
 }  // namespace cca

@@ -184,6 +184,18 @@ CC_API cc_status cc_decompress_volume_lossless(cc_instance* inst, int16_t* d_ima
                                                uint32_t sizeZ, uint32_t numLevels, const uint32_t* bits,
                                                uint64_t bitCount);
 
+/* ---- Kernel timing ------------------------------------------------------------------------ */
+/* No reference counterpart (the reference times host-side sections with CUDA events,
+ * src/cudaCompress/util/CudaTimer.cpp:65-169).  When enabled, every kernel launch of this library
+ * is bracketed by hipEvents on its stream; collect() waits for the recorded events and aggregates
+ * per kernel name: launch count, total milliseconds and algorithmic bytes (one read of each input
+ * plus one write of each output, summed over launches). */
+CC_API void cc_profile_enable(int enable);
+CC_API void cc_profile_reset(void);
+CC_API uint32_t cc_profile_collect(void);
+CC_API cc_status cc_profile_entry(uint32_t index, const char** name, uint64_t* launches, double* totalMs,
+                                  uint64_t* algorithmicBytes);
+
 #ifdef __cplusplus
 }
 #endif
