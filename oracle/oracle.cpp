@@ -6,7 +6,7 @@
 //
 // Pinning (what checks each part, beyond this restatement itself):
 //   bitstream layout, RLE, CPU-dialect tables, Huffman enc/dec  -- byte-compared against the
-//       reference's own src/CPU coder compiled into oracle/_ref (tests/test_oracle_ref.py) and
+//       reference's own src/CPU coder compiled into oracle/_ref (tests/test_oracle.py) and
 //       against committed golden vectors generated from it (tests/golden/).
 //   GPU-dialect table design  -- libstdc++ std::priority_queue(range ctor)/std::sort, the same
 //       library routines the reference calls; cross-dialect differential test on tie-free
@@ -16,10 +16,13 @@
 //       histogram kernels' word loop; parity unpinned by execution (no GPU reference here),
 //       pinned by the read of the code only.
 //   CDF 5/3 integer DWT -- exact invertibility (random and extreme inputs, with wrap),
-//       constant input -> zero highpass, ramps -> zero highpass away from the borders,
-//       brute-force recomputation of the lifting steps on tiny arrays by direct formula.
-//   CDF 9/7 float DWT -- fp64 twin agreement to 1e-5, constant input -> lowpass == c*sum(aL),
-//       perfect reconstruction to the 7-digit coefficient precision, linearity.
+//       constant input -> zero highpass, ramps -> zero highpass away from the borders.  The
+//       reference's rounding helpers (div2/div4 round half away from zero) are pinned by the
+//       read of DWTCommon.cuh only: the exact rounding of the lifting steps is parity unpinned
+//       by execution (no textbook routine has it).
+//   CDF 9/7 float DWT -- separable convolution with analysis taps derived from the CDF 9/7
+//       lifting factors (agree to 2e-7), fp32 vs fp64 agreement, constant input, perfect
+//       reconstruction to the 7-digit coefficient precision, linearity.
 //   quantizer -- closed forms (symbol zig-zag bijection, bin edges).
 #include "oracle.h"
 
