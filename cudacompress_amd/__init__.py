@@ -31,6 +31,7 @@ from .api import (  # noqa: E402
     decompressVolumeFloat,
     decompressVolumeLossless,
     destroyInstance,
+    encodeDevice,
     encodeHuff,
     encodeRLHuff,
     util,

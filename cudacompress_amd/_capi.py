@@ -36,6 +36,7 @@ SIGNATURES = {
     "cc_set_instance_dialect": (c_int, [c_vp, c_int]),
     "cc_encode_rlhuff": (c_int, [c_vp, c_int, c_vp, c_u32, c_u32, c_vp, PP_u32, PP_u64, PP_u64]),
     "cc_encode_huff": (c_int, [c_vp, c_int, c_vp, c_u32, c_u32, c_vp, PP_u32, PP_u64, PP_u64]),
+    "cc_encode_device": (c_int, [c_vp, c_int, c_int, c_vp, c_u32, c_u32, c_vp, c_u64, P_u64]),
     "cc_decode_rlhuff": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp, c_u32, c_vp, c_vp, c_u32, c_u32]),
     "cc_decode_huff": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp, c_u32, c_vp, c_vp, c_u32, c_u32]),
     "cc_dwt_float_forward": (c_int, [c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_vp]),
@@ -104,7 +105,7 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"cudacompress_amd native library not found at {LIB_PATH}; build it with "
-            "`python -m cudacompress_amd.build` (hipcc, gfx950)")
+            "`python cudacompress_amd/build.py` (hipcc, gfx950)")
     _share_hip_runtime_with_torch()
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():

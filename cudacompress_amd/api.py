@@ -342,6 +342,23 @@ def encodeRLHuff(inst, bitStream, symbolStreams, symbolCountPerStream):
     return _encode("cc_encode_rlhuff", inst, bitStream, symbolStreams, symbolCountPerStream)
 
 
+def encodeDevice(inst, rl, symbolStreams, symbolCountPerStream, out):
+    """Device-resident encodeRLHuff (rl=True) / encodeHuff: the bitstream of one call is written to
+    the device tensor `out` (int32/uint32 words) by the GPU pipeline alone.  Returns the word count,
+    or None when a symbol exceeds 2^log2HuffmanDistinctSymbolCountMax."""
+    lib = _capi.load()
+    ns = len(symbolStreams)
+    ptrs = (ctypes.c_void_p * max(ns, 1))(*[_ptr(s) for s in symbolStreams])
+    nw = ctypes.c_uint64()
+    cap = out.numel() * out.element_size() // 4
+    st = lib.cc_encode_device(inst.handle, 1 if rl else 0, inst.symbolSize, ptrs, ns, symbolCountPerStream, _ptr(out),
+                              cap, ctypes.byref(nw))
+    if st == CC_ERR_SYMBOL_OVERFLOW:
+        return None
+    _check(st, "cc_encode_device")
+    return int(nw.value)
+
+
 def decodeRLHuff(inst, bitStream, symbolStreams, symbolCountPerStream):
     """decodeRLHuff (Encode.h:29, :43).  Outputs need not be zeroed."""
     return _decode("cc_decode_rlhuff", inst, bitStream, symbolStreams, symbolCountPerStream)

@@ -3,7 +3,7 @@
 Every HIP/C++ source under csrc/ is compiled with hipcc for --offload-arch=gfx950 and linked
 into one shared library exporting the C ABI of include/cudacompress_amd.h.  Usage:
 
-    python -m cudacompress_amd.build        (or cudacompress_amd.build.build())
+    python cudacompress_amd/build.py        (run as a script: importing the package loads the library)
 """
 import concurrent.futures as cf
 import os

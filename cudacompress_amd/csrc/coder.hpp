@@ -1,0 +1,90 @@
+// Device-resident entropy coder: descriptors shared by the kernels (hip/coder.hip) and the host
+// orchestration (host/batch.cpp).
+//
+// An encode job is a list of coder calls (an encodeRLHuff or encodeHuff call each, src/cudaCompress/
+// Encode.cpp:208-471, :649-790); a call has one item per input stream; an item yields one coded
+// sequence (Huffman only) or two (zero-run RLE: symbols, then zero counts).  Everything from the
+// symbol streams to the final bitstream stays in device memory:
+//   stats   one workgroup per item: zero-run RLE computed on the fly, histograms in LDS
+//   design  one wave per coded sequence: the table design of huff_design.hpp
+//   pack    one workgroup per item: RLE recomputed, codewords packed through an LDS window
+//   layout  one workgroup: word offset of every section (sizes from the tables and bit counts)
+//   gather  one workgroup per coded sequence: table header, bit count, codewords, offsets
+// Intermediate data (histograms, tables, codewords, block offsets) live in a scratch arena
+// carved by an atomic bump allocator; the final layout does not depend on allocation order.
+#pragma once
+
+#include "common.hpp"
+
+namespace cca {
+
+constexpr uint32_t kStatBins = 4096;    // LDS histogram bins of the streaming passes
+constexpr uint32_t kCoderThreads = 256;
+constexpr uint32_t kPerThread = 8;
+constexpr uint32_t kChunk = kCoderThreads * kPerThread;  // symbols per workgroup step
+
+struct CodeItem {
+    const void* in;      // symbols
+    uint32_t n;          // symbol count
+    uint32_t rl;         // 1: zero-run RLE + two coded sequences
+    uint32_t seq;        // index of the first coded sequence
+    uint32_t call;       // coder call
+};
+
+enum CodeErr : uint32_t {
+    kErrNone = 0,
+    kErrOverflow = 1,   // symbol >= 2^log2HuffmanDistinctSymbolCountMax (the reference returns false)
+    kErrTooLong = 5,    // a codeword longer than 31 bits
+    kErrScratch = 7,    // scratch arena exhausted
+    kErrOutput = 8      // output buffer too small
+};
+
+struct SeqInfo {
+    uint32_t count;      // symbols coded (pairs for RLE sequences)
+    uint32_t smax;       // largest symbol (0 when count == 0)
+    uint32_t nsym;       // distinct symbols
+    uint32_t nlen;       // codeword lengths used
+    uint32_t err;
+    uint32_t quirk;      // histogram pad symbol counted in bin 65535
+    uint32_t wide;       // symbols beyond the LDS histogram: global-memory path
+    uint32_t item;
+    uint64_t bits;       // codeword bits
+    uint64_t hist_off;   // scratch word offsets ...
+    uint64_t code_off;   // code[smax+1] then len[smax+1]
+    uint64_t syms_off;   // symbols in codeword-index order
+    uint64_t words_off;  // codewords
+    uint64_t boff_off;   // absolute bit offset of every coding block
+    uint64_t out_off;    // output word offset of the table header
+    uint32_t cpl[32];    // codewords per length
+};
+
+struct CoderParams {
+    uint32_t cbs;
+    uint32_t log2;
+    uint32_t dialect;       // CC_DIALECT_GPU / CC_DIALECT_CPU
+    uint32_t count_bits;    // header field widths (log2 / 14)
+    uint32_t symbits_bits;  // 4 or 5 (GPU), 14 (CPU)
+    uint32_t pad;
+};
+
+struct CoderState {          // device-side counters of one job
+    unsigned long long scratch_used;  // words
+    unsigned long long out_words;     // total output words (written by layout)
+    uint32_t err;                     // first error of the job (CodeErr), 0 if none
+    uint32_t pad;
+};
+
+struct CallOut {              // per coder call
+    uint64_t word_begin;
+    uint64_t words;
+    uint32_t err;
+    uint32_t pad;
+};
+
+// Launchers (hip/coder.hip).  scratch: arena of `scratch_cap` words.
+void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uint32_t nseq, const CoderParams& p,
+                  int symSize, uint32_t* scratch, uint64_t scratch_cap, CoderState* d_state, CallOut* d_calls,
+                  uint32_t ncalls, const uint32_t* d_call_first_item, uint32_t* d_out, uint64_t out_cap,
+                  uint64_t total_symbols, hipStream_t st);
+
+}  // namespace cca

@@ -1,0 +1,785 @@
+// Device-resident entropy coder for gfx950: stats -> design -> pack -> layout -> gather.
+// The bitstream it writes is word for word the layout of src/cudaCompress/Encode.cpp:402-463
+// (encodeRLHuff) and :751-782 (encodeHuff) with the table headers of
+// src/cudaCompress/HuffmanTable.cu:470-485; see coder.hpp for the pipeline.
+//
+// Zero-run RLE is never materialised.  Every input position emits at most one (symbol, zero
+// count) pair: a nonzero symbol v emits (v, run % 256) where run is the number of zeros since
+// the previous nonzero, and the zero that completes each 256-zero block of a run emits the
+// filler (0, 255) -- but only when a nonzero follows later in the stream (trailing zeros emit
+// nothing).  This is the pair sequence of src/cudaCompress/RunLengthKernels.cui:5-44 (fillers
+// for runs >= 256, src/CPU/RunLengthCPU.cpp:8-32), with each filler attributed to the zero that
+// completes it, so a 2048-symbol step of a workgroup never yields more than 2048 pairs.
+#include "../coder.hpp"
+#include "../profile.hpp"
+#include "huff_design.hpp"
+#include "wave.hpp"
+
+namespace cca {
+namespace {
+
+using design::Ctl;
+using design::HeapEntry;
+using design::TableOut;
+using design::Work;
+
+constexpr uint32_t kDesignSmall = 512;   // design classes by distinct-symbol count
+constexpr uint32_t kDesignMid = 4096;
+constexpr uint32_t kWindowWords = kChunk + 4;  // LDS packing window (2048 pairs x 31 bits < 2048 words)
+
+// ------------------------------------------------------------------------ helpers
+__device__ __forceinline__ uint64_t shfl_up_u64(uint64_t v, int off) {
+    const uint32_t lo = __shfl_up((uint32_t)v, off, 64), hi = __shfl_up((uint32_t)(v >> 32), off, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int off) {
+    const uint32_t lo = __shfl_xor((uint32_t)v, off, 64), hi = __shfl_xor((uint32_t)(v >> 32), off, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ long long shfl_up_i64(long long v, int off) { return (long long)shfl_up_u64((uint64_t)v, off); }
+
+// Exclusive scans over a 256-thread workgroup.  sh: 4 entries of LDS.  All threads must call.
+__device__ uint64_t wg_excl_sum_u64(uint64_t v, uint64_t* sh, uint64_t& total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint64_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint64_t y = shfl_up_u64(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) sh[w] = x;
+    __syncthreads();
+    uint64_t base = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const uint64_t s = sh[i];
+        if (i < w) base += s;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return base + x - v;
+}
+__device__ long long wg_excl_max_i64(long long v, long long* sh, long long& total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    long long x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const long long y = shfl_up_i64(x, off);
+        if (lane >= off) x = x > y ? x : y;
+    }
+    long long ex = shfl_up_i64(x, 1);
+    if (lane == 0) ex = -1;
+    if (lane == 63) sh[w] = x;
+    __syncthreads();
+    long long base = -1, tot = -1;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const long long s = sh[i];
+        if (i < w) base = base > s ? base : s;
+        tot = tot > s ? tot : s;
+    }
+    __syncthreads();
+    total = tot;
+    return base > ex ? base : ex;
+}
+__device__ uint32_t wg_sum_u32(uint32_t v, uint32_t* sh) {
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const uint32_t t = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return t;
+}
+__device__ uint32_t wg_max_u32(uint32_t v, uint32_t* sh) {
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const uint32_t a = max(sh[0], sh[1]), b = max(sh[2], sh[3]);
+    __syncthreads();
+    return max(a, b);
+}
+
+// Scratch arena allocation (one thread).  Returns ~0 on exhaustion and flags the job.
+__device__ uint64_t arena_alloc(CoderState* st, uint64_t words, uint64_t cap, SeqInfo* seq) {
+    words = (words + 15) & ~uint64_t(15);
+    const uint64_t off = atomicAdd(&st->scratch_used, (unsigned long long)words);
+    if (off + words > cap) {
+        seq->err = kErrScratch;
+        atomicCAS(&st->err, 0u, (uint32_t)kErrScratch);
+        return ~uint64_t(0);
+    }
+    return off;
+}
+
+__device__ __forceinline__ uint32_t req_bits(uint32_t v) { return v ? 32u - (uint32_t)__clz(v) : 0u; }
+
+template <class S>
+__device__ __forceinline__ void load8(const S* in, uint64_t p0, uint32_t n, uint32_t (&v)[kPerThread]) {
+    if (p0 + kPerThread <= n && ((reinterpret_cast<uintptr_t>(in + p0) & 15) == 0)) {
+        if (sizeof(S) == 2) {
+            const uint4 q = *reinterpret_cast<const uint4*>(in + p0);
+            const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int j = 0; j < 4; j++) { v[2 * j] = w[j] & 0xFFFFu; v[2 * j + 1] = w[j] >> 16; }
+        } else {
+            const uint4 a = *reinterpret_cast<const uint4*>(in + p0);
+            const uint4 b = *reinterpret_cast<const uint4*>(in + p0 + 4);
+            v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < (int)kPerThread; j++) v[j] = (p0 + j < n) ? (uint32_t)in[p0 + j] : 0u;
+    }
+}
+
+// Pairs of one thread's 8 positions.  prev: position of the last nonzero before them (-1 none).
+// limit: fillers are emitted only at positions < limit (the stream's last nonzero; pass A uses n
+// and removes the trailing run's fillers afterwards).  f(sym, zc) per pair, in order.
+template <class F>
+__device__ __forceinline__ void thread_pairs(const uint32_t (&v)[kPerThread], uint64_t p0, uint32_t n, bool rl,
+                                             long long prev, long long limit, F f) {
+#pragma unroll
+    for (int j = 0; j < (int)kPerThread; j++) {
+        const long long pos = (long long)p0 + j;
+        if (pos >= (long long)n) break;
+        if (!rl) {
+            f(v[j], 0u);
+        } else if (v[j] != 0u) {
+            f(v[j], (uint32_t)((pos - prev - 1) & 255));
+            prev = pos;
+        } else if (((pos - prev - 1) & 255) == 255 && pos < limit) {
+            f(0u, 255u);
+        }
+    }
+}
+
+__device__ __forceinline__ long long thread_last_nz(const uint32_t (&v)[kPerThread], uint64_t p0, uint32_t n) {
+    long long r = -1;
+#pragma unroll
+    for (int j = 0; j < (int)kPerThread; j++)
+        if (v[j] != 0u && p0 + j < n) r = (long long)p0 + j;
+    return r;
+}
+
+// ------------------------------------------------------------------------ kernels
+__global__ void k_coder_init(CoderState* st) {
+    if (threadIdx.x == 0) {
+        st->scratch_used = 0;
+        st->out_words = 0;
+        st->err = 0;
+        st->pad = 0;
+    }
+}
+
+// Pass A: pair counts, histograms (LDS), maxima; tables of the LDS class get their histogram
+// copied to the arena; streams with symbols >= kStatBins are marked wide and get a zeroed
+// global histogram for k_code_stats_wide.
+template <class S>
+__global__ __launch_bounds__(kCoderThreads) void k_code_stats(const CodeItem* items, SeqInfo* seqs, CoderParams p,
+                                                              uint32_t* scratch, uint64_t cap, CoderState* st) {
+    __shared__ uint32_t hs[kStatBins];
+    __shared__ uint32_t hz[256];
+    __shared__ long long shl[4];
+    __shared__ uint32_t shu[4];
+    __shared__ uint64_t sh_off[2];
+    const CodeItem it = items[blockIdx.x];
+    const S* in = static_cast<const S*>(it.in);
+    const bool rl = it.rl != 0;
+    const uint32_t n = it.n;
+    for (uint32_t i = threadIdx.x; i < kStatBins; i += kCoderThreads) hs[i] = 0;
+    for (uint32_t i = threadIdx.x; i < 256; i += kCoderThreads) hz[i] = 0;
+    __syncthreads();
+    long long carry = -1;
+    uint32_t cnt = 0, vmax = 0, wide = 0;
+    for (uint64_t base = 0; base < n; base += kChunk) {
+        const uint64_t p0 = base + (uint64_t)threadIdx.x * kPerThread;
+        uint32_t v[kPerThread];
+        load8(in, p0, n, v);
+        long long chunk_max;
+        long long prev = wg_excl_max_i64(thread_last_nz(v, p0, n), shl, chunk_max);
+        prev = prev > carry ? prev : carry;
+        thread_pairs(v, p0, n, rl, prev, (long long)n, [&](uint32_t s, uint32_t z) {
+            cnt++;
+            vmax = max(vmax, s);
+            if (s < kStatBins) atomicAdd(&hs[s], 1u);
+            else wide = 1;
+            if (rl) atomicAdd(&hz[z], 1u);
+        });
+        carry = chunk_max > carry ? chunk_max : carry;
+    }
+    uint32_t M = wg_sum_u32(cnt, shu);
+    wide = wg_max_u32(wide, shu);
+    vmax = wg_max_u32(vmax, shu);
+    if (rl) {  // fillers of the trailing run are not emitted
+        const uint64_t r = (uint64_t)((long long)n - 1 - carry);
+        const uint32_t f = (uint32_t)(r / 256);
+        if (threadIdx.x == 0 && f) {
+            hs[0] -= f;
+            hz[255] -= f;
+        }
+        M -= f;
+        __syncthreads();
+    }
+    // maxima and distinct counts from the histograms
+    uint32_t smax_l = 0, ks = 0, zmax_l = 0, kz = 0;
+    if (!wide) {
+        for (uint32_t i = threadIdx.x; i < kStatBins; i += kCoderThreads)
+            if (hs[i]) { smax_l = i; ks++; }
+    }
+    for (uint32_t i = threadIdx.x; i < 256; i += kCoderThreads)
+        if (hz[i]) { zmax_l = i; kz++; }
+    const uint32_t smax = wide ? vmax : wg_max_u32(smax_l, shu);
+    ks = wg_sum_u32(ks, shu);
+    const uint32_t zmax = wg_max_u32(zmax_l, shu);
+    kz = wg_sum_u32(kz, shu);
+
+    SeqInfo* ss = &seqs[it.seq];
+    SeqInfo* sz = rl ? &seqs[it.seq + 1] : nullptr;
+    const uint64_t lim = 1ull << p.log2;
+    if (threadIdx.x == 0) {
+        ss->count = M;
+        ss->smax = M ? smax : 0u;
+        ss->nsym = wide ? 0u : ks;
+        ss->nlen = 0;
+        ss->err = kErrNone;
+        ss->quirk = 0;
+        ss->wide = wide;
+        ss->item = blockIdx.x;
+        ss->bits = 0;
+        ss->out_off = (uint64_t)(carry + 1);  // last nonzero + 1, read by the pack pass
+        if (p.dialect == CC_DIALECT_GPU && (uint64_t)ss->smax + 1 > lim) ss->err = kErrOverflow;
+        sh_off[0] = ss->err ? ~uint64_t(0) : arena_alloc(st, (uint64_t)ss->smax + 1, cap, ss);
+        if (sz) {
+            sz->count = M;
+            sz->smax = M ? zmax : 0u;
+            sz->nsym = kz;
+            sz->nlen = 0;
+            sz->err = kErrNone;
+            sz->quirk = 0;
+            sz->wide = 0;
+            sz->item = blockIdx.x;
+            sz->bits = 0;
+            sz->out_off = 0;
+            if (p.dialect == CC_DIALECT_GPU && (uint64_t)sz->smax + 1 > lim) sz->err = kErrOverflow;
+            sh_off[1] = sz->err ? ~uint64_t(0) : arena_alloc(st, (uint64_t)sz->smax + 1, cap, sz);
+        }
+    }
+    __syncthreads();
+    if (sh_off[0] != ~uint64_t(0)) {
+        uint32_t* h = scratch + sh_off[0];
+        const uint32_t nb = (M ? smax : 0u) + 1;
+        for (uint32_t i = threadIdx.x; i < nb; i += kCoderThreads) h[i] = wide ? 0u : hs[i];
+        if (threadIdx.x == 0) ss->hist_off = sh_off[0];
+    }
+    if (rl && sh_off[1] != ~uint64_t(0)) {
+        uint32_t* h = scratch + sh_off[1];
+        const uint32_t nb = (M ? zmax : 0u) + 1;
+        for (uint32_t i = threadIdx.x; i < nb; i += kCoderThreads) h[i] = hz[i];
+        if (threadIdx.x == 0) sz->hist_off = sh_off[1];
+    }
+}
+
+// Pass A for streams with symbols >= kStatBins: histogram with global atomics, then the pad
+// quirk (Symbol16, GPU dialect, max symbol 65535, odd count: src/cudaCompress/Histogram.cu:90-105).
+template <class S>
+__global__ __launch_bounds__(kCoderThreads) void k_code_stats_wide(const CodeItem* items, SeqInfo* seqs,
+                                                                   CoderParams p, uint32_t* scratch) {
+    __shared__ long long shl[4];
+    __shared__ uint32_t shu[4];
+    const CodeItem it = items[blockIdx.x];
+    SeqInfo* ss = &seqs[it.seq];
+    if (!ss->wide || ss->err) return;
+    const S* in = static_cast<const S*>(it.in);
+    const bool rl = it.rl != 0;
+    const uint32_t n = it.n;
+    uint32_t* h = scratch + ss->hist_off;
+    const long long last_nz = (long long)ss->out_off - 1;
+    long long carry = -1;
+    for (uint64_t base = 0; base < n; base += kChunk) {
+        const uint64_t p0 = base + (uint64_t)threadIdx.x * kPerThread;
+        uint32_t v[kPerThread];
+        load8(in, p0, n, v);
+        long long chunk_max;
+        long long prev = wg_excl_max_i64(thread_last_nz(v, p0, n), shl, chunk_max);
+        prev = prev > carry ? prev : carry;
+        thread_pairs(v, p0, n, rl, prev, rl ? last_nz : (long long)n,
+                     [&](uint32_t s, uint32_t) { atomicAdd(&h[s], 1u); });
+        carry = chunk_max > carry ? chunk_max : carry;
+    }
+    __syncthreads();
+    const uint32_t nb = ss->smax + 1;
+    const uint32_t quirk = (p.dialect == CC_DIALECT_GPU && sizeof(S) == 2 && ss->smax == 65535u && (ss->count & 1u)) ? 1u : 0u;
+    uint32_t k = 0;
+    for (uint32_t i = threadIdx.x; i < nb; i += kCoderThreads) {
+        uint32_t c = atomicAdd(&h[i], (quirk && i == 65535u) ? 1u : 0u);  // device-coherent read
+        if (quirk && i == 65535u) c++;
+        k += c ? 1u : 0u;
+    }
+    k = wg_sum_u32(k, shu);
+    if (threadIdx.x == 0) {
+        ss->nsym = k;
+        ss->quirk = quirk;
+    }
+}
+
+// wave64 executor of the design code (one wave per workgroup)
+struct WaveEx {
+    __device__ int lane() const { return (int)threadIdx.x; }
+    __device__ int lanes() const { return 64; }
+    __device__ void barrier() const { __syncthreads(); }
+    template <class P, class Sk>
+    __device__ int compact(int lo, int hi, P pred, Sk sink) const {
+        int total = 0;
+        for (int base = lo; base < hi; base += 64) {
+            const int i = base + lane();
+            const bool f = i < hi && pred(i);
+            const uint64_t m = __ballot(f);
+            if (f) sink(i, total + (int)__popcll(m & lanes_below(lane())));
+            total += (int)__popcll(m);
+        }
+        return total;
+    }
+    template <class P>
+    __device__ int count(int lo, int hi, P pred) const {
+        int total = 0;
+        for (int base = lo; base < hi; base += 64) {
+            const int i = base + lane();
+            total += (int)__popcll(__ballot(i < hi && pred(i)));
+        }
+        return total;
+    }
+    __device__ int max_all(int v) const { return (int)wave_max((uint32_t)v); }
+    __device__ uint64_t sum_all(uint64_t v) const {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += shfl_xor_u64(v, off);
+        return v;
+    }
+    template <class F>
+    __device__ void stable_rank_by_depth(const uint32_t* el, int K, uint32_t* cpl, F f) const {
+        // lane d-1 counts depth d (depths are 1..31)
+        uint32_t mycount = 0;
+        for (int base = 0; base < K; base += 64) {
+            const int i = base + lane();
+            const bool valid = i < K;
+            const uint32_t key = valid ? (el[i] >> 24) : 0u;
+            uint64_t rem = __ballot(valid);
+            while (rem) {
+                const int leader = __ffsll((unsigned long long)rem) - 1;
+                const uint32_t d = __shfl(key, leader, 64);
+                const uint64_t m = __ballot(valid && key == d);
+                if (lane() == (int)d - 1) mycount += (uint32_t)__popcll(m);
+                rem &= ~m;
+            }
+        }
+        if (lane() < 32) cpl[lane()] = lane() < 31 ? mycount : 0u;
+        __syncthreads();
+        uint32_t fi[32], fc[32];
+        design::first_codes(cpl, 31, fc, fi);
+        uint32_t seen = 0;  // lane d-1: elements of depth d seen so far
+        for (int base = 0; base < K; base += 64) {
+            const int i = base + lane();
+            const bool valid = i < K;
+            const uint32_t e = valid ? el[i] : 0u;
+            const uint32_t key = e >> 24;
+            uint64_t rem = __ballot(valid);
+            while (rem) {
+                const int leader = __ffsll((unsigned long long)rem) - 1;
+                const uint32_t d = __shfl(key, leader, 64);
+                const uint64_t m = __ballot(valid && key == d);
+                const uint32_t b = __shfl(seen, (int)d - 1, 64);
+                if (valid && key == d) f(i, e, b + (uint32_t)__popcll(m & lanes_below(lane())), fi, fc);
+                if (lane() == (int)d - 1) seen += (uint32_t)__popcll(m);
+                rem &= ~m;
+            }
+        }
+    }
+};
+
+// One wave per coded sequence whose distinct-symbol count lies in [kmin, kmax].  gws: the design
+// workspace comes from the arena instead of LDS.
+template <class Idx>
+__global__ __launch_bounds__(64) void k_design(SeqInfo* seqs, CoderParams p, uint32_t* scratch, uint64_t cap,
+                                               CoderState* st, uint32_t kmin, uint32_t kmax, int gws) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ Ctl ctl;
+    __shared__ uint64_t offs[4];
+    __shared__ uint64_t bits_sh;
+    SeqInfo* s = &seqs[blockIdx.x];
+    if (s->err) return;
+    const uint32_t K = s->nsym;
+    if (K < kmin || K > kmax) return;
+    const uint32_t nb = s->smax + 1;
+    const uint64_t hreg = design::heap_region_bytes<Idx>(K);
+    const uint64_t preg = ((uint64_t)sizeof(Idx) * (2ull * K + 2) + 15) & ~uint64_t(15);
+    if (threadIdx.x == 0) {
+        offs[0] = arena_alloc(st, 2ull * nb, cap, s);                   // code, len
+        offs[1] = offs[0] == ~uint64_t(0) ? offs[0] : arena_alloc(st, K ? K : 1, cap, s);  // symbol list
+        offs[2] = 0;
+        if (gws && offs[1] != ~uint64_t(0))
+            offs[2] = arena_alloc(st, (hreg + preg) / 4 + 3 * design::kStackRanges + 4, cap, s);
+    }
+    __syncthreads();
+    if (offs[0] == ~uint64_t(0) || offs[1] == ~uint64_t(0) || offs[2] == ~uint64_t(0)) return;
+    unsigned char* ws = gws ? reinterpret_cast<unsigned char*>(scratch + offs[2]) : smem;
+    Work<Idx> w{reinterpret_cast<HeapEntry*>(ws), reinterpret_cast<Idx*>(ws + hreg),
+                reinterpret_cast<int*>(ws + hreg + preg)};
+    uint32_t info[3];
+    __shared__ uint32_t info_sh[3];
+    __shared__ uint32_t cpl_sh[32];
+    TableOut out{scratch + offs[0], scratch + offs[0] + nb, scratch + offs[1], cpl_sh, info_sh};
+    WaveEx ex;
+    uint64_t bits = 0;
+    design::design_table(ex, scratch + s->hist_off, nb, (int)p.dialect, s->quirk, w, &ctl, out, &bits);
+    if (threadIdx.x == 0) bits_sh = bits;
+    __syncthreads();
+    info[0] = info_sh[0];
+    info[1] = info_sh[1];
+    info[2] = info_sh[2];
+    if (threadIdx.x < 32) s->cpl[threadIdx.x] = threadIdx.x < info[0] ? cpl_sh[threadIdx.x] : 0u;
+    if (threadIdx.x == 0) {
+        s->code_off = offs[0];
+        s->syms_off = offs[1];
+        s->nlen = info[0];
+        s->bits = bits_sh;
+        if (info[2]) {
+            s->err = info[2];
+            atomicCAS(&st->err, 0u, info[2]);
+        } else if (p.dialect == CC_DIALECT_CPU && K >= (1u << 14)) {
+            s->err = kErrOverflow;  // not describable by the 14-bit CPU-dialect header
+            atomicCAS(&st->err, 0u, (uint32_t)kErrOverflow);
+        } else if (bits_sh >= (1ull << 32)) {
+            s->err = kErrOutput;
+            atomicCAS(&st->err, 0u, (uint32_t)kErrOutput);
+        } else {
+            const uint64_t nwords = (bits_sh + 31) / 32 + 1;
+            const uint64_t nblk = (s->count + p.cbs - 1) / p.cbs;
+            s->words_off = arena_alloc(st, nwords, cap, s);
+            if (s->words_off != ~uint64_t(0)) s->boff_off = arena_alloc(st, nblk ? nblk : 1, cap, s);
+        }
+    }
+}
+
+// Pass B: pairs recomputed, codewords packed into an LDS window and flushed as whole words;
+// the absolute bit offset of every coding block is recorded.
+template <class S>
+__global__ __launch_bounds__(kCoderThreads) void k_code_pack(const CodeItem* items, SeqInfo* seqs, CoderParams p,
+                                                             uint32_t* scratch) {
+    __shared__ uint32_t cs[kStatBins];
+    __shared__ unsigned char lsh[kStatBins];
+    __shared__ uint32_t cz[256];
+    __shared__ unsigned char lz[256];
+    __shared__ uint32_t win[2][kWindowWords];
+    __shared__ long long shl[4];
+    __shared__ uint64_t shq[4];
+    const CodeItem it = items[blockIdx.x];
+    const SeqInfo ss = seqs[it.seq];
+    const bool rl = it.rl != 0;
+    if (ss.err || (rl && seqs[it.seq + 1].err)) return;
+    const SeqInfo sz = rl ? seqs[it.seq + 1] : ss;
+    const uint32_t M = ss.count;
+    if (M == 0) return;
+    const S* in = static_cast<const S*>(it.in);
+    const uint32_t n = it.n;
+    const bool sglob = ss.smax >= kStatBins;
+    const uint32_t* gcode = scratch + ss.code_off;
+    const uint32_t* glen = gcode + ss.smax + 1;
+    if (!sglob)
+        for (uint32_t i = threadIdx.x; i <= ss.smax; i += kCoderThreads) {
+            cs[i] = gcode[i];
+            lsh[i] = (unsigned char)glen[i];
+        }
+    if (rl) {
+        const uint32_t* zc = scratch + sz.code_off;
+        for (uint32_t i = threadIdx.x; i <= sz.smax; i += kCoderThreads) {
+            cz[i] = zc[i];
+            lz[i] = (unsigned char)zc[sz.smax + 1 + i];
+        }
+    }
+    for (uint32_t i = threadIdx.x; i < 2 * kWindowWords; i += kCoderThreads) (&win[0][0])[i] = 0;
+    __syncthreads();
+    const bool single = ss.nlen == 0;  // one distinct symbol: no codeword bits
+    const bool zsingle = rl && sz.nlen == 0;
+    uint32_t* ws = scratch + ss.words_off;
+    uint32_t* wz = scratch + sz.words_off;
+    uint32_t* bs = scratch + ss.boff_off;
+    uint32_t* bz = scratch + sz.boff_off;
+    const uint64_t nws = (ss.bits + 31) / 32 + 1, nwz = (sz.bits + 31) / 32 + 1;  // allocated words
+    const long long limit = (long long)ss.out_off - 1;  // last nonzero
+    long long carry = -1;
+    uint64_t pairs = 0, cbits_s = 0, cbits_z = 0;
+    const uint32_t cbs = p.cbs;
+    for (uint64_t base = 0; base < n && pairs < M; base += kChunk) {
+        const uint64_t p0 = base + (uint64_t)threadIdx.x * kPerThread;
+        uint32_t v[kPerThread];
+        load8(in, p0, n, v);
+        long long chunk_max;
+        long long prev = wg_excl_max_i64(thread_last_nz(v, p0, n), shl, chunk_max);
+        prev = prev > carry ? prev : carry;
+        // counts and bits of this thread's pairs
+        uint32_t np = 0, nbs = 0, nbz = 0;
+        thread_pairs(v, p0, n, rl, prev, rl ? limit : (long long)n, [&](uint32_t s, uint32_t z) {
+            np++;
+            nbs += single ? 0u : (sglob ? glen[s] : (uint32_t)lsh[s]);
+            if (rl) nbz += zsingle ? 0u : (uint32_t)lz[z];
+        });
+        uint64_t tot;
+        const uint64_t ex = wg_excl_sum_u64(((uint64_t)np << 40) | ((uint64_t)nbs << 20) | nbz, shq, tot);
+        uint64_t pi = pairs + (ex >> 40);
+        uint64_t os = cbits_s + ((ex >> 20) & 0xFFFFF);
+        uint64_t oz = cbits_z + (ex & 0xFFFFF);
+        const uint64_t wbs = cbits_s >> 5, wbz = cbits_z >> 5;
+        thread_pairs(v, p0, n, rl, prev, rl ? limit : (long long)n, [&](uint32_t s, uint32_t z) {
+            if (pi >= M) return;  // cannot happen when stats and pack agree; keeps writes in bounds
+            if (pi % cbs == 0) {
+                bs[pi / cbs] = (uint32_t)os;
+                if (rl) bz[pi / cbs] = (uint32_t)oz;
+            }
+            if (!single) {
+                const uint32_t l = sglob ? glen[s] : (uint32_t)lsh[s];
+                const uint32_t c = sglob ? gcode[s] : cs[s];
+                const uint32_t wi = (uint32_t)((os >> 5) - wbs), o = (uint32_t)(os & 31);
+                if (l && wi + 1 < kWindowWords) {
+                    if (o + l <= 32) {
+                        atomicOr(&win[0][wi], c << (32 - o - l));
+                    } else {
+                        atomicOr(&win[0][wi], c >> (o + l - 32));
+                        atomicOr(&win[0][wi + 1], c << (64 - o - l));
+                    }
+                }
+                os += l;
+            }
+            if (rl && !zsingle) {
+                const uint32_t l = lz[z], c = cz[z];
+                const uint32_t wi = (uint32_t)((oz >> 5) - wbz), o = (uint32_t)(oz & 31);
+                if (l && wi + 1 < kWindowWords) {
+                    if (o + l <= 32) {
+                        atomicOr(&win[1][wi], c << (32 - o - l));
+                    } else {
+                        atomicOr(&win[1][wi], c >> (o + l - 32));
+                        atomicOr(&win[1][wi + 1], c << (64 - o - l));
+                    }
+                }
+                oz += l;
+            }
+            pi++;
+        });
+        __syncthreads();
+        pairs += tot >> 40;
+        const uint64_t nend_s = cbits_s + ((tot >> 20) & 0xFFFFF), nend_z = cbits_z + (tot & 0xFFFFF);
+        // flush whole words, keep the partial last word at window[0]
+        const uint32_t full_s = (uint32_t)((nend_s >> 5) - wbs), full_z = (uint32_t)((nend_z >> 5) - wbz);
+        for (uint32_t i = threadIdx.x; i < full_s; i += kCoderThreads)
+            if (wbs + i < nws) ws[wbs + i] = win[0][i];
+        if (rl)
+            for (uint32_t i = threadIdx.x; i < full_z; i += kCoderThreads)
+                if (wbz + i < nwz) wz[wbz + i] = win[1][i];
+        __syncthreads();
+        const uint32_t keep_s = win[0][full_s], keep_z = win[1][full_z];
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i <= full_s + 1 && i < kWindowWords; i += kCoderThreads) win[0][i] = 0;
+        for (uint32_t i = threadIdx.x; i <= full_z + 1 && i < kWindowWords; i += kCoderThreads) win[1][i] = 0;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            win[0][0] = keep_s;
+            win[1][0] = keep_z;
+        }
+        __syncthreads();
+        cbits_s = nend_s;
+        cbits_z = nend_z;
+        carry = chunk_max > carry ? chunk_max : carry;
+    }
+    if (threadIdx.x == 0) {
+        if ((cbits_s & 31) && (cbits_s >> 5) < nws) ws[cbits_s >> 5] = win[0][0];
+        if (rl && (cbits_z & 31) && (cbits_z >> 5) < nwz) wz[cbits_z >> 5] = win[1][0];
+    }
+}
+
+__device__ __forceinline__ uint64_t table_bits(const SeqInfo& s, const CoderParams& p) {
+    const uint32_t sb = req_bits(s.smax);
+    return 5ull + (uint64_t)p.count_bits * (s.nlen + 1) + p.symbits_bits + (uint64_t)sb * s.nsym;
+}
+__device__ __forceinline__ uint64_t seq_words(const SeqInfo& s, const CoderParams& p) {
+    const uint64_t nblk = (s.count + p.cbs - 1) / p.cbs;
+    return (table_bits(s, p) + 31) / 32 + 1 + (s.bits + 31) / 32 + (nblk + 1) / 2;
+}
+
+// Section offsets in call order (one workgroup of 1024 threads).
+__global__ __launch_bounds__(1024) void k_layout(const CodeItem* items, uint32_t nitems, SeqInfo* seqs,
+                                                 CoderParams p, CoderState* st, CallOut* calls, uint32_t ncalls,
+                                                 const uint32_t* call_first, uint64_t* item_off, uint64_t out_cap) {
+    __shared__ uint64_t sh[16];
+    uint64_t carry = 0;
+    for (uint32_t base = 0; base < nitems; base += 1024) {
+        const uint32_t i = base + threadIdx.x;
+        uint64_t words = 0;
+        if (i < nitems) {
+            const CodeItem it = items[i];
+            const SeqInfo& a = seqs[it.seq];
+            words = it.rl ? 1 + seq_words(a, p) + seq_words(seqs[it.seq + 1], p) : seq_words(a, p);
+        }
+        // block exclusive scan over 16 waves
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        uint64_t x = words;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint64_t y = shfl_up_u64(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) sh[w] = x;
+        __syncthreads();
+        uint64_t b = 0, tot = 0;
+        for (int k = 0; k < 16; k++) {
+            if (k < w) b += sh[k];
+            tot += sh[k];
+        }
+        __syncthreads();
+        const uint64_t off = carry + b + x - words;
+        if (i < nitems) {
+            const CodeItem it = items[i];
+            item_off[i] = off;
+            SeqInfo& a = seqs[it.seq];
+            a.out_off = off + (it.rl ? 1 : 0);
+            if (it.rl) seqs[it.seq + 1].out_off = a.out_off + seq_words(a, p);
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        item_off[nitems] = carry;
+        st->out_words = carry;
+        if (carry > out_cap) atomicCAS(&st->err, 0u, (uint32_t)kErrOutput);
+    }
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < ncalls; c += 1024) {
+        const uint32_t f = call_first[c], l = call_first[c + 1];
+        uint32_t err = 0;
+        for (uint32_t i = f; i < l && !err; i++) {
+            const CodeItem it = items[i];
+            err = seqs[it.seq].err;
+            if (!err && it.rl) err = seqs[it.seq + 1].err;
+        }
+        calls[c].word_begin = item_off[f];
+        calls[c].words = item_off[l] - item_off[f];
+        calls[c].err = err ? err : (item_off[nitems] > out_cap ? (uint32_t)kErrOutput : 0u);
+        calls[c].pad = 0;
+    }
+}
+
+// 32 header bits starting at bit `pos` of a table header: [nlen:5][cpl x nlen][nsym][sb][symbols].
+__device__ uint32_t header_word(const SeqInfo& s, const uint32_t* cpl, const CoderParams& p, const uint32_t* syms,
+                                uint64_t pos) {
+    const uint32_t cb = p.count_bits, sbf = p.symbits_bits;
+    const uint32_t sb = req_bits(s.smax);
+    const uint64_t fixed = 5ull + (uint64_t)cb * (s.nlen + 1) + sbf;
+    uint32_t out = 0;
+    auto put = [&](uint64_t fpos, uint32_t width, uint32_t val) {  // field at fpos..fpos+width
+        if (width == 0 || fpos + width <= pos || fpos >= pos + 32) return;
+        if (width < 32) val &= (1u << width) - 1u;
+        // align the field's MSB to bit position (fpos - pos) counted from the word's MSB
+        const long long sh = (long long)(pos + 32) - (long long)(fpos + width);  // left shift amount
+        if (sh >= 0) out |= (sh >= 32) ? 0u : (val << sh);
+        else out |= (-sh >= 32) ? 0u : (val >> (-sh));
+    };
+    if (pos < fixed) {
+        put(0, 5, s.nlen);
+        for (uint32_t l = 0; l < s.nlen; l++) put(5 + (uint64_t)cb * l, cb, cpl[l]);
+        put(5 + (uint64_t)cb * s.nlen, cb, s.nsym);
+        put(5 + (uint64_t)cb * (s.nlen + 1), sbf, sb);
+    }
+    if (sb && s.nsym) {
+        const uint64_t lo = pos > fixed ? (pos - fixed) / sb : 0;
+        for (uint64_t k = lo; k < s.nsym; k++) {
+            const uint64_t fp = fixed + k * sb;
+            if (fp >= pos + 32) break;
+            put(fp, sb, syms[k]);
+        }
+    }
+    return out;
+}
+
+__global__ __launch_bounds__(kCoderThreads) void k_gather(const CodeItem* items, const SeqInfo* seqs, CoderParams p,
+                                                          const uint32_t* scratch, const CoderState* st,
+                                                          uint32_t* out, uint64_t out_cap) {
+    const SeqInfo& s = seqs[blockIdx.x];
+    const CodeItem it = items[s.item];
+    if (st->out_words > out_cap) return;
+    if (s.err) return;
+    if (it.rl && (seqs[it.seq].err || seqs[it.seq + 1].err)) return;
+    const bool is_first = it.seq == blockIdx.x;
+    if (it.rl && is_first && threadIdx.x == 0) out[s.out_off - 1] = s.count;  // aligned u32 M
+    const uint64_t tb = table_bits(s, p);
+    const uint32_t tw = (uint32_t)((tb + 31) / 32);
+    const uint32_t* syms = scratch + s.syms_off;
+    uint32_t* o = out + s.out_off;
+    for (uint32_t w = threadIdx.x; w < tw; w += kCoderThreads) o[w] = header_word(s, s.cpl, p, syms, 32ull * w);
+    o += tw;
+    if (threadIdx.x == 0) o[0] = (uint32_t)s.bits;
+    o += 1;
+    const uint64_t nw = (s.bits + 31) / 32;
+    const uint32_t* src = scratch + s.words_off;
+    for (uint64_t w = threadIdx.x; w < nw; w += kCoderThreads) o[w] = src[w];
+    o += nw;
+    const uint32_t nblk = (s.count + p.cbs - 1) / p.cbs;
+    const uint32_t* bo = scratch + s.boff_off;
+    for (uint32_t w = threadIdx.x; w < (nblk + 1) / 2; w += kCoderThreads) {
+        const uint32_t b0 = 2 * w, b1 = 2 * w + 1;
+        const uint32_t i0 = b0 == 0 ? bo[0] : bo[b0] - bo[b0 - 1];  // packInc16, PackInc.cu:210-220
+        const uint32_t i1 = b1 < nblk ? bo[b1] - bo[b1 - 1] : 0u;
+        o[w] = (i0 & 0xFFFFu) | (i1 << 16);
+    }
+}
+
+}  // namespace
+
+void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uint32_t nseq, const CoderParams& p,
+                  int symSize, uint32_t* scratch, uint64_t scratch_cap, CoderState* d_state, CallOut* d_calls,
+                  uint32_t ncalls, const uint32_t* d_call_first_item, uint32_t* d_out, uint64_t out_cap,
+                  uint64_t total_symbols, hipStream_t st) {
+    CCA_REQUIRE(symSize == 2 || symSize == 4, CC_ERR_INVALID_ARG, "symbol size must be 2 or 4");
+    uint64_t* item_off = reinterpret_cast<uint64_t*>(scratch + scratch_cap);  // caller reserves nitems+1 u64
+    CCA_LAUNCH("coder_init", 32, k_coder_init, dim3(1), dim3(64), 0, st, d_state);
+    CCA_LAUNCH_CHECK();
+    if (nitems == 0) return;
+    const uint64_t inb = total_symbols * symSize;
+    if (symSize == 2) {
+        CCA_LAUNCH("code_stats", inb, k_code_stats<uint16_t>, dim3(nitems), dim3(kCoderThreads), 0, st, d_items, d_seqs,
+                   p, scratch, scratch_cap, d_state);
+        CCA_LAUNCH_CHECK();
+        CCA_LAUNCH("code_stats_wide", 0, k_code_stats_wide<uint16_t>, dim3(nitems), dim3(kCoderThreads), 0, st,
+                   d_items, d_seqs, p, scratch);
+    } else {
+        CCA_LAUNCH("code_stats", inb, k_code_stats<uint32_t>, dim3(nitems), dim3(kCoderThreads), 0, st, d_items, d_seqs,
+                   p, scratch, scratch_cap, d_state);
+        CCA_LAUNCH_CHECK();
+        CCA_LAUNCH("code_stats_wide", 0, k_code_stats_wide<uint32_t>, dim3(nitems), dim3(kCoderThreads), 0, st,
+                   d_items, d_seqs, p, scratch);
+    }
+    CCA_LAUNCH_CHECK();
+    const size_t lds_small = design::heap_region_bytes<uint16_t>(kDesignSmall) + 2 * (2 * kDesignSmall + 2) + 16 +
+                             12 * design::kStackRanges + 16;
+    const size_t lds_mid = design::heap_region_bytes<uint16_t>(kDesignMid) + 2 * (2 * kDesignMid + 2) + 16 +
+                           12 * design::kStackRanges + 16;
+    CCA_LAUNCH("huff_design", 0, k_design<uint16_t>, dim3(nseq), dim3(64), lds_small, st, d_seqs, p, scratch,
+               scratch_cap, d_state, 0u, kDesignSmall, 0);
+    CCA_LAUNCH_CHECK();
+    CCA_LAUNCH("huff_design", 0, k_design<uint16_t>, dim3(nseq), dim3(64), lds_mid, st, d_seqs, p, scratch,
+               scratch_cap, d_state, kDesignSmall + 1, kDesignMid, 0);
+    CCA_LAUNCH_CHECK();
+    CCA_LAUNCH("huff_design", 0, k_design<uint32_t>, dim3(nseq), dim3(64), 0, st, d_seqs, p, scratch, scratch_cap,
+               d_state, kDesignMid + 1, 0xFFFFFFFFu, 1);
+    CCA_LAUNCH_CHECK();
+    if (symSize == 2)
+        CCA_LAUNCH("code_pack", inb, k_code_pack<uint16_t>, dim3(nitems), dim3(kCoderThreads), 0, st, d_items, d_seqs,
+                   p, scratch);
+    else
+        CCA_LAUNCH("code_pack", inb, k_code_pack<uint32_t>, dim3(nitems), dim3(kCoderThreads), 0, st, d_items, d_seqs,
+                   p, scratch);
+    CCA_LAUNCH_CHECK();
+    CCA_LAUNCH("code_layout", 0, k_layout, dim3(1), dim3(1024), 0, st, d_items, nitems, d_seqs, p, d_state, d_calls,
+               ncalls, d_call_first_item, item_off, out_cap);
+    CCA_LAUNCH_CHECK();
+    CCA_LAUNCH("code_gather", 0, k_gather, dim3(nseq), dim3(kCoderThreads), 0, st, d_items, d_seqs, p, scratch,
+               d_state, d_out, out_cap);
+    CCA_LAUNCH_CHECK();
+}
+
+}  // namespace cca

@@ -1,0 +1,456 @@
+// Huffman table design, reproduced step for step so that ties resolve as in the reference.
+//
+// The reference designs its tables on the host (src/cudaCompress/HuffmanTable.cu:487-586,
+// src/cudaCompress/HuffmanDesign.h:13-65): a std::priority_queue of leaf pointers compared by
+// count only (the rarest on top), seeded with the range constructor (std::make_heap) in the GPU
+// dialect or by one push per leaf in the CPU dialect (src/CPU/HuffmanTableCPU.cpp:60-62); the two
+// rarest nodes are merged until one remains; then std::sort orders the leaves by depth only.
+// Neither the heap nor the sort is stable, so the tree shape and the order of equal-length
+// codewords -- hence the bitstream -- depend on the exact element moves of libstdc++'s
+// std::make_heap / pop_heap / push_heap (stl_heap.h) and std::sort (introsort with median-of-3,
+// unguarded partition, a 2*floor(log2 n) depth limit falling back to heap sort, and a final
+// insertion sort with threshold 16; stl_algo.h).  Those algorithms are restated here from their
+// published steps.
+//
+// The same code runs on the GPU (one wave per table, tables in LDS) and on the host (the CPU
+// test compares it against the standard library on tie-heavy inputs).  Parallel steps are
+// written against an executor with a host (serial) and a device (wave64) implementation:
+//  * leaves are collected in symbol order;
+//  * the merge loop runs on one lane (the heap is inherently sequential);
+//  * leaf depths are found by walking parent links, one leaf per lane;
+//  * sort: every partition is computed by the whole wave from the closed form of the unguarded
+//    partition (partition_cut); the final insertion sort equals a stable sort by depth (every
+//    element only moves inside its <=16-element block and blocks are ordered), which is done as a
+//    stable counting sort over the 32 possible depths.
+#pragma once
+
+#include <cstdint>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define CCA_HD __host__ __device__ __forceinline__
+#else
+#define CCA_HD inline
+#endif
+
+namespace cca {
+namespace design {
+
+constexpr int kSortThreshold = 16;  // std::sort's _S_threshold
+constexpr int kMaxDepth = 31;       // codeword lengths must fit the 5-bit header field
+
+struct HeapEntry {
+    uint32_t count;
+    uint32_t node;
+};
+
+// ---------------------------------------------------------------- heap (stl_heap.h semantics)
+// Comparator of the reference's queue (HuffmanDesign.h:47-55): a is "less" than b when b is rarer.
+struct QueueLess {
+    CCA_HD bool operator()(const HeapEntry& a, const HeapEntry& b) const { return b.count < a.count; }
+};
+// Comparator of the reference's leaf sort (HuffmanDesign.h:57-63), on packed (depth << 24 | symbol).
+struct DepthLess {
+    CCA_HD bool operator()(uint32_t a, uint32_t b) const { return (a >> 24) < (b >> 24); }
+};
+
+template <class T, class Less>
+CCA_HD void push_heap_at(T* h, int hole, int top, T v, Less less) {
+    int parent = (hole - 1) / 2;
+    while (hole > top && less(h[parent], v)) {
+        h[hole] = h[parent];
+        hole = parent;
+        parent = (hole - 1) / 2;
+    }
+    h[hole] = v;
+}
+
+template <class T, class Less>
+CCA_HD void adjust_heap(T* h, int hole, int len, T v, Less less) {
+    const int top = hole;
+    int child = hole;
+    while (child < (len - 1) / 2) {
+        child = 2 * (child + 1);
+        if (less(h[child], h[child - 1])) child--;
+        h[hole] = h[child];
+        hole = child;
+    }
+    if ((len & 1) == 0 && child == (len - 2) / 2) {
+        child = 2 * (child + 1);
+        h[hole] = h[child - 1];
+        hole = child - 1;
+    }
+    push_heap_at(h, hole, top, v, less);
+}
+
+template <class T, class Less>
+CCA_HD void make_heap(T* h, int len, Less less) {
+    if (len < 2) return;
+    for (int parent = (len - 2) / 2;; parent--) {
+        T v = h[parent];
+        adjust_heap(h, parent, len, v, less);
+        if (parent == 0) return;
+    }
+}
+
+// std::pop_heap over h[0, len): the top moves to h[len-1].
+template <class T, class Less>
+CCA_HD void pop_heap(T* h, int len, Less less) {
+    if (len > 1) {
+        T v = h[len - 1];
+        h[len - 1] = h[0];
+        adjust_heap(h, 0, len - 1, v, less);
+    }
+}
+
+// std::push_heap over h[0, len) whose new element is h[len-1].
+template <class T, class Less>
+CCA_HD void push_heap(T* h, int len, Less less) {
+    T v = h[len - 1];
+    push_heap_at(h, len - 1, 0, v, less);
+}
+
+// std::partial_sort(first, last, last): heap select (make_heap) then sort_heap.
+template <class T, class Less>
+CCA_HD void heap_sort(T* a, int len, Less less) {
+    make_heap(a, len, less);
+    while (len > 1) {
+        len--;
+        T v = a[len];
+        a[len] = a[0];
+        adjust_heap(a, 0, len, v, less);
+    }
+}
+
+// ---------------------------------------------------------------- sort (stl_algo.h semantics)
+template <class T>
+CCA_HD void swap_elems(T* a, int i, int j) {
+    T t = a[i];
+    a[i] = a[j];
+    a[j] = t;
+}
+
+template <class T, class Less>
+CCA_HD void move_median_to_first(T* a, int result, int x, int y, int z, Less less) {
+    if (less(a[x], a[y])) {
+        if (less(a[y], a[z])) swap_elems(a, result, y);
+        else if (less(a[x], a[z])) swap_elems(a, result, z);
+        else swap_elems(a, result, x);
+    } else if (less(a[x], a[z])) {
+        swap_elems(a, result, x);
+    } else if (less(a[y], a[z])) {
+        swap_elems(a, result, z);
+    } else {
+        swap_elems(a, result, y);
+    }
+}
+
+template <class T, class Less>
+CCA_HD int unguarded_partition(T* a, int first, int last, int pivot, Less less) {
+    for (;;) {
+        while (less(a[first], a[pivot])) first++;
+        last--;
+        while (less(a[pivot], a[last])) last--;
+        if (!(first < last)) return first;
+        swap_elems(a, first, last);
+        first++;
+    }
+}
+
+// __unguarded_partition_pivot: median of (first+1, mid, last-1) moved to first.
+template <class T, class Less>
+CCA_HD void place_pivot(T* a, int first, int last, Less less) {
+    const int mid = first + (last - first) / 2;
+    move_median_to_first(a, first, first + 1, mid, last - 1, less);
+}
+
+CCA_HD int floor_log2(uint32_t n) {
+    int r = 0;
+    while (n >>= 1) r++;
+    return r;
+}
+
+// The serial introsort loop (without the final insertion sort) over [first, last) with the given
+// remaining depth.  stack: room for 2*floor(log2 n)+2 ranges (3 ints each).
+template <class T, class Less>
+CCA_HD void introsort_loop(T* a, int first, int last, int depth, Less less, int* stack) {
+    int sp = 0;
+    for (;;) {
+        while (last - first > kSortThreshold) {
+            if (depth == 0) {
+                heap_sort(a + first, last - first, less);
+                break;
+            }
+            depth--;
+            place_pivot(a, first, last, less);
+            const int cut = unguarded_partition(a, first + 1, last, first, less);
+            stack[sp++] = cut;  // right part [cut, last) with the decremented depth, later
+            stack[sp++] = last;
+            stack[sp++] = depth;
+            last = cut;
+        }
+        if (sp == 0) return;
+        depth = stack[--sp];
+        last = stack[--sp];
+        first = stack[--sp];
+    }
+}
+
+// Closed form of unguarded_partition(a, first+1, last, pivot=first) given, in increasing
+// position order, the positions ge[0..G) of elements not less than the pivot and le[0..L) of
+// elements not greater than it.  The k-th swap exchanges ge[k] with the k-th element of le
+// counted from the right, while ge[k] lies left of it; the scan then stops at the next ge
+// position or at the last swapped right position, whichever comes first.  nswap = the number
+// of k with ge[k] < le[L-1-k]; returns the cut position.
+template <class Idx>
+CCA_HD int partition_cut(const Idx* ge, int G, const Idx* le, int L, int last, int nswap) {
+    int cut = nswap == 0 ? last : (int)le[L - nswap];  // (k-1)-th from the right == le[L-1-(k-1)]
+    if (nswap < G && (int)ge[nswap] < cut) cut = (int)ge[nswap];
+    return cut;
+}
+
+// ---------------------------------------------------------------- canonical codes
+// Codes from the count per length, in codeword-index order: c_0 = 0 and
+// c_i = (c_{i-1} + 1) << (l_i - l_{i-1}) (HuffmanTable.cu:553-580), i.e. the first code of length
+// l is (first(l-1) + count(l-1)) << 1.
+CCA_HD void first_codes(const uint32_t* count_per_length, int nlen, uint32_t* first_code, uint32_t* first_index) {
+    uint32_t code = 0, idx = 0;
+    for (int l = 1; l <= nlen; l++) {
+        first_code[l - 1] = code;
+        first_index[l - 1] = idx;
+        code = (code + count_per_length[l - 1]) << 1;
+        idx += count_per_length[l - 1];
+    }
+}
+
+// ---------------------------------------------------------------- the design driver
+// Memory of one design (LDS on the device).  The heap region holds K HeapEntry during the merge
+// loop and is then reused for the sort: K packed elements followed by the two position lists.
+template <class Idx>
+struct Work {
+    HeapEntry* heap;   // K entries (8K bytes)
+    Idx* parent;       // 2K-1 entries
+    int* stack;        // 3 * kStackRanges ints
+    // views of the heap region after the merge loop
+    CCA_HD uint32_t* elems() const { return reinterpret_cast<uint32_t*>(heap); }
+    CCA_HD Idx* ge(int K) const { return reinterpret_cast<Idx*>(elems() + K); }
+    CCA_HD Idx* le(int K) const { return ge(K) + K; }
+};
+constexpr int kStackRanges = 128;
+
+// bytes of the heap region needed for K leaves with index type Idx
+template <class Idx>
+CCA_HD uint64_t heap_region_bytes(uint32_t K) {
+    const uint64_t a = 8ull * K, b = 4ull * K + 2ull * sizeof(Idx) * K;
+    return a > b ? a : b;
+}
+
+// Shared scalars of a design (LDS on the device).
+struct Ctl {
+    int K, size, root, deepest, err;
+    int first, last, depth, sp;
+    int G, L, T;
+};
+
+// Result of one design.
+struct TableOut {
+    uint32_t* code;   // by symbol, smax+1 entries (only used symbols are written)
+    uint32_t* len;    // by symbol
+    uint32_t* syms;   // symbols in codeword-index order (K entries)
+    uint32_t* cpl;    // count per length, 32 entries (first nlen valid)
+    uint32_t* info;   // [0] nlen, [1] nsym, [2] error (0 ok, 5 codeword too long)
+};
+
+enum { kDialectGpu = 0, kDialectCpu = 1 };
+
+template <class KeyF>
+struct KeyLess {
+    KeyF key;
+    template <class T>
+    CCA_HD bool operator()(const T& a, const T& b) const { return key(a) < key(b); }
+};
+
+// std::sort's introsort loop (without the final insertion sort) over el[0, n), comparing key(e).
+// Partitions are computed by the whole executor from the closed form above; ranges whose depth
+// budget runs out are heap-sorted by lane 0.  ge/le: n entries each; stack: 3*kStackRanges ints.
+template <class Ex, class T, class KeyF, class Idx>
+CCA_HD void introsort_loop_wave(Ex& ex, T* el, int n, KeyF key, Idx* ge, Idx* le, int* stack, Ctl* ctl) {
+    if (n <= kSortThreshold) return;
+    const int lane = ex.lane(), nl = ex.lanes();
+    const KeyLess<KeyF> less{key};
+    if (lane == 0) {
+        stack[0] = 0; stack[1] = n; stack[2] = 2 * floor_log2((uint32_t)n);
+        ctl->sp = 1;
+    }
+    ex.barrier();
+    for (;;) {
+        if (lane == 0) {
+            const int sp = --ctl->sp;
+            ctl->first = stack[3 * sp];
+            ctl->last = stack[3 * sp + 1];
+            ctl->depth = stack[3 * sp + 2];
+            if (ctl->last - ctl->first > kSortThreshold) {
+                if (ctl->depth == 0) {
+                    heap_sort(el + ctl->first, ctl->last - ctl->first, less);
+                    ctl->last = ctl->first;  // this range is finished
+                } else {
+                    ctl->depth--;
+                    place_pivot(el, ctl->first, ctl->last, less);
+                }
+            }
+        }
+        ex.barrier();
+        const int first = ctl->first, last = ctl->last, depth = ctl->depth;
+        if (last - first > kSortThreshold) {
+            const auto p = key(el[first]);
+            const int G = ex.compact(first + 1, last, [&](int i) { return !(key(el[i]) < p); },
+                                     [&](int i, int r) { ge[r] = (Idx)i; });
+            const int L = ex.compact(first + 1, last, [&](int i) { return !(p < key(el[i])); },
+                                     [&](int i, int r) { le[r] = (Idx)i; });
+            ex.barrier();
+            const int mn = G < L ? G : L;
+            const int nsw = ex.count(0, mn, [&](int t) { return (int)ge[t] < (int)le[L - 1 - t]; });
+            for (int t = lane; t < nsw; t += nl) swap_elems(el, (int)ge[t], (int)le[L - 1 - t]);
+            ex.barrier();
+            if (lane == 0) {
+                const int cut = partition_cut(ge, G, le, L, last, nsw);
+                int sp = ctl->sp;
+                stack[3 * sp] = cut; stack[3 * sp + 1] = last; stack[3 * sp + 2] = depth; sp++;
+                stack[3 * sp] = first; stack[3 * sp + 1] = cut; stack[3 * sp + 2] = depth; sp++;
+                ctl->sp = sp;
+            }
+            ex.barrier();
+        }
+        const int sp = ctl->sp;
+        ex.barrier();
+        if (sp == 0) return;
+    }
+}
+
+// Designs one table.  Every lane of the executor calls this with the same arguments.
+// Returns nothing; results in `out`, Σ hist*len (minus one codeword when `quirk`) in *bits (lane 0).
+template <class Ex, class Idx>
+CCA_HD void design_table(Ex& ex, const uint32_t* hist, uint32_t nbins, int dialect, uint32_t quirk, Work<Idx> w,
+                         Ctl* ctl, TableOut out, uint64_t* bits) {
+    const int lane = ex.lane(), nl = ex.lanes();
+    // (1) leaves in symbol order
+    int K = ex.compact(0, (int)nbins, [&](int s) { return hist[s] != 0u; },
+                       [&](int s, int rank) { w.heap[rank] = HeapEntry{hist[s], (uint32_t)rank}; });
+    ex.barrier();
+    if (K == 0) {
+        if (lane == 0) {
+            out.info[0] = 0; out.info[1] = 0; out.info[2] = 0;
+            *bits = 0;
+        }
+        return;
+    }
+    // (2) merge loop, one lane
+    if (lane == 0) {
+        if (dialect == kDialectGpu) {
+            make_heap(w.heap, K, QueueLess());
+        } else {
+            for (int i = 2; i <= K; i++) push_heap(w.heap, i, QueueLess());
+        }
+        int size = K, next = K;
+        while (size > 1) {
+            const HeapEntry a = w.heap[0];
+            pop_heap(w.heap, size--, QueueLess());
+            const HeapEntry b = w.heap[0];
+            pop_heap(w.heap, size--, QueueLess());
+            w.parent[a.node] = (Idx)next;
+            w.parent[b.node] = (Idx)next;
+            w.heap[size++] = HeapEntry{a.count + b.count, (uint32_t)next};
+            push_heap(w.heap, size, QueueLess());
+            next++;
+        }
+        ctl->root = next - 1;
+        ctl->deepest = 0;
+        ctl->err = 0;
+    }
+    ex.barrier();
+    // (3) leaf depths, packed with the symbol into the (now free) heap region
+    const int root = ctl->root;
+    uint32_t* el = w.elems();
+    int deepest = 0;
+    ex.compact(0, (int)nbins, [&](int s) { return hist[s] != 0u; }, [&](int s, int rank) {
+        int d = 0;
+        for (int x = rank; x != root; x = (int)w.parent[x]) d++;
+        deepest = d > deepest ? d : deepest;
+        el[rank] = ((uint32_t)(d > kMaxDepth ? kMaxDepth : d) << 24) | (uint32_t)s;
+    });
+    deepest = ex.max_all(deepest);
+    ex.barrier();
+    if (deepest > kMaxDepth) {
+        if (lane == 0) { out.info[0] = 0; out.info[1] = 0; out.info[2] = 5; *bits = 0; }
+        return;
+    }
+    // (4) introsort loop over the depths
+    introsort_loop_wave(ex, el, K, [](uint32_t e) { return e >> 24; }, w.ge(K), w.le(K), w.stack, ctl);
+    // (5) final insertion sort == stable sort by depth: counting sort into out.syms; (6) codes
+    uint32_t* cpl = out.cpl;
+    for (int l = lane; l < 32; l += nl) cpl[l] = 0;
+    ex.barrier();
+    uint64_t b = 0;
+    if (deepest == 0) {  // a single leaf: no codeword bits
+        if (lane == 0) {
+            out.syms[0] = el[0] & 0xFFFFFFu;
+            out.code[el[0] & 0xFFFFFFu] = 0;
+            out.len[el[0] & 0xFFFFFFu] = 0;
+        }
+    } else {
+        ex.stable_rank_by_depth(el, K, cpl, [&](int i, uint32_t e, uint32_t rank_in_depth, const uint32_t* first_idx,
+                                                 const uint32_t* first_code) {
+            const uint32_t d = e >> 24, s = e & 0xFFFFFFu;
+            const uint32_t dest = first_idx[d - 1] + rank_in_depth;
+            out.syms[dest] = s;
+            out.code[s] = first_code[d - 1] + rank_in_depth;
+            out.len[s] = d;
+            // the pad symbol counted by the histogram quirk is not encoded
+            b += (uint64_t)(hist[s] - ((quirk && s == 65535u) ? 1u : 0u)) * d;
+        });
+    }
+    b = ex.sum_all(b);
+    if (lane == 0) {
+        out.info[0] = (uint32_t)deepest;
+        out.info[1] = (uint32_t)K;
+        out.info[2] = 0;
+        *bits = b;
+    }
+}
+
+// ---------------------------------------------------------------- serial executor (host)
+struct SerialEx {
+    CCA_HD int lane() const { return 0; }
+    CCA_HD int lanes() const { return 1; }
+    CCA_HD void barrier() const {}
+    template <class P, class S>
+    CCA_HD int compact(int lo, int hi, P pred, S sink) const {
+        int r = 0;
+        for (int i = lo; i < hi; i++)
+            if (pred(i)) sink(i, r++);
+        return r;
+    }
+    template <class P>
+    CCA_HD int count(int lo, int hi, P pred) const {
+        int r = 0;
+        for (int i = lo; i < hi; i++) r += pred(i) ? 1 : 0;
+        return r;
+    }
+    CCA_HD int max_all(int v) const { return v; }
+    CCA_HD uint64_t sum_all(uint64_t v) const { return v; }
+    // counts per depth into cpl, then visits elements in order with their rank among equal depths
+    template <class F>
+    CCA_HD void stable_rank_by_depth(const uint32_t* el, int K, uint32_t* cpl, F f) const {
+        for (int i = 0; i < K; i++) cpl[(el[i] >> 24) - 1]++;
+        uint32_t fi[32], fc[32], seen[32];
+        first_codes(cpl, 31, fc, fi);
+        for (int l = 0; l < 32; l++) seen[l] = 0;
+        for (int i = 0; i < K; i++) {
+            const uint32_t d = el[i] >> 24;
+            f(i, el[i], seen[d - 1]++, fi, fc);
+        }
+    }
+};
+
+}  // namespace design
+}  // namespace cca

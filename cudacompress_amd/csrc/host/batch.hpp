@@ -1,0 +1,26 @@
+// Host side of the device-resident coder (hip/coder.hip): builds the item list of a job of coder
+// calls, sizes the scratch arena, launches the pipeline and reports per-call results.
+#pragma once
+
+#include <vector>
+
+#include "../coder.hpp"
+#include "instance.hpp"
+
+namespace cca {
+
+struct CoderCall {
+    bool rl;                       // encodeRLHuff (true) or encodeHuff
+    uint32_t first_stream;         // index into the job's stream pointer list
+    uint32_t nstreams;
+    uint32_t n;                    // symbols per stream
+};
+
+// Encodes `calls` (streams: device pointers) into d_out (capacity out_cap words) without any
+// host synchronisation inside the pipeline; when `wait` is set the call synchronises once at the
+// end, retries with a larger arena if it ran out, and fills `results` (one per call).
+// Returns the total word count (valid when waited).
+uint64_t device_encode(cc_instance* I, const std::vector<CoderCall>& calls, const std::vector<const void*>& streams,
+                       int symSize, uint32_t* d_out, uint64_t out_cap, std::vector<CallOut>* results, bool wait);
+
+}  // namespace cca

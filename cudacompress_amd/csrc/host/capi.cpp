@@ -4,6 +4,7 @@
 #include <string>
 
 #include "../kernels.hpp"
+#include "batch.hpp"
 #include "codec.hpp"
 #include "instance.hpp"
 #include "volume.hpp"
@@ -144,6 +145,24 @@ cc_status cc_set_instance_dialect(cc_instance* inst, int dialect) {
         CCA_REQUIRE(inst && (dialect == CC_DIALECT_GPU || dialect == CC_DIALECT_CPU), CC_ERR_INVALID_ARG,
                     "bad dialect");
         inst->dialect = dialect;
+    });
+}
+
+cc_status cc_encode_device(cc_instance* inst, int rl, int symbolSize, const void* const* d_symbolStreams,
+                           uint32_t streamCount, uint32_t symbolCountPerStream, uint32_t* d_out,
+                           uint64_t outCapacityWords, uint64_t* outWords) {
+    return guarded([&] {
+        CCA_REQUIRE(inst && outWords && (d_out || outCapacityWords == 0), CC_ERR_INVALID_ARG, "null argument");
+        CCA_REQUIRE(streamCount == 0 || d_symbolStreams, CC_ERR_INVALID_ARG, "null stream array");
+        std::vector<const void*> streams(d_symbolStreams, d_symbolStreams + streamCount);
+        std::vector<cca::CoderCall> calls{cca::CoderCall{rl != 0, 0, streamCount, symbolCountPerStream}};
+        std::vector<cca::CallOut> res;
+        *outWords = cca::device_encode(inst, calls, streams, symbolSize, d_out, outCapacityWords, &res, true);
+        const uint32_t err = res.empty() ? 0u : res[0].err;
+        CCA_REQUIRE(err != cca::kErrOverflow, CC_ERR_SYMBOL_OVERFLOW, "symbol exceeds 2^log2HuffmanDistinctSymbolCountMax");
+        CCA_REQUIRE(err != cca::kErrTooLong, CC_ERR_CODEWORD_TOO_LONG, "Huffman codeword longer than 31 bits");
+        CCA_REQUIRE(err != cca::kErrOutput, CC_ERR_INVALID_ARG, "output buffer too small");
+        CCA_REQUIRE(err == 0, CC_ERR_OUT_OF_MEMORY, "device coder scratch exhausted");
     });
 }
 

@@ -40,6 +40,13 @@ struct DeviceGuard {
 
 constexpr int kDecodeRing = 4;  // decode staging slots (the reference keeps 8, InstanceImpl.h:108)
 
+// Buffers of the device-resident coder (host/batch.cpp).
+struct DeviceCoder {
+    DevBuf d_items, d_seqs, d_scratch, d_state, d_calls, d_call_first;
+    PinBuf h_stage, h_result;
+    uint64_t scratch_cap = 0;  // words, excluding the item-offset tail
+};
+
 }  // namespace cca
 
 struct cc_instance {
@@ -70,6 +77,9 @@ struct cc_instance {
     cca::DevBuf d_vol_ptrs;
     cca::PinBuf h_vol_ptrs;
     std::vector<uint32_t> vol_bits;
+
+    // device-resident coder
+    cca::DeviceCoder coder;
 
     ~cc_instance();
 };

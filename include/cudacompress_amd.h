@@ -104,6 +104,17 @@ CC_API cc_status cc_decode_huff(cc_instance* inst, int symbolSize, const uint32_
                                 const uint32_t* streamToBitstream, void* const* d_symbolStreams,
                                 uint32_t streamCount, uint32_t symbolCountPerStream);
 
+/* Device-resident encodeRLHuff (rl != 0) / encodeHuff (rl == 0): the bitstream of one call --
+ * the same words cc_encode_rlhuff / cc_encode_huff return with all start bit offsets 0 -- is
+ * written to device memory d_out (capacity outCapacityWords) by the GPU pipeline alone (RLE,
+ * histograms, Huffman table design, packing and layout all on the device).  *outWords receives
+ * the word count.  Synchronises the instance stream once, at the end.  No reference counterpart
+ * as an entry point: it is encodeRLHuff/encodeHuff (include/cudaCompress/Encode.h:28-52) with the
+ * BitStream kept in HBM. */
+CC_API cc_status cc_encode_device(cc_instance* inst, int rl, int symbolSize, const void* const* d_symbolStreams,
+                                  uint32_t streamCount, uint32_t symbolCountPerStream, uint32_t* d_out,
+                                  uint64_t outCapacityWords, uint64_t* outWords);
+
 /* ---- DWT: include/cudaCompress/util/DWT.h:21-155 ------------------------------------------ */
 
 /* dwtFloat3DForward / dwtFloat2DForward (sizeZ == 1, dpBuffer2 unused), util/DWT.h:21-31.
