@@ -26,6 +26,7 @@ from .api import (  # noqa: E402
     compressVolumeFloat,
     compressVolumeLossless,
     createInstance,
+    decodeDevice,
     decodeHuff,
     decodeRLHuff,
     decompressVolumeFloat,

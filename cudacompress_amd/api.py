@@ -359,6 +359,20 @@ def encodeDevice(inst, rl, symbolStreams, symbolCountPerStream, out):
     return int(nw.value)
 
 
+def decodeDevice(inst, rl, words, symbolStreams, symbolCountPerStream, wordCount=None):
+    """Device-resident decodeRLHuff (rl=True) / decodeHuff of one call whose bitstream is the
+    device tensor `words` (32-bit words).  Returns the number of words the call occupies."""
+    lib = _capi.load()
+    ns = len(symbolStreams)
+    ptrs = (ctypes.c_void_p * max(ns, 1))(*[_ptr(s) for s in symbolStreams])
+    nw = ctypes.c_uint64()
+    avail = wordCount if wordCount is not None else words.numel() * words.element_size() // 4
+    st = lib.cc_decode_device(inst.handle, 1 if rl else 0, inst.symbolSize, _ptr(words), avail, ptrs, ns,
+                              symbolCountPerStream, ctypes.byref(nw))
+    _check(st, "cc_decode_device")
+    return int(nw.value)
+
+
 def decodeRLHuff(inst, bitStream, symbolStreams, symbolCountPerStream):
     """decodeRLHuff (Encode.h:29, :43).  Outputs need not be zeroed."""
     return _decode("cc_decode_rlhuff", inst, bitStream, symbolStreams, symbolCountPerStream)

@@ -81,10 +81,61 @@ struct CallOut {              // per coder call
     uint32_t pad;
 };
 
+// ---- decoder ---------------------------------------------------------------------------------
+// A decode job: calls, each a word-aligned bitstream segment in device memory holding the items
+// of one encodeRLHuff/encodeHuff call in order.  parse (one thread per call: the section sizes
+// are in-band, so a call is walked sequentially) -> tables (one workgroup per coded sequence:
+// first/min/max per length, absolute block offsets) -> blocks (one wave per 64 coding blocks,
+// LDS lookup table) -> rle (one workgroup per RLE item: zero runs expanded into the output).
+struct DecItem {
+    void* out;           // n decoded symbols
+    uint32_t n;
+    uint32_t rl;
+    uint32_t seq;        // first coded sequence
+    uint32_t call;
+};
+
+struct DecCall {
+    const uint32_t* words;  // the call's bitstream
+    uint64_t nwords;        // words available from `words` on
+    uint32_t first_item, nitems;
+};
+
+enum DecErr : uint32_t { kDecOk = 0, kDecCorrupt = 6, kDecScratch = 7 };
+
+struct DecSeq {
+    const uint32_t* base;   // the call's words
+    uint64_t nwords;
+    uint64_t hdr_bit;       // table header
+    uint64_t sym_bit;       // first symbol field of the header
+    uint64_t words_off;     // codewords (word offset from base)
+    uint64_t incs_off;      // u16 increments (word offset from base)
+    uint64_t boff_off;      // arena: absolute block offsets
+    void* out;              // decoded values (arena for RLE sequences, the item output otherwise)
+    uint32_t nlen, nsym, sb, count;
+    uint32_t nwords_cw;     // codeword words
+    uint32_t err;
+    uint32_t item;
+    uint32_t pad;
+    int32_t tab[96];        // first | min | max codeword per length
+};
+
+struct DecCallOut {
+    uint64_t words;         // words consumed by the call
+    uint32_t err;
+    uint32_t pad;
+};
+
 // Launchers (hip/coder.hip).  scratch: arena of `scratch_cap` words.
 void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uint32_t nseq, const CoderParams& p,
                   int symSize, uint32_t* scratch, uint64_t scratch_cap, CoderState* d_state, CallOut* d_calls,
                   uint32_t ncalls, const uint32_t* d_call_first_item, uint32_t* d_out, uint64_t out_cap,
                   uint64_t total_symbols, hipStream_t st);
+// wave_seq: for every decode wave the coded sequence it serves (upper bound of the block count:
+// RLE sequences have at most n pairs); wave_first: that wave's first block.
+void coder_decode(const DecCall* d_calls, uint32_t ncalls, const DecItem* d_items, uint32_t nitems, DecSeq* d_seqs,
+                  uint32_t nseq, const CoderParams& p, int symSize, uint32_t* scratch, uint64_t scratch_cap,
+                  CoderState* d_state, DecCallOut* d_out, const uint32_t* d_wave_seq, const uint32_t* d_wave_first,
+                  uint32_t nwaves, uint64_t total_symbols, hipStream_t st);
 
 }  // namespace cca

@@ -1,0 +1,374 @@
+// Device-resident entropy decoder for gfx950: parse -> tables -> blocks -> rle.
+// Reads the layouts of src/cudaCompress/Encode.cpp:474-646 (decodeRLHuff) and :793-917
+// (decodeHuff) straight from a bitstream in device memory: table headers as
+// src/cudaCompress/HuffmanTable.cu:117-151 reads them (including the 4-bit symbolBits field whose
+// stored 0 means 16 when more than one symbol follows), decode tables as :205-266, block offsets
+// as unpackInc16 (src/cudaCompress/PackInc.cu:222-229), canonical decoding per coding block as
+// src/cudaCompress/HuffmanKernels.cui:197-371, and the zero-run expansion of
+// src/cudaCompress/RunLengthKernels.cui:57-76.  Every read is bounded by the call's word count,
+// so a corrupt stream yields an error status, never an out-of-bounds access.
+#include "../coder.hpp"
+#include "../profile.hpp"
+#include "huff_design.hpp"
+#include "wave.hpp"
+
+namespace cca {
+namespace {
+
+constexpr uint32_t kLut = 10;  // lookup-table bits of the block decoder
+
+// n <= 32 bits at bit position pos of w[0, nwords); ok cleared when out of range
+__device__ __forceinline__ uint32_t rd_bits(const uint32_t* w, uint64_t nwords, uint64_t pos, uint32_t n, bool& ok) {
+    if (n == 0) return 0;
+    if (((pos + n - 1) >> 5) >= nwords) {
+        ok = false;
+        return 0;
+    }
+    const uint64_t wi = pos >> 5;
+    const uint32_t o = (uint32_t)(pos & 31);
+    uint64_t win = (uint64_t)w[wi] << 32;
+    if (o + n > 32) win |= w[wi + 1];
+    return (uint32_t)((win << o) >> (64 - n));
+}
+
+__device__ uint64_t arena_take(CoderState* st, uint64_t words, uint64_t cap, bool& ok) {
+    words = (words + 15) & ~uint64_t(15);
+    const uint64_t off = atomicAdd(&st->scratch_used, (unsigned long long)words);
+    if (off + words > cap) {
+        ok = false;
+        atomicCAS(&st->err, 0u, (uint32_t)kDecScratch);
+        return 0;
+    }
+    return off;
+}
+
+__global__ void k_dec_init(CoderState* st) {
+    if (threadIdx.x == 0) {
+        st->scratch_used = 0;
+        st->out_words = 0;
+        st->err = 0;
+        st->pad = 0;
+    }
+}
+
+// One thread per call: walks the sections (their sizes are in-band) and records every coded
+// sequence's header, codeword and offset positions; carves the arena for block offsets and for
+// the decoded values of RLE sequences.
+__global__ void k_dec_parse(const DecCall* calls, uint32_t ncalls, const DecItem* items, DecSeq* seqs, CoderParams p,
+                            int symSize, uint32_t* scratch, uint64_t cap, CoderState* st, DecCallOut* outc) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncalls) return;
+    const DecCall C = calls[c];
+    const uint32_t cb = p.count_bits, sbf = p.symbits_bits;
+    uint64_t w = 0;
+    uint32_t err = 0;
+    uint32_t i = C.first_item;
+    for (; i < C.first_item + C.nitems && !err; i++) {
+        const DecItem it = items[i];
+        uint32_t count = it.n;
+        if (it.rl) {
+            if (w >= C.nwords) { err = kDecCorrupt; break; }
+            count = C.words[w++];
+            if (count > it.n) { err = kDecCorrupt; break; }
+        }
+        for (uint32_t q = 0; q < (it.rl ? 2u : 1u); q++) {
+            DecSeq& s = seqs[it.seq + q];
+            bool ok = true;
+            const uint64_t hb = w * 32;
+            const uint32_t nlen = rd_bits(C.words, C.nwords, hb, 5, ok);
+            const uint32_t nsym = rd_bits(C.words, C.nwords, hb + 5 + (uint64_t)cb * nlen, cb, ok);
+            uint32_t sb = rd_bits(C.words, C.nwords, hb + 5 + (uint64_t)cb * (nlen + 1), sbf, ok);
+            if (p.dialect == CC_DIALECT_GPU && sbf == 4 && sb == 0 && nsym > 1) sb = 16;
+            const uint64_t sym_bit = hb + 5 + (uint64_t)cb * (nlen + 1) + sbf;
+            const uint64_t tb = 5 + (uint64_t)cb * (nlen + 1) + sbf + (uint64_t)sb * nsym;
+            w = (hb + tb + 31) / 32;
+            if (!ok || sb > 32 || w >= C.nwords || (count > 0 && nsym == 0)) { err = kDecCorrupt; break; }
+            const uint32_t bits = C.words[w++];
+            const uint64_t words_off = w;
+            const uint64_t nwc = ((uint64_t)bits + 31) / 32;
+            w += nwc;
+            const uint64_t incs_off = w;
+            const uint64_t nblk = ((uint64_t)count + p.cbs - 1) / p.cbs;
+            w += (nblk + 1) / 2;
+            if (w > C.nwords) { err = kDecCorrupt; break; }
+            bool aok = true;
+            const uint64_t boff = arena_take(st, nblk ? nblk : 1, cap, aok);
+            void* out = it.out;
+            if (it.rl) {
+                const uint64_t o = arena_take(st, ((uint64_t)count * symSize + 3) / 4 + 1, cap, aok);
+                out = scratch + o;
+            }
+            if (!aok) { err = kDecScratch; break; }
+            s.base = C.words;
+            s.nwords = C.nwords;
+            s.hdr_bit = hb;
+            s.sym_bit = sym_bit;
+            s.words_off = words_off;
+            s.incs_off = incs_off;
+            s.boff_off = boff;
+            s.out = out;
+            s.nlen = nlen;
+            s.nsym = nsym;
+            s.sb = sb;
+            s.count = count;
+            s.nwords_cw = (uint32_t)nwc;
+            s.err = 0;
+            s.item = i;
+        }
+    }
+    if (err) {  // mark every sequence of the call failed
+        for (uint32_t k = C.first_item; k < C.first_item + C.nitems; k++) {
+            const DecItem it = items[k];
+            seqs[it.seq].err = err;
+            seqs[it.seq].item = k;
+            if (it.rl) {
+                seqs[it.seq + 1].err = err;
+                seqs[it.seq + 1].item = k;
+            }
+        }
+        atomicCAS(&st->err, 0u, err);
+    }
+    outc[c].words = w;
+    outc[c].err = err;
+    outc[c].pad = 0;
+}
+
+// One workgroup per coded sequence: first/min/max codeword per length (closed form of the
+// canonical assignment) and absolute coding-block offsets.
+__global__ __launch_bounds__(256) void k_dec_tables(DecSeq* seqs, CoderParams p, uint32_t* scratch,
+                                                    CoderState* st) {
+    __shared__ uint32_t cpl[32];
+    __shared__ uint32_t sh[4];
+    __shared__ uint32_t bad;
+    DecSeq& s = seqs[blockIdx.x];
+    if (s.err) return;
+    if (threadIdx.x == 0) bad = 0;
+    if (threadIdx.x < 32) {
+        bool ok = true;
+        cpl[threadIdx.x] = threadIdx.x < s.nlen
+                               ? rd_bits(s.base, s.nwords, s.hdr_bit + 5 + (uint64_t)p.count_bits * threadIdx.x,
+                                         p.count_bits, ok)
+                               : 0u;
+        if (!ok) atomicOr(&bad, 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t fc[32], fi[32];
+        design::first_codes(cpl, (int)s.nlen, fc, fi);
+        uint64_t total = 0;
+        for (uint32_t l = 0; l < 32; l++) {
+            const bool used = l < s.nlen && cpl[l] > 0;
+            s.tab[l] = used ? (int32_t)fi[l] : -1;
+            s.tab[32 + l] = used ? (int32_t)fc[l] : -1;
+            s.tab[64 + l] = used ? (int32_t)(fc[l] + cpl[l] - 1) : -1;
+            if (l < s.nlen) total += cpl[l];
+        }
+        if (s.nlen > 0 && total != s.nsym) bad = 1;
+    }
+    // absolute block offsets: running sum of the u16 increments
+    const uint32_t nblk = (s.count + p.cbs - 1) / p.cbs;
+    uint32_t* boff = scratch + s.boff_off;
+    const uint32_t* inc = s.base + s.incs_off;
+    const uint64_t limit = 32ull * s.nwords_cw;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nblk; base += 256 * 4) {
+        uint32_t v[4], sum = 0;
+        for (int k = 0; k < 4; k++) {
+            const uint32_t b = base + threadIdx.x * 4 + k;
+            v[k] = b < nblk ? (inc[b >> 1] >> (16 * (b & 1))) & 0xFFFFu : 0u;
+            sum += v[k];
+        }
+        uint32_t tot;
+        uint32_t ex = block_excl_scan<256>(sum, sh, tot);
+        uint32_t acc = carry + ex;
+        for (int k = 0; k < 4; k++) {
+            const uint32_t b = base + threadIdx.x * 4 + k;
+            acc += v[k];
+            if (b < nblk) {
+                boff[b] = acc;
+                if (acc > limit) atomicOr(&bad, 1u);
+            }
+        }
+        carry += tot;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && bad) {
+        s.err = kDecCorrupt;
+        atomicCAS(&st->err, 0u, (uint32_t)kDecCorrupt);
+    }
+}
+
+// One wave per 64 coding blocks of a sequence.
+template <class S>
+__global__ __launch_bounds__(64) void k_dec_blocks(const DecSeq* seqs, const uint32_t* wave_seq,
+                                                   const uint32_t* wave_first, CoderParams p,
+                                                   const uint32_t* scratch) {
+    __shared__ uint32_t lut[1 << kLut];
+    __shared__ int tab[96];
+    __shared__ uint32_t stage[64][33];
+    const DecSeq& s = seqs[wave_seq[blockIdx.x]];
+    if (s.err) return;
+    const uint32_t cbs = p.cbs;
+    const uint32_t nblk = (s.count + cbs - 1) / cbs;
+    const uint32_t fb = wave_first[blockIdx.x];
+    if (fb >= nblk) return;
+    const int lane = threadIdx.x;
+    S* out = static_cast<S*>(s.out);
+    const bool multi = s.nlen > 0;
+    const uint32_t nsym = s.nsym, sb = s.sb;
+    auto symbol_at = [&](int idx) -> uint32_t {  // symbol field idx of the header
+        if (idx < 0 || (uint32_t)idx >= nsym) return 0u;
+        bool ok = true;
+        return rd_bits(s.base, s.nwords, s.sym_bit + (uint64_t)sb * (uint32_t)idx, sb, ok);
+    };
+    if (multi) {
+        for (int i = lane; i < 96; i += 64) tab[i] = s.tab[i];
+        __syncthreads();
+        for (uint32_t e = lane; e < (1u << kLut); e += 64) {
+            uint32_t entry = 0;
+            for (int l = 1; l <= (int)kLut && l <= (int)s.nlen; l++) {
+                const int cw = (int)(e >> (kLut - l));
+                if (cw <= tab[64 + l - 1]) {
+                    entry = ((uint32_t)l << 24) | (symbol_at(tab[l - 1] + cw - tab[32 + l - 1]) & 0xFFFFFFu);
+                    break;
+                }
+            }
+            lut[e] = entry;
+        }
+        __syncthreads();
+    }
+    const uint32_t single = nsym >= 1 ? symbol_at(0) : 0u;
+    const uint32_t b = fb + lane;
+    const bool valid = b < nblk;
+    const uint32_t cnt = valid ? min(cbs, s.count - b * cbs) : 0u;
+    const uint32_t* words = s.base + s.words_off;
+    const uint64_t avail = s.nwords - s.words_off;  // readable words from the codeword start
+    auto word = [&](uint64_t i) -> uint32_t { return i < avail ? words[i] : 0u; };
+    uint64_t cache = 0;
+    uint32_t bp = 0;
+    uint64_t nxt = 0;
+    if (valid && multi) {
+        const uint32_t pos = b == 0 ? 0u : scratch[s.boff_off + b];
+        const uint64_t wi = pos >> 5;
+        cache = ((uint64_t)word(wi) << 32) | (uint64_t)word(wi + 1);
+        bp = pos & 31u;
+        nxt = wi + 2;
+    }
+    const uint32_t mlen = s.nlen;
+    for (uint32_t r = 0; r < cbs; r += 32) {
+        const uint32_t rend = min(r + 32, cnt);
+        for (uint32_t i = r; i < rend; i++) {
+            uint32_t sym;
+            if (multi) {
+                if (bp >= 32) {
+                    cache = (cache << 32) | (uint64_t)word(nxt);
+                    nxt++;
+                    bp -= 32;
+                }
+                const uint32_t peek = (uint32_t)(cache >> (32 - bp));
+                const uint32_t e = lut[peek >> (32 - kLut)];
+                uint32_t len;
+                if (e) {
+                    len = e >> 24;
+                    sym = e & 0xFFFFFFu;
+                } else {
+                    len = kLut + 1;
+                    while (len < mlen && (int)(peek >> (32 - len)) > tab[64 + len - 1]) len++;
+                    const int cw = (int)(peek >> (32 - len));
+                    sym = symbol_at(tab[len - 1] + cw - tab[32 + len - 1]);
+                }
+                bp += len;
+            } else {
+                sym = single;
+            }
+            stage[lane][i - r] = sym;
+        }
+        __syncthreads();
+        for (uint32_t idx = lane; idx < 64 * 32; idx += 64) {
+            const uint32_t blk = idx >> 5, k = idx & 31u;
+            const uint32_t gb = fb + blk;
+            if (gb < nblk) {
+                const uint32_t pos = gb * cbs + r + k;
+                const uint32_t bend = min((gb + 1) * cbs, s.count);
+                if (pos < bend) out[pos] = (S)stage[blk][k];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// One workgroup per RLE item: output position of pair i = sum_{j<i}(z_j + 1) + z_i; zeros fill
+// every position no pair names.
+template <class S>
+__global__ __launch_bounds__(256) void k_dec_rle(const DecItem* items, const DecSeq* seqs, CoderState* st) {
+    __shared__ uint32_t sh[4];
+    const DecItem it = items[blockIdx.x];
+    if (!it.rl) return;
+    const DecSeq& ss = seqs[it.seq];
+    const DecSeq& sz = seqs[it.seq + 1];
+    if (ss.err || sz.err) return;
+    const uint32_t M = ss.count, n = it.n;
+    const S* sym = static_cast<const S*>(ss.out);
+    const S* zc = static_cast<const S*>(sz.out);
+    S* out = static_cast<S*>(it.out);
+    uint64_t carry = 0;
+    for (uint32_t base = 0; base < M; base += 256 * 8) {
+        uint32_t z[8], v[8], sum = 0;
+        for (int k = 0; k < 8; k++) {
+            const uint32_t i = base + threadIdx.x * 8 + k;
+            z[k] = i < M ? (uint32_t)zc[i] : 0u;
+            v[k] = i < M ? (uint32_t)sym[i] : 0u;
+            sum += i < M ? z[k] + 1 : 0u;
+        }
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan<256>(sum, sh, tot);
+        uint64_t pos = carry + ex;
+        for (int k = 0; k < 8; k++) {
+            const uint32_t i = base + threadIdx.x * 8 + k;
+            if (i >= M) break;
+            for (uint32_t j = 0; j < z[k]; j++)
+                if (pos + j < n) out[pos + j] = (S)0;
+            pos += z[k];
+            if (pos < n) out[pos] = (S)v[k];
+            else atomicCAS(&st->err, 0u, (uint32_t)kDecCorrupt);
+            pos++;
+        }
+        carry += tot;
+    }
+    for (uint64_t i = carry + threadIdx.x; i < n; i += 256) out[i] = (S)0;
+}
+
+}  // namespace
+
+void coder_decode(const DecCall* d_calls, uint32_t ncalls, const DecItem* d_items, uint32_t nitems, DecSeq* d_seqs,
+                  uint32_t nseq, const CoderParams& p, int symSize, uint32_t* scratch, uint64_t scratch_cap,
+                  CoderState* d_state, DecCallOut* d_out, const uint32_t* d_wave_seq, const uint32_t* d_wave_first,
+                  uint32_t nwaves, uint64_t total_symbols, hipStream_t st) {
+    CCA_REQUIRE(symSize == 2 || symSize == 4, CC_ERR_INVALID_ARG, "symbol size must be 2 or 4");
+    CCA_LAUNCH("dec_init", 32, k_dec_init, dim3(1), dim3(64), 0, st, d_state);
+    CCA_LAUNCH_CHECK();
+    if (ncalls == 0) return;
+    CCA_LAUNCH("dec_parse", 0, k_dec_parse, dim3(div_up(ncalls, 64)), dim3(64), 0, st, d_calls, ncalls, d_items, d_seqs,
+               p, symSize, scratch, scratch_cap, d_state, d_out);
+    CCA_LAUNCH_CHECK();
+    if (nseq == 0) return;
+    CCA_LAUNCH("dec_tables", 0, k_dec_tables, dim3(nseq), dim3(256), 0, st, d_seqs, p, scratch, d_state);
+    CCA_LAUNCH_CHECK();
+    const uint64_t outb = total_symbols * symSize;
+    if (nwaves) {
+        if (symSize == 2)
+            CCA_LAUNCH("dec_blocks", outb, k_dec_blocks<uint16_t>, dim3(nwaves), dim3(64), 0, st, d_seqs, d_wave_seq,
+                       d_wave_first, p, scratch);
+        else
+            CCA_LAUNCH("dec_blocks", outb, k_dec_blocks<uint32_t>, dim3(nwaves), dim3(64), 0, st, d_seqs, d_wave_seq,
+                       d_wave_first, p, scratch);
+        CCA_LAUNCH_CHECK();
+    }
+    if (symSize == 2)
+        CCA_LAUNCH("dec_rle", outb, k_dec_rle<uint16_t>, dim3(nitems), dim3(256), 0, st, d_items, d_seqs, d_state);
+    else
+        CCA_LAUNCH("dec_rle", outb, k_dec_rle<uint32_t>, dim3(nitems), dim3(256), 0, st, d_items, d_seqs, d_state);
+    CCA_LAUNCH_CHECK();
+}
+
+}  // namespace cca

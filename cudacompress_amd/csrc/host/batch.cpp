@@ -96,4 +96,84 @@ uint64_t device_encode(cc_instance* I, const std::vector<CoderCall>& calls, cons
     }
 }
 
+void device_decode(cc_instance* I, const std::vector<DecodeCall>& calls, const std::vector<void*>& outs, int symSize,
+                   std::vector<DecCallOut>* results, bool wait) {
+    DeviceGuard guard(I->device);
+    hipStream_t st = I->stream;
+    DeviceCoder& D = I->coder;
+    const bool longSymbols = I->log2 > 16;
+    CCA_REQUIRE(symSize == (longSymbols ? 4 : 2), CC_ERR_INVALID_ARG, "symbol size does not match the instance");
+    CCA_REQUIRE(I->dialect == CC_DIALECT_GPU || symSize == 2, CC_ERR_INVALID_ARG, "the CPU dialect is Symbol16 only");
+    const uint32_t ncalls = (uint32_t)calls.size();
+    const uint32_t cbs = I->cbs;
+    uint32_t nitems = 0, nseq = 0, nwaves = 0;
+    uint64_t total_syms = 0, arena = 64;
+    for (const DecodeCall& c : calls) {
+        CCA_REQUIRE((uint64_t)c.first_stream + c.nstreams <= outs.size(), CC_ERR_INVALID_ARG, "bad stream range");
+        CCA_REQUIRE(c.words != nullptr || c.nwords == 0, CC_ERR_INVALID_ARG, "null bitstream");
+        nitems += c.nstreams;
+        const uint32_t q = c.rl ? 2 : 1;
+        nseq += c.nstreams * q;
+        const uint32_t nblk = div_up(c.n, cbs);
+        nwaves += c.nstreams * q * div_up(nblk, 64);
+        total_syms += (uint64_t)c.nstreams * c.n;
+        arena += (uint64_t)c.nstreams * q * (nblk + 16 + (c.rl ? ((uint64_t)c.n * symSize + 3) / 4 + 17 : 0));
+    }
+    const size_t o_calls = 0;
+    const size_t o_items = align_up(sizeof(DecCall) * std::max<uint32_t>(ncalls, 1), 256);
+    const size_t o_wseq = o_items + align_up(sizeof(DecItem) * std::max<uint32_t>(nitems, 1), 256);
+    const size_t o_wfirst = o_wseq + align_up(4ull * std::max<uint32_t>(nwaves, 1), 256);
+    const size_t total = o_wfirst + 4ull * std::max<uint32_t>(nwaves, 1);
+    unsigned char* h = (unsigned char*)D.h_dstage.get(total);
+    DecCall* hc = (DecCall*)(h + o_calls);
+    DecItem* hi = (DecItem*)(h + o_items);
+    uint32_t* hws = (uint32_t*)(h + o_wseq);
+    uint32_t* hwf = (uint32_t*)(h + o_wfirst);
+    uint32_t item = 0, seq = 0, wv = 0;
+    for (uint32_t c = 0; c < ncalls; c++) {
+        const DecodeCall& dc = calls[c];
+        hc[c] = DecCall{dc.words, dc.nwords, item, dc.nstreams};
+        const uint32_t nblk = div_up(dc.n, cbs);
+        for (uint32_t s = 0; s < dc.nstreams; s++) {
+            void* o = outs[dc.first_stream + s];
+            CCA_REQUIRE(o != nullptr || dc.n == 0, CC_ERR_INVALID_ARG, "null output stream");
+            hi[item] = DecItem{o, dc.n, dc.rl ? 1u : 0u, seq, c};
+            for (uint32_t q = 0; q < (dc.rl ? 2u : 1u); q++) {
+                for (uint32_t b = 0; b < nblk; b += 64) {
+                    hws[wv] = seq + q;
+                    hwf[wv] = b;
+                    wv++;
+                }
+            }
+            item++;
+            seq += dc.rl ? 2 : 1;
+        }
+    }
+    unsigned char* d = (unsigned char*)D.d_dcalls.get(total);
+    CCA_HIP(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, st));
+    DecSeq* d_seqs = D.d_dseqs.as<DecSeq>(std::max<uint32_t>(nseq, 1));
+    CoderState* d_state = D.d_dstate.as<CoderState>(1);
+    DecCallOut* d_out = D.d_dout.as<DecCallOut>(std::max<uint32_t>(ncalls, 1));
+    uint32_t* scratch = D.d_dscratch.as<uint32_t>(arena);
+    const HeaderFormat fmt = header_format(I->dialect, I->log2);
+    CoderParams p{cbs, I->log2, (uint32_t)I->dialect, fmt.count_bits, fmt.symbits_bits, 0};
+    coder_decode((const DecCall*)(d + o_calls), ncalls, (const DecItem*)(d + o_items), nitems, d_seqs, nseq, p,
+                 symSize, scratch, arena, d_state, d_out, (const uint32_t*)(d + o_wseq), (const uint32_t*)(d + o_wfirst),
+                 nwaves, total_syms, st);
+    if (!wait) return;
+    const size_t rbytes = sizeof(CoderState) + sizeof(DecCallOut) * ncalls;
+    unsigned char* r = (unsigned char*)D.h_dresult.get(rbytes);
+    CCA_HIP(hipMemcpyAsync(r, d_state, sizeof(CoderState), hipMemcpyDeviceToHost, st));
+    CCA_HIP(hipMemcpyAsync(r + sizeof(CoderState), d_out, sizeof(DecCallOut) * ncalls, hipMemcpyDeviceToHost, st));
+    CCA_HIP(hipStreamSynchronize(st));
+    const CoderState* hs = (const CoderState*)r;
+    if (results) {
+        results->assign((const DecCallOut*)(r + sizeof(CoderState)),
+                        (const DecCallOut*)(r + sizeof(CoderState)) + ncalls);
+        if (hs->err)  // errors found after the parse (tables, RLE) are job-wide
+            for (auto& o : *results)
+                if (!o.err) o.err = hs->err;
+    }
+}
+
 }  // namespace cca

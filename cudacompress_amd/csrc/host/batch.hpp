@@ -23,4 +23,18 @@ struct CoderCall {
 uint64_t device_encode(cc_instance* I, const std::vector<CoderCall>& calls, const std::vector<const void*>& streams,
                        int symSize, uint32_t* d_out, uint64_t out_cap, std::vector<CallOut>* results, bool wait);
 
+struct DecodeCall {
+    const uint32_t* words;         // device bitstream of the call (word aligned)
+    uint64_t nwords;               // words readable from `words` on
+    bool rl;
+    uint32_t first_stream;         // index into the job's output pointer list
+    uint32_t nstreams;
+    uint32_t n;
+};
+
+// Decodes `calls` into the device outputs without host synchronisation inside the pipeline;
+// with `wait`, synchronises once and fills `results` (words consumed and status per call).
+void device_decode(cc_instance* I, const std::vector<DecodeCall>& calls, const std::vector<void*>& outs, int symSize,
+                   std::vector<DecCallOut>* results, bool wait);
+
 }  // namespace cca

@@ -166,6 +166,24 @@ cc_status cc_encode_device(cc_instance* inst, int rl, int symbolSize, const void
     });
 }
 
+cc_status cc_decode_device(cc_instance* inst, int rl, int symbolSize, const uint32_t* d_words, uint64_t wordCount,
+                           void* const* d_symbolStreams, uint32_t streamCount, uint32_t symbolCountPerStream,
+                           uint64_t* outWordsConsumed) {
+    return guarded([&] {
+        CCA_REQUIRE(inst && outWordsConsumed, CC_ERR_INVALID_ARG, "null argument");
+        CCA_REQUIRE(streamCount == 0 || d_symbolStreams, CC_ERR_INVALID_ARG, "null stream array");
+        std::vector<void*> outs(d_symbolStreams, d_symbolStreams + streamCount);
+        std::vector<cca::DecodeCall> calls{
+            cca::DecodeCall{d_words, wordCount, rl != 0, 0, streamCount, symbolCountPerStream}};
+        std::vector<cca::DecCallOut> res;
+        cca::device_decode(inst, calls, outs, symbolSize, &res, true);
+        *outWordsConsumed = res.empty() ? 0 : res[0].words;
+        const uint32_t err = res.empty() ? 0u : res[0].err;
+        CCA_REQUIRE(err != cca::kDecCorrupt, CC_ERR_CORRUPT_STREAM, "malformed bitstream");
+        CCA_REQUIRE(err == 0, CC_ERR_OUT_OF_MEMORY, "device decoder scratch exhausted");
+    });
+}
+
 cc_status cc_encode_rlhuff(cc_instance* inst, int symbolSize, const void* const* d_symbolStreams, uint32_t streamCount,
                            uint32_t symbolCountPerStream, const uint32_t* startBitOffsets, const uint32_t** outWords,
                            const uint64_t** outSegmentWordBegin, const uint64_t** outSegmentBits) {

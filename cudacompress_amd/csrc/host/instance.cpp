@@ -70,9 +70,12 @@ cc_instance::~cc_instance() {
                            &d_tables, &d_words, &d_offsets, &d_block_bits, &d_stream_bits, &d_dec_stage,
                            &d_dec_sym, &d_dec_zc, &d_dec_sums, &d_dec_desc, &d_vol[0], &d_vol[1], &d_vol[2],
                            &d_vol[3], &d_vol_ptrs, &coder.d_items, &coder.d_seqs, &coder.d_scratch,
-                           &coder.d_state, &coder.d_calls, &coder.d_call_first};
+                           &coder.d_state, &coder.d_calls, &coder.d_call_first, &coder.d_dcalls,
+                           &coder.d_ditems, &coder.d_dseqs, &coder.d_dscratch, &coder.d_dstate, &coder.d_dout,
+                           &coder.d_waves};
     for (auto* b : devs) b->release();
-    cca::PinBuf* pins[] = {&h_up, &h_down, &h_hist, &h_words, &h_vol_ptrs, &coder.h_stage, &coder.h_result};
+    cca::PinBuf* pins[] = {&h_up, &h_down, &h_hist, &h_words, &h_vol_ptrs, &coder.h_stage, &coder.h_result,
+                           &coder.h_dstage, &coder.h_dresult};
     for (auto* b : pins) b->release();
     for (auto& b : h_dec_stage) b.release();
     if (prev >= 0) (void)hipSetDevice(prev);

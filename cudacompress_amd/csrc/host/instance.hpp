@@ -44,6 +44,8 @@ constexpr int kDecodeRing = 4;  // decode staging slots (the reference keeps 8, 
 struct DeviceCoder {
     DevBuf d_items, d_seqs, d_scratch, d_state, d_calls, d_call_first;
     PinBuf h_stage, h_result;
+    DevBuf d_dcalls, d_ditems, d_dseqs, d_dscratch, d_dstate, d_dout, d_waves;
+    PinBuf h_dstage, h_dresult;
     uint64_t scratch_cap = 0;  // words, excluding the item-offset tail
 };
 

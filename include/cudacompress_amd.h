@@ -115,6 +115,16 @@ CC_API cc_status cc_encode_device(cc_instance* inst, int rl, int symbolSize, con
                                   uint32_t streamCount, uint32_t symbolCountPerStream, uint32_t* d_out,
                                   uint64_t outCapacityWords, uint64_t* outWords);
 
+/* Device-resident decodeRLHuff (rl != 0) / decodeHuff (rl == 0) of one call whose bitstream
+ * starts at d_words (word aligned, wordCount words readable): table headers, block offsets,
+ * codewords and zero runs are all read and decoded on the GPU into the streamCount device output
+ * streams.  *outWordsConsumed receives the call's length in words.  A malformed stream returns
+ * CC_ERR_CORRUPT_STREAM.  Synchronises the instance stream once, at the end.  decodeRLHuff /
+ * decodeHuff (include/cudaCompress/Encode.h:29-52) with the BitStream in HBM. */
+CC_API cc_status cc_decode_device(cc_instance* inst, int rl, int symbolSize, const uint32_t* d_words,
+                                  uint64_t wordCount, void* const* d_symbolStreams, uint32_t streamCount,
+                                  uint32_t symbolCountPerStream, uint64_t* outWordsConsumed);
+
 /* ---- DWT: include/cudaCompress/util/DWT.h:21-155 ------------------------------------------ */
 
 /* dwtFloat3DForward / dwtFloat2DForward (sizeZ == 1, dpBuffer2 unused), util/DWT.h:21-31.

@@ -118,3 +118,90 @@ def test_device_coder_overflow(cc, torch_gpu, inst14):
     log("device coder overflow")
     s = np.full(100, 20000, np.uint16)
     assert device_words(cc, torch_gpu, inst14, [s], True) is None
+
+
+# ---------------------------------------------------------------------------- device decoder
+def device_decode(cc, torch, inst, words, ns, n, dtype, rl, avail=None):
+    dw = torch.from_numpy(np.ascontiguousarray(words, dtype=np.uint32).view(np.int32)).cuda()
+    tdt = torch.int16 if dtype == np.uint16 else torch.int32
+    outs = [torch.full((max(n, 1),), -7, dtype=tdt, device="cuda") for _ in range(ns)]
+    used = cc.decodeDevice(inst, rl, dw, outs, n, wordCount=avail)
+    torch.cuda.synchronize()
+    return [o.cpu().numpy().view(dtype)[:n] for o in outs], used
+
+
+@pytest.mark.parametrize("name,gen", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("rl", [True, False], ids=["rlhuff", "huff"])
+def test_device_decoder_roundtrip(cc, torch_gpu, inst14, name, gen, rl):
+    """The device decoder reads the oracle's bitstream back to the original symbols."""
+    log("device decoder case", name, "rl", rl)
+    rng = np.random.default_rng(hash(name) & 0xFFFF)
+    streams = gen(rng)
+    words, bits = o.encode(streams, rl=rl, log2=14, cbs=128)
+    dec, used = device_decode(cc, torch_gpu, inst14, words, len(streams), int(streams[0].size), streams[0].dtype, rl)
+    assert used == words.size
+    for a, b in zip(dec, streams):
+        assert np.array_equal(a, b)
+
+
+def test_device_decoder_variants(cc, torch_gpu):
+    """log2 16 with the pad quirk, Symbol32, the CPU dialect (reference-coder bitstreams) and
+    coding block 32."""
+    log("device decoder variants")
+    rng = np.random.default_rng(77)
+    inst = cc.createInstance(0, 4, 1 << 16, 128, 16)
+    s = laplace_symbols(rng, 20001, scale=2000, density=0.8)
+    s[5] = 65535
+    for rl in (True, False):
+        w, _ = o.encode([s], rl=rl, log2=16, cbs=128)
+        dec, used = device_decode(cc, torch_gpu, inst, w, 1, s.size, np.uint16, rl)
+        assert used == w.size and np.array_equal(dec[0], s)
+    inst.close()
+    inst = cc.createInstance(0, 4, 1 << 16, 128, 20)
+    s32 = [laplace_symbols(rng, 15000, scale=50000, density=0.7, dtype=np.uint32, maxsym=(1 << 20) - 1)]
+    for rl in (True, False):
+        w, _ = o.encode(s32, rl=rl, log2=20, cbs=128)
+        dec, used = device_decode(cc, torch_gpu, inst, w, 1, s32[0].size, np.uint32, rl)
+        assert used == w.size and np.array_equal(dec[0], s32[0])
+    inst.close()
+    inst = cc.createInstance(0, 4, 1 << 16, 128, 14)
+    inst.setDialect(cc.DIALECT_CPU)
+    sc = [laplace_symbols(rng, 9000, scale=20, density=0.5) for _ in range(3)]
+    for rl in (True, False):
+        w, _ = o.encode(sc, rl=rl, dialect=o.DIALECT_CPU, cbs=128)
+        dec, used = device_decode(cc, torch_gpu, inst, w, 3, 9000, np.uint16, rl)
+        assert used == w.size and all(np.array_equal(a, b) for a, b in zip(dec, sc))
+    inst.close()
+    inst = cc.createInstance(0, 4, 1 << 16, 32, 14)
+    s = [laplace_symbols(rng, 20000, scale=8) for _ in range(2)]
+    for rl in (True, False):
+        w, _ = o.encode(s, rl=rl, log2=14, cbs=32)
+        dec, used = device_decode(cc, torch_gpu, inst, w, 2, 20000, np.uint16, rl)
+        assert used == w.size and all(np.array_equal(a, b) for a, b in zip(dec, s))
+    inst.close()
+
+
+def test_device_decoder_truncated_stream_reports_error(cc, torch_gpu, inst14):
+    """A bitstream cut short is reported as corrupt (every read is bounded by the word count)."""
+    log("device decoder truncated")
+    rng = np.random.default_rng(5)
+    s = [laplace_symbols(rng, 5000) for _ in range(2)]
+    w, _ = o.encode(s, rl=True, log2=14, cbs=128)
+    with pytest.raises(cc.CCError):
+        device_decode(cc, torch_gpu, inst14, w, 2, 5000, np.uint16, True, avail=w.size // 2)
+
+
+def test_device_encode_decode_chain(cc, torch_gpu, inst14):
+    """Device encode followed by device decode, no host copy of the bitstream in between."""
+    log("device encode->decode chain")
+    rng = np.random.default_rng(6)
+    s = [laplace_symbols(rng, 40000, scale=12, density=0.4) for _ in range(4)]
+    dev = [to_dev(torch_gpu, x) for x in s]
+    out = torch_gpu.zeros(200000, dtype=torch_gpu.int32, device="cuda")
+    nw = cc.encodeDevice(inst14, True, dev, 40000, out)
+    outs = [torch_gpu.full((40000,), -1, dtype=torch_gpu.int16, device="cuda") for _ in range(4)]
+    used = cc.decodeDevice(inst14, True, out, outs, 40000, wordCount=nw)
+    torch_gpu.cuda.synchronize()
+    assert used == nw
+    for a, b in zip(outs, s):
+        assert np.array_equal(a.cpu().numpy().view(np.uint16), b)
