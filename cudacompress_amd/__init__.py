@@ -23,12 +23,14 @@ from .api import (  # noqa: E402
     Instance,
     QUANTIZE_DEADZONE,
     QUANTIZE_UNIFORM,
+    compressBricksFloat,
     compressVolumeFloat,
     compressVolumeLossless,
     createInstance,
     decodeDevice,
     decodeHuff,
     decodeRLHuff,
+    decompressBricksFloat,
     decompressVolumeFloat,
     decompressVolumeLossless,
     destroyInstance,

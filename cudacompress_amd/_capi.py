@@ -58,11 +58,19 @@ SIGNATURES = {
     "cc_decompress_volume_float": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, c_vp, c_u64, c_float, c_int, c_int]),
     "cc_compress_volume_lossless": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, PP_u32, P_u64]),
     "cc_decompress_volume_lossless": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, c_vp, c_u64]),
+    "cc_compress_bricks_float": (c_int, [c_vp, c_vp, c_u32, c_u32, c_int, c_vp, c_u64, P_u64, P_u32]),
+    "cc_decompress_bricks_float": (c_int, [c_vp, c_vp, c_u32, c_u32, c_int, c_vp, P_u64, P_u32]),
     "cc_profile_enable": (None, [c_int]),
     "cc_profile_reset": (None, []),
     "cc_profile_collect": (c_u32, []),
     "cc_profile_entry": (c_int, [c_u32, ctypes.POINTER(ctypes.c_char_p), P_u64, ctypes.POINTER(ctypes.c_double), P_u64]),
 }
+
+
+class Brick(ctypes.Structure):
+    """struct cc_brick of include/cudacompress_amd.h"""
+    _fields_ = [("src", c_vp), ("dst", c_vp), ("sizeX", c_u32), ("sizeY", c_u32), ("sizeZ", c_u32),
+                ("quantStep", c_float)]
 
 
 def profile_entries():

@@ -373,6 +373,46 @@ def decodeDevice(inst, rl, words, symbolStreams, symbolCountPerStream, wordCount
     return int(nw.value)
 
 
+def _bricks(srcs, dsts, shapes, steps):
+    n = len(shapes)
+    arr = (_capi.Brick * max(n, 1))()
+    for i, (z, y, x) in enumerate(shapes):
+        arr[i].src = _ptr(srcs[i]) if srcs is not None else None
+        arr[i].dst = _ptr(dsts[i]) if dsts is not None else None
+        arr[i].sizeX, arr[i].sizeY, arr[i].sizeZ = int(x), int(y), int(z)
+        arr[i].quantStep = float(steps[i])
+    return arr
+
+
+def compressBricksFloat(inst, sources, shapes, quantSteps, numLevels, out, symbolSize=2):
+    """Batched compressVolumeFloat of many bricks (device tensors, shapes (z, y, x)) into the
+    device word tensor `out`.  Returns (offsets, status): brick i is out[offsets[i]:offsets[i+1]]."""
+    lib = _capi.load()
+    n = len(shapes)
+    arr = _bricks(sources, None, shapes, quantSteps)
+    offs = np.zeros(n + 1, dtype=np.uint64)
+    stat = np.zeros(max(n, 1), dtype=np.uint32)
+    cap = out.numel() * out.element_size() // 4
+    st = lib.cc_compress_bricks_float(inst.handle, arr, n, numLevels, symbolSize, _ptr(out), cap,
+                                      offs.ctypes.data_as(_capi.P_u64), stat.ctypes.data_as(_capi.P_u32))
+    if st not in (CC_OK, CC_ERR_SYMBOL_OVERFLOW, CC_ERR_CODEWORD_TOO_LONG):
+        _check(st, "cc_compress_bricks_float")
+    return offs, stat[:n]
+
+
+def decompressBricksFloat(inst, dests, shapes, quantSteps, numLevels, words, offsets, symbolSize=2):
+    """Batched decompressVolumeFloat of the bricks written by compressBricksFloat."""
+    lib = _capi.load()
+    n = len(shapes)
+    arr = _bricks(None, dests, shapes, quantSteps)
+    offs = np.ascontiguousarray(offsets, dtype=np.uint64)
+    stat = np.zeros(max(n, 1), dtype=np.uint32)
+    st = lib.cc_decompress_bricks_float(inst.handle, arr, n, numLevels, symbolSize, _ptr(words),
+                                        offs.ctypes.data_as(_capi.P_u64), stat.ctypes.data_as(_capi.P_u32))
+    _check(st, "cc_decompress_bricks_float")
+    return stat[:n]
+
+
 def decodeRLHuff(inst, bitStream, symbolStreams, symbolCountPerStream):
     """decodeRLHuff (Encode.h:29, :43).  Outputs need not be zeroed."""
     return _decode("cc_decode_rlhuff", inst, bitStream, symbolStreams, symbolCountPerStream)

@@ -300,7 +300,7 @@ __global__ __launch_bounds__(64) void k_dec_blocks(const DecSeq* seqs, const uin
 // One workgroup per RLE item: output position of pair i = sum_{j<i}(z_j + 1) + z_i; zeros fill
 // every position no pair names.
 template <class S>
-__global__ __launch_bounds__(256) void k_dec_rle(const DecItem* items, const DecSeq* seqs, CoderState* st) {
+__global__ __launch_bounds__(256) void k_dec_rle(const DecItem* items, DecSeq* seqs, CoderState* st) {
     __shared__ uint32_t sh[4];
     const DecItem it = items[blockIdx.x];
     if (!it.rl) return;
@@ -329,13 +329,34 @@ __global__ __launch_bounds__(256) void k_dec_rle(const DecItem* items, const Dec
             for (uint32_t j = 0; j < z[k]; j++)
                 if (pos + j < n) out[pos + j] = (S)0;
             pos += z[k];
-            if (pos < n) out[pos] = (S)v[k];
-            else atomicCAS(&st->err, 0u, (uint32_t)kDecCorrupt);
+            if (pos < n) {
+                out[pos] = (S)v[k];
+            } else {
+                seqs[it.seq].err = kDecCorrupt;
+                atomicCAS(&st->err, 0u, (uint32_t)kDecCorrupt);
+            }
             pos++;
         }
         carry += tot;
     }
     for (uint64_t i = carry + threadIdx.x; i < n; i += 256) out[i] = (S)0;
+}
+
+// Per-call status: the first error among the call's coded sequences.
+__global__ void k_dec_finish(const DecCall* calls, uint32_t ncalls, const DecItem* items, const DecSeq* seqs,
+                             DecCallOut* outc) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncalls || outc[c].err) return;
+    const DecCall C = calls[c];
+    for (uint32_t k = C.first_item; k < C.first_item + C.nitems; k++) {
+        const DecItem it = items[k];
+        uint32_t e = seqs[it.seq].err;
+        if (!e && it.rl) e = seqs[it.seq + 1].err;
+        if (e) {
+            outc[c].err = e;
+            return;
+        }
+    }
 }
 
 }  // namespace
@@ -368,6 +389,9 @@ void coder_decode(const DecCall* d_calls, uint32_t ncalls, const DecItem* d_item
         CCA_LAUNCH("dec_rle", outb, k_dec_rle<uint16_t>, dim3(nitems), dim3(256), 0, st, d_items, d_seqs, d_state);
     else
         CCA_LAUNCH("dec_rle", outb, k_dec_rle<uint32_t>, dim3(nitems), dim3(256), 0, st, d_items, d_seqs, d_state);
+    CCA_LAUNCH_CHECK();
+    CCA_LAUNCH("dec_finish", 0, k_dec_finish, dim3(div_up(ncalls, 64)), dim3(64), 0, st, d_calls, ncalls, d_items,
+               d_seqs, d_out);
     CCA_LAUNCH_CHECK();
 }
 

@@ -9,6 +9,24 @@
 
 namespace cca {
 
+// One input stream (encode) or output stream (decode) of a job.  Items are grouped by call, in
+// call order; a call is one contiguous bitstream segment (items may differ in length and rl).
+struct ItemSpec {
+    void* ptr;
+    uint32_t n;
+    bool rl;
+    uint32_t call;
+};
+struct CallWords {
+    const uint32_t* words;  // device bitstream of a decode call
+    uint64_t nwords;
+};
+
+uint64_t device_encode_items(cc_instance* I, const std::vector<ItemSpec>& items, uint32_t ncalls, int symSize,
+                             uint32_t* d_out, uint64_t out_cap, std::vector<CallOut>* results, bool wait);
+void device_decode_items(cc_instance* I, const std::vector<ItemSpec>& items, const std::vector<CallWords>& calls,
+                         int symSize, std::vector<DecCallOut>* results, bool wait);
+
 struct CoderCall {
     bool rl;                       // encodeRLHuff (true) or encodeHuff
     uint32_t first_stream;         // index into the job's stream pointer list

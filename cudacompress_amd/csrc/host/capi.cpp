@@ -5,6 +5,7 @@
 
 #include "../kernels.hpp"
 #include "batch.hpp"
+#include "bricks.hpp"
 #include "codec.hpp"
 #include "instance.hpp"
 #include "volume.hpp"
@@ -181,6 +182,25 @@ cc_status cc_decode_device(cc_instance* inst, int rl, int symbolSize, const uint
         const uint32_t err = res.empty() ? 0u : res[0].err;
         CCA_REQUIRE(err != cca::kDecCorrupt, CC_ERR_CORRUPT_STREAM, "malformed bitstream");
         CCA_REQUIRE(err == 0, CC_ERR_OUT_OF_MEMORY, "device decoder scratch exhausted");
+    });
+}
+
+cc_status cc_compress_bricks_float(cc_instance* inst, const cc_brick* bricks, uint32_t count, uint32_t numLevels,
+                                   int symbolSize, uint32_t* d_out, uint64_t outCapacityWords,
+                                   uint64_t* brickWordOffsets, uint32_t* brickStatus) {
+    return guarded([&] {
+        CCA_REQUIRE(inst && brickWordOffsets && (d_out || outCapacityWords == 0), CC_ERR_INVALID_ARG, "null argument");
+        cca::compress_bricks(inst, bricks, count, numLevels, symbolSize, d_out, outCapacityWords, brickWordOffsets,
+                             brickStatus);
+    });
+}
+
+cc_status cc_decompress_bricks_float(cc_instance* inst, const cc_brick* bricks, uint32_t count, uint32_t numLevels,
+                                     int symbolSize, const uint32_t* d_in, const uint64_t* brickWordOffsets,
+                                     uint32_t* brickStatus) {
+    return guarded([&] {
+        CCA_REQUIRE(inst && brickWordOffsets && (d_in || count == 0), CC_ERR_INVALID_ARG, "null argument");
+        cca::decompress_bricks(inst, bricks, count, numLevels, symbolSize, d_in, brickWordOffsets, brickStatus);
     });
 }
 

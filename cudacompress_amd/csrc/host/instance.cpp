@@ -72,10 +72,10 @@ cc_instance::~cc_instance() {
                            &d_vol[3], &d_vol_ptrs, &coder.d_items, &coder.d_seqs, &coder.d_scratch,
                            &coder.d_state, &coder.d_calls, &coder.d_call_first, &coder.d_dcalls,
                            &coder.d_ditems, &coder.d_dseqs, &coder.d_dscratch, &coder.d_dstate, &coder.d_dout,
-                           &coder.d_waves};
+                           &coder.d_waves, &d_bwork, &d_bsym, &d_blevels};
     for (auto* b : devs) b->release();
     cca::PinBuf* pins[] = {&h_up, &h_down, &h_hist, &h_words, &h_vol_ptrs, &coder.h_stage, &coder.h_result,
-                           &coder.h_dstage, &coder.h_dresult};
+                           &coder.h_dstage, &coder.h_dresult, &h_blevels};
     for (auto* b : pins) b->release();
     for (auto& b : h_dec_stage) b.release();
     if (prev >= 0) (void)hipSetDevice(prev);

@@ -83,5 +83,9 @@ struct cc_instance {
     // device-resident coder
     cca::DeviceCoder coder;
 
+    // brick batches (host/bricks.cpp)
+    cca::DevBuf d_bwork, d_bsym, d_blevels;
+    cca::PinBuf h_blevels;
+
     ~cc_instance();
 };

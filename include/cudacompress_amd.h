@@ -205,6 +205,33 @@ CC_API cc_status cc_decompress_volume_lossless(cc_instance* inst, int16_t* d_ima
                                                uint32_t sizeZ, uint32_t numLevels, const uint32_t* bits,
                                                uint64_t bitCount);
 
+/* ---- Brick batches ------------------------------------------------------------------------ */
+/* No single reference counterpart: a batch is compressVolumeFloat / decompressVolumeFloat
+ * (src/examples/CompressVolume.cpp:313-516, doRLEOnlyOnLvl0 = false) applied to many bricks of
+ * different sizes in one pipeline of launches (DWT passes over all bricks at once, quantization
+ * fused into the last pass, the device-resident coder for all bricks' streams).  Brick i's
+ * bitstream is word for word what cc_compress_volume_float returns for that brick alone. */
+typedef struct cc_brick {
+    const float* src;   /* device, dense sizeX*sizeY*sizeZ (compress input) */
+    float* dst;         /* device, dense sizeX*sizeY*sizeZ (decompress output) */
+    uint32_t sizeX, sizeY, sizeZ;  /* each divisible by 2^numLevels; sizeZ > 1 */
+    float quantStep;    /* level-0 quantization step */
+} cc_brick;
+
+/* Compresses `count` bricks into d_out (device, outCapacityWords words): brick i occupies words
+ * [brickWordOffsets[i], brickWordOffsets[i+1]) (host array of count+1 entries).  brickStatus
+ * (host, count entries, may be NULL) receives each brick's status; the return value is CC_OK only
+ * when every brick succeeded.  Synchronises the instance stream once, at the end. */
+CC_API cc_status cc_compress_bricks_float(cc_instance* inst, const cc_brick* bricks, uint32_t count,
+                                          uint32_t numLevels, int symbolSize, uint32_t* d_out,
+                                          uint64_t outCapacityWords, uint64_t* brickWordOffsets,
+                                          uint32_t* brickStatus);
+/* Decompresses the bricks written by cc_compress_bricks_float (same bricks, levels and steps)
+ * from d_in into each brick's dst. */
+CC_API cc_status cc_decompress_bricks_float(cc_instance* inst, const cc_brick* bricks, uint32_t count,
+                                            uint32_t numLevels, int symbolSize, const uint32_t* d_in,
+                                            const uint64_t* brickWordOffsets, uint32_t* brickStatus);
+
 /* ---- Kernel timing ------------------------------------------------------------------------ */
 /* No reference counterpart (the reference times host-side sections with CUDA events,
  * src/cudaCompress/util/CudaTimer.cpp:65-169).  When enabled, every kernel launch of this library
