@@ -1,18 +1,23 @@
 """Benchmark of the wavelet compression path on one MI355X.
 
-Default workload (the lossless integer configuration): a 4096x4096 uint16 heightfield of spectral
-fractional Brownian motion (Hurst 0.8, seed 2) stored as int16 = u - 32768, compressed with a
-3-level 2D CDF 5/3 DWT, three RLE+Huffman highpass streams per level and a Huffman lowpass stream
-(log2HuffmanDistinctSymbolCountMax 16, coding block 128), then decompressed.  One step = one
-compress + one decompress with the input already resident in HBM.
+Default workload (`bricks`, the batched configuration the headline metric compares with plain
+calls): 2048 float32 bricks, each axis drawn from {16,32,48,64,96,128} (seeded), cut from seeded
+k^-5/3 Gaussian random fields; 2-level 3D CDF 9/7, deadzone quantization at 0.005 x the brick's
+std, RLE+Huffman (log2 14, coding block 128).  One step = batched compression of every brick into
+one device buffer + batched decompression, inputs and bitstreams resident in HBM.  The same
+bricks are also run through the plain per-brick API (compressVolumeFloat/decompressVolumeFloat,
+host bitstreams) on a sample, and the batched bitstreams are checked against the plain ones.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload lossless2d|float3d]
+Other workloads: `lossless2d` (4096^2 uint16 fBm, 3-level 2D CDF 5/3, log2 16, plain API) and
+`float3d` (128^3 k^-5/3 field, 2-level 9/7, step 0.01, plain API).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload bricks|lossless2d|float3d]
 
 Prints one JSON line.  value = raw GB/s = (raw bytes encoded + raw bytes decoded) / step time.
-roofline: the kernel with the largest total time over the timed steps, its algorithmic bytes per
-launch divided by its average launch time (hipEvents on its stream, via cc_profile_*).
-cpu_baseline: the oracle's restatement of the same pipeline (oracle/pipeline.py over the C++
-oracle), one host thread, timed on the same input.
+roofline: the kernel with the largest total time over the timed steps -- its algorithmic bytes
+per launch over its average launch time, from HIP events on its stream (cc_profile_*).
+cpu_baseline: the oracle (C++ restatement of the reference path, one host thread) on a bounded
+sample of the same workload.
 """
 import argparse
 import json
@@ -32,10 +37,13 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="lossless2d", choices=["lossless2d", "float3d"])
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="bricks", choices=["bricks", "lossless2d", "float3d"])
+    ap.add_argument("--bricks", type=int, default=2048)
+    ap.add_argument("--plain-sample", type=int, default=256, help="bricks timed through the plain per-brick API")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic.json"))
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device-to-device copy first (the reference kernel whose "
@@ -43,67 +51,67 @@ def parse():
     return ap.parse_args()
 
 
-def lossless2d():
-    from cudacompress_amd import synthetic as S
-    n = 4096
-    img = S.fbm_heightfield(n, 0.8, 2)
-    return dict(name="lossless2d", data=img, shape=(n, n), levels=3, log2=16, cbs=128,
-                desc=("C2 lossless: 4096x4096 uint16 spectral fBm (H=0.8, seed 2) as int16, 3-level 2D CDF 5/3, "
-                      "3 RLHuff highpass streams per level + Huffman lowpass, log2 16, coding block 128, "
-                      "plain per-call API"))
-
-
-def float3d():
-    from cudacompress_amd import synthetic as S
-    n = 128
-    vol = S.gaussian_field((n, n, n), 1)
-    return dict(name="float3d", data=vol, shape=(n, n, n), levels=2, log2=14, cbs=128, q0=0.01,
-                desc=("C1 float: 128^3 k^-5/3 Gaussian random field (seed 1), 2-level 3D CDF 9/7, deadzone step "
-                      "0.01, 7 RLHuff streams per level + Huffman lowpass, log2 14, coding block 128"))
-
-
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (requested --gpus {args.gpus}); "
-              "see DESIGN.md for how bricks would shard", file=sys.stderr)
-        sys.exit(2)
-
-    import torch
-
-    import cudacompress_amd as cc
+def timed(torch, lib, step, steps):
+    """Runs `step` `steps` times between device synchronisations with the kernel profiler on.
+    Returns (ms per step, per-kernel entries)."""
     from cudacompress_amd import _capi
+    lib.cc_profile_reset()
+    lib.cc_profile_enable(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    lib.cc_profile_enable(0)
+    kernels = _capi.profile_entries()
+    lib.cc_profile_reset()
+    return (t1 - t0) * 1e3 / steps, kernels
+
+
+def roofline_of(kernels, steps, workload, traffic_file):
+    kernels = sorted(kernels, key=lambda k: -k[2])
+    dom = kernels[0]
+    per_launch_ms = dom[2] / dom[1]
+    per_launch_bytes = dom[3] / dom[1]
+    achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
+    traffic = None
+    try:
+        with open(traffic_file) as f:
+            t = json.load(f)
+        key = f"{workload}:{dom[0]}"
+        if key in t:
+            traffic = t[key]["hbm_bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        traffic = None
+    roof = {"bound": "hbm", "kernel": dom[0], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+            "algorithmic_bytes_per_launch": int(per_launch_bytes), "avg_launch_us": round(per_launch_ms * 1e3, 3),
+            "launches_per_step": dom[1] / steps}
+    table = [{"kernel": k[0], "launches_per_step": round(k[1] / steps, 2), "ms_per_step": round(k[2] / steps, 4),
+              "GBps": round(k[3] / max(k[2], 1e-9) / 1e6, 1)} for k in kernels[:14]]
+    return roof, table, sum(k[2] for k in kernels) / steps
+
+
+# ------------------------------------------------------------------------------------ bricks
+def run_bricks(args, torch, cc, lib):
     from cudacompress_amd import synthetic as S
-
-    wl = lossless2d() if args.workload == "lossless2d" else float3d()
-    data = wl["data"]
-    raw_bytes = data.nbytes
     dev = torch.device("cuda:0")
-    src = torch.from_numpy(data).to(dev)
-    out = torch.empty_like(src)
-    if wl["name"] == "lossless2d":
-        y, x = wl["shape"]
-        inst = cc.createInstance(0, 3, (x // 2) * (y // 2), wl["cbs"], wl["log2"])
-
-        def compress():
-            return cc.compressVolumeLossless(inst, src, x, y, 1, wl["levels"])
-
-        def decompress(words):
-            cc.decompressVolumeLossless(inst, out, x, y, 1, wl["levels"], words)
-    else:
-        z, y, x = wl["shape"]
-        inst = cc.createInstance(0, 7, (x // 2) * (y // 2) * (z // 2), wl["cbs"], wl["log2"])
-
-        def compress():
-            return cc.compressVolumeFloat(inst, src, x, y, z, wl["levels"], wl["q0"])
-
-        def decompress(words):
-            cc.decompressVolumeFloat(inst, out, x, y, z, wl["levels"], words, wl["q0"])
+    bricks = S.brick_set(args.bricks)
+    shapes = [b.shape for b in bricks]
+    raw_bytes = int(sum(b.nbytes for b in bricks))
+    steps_q = [0.005 * float(b.std()) for b in bricks]
+    srcs = [torch.from_numpy(b).to(dev) for b in bricks]
+    dsts = [torch.empty_like(s) for s in srcs]
+    nvox = sum(b.size for b in bricks)
+    out = torch.empty(nvox + 4096 * len(bricks), dtype=torch.int32, device=dev)
+    inst = cc.createInstance(0, 7, 128 * 128 * 128 // 8, 128, 14)
+    state = {}
 
     def step():
-        w = compress()
-        decompress(w)
-        return w
+        offs, stat = cc.compressBricksFloat(inst, srcs, shapes, steps_q, 2, out)
+        cc.decompressBricksFloat(inst, dsts, shapes, steps_q, 2, out, offs)
+        state["offs"], state["stat"] = offs, stat
 
     if args.copy_calibration:
         a = torch.empty(256 << 20, dtype=torch.uint8, device=dev)
@@ -112,90 +120,183 @@ def main():
         torch.cuda.synchronize()
         del a, b
     for _ in range(args.warmup):
-        words = step()
+        step()
     torch.cuda.synchronize()
+    ms, kernels = timed(torch, lib, step, args.steps)
+    value = 2.0 * raw_bytes / (ms * 1e-3) / 1e9
+    offs, stat = state["offs"], state["stat"]
+    assert np.all(stat == 0), "batched compression failed for some bricks"
+    compressed = int(offs[-1]) * 4
 
-    lib = _capi.load()
-    lib.cc_profile_reset()
-    lib.cc_profile_enable(1)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        words = step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    lib.cc_profile_enable(0)
-    kernels = _capi.profile_entries()
-    lib.cc_profile_reset()
+    # quality: aggregate PSNR (range over all bricks / RMSE over all voxels) and the worst brick
+    se, vmin, vmax, worst = 0.0, np.inf, -np.inf, np.inf
+    for s, d in zip(srcs, dsts):
+        e = (s.double() - d.double())
+        se += float((e * e).sum())
+        vmin, vmax = min(vmin, float(s.min())), max(vmax, float(s.max()))
+        mse_b = float((e * e).mean())
+        rng_b = float(s.max() - s.min())
+        worst = min(worst, 20 * np.log10(rng_b / np.sqrt(mse_b)) if mse_b > 0 else np.inf)
+    psnr = 20 * np.log10((vmax - vmin) / np.sqrt(se / nvox))
 
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    value = 2.0 * raw_bytes / (ms_per_step * 1e-3) / 1e9
-
-    # exactness / quality of the last step
-    rec = out.cpu().numpy()
-    if wl["name"] == "lossless2d":
-        quality = {"lossless_exact": bool(np.array_equal(rec, data)), "psnr_db": "inf" if np.array_equal(rec, data) else S.psnr(data, rec)}
-    else:
-        quality = {"psnr_db": round(S.psnr(data, rec), 3)}
-    compressed_bytes = int(words.size * 4)
-
-    kernels.sort(key=lambda k: -k[2])
-    dom = kernels[0]
-    per_launch_ms = dom[2] / dom[1]
-    per_launch_bytes = dom[3] / dom[1]
-    achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
-    traffic = None
-    try:
-        with open(args.traffic_file) as f:
-            t = json.load(f)
-        key = f"{wl['name']}:{dom[0]}"
-        if key in t:
-            traffic = t[key]["hbm_bytes_per_launch"]
-    except (OSError, ValueError, KeyError):
-        traffic = None
-    roofline = {"bound": "hbm", "kernel": dom[0], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "algorithmic_bytes_per_launch": int(per_launch_bytes), "avg_launch_us": round(per_launch_ms * 1e3, 3),
-                "launches_per_step": dom[1] / args.steps}
-    kernel_table = [{"kernel": k[0], "launches_per_step": round(k[1] / args.steps, 2),
-                     "ms_per_step": round(k[2] / args.steps, 4), "GBps": round(k[3] / max(k[2], 1e-9) / 1e6, 1)}
-                    for k in kernels[:12]]
-    gpu_kernel_ms = sum(k[2] for k in kernels) / args.steps
+    # plain per-brick API on a sample, and bitstream identity with the batch
+    plain = None
+    equal = None
+    if not args.no_plain:
+        words = out[: int(offs[-1])].cpu().numpy().view(np.uint32)
+        k = min(args.plain_sample, len(bricks))
+        pin = cc.createInstance(0, 7, 128 * 128 * 128 // 8, 128, 14)
+        ref = torch.empty_like(srcs[0])
+        sample_bytes = sum(bricks[i].nbytes for i in range(k))
+        for i in range(min(4, k)):  # warm the plain path
+            z, y, x = shapes[i]
+            w = cc.compressVolumeFloat(pin, srcs[i], x, y, z, 2, steps_q[i])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        equal = True
+        plain_words = []
+        for i in range(k):
+            z, y, x = shapes[i]
+            w = cc.compressVolumeFloat(pin, srcs[i], x, y, z, 2, steps_q[i])
+            plain_words.append(w)
+            d = torch.empty_like(srcs[i])
+            cc.decompressVolumeFloat(pin, d, x, y, z, 2, w, steps_q[i])
+        torch.cuda.synchronize()
+        t_plain = time.perf_counter() - t0
+        for i in range(k):
+            if not np.array_equal(words[offs[i]:offs[i + 1]], plain_words[i]):
+                equal = False
+        plain = {"GBps": round(2.0 * sample_bytes / t_plain / 1e9, 4), "bricks": k, "seconds": round(t_plain, 3),
+                 "api": "compressVolumeFloat + decompressVolumeFloat per brick (host bitstreams)"}
+        pin.close()
+        del ref
 
     cpu = None
-    bitexact = None
+    if not args.no_cpu_baseline:
+        from oracle import pipeline as P
+        runs, t_total, done_bytes = 0, 0.0, 0
+        while t_total < 10.0 and runs < len(bricks):
+            b = bricks[runs]
+            t0 = time.perf_counter()
+            ow = P.compress_volume_float(b, 2, steps_q[runs])
+            P.decompress_volume_float(ow, b.shape, 2, steps_q[runs])
+            t_total += time.perf_counter() - t0
+            done_bytes += b.nbytes
+            runs += 1
+        cpu = {"value": round(2.0 * done_bytes / t_total / 1e9, 5), "unit": "GB/s", "cores": 1, "kind": "port",
+               "sample": f"the first {runs} bricks ({done_bytes} raw bytes) compressed and decompressed one by one "
+                         f"through the oracle (C++ restatement of the reference path, one host thread), {t_total:.2f} s"}
+
+    inst.close()
+    cfg = {"workload": "C4 batched: %d float32 bricks, axes from {16,32,48,64,96,128} (seeded), cut from seeded "
+                       "k^-5/3 fields; 2-level 3D CDF 9/7, deadzone step 0.005*std(brick), RLE+Huffman log2 14, "
+                       "coding block 128; one batched compress + one batched decompress per step" % len(bricks),
+           "bricks": len(bricks), "raw_bytes_per_direction": raw_bytes, "compressed_bytes": compressed,
+           "compression_ratio": round(raw_bytes / compressed, 3), "psnr_db": round(float(psnr), 3),
+           "worst_brick_psnr_db": round(float(worst), 3), "mode": "batched",
+           "plain": plain, "batched_vs_plain": round(value / plain["GBps"], 2) if plain else None,
+           "bitstreams_equal_plain_sample": equal}
+    return value, ms, raw_bytes, kernels, cfg, cpu, "f32"
+
+
+# ------------------------------------------------------------------------------------ plain volumes
+def run_plain(args, torch, cc, lib):
+    from cudacompress_amd import synthetic as S
+    dev = torch.device("cuda:0")
+    if args.workload == "lossless2d":
+        n = 4096
+        data = S.fbm_heightfield(n, 0.8, 2)
+        src = torch.from_numpy(data).to(dev)
+        out = torch.empty_like(src)
+        inst = cc.createInstance(0, 3, (n // 2) * (n // 2), 128, 16)
+
+        def compress():
+            return cc.compressVolumeLossless(inst, src, n, n, 1, 3)
+
+        def decompress(w):
+            cc.decompressVolumeLossless(inst, out, n, n, 1, 3, w)
+        desc = ("C2 lossless: 4096x4096 uint16 spectral fBm (H=0.8, seed 2) as int16, 3-level 2D CDF 5/3, "
+                "3 RLHuff highpass streams per level + Huffman lowpass, log2 16, coding block 128, plain per-call API")
+    else:
+        n = 128
+        data = S.gaussian_field((n, n, n), 1)
+        src = torch.from_numpy(data).to(dev)
+        out = torch.empty_like(src)
+        inst = cc.createInstance(0, 7, (n // 2) ** 3, 128, 14)
+
+        def compress():
+            return cc.compressVolumeFloat(inst, src, n, n, n, 2, 0.01)
+
+        def decompress(w):
+            cc.decompressVolumeFloat(inst, out, n, n, n, 2, w, 0.01)
+        desc = ("C1 float: 128^3 k^-5/3 Gaussian random field (seed 1), 2-level 3D CDF 9/7, deadzone step 0.01, "
+                "7 RLHuff streams per level + Huffman lowpass, log2 14, coding block 128, plain per-call API")
+    raw_bytes = data.nbytes
+    state = {}
+
+    def step():
+        w = compress()
+        decompress(w)
+        state["w"] = w
+
+    for _ in range(args.warmup):
+        step()
+    ms, kernels = timed(torch, lib, step, args.steps)
+    value = 2.0 * raw_bytes / (ms * 1e-3) / 1e9
+    words = state["w"]
+    rec = out.cpu().numpy()
+    cfg = {"workload": desc, "raw_bytes_per_direction": raw_bytes, "compressed_bytes": int(words.size * 4),
+           "compression_ratio": round(raw_bytes / (words.size * 4), 3), "mode": "plain"}
+    if args.workload == "lossless2d":
+        cfg["lossless_exact"] = bool(np.array_equal(rec, data))
+    else:
+        cfg["psnr_db"] = round(S.psnr(data, rec), 3)
+    cpu = None
     if not args.no_cpu_baseline:
         from oracle import pipeline as P
         runs, t_total = 0, 0.0
-        while runs == 0 or (t_total < 10.0 and runs < 20):  # bounded sample: ~10 s of CPU work
+        while runs == 0 or (t_total < 10.0 and runs < 20):
             t0 = time.perf_counter()
-            if wl["name"] == "lossless2d":
-                ow = P.compress_lossless(data, wl["levels"], log2=wl["log2"], cbs=wl["cbs"])
-                P.decompress_lossless(ow, data.shape, wl["levels"], log2=wl["log2"], cbs=wl["cbs"])
+            if args.workload == "lossless2d":
+                ow = P.compress_lossless(data, 3, log2=16, cbs=128)
+                P.decompress_lossless(ow, data.shape, 3, log2=16, cbs=128)
             else:
-                ow = P.compress_volume_float(data, wl["levels"], wl["q0"], log2=wl["log2"], cbs=wl["cbs"])
-                P.decompress_volume_float(ow, data.shape, wl["levels"], wl["q0"], log2=wl["log2"], cbs=wl["cbs"])
+                ow = P.compress_volume_float(data, 2, 0.01)
+                P.decompress_volume_float(ow, data.shape, 2, 0.01)
             t_total += time.perf_counter() - t0
             runs += 1
-        tc = t_total / runs
-        bitexact = bool(np.array_equal(ow, words))
-        cpu = {"value": round(2.0 * raw_bytes / tc / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
-               "sample": f"{runs} full compress+decompress runs of the same {wl['name']} input ({raw_bytes} raw "
-                         f"bytes each way) through the oracle (C++ restatement, one thread), {tc:.3f} s per run"}
+        cfg["bitstream_equals_oracle"] = bool(np.array_equal(ow, words))
+        cpu = {"value": round(2.0 * raw_bytes * runs / t_total / 1e9, 5), "unit": "GB/s", "cores": 1, "kind": "port",
+               "sample": f"{runs} full compress+decompress runs of the same input through the oracle "
+                         f"(C++ restatement, one thread), {t_total / runs:.3f} s per run"}
+    inst.close()
+    return value, ms, raw_bytes, kernels, cfg, cpu, "int16" if args.workload == "lossless2d" else "f32"
 
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (requested --gpus {args.gpus}); "
+              "bricks would shard by brick id with no exchange on the data path", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import cudacompress_amd as cc
+    from cudacompress_amd import _capi
+    lib = _capi.load()
+    if args.workload == "bricks":
+        value, ms, raw, kernels, cfg, cpu, dtype = run_bricks(args, torch, cc, lib)
+    else:
+        value, ms, raw, kernels, cfg, cpu, dtype = run_plain(args, torch, cc, lib)
+    roof, table, gpu_ms = roofline_of(kernels, args.steps, args.workload, args.traffic_file)
+    cfg["gpu_kernel_ms_per_step"] = round(gpu_ms, 4)
     res = {
         "metric": METRIC, "value": round(value, 3), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "int16" if wl["name"] == "lossless2d" else "f32", "data": "synthetic",
-        "config": {"workload": wl["desc"], "raw_bytes_per_direction": raw_bytes, "compressed_bytes": compressed_bytes,
-                   "compression_ratio": round(raw_bytes / compressed_bytes, 3), "mode": "plain",
-                   "bitstream_equals_oracle": bitexact, **quality, "gpu_kernel_ms_per_step": round(gpu_kernel_ms, 4)},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "kernels": kernel_table,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": cfg, "roofline": roof,
+        "cpu_baseline": cpu, "kernels": table,
     }
     print(json.dumps(res))
-    inst.close()
 
 
 if __name__ == "__main__":

@@ -10,6 +10,7 @@ _capi.load()
 
 from .api import (  # noqa: E402
     BitStream,
+    BricksPlan,
     CCError,
     CC_ERR_CODEWORD_TOO_LONG,
     CC_ERR_CORRUPT_STREAM,
@@ -23,6 +24,7 @@ from .api import (  # noqa: E402
     Instance,
     QUANTIZE_DEADZONE,
     QUANTIZE_UNIFORM,
+    Server,
     compressBricksFloat,
     compressVolumeFloat,
     compressVolumeLossless,

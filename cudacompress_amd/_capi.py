@@ -60,6 +60,18 @@ SIGNATURES = {
     "cc_decompress_volume_lossless": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, c_vp, c_u64]),
     "cc_compress_bricks_float": (c_int, [c_vp, c_vp, c_u32, c_u32, c_int, c_vp, c_u64, P_u64, P_u32]),
     "cc_decompress_bricks_float": (c_int, [c_vp, c_vp, c_u32, c_u32, c_int, c_vp, P_u64, P_u32]),
+    "cc_bricks_plan_create": (c_int, [c_vp, c_vp, c_u32, c_u32, c_int, c_vp, c_u64, ctypes.POINTER(c_vp)]),
+    "cc_bricks_plan_compress": (c_int, [c_vp, P_u64, P_u32]),
+    "cc_bricks_plan_decompress": (c_int, [c_vp, P_u32]),
+    "cc_bricks_plan_destroy": (None, [c_vp]),
+    "cc_server_create": (c_int, [c_vp, c_vp, c_u32, P_u64, P_u32, ctypes.POINTER(c_float), c_u32, c_int, c_u32, c_u64,
+                                 ctypes.POINTER(c_vp)]),
+    "cc_server_submit": (c_int, [c_vp, c_u64, P_u32, c_u32, ctypes.POINTER(c_vp), ctypes.c_double]),
+    "cc_server_poll": (c_int, [c_vp, ctypes.c_double, P_u32]),
+    "cc_server_pending": (c_u32, [c_vp]),
+    "cc_server_completed": (c_u32, [c_vp, P_u64, ctypes.POINTER(ctypes.c_double), c_u32]),
+    "cc_server_failed": (c_u32, [c_vp, P_u64, P_u32, c_u32]),
+    "cc_server_destroy": (None, [c_vp]),
     "cc_profile_enable": (None, [c_int]),
     "cc_profile_reset": (None, []),
     "cc_profile_collect": (c_u32, []),

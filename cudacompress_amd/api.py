@@ -413,6 +413,109 @@ def decompressBricksFloat(inst, dests, shapes, quantSteps, numLevels, words, off
     return stat[:n]
 
 
+class BricksPlan:
+    """A brick batch captured into hipGraphs (cc_bricks_plan_*): compress() and decompress()
+    replay the captured pipelines on the same bricks and output buffer."""
+
+    def __init__(self, inst, sources, dests, shapes, quantSteps, numLevels, out, symbolSize=2):
+        self._lib = _capi.load()
+        self._n = len(shapes)
+        self._arr = _bricks(sources, dests, shapes, quantSteps)
+        self._keep = (sources, dests, out)
+        h = ctypes.c_void_p()
+        cap = out.numel() * out.element_size() // 4
+        _check(self._lib.cc_bricks_plan_create(inst.handle, self._arr, self._n, numLevels, symbolSize, _ptr(out), cap,
+                                               ctypes.byref(h)), "cc_bricks_plan_create")
+        self.handle = h
+
+    def compress(self):
+        offs = np.zeros(self._n + 1, dtype=np.uint64)
+        stat = np.zeros(max(self._n, 1), dtype=np.uint32)
+        st = self._lib.cc_bricks_plan_compress(self.handle, offs.ctypes.data_as(_capi.P_u64),
+                                               stat.ctypes.data_as(_capi.P_u32))
+        if st not in (CC_OK, CC_ERR_SYMBOL_OVERFLOW, CC_ERR_CODEWORD_TOO_LONG):
+            _check(st, "cc_bricks_plan_compress")
+        return offs, stat[:self._n]
+
+    def decompress(self):
+        stat = np.zeros(max(self._n, 1), dtype=np.uint32)
+        _check(self._lib.cc_bricks_plan_decompress(self.handle, stat.ctypes.data_as(_capi.P_u32)),
+               "cc_bricks_plan_decompress")
+        return stat[:self._n]
+
+    def close(self):
+        if self.handle:
+            self._lib.cc_bricks_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Decode serving loop over resident compressed bricks (cc_server_*).
+
+    words: device word tensor holding the bricks' bitstreams; ranges: (n, 2) word ranges;
+    shapes: (z, y, x) per brick; steps: quantization step per brick."""
+
+    def __init__(self, inst, words, ranges, shapes, steps, numLevels, slots=2, workspaceBytes=32 << 30,
+                 symbolSize=2):
+        self._lib = _capi.load()
+        self._words = words
+        n = len(shapes)
+        rng = np.ascontiguousarray(np.asarray(ranges, dtype=np.uint64).reshape(n, 2))
+        dims = np.ascontiguousarray(np.array([(x, y, z) for (z, y, x) in shapes], dtype=np.uint32).reshape(n, 3))
+        st = np.ascontiguousarray(np.asarray(steps, dtype=np.float32))
+        h = ctypes.c_void_p()
+        _check(self._lib.cc_server_create(inst.handle, _ptr(words), n, rng.ctypes.data_as(_capi.P_u64),
+                                          dims.ctypes.data_as(_capi.P_u32),
+                                          st.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), numLevels, symbolSize,
+                                          slots, workspaceBytes, ctypes.byref(h)), "cc_server_create")
+        self.handle = h
+
+    def submit(self, request_id, brick_ids, outputs, arrival):
+        ids = np.ascontiguousarray(brick_ids, dtype=np.uint32)
+        outs = (ctypes.c_void_p * len(outputs))(*[_ptr(o) for o in outputs])
+        _check(self._lib.cc_server_submit(self.handle, request_id, ids.ctypes.data_as(_capi.P_u32), ids.size, outs,
+                                          float(arrival)), "cc_server_submit")
+
+    def poll(self, now):
+        n = ctypes.c_uint32()
+        _check(self._lib.cc_server_poll(self.handle, float(now), ctypes.byref(n)), "cc_server_poll")
+        return n.value
+
+    def pending(self):
+        return self._lib.cc_server_pending(self.handle)
+
+    def completed(self, maxn=65536):
+        ids = np.zeros(maxn, dtype=np.uint64)
+        t = np.zeros(maxn, dtype=np.float64)
+        k = self._lib.cc_server_completed(self.handle, ids.ctypes.data_as(_capi.P_u64),
+                                          t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), maxn)
+        return ids[:k], t[:k]
+
+    def failed(self, maxn=65536):
+        ids = np.zeros(maxn, dtype=np.uint64)
+        stt = np.zeros(maxn, dtype=np.uint32)
+        k = self._lib.cc_server_failed(self.handle, ids.ctypes.data_as(_capi.P_u64), stt.ctypes.data_as(_capi.P_u32),
+                                       maxn)
+        return ids[:k], stt[:k]
+
+    def close(self):
+        if self.handle:
+            self._lib.cc_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def decodeRLHuff(inst, bitStream, symbolStreams, symbolCountPerStream):
     """decodeRLHuff (Encode.h:29, :43).  Outputs need not be zeroed."""
     return _decode("cc_decode_rlhuff", inst, bitStream, symbolStreams, symbolCountPerStream)

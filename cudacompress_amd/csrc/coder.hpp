@@ -111,6 +111,7 @@ struct DecSeq {
     uint64_t words_off;     // codewords (word offset from base)
     uint64_t incs_off;      // u16 increments (word offset from base)
     uint64_t boff_off;      // arena: absolute block offsets
+    uint64_t lut_off;       // arena: 1024-entry lookup table (len << 24 | symbol), then the symbol table
     void* out;              // decoded values (arena for RLE sequences, the item output otherwise)
     uint32_t nlen, nsym, sb, count;
     uint32_t nwords_cw;     // codeword words
@@ -127,15 +128,25 @@ struct DecCallOut {
 };
 
 // Launchers (hip/coder.hip).  scratch: arena of `scratch_cap` words.
+// Side streams for launches that may run concurrently (the table-design classes): `side` are three
+// non-blocking streams, `fork` / `join` events owned by the caller; all null -> everything on st.
+struct SideStreams {
+    hipStream_t s[3];
+    hipEvent_t fork;
+    hipEvent_t join[3];
+};
 void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uint32_t nseq, const CoderParams& p,
                   int symSize, uint32_t* scratch, uint64_t scratch_cap, CoderState* d_state, CallOut* d_calls,
                   uint32_t ncalls, const uint32_t* d_call_first_item, uint32_t* d_out, uint64_t out_cap,
-                  uint64_t total_symbols, hipStream_t st);
+                  uint64_t total_symbols, hipStream_t st, const SideStreams* side);
 // wave_seq: for every decode wave the coded sequence it serves (upper bound of the block count:
 // RLE sequences have at most n pairs); wave_first: that wave's first block.
 void coder_decode(const DecCall* d_calls, uint32_t ncalls, const DecItem* d_items, uint32_t nitems, DecSeq* d_seqs,
                   uint32_t nseq, const CoderParams& p, int symSize, uint32_t* scratch, uint64_t scratch_cap,
                   CoderState* d_state, DecCallOut* d_out, const uint32_t* d_wave_seq, const uint32_t* d_wave_first,
                   uint32_t nwaves, uint64_t total_symbols, hipStream_t st);
+// calls[i].words = base + enc[i].word_begin, calls[i].nwords = enc[i].words (the rest unchanged)
+void coder_calls_from_encode(DecCall* d_calls, const CallOut* d_enc, const uint32_t* base, uint32_t ncalls,
+                             hipStream_t st);
 
 }  // namespace cca

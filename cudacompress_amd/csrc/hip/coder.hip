@@ -23,7 +23,6 @@ using design::HeapEntry;
 using design::TableOut;
 using design::Work;
 
-constexpr uint32_t kDesignSmall = 512;   // design classes by distinct-symbol count
 constexpr uint32_t kDesignMid = 4096;
 constexpr uint32_t kWindowWords = kChunk + 4;  // LDS packing window (2048 pairs x 31 bits < 2048 words)
 
@@ -192,22 +191,46 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_stats(const CodeItem* it
     __syncthreads();
     long long carry = -1;
     uint32_t cnt = 0, vmax = 0, wide = 0;
+    // histogram updates are coalesced per thread: runs of one symbol and all zero counts of zero
+    // are counted in registers (zero runs dominate the zero-count histogram)
+    uint32_t run_sym = 0xFFFFFFFFu, run_len = 0, zc0 = 0;
+    auto flush_run = [&]() {
+        if (run_len) {
+            if (run_sym < kStatBins) atomicAdd(&hs[run_sym], run_len);
+            else wide = 1;
+        }
+    };
+    uint32_t vn[kPerThread];
+    load8(in, (uint64_t)threadIdx.x * kPerThread, n, vn);
     for (uint64_t base = 0; base < n; base += kChunk) {
         const uint64_t p0 = base + (uint64_t)threadIdx.x * kPerThread;
         uint32_t v[kPerThread];
-        load8(in, p0, n, v);
+#pragma unroll
+        for (int j = 0; j < (int)kPerThread; j++) v[j] = vn[j];
+        if (base + kChunk < n) load8(in, p0 + kChunk, n, vn);  // next step's symbols in flight
         long long chunk_max;
         long long prev = wg_excl_max_i64(thread_last_nz(v, p0, n), shl, chunk_max);
         prev = prev > carry ? prev : carry;
         thread_pairs(v, p0, n, rl, prev, (long long)n, [&](uint32_t s, uint32_t z) {
             cnt++;
             vmax = max(vmax, s);
-            if (s < kStatBins) atomicAdd(&hs[s], 1u);
-            else wide = 1;
-            if (rl) atomicAdd(&hz[z], 1u);
+            if (s == run_sym) {
+                run_len++;
+            } else {
+                flush_run();
+                run_sym = s;
+                run_len = 1;
+            }
+            if (rl) {
+                if (z == 0) zc0++;
+                else atomicAdd(&hz[z], 1u);
+            }
         });
         carry = chunk_max > carry ? chunk_max : carry;
     }
+    flush_run();
+    zc0 = wave_sum(zc0);
+    if (rl && (threadIdx.x & 63) == 0 && zc0) atomicAdd(&hz[0], zc0);
     uint32_t M = wg_sum_u32(cnt, shu);
     wide = wg_max_u32(wide, shu);
     vmax = wg_max_u32(vmax, shu);
@@ -328,6 +351,78 @@ struct WaveEx {
     __device__ int lane() const { return (int)threadIdx.x; }
     __device__ int lanes() const { return 64; }
     __device__ void barrier() const { __syncthreads(); }
+
+    // make_by_levels: one lane per node of a level, levels bottom-up
+    __device__ void heap_make(HeapEntry* h, int len) const {
+        if (len < 2) return;
+        const int last = (len - 2) / 2;
+        int d = 0;
+        while ((1 << (d + 1)) - 1 <= last) d++;
+        for (; d >= 0; d--) {
+            const int lo = (1 << d) - 1, hi = min((1 << (d + 1)) - 2, last);
+            for (int x = lo + lane(); x <= hi; x += 64) design::adjust_heap(h, x, len, h[x], design::QueueLess());
+            __syncthreads();
+        }
+    }
+    // pop_by_path: the descent is resolved six levels per LDS round (lane j evaluates node j of
+    // the 63-node subtree below the current path node); lane k then holds path node k.
+    __device__ void heap_pop(HeapEntry* h, int len) const {
+        if (len <= 1) return;
+        const HeapEntry v = h[len - 1];
+        const int n = len - 1, ln = lane();
+        const int lim = (n - 1) / 2, lastint = (n - 2) / 2;
+        const bool even = (n & 1) == 0;
+        const int sd = ln < 63 ? 31 - __clz(ln + 1) : 0;  // depth of this lane's node in the subtree
+        const int so = ln + 1 - (1 << sd);
+        int mypath = 0, m = 0, cur = 0;
+        for (;;) {
+            bool two = false, right = false, only = false;
+            if (ln < 63) {
+                const int g = ((cur + 1) << sd) - 1 + so;
+                two = g < lim;
+                if (two) right = !(h[2 * g + 1].count < h[2 * g + 2].count);
+                only = even && g == lastint;
+            }
+            const uint64_t mt = __ballot(two), mr = __ballot(right), mo = __ballot(only);
+            int j = 0, gnode = cur;
+            bool done = false;
+            for (int d = 0; d < 6; d++) {
+                if ((mt >> j) & 1ull) {
+                    const int r = (int)((mr >> j) & 1ull);
+                    gnode = 2 * gnode + 1 + r;
+                    j = 2 * j + 1 + r;
+                    m++;
+                    if (ln == m) mypath = gnode;
+                } else {
+                    if ((mo >> j) & 1ull) {
+                        gnode = 2 * gnode + 1;
+                        m++;
+                        if (ln == m) mypath = gnode;
+                    }
+                    done = true;
+                    break;
+                }
+            }
+            if (done) break;
+            cur = gnode;
+        }
+        const HeapEntry pv = ln <= m ? h[mypath] : HeapEntry{0u, 0u};
+        const int is = m - (int)__popcll(__ballot(ln >= 1 && ln <= m && v.count < pv.count));
+        const uint32_t nc = __shfl_down(pv.count, 1, 64), nn = __shfl_down(pv.node, 1, 64);
+        if (ln < is) h[mypath] = HeapEntry{nc, nn};
+        else if (ln == is) h[mypath] = v;
+    }
+    // push_by_chain: lane t holds ancestor t of the new position
+    __device__ void heap_push(HeapEntry* h, int pos, HeapEntry v) const {
+        const int ln = lane();
+        const int T = 31 - __clz(pos + 1);
+        const int c = ((pos + 1) >> ln) - 1;
+        const bool anc = ln >= 1 && ln <= T;
+        const HeapEntry e = anc ? h[c] : HeapEntry{0u, 0u};
+        const int j = (int)__popcll(__ballot(anc && v.count < e.count));
+        if (ln >= 1 && ln <= j) h[((pos + 1) >> (ln - 1)) - 1] = e;
+        if (ln == j) h[c] = v;
+    }
     template <class P, class Sk>
     __device__ int compact(int lo, int hi, P pred, Sk sink) const {
         int total = 0;
@@ -372,10 +467,11 @@ struct WaveEx {
                 rem &= ~m;
             }
         }
+        __shared__ uint32_t fi[32], fc[32];
         if (lane() < 32) cpl[lane()] = lane() < 31 ? mycount : 0u;
         __syncthreads();
-        uint32_t fi[32], fc[32];
-        design::first_codes(cpl, 31, fc, fi);
+        if (lane() == 0) design::first_codes(cpl, 31, fc, fi);
+        __syncthreads();
         uint32_t seen = 0;  // lane d-1: elements of depth d seen so far
         for (int base = 0; base < K; base += 64) {
             const int i = base + lane();
@@ -509,10 +605,14 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_pack(const CodeItem* ite
     long long carry = -1;
     uint64_t pairs = 0, cbits_s = 0, cbits_z = 0;
     const uint32_t cbs = p.cbs;
+    uint32_t vn[kPerThread];
+    load8(in, (uint64_t)threadIdx.x * kPerThread, n, vn);
     for (uint64_t base = 0; base < n && pairs < M; base += kChunk) {
         const uint64_t p0 = base + (uint64_t)threadIdx.x * kPerThread;
         uint32_t v[kPerThread];
-        load8(in, p0, n, v);
+#pragma unroll
+        for (int j = 0; j < (int)kPerThread; j++) v[j] = vn[j];
+        if (base + kChunk < n) load8(in, p0 + kChunk, n, vn);  // next step's symbols in flight
         long long chunk_max;
         long long prev = wg_excl_max_i64(thread_last_nz(v, p0, n), shl, chunk_max);
         prev = prev > carry ? prev : carry;
@@ -733,7 +833,7 @@ __global__ __launch_bounds__(kCoderThreads) void k_gather(const CodeItem* items,
 void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uint32_t nseq, const CoderParams& p,
                   int symSize, uint32_t* scratch, uint64_t scratch_cap, CoderState* d_state, CallOut* d_calls,
                   uint32_t ncalls, const uint32_t* d_call_first_item, uint32_t* d_out, uint64_t out_cap,
-                  uint64_t total_symbols, hipStream_t st) {
+                  uint64_t total_symbols, hipStream_t st, const SideStreams* side) {
     CCA_REQUIRE(symSize == 2 || symSize == 4, CC_ERR_INVALID_ARG, "symbol size must be 2 or 4");
     uint64_t* item_off = reinterpret_cast<uint64_t*>(scratch + scratch_cap);  // caller reserves nitems+1 u64
     CCA_LAUNCH("coder_init", 32, k_coder_init, dim3(1), dim3(64), 0, st, d_state);
@@ -754,19 +854,34 @@ void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uin
                    d_items, d_seqs, p, scratch);
     }
     CCA_LAUNCH_CHECK();
-    const size_t lds_small = design::heap_region_bytes<uint16_t>(kDesignSmall) + 2 * (2 * kDesignSmall + 2) + 16 +
-                             12 * design::kStackRanges + 16;
-    const size_t lds_mid = design::heap_region_bytes<uint16_t>(kDesignMid) + 2 * (2 * kDesignMid + 2) + 16 +
+    // Design classes by distinct-symbol count (small LDS footprints keep many tables resident).
+    // The classes touch disjoint sequences, so they run concurrently on side streams when given.
+    const uint32_t classes[3] = {256, 1024, kDesignMid};
+    const char* names[3] = {"huff_design_k256", "huff_design_k1024", "huff_design_k4096"};
+    if (side) {
+        CCA_HIP(hipEventRecord(side->fork, st));
+        for (int c = 0; c < 3; c++) CCA_HIP(hipStreamWaitEvent(side->s[c], side->fork, 0));
+    }
+    uint32_t kmin = 0;
+    for (int c = 0; c < 3; c++) {
+        const uint32_t kmax = classes[c];
+        const size_t lds = design::heap_region_bytes<uint16_t>(kmax) + 2 * (2 * kmax + 2) + 16 +
                            12 * design::kStackRanges + 16;
-    CCA_LAUNCH("huff_design", 0, k_design<uint16_t>, dim3(nseq), dim3(64), lds_small, st, d_seqs, p, scratch,
-               scratch_cap, d_state, 0u, kDesignSmall, 0);
+        hipStream_t sc = side ? side->s[c] : st;
+        CCA_LAUNCH(names[c], 0, k_design<uint16_t>, dim3(nseq), dim3(64), lds, sc, d_seqs, p, scratch, scratch_cap,
+                   d_state, kmin, kmax, 0);
+        CCA_LAUNCH_CHECK();
+        kmin = kmax + 1;
+    }
+    CCA_LAUNCH("huff_design_global", 0, k_design<uint32_t>, dim3(nseq), dim3(64), 0, st, d_seqs, p, scratch,
+               scratch_cap, d_state, kDesignMid + 1, 0xFFFFFFFFu, 1);
     CCA_LAUNCH_CHECK();
-    CCA_LAUNCH("huff_design", 0, k_design<uint16_t>, dim3(nseq), dim3(64), lds_mid, st, d_seqs, p, scratch,
-               scratch_cap, d_state, kDesignSmall + 1, kDesignMid, 0);
-    CCA_LAUNCH_CHECK();
-    CCA_LAUNCH("huff_design", 0, k_design<uint32_t>, dim3(nseq), dim3(64), 0, st, d_seqs, p, scratch, scratch_cap,
-               d_state, kDesignMid + 1, 0xFFFFFFFFu, 1);
-    CCA_LAUNCH_CHECK();
+    if (side) {
+        for (int c = 0; c < 3; c++) {
+            CCA_HIP(hipEventRecord(side->join[c], side->s[c]));
+            CCA_HIP(hipStreamWaitEvent(st, side->join[c], 0));
+        }
+    }
     if (symSize == 2)
         CCA_LAUNCH("code_pack", inb, k_code_pack<uint16_t>, dim3(nitems), dim3(kCoderThreads), 0, st, d_items, d_seqs,
                    p, scratch);

@@ -82,7 +82,7 @@ __global__ void k_dec_parse(const DecCall* calls, uint32_t ncalls, const DecItem
             const uint64_t sym_bit = hb + 5 + (uint64_t)cb * (nlen + 1) + sbf;
             const uint64_t tb = 5 + (uint64_t)cb * (nlen + 1) + sbf + (uint64_t)sb * nsym;
             w = (hb + tb + 31) / 32;
-            if (!ok || sb > 32 || w >= C.nwords || (count > 0 && nsym == 0)) { err = kDecCorrupt; break; }
+            if (!ok || sb > 32 || w >= C.nwords || (count > 0 && nsym == 0) || nsym > count) { err = kDecCorrupt; break; }
             const uint32_t bits = C.words[w++];
             const uint64_t words_off = w;
             const uint64_t nwc = ((uint64_t)bits + 31) / 32;
@@ -93,6 +93,7 @@ __global__ void k_dec_parse(const DecCall* calls, uint32_t ncalls, const DecItem
             if (w > C.nwords) { err = kDecCorrupt; break; }
             bool aok = true;
             const uint64_t boff = arena_take(st, nblk ? nblk : 1, cap, aok);
+            const uint64_t lut = arena_take(st, (1u << kLut) + (uint64_t)nsym + 1, cap, aok);
             void* out = it.out;
             if (it.rl) {
                 const uint64_t o = arena_take(st, ((uint64_t)count * symSize + 3) / 4 + 1, cap, aok);
@@ -106,6 +107,7 @@ __global__ void k_dec_parse(const DecCall* calls, uint32_t ncalls, const DecItem
             s.words_off = words_off;
             s.incs_off = incs_off;
             s.boff_off = boff;
+            s.lut_off = lut;
             s.out = out;
             s.nlen = nlen;
             s.nsym = nsym;
@@ -152,18 +154,50 @@ __global__ __launch_bounds__(256) void k_dec_tables(DecSeq* seqs, CoderParams p,
         if (!ok) atomicOr(&bad, 1u);
     }
     __syncthreads();
+    __shared__ int tab_sh[96];
     if (threadIdx.x == 0) {
         uint32_t fc[32], fi[32];
         design::first_codes(cpl, (int)s.nlen, fc, fi);
         uint64_t total = 0;
         for (uint32_t l = 0; l < 32; l++) {
             const bool used = l < s.nlen && cpl[l] > 0;
-            s.tab[l] = used ? (int32_t)fi[l] : -1;
-            s.tab[32 + l] = used ? (int32_t)fc[l] : -1;
-            s.tab[64 + l] = used ? (int32_t)(fc[l] + cpl[l] - 1) : -1;
+            tab_sh[l] = used ? (int32_t)fi[l] : -1;
+            tab_sh[32 + l] = used ? (int32_t)fc[l] : -1;
+            tab_sh[64 + l] = used ? (int32_t)(fc[l] + cpl[l] - 1) : -1;
             if (l < s.nlen) total += cpl[l];
         }
         if (s.nlen > 0 && total != s.nsym) bad = 1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 96) s.tab[threadIdx.x] = tab_sh[threadIdx.x];
+    // symbol table (codeword-index order) and the lookup table for codes of up to kLut bits:
+    // canonical codes of length <= kLut belong to the first 2^kLut symbols
+    __shared__ uint32_t sym_lo[1 << kLut];
+    __shared__ int tb[96];
+    __syncthreads();
+    uint32_t* gsyms = scratch + s.lut_off + (1u << kLut);
+    uint32_t* glut = scratch + s.lut_off;
+    for (uint32_t k = threadIdx.x; k < s.nsym; k += 256) {
+        bool ok = true;
+        const uint32_t v = rd_bits(s.base, s.nwords, s.sym_bit + (uint64_t)s.sb * k, s.sb, ok);
+        if (!ok) atomicOr(&bad, 1u);
+        gsyms[k] = v;
+        if (k < (1u << kLut)) sym_lo[k] = v;
+    }
+    for (int i = threadIdx.x; i < 96; i += 256) tb[i] = tab_sh[i];
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < (1u << kLut); e += 256) {
+        uint32_t entry = 0;
+        for (int l = 1; l <= (int)kLut && l <= (int)s.nlen; l++) {
+            const int cw = (int)(e >> (kLut - l));
+            if (cw <= tb[64 + l - 1]) {
+                const int idx = tb[l - 1] + cw - tb[32 + l - 1];
+                const uint32_t sym = (idx >= 0 && (uint32_t)idx < s.nsym && idx < (1 << kLut)) ? sym_lo[idx] : 0u;
+                entry = ((uint32_t)l << 24) | (sym & 0xFFFFFFu);
+                break;
+            }
+        }
+        glut[e] = entry;
     }
     // absolute block offsets: running sum of the u16 increments
     const uint32_t nblk = (s.count + p.cbs - 1) / p.cbs;
@@ -215,26 +249,13 @@ __global__ __launch_bounds__(64) void k_dec_blocks(const DecSeq* seqs, const uin
     const int lane = threadIdx.x;
     S* out = static_cast<S*>(s.out);
     const bool multi = s.nlen > 0;
-    const uint32_t nsym = s.nsym, sb = s.sb;
-    auto symbol_at = [&](int idx) -> uint32_t {  // symbol field idx of the header
-        if (idx < 0 || (uint32_t)idx >= nsym) return 0u;
-        bool ok = true;
-        return rd_bits(s.base, s.nwords, s.sym_bit + (uint64_t)sb * (uint32_t)idx, sb, ok);
-    };
+    const uint32_t nsym = s.nsym;
+    const uint32_t* gsyms = scratch + s.lut_off + (1u << kLut);
+    auto symbol_at = [&](int idx) -> uint32_t { return (idx >= 0 && (uint32_t)idx < nsym) ? gsyms[idx] : 0u; };
     if (multi) {
+        const uint32_t* glut = scratch + s.lut_off;
         for (int i = lane; i < 96; i += 64) tab[i] = s.tab[i];
-        __syncthreads();
-        for (uint32_t e = lane; e < (1u << kLut); e += 64) {
-            uint32_t entry = 0;
-            for (int l = 1; l <= (int)kLut && l <= (int)s.nlen; l++) {
-                const int cw = (int)(e >> (kLut - l));
-                if (cw <= tab[64 + l - 1]) {
-                    entry = ((uint32_t)l << 24) | (symbol_at(tab[l - 1] + cw - tab[32 + l - 1]) & 0xFFFFFFu);
-                    break;
-                }
-            }
-            lut[e] = entry;
-        }
+        for (uint32_t e = lane; e < (1u << kLut); e += 64) lut[e] = glut[e];
         __syncthreads();
     }
     const uint32_t single = nsym >= 1 ? symbol_at(0) : 0u;
@@ -359,7 +380,22 @@ __global__ void k_dec_finish(const DecCall* calls, uint32_t ncalls, const DecIte
     }
 }
 
+__global__ void k_calls_from_encode(DecCall* calls, const CallOut* enc, const uint32_t* base, uint32_t n) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    calls[c].words = base + enc[c].word_begin;
+    calls[c].nwords = enc[c].err ? 0 : enc[c].words;
+}
+
 }  // namespace
+
+void coder_calls_from_encode(DecCall* d_calls, const CallOut* d_enc, const uint32_t* base, uint32_t ncalls,
+                             hipStream_t st) {
+    if (ncalls == 0) return;
+    CCA_LAUNCH("dec_calls_from_encode", 0, k_calls_from_encode, dim3(div_up(ncalls, 64)), dim3(64), 0, st, d_calls,
+               d_enc, base, ncalls);
+    CCA_LAUNCH_CHECK();
+}
 
 void coder_decode(const DecCall* d_calls, uint32_t ncalls, const DecItem* d_items, uint32_t nitems, DecSeq* d_seqs,
                   uint32_t nseq, const CoderParams& p, int symSize, uint32_t* scratch, uint64_t scratch_cap,

@@ -16,7 +16,8 @@
 // test compares it against the standard library on tie-heavy inputs).  Parallel steps are
 // written against an executor with a host (serial) and a device (wave64) implementation:
 //  * leaves are collected in symbol order;
-//  * the merge loop runs on one lane (the heap is inherently sequential);
+//  * the merge loop's heap operations are restated over their descent paths and ancestor
+//    chains (pop_by_path, push_by_chain, make_by_levels), which the wave evaluates in parallel;
 //  * leaf depths are found by walking parent links, one leaf per lane;
 //  * sort: every partition is computed by the whole wave from the closed form of the unguarded
 //    partition (partition_cut); the final insertion sort equals a stable sort by depth (every
@@ -236,7 +237,8 @@ struct Work {
     CCA_HD Idx* ge(int K) const { return reinterpret_cast<Idx*>(elems() + K); }
     CCA_HD Idx* le(int K) const { return ge(K) + K; }
 };
-constexpr int kStackRanges = 128;
+// pending ranges never exceed 2*floor(log2 n) + 2 <= 50 for n < 2^24 (their depths strictly decrease)
+constexpr int kStackRanges = 64;
 
 // bytes of the heap region needed for K leaves with index type Idx
 template <class Idx>
@@ -344,28 +346,31 @@ CCA_HD void design_table(Ex& ex, const uint32_t* hist, uint32_t nbins, int diale
         }
         return;
     }
-    // (2) merge loop, one lane
-    if (lane == 0) {
+    // (2) merge loop (every lane runs it; the executor parallelises each heap operation)
+    {
         if (dialect == kDialectGpu) {
-            make_heap(w.heap, K, QueueLess());
+            ex.heap_make(w.heap, K);
         } else {
-            for (int i = 2; i <= K; i++) push_heap(w.heap, i, QueueLess());
+            for (int i = 1; i < K; i++) ex.heap_push(w.heap, i, w.heap[i]);
         }
         int size = K, next = K;
         while (size > 1) {
             const HeapEntry a = w.heap[0];
-            pop_heap(w.heap, size--, QueueLess());
+            ex.heap_pop(w.heap, size--);
             const HeapEntry b = w.heap[0];
-            pop_heap(w.heap, size--, QueueLess());
-            w.parent[a.node] = (Idx)next;
-            w.parent[b.node] = (Idx)next;
-            w.heap[size++] = HeapEntry{a.count + b.count, (uint32_t)next};
-            push_heap(w.heap, size, QueueLess());
+            ex.heap_pop(w.heap, size--);
+            if (lane == 0) {
+                w.parent[a.node] = (Idx)next;
+                w.parent[b.node] = (Idx)next;
+            }
+            ex.heap_push(w.heap, size++, HeapEntry{a.count + b.count, (uint32_t)next});
             next++;
         }
-        ctl->root = next - 1;
-        ctl->deepest = 0;
-        ctl->err = 0;
+        if (lane == 0) {
+            ctl->root = next - 1;
+            ctl->deepest = 0;
+            ctl->err = 0;
+        }
     }
     ex.barrier();
     // (3) leaf depths, packed with the symbol into the (now free) heap region
@@ -418,11 +423,76 @@ CCA_HD void design_table(Ex& ex, const uint32_t* hist, uint32_t nbins, int diale
     }
 }
 
+// ---------------------------------------------------------------- heap operations by path
+// The same element moves as pop_heap / push_heap / make_heap, restated so that each operation
+// becomes a few data-parallel steps (the wave executor's form):
+//  * pop: the hole left at the root descends along the path on which every node continues to
+//    its left child iff that child's count is strictly smaller (ties go right), down to the last
+//    node with two children, plus the lone left child of the last internal node when the heap
+//    size is even; counts along that path are non-decreasing, so the former last element v
+//    lands at path position i* = m - #{k in 1..m : v.count < count(p_k)} and p_0..p_{i*-1} take
+//    the values of p_1..p_{i*}.
+//  * push: the ancestors of the new position have non-increasing counts going up, so the new
+//    element rises past exactly the j ancestors with a larger count.
+//  * make_heap: nodes of one level have disjoint subtrees, so each level's adjustments are
+//    independent of their order.
+CCA_HD int descend_path(const HeapEntry* h, int len, int start, int* path) {
+    int m = 0, x = start;
+    path[0] = start;
+    while (x < (len - 1) / 2) {
+        int r = 2 * (x + 1);
+        if (h[r - 1].count < h[r].count) r--;
+        path[++m] = r;
+        x = r;
+    }
+    if ((len & 1) == 0 && x == (len - 2) / 2) path[++m] = 2 * x + 1;
+    return m;
+}
+
+// pop over h[0, len): afterwards h[0, len-1) is the heap pop_heap leaves (h[len-1] not written)
+CCA_HD void pop_by_path(HeapEntry* h, int len, int* path) {
+    if (len <= 1) return;
+    const HeapEntry v = h[len - 1];
+    const int m = descend_path(h, len - 1, 0, path);
+    int larger = 0;
+    for (int k = 1; k <= m; k++) larger += (v.count < h[path[k]].count) ? 1 : 0;
+    const int is = m - larger;
+    for (int i = 0; i < is; i++) h[path[i]] = h[path[i + 1]];
+    h[path[is]] = v;
+}
+
+// push of v at position pos over the heap h[0, pos)
+CCA_HD void push_by_chain(HeapEntry* h, int pos, HeapEntry v) {
+    int T = 0;
+    while (((pos + 1) >> (T + 1)) > 0) T++;
+    int j = 0;
+    for (int t = 1; t <= T; t++) j += (v.count < h[((pos + 1) >> t) - 1].count) ? 1 : 0;
+    for (int t = 1; t <= j; t++) h[((pos + 1) >> (t - 1)) - 1] = h[((pos + 1) >> t) - 1];
+    h[((pos + 1) >> j) - 1] = v;
+}
+
+CCA_HD void make_by_levels(HeapEntry* h, int len) {
+    if (len < 2) return;
+    const int last = (len - 2) / 2;
+    int d = 0;
+    while ((1 << (d + 1)) - 1 <= last) d++;
+    for (; d >= 0; d--) {
+        const int lo = (1 << d) - 1, hi = ((1 << (d + 1)) - 2) < last ? ((1 << (d + 1)) - 2) : last;
+        for (int x = lo; x <= hi; x++) adjust_heap(h, x, len, h[x], QueueLess());
+    }
+}
+
 // ---------------------------------------------------------------- serial executor (host)
 struct SerialEx {
     CCA_HD int lane() const { return 0; }
     CCA_HD int lanes() const { return 1; }
     CCA_HD void barrier() const {}
+    CCA_HD void heap_make(HeapEntry* h, int len) const { make_heap(h, len, QueueLess()); }
+    CCA_HD void heap_pop(HeapEntry* h, int len) const { pop_heap(h, len, QueueLess()); }
+    CCA_HD void heap_push(HeapEntry* h, int pos, HeapEntry v) const {
+        h[pos] = v;
+        push_heap(h, pos + 1, QueueLess());
+    }
     template <class P, class S>
     CCA_HD int compact(int lo, int hi, P pred, S sink) const {
         int r = 0;
@@ -450,6 +520,15 @@ struct SerialEx {
             f(i, el[i], seen[d - 1]++, fi, fc);
         }
     }
+};
+
+// Serial executor using the path formulations of the heap operations (what the wave executor
+// computes in parallel); the host check runs both executors against the standard library.
+struct PathEx : SerialEx {
+    mutable int path[64];
+    CCA_HD void heap_make(HeapEntry* h, int len) const { make_by_levels(h, len); }
+    CCA_HD void heap_pop(HeapEntry* h, int len) const { pop_by_path(h, len, path); }
+    CCA_HD void heap_push(HeapEntry* h, int pos, HeapEntry v) const { push_by_chain(h, pos, v); }
 };
 
 }  // namespace design

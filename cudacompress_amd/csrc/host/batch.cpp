@@ -28,6 +28,23 @@ CoderParams params_of(const cc_instance* I) {
     return CoderParams{I->cbs, I->log2, (uint32_t)I->dialect, fmt.count_bits, fmt.symbits_bits, 0};
 }
 
+// Side streams of a coder context (created on first use, on the current device).
+const SideStreams* side_streams(DeviceCoder& D, SideStreams& out) {
+    if (!D.fork) {
+        for (int i = 0; i < 3; i++) {
+            CCA_HIP(hipStreamCreateWithFlags(&D.side[i], hipStreamNonBlocking));
+            CCA_HIP(hipEventCreateWithFlags(&D.join[i], hipEventDisableTiming));
+        }
+        CCA_HIP(hipEventCreateWithFlags(&D.fork, hipEventDisableTiming));
+    }
+    for (int i = 0; i < 3; i++) {
+        out.s[i] = D.side[i];
+        out.join[i] = D.join[i];
+    }
+    out.fork = D.fork;
+    return &out;
+}
+
 void check_symbols(const cc_instance* I, int symSize) {
     const bool longSymbols = I->log2 > 16;
     CCA_REQUIRE(symSize == (longSymbols ? 4 : 2), CC_ERR_INVALID_ARG,
@@ -37,11 +54,9 @@ void check_symbols(const cc_instance* I, int symSize) {
 
 }  // namespace
 
-uint64_t device_encode_items(cc_instance* I, const std::vector<ItemSpec>& items, uint32_t ncalls, int symSize,
-                             uint32_t* d_out, uint64_t out_cap, std::vector<CallOut>* results, bool wait) {
-    DeviceGuard guard(I->device);
-    hipStream_t st = I->stream;
-    DeviceCoder& D = I->coder;
+// ------------------------------------------------------------------------------------- encode
+void encode_prepare(cc_instance* I, DeviceCoder& D, const std::vector<ItemSpec>& items, uint32_t ncalls, int symSize,
+                    EncodeJob& job, hipStream_t st) {
     check_symbols(I, symSize);
     const uint32_t nitems = (uint32_t)items.size();
     const size_t o_first = align_up(sizeof(CodeItem) * std::max<uint32_t>(nitems, 1), 256);
@@ -62,36 +77,70 @@ uint64_t device_encode_items(cc_instance* I, const std::vector<ItemSpec>& items,
         total_syms += it.n;
     }
     while (call < ncalls) hf[++call] = nitems;
-    const uint32_t nseq = seq;
-    CodeItem* d_items = D.d_items.as<CodeItem>(std::max<uint32_t>(nitems, 1));
-    uint32_t* d_first = D.d_call_first.as<uint32_t>(ncalls + 1);
-    SeqInfo* d_seqs = D.d_seqs.as<SeqInfo>(std::max<uint32_t>(nseq, 1));
-    CoderState* d_state = D.d_state.as<CoderState>(1);
-    CallOut* d_calls = D.d_calls.as<CallOut>(std::max<uint32_t>(ncalls, 1));
-    CCA_HIP(hipMemcpyAsync(d_items, hi, sizeof(CodeItem) * nitems, hipMemcpyHostToDevice, st));
-    CCA_HIP(hipMemcpyAsync(d_first, hf, sizeof(uint32_t) * (ncalls + 1), hipMemcpyHostToDevice, st));
-    const CoderParams p = params_of(I);
-    uint64_t cap = std::max<uint64_t>(D.scratch_cap, arena_estimate(items, I->log2, I->cbs));
+    job.nitems = nitems;
+    job.nseq = seq;
+    job.ncalls = ncalls;
+    job.total_syms = total_syms;
+    job.symSize = symSize;
+    job.p = params_of(I);
+    D.d_items.as<CodeItem>(std::max<uint32_t>(nitems, 1));
+    D.d_call_first.as<uint32_t>(ncalls + 1);
+    D.d_seqs.as<SeqInfo>(std::max<uint32_t>(job.nseq, 1));
+    D.d_state.as<CoderState>(1);
+    D.d_calls.as<CallOut>(std::max<uint32_t>(ncalls, 1));
+    D.h_result.get(sizeof(CoderState) + sizeof(CallOut) * std::max<uint32_t>(ncalls, 1));
+    CCA_HIP(hipMemcpyAsync(D.d_items.p, hi, sizeof(CodeItem) * nitems, hipMemcpyHostToDevice, st));
+    CCA_HIP(hipMemcpyAsync(D.d_call_first.p, hf, sizeof(uint32_t) * (ncalls + 1), hipMemcpyHostToDevice, st));
+    encode_reserve(D, job, std::max<uint64_t>(D.scratch_cap, arena_estimate(items, I->log2, I->cbs)));
+    SideStreams ss;
+    side_streams(D, ss);
+}
+
+void encode_reserve(DeviceCoder& D, EncodeJob& job, uint64_t words) {
+    const uint64_t tail = 2ull * (job.nitems + 1) + 16;  // item offsets (u64) after the arena
+    D.d_scratch.as<uint32_t>(words + tail);
+    D.scratch_cap = words;
+    job.scratch_cap = words;
+}
+
+void encode_launch(DeviceCoder& D, const EncodeJob& job, uint32_t* d_out, uint64_t out_cap, hipStream_t st) {
+    SideStreams ss;
+    coder_encode((const CodeItem*)D.d_items.p, job.nitems, (SeqInfo*)D.d_seqs.p, job.nseq, job.p, job.symSize,
+                 (uint32_t*)D.d_scratch.p, job.scratch_cap, (CoderState*)D.d_state.p, (CallOut*)D.d_calls.p,
+                 job.ncalls, (const uint32_t*)D.d_call_first.p, d_out, out_cap, job.total_syms, st,
+                 side_streams(D, ss));
+}
+
+CoderState encode_finish(DeviceCoder& D, const EncodeJob& job, std::vector<CallOut>* results, hipStream_t st) {
+    unsigned char* r =
+        (unsigned char*)D.h_result.get(sizeof(CoderState) + sizeof(CallOut) * std::max<uint32_t>(job.ncalls, 1));
+    CCA_HIP(hipMemcpyAsync(r, D.d_state.p, sizeof(CoderState), hipMemcpyDeviceToHost, st));
+    if (job.ncalls)
+        CCA_HIP(hipMemcpyAsync(r + sizeof(CoderState), D.d_calls.p, sizeof(CallOut) * job.ncalls,
+                               hipMemcpyDeviceToHost, st));
+    CCA_HIP(hipStreamSynchronize(st));
+    if (results)
+        results->assign((const CallOut*)(r + sizeof(CoderState)),
+                        (const CallOut*)(r + sizeof(CoderState)) + job.ncalls);
+    return *(const CoderState*)r;
+}
+
+uint64_t device_encode_items(cc_instance* I, const std::vector<ItemSpec>& items, uint32_t ncalls, int symSize,
+                             uint32_t* d_out, uint64_t out_cap, std::vector<CallOut>* results, bool wait) {
+    DeviceGuard guard(I->device);
+    hipStream_t st = I->stream;
+    DeviceCoder& D = I->coder;
+    EncodeJob job;
+    encode_prepare(I, D, items, ncalls, symSize, job, st);
     for (int attempt = 0;; attempt++) {
-        const uint64_t tail = 2ull * (nitems + 1) + 16;  // item offsets (u64)
-        uint32_t* scratch = D.d_scratch.as<uint32_t>(cap + tail);
-        D.scratch_cap = cap;
-        coder_encode(d_items, nitems, d_seqs, nseq, p, symSize, scratch, cap, d_state, d_calls, ncalls, d_first, d_out,
-                     out_cap, total_syms, st);
+        encode_launch(D, job, d_out, out_cap, st);
         if (!wait) return 0;
-        const size_t rbytes = sizeof(CoderState) + sizeof(CallOut) * ncalls;
-        unsigned char* r = (unsigned char*)D.h_result.get(rbytes);
-        CCA_HIP(hipMemcpyAsync(r, d_state, sizeof(CoderState), hipMemcpyDeviceToHost, st));
-        CCA_HIP(hipMemcpyAsync(r + sizeof(CoderState), d_calls, sizeof(CallOut) * ncalls, hipMemcpyDeviceToHost, st));
-        CCA_HIP(hipStreamSynchronize(st));
-        const CoderState* hs = (const CoderState*)r;
-        if (hs->err == kErrScratch && attempt < 4) {
-            cap = std::max<uint64_t>(2 * cap, hs->scratch_used + hs->scratch_used / 2);
+        const CoderState s = encode_finish(D, job, results, st);
+        if (s.err == kErrScratch && attempt < 4) {
+            encode_reserve(D, job, std::max<uint64_t>(2 * job.scratch_cap, s.scratch_used + s.scratch_used / 2));
             continue;
         }
-        if (results)
-            results->assign((const CallOut*)(r + sizeof(CoderState)), (const CallOut*)(r + sizeof(CoderState)) + ncalls);
-        return hs->out_words;
+        return s.out_words;
     }
 }
 
@@ -102,16 +151,15 @@ uint64_t device_encode(cc_instance* I, const std::vector<CoderCall>& calls, cons
         CCA_REQUIRE((uint64_t)calls[c].first_stream + calls[c].nstreams <= streams.size(), CC_ERR_INVALID_ARG,
                     "bad stream range");
         for (uint32_t s = 0; s < calls[c].nstreams; s++)
-            items.push_back(ItemSpec{const_cast<void*>(streams[calls[c].first_stream + s]), calls[c].n, calls[c].rl, c});
+            items.push_back(
+                ItemSpec{const_cast<void*>(streams[calls[c].first_stream + s]), calls[c].n, calls[c].rl, c});
     }
     return device_encode_items(I, items, (uint32_t)calls.size(), symSize, d_out, out_cap, results, wait);
 }
 
-void device_decode_items(cc_instance* I, const std::vector<ItemSpec>& items, const std::vector<CallWords>& calls,
-                         int symSize, std::vector<DecCallOut>* results, bool wait) {
-    DeviceGuard guard(I->device);
-    hipStream_t st = I->stream;
-    DeviceCoder& D = I->coder;
+// ------------------------------------------------------------------------------------- decode
+void decode_prepare(cc_instance* I, DeviceCoder& D, const std::vector<ItemSpec>& items,
+                    const std::vector<CallWords>& calls, int symSize, DecodeJob& job, hipStream_t st) {
     check_symbols(I, symSize);
     const uint32_t ncalls = (uint32_t)calls.size();
     const uint32_t nitems = (uint32_t)items.size();
@@ -124,21 +172,19 @@ void device_decode_items(cc_instance* I, const std::vector<ItemSpec>& items, con
         nseq += q;
         nwaves += q * div_up(nblk, 64);
         total_syms += it.n;
-        arena += q * ((uint64_t)nblk + 16 + (it.rl ? ((uint64_t)it.n * symSize + 3) / 4 + 17 : 0));
+        // block offsets, lookup + symbol table (nsym <= count <= n), RLE decode buffers
+        arena += q * ((uint64_t)nblk + 16 + 1024 + it.n + 17 + (it.rl ? ((uint64_t)it.n * symSize + 3) / 4 + 17 : 0));
     }
-    const size_t o_items = align_up(sizeof(DecCall) * std::max<uint32_t>(ncalls, 1), 256);
-    const size_t o_wseq = o_items + align_up(sizeof(DecItem) * std::max<uint32_t>(nitems, 1), 256);
-    const size_t o_wfirst = o_wseq + align_up(4ull * std::max<uint32_t>(nwaves, 1), 256);
-    const size_t total = o_wfirst + 4ull * std::max<uint32_t>(nwaves, 1);
+    job.o_items = align_up(sizeof(DecCall) * std::max<uint32_t>(ncalls, 1), 256);
+    job.o_wseq = job.o_items + align_up(sizeof(DecItem) * std::max<uint32_t>(nitems, 1), 256);
+    job.o_wfirst = job.o_wseq + align_up(4ull * std::max<uint32_t>(nwaves, 1), 256);
+    const size_t total = job.o_wfirst + 4ull * std::max<uint32_t>(nwaves, 1);
     unsigned char* h = (unsigned char*)D.h_dstage.get(total);
     DecCall* hc = (DecCall*)h;
-    DecItem* hi = (DecItem*)(h + o_items);
-    uint32_t* hws = (uint32_t*)(h + o_wseq);
-    uint32_t* hwf = (uint32_t*)(h + o_wfirst);
-    for (uint32_t c = 0; c < ncalls; c++) {
-        CCA_REQUIRE(calls[c].words != nullptr || calls[c].nwords == 0, CC_ERR_INVALID_ARG, "null bitstream");
-        hc[c] = DecCall{calls[c].words, calls[c].nwords, 0, 0};
-    }
+    DecItem* hi = (DecItem*)(h + job.o_items);
+    uint32_t* hws = (uint32_t*)(h + job.o_wseq);
+    uint32_t* hwf = (uint32_t*)(h + job.o_wfirst);
+    for (uint32_t c = 0; c < ncalls; c++) hc[c] = DecCall{calls[c].words, calls[c].nwords, 0, 0};
     uint32_t seq = 0, wv = 0, call = 0;
     for (uint32_t i = 0; i < nitems; i++) {
         const ItemSpec& it = items[i];
@@ -157,24 +203,60 @@ void device_decode_items(cc_instance* I, const std::vector<ItemSpec>& items, con
             }
         seq += it.rl ? 2 : 1;
     }
-    unsigned char* d = (unsigned char*)D.d_dcalls.get(total);
-    CCA_HIP(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, st));
-    DecSeq* d_seqs = D.d_dseqs.as<DecSeq>(std::max<uint32_t>(nseq, 1));
-    CoderState* d_state = D.d_dstate.as<CoderState>(1);
-    DecCallOut* d_out = D.d_dout.as<DecCallOut>(std::max<uint32_t>(ncalls, 1));
-    uint32_t* scratch = D.d_dscratch.as<uint32_t>(arena);
-    coder_decode((const DecCall*)d, ncalls, (const DecItem*)(d + o_items), nitems, d_seqs, nseq, params_of(I), symSize,
-                 scratch, arena, d_state, d_out, (const uint32_t*)(d + o_wseq), (const uint32_t*)(d + o_wfirst), nwaves,
-                 total_syms, st);
-    if (!wait) return;
-    const size_t rbytes = sizeof(CoderState) + sizeof(DecCallOut) * ncalls;
-    unsigned char* r = (unsigned char*)D.h_dresult.get(rbytes);
-    CCA_HIP(hipMemcpyAsync(r, d_state, sizeof(CoderState), hipMemcpyDeviceToHost, st));
-    CCA_HIP(hipMemcpyAsync(r + sizeof(CoderState), d_out, sizeof(DecCallOut) * ncalls, hipMemcpyDeviceToHost, st));
+    job.nitems = nitems;
+    job.nseq = nseq;
+    job.ncalls = ncalls;
+    job.nwaves = nwaves;
+    job.total_syms = total_syms;
+    job.arena = arena;
+    job.symSize = symSize;
+    job.p = params_of(I);
+    D.d_dcalls.get(total);
+    D.d_dseqs.as<DecSeq>(std::max<uint32_t>(nseq, 1));
+    D.d_dstate.as<CoderState>(1);
+    D.d_dout.as<DecCallOut>(std::max<uint32_t>(ncalls, 1));
+    D.d_dscratch.as<uint32_t>(arena);
+    D.h_dresult.get(sizeof(CoderState) + sizeof(DecCallOut) * std::max<uint32_t>(ncalls, 1));
+    CCA_HIP(hipMemcpyAsync(D.d_dcalls.p, h, total, hipMemcpyHostToDevice, st));
+}
+
+void decode_launch(DeviceCoder& D, const DecodeJob& job, hipStream_t st) {
+    const unsigned char* d = (const unsigned char*)D.d_dcalls.p;
+    coder_decode((const DecCall*)d, job.ncalls, (const DecItem*)(d + job.o_items), job.nitems, (DecSeq*)D.d_dseqs.p,
+                 job.nseq, job.p, job.symSize, (uint32_t*)D.d_dscratch.p, job.arena, (CoderState*)D.d_dstate.p,
+                 (DecCallOut*)D.d_dout.p, (const uint32_t*)(d + job.o_wseq), (const uint32_t*)(d + job.o_wfirst),
+                 job.nwaves, job.total_syms, st);
+}
+
+void decode_finish(DeviceCoder& D, const DecodeJob& job, std::vector<DecCallOut>* results, hipStream_t st) {
+    unsigned char* r =
+        (unsigned char*)D.h_dresult.get(sizeof(CoderState) + sizeof(DecCallOut) * std::max<uint32_t>(job.ncalls, 1));
+    CCA_HIP(hipMemcpyAsync(r, D.d_dstate.p, sizeof(CoderState), hipMemcpyDeviceToHost, st));
+    if (job.ncalls)
+        CCA_HIP(hipMemcpyAsync(r + sizeof(CoderState), D.d_dout.p, sizeof(DecCallOut) * job.ncalls,
+                               hipMemcpyDeviceToHost, st));
     CCA_HIP(hipStreamSynchronize(st));
     if (results)
         results->assign((const DecCallOut*)(r + sizeof(CoderState)),
-                        (const DecCallOut*)(r + sizeof(CoderState)) + ncalls);
+                        (const DecCallOut*)(r + sizeof(CoderState)) + job.ncalls);
+}
+
+void decode_calls_from_encode(DeviceCoder& D, const DecodeJob& job, const CallOut* d_enc, const uint32_t* base,
+                              hipStream_t st) {
+    coder_calls_from_encode((DecCall*)D.d_dcalls.p, d_enc, base, job.ncalls, st);
+}
+
+void device_decode_items(cc_instance* I, const std::vector<ItemSpec>& items, const std::vector<CallWords>& calls,
+                         int symSize, std::vector<DecCallOut>* results, bool wait) {
+    DeviceGuard guard(I->device);
+    hipStream_t st = I->stream;
+    DeviceCoder& D = I->coder;
+    for (const CallWords& c : calls)
+        CCA_REQUIRE(c.words != nullptr || c.nwords == 0, CC_ERR_INVALID_ARG, "null bitstream");
+    DecodeJob job;
+    decode_prepare(I, D, items, calls, symSize, job, st);
+    decode_launch(D, job, st);
+    if (wait) decode_finish(D, job, results, st);
 }
 
 void device_decode(cc_instance* I, const std::vector<DecodeCall>& calls, const std::vector<void*>& outs, int symSize,

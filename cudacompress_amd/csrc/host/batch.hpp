@@ -1,5 +1,6 @@
-// Host side of the device-resident coder (hip/coder.hip): builds the item list of a job of coder
-// calls, sizes the scratch arena, launches the pipeline and reports per-call results.
+// Host side of the device-resident coder (hip/coder.hip, hip/decoder.hip).  A job is staged once
+// (prepare: host tables + one pinned upload), launched as kernels only (launch: capturable into a
+// hipGraph), and completed by one download + synchronisation (finish).
 #pragma once
 
 #include <vector>
@@ -22,6 +23,38 @@ struct CallWords {
     uint64_t nwords;
 };
 
+struct EncodeJob {
+    uint32_t nitems = 0, nseq = 0, ncalls = 0;
+    uint64_t total_syms = 0;
+    int symSize = 2;
+    CoderParams p{};
+    uint64_t scratch_cap = 0;  // arena words
+};
+void encode_prepare(cc_instance* I, DeviceCoder& D, const std::vector<ItemSpec>& items, uint32_t ncalls, int symSize,
+                    EncodeJob& job, hipStream_t st);
+// Sizes the arena to at least `words` (allocates: not during graph capture).
+void encode_reserve(DeviceCoder& D, EncodeJob& job, uint64_t words);
+void encode_launch(DeviceCoder& D, const EncodeJob& job, uint32_t* d_out, uint64_t out_cap, hipStream_t st);
+CoderState encode_finish(DeviceCoder& D, const EncodeJob& job, std::vector<CallOut>* results, hipStream_t st);
+
+struct DecodeJob {
+    uint32_t nitems = 0, nseq = 0, ncalls = 0, nwaves = 0;
+    uint64_t total_syms = 0, arena = 0;
+    int symSize = 2;
+    CoderParams p{};
+    size_t o_items = 0, o_wseq = 0, o_wfirst = 0;
+};
+// Calls may carry null words when decode_calls_from_encode fills them on the device.
+void decode_prepare(cc_instance* I, DeviceCoder& D, const std::vector<ItemSpec>& items,
+                    const std::vector<CallWords>& calls, int symSize, DecodeJob& job, hipStream_t st);
+void decode_launch(DeviceCoder& D, const DecodeJob& job, hipStream_t st);
+void decode_finish(DeviceCoder& D, const DecodeJob& job, std::vector<DecCallOut>* results, hipStream_t st);
+// Call i of the decode reads base + enc[i].word_begin, enc[i].words words (one kernel).
+void decode_calls_from_encode(DeviceCoder& D, const DecodeJob& job, const CallOut* d_enc, const uint32_t* base,
+                              hipStream_t st);
+
+// Eager forms: prepare + launch (+ finish with one synchronisation when `wait`; the encoder
+// retries with a larger arena when it ran out).
 uint64_t device_encode_items(cc_instance* I, const std::vector<ItemSpec>& items, uint32_t ncalls, int symSize,
                              uint32_t* d_out, uint64_t out_cap, std::vector<CallOut>* results, bool wait);
 void device_decode_items(cc_instance* I, const std::vector<ItemSpec>& items, const std::vector<CallWords>& calls,
@@ -33,11 +66,6 @@ struct CoderCall {
     uint32_t nstreams;
     uint32_t n;                    // symbols per stream
 };
-
-// Encodes `calls` (streams: device pointers) into d_out (capacity out_cap words) without any
-// host synchronisation inside the pipeline; when `wait` is set the call synchronises once at the
-// end, retries with a larger arena if it ran out, and fills `results` (one per call).
-// Returns the total word count (valid when waited).
 uint64_t device_encode(cc_instance* I, const std::vector<CoderCall>& calls, const std::vector<const void*>& streams,
                        int symSize, uint32_t* d_out, uint64_t out_cap, std::vector<CallOut>* results, bool wait);
 
@@ -49,9 +77,6 @@ struct DecodeCall {
     uint32_t nstreams;
     uint32_t n;
 };
-
-// Decodes `calls` into the device outputs without host synchronisation inside the pipeline;
-// with `wait`, synchronises once and fills `results` (words consumed and status per call).
 void device_decode(cc_instance* I, const std::vector<DecodeCall>& calls, const std::vector<void*>& outs, int symSize,
                    std::vector<DecCallOut>* results, bool wait);
 

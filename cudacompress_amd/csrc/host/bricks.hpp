@@ -1,14 +1,52 @@
 // Batched float bricks (host/bricks.cpp).
 #pragma once
 
+#include <functional>
+#include <memory>
+#include <vector>
+
+#include "../bricks.hpp"
+#include "batch.hpp"
 #include "instance.hpp"
 
 namespace cca {
+
+struct BrickJob {
+    std::vector<cc_brick> bricks;
+    uint32_t L = 2;
+    int symSize = 2;
+    // layout (elements): per-brick work buffers a, b, lowpass buffers [level][brick], symbols
+    std::vector<uint64_t> a, b, sym;
+    std::vector<std::vector<uint64_t>> lp;
+    std::vector<uint64_t> level_elems;
+    uint64_t work_floats = 0, sym_elems = 0;
+    // device tables
+    const BrickLevel* d_levels = nullptr;  // [L][n]
+    const uint32_t* d_tiles = nullptr;     // [L][3][n+1]
+    std::vector<uint32_t> tiles_total;     // [L][3]
+    EncodeJob ej;
+    DecodeJob dj;
+    uint32_t* d_out = nullptr;
+    uint64_t out_cap = 0;
+};
+
+// decode calls: one bitstream per brick, or when `calls` is null the device results of the
+// encode staged in the same context
+void bricks_prepare(cc_instance* I, BrickCtx& ctx, BrickJob& job, bool enc, bool dec,
+                    const std::vector<CallWords>* calls, hipStream_t st);
+bool bricks_decode_results(const BrickJob& job, const std::vector<DecCallOut>& res, uint32_t* status);
+void bricks_encode_launch(BrickCtx& ctx, const BrickJob& job, hipStream_t st);
+void bricks_decode_launch(BrickCtx& ctx, const BrickJob& job, hipStream_t st, bool from_encode);
 
 // offsets: nbricks+1 word offsets into d_out (host); status: per-brick cc_status (may be null).
 void compress_bricks(cc_instance* I, const cc_brick* bricks, uint32_t n, uint32_t levels, int symSize, uint32_t* d_out,
                      uint64_t cap, uint64_t* offsets, uint32_t* status);
 void decompress_bricks(cc_instance* I, const cc_brick* bricks, uint32_t n, uint32_t levels, int symSize,
                        const uint32_t* d_in, const uint64_t* offsets, uint32_t* status);
+
+cc_bricks_plan* plan_create(cc_instance* I, const cc_brick* bricks, uint32_t n, uint32_t levels, int symSize,
+                            uint32_t* d_out, uint64_t cap);
+void plan_compress(cc_bricks_plan* P, uint64_t* offsets, uint32_t* status);
+void plan_decompress(cc_bricks_plan* P, uint32_t* status);
 
 }  // namespace cca

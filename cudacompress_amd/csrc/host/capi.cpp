@@ -6,6 +6,7 @@
 #include "../kernels.hpp"
 #include "batch.hpp"
 #include "bricks.hpp"
+#include "server.hpp"
 #include "codec.hpp"
 #include "instance.hpp"
 #include "volume.hpp"
@@ -203,6 +204,71 @@ cc_status cc_decompress_bricks_float(cc_instance* inst, const cc_brick* bricks, 
         cca::decompress_bricks(inst, bricks, count, numLevels, symbolSize, d_in, brickWordOffsets, brickStatus);
     });
 }
+
+cc_status cc_bricks_plan_create(cc_instance* inst, const cc_brick* bricks, uint32_t count, uint32_t numLevels,
+                                int symbolSize, uint32_t* d_out, uint64_t outCapacityWords, cc_bricks_plan** plan) {
+    return guarded([&] {
+        CCA_REQUIRE(inst && plan && d_out, CC_ERR_INVALID_ARG, "null argument");
+        *plan = nullptr;
+        *plan = cca::plan_create(inst, bricks, count, numLevels, symbolSize, d_out, outCapacityWords);
+    });
+}
+
+cc_status cc_bricks_plan_compress(cc_bricks_plan* plan, uint64_t* brickWordOffsets, uint32_t* brickStatus) {
+    return guarded([&] {
+        CCA_REQUIRE(plan && brickWordOffsets, CC_ERR_INVALID_ARG, "null argument");
+        cca::plan_compress(plan, brickWordOffsets, brickStatus);
+    });
+}
+
+cc_status cc_bricks_plan_decompress(cc_bricks_plan* plan, uint32_t* brickStatus) {
+    return guarded([&] {
+        CCA_REQUIRE(plan, CC_ERR_INVALID_ARG, "null plan");
+        cca::plan_decompress(plan, brickStatus);
+    });
+}
+
+void cc_bricks_plan_destroy(cc_bricks_plan* plan) { delete plan; }
+
+cc_status cc_server_create(cc_instance* inst, const uint32_t* d_words, uint32_t brickCount,
+                           const uint64_t* brickWordRanges, const uint32_t* brickSizes, const float* quantSteps,
+                           uint32_t numLevels, int symbolSize, uint32_t slots, uint64_t workspaceBytes,
+                           cc_server** out) {
+    return guarded([&] {
+        CCA_REQUIRE(inst && out, CC_ERR_INVALID_ARG, "null argument");
+        *out = nullptr;
+        *out = cca::server_create(inst, d_words, brickCount, brickWordRanges, brickSizes, quantSteps, numLevels,
+                                  symbolSize, slots, workspaceBytes);
+    });
+}
+
+cc_status cc_server_submit(cc_server* server, uint64_t requestId, const uint32_t* brickIds, uint32_t count,
+                           float* const* d_outputs, double arrivalTime) {
+    return guarded([&] {
+        CCA_REQUIRE(server, CC_ERR_INVALID_ARG, "null server");
+        cca::server_submit(server, requestId, brickIds, count, d_outputs, arrivalTime);
+    });
+}
+
+cc_status cc_server_poll(cc_server* server, double now, uint32_t* batchesLaunched) {
+    return guarded([&] {
+        CCA_REQUIRE(server, CC_ERR_INVALID_ARG, "null server");
+        const uint32_t n = cca::server_poll(server, now);
+        if (batchesLaunched) *batchesLaunched = n;
+    });
+}
+
+uint32_t cc_server_pending(const cc_server* server) { return server ? cca::server_busy(server) : 0u; }
+
+uint32_t cc_server_completed(cc_server* server, uint64_t* requestIds, double* times, uint32_t max) {
+    return server ? cca::server_take_completed(server, requestIds, times, max) : 0u;
+}
+
+uint32_t cc_server_failed(cc_server* server, uint64_t* requestIds, uint32_t* status, uint32_t max) {
+    return server ? cca::server_take_failed(server, requestIds, status, max) : 0u;
+}
+
+void cc_server_destroy(cc_server* server) { delete server; }
 
 cc_status cc_encode_rlhuff(cc_instance* inst, int symbolSize, const void* const* d_symbolStreams, uint32_t streamCount,
                            uint32_t symbolCountPerStream, const uint32_t* startBitOffsets, const uint32_t** outWords,

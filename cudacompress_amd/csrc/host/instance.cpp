@@ -56,6 +56,35 @@ DeviceGuard::~DeviceGuard() {
     if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
 }
 
+void DeviceCoder::release() {
+    DevBuf* devs[] = {&d_items, &d_seqs, &d_scratch, &d_state, &d_calls, &d_call_first, &d_dcalls,
+                      &d_ditems, &d_dseqs, &d_dscratch, &d_dstate, &d_dout, &d_waves};
+    for (auto* b : devs) b->release();
+    h_stage.release();
+    h_result.release();
+    h_dstage.release();
+    h_dresult.release();
+    for (auto& e : join) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
+    if (fork) (void)hipEventDestroy(fork);
+    fork = nullptr;
+    for (auto& st : side) {
+        if (st) (void)hipStreamDestroy(st);
+        st = nullptr;
+    }
+    scratch_cap = 0;
+}
+
+void BrickCtx::release() {
+    coder.release();
+    d_bwork.release();
+    d_bsym.release();
+    d_blevels.release();
+    h_blevels.release();
+}
+
 }  // namespace cca
 
 cc_instance::~cc_instance() {
@@ -69,14 +98,12 @@ cc_instance::~cc_instance() {
     cca::DevBuf* devs[] = {&d_rle_sym, &d_rle_zc, &d_desc, &d_tiles, &d_map, &d_map2, &d_results, &d_hist,
                            &d_tables, &d_words, &d_offsets, &d_block_bits, &d_stream_bits, &d_dec_stage,
                            &d_dec_sym, &d_dec_zc, &d_dec_sums, &d_dec_desc, &d_vol[0], &d_vol[1], &d_vol[2],
-                           &d_vol[3], &d_vol_ptrs, &coder.d_items, &coder.d_seqs, &coder.d_scratch,
-                           &coder.d_state, &coder.d_calls, &coder.d_call_first, &coder.d_dcalls,
-                           &coder.d_ditems, &coder.d_dseqs, &coder.d_dscratch, &coder.d_dstate, &coder.d_dout,
-                           &coder.d_waves, &d_bwork, &d_bsym, &d_blevels};
+                           &d_vol[3], &d_vol_ptrs};
     for (auto* b : devs) b->release();
-    cca::PinBuf* pins[] = {&h_up, &h_down, &h_hist, &h_words, &h_vol_ptrs, &coder.h_stage, &coder.h_result,
-                           &coder.h_dstage, &coder.h_dresult, &h_blevels};
+    cca::PinBuf* pins[] = {&h_up, &h_down, &h_hist, &h_words, &h_vol_ptrs};
     for (auto* b : pins) b->release();
     for (auto& b : h_dec_stage) b.release();
+    coder.release();
+    batch.release();
     if (prev >= 0) (void)hipSetDevice(prev);
 }

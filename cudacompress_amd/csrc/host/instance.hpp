@@ -46,7 +46,19 @@ struct DeviceCoder {
     PinBuf h_stage, h_result;
     DevBuf d_dcalls, d_ditems, d_dseqs, d_dscratch, d_dstate, d_dout, d_waves;
     PinBuf h_dstage, h_dresult;
+    // side streams for independent launches (default priority, non-blocking) and their events
+    hipStream_t side[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t fork = nullptr, join[3] = {nullptr, nullptr, nullptr};
     uint64_t scratch_cap = 0;  // words, excluding the item-offset tail
+    void release();  // frees buffers, streams and events (device must be current)
+};
+
+// Buffers of one brick-batch pipeline: coder context, brick work buffers and symbol streams.
+struct BrickCtx {
+    DeviceCoder coder;
+    DevBuf d_bwork, d_bsym, d_blevels;
+    PinBuf h_blevels;
+    void release();
 };
 
 }  // namespace cca
@@ -83,9 +95,8 @@ struct cc_instance {
     // device-resident coder
     cca::DeviceCoder coder;
 
-    // brick batches (host/bricks.cpp)
-    cca::DevBuf d_bwork, d_bsym, d_blevels;
-    cca::PinBuf h_blevels;
+    // brick batches of the eager API (host/bricks.cpp)
+    cca::BrickCtx batch;
 
     ~cc_instance();
 };

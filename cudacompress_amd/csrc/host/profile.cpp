@@ -49,6 +49,11 @@ bool profiling_enabled() { return g_on.load(std::memory_order_relaxed); }
 
 KernelTimer::KernelTimer(const char* name, hipStream_t st, uint64_t bytes) : name_(name), st_(st), bytes_(bytes) {
     if (!profiling_enabled()) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st_, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+        (void)hipGetLastError();
+        return;  // launches recorded into a graph are timed by the graph's replay, not per kernel
+    }
     std::lock_guard<std::mutex> lk(g_mu);
     start_ = take_event();
     if (start_) (void)hipEventRecord(start_, st_);

@@ -1,0 +1,218 @@
+// Decode serving loop: compressed bricks stay resident in device memory; requests (lists of
+// brick ids with caller-owned output bricks) are admitted as they arrive, batched by a
+// decode-workspace budget into one brick batch per free slot, and launched asynchronously on that
+// slot's stream, so batches of different slots overlap on the device.  Completion is observed
+// with an event per slot; a request completes when its batch does.
+//
+// No reference counterpart (the reference decodes one volume per call).  Each slot owns a brick
+// batch context (host/bricks.cpp), so a request decodes to exactly the values of a batch that
+// holds it alone.
+#include "server.hpp"
+
+#include <algorithm>
+#include <deque>
+
+#include "bricks.hpp"
+
+namespace {
+
+struct Request {
+    uint64_t id;
+    std::vector<uint32_t> bricks;
+    std::vector<float*> outs;
+    double arrival;
+    uint64_t workspace;
+};
+
+struct Slot {
+    cca::BrickCtx ctx;
+    cca::BrickJob job;
+    hipStream_t st = nullptr;
+    hipEvent_t done = nullptr;
+    bool busy = false;
+    std::vector<uint64_t> reqs;
+    std::vector<uint32_t> req_nbricks;
+};
+
+uint64_t elems3(const uint32_t* d) { return (uint64_t)d[0] * d[1] * d[2]; }
+
+}  // namespace
+
+struct cc_server {
+    cc_instance* inst = nullptr;
+    const uint32_t* words = nullptr;
+    std::vector<uint64_t> begin, end;   // per brick word range
+    std::vector<uint32_t> dims;         // per brick x, y, z
+    std::vector<float> steps;
+    uint32_t levels = 2;
+    int symSize = 2;
+    uint64_t slot_budget = 0;
+    std::vector<Slot> slots;
+    std::deque<Request> pending;
+    std::vector<std::pair<uint64_t, double>> completed;
+    std::vector<std::pair<uint64_t, uint32_t>> failed;
+    ~cc_server();
+};
+
+cc_server::~cc_server() {
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    if (inst) (void)hipSetDevice(inst->device);
+    for (Slot& s : slots) {
+        if (s.st) (void)hipStreamSynchronize(s.st);
+        s.ctx.release();
+        if (s.done) (void)hipEventDestroy(s.done);
+        if (s.st) (void)hipStreamDestroy(s.st);
+    }
+    if (prev >= 0) (void)hipSetDevice(prev);
+}
+
+namespace cca {
+
+namespace {
+
+// decode workspace of one brick: work buffers (2 N + N/7 floats), symbols, coder arena
+uint64_t brick_workspace(uint64_t N, int symSize) { return N * 4 * 2 + N * 4 / 7 + N * symSize + N * 6 + 64 * 1024; }
+
+void reap(cc_server* S, Slot& s, double now) {
+    if (!s.busy) return;
+    const hipError_t q = hipEventQuery(s.done);
+    if (q == hipErrorNotReady) return;
+    CCA_HIP(q);
+    // per-brick statuses were copied to the pinned result buffer before the event
+    const DecCallOut* r = (const DecCallOut*)((const unsigned char*)s.ctx.coder.h_dresult.p + sizeof(CoderState));
+    uint32_t k = 0;
+    for (size_t i = 0; i < s.reqs.size(); i++) {
+        uint32_t err = 0;
+        for (uint32_t j = 0; j < s.req_nbricks[i]; j++, k++)
+            if (r[k].err && !err) err = r[k].err;
+        if (err) S->failed.emplace_back(s.reqs[i], err);
+        else S->completed.emplace_back(s.reqs[i], now);
+    }
+    s.busy = false;
+    s.reqs.clear();
+    s.req_nbricks.clear();
+}
+
+void launch(cc_server* S, Slot& s) {
+    // admit requests in arrival order while the batch fits the slot's workspace budget
+    std::vector<CallWords> calls;
+    s.job = BrickJob();
+    s.job.L = S->levels;
+    s.job.symSize = S->symSize;
+    uint64_t ws = 0;
+    while (!S->pending.empty()) {
+        Request& rq = S->pending.front();
+        if (!s.reqs.empty() && ws + rq.workspace > S->slot_budget) break;
+        ws += rq.workspace;
+        for (size_t j = 0; j < rq.bricks.size(); j++) {
+            const uint32_t b = rq.bricks[j];
+            const uint32_t* d = &S->dims[3 * b];
+            s.job.bricks.push_back(cc_brick{nullptr, rq.outs[j], d[0], d[1], d[2], S->steps[b]});
+            calls.push_back(CallWords{S->words + S->begin[b], S->end[b] - S->begin[b]});
+        }
+        s.reqs.push_back(rq.id);
+        s.req_nbricks.push_back((uint32_t)rq.bricks.size());
+        S->pending.pop_front();
+    }
+    if (s.reqs.empty()) return;
+    bricks_prepare(S->inst, s.ctx, s.job, false, true, &calls, s.st);
+    bricks_decode_launch(s.ctx, s.job, s.st, false);
+    // statuses to the pinned buffer, then the completion event
+    DeviceCoder& D = s.ctx.coder;
+    unsigned char* r = (unsigned char*)D.h_dresult.get(sizeof(CoderState) + sizeof(DecCallOut) * s.job.dj.ncalls);
+    CCA_HIP(hipMemcpyAsync(r + sizeof(CoderState), D.d_dout.p, sizeof(DecCallOut) * s.job.dj.ncalls,
+                           hipMemcpyDeviceToHost, s.st));
+    CCA_HIP(hipEventRecord(s.done, s.st));
+    s.busy = true;
+}
+
+}  // namespace
+
+cc_server* server_create(cc_instance* I, const uint32_t* d_words, uint32_t nbricks, const uint64_t* ranges,
+                         const uint32_t* dims, const float* steps, uint32_t levels, int symSize, uint32_t nslots,
+                         uint64_t workspace) {
+    DeviceGuard guard(I->device);
+    CCA_REQUIRE(nslots >= 1 && nslots <= 16, CC_ERR_INVALID_ARG, "slots must be in [1, 16]");
+    CCA_REQUIRE(d_words && ranges && dims && steps, CC_ERR_INVALID_ARG, "null argument");
+    std::unique_ptr<cc_server> S(new cc_server());
+    S->inst = I;
+    S->words = d_words;
+    S->levels = levels;
+    S->symSize = symSize;
+    S->begin.resize(nbricks);
+    S->end.resize(nbricks);
+    for (uint32_t i = 0; i < nbricks; i++) {
+        S->begin[i] = ranges[2 * i];
+        S->end[i] = ranges[2 * i + 1];
+        CCA_REQUIRE(S->end[i] >= S->begin[i], CC_ERR_INVALID_ARG, "bad brick word range");
+    }
+    S->dims.assign(dims, dims + 3 * (size_t)nbricks);
+    S->steps.assign(steps, steps + nbricks);
+    S->slot_budget = workspace / nslots;
+    S->slots.resize(nslots);
+    for (Slot& s : S->slots) {
+        CCA_HIP(hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking));
+        CCA_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+    }
+    return S.release();
+}
+
+void server_submit(cc_server* S, uint64_t id, const uint32_t* bricks, uint32_t count, float* const* outs,
+                   double arrival) {
+    CCA_REQUIRE(count >= 1 && bricks && outs, CC_ERR_INVALID_ARG, "a request names at least one brick");
+    Request r;
+    r.id = id;
+    r.arrival = arrival;
+    r.workspace = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        CCA_REQUIRE(bricks[i] < S->begin.size(), CC_ERR_INVALID_ARG, "brick id out of range");
+        CCA_REQUIRE(outs[i], CC_ERR_INVALID_ARG, "null output brick");
+        r.bricks.push_back(bricks[i]);
+        r.outs.push_back(outs[i]);
+        r.workspace += brick_workspace(elems3(&S->dims[3 * bricks[i]]), S->symSize);
+    }
+    S->pending.push_back(std::move(r));
+}
+
+uint32_t server_poll(cc_server* S, double now) {
+    DeviceGuard guard(S->inst->device);
+    uint32_t launched = 0;
+    for (Slot& s : S->slots) reap(S, s, now);
+    for (Slot& s : S->slots) {
+        if (S->pending.empty()) break;
+        if (s.busy) continue;
+        launch(S, s);
+        launched++;
+    }
+    return launched;
+}
+
+uint32_t server_busy(const cc_server* S) {
+    uint32_t b = 0;
+    for (const Slot& s : S->slots) b += s.busy ? 1 : 0;
+    return b + (uint32_t)S->pending.size();
+}
+
+uint32_t server_take_completed(cc_server* S, uint64_t* ids, double* times, uint32_t max) {
+    const uint32_t n = (uint32_t)std::min<size_t>(max, S->completed.size());
+    for (uint32_t i = 0; i < n; i++) {
+        if (ids) ids[i] = S->completed[i].first;
+        if (times) times[i] = S->completed[i].second;
+    }
+    S->completed.erase(S->completed.begin(), S->completed.begin() + n);
+    return n;
+}
+
+uint32_t server_take_failed(cc_server* S, uint64_t* ids, uint32_t* status, uint32_t max) {
+    const uint32_t n = (uint32_t)std::min<size_t>(max, S->failed.size());
+    for (uint32_t i = 0; i < n; i++) {
+        if (ids) ids[i] = S->failed[i].first;
+        if (status) status[i] = S->failed[i].second == kDecCorrupt ? (uint32_t)CC_ERR_CORRUPT_STREAM
+                                                                  : (uint32_t)CC_ERR_OUT_OF_MEMORY;
+    }
+    S->failed.erase(S->failed.begin(), S->failed.begin() + n);
+    return n;
+}
+
+}  // namespace cca

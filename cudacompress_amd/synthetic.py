@@ -6,6 +6,8 @@
   min-max mapped to [0, 65535] and returned as int16 = u - 32768.
 * velocity_field(n, seed): divergence-free u = curl A with |A_hat| ~ k^-17/6 (E_u ~ k^-5/3),
   unit RMS, shape (3, n, n, n).
+* brick_set(count, axes, seed): bricks with each axis drawn uniformly from `axes`, cut at seeded
+  offsets from seeded k^-5/3 fields.
 """
 import hashlib
 
@@ -62,6 +64,21 @@ def velocity_field(n, seed):
     u = np.stack([np.fft.irfftn(c, s=shape, axes=(0, 1, 2)) for c in (ux, uy, uz)])
     u /= np.sqrt((u ** 2).sum(axis=0).mean())
     return u.astype(np.float32)
+
+
+def brick_set(count, axes=(16, 32, 48, 64, 96, 128), seed=4, nfields=2, field_n=256):
+    """`count` float32 bricks; brick i: sizes and offsets from default_rng(seed + i), cut from
+    field i % nfields (gaussian_field((field_n,)*3, seed + 1000 + f)).  Returns a list of
+    C-contiguous (z, y, x) arrays."""
+    fields = [gaussian_field((field_n,) * 3, seed + 1000 + f) for f in range(nfields)]
+    out = []
+    for i in range(count):
+        r = np.random.default_rng(seed + i)
+        z, y, x = (int(r.choice(axes)) for _ in range(3))
+        f = fields[i % nfields]
+        oz, oy, ox = (int(r.integers(0, field_n - s + 1)) for s in (z, y, x))
+        out.append(np.ascontiguousarray(f[oz:oz + z, oy:oy + y, ox:ox + x]))
+    return out
 
 
 def sha256(a):

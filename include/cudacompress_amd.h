@@ -232,6 +232,50 @@ CC_API cc_status cc_decompress_bricks_float(cc_instance* inst, const cc_brick* b
                                             uint32_t numLevels, int symbolSize, const uint32_t* d_in,
                                             const uint64_t* brickWordOffsets, uint32_t* brickStatus);
 
+/* Plans: the same batch captured once into two hipGraphs (compression: every DWT pass and coder
+ * kernel of all bricks; decompression: the coder's decode kernels, the inverse passes, and the
+ * per-brick bitstream positions taken on the device from the last compression) and replayed.
+ * The bricks' src/dst pointers, sizes, steps, numLevels and d_out are fixed at creation; the data
+ * behind them may change between replays.  Creation runs one compression eagerly to size the
+ * coder's arena (with headroom); a replay whose data needs more arena fails with
+ * CC_ERR_OUT_OF_MEMORY and the plan must be recreated.  No reference counterpart: the reference
+ * has no batched or graph-captured path. */
+typedef struct cc_bricks_plan cc_bricks_plan;
+CC_API cc_status cc_bricks_plan_create(cc_instance* inst, const cc_brick* bricks, uint32_t count, uint32_t numLevels,
+                                       int symbolSize, uint32_t* d_out, uint64_t outCapacityWords,
+                                       cc_bricks_plan** plan);
+/* Replays the compression graph; fills the host arrays like cc_compress_bricks_float. */
+CC_API cc_status cc_bricks_plan_compress(cc_bricks_plan* plan, uint64_t* brickWordOffsets, uint32_t* brickStatus);
+/* Replays the decompression graph (bitstreams of the plan's last compression) into the dst bricks. */
+CC_API cc_status cc_bricks_plan_decompress(cc_bricks_plan* plan, uint32_t* brickStatus);
+CC_API void cc_bricks_plan_destroy(cc_bricks_plan* plan);
+
+/* ---- Decode serving ----------------------------------------------------------------------- */
+/* A request loop over compressed bricks resident in device memory (no reference counterpart).
+ * Brick i is the bitstream d_words[brickWordRanges[2i], brickWordRanges[2i+1]) of a brick of
+ * brickSizes[3i..3i+2] (x, y, z) compressed with quantSteps[i] and numLevels (as written by
+ * cc_compress_bricks_float).  Requests name bricks and caller-owned device output bricks;
+ * cc_server_poll launches every pending request that fits on a free slot -- one batch per slot,
+ * up to workspaceBytes / slots of decode workspace -- asynchronously on that slot's stream, and
+ * records the requests of finished batches as completed at time `now` (the caller's clock).
+ * Batches of different slots run concurrently.  Requests decode to the same values whatever they
+ * are batched with.  A server is driven from one host thread. */
+typedef struct cc_server cc_server;
+CC_API cc_status cc_server_create(cc_instance* inst, const uint32_t* d_words, uint32_t brickCount,
+                                  const uint64_t* brickWordRanges, const uint32_t* brickSizes,
+                                  const float* quantSteps, uint32_t numLevels, int symbolSize, uint32_t slots,
+                                  uint64_t workspaceBytes, cc_server** out);
+CC_API cc_status cc_server_submit(cc_server* server, uint64_t requestId, const uint32_t* brickIds, uint32_t count,
+                                  float* const* d_outputs, double arrivalTime);
+CC_API cc_status cc_server_poll(cc_server* server, double now, uint32_t* batchesLaunched);
+/* batches in flight plus requests waiting */
+CC_API uint32_t cc_server_pending(const cc_server* server);
+/* moves up to `max` completed requests (id, completion time) out; returns how many */
+CC_API uint32_t cc_server_completed(cc_server* server, uint64_t* requestIds, double* times, uint32_t max);
+/* moves up to `max` failed requests (id, decoder status code) out */
+CC_API uint32_t cc_server_failed(cc_server* server, uint64_t* requestIds, uint32_t* status, uint32_t max);
+CC_API void cc_server_destroy(cc_server* server);
+
 /* ---- Kernel timing ------------------------------------------------------------------------ */
 /* No reference counterpart (the reference times host-side sections with CUDA events,
  * src/cudaCompress/util/CudaTimer.cpp:65-169).  When enabled, every kernel launch of this library

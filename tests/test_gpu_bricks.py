@@ -98,3 +98,92 @@ def test_bricks_symbol32_and_overflow(cc, torch_gpu, field):
     want = cc.compressVolumeFloat(inst, srcs[0], x, y, z, 2, steps[0])
     assert np.array_equal(words[offs[0]:offs[1]], want)
     inst.close()
+
+
+def test_bricks_plan_graph_replays(cc, torch_gpu, field):
+    """A plan (compression and decompression captured in hipGraphs) gives the eager batch's
+    bitstreams and reconstructions, replay after replay, also after the source data changes."""
+    log("bricks plan")
+    torch = torch_gpu
+    rng = np.random.default_rng(21)
+    bricks = make_bricks(rng, field, 20)
+    std = float(field.std())
+    steps = [0.005 * std] * len(bricks)
+    shapes = [b.shape for b in bricks]
+    inst = cc.createInstance(0, 7, 64 * 64 * 64, 128, 14)
+    srcs = [torch.from_numpy(b).cuda() for b in bricks]
+    dsts = [torch.zeros_like(s) for s in srcs]
+    total = sum(b.size for b in bricks)
+    out = torch.zeros(total + 4096 * len(bricks), dtype=torch.int32, device="cuda")
+    plan = cc.BricksPlan(inst, srcs, dsts, shapes, steps, 2, out)
+    for rep in range(3):
+        if rep == 2:  # new data of the same geometry
+            for s in srcs:
+                s.mul_(1.0 + 0.01 * rep).add_(0.001)
+        offs, stat = plan.compress()
+        assert np.all(stat == 0)
+        pw = out[: int(offs[-1])].cpu().numpy().view(np.uint32).copy()
+        assert np.all(plan.decompress() == 0)
+        torch.cuda.synchronize()
+        prec = [d.cpu().numpy().copy() for d in dsts]
+        # eager batch on the same data
+        out2 = torch.zeros_like(out)
+        offs2, stat2 = cc.compressBricksFloat(inst, srcs, shapes, steps, 2, out2)
+        assert np.array_equal(offs, offs2), rep
+        assert np.array_equal(pw, out2[: int(offs2[-1])].cpu().numpy().view(np.uint32)), rep
+        d2 = [torch.zeros_like(s) for s in srcs]
+        cc.decompressBricksFloat(inst, d2, shapes, steps, 2, out2, offs2)
+        torch.cuda.synchronize()
+        for a, b in zip(prec, d2):
+            assert np.array_equal(a, b.cpu().numpy()), rep
+    plan.close()
+    inst.close()
+
+
+def test_server_results_same_alone_and_in_the_mix(cc, torch_gpu, field):
+    """Requests decoded through the serving loop -- batched with others, several slots in flight --
+    equal the same requests decoded alone, bit for bit, and equal the eager batch decode."""
+    import time
+    log("server alone vs mix")
+    torch = torch_gpu
+    rng = np.random.default_rng(33)
+    bricks = make_bricks(rng, field, 30)
+    std = float(field.std())
+    steps = [0.005 * std] * len(bricks)
+    inst = cc.createInstance(0, 7, 64 * 64 * 64, 128, 14)
+    srcs, out, offs, stat = run_batch(cc, torch, inst, bricks, steps, 2)
+    assert np.all(stat == 0)
+    shapes = [b.shape for b in bricks]
+    ranges = np.stack([offs[:-1], offs[1:]], axis=1)
+    reqs = [list(rng.choice(len(bricks), size=int(rng.integers(1, 9)), replace=True)) for _ in range(40)]
+
+    def serve(requests, slots, budget):
+        srv = cc.Server(inst, out, ranges, shapes, steps, 2, slots=slots, workspaceBytes=budget)
+        outs = {i: [torch.full_like(srcs[b], -1.0) for b in r] for i, r in enumerate(requests)}
+        torch.cuda.synchronize()  # the server's slot streams do not wait on torch's stream
+        t0 = time.perf_counter()
+        for i, r in enumerate(requests):
+            srv.submit(i, r, outs[i], time.perf_counter() - t0)
+            srv.poll(time.perf_counter() - t0)
+        while srv.pending():
+            srv.poll(time.perf_counter() - t0)
+        done, _ = srv.completed()
+        failed, _ = srv.failed()
+        srv.close()
+        torch.cuda.synchronize()
+        assert failed.size == 0 and sorted(done.tolist()) == list(range(len(requests)))
+        return {i: [o.cpu().numpy() for o in v] for i, v in outs.items()}
+
+    mix = serve(reqs, slots=3, budget=64 << 20)
+    alone = {}
+    for i, r in enumerate(reqs[:10]):
+        alone[i] = serve([r], slots=1, budget=1 << 30)[0]
+    eager = [torch.zeros_like(s) for s in srcs]
+    cc.decompressBricksFloat(inst, eager, shapes, steps, 2, out, offs)
+    torch.cuda.synchronize()
+    for i, r in enumerate(reqs):
+        for j, b in enumerate(r):
+            assert np.array_equal(mix[i][j], eager[b].cpu().numpy()), (i, j)
+            if i in alone:
+                assert np.array_equal(alone[i][j], mix[i][j]), (i, j)
+    inst.close()
