@@ -8,6 +8,15 @@ one device buffer + batched decompression, inputs and bitstreams resident in HBM
 bricks are also run through the plain per-brick API (compressVolumeFloat/decompressVolumeFloat,
 host bitstreams) on a sample, and the batched bitstreams are checked against the plain ones.
 
+Serving (on by default after the bricks run, alone with `--workload serving`): a 1024^3 k^-5/3
+field (seed 5, generated on the device), 729 overlapping 128^3 level-0 bricks (stride 120) plus
+729 64^3 bricks of its 2x box-downsampled level of detail, compressed in one batch at step
+0.005 x std of each level and kept resident; decode requests of 1-64 bricks (uniform, each brick
+level 0 or LOD with probability 1/2) arrive as a Poisson process at 1k-50k bricks/s and are
+replayed in real time through the native serving loop (cc_server_run, 32 GB decode-workspace
+budget).  Reports p50/p99 request latency (arrival to completion of the request's batch) and the
+sustained decoded GB/s per arrival rate.
+
 Other workloads: `lossless2d` (4096^2 uint16 fBm, 3-level 2D CDF 5/3, log2 16, plain API) and
 `float3d` (128^3 k^-5/3 field, 2-level 9/7, step 0.01, plain API).
 
@@ -39,8 +48,15 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="bricks", choices=["bricks", "lossless2d", "float3d"])
+    ap.add_argument("--workload", default="bricks", choices=["bricks", "serving", "lossless2d", "float3d"])
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving run after the bricks run")
+    ap.add_argument("--serving-n", type=int, default=1024, help="side of the serving field")
+    ap.add_argument("--serving-slots", type=int, default=4)
+    ap.add_argument("--serving-rates", default="1000,5000,10000,20000,50000", help="arrival rates, bricks/s")
     ap.add_argument("--bricks", type=int, default=2048)
+    ap.add_argument("--mode", default="plan", choices=["plan", "eager"],
+                    help="plan: compression and decompression replayed from hipGraphs (BricksPlan); "
+                         "eager: compressBricksFloat + decompressBricksFloat calls")
     ap.add_argument("--plain-sample", type=int, default=256, help="bricks timed through the plain per-brick API")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -107,11 +123,16 @@ def run_bricks(args, torch, cc, lib):
     out = torch.empty(nvox + 4096 * len(bricks), dtype=torch.int32, device=dev)
     inst = cc.createInstance(0, 7, 128 * 128 * 128 // 8, 128, 14)
     state = {}
+    plan = cc.BricksPlan(inst, srcs, dsts, shapes, steps_q, 2, out) if args.mode == "plan" else None
 
     def step():
-        offs, stat = cc.compressBricksFloat(inst, srcs, shapes, steps_q, 2, out)
-        cc.decompressBricksFloat(inst, dsts, shapes, steps_q, 2, out, offs)
-        state["offs"], state["stat"] = offs, stat
+        if plan is not None:
+            offs, stat = plan.compress()
+            dstat = plan.decompress()
+        else:
+            offs, stat = cc.compressBricksFloat(inst, srcs, shapes, steps_q, 2, out)
+            dstat = cc.decompressBricksFloat(inst, dsts, shapes, steps_q, 2, out, offs)
+        state["offs"], state["stat"], state["dstat"] = offs, stat, dstat
 
     if args.copy_calibration:
         a = torch.empty(256 << 20, dtype=torch.uint8, device=dev)
@@ -124,8 +145,16 @@ def run_bricks(args, torch, cc, lib):
     torch.cuda.synchronize()
     ms, kernels = timed(torch, lib, step, args.steps)
     value = 2.0 * raw_bytes / (ms * 1e-3) / 1e9
+    kernel_source = "HIP events around every kernel inside the timed steps"
+    if not kernels:  # graph replays carry no per-kernel events: time the same kernels in eager calls
+        def eager():
+            o, _ = cc.compressBricksFloat(inst, srcs, shapes, steps_q, 2, out)
+            cc.decompressBricksFloat(inst, dsts, shapes, steps_q, 2, out, o)
+        _, kernels = timed(torch, lib, eager, args.steps)
+        kernel_source = "HIP events around every kernel of eager batch calls on the same bricks"
     offs, stat = state["offs"], state["stat"]
     assert np.all(stat == 0), "batched compression failed for some bricks"
+    assert np.all(state["dstat"] == 0), "batched decompression failed for some bricks"
     compressed = int(offs[-1]) * 4
 
     # quality: aggregate PSNR (range over all bricks / RMSE over all voxels) and the worst brick
@@ -187,16 +216,113 @@ def run_bricks(args, torch, cc, lib):
                "sample": f"the first {runs} bricks ({done_bytes} raw bytes) compressed and decompressed one by one "
                          f"through the oracle (C++ restatement of the reference path, one host thread), {t_total:.2f} s"}
 
+    if plan is not None:
+        plan.close()
     inst.close()
     cfg = {"workload": "C4 batched: %d float32 bricks, axes from {16,32,48,64,96,128} (seeded), cut from seeded "
                        "k^-5/3 fields; 2-level 3D CDF 9/7, deadzone step 0.005*std(brick), RLE+Huffman log2 14, "
                        "coding block 128; one batched compress + one batched decompress per step" % len(bricks),
            "bricks": len(bricks), "raw_bytes_per_direction": raw_bytes, "compressed_bytes": compressed,
            "compression_ratio": round(raw_bytes / compressed, 3), "psnr_db": round(float(psnr), 3),
-           "worst_brick_psnr_db": round(float(worst), 3), "mode": "batched",
+           "worst_brick_psnr_db": round(float(worst), 3),
+           "mode": "batched, " + ("hipGraph plan replays" if args.mode == "plan" else "eager batch calls"),
+           "kernel_timing": kernel_source,
            "plain": plain, "batched_vs_plain": round(value / plain["GBps"], 2) if plain else None,
            "bitstreams_equal_plain_sample": equal}
     return value, ms, raw_bytes, kernels, cfg, cpu, "f32"
+
+
+# ------------------------------------------------------------------------------------ serving
+def run_serving(args, torch, cc, lib):
+    from cudacompress_amd import synthetic as S
+    dev = torch.device("cuda:0")
+    t_setup = time.perf_counter()
+    field = S.gaussian_field_device(args.serving_n, 5, dev)
+    l0, lod, lod_field = S.serving_bricks(field, 128, 8)
+    std0, std1 = float(field.std()), float(lod_field.std())
+    del field, lod_field
+    bricks = l0 + lod
+    n0 = len(l0)
+    shapes = [tuple(b.shape) for b in bricks]
+    steps = [0.005 * std0] * n0 + [0.005 * std1] * len(lod)
+    nvox = sum(b.numel() for b in bricks)
+    inst = cc.createInstance(0, 7, 128 * 128 * 128 // 8, 128, 14)
+    out = torch.empty(nvox, dtype=torch.int32, device=dev)
+    # compressed in batches of 243 bricks (at most ~5.1e8 voxels per batch) into one word buffer
+    offs = np.zeros(len(bricks) + 1, dtype=np.uint64)
+    dec = [torch.empty_like(b) for b in bricks]
+    for lo in range(0, len(bricks), 243):
+        hi = min(lo + 243, len(bricks))
+        w0 = int(offs[lo])
+        o, stat = cc.compressBricksFloat(inst, bricks[lo:hi], shapes[lo:hi], steps[lo:hi], 2, out[w0:])
+        assert np.all(stat == 0), "serving bricks failed to compress"
+        # quality of every brick through the batched decode
+        cc.decompressBricksFloat(inst, dec[lo:hi], shapes[lo:hi], steps[lo:hi], 2, out[w0:], o)
+        offs[lo + 1:hi + 1] = w0 + o[1:]
+    torch.cuda.synchronize()
+    compressed = int(offs[-1]) * 4
+    psnr = []
+    for lo, hi in ((0, n0), (n0, len(bricks))):
+        se, vmin, vmax, cnt = 0.0, np.inf, -np.inf, 0
+        for b, d in zip(bricks[lo:hi], dec[lo:hi]):
+            e = b.double() - d.double()
+            se += float((e * e).sum())
+            vmin, vmax, cnt = min(vmin, float(b.min())), max(vmax, float(b.max())), cnt + b.numel()
+        psnr.append(round(float(20 * np.log10((vmax - vmin) / np.sqrt(se / cnt))), 3))
+    del dec, bricks, l0, lod
+    ranges = np.stack([offs[:-1], offs[1:]], axis=1)
+    pool_n = 2048
+    pool = torch.empty((pool_n, 128 ** 3), dtype=torch.float32, device=dev)
+    base = pool.data_ptr()
+    stride = 128 ** 3 * 4
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t_setup
+    srv = cc.Server(inst, out, ranges, shapes, steps, 2, slots=args.serving_slots, workspaceBytes=32 << 30)
+    brick_bytes = np.array([4 * int(np.prod(sh)) for sh in shapes], dtype=np.int64)
+
+    def trace(rng, lam, nreq):
+        sizes = rng.integers(1, 65, nreq)
+        gaps = rng.exponential(sizes.mean() / lam, nreq)  # request rate lam / E[size]
+        arrivals = np.cumsum(gaps) - gaps[0]
+        reqs, outs, k = [], [], 0
+        for c in sizes:
+            ids = rng.integers(0, n0, c) + n0 * rng.integers(0, 2, c)
+            reqs.append(ids.astype(np.uint32))
+            outs.append([base + ((k + j) % pool_n) * stride for j in range(c)])
+            k += c
+        return arrivals, reqs, outs
+
+    # warm-up: the largest requests on every slot grow the slots' buffers once
+    big = [np.arange(64, dtype=np.uint32) % n0 for _ in range(2 * args.serving_slots)]
+    srv.run(np.zeros(len(big)), big, [[base + j * stride for j in range(64)] for _ in big])
+    rows = []
+    for lam in [int(x) for x in args.serving_rates.split(",") if x]:
+        rng = np.random.default_rng(5000 + lam)
+        nreq = int(max(100, min(400, lam / 32.5 * 2.0)))
+        arrivals, reqs, outs = trace(rng, lam, nreq)
+        lat, elapsed = srv.run(arrivals, reqs, outs)
+        ok = lat >= 0
+        dec_bytes = int(sum(brick_bytes[r].sum() for r in reqs))
+        ms = np.sort(lat[ok] * 1e3)
+        rows.append({"rate_bricks_per_s": lam, "requests": nreq, "bricks": int(sum(len(r) for r in reqs)),
+                     "failed": int((~ok).sum()), "p50_ms": round(float(np.percentile(ms, 50)), 3),
+                     "p99_ms": round(float(np.percentile(ms, 99)), 3), "max_ms": round(float(ms[-1]), 3),
+                     "offered_GBps": round(dec_bytes / max(arrivals[-1], 1e-9) / 1e9, 2),
+                     "decoded_GBps": round(dec_bytes / elapsed / 1e9, 2), "seconds": round(elapsed, 3)})
+        print(f"serving rate {lam}: p50 {rows[-1]['p50_ms']} ms p99 {rows[-1]['p99_ms']} ms "
+              f"{rows[-1]['decoded_GBps']} GB/s", file=sys.stderr, flush=True)
+    srv.close()
+    inst.close()
+    del pool, out
+    return {"workload": "C5 serving: %d^3 k^-5/3 field (seed 5), %d level-0 128^3 bricks (stride 120, last origin "
+                        "clamped) + %d 64^3 LOD bricks of the 2x box-downsampled field, 2-level 9/7, step "
+                        "0.005*std per level, resident in HBM; Poisson arrivals, requests of U{1..64} bricks, "
+                        "each brick LOD with p=1/2; %d slots, 32 GB decode-workspace budget"
+                        % (args.serving_n, n0, n0, args.serving_slots),
+            "latency": "request arrival to completion of its batch, observed by the host loop",
+            "compressed_bytes": compressed, "raw_bytes": 4 * nvox, "compression_ratio": round(4 * nvox / compressed, 3),
+            "psnr_db_level0": psnr[0], "psnr_db_lod": psnr[1], "setup_seconds": round(setup_s, 2), "rates": rows,
+            "p99_ms_max_over_rates": max(r["p99_ms"] for r in rows)}
 
 
 # ------------------------------------------------------------------------------------ plain volumes
@@ -284,8 +410,18 @@ def main():
     import cudacompress_amd as cc
     from cudacompress_amd import _capi
     lib = _capi.load()
+    if args.workload == "serving":
+        srv = run_serving(args, torch, cc, lib)
+        top = max(srv["rates"], key=lambda r: r["decoded_GBps"])
+        res = {"metric": METRIC, "value": top["decoded_GBps"], "unit": "GB/s", "n_gpus": 1, "steps": len(srv["rates"]),
+               "warmup": 1, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "f32", "data": "synthetic", "config": srv, "roofline": None, "cpu_baseline": None}
+        print(json.dumps(res))
+        return
     if args.workload == "bricks":
         value, ms, raw, kernels, cfg, cpu, dtype = run_bricks(args, torch, cc, lib)
+        if not args.no_serving:
+            cfg["serving"] = run_serving(args, torch, cc, lib)
     else:
         value, ms, raw, kernels, cfg, cpu, dtype = run_plain(args, torch, cc, lib)
     roof, table, gpu_ms = roofline_of(kernels, args.steps, args.workload, args.traffic_file)

@@ -71,6 +71,8 @@ SIGNATURES = {
     "cc_server_pending": (c_u32, [c_vp]),
     "cc_server_completed": (c_u32, [c_vp, P_u64, ctypes.POINTER(ctypes.c_double), c_u32]),
     "cc_server_failed": (c_u32, [c_vp, P_u64, P_u32, c_u32]),
+    "cc_server_run": (c_int, [c_vp, c_u32, ctypes.POINTER(ctypes.c_double), P_u32, P_u32, ctypes.POINTER(c_vp),
+                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "cc_server_destroy": (None, [c_vp]),
     "cc_profile_enable": (None, [c_int]),
     "cc_profile_reset": (None, []),

@@ -504,6 +504,26 @@ class Server:
                                        maxn)
         return ids[:k], stt[:k]
 
+    def run(self, arrivals, requests, outputs):
+        """Replays a request trace in real time (cc_server_run).  requests: lists of brick ids;
+        outputs: matching lists of device output pointers or tensors.  Returns (latencies in
+        seconds, -1 for failed requests; elapsed seconds to the last completion)."""
+        n = len(requests)
+        arr = np.ascontiguousarray(arrivals, dtype=np.float64)
+        cnt = np.array([len(r) for r in requests], dtype=np.uint32)
+        ids = np.ascontiguousarray(np.concatenate([np.asarray(r, dtype=np.uint32) for r in requests])
+                                   if n else np.zeros(0, np.uint32))
+        flat = [o for req in outputs for o in req]
+        outs = (ctypes.c_void_p * max(len(flat), 1))(*[int(o) if isinstance(o, (int, np.integer)) else _ptr(o)
+                                                        for o in flat])
+        lat = np.zeros(n, dtype=np.float64)
+        el = ctypes.c_double()
+        _check(self._lib.cc_server_run(self.handle, n, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                       cnt.ctypes.data_as(_capi.P_u32), ids.ctypes.data_as(_capi.P_u32), outs,
+                                       lat.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(el)),
+               "cc_server_run")
+        return lat, el.value
+
     def close(self):
         if self.handle:
             self._lib.cc_server_destroy(self.handle)

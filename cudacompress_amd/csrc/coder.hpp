@@ -128,12 +128,14 @@ struct DecCallOut {
 };
 
 // Launchers (hip/coder.hip).  scratch: arena of `scratch_cap` words.
-// Side streams for launches that may run concurrently (the table-design classes): `side` are three
-// non-blocking streams, `fork` / `join` events owned by the caller; all null -> everything on st.
+// Side streams for launches that may run concurrently (the table-design classes): `s` are
+// kSideStreams non-blocking streams, `fork` / `join` events owned by the caller; a null `side`
+// argument puts everything on st.
+constexpr int kSideStreams = 4;
 struct SideStreams {
-    hipStream_t s[3];
+    hipStream_t s[kSideStreams];
     hipEvent_t fork;
-    hipEvent_t join[3];
+    hipEvent_t join[kSideStreams];
 };
 void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uint32_t nseq, const CoderParams& p,
                   int symSize, uint32_t* scratch, uint64_t scratch_cap, CoderState* d_state, CallOut* d_calls,

@@ -27,25 +27,10 @@ constexpr uint32_t kDesignMid = 4096;
 constexpr uint32_t kWindowWords = kChunk + 4;  // LDS packing window (2048 pairs x 31 bits < 2048 words)
 
 // ------------------------------------------------------------------------ helpers
-__device__ __forceinline__ uint64_t shfl_up_u64(uint64_t v, int off) {
-    const uint32_t lo = __shfl_up((uint32_t)v, off, 64), hi = __shfl_up((uint32_t)(v >> 32), off, 64);
-    return ((uint64_t)hi << 32) | lo;
-}
-__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int off) {
-    const uint32_t lo = __shfl_xor((uint32_t)v, off, 64), hi = __shfl_xor((uint32_t)(v >> 32), off, 64);
-    return ((uint64_t)hi << 32) | lo;
-}
-__device__ __forceinline__ long long shfl_up_i64(long long v, int off) { return (long long)shfl_up_u64((uint64_t)v, off); }
-
 // Exclusive scans over a 256-thread workgroup.  sh: 4 entries of LDS.  All threads must call.
 __device__ uint64_t wg_excl_sum_u64(uint64_t v, uint64_t* sh, uint64_t& total) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint64_t x = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint64_t y = shfl_up_u64(x, off);
-        if (lane >= off) x += y;
-    }
+    const uint64_t x = wave_incl_scan_u64(v);
     if (lane == 63) sh[w] = x;
     __syncthreads();
     uint64_t base = 0, tot = 0;
@@ -61,14 +46,8 @@ __device__ uint64_t wg_excl_sum_u64(uint64_t v, uint64_t* sh, uint64_t& total) {
 }
 __device__ long long wg_excl_max_i64(long long v, long long* sh, long long& total) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    long long x = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const long long y = shfl_up_i64(x, off);
-        if (lane >= off) x = x > y ? x : y;
-    }
-    long long ex = shfl_up_i64(x, 1);
-    if (lane == 0) ex = -1;
+    const long long x = wave_incl_max_i64(v);
+    const long long ex = wave_shift_up_i64(x, -1);
     if (lane == 63) sh[w] = x;
     __syncthreads();
     long long base = -1, tot = -1;
@@ -364,56 +343,51 @@ struct WaveEx {
             __syncthreads();
         }
     }
-    // pop_by_path: the descent is resolved six levels per LDS round (lane j evaluates node j of
-    // the 63-node subtree below the current path node); lane k then holds path node k.
-    __device__ void heap_pop(HeapEntry* h, int len) const {
-        if (len <= 1) return;
+    // pop in the form of design::pop_by_walk: per round, lane j < 63 evaluates the child choice
+    // of local node j of the six-level subtree below the round's root (one LDS read of both
+    // children's counts); the walk over the ballot of choices runs on the scalar unit; lane k ends
+    // up holding path node k (mypath) and path node k+1 (mynext).  Returns the new top.
+    __device__ HeapEntry heap_pop(HeapEntry* h, int len) const {
         const HeapEntry v = h[len - 1];
+        if (len <= 1) return v;
         const int n = len - 1, ln = lane();
         const int lim = (n - 1) / 2, lastint = (n - 2) / 2;
-        const bool even = (n & 1) == 0;
-        const int sd = ln < 63 ? 31 - __clz(ln + 1) : 0;  // depth of this lane's node in the subtree
-        const int so = ln + 1 - (1 << sd);
-        int mypath = 0, m = 0, cur = 0;
+        const int sd = ln < 63 ? 31 - __clz(ln + 1) : 0;  // depth of this lane's local node
+        int mypath = 0, mynext = 0, base = 0, cur = 0, m;
         for (;;) {
-            bool two = false, right = false, only = false;
-            if (ln < 63) {
-                const int g = ((cur + 1) << sd) - 1 + so;
-                two = g < lim;
-                if (two) right = !(h[2 * g + 1].count < h[2 * g + 2].count);
-                only = even && g == lastint;
-            }
-            const uint64_t mt = __ballot(two), mr = __ballot(right), mo = __ballot(only);
-            int j = 0, gnode = cur;
-            bool done = false;
+            const int g = (cur << sd) + ln;
+            bool right = false;
+            if (ln < 63 && g < lim) right = !(h[2 * g + 1].count < h[2 * g + 2].count);
+            const uint64_t mr = __ballot(right);
+            uint32_t j = 0;
+#pragma unroll
             for (int d = 0; d < 6; d++) {
-                if ((mt >> j) & 1ull) {
-                    const int r = (int)((mr >> j) & 1ull);
-                    gnode = 2 * gnode + 1 + r;
-                    j = 2 * j + 1 + r;
-                    m++;
-                    if (ln == m) mypath = gnode;
-                } else {
-                    if ((mo >> j) & 1ull) {
-                        gnode = 2 * gnode + 1;
-                        m++;
-                        if (ln == m) mypath = gnode;
-                    }
-                    done = true;
-                    break;
-                }
+                j = 2 * j + 1 + (uint32_t)((mr >> j) & 1ull);
+                const int gn = (cur << (d + 1)) + (int)j;
+                if (ln == base + d + 1) mypath = gn;
+                if (ln == base + d) mynext = gn;
             }
-            if (done) break;
-            cur = gnode;
+            const bool real = ln > base && ln <= base + 6 && ((mypath - 1) >> 1) < lim;
+            const int steps = (int)__popcll(__ballot(real));
+            if (steps < 6) {
+                m = base + steps;
+                if ((n & 1) == 0 && __builtin_amdgcn_readlane(mypath, m) == lastint) m++;
+                break;
+            }
+            base += 6;
+            cur = __builtin_amdgcn_readlane(mypath, base);
         }
-        const HeapEntry pv = ln <= m ? h[mypath] : HeapEntry{0u, 0u};
-        const int is = m - (int)__popcll(__ballot(ln >= 1 && ln <= m && v.count < pv.count));
-        const uint32_t nc = __shfl_down(pv.count, 1, 64), nn = __shfl_down(pv.node, 1, 64);
-        if (ln < is) h[mypath] = HeapEntry{nc, nn};
+        // lanes 0..m-1 read their successor; v lands at i* = m - #{k in 1..m : v.count < count(p_k)}
+        const HeapEntry pn = ln < m ? h[mynext] : HeapEntry{0u, 0u};
+        const int is = m - (int)__popcll(__ballot(ln < m && v.count < pn.count));
+        if (ln < is) h[mypath] = pn;
         else if (ln == is) h[mypath] = v;
+        if (is == 0) return v;
+        return HeapEntry{(uint32_t)__builtin_amdgcn_readlane((int)pn.count, 0),
+                         (uint32_t)__builtin_amdgcn_readlane((int)pn.node, 0)};
     }
-    // push_by_chain: lane t holds ancestor t of the new position
-    __device__ void heap_push(HeapEntry* h, int pos, HeapEntry v) const {
+    // push_by_chain: lane t holds ancestor t of the new position; returns the new top
+    __device__ HeapEntry heap_push(HeapEntry* h, int pos, HeapEntry v, HeapEntry top) const {
         const int ln = lane();
         const int T = 31 - __clz(pos + 1);
         const int c = ((pos + 1) >> ln) - 1;
@@ -422,6 +396,7 @@ struct WaveEx {
         const int j = (int)__popcll(__ballot(anc && v.count < e.count));
         if (ln >= 1 && ln <= j) h[((pos + 1) >> (ln - 1)) - 1] = e;
         if (ln == j) h[c] = v;
+        return j == T ? v : top;
     }
     template <class P, class Sk>
     __device__ int compact(int lo, int hi, P pred, Sk sink) const {
@@ -445,11 +420,7 @@ struct WaveEx {
         return total;
     }
     __device__ int max_all(int v) const { return (int)wave_max((uint32_t)v); }
-    __device__ uint64_t sum_all(uint64_t v) const {
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v += shfl_xor_u64(v, off);
-        return v;
-    }
+    __device__ uint64_t sum_all(uint64_t v) const { return wave_sum_u64(v); }
     template <class F>
     __device__ void stable_rank_by_depth(const uint32_t* el, int K, uint32_t* cpl, F f) const {
         // lane d-1 counts depth d (depths are 1..31)
@@ -461,7 +432,7 @@ struct WaveEx {
             uint64_t rem = __ballot(valid);
             while (rem) {
                 const int leader = __ffsll((unsigned long long)rem) - 1;
-                const uint32_t d = __shfl(key, leader, 64);
+                const uint32_t d = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
                 const uint64_t m = __ballot(valid && key == d);
                 if (lane() == (int)d - 1) mycount += (uint32_t)__popcll(m);
                 rem &= ~m;
@@ -481,9 +452,9 @@ struct WaveEx {
             uint64_t rem = __ballot(valid);
             while (rem) {
                 const int leader = __ffsll((unsigned long long)rem) - 1;
-                const uint32_t d = __shfl(key, leader, 64);
+                const uint32_t d = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
                 const uint64_t m = __ballot(valid && key == d);
-                const uint32_t b = __shfl(seen, (int)d - 1, 64);
+                const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)seen, (int)d - 1);
                 if (valid && key == d) f(i, e, b + (uint32_t)__popcll(m & lanes_below(lane())), fi, fc);
                 if (lane() == (int)d - 1) seen += (uint32_t)__popcll(m);
                 rem &= ~m;
@@ -492,11 +463,13 @@ struct WaveEx {
     }
 };
 
-// One wave per coded sequence whose distinct-symbol count lies in [kmin, kmax].  gws: the design
-// workspace comes from the arena instead of LDS.
-template <class Idx>
+// One wave per coded sequence whose distinct-symbol count lies in [kmin, kmax].  Gws: the design
+// workspace comes from the arena instead of LDS (a compile-time choice, so that the LDS classes
+// address their workspace with LDS instructions rather than generic ones).
+template <class Idx, bool Gws>
 __global__ __launch_bounds__(64) void k_design(SeqInfo* seqs, CoderParams p, uint32_t* scratch, uint64_t cap,
-                                               CoderState* st, uint32_t kmin, uint32_t kmax, int gws) {
+                                               CoderState* st, uint32_t kmin, uint32_t kmax) {
+    constexpr bool gws = Gws;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ Ctl ctl;
     __shared__ uint64_t offs[4];
@@ -505,6 +478,9 @@ __global__ __launch_bounds__(64) void k_design(SeqInfo* seqs, CoderParams p, uin
     if (s->err) return;
     const uint32_t K = s->nsym;
     if (K < kmin || K > kmax) return;
+    // the largest tables set the length of the design phase: their waves take the SIMD's issue
+    // slots first when they share it with the many small designs running beside them
+    if (K > 1024) __builtin_amdgcn_s_setprio(2);
     const uint32_t nb = s->smax + 1;
     const uint64_t hreg = design::heap_region_bytes<Idx>(K);
     const uint64_t preg = ((uint64_t)sizeof(Idx) * (2ull * K + 2) + 15) & ~uint64_t(15);
@@ -517,7 +493,9 @@ __global__ __launch_bounds__(64) void k_design(SeqInfo* seqs, CoderParams p, uin
     }
     __syncthreads();
     if (offs[0] == ~uint64_t(0) || offs[1] == ~uint64_t(0) || offs[2] == ~uint64_t(0)) return;
-    unsigned char* ws = gws ? reinterpret_cast<unsigned char*>(scratch + offs[2]) : smem;
+    unsigned char* ws;
+    if constexpr (Gws) ws = reinterpret_cast<unsigned char*>(scratch + offs[2]);
+    else ws = smem;
     Work<Idx> w{reinterpret_cast<HeapEntry*>(ws), reinterpret_cast<Idx*>(ws + hreg),
                 reinterpret_cast<int*>(ws + hreg + preg)};
     uint32_t info[3];
@@ -720,11 +698,7 @@ __global__ __launch_bounds__(1024) void k_layout(const CodeItem* items, uint32_t
         }
         // block exclusive scan over 16 waves
         const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-        uint64_t x = words;
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint64_t y = shfl_up_u64(x, off);
-            if (lane >= off) x += y;
-        }
+        const uint64_t x = wave_incl_scan_u64(words);
         if (lane == 63) sh[w] = x;
         __syncthreads();
         uint64_t b = 0, tot = 0;
@@ -856,28 +830,31 @@ void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uin
     CCA_LAUNCH_CHECK();
     // Design classes by distinct-symbol count (small LDS footprints keep many tables resident).
     // The classes touch disjoint sequences, so they run concurrently on side streams when given.
-    const uint32_t classes[3] = {256, 1024, kDesignMid};
-    const char* names[3] = {"huff_design_k256", "huff_design_k1024", "huff_design_k4096"};
+    constexpr int kClasses = 4;
+    static_assert(kClasses <= kSideStreams, "one side stream per design class");
+    const uint32_t classes[kClasses] = {256, 1024, 2048, kDesignMid};
+    const char* names[kClasses] = {"huff_design_k256", "huff_design_k1024", "huff_design_k2048",
+                                   "huff_design_k4096"};
     if (side) {
         CCA_HIP(hipEventRecord(side->fork, st));
-        for (int c = 0; c < 3; c++) CCA_HIP(hipStreamWaitEvent(side->s[c], side->fork, 0));
+        for (int c = 0; c < kClasses; c++) CCA_HIP(hipStreamWaitEvent(side->s[c], side->fork, 0));
     }
     uint32_t kmin = 0;
-    for (int c = 0; c < 3; c++) {
+    for (int c = 0; c < kClasses; c++) {
         const uint32_t kmax = classes[c];
         const size_t lds = design::heap_region_bytes<uint16_t>(kmax) + 2 * (2 * kmax + 2) + 16 +
                            12 * design::kStackRanges + 16;
         hipStream_t sc = side ? side->s[c] : st;
-        CCA_LAUNCH(names[c], 0, k_design<uint16_t>, dim3(nseq), dim3(64), lds, sc, d_seqs, p, scratch, scratch_cap,
-                   d_state, kmin, kmax, 0);
+        CCA_LAUNCH(names[c], 0, (k_design<uint16_t, false>), dim3(nseq), dim3(64), lds, sc, d_seqs, p, scratch,
+                   scratch_cap, d_state, kmin, kmax);
         CCA_LAUNCH_CHECK();
         kmin = kmax + 1;
     }
-    CCA_LAUNCH("huff_design_global", 0, k_design<uint32_t>, dim3(nseq), dim3(64), 0, st, d_seqs, p, scratch,
-               scratch_cap, d_state, kDesignMid + 1, 0xFFFFFFFFu, 1);
+    CCA_LAUNCH("huff_design_global", 0, (k_design<uint32_t, true>), dim3(nseq), dim3(64), 0, st, d_seqs, p, scratch,
+               scratch_cap, d_state, kDesignMid + 1, 0xFFFFFFFFu);
     CCA_LAUNCH_CHECK();
     if (side) {
-        for (int c = 0; c < 3; c++) {
+        for (int c = 0; c < kClasses; c++) {
             CCA_HIP(hipEventRecord(side->join[c], side->s[c]));
             CCA_HIP(hipStreamWaitEvent(st, side->join[c], 0));
         }

@@ -239,7 +239,7 @@ __global__ __launch_bounds__(64) void k_dec_blocks(const DecSeq* seqs, const uin
                                                    const uint32_t* scratch) {
     __shared__ uint32_t lut[1 << kLut];
     __shared__ int tab[96];
-    __shared__ uint32_t stage[64][33];
+    __shared__ S stage[64][33];  // odd row length: the per-lane column writes spread over the banks
     const DecSeq& s = seqs[wave_seq[blockIdx.x]];
     if (s.err) return;
     const uint32_t cbs = p.cbs;
@@ -265,15 +265,18 @@ __global__ __launch_bounds__(64) void k_dec_blocks(const DecSeq* seqs, const uin
     const uint32_t* words = s.base + s.words_off;
     const uint64_t avail = s.nwords - s.words_off;  // readable words from the codeword start
     auto word = [&](uint64_t i) -> uint32_t { return i < avail ? words[i] : 0u; };
+    // bit window: `cache` holds 64 bits starting at the current word, `pf` the word after them,
+    // loaded one refill ahead so that its latency overlaps the decoding of the cached bits
     uint64_t cache = 0;
-    uint32_t bp = 0;
+    uint32_t bp = 0, pf = 0;
     uint64_t nxt = 0;
     if (valid && multi) {
         const uint32_t pos = b == 0 ? 0u : scratch[s.boff_off + b];
         const uint64_t wi = pos >> 5;
         cache = ((uint64_t)word(wi) << 32) | (uint64_t)word(wi + 1);
+        pf = word(wi + 2);
         bp = pos & 31u;
-        nxt = wi + 2;
+        nxt = wi + 3;
     }
     const uint32_t mlen = s.nlen;
     for (uint32_t r = 0; r < cbs; r += 32) {
@@ -282,7 +285,8 @@ __global__ __launch_bounds__(64) void k_dec_blocks(const DecSeq* seqs, const uin
             uint32_t sym;
             if (multi) {
                 if (bp >= 32) {
-                    cache = (cache << 32) | (uint64_t)word(nxt);
+                    cache = (cache << 32) | (uint64_t)pf;
+                    pf = word(nxt);
                     nxt++;
                     bp -= 32;
                 }
@@ -302,7 +306,7 @@ __global__ __launch_bounds__(64) void k_dec_blocks(const DecSeq* seqs, const uin
             } else {
                 sym = single;
             }
-            stage[lane][i - r] = sym;
+            stage[lane][i - r] = (S)sym;
         }
         __syncthreads();
         for (uint32_t idx = lane; idx < 64 * 32; idx += 64) {
@@ -311,18 +315,36 @@ __global__ __launch_bounds__(64) void k_dec_blocks(const DecSeq* seqs, const uin
             if (gb < nblk) {
                 const uint32_t pos = gb * cbs + r + k;
                 const uint32_t bend = min((gb + 1) * cbs, s.count);
-                if (pos < bend) out[pos] = (S)stage[blk][k];
+                if (pos < bend) out[pos] = stage[blk][k];
             }
         }
         __syncthreads();
     }
 }
 
-// One workgroup per RLE item: output position of pair i = sum_{j<i}(z_j + 1) + z_i; zeros fill
-// every position no pair names.
+// One workgroup per RLE item.  The output is cleared first with 16-byte coalesced stores; then
+// pair i (value v_i after z_i zeros) lands at sum_{j<i}(z_j + 1) + z_i.  Pairs are taken in chunks
+// of 2048 (8 consecutive per thread for the scan); their positions and values are staged in LDS
+// and stored with a 256-pair stride, so one store instruction writes 64 consecutive pairs.
+template <class S>
+__device__ __forceinline__ void clear_symbols(S* out, uint32_t n) {
+    const uintptr_t a = reinterpret_cast<uintptr_t>(out);
+    constexpr uint32_t per = 16 / sizeof(S);
+    uint32_t head = (uint32_t)(((16 - (a & 15)) & 15) / sizeof(S));
+    if ((a % sizeof(S)) != 0 || head > n) head = n;
+    for (uint32_t i = threadIdx.x; i < head; i += blockDim.x) out[i] = (S)0;
+    const uint32_t nv = (n - head) / per;
+    uint4* v = reinterpret_cast<uint4*>(out + head);
+    for (uint32_t j = threadIdx.x; j < nv; j += blockDim.x) v[j] = make_uint4(0u, 0u, 0u, 0u);
+    for (uint32_t i = head + nv * per + threadIdx.x; i < n; i += blockDim.x) out[i] = (S)0;
+}
+
 template <class S>
 __global__ __launch_bounds__(256) void k_dec_rle(const DecItem* items, DecSeq* seqs, CoderState* st) {
+    constexpr uint32_t kChunk = 256 * 8;
     __shared__ uint32_t sh[4];
+    __shared__ uint32_t spos[kChunk];
+    __shared__ uint32_t sval[kChunk];
     const DecItem it = items[blockIdx.x];
     if (!it.rl) return;
     const DecSeq& ss = seqs[it.seq];
@@ -332,9 +354,16 @@ __global__ __launch_bounds__(256) void k_dec_rle(const DecItem* items, DecSeq* s
     const S* sym = static_cast<const S*>(ss.out);
     const S* zc = static_cast<const S*>(sz.out);
     S* out = static_cast<S*>(it.out);
+    clear_symbols(out, n);
+    __syncthreads();  // the clearing stores complete before values land on them
+    // LDS index swizzle: thread t's 8 consecutive entries and the strided reads are both
+    // conflict-free across the 64 banks
+    auto sw = [](uint32_t i) { return i ^ ((i >> 6) & 7u); };
     uint64_t carry = 0;
-    for (uint32_t base = 0; base < M; base += 256 * 8) {
+    bool bad = false;
+    for (uint32_t base = 0; base < M; base += kChunk) {
         uint32_t z[8], v[8], sum = 0;
+#pragma unroll
         for (int k = 0; k < 8; k++) {
             const uint32_t i = base + threadIdx.x * 8 + k;
             z[k] = i < M ? (uint32_t)zc[i] : 0u;
@@ -344,23 +373,30 @@ __global__ __launch_bounds__(256) void k_dec_rle(const DecItem* items, DecSeq* s
         uint32_t tot;
         const uint32_t ex = block_excl_scan<256>(sum, sh, tot);
         uint64_t pos = carry + ex;
+#pragma unroll
         for (int k = 0; k < 8; k++) {
-            const uint32_t i = base + threadIdx.x * 8 + k;
-            if (i >= M) break;
-            for (uint32_t j = 0; j < z[k]; j++)
-                if (pos + j < n) out[pos + j] = (S)0;
             pos += z[k];
-            if (pos < n) {
-                out[pos] = (S)v[k];
-            } else {
-                seqs[it.seq].err = kDecCorrupt;
-                atomicCAS(&st->err, 0u, (uint32_t)kDecCorrupt);
-            }
+            const uint32_t e = sw(threadIdx.x * 8 + k);
+            const bool inb = pos < n;
+            bad |= !inb && base + threadIdx.x * 8 + k < M;
+            spos[e] = inb ? (uint32_t)pos : 0xFFFFFFFFu;
+            sval[e] = v[k];
             pos++;
         }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const uint32_t idx = k * 256 + threadIdx.x;
+            const uint32_t q = spos[sw(idx)];
+            if (base + idx < M && q != 0xFFFFFFFFu) out[q] = (S)sval[sw(idx)];
+        }
+        __syncthreads();
         carry += tot;
     }
-    for (uint64_t i = carry + threadIdx.x; i < n; i += 256) out[i] = (S)0;
+    if (bad) {
+        seqs[it.seq].err = kDecCorrupt;
+        atomicCAS(&st->err, 0u, (uint32_t)kDecCorrupt);
+    }
 }
 
 // Per-call status: the first error among the call's coded sequences.

@@ -351,19 +351,21 @@ CCA_HD void design_table(Ex& ex, const uint32_t* hist, uint32_t nbins, int diale
         if (dialect == kDialectGpu) {
             ex.heap_make(w.heap, K);
         } else {
-            for (int i = 1; i < K; i++) ex.heap_push(w.heap, i, w.heap[i]);
+            for (int i = 1; i < K; i++) ex.heap_push(w.heap, i, w.heap[i], HeapEntry{0u, 0u});
         }
         int size = K, next = K;
+        ex.barrier();
+        HeapEntry top = w.heap[0];  // the executor's pop and push return the new top
         while (size > 1) {
-            const HeapEntry a = w.heap[0];
-            ex.heap_pop(w.heap, size--);
-            const HeapEntry b = w.heap[0];
-            ex.heap_pop(w.heap, size--);
+            const HeapEntry a = top;
+            top = ex.heap_pop(w.heap, size--);
+            const HeapEntry b = top;
+            top = ex.heap_pop(w.heap, size--);
             if (lane == 0) {
                 w.parent[a.node] = (Idx)next;
                 w.parent[b.node] = (Idx)next;
             }
-            ex.heap_push(w.heap, size++, HeapEntry{a.count + b.count, (uint32_t)next});
+            top = ex.heap_push(w.heap, size++, HeapEntry{a.count + b.count, (uint32_t)next}, top);
             next++;
         }
         if (lane == 0) {
@@ -449,6 +451,50 @@ CCA_HD int descend_path(const HeapEntry* h, int len, int start, int* path) {
     return m;
 }
 
+// The descent in the wave's form: rounds of six levels below the round's root `cur`.  Inside a
+// round the walk goes from local node j (0 = cur; global index (cur << depth) + j) to local child
+// 2j + 1 + r(j), with r(j) = 1 iff the node has two children and its left child's count is not
+// smaller -- unconditionally, for all six levels.  The real path positions are those whose parent
+// has two children, a prefix of the walk (indices grow along it, and a node has two children iff
+// its index is below (n-1)/2).  Where the path ends at the last internal node of an even-sized
+// heap, the walk's next step (r = 0 there) is that node's lone left child.  Same path as
+// descend_path; path needs room for the last round's six positions.
+CCA_HD int descend_walk(const HeapEntry* h, int n, int* path) {
+    const int lim = (n - 1) / 2, lastint = (n - 2) / 2;
+    int base = 0, cur = 0;
+    path[0] = 0;
+    for (;;) {
+        int j = 0;
+        for (int d = 0; d < 6; d++) {
+            const int g = (cur << d) + j;
+            const int r = (g < lim && !(h[2 * g + 1].count < h[2 * g + 2].count)) ? 1 : 0;
+            j = 2 * j + 1 + r;
+            path[base + d + 1] = (cur << (d + 1)) + j;
+        }
+        int s = 0;
+        while (s < 6 && ((path[base + s + 1] - 1) >> 1) < lim) s++;
+        if (s < 6) {
+            int m = base + s;
+            if ((n & 1) == 0 && path[m] == lastint) m++;
+            return m;
+        }
+        base += 6;
+        cur = path[base];
+    }
+}
+
+// pop over h[0, len): afterwards h[0, len-1) is the heap pop_heap leaves (h[len-1] not written)
+CCA_HD void pop_by_walk(HeapEntry* h, int len, int* path) {
+    if (len <= 1) return;
+    const HeapEntry v = h[len - 1];
+    const int m = descend_walk(h, len - 1, path);
+    int larger = 0;
+    for (int k = 1; k <= m; k++) larger += (v.count < h[path[k]].count) ? 1 : 0;
+    const int is = m - larger;
+    for (int i = 0; i < is; i++) h[path[i]] = h[path[i + 1]];
+    h[path[is]] = v;
+}
+
 // pop over h[0, len): afterwards h[0, len-1) is the heap pop_heap leaves (h[len-1] not written)
 CCA_HD void pop_by_path(HeapEntry* h, int len, int* path) {
     if (len <= 1) return;
@@ -488,10 +534,15 @@ struct SerialEx {
     CCA_HD int lanes() const { return 1; }
     CCA_HD void barrier() const {}
     CCA_HD void heap_make(HeapEntry* h, int len) const { make_heap(h, len, QueueLess()); }
-    CCA_HD void heap_pop(HeapEntry* h, int len) const { pop_heap(h, len, QueueLess()); }
-    CCA_HD void heap_push(HeapEntry* h, int pos, HeapEntry v) const {
+    // pop / push return the new top (h[0])
+    CCA_HD HeapEntry heap_pop(HeapEntry* h, int len) const {
+        pop_heap(h, len, QueueLess());
+        return h[0];
+    }
+    CCA_HD HeapEntry heap_push(HeapEntry* h, int pos, HeapEntry v, HeapEntry) const {
         h[pos] = v;
         push_heap(h, pos + 1, QueueLess());
+        return h[0];
     }
     template <class P, class S>
     CCA_HD int compact(int lo, int hi, P pred, S sink) const {
@@ -525,10 +576,24 @@ struct SerialEx {
 // Serial executor using the path formulations of the heap operations (what the wave executor
 // computes in parallel); the host check runs both executors against the standard library.
 struct PathEx : SerialEx {
-    mutable int path[64];
+    mutable int path[96];
     CCA_HD void heap_make(HeapEntry* h, int len) const { make_by_levels(h, len); }
-    CCA_HD void heap_pop(HeapEntry* h, int len) const { pop_by_path(h, len, path); }
-    CCA_HD void heap_push(HeapEntry* h, int pos, HeapEntry v) const { push_by_chain(h, pos, v); }
+    CCA_HD HeapEntry heap_pop(HeapEntry* h, int len) const {
+        pop_by_path(h, len, path);
+        return h[0];
+    }
+    CCA_HD HeapEntry heap_push(HeapEntry* h, int pos, HeapEntry v, HeapEntry) const {
+        push_by_chain(h, pos, v);
+        return h[0];
+    }
+};
+
+// Serial executor using the wave's six-level walk for the descent.
+struct WalkEx : PathEx {
+    CCA_HD HeapEntry heap_pop(HeapEntry* h, int len) const {
+        pop_by_walk(h, len, path);
+        return h[0];
+    }
 };
 
 }  // namespace design

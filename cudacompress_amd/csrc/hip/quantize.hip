@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void k_unquantize_short(float* __restrict__ da
     data[i.x + (size_t)i.y * row + (size_t)i.z * slice] = unquantize_i(q[i.x + (size_t)i.y * sx + (size_t)i.z * sx * sy], step, qtype);
 }
 
-// max |v| over n floats: grid-stride, wave64 reduction through DPP-backed shuffles, one
+// max |v| over n floats: grid-stride, wave64 reduction through lane shuffles, one
 // atomicMax on the float's bit pattern per workgroup (non-negative floats order as integers).
 __global__ __launch_bounds__(256) void k_max_abs(const float* __restrict__ v, uint32_t n, uint32_t* __restrict__ out) {
     float m = 0.0f;

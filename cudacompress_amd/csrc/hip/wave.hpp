@@ -1,5 +1,5 @@
 // wave64 / workgroup primitives for gfx950 (scan, reduce, ballot helpers).
-// Written from the standard Kogge-Stone formulation; the shuffles lower to DPP / ds_bpermute.
+// Written from the standard Kogge-Stone formulation on DPP lane moves.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -10,24 +10,67 @@ namespace cca {
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 __device__ __forceinline__ uint64_t lanes_below(int lane) { return (1ull << lane) - 1ull; }
 
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-    const int lane = lane_id();
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t t = __shfl_up(v, off, 64);
-        if (lane >= off) v += t;
+// Inclusive wave scans on DPP moves (GFX9 data-parallel primitives): Kogge-Stone inside each row
+// of 16 lanes (row_shr 1, 2, 4, 8), then row_bcast:15 carries row 0 into row 1 and row 2 into
+// row 3, and row_bcast:31 carries lanes 0-31 into rows 2-3.  Lanes that receive nothing in a step
+// combine with the identity (`old` operand, bound_ctrl off).
+namespace dpp {
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
+constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143, kWaveShr1 = 0x138;
+
+template <int Ctl, int RowMask = 0xf>
+__device__ __forceinline__ uint32_t mov(uint32_t identity, uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)identity, (int)v, Ctl, RowMask, 0xf, false);
+}
+template <int Ctl, int RowMask = 0xf>
+__device__ __forceinline__ uint64_t mov64(uint64_t identity, uint64_t v) {
+    const uint32_t lo = mov<Ctl, RowMask>((uint32_t)identity, (uint32_t)v);
+    const uint32_t hi = mov<Ctl, RowMask>((uint32_t)(identity >> 32), (uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+template <class T, class Op>
+__device__ __forceinline__ T incl_scan(T v, T identity, Op op) {
+    if constexpr (sizeof(T) == 8) {
+        v = op(v, (T)mov64<kRowShr1>((uint64_t)identity, (uint64_t)v));
+        v = op(v, (T)mov64<kRowShr2>((uint64_t)identity, (uint64_t)v));
+        v = op(v, (T)mov64<kRowShr4>((uint64_t)identity, (uint64_t)v));
+        v = op(v, (T)mov64<kRowShr8>((uint64_t)identity, (uint64_t)v));
+        v = op(v, (T)mov64<kRowBcast15, 0xa>((uint64_t)identity, (uint64_t)v));
+        v = op(v, (T)mov64<kRowBcast31, 0xc>((uint64_t)identity, (uint64_t)v));
+    } else {
+        v = op(v, (T)mov<kRowShr1>((uint32_t)identity, (uint32_t)v));
+        v = op(v, (T)mov<kRowShr2>((uint32_t)identity, (uint32_t)v));
+        v = op(v, (T)mov<kRowShr4>((uint32_t)identity, (uint32_t)v));
+        v = op(v, (T)mov<kRowShr8>((uint32_t)identity, (uint32_t)v));
+        v = op(v, (T)mov<kRowBcast15, 0xa>((uint32_t)identity, (uint32_t)v));
+        v = op(v, (T)mov<kRowBcast31, 0xc>((uint32_t)identity, (uint32_t)v));
     }
     return v;
 }
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+// the value of lane 63 on every lane
+__device__ __forceinline__ uint32_t last_lane(uint32_t v) { return (uint32_t)__builtin_amdgcn_readlane((int)v, 63); }
+__device__ __forceinline__ uint64_t last_lane64(uint64_t v) {
+    return ((uint64_t)last_lane((uint32_t)(v >> 32)) << 32) | last_lane((uint32_t)v);
 }
+}  // namespace dpp
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    return dpp::incl_scan<uint32_t>(v, 0u, [](uint32_t a, uint32_t b) { return a + b; });
+}
+__device__ __forceinline__ uint64_t wave_incl_scan_u64(uint64_t v) {
+    return dpp::incl_scan<uint64_t>(v, 0ull, [](uint64_t a, uint64_t b) { return a + b; });
+}
+__device__ __forceinline__ long long wave_incl_max_i64(long long v) {
+    return dpp::incl_scan<long long>(v, -1ll, [](long long a, long long b) { return a > b ? a : b; });
+}
+// the previous lane's value (lane 0: identity)
+__device__ __forceinline__ long long wave_shift_up_i64(long long v, long long identity) {
+    return (long long)dpp::mov64<dpp::kWaveShr1>((uint64_t)identity, (uint64_t)v);
+}
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) { return dpp::last_lane(wave_incl_scan(v)); }
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) { return dpp::last_lane64(wave_incl_scan_u64(v)); }
 __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor(v, off, 64));
-    return v;
+    return dpp::last_lane(dpp::incl_scan<uint32_t>(v, 0u, [](uint32_t a, uint32_t b) { return a > b ? a : b; }));
 }
 
 // Exclusive scan over a workgroup of NT threads (NT multiple of 64).  `total` receives the sum.

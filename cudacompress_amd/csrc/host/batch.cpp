@@ -31,13 +31,13 @@ CoderParams params_of(const cc_instance* I) {
 // Side streams of a coder context (created on first use, on the current device).
 const SideStreams* side_streams(DeviceCoder& D, SideStreams& out) {
     if (!D.fork) {
-        for (int i = 0; i < 3; i++) {
+        for (int i = 0; i < kSideStreams; i++) {
             CCA_HIP(hipStreamCreateWithFlags(&D.side[i], hipStreamNonBlocking));
             CCA_HIP(hipEventCreateWithFlags(&D.join[i], hipEventDisableTiming));
         }
         CCA_HIP(hipEventCreateWithFlags(&D.fork, hipEventDisableTiming));
     }
-    for (int i = 0; i < 3; i++) {
+    for (int i = 0; i < kSideStreams; i++) {
         out.s[i] = D.side[i];
         out.join[i] = D.join[i];
     }

@@ -370,4 +370,6 @@ void plan_decompress(cc_bricks_plan* P, uint32_t* status) {
                 "some bricks failed to decode (see the per-brick status)");
 }
 
+void plan_destroy(cc_bricks_plan* P) { delete P; }
+
 }  // namespace cca

@@ -228,7 +228,7 @@ cc_status cc_bricks_plan_decompress(cc_bricks_plan* plan, uint32_t* brickStatus)
     });
 }
 
-void cc_bricks_plan_destroy(cc_bricks_plan* plan) { delete plan; }
+void cc_bricks_plan_destroy(cc_bricks_plan* plan) { cca::plan_destroy(plan); }
 
 cc_status cc_server_create(cc_instance* inst, const uint32_t* d_words, uint32_t brickCount,
                            const uint64_t* brickWordRanges, const uint32_t* brickSizes, const float* quantSteps,
@@ -268,7 +268,15 @@ uint32_t cc_server_failed(cc_server* server, uint64_t* requestIds, uint32_t* sta
     return server ? cca::server_take_failed(server, requestIds, status, max) : 0u;
 }
 
-void cc_server_destroy(cc_server* server) { delete server; }
+cc_status cc_server_run(cc_server* server, uint32_t requestCount, const double* arrivals, const uint32_t* counts,
+                        const uint32_t* brickIds, float* const* d_outputs, double* latencies, double* elapsed) {
+    return guarded([&] {
+        CCA_REQUIRE(server, CC_ERR_INVALID_ARG, "null server");
+        cca::server_run(server, requestCount, arrivals, counts, brickIds, d_outputs, latencies, elapsed);
+    });
+}
+
+void cc_server_destroy(cc_server* server) { cca::server_destroy(server); }
 
 cc_status cc_encode_rlhuff(cc_instance* inst, int symbolSize, const void* const* d_symbolStreams, uint32_t streamCount,
                            uint32_t symbolCountPerStream, const uint32_t* startBitOffsets, const uint32_t** outWords,

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <vector>
 
+#include "../coder.hpp"
 #include "../common.hpp"
 
 namespace cca {
@@ -47,8 +48,8 @@ struct DeviceCoder {
     DevBuf d_dcalls, d_ditems, d_dseqs, d_dscratch, d_dstate, d_dout, d_waves;
     PinBuf h_dstage, h_dresult;
     // side streams for independent launches (default priority, non-blocking) and their events
-    hipStream_t side[3] = {nullptr, nullptr, nullptr};
-    hipEvent_t fork = nullptr, join[3] = {nullptr, nullptr, nullptr};
+    hipStream_t side[kSideStreams] = {};
+    hipEvent_t fork = nullptr, join[kSideStreams] = {};
     uint64_t scratch_cap = 0;  // words, excluding the item-offset tail
     void release();  // frees buffers, streams and events (device must be current)
 };

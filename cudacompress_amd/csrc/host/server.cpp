@@ -10,6 +10,7 @@
 #include "server.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <deque>
 
 #include "bricks.hpp"
@@ -188,6 +189,8 @@ uint32_t server_poll(cc_server* S, double now) {
     return launched;
 }
 
+void server_destroy(cc_server* S) { delete S; }
+
 uint32_t server_busy(const cc_server* S) {
     uint32_t b = 0;
     for (const Slot& s : S->slots) b += s.busy ? 1 : 0;
@@ -213,6 +216,50 @@ uint32_t server_take_failed(cc_server* S, uint64_t* ids, uint32_t* status, uint3
     }
     S->failed.erase(S->failed.begin(), S->failed.begin() + n);
     return n;
+}
+
+void server_run(cc_server* S, uint32_t nreq, const double* arrivals, const uint32_t* counts, const uint32_t* ids,
+                float* const* outs, double* latency, double* elapsed) {
+    CCA_REQUIRE(server_busy(S) == 0 && S->completed.empty() && S->failed.empty(), CC_ERR_INVALID_ARG,
+                "the server must be idle with no unclaimed results");
+    CCA_REQUIRE(nreq == 0 || (arrivals && counts && ids && outs && latency), CC_ERR_INVALID_ARG, "null argument");
+    std::vector<uint64_t> first(nreq + 1, 0);
+    for (uint32_t r = 0; r < nreq; r++) {
+        CCA_REQUIRE(r == 0 || arrivals[r] >= arrivals[r - 1], CC_ERR_INVALID_ARG, "arrivals must be sorted");
+        CCA_REQUIRE(counts[r] >= 1, CC_ERR_INVALID_ARG, "a request names at least one brick");
+        first[r + 1] = first[r] + counts[r];
+    }
+    // the whole trace is checked before anything is submitted
+    for (uint64_t k = 0; k < first[nreq]; k++) {
+        CCA_REQUIRE(ids[k] < S->begin.size(), CC_ERR_INVALID_ARG, "brick id out of range");
+        CCA_REQUIRE(outs[k], CC_ERR_INVALID_ARG, "null output brick");
+    }
+    for (uint32_t r = 0; r < nreq; r++) latency[r] = -1.0;
+    using clk = std::chrono::steady_clock;
+    const clk::time_point t0 = clk::now();
+    auto now = [&] { return std::chrono::duration<double>(clk::now() - t0).count(); };
+    uint32_t next = 0, done = 0;
+    double t_last = 0.0;
+    while (done < nreq) {
+        double t = now();
+        while (next < nreq && arrivals[next] <= t) {
+            server_submit(S, next, ids + first[next], counts[next], outs + first[next], arrivals[next]);
+            next++;
+        }
+        server_poll(S, now());
+        for (const auto& c : S->completed) {
+            latency[c.first] = c.second - arrivals[c.first];
+            t_last = std::max(t_last, c.second);
+            done++;
+        }
+        S->completed.clear();
+        for (const auto& f : S->failed) {
+            latency[f.first] = -1.0;
+            done++;
+        }
+        S->failed.clear();
+    }
+    if (elapsed) *elapsed = t_last;
 }
 
 }  // namespace cca

@@ -16,9 +16,14 @@ cc_server* server_create(cc_instance* I, const uint32_t* d_words, uint32_t nbric
 void server_submit(cc_server* S, uint64_t id, const uint32_t* bricks, uint32_t count, float* const* outs, double arrival);
 // reaps finished batches (completion time `now`) and launches pending requests on free slots
 uint32_t server_poll(cc_server* S, double now);
+void server_destroy(cc_server* S);
 // slots in flight + requests waiting
 uint32_t server_busy(const cc_server* S);
 uint32_t server_take_completed(cc_server* S, uint64_t* ids, double* times, uint32_t max);
 uint32_t server_take_failed(cc_server* S, uint64_t* ids, uint32_t* status, uint32_t max);
+// drives an idle server through a request trace in real time (seconds from the call's start);
+// latency[r] = completion - arrival, -1 for a failed request; elapsed = last completion
+void server_run(cc_server* S, uint32_t nreq, const double* arrivals, const uint32_t* counts, const uint32_t* ids,
+                float* const* outs, double* latency, double* elapsed);
 
 }  // namespace cca

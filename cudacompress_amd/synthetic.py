@@ -8,6 +8,11 @@
   unit RMS, shape (3, n, n, n).
 * brick_set(count, axes, seed): bricks with each axis drawn uniformly from `axes`, cut at seeded
   offsets from seeded k^-5/3 fields.
+* gaussian_field_device(n, seed, device): the k^-5/3 field of gaussian_field on a cube of side n,
+  generated on a torch device (Philox white noise from torch.Generator(seed)) for sizes the host
+  FFT makes slow (1024^3).
+* serving_bricks(field, ...): the overlapping brick decomposition of a cube plus its 2x
+  box-downsampled level of detail.
 """
 import hashlib
 
@@ -79,6 +84,55 @@ def brick_set(count, axes=(16, 32, 48, 64, 96, 128), seed=4, nfields=2, field_n=
         oz, oy, ox = (int(r.integers(0, field_n - s + 1)) for s in (z, y, x))
         out.append(np.ascontiguousarray(f[oz:oz + z, oy:oy + y, ox:ox + x]))
     return out
+
+
+def gaussian_field_device(n, seed, device, slope=-5.0 / 3.0):
+    """float32 (n, n, n) tensor on `device`: white noise -> rfftn -> amplitude k^((slope - 2) / 2)
+    (k = 0 removed) -> irfftn, then zero mean and unit std.  The spectrum is complex64."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    spec = torch.fft.rfftn(torch.randn((n, n, n), generator=g, device=device, dtype=torch.float32))
+    f = torch.fft.fftfreq(n, d=1.0 / n, device=device, dtype=torch.float32)
+    fr = torch.fft.rfftfreq(n, d=1.0 / n, device=device, dtype=torch.float32)
+    e = (slope - 2.0) / 2.0
+    for z in range(n):  # one z plane at a time keeps the amplitude temporary small
+        k2 = f[z] ** 2 + f.view(n, 1) ** 2 + fr.view(1, -1) ** 2
+        amp = torch.where(k2 > 0, k2.clamp_min(1e-30) ** (e / 2.0), torch.zeros_like(k2))
+        spec[z].mul_(amp)
+    field = torch.fft.irfftn(spec, s=(n, n, n))
+    del spec
+    field -= field.mean(dtype=torch.float64).float()
+    field /= field.std().float()
+    return field
+
+
+def serving_offsets(n, brick, overlap):
+    """Brick origins along one axis: stride brick - overlap, the last one clamped to n - brick."""
+    stride = brick - overlap
+    count = (n - brick + stride - 1) // stride + 1
+    return [min(i * stride, n - brick) for i in range(count)]
+
+
+def serving_bricks(field, brick=128, overlap=8):
+    """Level-0 bricks of side `brick` overlapping by `overlap` voxels (origins serving_offsets, the
+    last clamped to the edge; 1024 -> 9 per axis at stride 120) and the level-of-detail bricks: the
+    2x box-downsampled field cut into bricks of side brick/2 at half the level-0 origins (aligned
+    1:1 with the level-0 bricks).  Returns (l0, lod, lod_field): lists of contiguous tensors in
+    z-major brick order."""
+    n = field.shape[0]
+    offs = serving_offsets(n, brick, overlap)
+    h = n // 2
+    lod_field = field.view(h, 2, h, 2, h, 2).mean(dim=(1, 3, 5))
+    hb = brick // 2
+    l0, lod = [], []
+    for oz in offs:
+        for oy in offs:
+            for ox in offs:
+                l0.append(field[oz:oz + brick, oy:oy + brick, ox:ox + brick].contiguous())
+                lz, ly, lx = (min(o // 2, h - hb) for o in (oz, oy, ox))
+                lod.append(lod_field[lz:lz + hb, ly:ly + hb, lx:lx + hb].contiguous())
+    return l0, lod, lod_field
 
 
 def sha256(a):

@@ -274,6 +274,14 @@ CC_API uint32_t cc_server_pending(const cc_server* server);
 CC_API uint32_t cc_server_completed(cc_server* server, uint64_t* requestIds, double* times, uint32_t max);
 /* moves up to `max` failed requests (id, decoder status code) out */
 CC_API uint32_t cc_server_failed(cc_server* server, uint64_t* requestIds, uint32_t* status, uint32_t max);
+/* Drives an idle server through a request trace in real time: request r (id r) arrives
+ * arrivals[r] seconds after the call starts (non-decreasing) and asks for counts[r] bricks,
+ * brickIds / d_outputs holding the requests' lists one after the other.  The call submits each
+ * request at its arrival time, polls continuously and returns when every request has finished:
+ * latencies[r] = completion - arrival in seconds (-1 for a failed request), *elapsed = time of
+ * the last completion. */
+CC_API cc_status cc_server_run(cc_server* server, uint32_t requestCount, const double* arrivals, const uint32_t* counts,
+                               const uint32_t* brickIds, float* const* d_outputs, double* latencies, double* elapsed);
 CC_API void cc_server_destroy(cc_server* server);
 
 /* ---- Kernel timing ------------------------------------------------------------------------ */

@@ -78,7 +78,7 @@ RefTable ref_design(const std::vector<uint32_t>& hist, int dialect, uint32_t qui
 }
 
 template <class Idx>
-bool check_one(const std::vector<uint32_t>& hist, int dialect, uint32_t quirk, const char* what, bool path_form) {
+bool check_one(const std::vector<uint32_t>& hist, int dialect, uint32_t quirk, const char* what, int form) {
     const uint32_t nb = (uint32_t)hist.size();
     RefTable r = ref_design(hist, dialect, quirk);
     uint32_t K = 0;
@@ -91,7 +91,10 @@ bool check_one(const std::vector<uint32_t>& hist, int dialect, uint32_t quirk, c
     std::vector<uint32_t> code(nb, 0xDEADBEEFu), len(nb, 0xDEADBEEFu), syms(K + 1), cpl(32), info(3);
     TableOut out{code.data(), len.data(), syms.data(), cpl.data(), info.data()};
     uint64_t bits = 0;
-    if (path_form) {
+    if (form == 2) {
+        WalkEx ex;
+        design_table(ex, hist.data(), nb, dialect, quirk, w, &ctl, out, &bits);
+    } else if (form == 1) {
         PathEx ex;
         design_table(ex, hist.data(), nb, dialect, quirk, w, &ctl, out, &bits);
     } else {
@@ -108,7 +111,7 @@ bool check_one(const std::vector<uint32_t>& hist, int dialect, uint32_t quirk, c
     for (uint32_t s = 0; ok && s < nb; s++)
         if (hist[s]) ok = code[s] == r.code[s] && len[s] == r.len[s];
     if (!ok) {
-        printf("FAIL %s%s: K=%u nb=%u dialect=%d nlen %u/%zu bits %llu/%llu\n", what, path_form ? " (path form)" : "", K, nb, dialect, info[0],
+        printf("FAIL %s (form %d): K=%u nb=%u dialect=%d nlen %u/%zu bits %llu/%llu\n", what, form, K, nb, dialect, info[0],
                r.cpl.size(), (unsigned long long)bits, (unsigned long long)r.bits);
         return false;
     }
@@ -156,7 +159,7 @@ int main(int argc, char** argv) {
                                                 {0, 7, 0, 7, 0, 7}, {1, 2, 4, 8, 16, 32, 64, 128}};
     for (auto& h : fixed)
         for (int d = 0; d < 2; d++) {
-            for (bool pf : {false, true}) {
+            for (int pf : {0, 1, 2}) {
                 ok &= check_one<uint16_t>(h, d, 0, "fixed16", pf);
                 ok &= check_one<uint32_t>(h, d, 0, "fixed32", pf);
                 cases += 2;
@@ -166,8 +169,9 @@ int main(int argc, char** argv) {
     {
         std::vector<uint32_t> h{1, 1};
         while (h.size() < 40) h.push_back(h[h.size() - 1] + h[h.size() - 2]);
-        ok &= check_one<uint32_t>(h, 0, 0, "fib", false) && check_one<uint32_t>(h, 0, 0, "fib", true);
-        cases += 2;
+        ok &= check_one<uint32_t>(h, 0, 0, "fib", 0) && check_one<uint32_t>(h, 0, 0, "fib", 1) &&
+              check_one<uint32_t>(h, 0, 0, "fib", 2);
+        cases += 3;
     }
     for (int t = 0; t < trials && ok; t++) {
         const int shape = t % 6;
@@ -190,10 +194,10 @@ int main(int argc, char** argv) {
             h[65535] += 1;
             quirk = 1;
         }
-        ok &= check_one<uint32_t>(h, dialect, quirk, "rand32", t % 2 == 0);
+        ok &= check_one<uint32_t>(h, dialect, quirk, "rand32", t % 3);
         uint32_t K = 0;
         for (uint32_t x : h) K += x ? 1 : 0;
-        if (K <= 32768) ok &= check_one<uint16_t>(h, dialect, quirk, "rand16", t % 2 == 1);
+        if (K <= 32768) ok &= check_one<uint16_t>(h, dialect, quirk, "rand16", (t + 1) % 3);
         cases += 2;
     }
     // sorts: tie-heavy, sorted, reversed, organ pipe, median-of-3 killers
