@@ -34,6 +34,7 @@ SIGNATURES = {
     "cc_set_instance_stream": (None, [c_vp, c_vp]),
     "cc_get_instance_stream": (c_vp, [c_vp]),
     "cc_set_instance_dialect": (c_int, [c_vp, c_int]),
+    "cc_set_instance_brick_groups": (c_int, [c_vp, c_u32]),
     "cc_encode_rlhuff": (c_int, [c_vp, c_int, c_vp, c_u32, c_u32, c_vp, PP_u32, PP_u64, PP_u64]),
     "cc_encode_huff": (c_int, [c_vp, c_int, c_vp, c_u32, c_u32, c_vp, PP_u32, PP_u64, PP_u64]),
     "cc_encode_device": (c_int, [c_vp, c_int, c_int, c_vp, c_u32, c_u32, c_vp, c_u64, P_u64]),

@@ -124,6 +124,10 @@ class Instance:
     def setDialect(self, dialect):
         _check(_capi.load().cc_set_instance_dialect(self._h, dialect), "setDialect")
 
+    def setBrickGroups(self, groups):
+        """Concurrent groups of a brick batch (0: chosen from the batch size)."""
+        _check(_capi.load().cc_set_instance_brick_groups(self._h, groups), "setBrickGroups")
+
 
 def createInstance(cudaDevice=-1, streamCountMax=1, elemCountPerStreamMax=1, codingBlockSize=0,
                    log2HuffmanDistinctSymbolCountMax=0):

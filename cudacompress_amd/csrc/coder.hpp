@@ -70,6 +70,7 @@ struct CoderParams {
 struct CoderState {          // device-side counters of one job
     unsigned long long scratch_used;  // words
     unsigned long long out_words;     // total output words (written by layout)
+    unsigned long long out_base;      // word offset of the job's first section in the output
     uint32_t err;                     // first error of the job (CodeErr), 0 if none
     uint32_t pad;
 };
@@ -137,10 +138,19 @@ struct SideStreams {
     hipEvent_t fork;
     hipEvent_t join[kSideStreams];
 };
+// Jobs writing one output one after the other: a job's sections start where the previous job's
+// end (prev: that job's device state, null for the first); the layout waits for `wait` (the
+// previous job's layout) and records `signal` after its own.
+struct LayoutLink {
+    const CoderState* prev;
+    hipEvent_t wait;
+    hipEvent_t signal;
+};
 void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uint32_t nseq, const CoderParams& p,
                   int symSize, uint32_t* scratch, uint64_t scratch_cap, CoderState* d_state, CallOut* d_calls,
                   uint32_t ncalls, const uint32_t* d_call_first_item, uint32_t* d_out, uint64_t out_cap,
-                  uint64_t total_symbols, hipStream_t st, const SideStreams* side);
+                  uint64_t total_symbols, hipStream_t st, const SideStreams* side,
+                  const LayoutLink* link = nullptr);
 // wave_seq: for every decode wave the coded sequence it serves (upper bound of the block count:
 // RLE sequences have at most n pairs); wave_first: that wave's first block.
 void coder_decode(const DecCall* d_calls, uint32_t ncalls, const DecItem* d_items, uint32_t nitems, DecSeq* d_seqs,

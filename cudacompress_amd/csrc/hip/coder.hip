@@ -145,6 +145,7 @@ __global__ void k_coder_init(CoderState* st) {
     if (threadIdx.x == 0) {
         st->scratch_used = 0;
         st->out_words = 0;
+        st->out_base = 0;
         st->err = 0;
         st->pad = 0;
     }
@@ -685,9 +686,11 @@ __device__ __forceinline__ uint64_t seq_words(const SeqInfo& s, const CoderParam
 // Section offsets in call order (one workgroup of 1024 threads).
 __global__ __launch_bounds__(1024) void k_layout(const CodeItem* items, uint32_t nitems, SeqInfo* seqs,
                                                  CoderParams p, CoderState* st, CallOut* calls, uint32_t ncalls,
-                                                 const uint32_t* call_first, uint64_t* item_off, uint64_t out_cap) {
+                                                 const uint32_t* call_first, uint64_t* item_off, uint64_t out_cap,
+                                                 const CoderState* prev) {
     __shared__ uint64_t sh[16];
-    uint64_t carry = 0;
+    const uint64_t base0 = prev ? prev->out_base + prev->out_words : 0;  // where this job's output starts
+    uint64_t carry = base0;
     for (uint32_t base = 0; base < nitems; base += 1024) {
         const uint32_t i = base + threadIdx.x;
         uint64_t words = 0;
@@ -719,7 +722,8 @@ __global__ __launch_bounds__(1024) void k_layout(const CodeItem* items, uint32_t
     }
     if (threadIdx.x == 0) {
         item_off[nitems] = carry;
-        st->out_words = carry;
+        st->out_base = base0;
+        st->out_words = carry - base0;
         if (carry > out_cap) atomicCAS(&st->err, 0u, (uint32_t)kErrOutput);
     }
     __syncthreads();
@@ -775,7 +779,7 @@ __global__ __launch_bounds__(kCoderThreads) void k_gather(const CodeItem* items,
                                                           uint32_t* out, uint64_t out_cap) {
     const SeqInfo& s = seqs[blockIdx.x];
     const CodeItem it = items[s.item];
-    if (st->out_words > out_cap) return;
+    if (st->out_base + st->out_words > out_cap) return;
     if (s.err) return;
     if (it.rl && (seqs[it.seq].err || seqs[it.seq + 1].err)) return;
     const bool is_first = it.seq == blockIdx.x;
@@ -807,7 +811,7 @@ __global__ __launch_bounds__(kCoderThreads) void k_gather(const CodeItem* items,
 void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uint32_t nseq, const CoderParams& p,
                   int symSize, uint32_t* scratch, uint64_t scratch_cap, CoderState* d_state, CallOut* d_calls,
                   uint32_t ncalls, const uint32_t* d_call_first_item, uint32_t* d_out, uint64_t out_cap,
-                  uint64_t total_symbols, hipStream_t st, const SideStreams* side) {
+                  uint64_t total_symbols, hipStream_t st, const SideStreams* side, const LayoutLink* link) {
     CCA_REQUIRE(symSize == 2 || symSize == 4, CC_ERR_INVALID_ARG, "symbol size must be 2 or 4");
     uint64_t* item_off = reinterpret_cast<uint64_t*>(scratch + scratch_cap);  // caller reserves nitems+1 u64
     CCA_LAUNCH("coder_init", 32, k_coder_init, dim3(1), dim3(64), 0, st, d_state);
@@ -866,9 +870,11 @@ void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uin
         CCA_LAUNCH("code_pack", inb, k_code_pack<uint32_t>, dim3(nitems), dim3(kCoderThreads), 0, st, d_items, d_seqs,
                    p, scratch);
     CCA_LAUNCH_CHECK();
+    if (link && link->wait) CCA_HIP(hipStreamWaitEvent(st, link->wait, 0));
     CCA_LAUNCH("code_layout", 0, k_layout, dim3(1), dim3(1024), 0, st, d_items, nitems, d_seqs, p, d_state, d_calls,
-               ncalls, d_call_first_item, item_off, out_cap);
+               ncalls, d_call_first_item, item_off, out_cap, link ? link->prev : nullptr);
     CCA_LAUNCH_CHECK();
+    if (link && link->signal) CCA_HIP(hipEventRecord(link->signal, st));
     CCA_LAUNCH("code_gather", 0, k_gather, dim3(nseq), dim3(kCoderThreads), 0, st, d_items, d_seqs, p, scratch,
                d_state, d_out, out_cap);
     CCA_LAUNCH_CHECK();

@@ -103,12 +103,13 @@ void encode_reserve(DeviceCoder& D, EncodeJob& job, uint64_t words) {
     job.scratch_cap = words;
 }
 
-void encode_launch(DeviceCoder& D, const EncodeJob& job, uint32_t* d_out, uint64_t out_cap, hipStream_t st) {
+void encode_launch(DeviceCoder& D, const EncodeJob& job, uint32_t* d_out, uint64_t out_cap, hipStream_t st,
+                   const LayoutLink* link) {
     SideStreams ss;
     coder_encode((const CodeItem*)D.d_items.p, job.nitems, (SeqInfo*)D.d_seqs.p, job.nseq, job.p, job.symSize,
                  (uint32_t*)D.d_scratch.p, job.scratch_cap, (CoderState*)D.d_state.p, (CallOut*)D.d_calls.p,
                  job.ncalls, (const uint32_t*)D.d_call_first.p, d_out, out_cap, job.total_syms, st,
-                 side_streams(D, ss));
+                 side_streams(D, ss), link);
 }
 
 CoderState encode_finish(DeviceCoder& D, const EncodeJob& job, std::vector<CallOut>* results, hipStream_t st) {

@@ -34,7 +34,8 @@ void encode_prepare(cc_instance* I, DeviceCoder& D, const std::vector<ItemSpec>&
                     EncodeJob& job, hipStream_t st);
 // Sizes the arena to at least `words` (allocates: not during graph capture).
 void encode_reserve(DeviceCoder& D, EncodeJob& job, uint64_t words);
-void encode_launch(DeviceCoder& D, const EncodeJob& job, uint32_t* d_out, uint64_t out_cap, hipStream_t st);
+void encode_launch(DeviceCoder& D, const EncodeJob& job, uint32_t* d_out, uint64_t out_cap, hipStream_t st,
+                   const LayoutLink* link = nullptr);
 CoderState encode_finish(DeviceCoder& D, const EncodeJob& job, std::vector<CallOut>* results, hipStream_t st);
 
 struct DecodeJob {

@@ -174,12 +174,12 @@ void bricks_prepare(cc_instance* I, BrickCtx& ctx, BrickJob& job, bool enc, bool
     }
 }
 
-void bricks_encode_launch(BrickCtx& ctx, const BrickJob& job, hipStream_t st) {
+void bricks_encode_launch(BrickCtx& ctx, const BrickJob& job, hipStream_t st, const LayoutLink* link) {
     const uint32_t n = (uint32_t)job.bricks.size();
     for (uint32_t l = 0; l < job.L; l++)
         bricks_forward_level(job.d_levels + (size_t)l * n, job.d_tiles + (size_t)l * 3 * (n + 1), n,
                              &job.tiles_total[3 * l], job.symSize, job.level_elems[l], st);
-    encode_launch(ctx.coder, job.ej, job.d_out, job.out_cap, st);
+    encode_launch(ctx.coder, job.ej, job.d_out, job.out_cap, st, link);
 }
 
 void bricks_decode_launch(BrickCtx& ctx, const BrickJob& job, hipStream_t st, bool from_encode) {
@@ -217,48 +217,164 @@ bool bricks_decode_results(const BrickJob& job, const std::vector<DecCallOut>& r
     return ok;
 }
 
+// ------------------------------------------------------------------------------------- groups
+// A batch runs as groups: contiguous brick ranges of about equal voxel count, each a pipeline of
+// its own (context, stream).  The groups' kernels overlap on the device -- in particular the
+// latency-bound table design of one group with the bandwidth-bound passes of the others -- and
+// their layouts are chained so that the bitstreams land back to back in brick order, exactly
+// where a single batch puts them.
+
+// Groups are opt-in (cc_set_instance_brick_groups): each group adds a stream plus its coder's
+// design streams, and with the runtime's four hardware queues per process those streams share
+// queues, so one group's long table designs hold back the next group's passes queued behind them.
+uint32_t groups_for(const cc_instance* I, const cc_brick* bricks, uint32_t n) {
+    (void)bricks;
+    return I->brick_groups ? std::min<uint32_t>(I->brick_groups, std::max<uint32_t>(n, 1)) : 1u;
+}
+
+// group boundaries: group g holds bricks [first[g], first[g+1]), never empty
+std::vector<uint32_t> split_groups(const cc_brick* bricks, uint32_t n, uint32_t G) {
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < n; i++) total += elems(bricks[i]);
+    std::vector<uint32_t> first{0};
+    uint64_t acc = 0;
+    for (uint32_t i = 0; i + 1 < n; i++) {
+        acc += elems(bricks[i]);
+        if (first.size() < G && acc * G >= total * first.size()) first.push_back(i + 1);
+    }
+    first.push_back(n);
+    return first;
+}
+
+void Groups::setup(cc_instance* I, const cc_brick* bricks, uint32_t n, uint32_t L, int symSize, uint32_t* d_out,
+                   uint64_t cap, uint32_t G, std::vector<std::unique_ptr<BrickCtx>>& pool) {
+    first = split_groups(bricks, n, std::max<uint32_t>(G, 1));
+    const uint32_t ng = (uint32_t)first.size() - 1;
+    while (pool.size() < ng) pool.emplace_back(new BrickCtx());
+    ctx.assign(ng, nullptr);
+    jobs.assign(ng, BrickJob());
+    for (uint32_t g = 0; g < ng; g++) {
+        ctx[g] = pool[g].get();
+        ctx[g]->ensure_stream();
+        BrickJob& job = jobs[g];
+        job.bricks.assign(bricks + first[g], bricks + first[g + 1]);
+        job.L = L;
+        job.symSize = symSize;
+        job.d_out = d_out;
+        job.out_cap = cap;
+    }
+    (void)I;
+}
+
+void Groups::fork(hipStream_t from, hipEvent_t ev) {
+    CCA_HIP(hipEventRecord(ev, from));
+    for (BrickCtx* c : ctx) CCA_HIP(hipStreamWaitEvent(c->st, ev, 0));
+}
+
+void Groups::join(hipStream_t into) {
+    for (BrickCtx* c : ctx) {
+        CCA_HIP(hipEventRecord(c->ev_done, c->st));
+        CCA_HIP(hipStreamWaitEvent(into, c->ev_done, 0));
+    }
+}
+
+void Groups::encode_launch(hipStream_t from, hipEvent_t ev) {
+    fork(from, ev);
+    for (size_t g = 0; g < ctx.size(); g++) {
+        const LayoutLink link{g ? (const CoderState*)ctx[g - 1]->coder.d_state.p : nullptr,
+                              g ? ctx[g - 1]->ev_layout : nullptr, ctx[g]->ev_layout};
+        bricks_encode_launch(*ctx[g], jobs[g], ctx[g]->st, &link);
+    }
+    join(from);
+}
+
+void Groups::decode_launch(hipStream_t from, hipEvent_t ev, bool from_encode) {
+    fork(from, ev);
+    for (size_t g = 0; g < ctx.size(); g++) bricks_decode_launch(*ctx[g], jobs[g], ctx[g]->st, from_encode);
+    join(from);
+}
+
+// Encodes (retrying with larger arenas when a group's runs out) and reads the per-brick results.
+// Returns the total word count.
+uint64_t Groups::encode_run(hipStream_t from, hipEvent_t ev, uint64_t* offsets, uint32_t* status, bool* ok,
+                            uint64_t headroom_div) {
+    const size_t ng = ctx.size();
+    std::vector<std::vector<CallOut>> res(ng);
+    std::vector<CoderState> s(ng);
+    for (int attempt = 0;; attempt++) {
+        encode_launch(from, ev);
+        bool retry = false;
+        for (size_t g = 0; g < ng; g++) {
+            s[g] = encode_finish(ctx[g]->coder, jobs[g].ej, &res[g], ctx[g]->st);
+            if (s[g].err == kErrScratch && attempt < 4) {
+                encode_reserve(ctx[g]->coder, jobs[g].ej,
+                               std::max<uint64_t>(2 * jobs[g].ej.scratch_cap, s[g].scratch_used + s[g].scratch_used / 2));
+                retry = true;
+            }
+        }
+        if (!retry) break;
+    }
+    if (headroom_div)
+        for (size_t g = 0; g < ng; g++)
+            encode_reserve(ctx[g]->coder, jobs[g].ej,
+                           std::max<uint64_t>(jobs[g].ej.scratch_cap, s[g].scratch_used + s[g].scratch_used / headroom_div));
+    bool all = true;
+    for (size_t g = 0; g < ng; g++) {
+        CCA_REQUIRE(s[g].err != kErrScratch, CC_ERR_OUT_OF_MEMORY, "coder arena exhausted");
+        const uint64_t end = s[g].out_base + s[g].out_words;
+        all = bricks_encode_results(jobs[g], res[g], end, offsets + first[g], status ? status + first[g] : nullptr) && all;
+    }
+    *ok = all;
+    return s[ng - 1].out_base + s[ng - 1].out_words;
+}
+
+bool Groups::decode_results(uint32_t* status) {
+    bool all = true;
+    for (size_t g = 0; g < ctx.size(); g++) {
+        std::vector<DecCallOut> res;
+        decode_finish(ctx[g]->coder, jobs[g].dj, &res, ctx[g]->st);
+        all = bricks_decode_results(jobs[g], res, status ? status + first[g] : nullptr) && all;
+    }
+    return all;
+}
+
 void compress_bricks(cc_instance* I, const cc_brick* bricks, uint32_t n, uint32_t L, int symSize, uint32_t* d_out,
                      uint64_t cap, uint64_t* offsets, uint32_t* status) {
     DeviceGuard guard(I->device);
-    hipStream_t st = I->stream;
-    BrickJob job;
-    job.bricks.assign(bricks, bricks + n);
-    job.L = L;
-    job.symSize = symSize;
-    job.d_out = d_out;
-    job.out_cap = cap;
-    bricks_prepare(I, I->batch, job, true, false, nullptr, st);
-    std::vector<CallOut> res;
-    CoderState s{};
-    for (int attempt = 0;; attempt++) {
-        bricks_encode_launch(I->batch, job, st);
-        s = encode_finish(I->batch.coder, job.ej, &res, st);
-        if (s.err != kErrScratch || attempt >= 4) break;
-        encode_reserve(I->batch.coder, job.ej, std::max<uint64_t>(2 * job.ej.scratch_cap, s.scratch_used + s.scratch_used / 2));
+    check_bricks(bricks, n, L);
+    if (n == 0) {
+        offsets[0] = 0;
+        return;
     }
-    CCA_REQUIRE(s.out_words <= cap, CC_ERR_INVALID_ARG, "output buffer too small for the compressed bricks");
-    const bool ok = bricks_encode_results(job, res, s.out_words, offsets, status);
+    if (!I->group_fork) CCA_HIP(hipEventCreateWithFlags(&I->group_fork, hipEventDisableTiming));
+    Groups gr;
+    gr.setup(I, bricks, n, L, symSize, d_out, cap, groups_for(I, bricks, n), I->groups);
+    for (size_t g = 0; g < gr.ctx.size(); g++)
+        bricks_prepare(I, *gr.ctx[g], gr.jobs[g], true, false, nullptr, gr.ctx[g]->st);
+    bool ok = false;
+    const uint64_t total = gr.encode_run(I->stream, I->group_fork, offsets, status, &ok, 0);
+    CCA_REQUIRE(total <= cap, CC_ERR_INVALID_ARG, "output buffer too small for the compressed bricks");
     CCA_REQUIRE(ok, CC_ERR_SYMBOL_OVERFLOW, "some bricks failed to encode (see the per-brick status)");
 }
 
 void decompress_bricks(cc_instance* I, const cc_brick* bricks, uint32_t n, uint32_t L, int symSize,
                        const uint32_t* d_in, const uint64_t* offsets, uint32_t* status) {
     DeviceGuard guard(I->device);
-    hipStream_t st = I->stream;
-    BrickJob job;
-    job.bricks.assign(bricks, bricks + n);
-    job.L = L;
-    job.symSize = symSize;
-    std::vector<CallWords> calls(n);
-    for (uint32_t i = 0; i < n; i++) {
+    check_bricks(bricks, n, L);
+    if (n == 0) return;
+    for (uint32_t i = 0; i < n; i++)
         CCA_REQUIRE(offsets[i + 1] >= offsets[i], CC_ERR_INVALID_ARG, "brick offsets must be nondecreasing");
-        calls[i] = CallWords{d_in + offsets[i], offsets[i + 1] - offsets[i]};
+    if (!I->group_fork) CCA_HIP(hipEventCreateWithFlags(&I->group_fork, hipEventDisableTiming));
+    Groups gr;
+    gr.setup(I, bricks, n, L, symSize, nullptr, 0, groups_for(I, bricks, n), I->groups);
+    for (size_t g = 0; g < gr.ctx.size(); g++) {
+        std::vector<CallWords> calls;
+        for (uint32_t i = gr.first[g]; i < gr.first[g + 1]; i++)
+            calls.push_back(CallWords{d_in + offsets[i], offsets[i + 1] - offsets[i]});
+        bricks_prepare(I, *gr.ctx[g], gr.jobs[g], false, true, &calls, gr.ctx[g]->st);
     }
-    bricks_prepare(I, I->batch, job, false, true, &calls, st);
-    bricks_decode_launch(I->batch, job, st, false);
-    std::vector<DecCallOut> res;
-    decode_finish(I->batch.coder, job.dj, &res, st);
-    CCA_REQUIRE(bricks_decode_results(job, res, status), CC_ERR_CORRUPT_STREAM,
+    gr.decode_launch(I->stream, I->group_fork, false);
+    CCA_REQUIRE(gr.decode_results(status), CC_ERR_CORRUPT_STREAM,
                 "some bricks failed to decode (see the per-brick status)");
 }
 
@@ -267,9 +383,9 @@ void decompress_bricks(cc_instance* I, const cc_brick* bricks, uint32_t n, uint3
 // ------------------------------------------------------------------------------------- plans
 struct cc_bricks_plan {
     cc_instance* inst = nullptr;
-    cca::BrickCtx ctx;
-    cca::BrickJob job;
-    hipStream_t st = nullptr;
+    std::vector<std::unique_ptr<cca::BrickCtx>> pool;  // one context (and stream) per group
+    cca::Groups gr;
+    hipStream_t st = nullptr;  // the group context's stream: capture origin and replay stream
     hipGraph_t g_enc = nullptr, g_dec = nullptr;
     hipGraphExec_t x_enc = nullptr, x_dec = nullptr;
     hipEvent_t order = nullptr;  // orders replays after the instance stream's earlier work
@@ -287,8 +403,7 @@ cc_bricks_plan::~cc_bricks_plan() {
     if (g_enc) (void)hipGraphDestroy(g_enc);
     if (g_dec) (void)hipGraphDestroy(g_dec);
     if (order) (void)hipEventDestroy(order);
-    ctx.release();
-    if (st) (void)hipStreamDestroy(st);
+    for (auto& c : pool) c->release();  // also destroys the context stream `st` points to
     if (prev >= 0) (void)hipSetDevice(prev);
 }
 
@@ -313,48 +428,51 @@ void capture(hipStream_t st, hipGraph_t* g, hipGraphExec_t* x, const std::functi
 cc_bricks_plan* plan_create(cc_instance* I, const cc_brick* bricks, uint32_t n, uint32_t L, int symSize,
                             uint32_t* d_out, uint64_t cap) {
     DeviceGuard guard(I->device);
+    check_bricks(bricks, n, L);
+    CCA_REQUIRE(n > 0, CC_ERR_INVALID_ARG, "a plan needs at least one brick");
     std::unique_ptr<cc_bricks_plan> P(new cc_bricks_plan());
     P->inst = I;
-    CCA_HIP(hipStreamCreateWithFlags(&P->st, hipStreamNonBlocking));
     CCA_HIP(hipEventCreateWithFlags(&P->order, hipEventDisableTiming));
+    // A plan captures one group on its own stream (the capture origin), with the table-design
+    // classes forked to that context's side streams; brick groups are not captured.
+    Groups& gr = P->gr;
+    gr.setup(I, bricks, n, L, symSize, d_out, cap, 1, P->pool);
+    BrickCtx& ctx = *gr.ctx[0];
+    BrickJob& job = gr.jobs[0];
+    P->st = ctx.st;
     CCA_HIP(hipEventRecord(P->order, I->stream));
     CCA_HIP(hipStreamWaitEvent(P->st, P->order, 0));
-    BrickJob& job = P->job;
-    job.bricks.assign(bricks, bricks + n);
-    job.L = L;
-    job.symSize = symSize;
-    job.d_out = d_out;
-    job.out_cap = cap;
-    bricks_prepare(I, P->ctx, job, true, true, nullptr, P->st);
+    bricks_prepare(I, ctx, job, true, true, nullptr, P->st);
     // one eager pass sizes the arena (with headroom for other data of the same geometry) and
     // allocates every buffer before capture
     std::vector<CallOut> res;
     for (int attempt = 0;; attempt++) {
-        bricks_encode_launch(P->ctx, job, P->st);
-        const CoderState s = encode_finish(P->ctx.coder, job.ej, &res, P->st);
+        bricks_encode_launch(ctx, job, P->st);
+        const CoderState s = encode_finish(ctx.coder, job.ej, &res, P->st);
         if (s.err != kErrScratch || attempt >= 4) {
-            encode_reserve(P->ctx.coder, job.ej, std::max<uint64_t>(job.ej.scratch_cap, s.scratch_used + s.scratch_used / 4));
+            encode_reserve(ctx.coder, job.ej, std::max<uint64_t>(job.ej.scratch_cap, s.scratch_used + s.scratch_used / 4));
             break;
         }
-        encode_reserve(P->ctx.coder, job.ej, std::max<uint64_t>(2 * job.ej.scratch_cap, s.scratch_used + s.scratch_used / 2));
+        encode_reserve(ctx.coder, job.ej, std::max<uint64_t>(2 * job.ej.scratch_cap, s.scratch_used + s.scratch_used / 2));
     }
-    capture(P->st, &P->g_enc, &P->x_enc, [&] { bricks_encode_launch(P->ctx, job, P->st); });
-    capture(P->st, &P->g_dec, &P->x_dec, [&] { bricks_decode_launch(P->ctx, job, P->st, true); });
+    capture(P->st, &P->g_enc, &P->x_enc, [&] { bricks_encode_launch(ctx, job, P->st); });
+    capture(P->st, &P->g_dec, &P->x_dec, [&] { bricks_decode_launch(ctx, job, P->st, true); });
     return P.release();
 }
 
 void plan_compress(cc_bricks_plan* P, uint64_t* offsets, uint32_t* status) {
     DeviceGuard guard(P->inst->device);
+    Groups& gr = P->gr;
     CCA_HIP(hipEventRecord(P->order, P->inst->stream));
     CCA_HIP(hipStreamWaitEvent(P->st, P->order, 0));
     CCA_HIP(hipGraphLaunch(P->x_enc, P->st));
     std::vector<CallOut> res;
-    const CoderState s = encode_finish(P->ctx.coder, P->job.ej, &res, P->st);
+    const CoderState s = encode_finish(gr.ctx[0]->coder, gr.jobs[0].ej, &res, P->st);
     P->encoded = true;
-    CCA_REQUIRE(s.err != kErrScratch, CC_ERR_OUT_OF_MEMORY,
-                "plan arena too small for this data; create a new plan");
-    CCA_REQUIRE(s.out_words <= P->job.out_cap, CC_ERR_INVALID_ARG, "output buffer too small for the compressed bricks");
-    const bool ok = bricks_encode_results(P->job, res, s.out_words, offsets, status);
+    CCA_REQUIRE(s.err != kErrScratch, CC_ERR_OUT_OF_MEMORY, "plan arena too small for this data; create a new plan");
+    const uint64_t total = s.out_base + s.out_words;
+    CCA_REQUIRE(total <= gr.jobs[0].out_cap, CC_ERR_INVALID_ARG, "output buffer too small for the compressed bricks");
+    const bool ok = bricks_encode_results(gr.jobs[0], res, total, offsets, status);
     CCA_REQUIRE(ok, CC_ERR_SYMBOL_OVERFLOW, "some bricks failed to encode (see the per-brick status)");
 }
 
@@ -365,8 +483,8 @@ void plan_decompress(cc_bricks_plan* P, uint32_t* status) {
     CCA_HIP(hipStreamWaitEvent(P->st, P->order, 0));
     CCA_HIP(hipGraphLaunch(P->x_dec, P->st));
     std::vector<DecCallOut> res;
-    decode_finish(P->ctx.coder, P->job.dj, &res, P->st);
-    CCA_REQUIRE(bricks_decode_results(P->job, res, status), CC_ERR_CORRUPT_STREAM,
+    decode_finish(P->gr.ctx[0]->coder, P->gr.jobs[0].dj, &res, P->st);
+    CCA_REQUIRE(bricks_decode_results(P->gr.jobs[0], res, status), CC_ERR_CORRUPT_STREAM,
                 "some bricks failed to decode (see the per-brick status)");
 }
 

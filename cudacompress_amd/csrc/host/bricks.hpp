@@ -35,8 +35,26 @@ struct BrickJob {
 void bricks_prepare(cc_instance* I, BrickCtx& ctx, BrickJob& job, bool enc, bool dec,
                     const std::vector<CallWords>* calls, hipStream_t st);
 bool bricks_decode_results(const BrickJob& job, const std::vector<DecCallOut>& res, uint32_t* status);
-void bricks_encode_launch(BrickCtx& ctx, const BrickJob& job, hipStream_t st);
+void bricks_encode_launch(BrickCtx& ctx, const BrickJob& job, hipStream_t st, const LayoutLink* link = nullptr);
 void bricks_decode_launch(BrickCtx& ctx, const BrickJob& job, hipStream_t st, bool from_encode);
+
+// A batch split into groups of contiguous bricks, each with its own context and stream (host/bricks.cpp).
+struct Groups {
+    std::vector<uint32_t> first;  // group g: bricks [first[g], first[g+1])
+    std::vector<BrickCtx*> ctx;
+    std::vector<BrickJob> jobs;
+    void setup(cc_instance* I, const cc_brick* bricks, uint32_t n, uint32_t L, int symSize, uint32_t* d_out,
+               uint64_t cap, uint32_t G, std::vector<std::unique_ptr<BrickCtx>>& pool);
+    void fork(hipStream_t from, hipEvent_t ev);
+    void join(hipStream_t into);
+    void encode_launch(hipStream_t from, hipEvent_t ev);
+    void decode_launch(hipStream_t from, hipEvent_t ev, bool from_encode);
+    // headroom_div: when nonzero, arenas keep used / headroom_div words of headroom afterwards
+    uint64_t encode_run(hipStream_t from, hipEvent_t ev, uint64_t* offsets, uint32_t* status, bool* ok,
+                        uint64_t headroom_div);
+    bool decode_results(uint32_t* status);
+};
+uint32_t groups_for(const cc_instance* I, const cc_brick* bricks, uint32_t n);
 
 // offsets: nbricks+1 word offsets into d_out (host); status: per-brick cc_status (may be null).
 void compress_bricks(cc_instance* I, const cc_brick* bricks, uint32_t n, uint32_t levels, int symSize, uint32_t* d_out,

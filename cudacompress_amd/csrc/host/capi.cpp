@@ -149,6 +149,12 @@ cc_status cc_set_instance_dialect(cc_instance* inst, int dialect) {
         inst->dialect = dialect;
     });
 }
+cc_status cc_set_instance_brick_groups(cc_instance* inst, uint32_t groups) {
+    return guarded([&] {
+        CCA_REQUIRE(inst && groups <= 64, CC_ERR_INVALID_ARG, "brick groups must be in [0, 64]");
+        inst->brick_groups = groups;
+    });
+}
 
 cc_status cc_encode_device(cc_instance* inst, int rl, int symbolSize, const void* const* d_symbolStreams,
                            uint32_t streamCount, uint32_t symbolCountPerStream, uint32_t* d_out,

@@ -77,12 +77,24 @@ void DeviceCoder::release() {
     scratch_cap = 0;
 }
 
+void BrickCtx::ensure_stream() {
+    if (!st) CCA_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    if (!ev_layout) CCA_HIP(hipEventCreateWithFlags(&ev_layout, hipEventDisableTiming));
+    if (!ev_done) CCA_HIP(hipEventCreateWithFlags(&ev_done, hipEventDisableTiming));
+}
+
 void BrickCtx::release() {
+    if (st) (void)hipStreamSynchronize(st);
     coder.release();
     d_bwork.release();
     d_bsym.release();
     d_blevels.release();
     h_blevels.release();
+    if (ev_layout) (void)hipEventDestroy(ev_layout);
+    if (ev_done) (void)hipEventDestroy(ev_done);
+    if (st) (void)hipStreamDestroy(st);
+    st = nullptr;
+    ev_layout = ev_done = nullptr;
 }
 
 }  // namespace cca
@@ -105,5 +117,7 @@ cc_instance::~cc_instance() {
     for (auto& b : h_dec_stage) b.release();
     coder.release();
     batch.release();
+    for (auto& g : groups) g->release();
+    if (group_fork) (void)hipEventDestroy(group_fork);
     if (prev >= 0) (void)hipSetDevice(prev);
 }

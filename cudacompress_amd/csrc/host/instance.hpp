@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <vector>
 
 #include "../coder.hpp"
@@ -59,6 +60,10 @@ struct BrickCtx {
     DeviceCoder coder;
     DevBuf d_bwork, d_bsym, d_blevels;
     PinBuf h_blevels;
+    // stream and events of the context when it runs one group of a grouped batch
+    hipStream_t st = nullptr;
+    hipEvent_t ev_layout = nullptr, ev_done = nullptr;
+    void ensure_stream();
     void release();
 };
 
@@ -96,8 +101,11 @@ struct cc_instance {
     // device-resident coder
     cca::DeviceCoder coder;
 
-    // brick batches of the eager API (host/bricks.cpp)
+    // brick batches of the eager API (host/bricks.cpp): one context per group of a grouped batch
     cca::BrickCtx batch;
+    std::vector<std::unique_ptr<cca::BrickCtx>> groups;
+    hipEvent_t group_fork = nullptr;
+    uint32_t brick_groups = 0;  // groups per brick batch, 0: chosen from the batch size
 
     ~cc_instance();
 };

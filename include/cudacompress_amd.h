@@ -66,6 +66,10 @@ CC_API void* cc_get_instance_stream(const cc_instance* inst);
 /* Bitstream dialect of this instance (CC_DIALECT_GPU by default).  No reference counterpart as a
  * switch: the CPU dialect is what src/CPU/EncodeCPU.cpp:98-354 reads and writes. */
 CC_API cc_status cc_set_instance_dialect(cc_instance* inst, int dialect);
+/* Groups of an eager brick batch (cc_compress/decompress_bricks_float): contiguous brick ranges
+ * run as concurrent pipelines on their own streams, their bitstreams still back to back in brick
+ * order.  0 or 1 (default): one pipeline.  Plans always capture one.  No reference counterpart. */
+CC_API cc_status cc_set_instance_brick_groups(cc_instance* inst, uint32_t groups);
 
 /* ---- Entropy coding: include/cudaCompress/Encode.h:28-57 ---------------------------------- */
 

@@ -140,6 +140,51 @@ def test_bricks_plan_graph_replays(cc, torch_gpu, field):
     inst.close()
 
 
+@pytest.mark.parametrize("groups", [2, 3, 7])
+def test_grouped_batches_equal_single_batch(cc, torch_gpu, field, groups):
+    """A batch run as concurrent groups writes the single batch's words at the same offsets and
+    reconstructs the same values; a plan made on the same instance replays the same (plans capture
+    one group)."""
+    log("bricks groups", groups)
+    torch = torch_gpu
+    rng = np.random.default_rng(40 + groups)
+    bricks = make_bricks(rng, field, 19)
+    std = float(field.std())
+    steps = [float(rng.choice([0.005, 0.05])) * std for _ in bricks]
+    shapes = [b.shape for b in bricks]
+    inst = cc.createInstance(0, 7, 64 * 64 * 64, 128, 14)
+    srcs, out1, offs1, stat1 = run_batch(cc, torch, inst, bricks, steps, 2)
+    assert np.all(stat1 == 0)
+    d1 = [torch.zeros_like(s) for s in srcs]
+    cc.decompressBricksFloat(inst, d1, shapes, steps, 2, out1, offs1)
+    inst.setBrickGroups(groups)
+    out2 = torch.full_like(out1, -1)
+    offs2, stat2 = cc.compressBricksFloat(inst, srcs, shapes, steps, 2, out2)
+    torch.cuda.synchronize()
+    assert np.all(stat2 == 0)
+    assert np.array_equal(offs1, offs2)
+    n = int(offs1[-1])
+    assert np.array_equal(out1[:n].cpu().numpy(), out2[:n].cpu().numpy())
+    d2 = [torch.zeros_like(s) for s in srcs]
+    cc.decompressBricksFloat(inst, d2, shapes, steps, 2, out2, offs2)
+    torch.cuda.synchronize()
+    for a, b in zip(d1, d2):
+        assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+    out3 = torch.full_like(out1, -1)
+    d3 = [torch.zeros_like(s) for s in srcs]
+    plan = cc.BricksPlan(inst, srcs, d3, shapes, steps, 2, out3)
+    for _ in range(2):
+        offs3, stat3 = plan.compress()
+        assert np.all(stat3 == 0) and np.array_equal(offs1, offs3)
+        assert np.all(plan.decompress() == 0)
+        torch.cuda.synchronize()
+        assert np.array_equal(out1[:n].cpu().numpy(), out3[:n].cpu().numpy())
+        for a, b in zip(d1, d3):
+            assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+    plan.close()
+    inst.close()
+
+
 def test_server_results_same_alone_and_in_the_mix(cc, torch_gpu, field):
     """Requests decoded through the serving loop -- batched with others, several slots in flight --
     equal the same requests decoded alone, bit for bit, and equal the eager batch decode."""
