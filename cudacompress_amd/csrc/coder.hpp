@@ -133,6 +133,9 @@ struct DecCallOut {
 // kSideStreams non-blocking streams, `fork` / `join` events owned by the caller; a null `side`
 // argument puts everything on st.
 constexpr int kSideStreams = 4;
+// Table-design work lists (device buffer of kDesignListBase + kDesignLists * nseq words).
+constexpr uint32_t kDesignLists = 5;
+constexpr uint32_t kDesignListBase = 8;
 struct SideStreams {
     hipStream_t s[kSideStreams];
     hipEvent_t fork;
@@ -146,8 +149,10 @@ struct LayoutLink {
     hipEvent_t wait;
     hipEvent_t signal;
 };
+// d_classes: kDesignListBase + kDesignLists * nseq words of table-design work lists.
 void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uint32_t nseq, const CoderParams& p,
-                  int symSize, uint32_t* scratch, uint64_t scratch_cap, CoderState* d_state, CallOut* d_calls,
+                  int symSize, uint32_t* scratch, uint64_t scratch_cap, CoderState* d_state, uint32_t* d_classes,
+                  CallOut* d_calls,
                   uint32_t ncalls, const uint32_t* d_call_first_item, uint32_t* d_out, uint64_t out_cap,
                   uint64_t total_symbols, hipStream_t st, const SideStreams* side,
                   const LayoutLink* link = nullptr);

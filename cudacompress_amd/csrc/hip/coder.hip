@@ -141,7 +141,8 @@ __device__ __forceinline__ long long thread_last_nz(const uint32_t (&v)[kPerThre
 }
 
 // ------------------------------------------------------------------------ kernels
-__global__ void k_coder_init(CoderState* st) {
+__global__ void k_coder_init(CoderState* st, uint32_t* classes) {
+    if (threadIdx.x < kDesignLists) classes[threadIdx.x] = 0;
     if (threadIdx.x == 0) {
         st->scratch_used = 0;
         st->out_words = 0;
@@ -360,14 +361,18 @@ struct WaveEx {
             bool right = false;
             if (ln < 63 && g < lim) right = !(h[2 * g + 1].count < h[2 * g + 2].count);
             const uint64_t mr = __ballot(right);
+            // the walk on the scalar unit, its six local nodes packed 7 bits each; each lane then
+            // takes its own path position (depth dk of the round) and the next one
             uint32_t j = 0;
+            uint64_t J = 0;
 #pragma unroll
             for (int d = 0; d < 6; d++) {
                 j = 2 * j + 1 + (uint32_t)((mr >> j) & 1ull);
-                const int gn = (cur << (d + 1)) + (int)j;
-                if (ln == base + d + 1) mypath = gn;
-                if (ln == base + d) mynext = gn;
+                J |= (uint64_t)j << (7 * d);
             }
+            const int dk = ln - base;
+            if (dk >= 1 && dk <= 6) mypath = (cur << dk) + (int)((J >> (7 * (dk - 1))) & 127u);
+            if (dk >= 0 && dk <= 5) mynext = (cur << (dk + 1)) + (int)((J >> (7 * dk)) & 127u);
             const bool real = ln > base && ln <= base + 6 && ((mypath - 1) >> 1) < lim;
             const int steps = (int)__popcll(__ballot(real));
             if (steps < 6) {
@@ -464,24 +469,46 @@ struct WaveEx {
     }
 };
 
-// One wave per coded sequence whose distinct-symbol count lies in [kmin, kmax].  Gws: the design
-// workspace comes from the arena instead of LDS (a compile-time choice, so that the LDS classes
-// address their workspace with LDS instructions rather than generic ones).
+// Design class of a sequence by distinct-symbol count: four LDS classes and one whose workspace
+// lives in the arena.  Class lists: counters at classes[0, kDesignLists), list c at
+// classes + kDesignListBase + c * nseq.
+__device__ __forceinline__ uint32_t design_class(uint32_t K) {
+    return K <= 256 ? 0u : K <= 1024 ? 1u : K <= 2048 ? 2u : K <= kDesignMid ? 3u : 4u;
+}
+
+// One thread per sequence: appends every sequence still to be designed to its class list (one
+// atomic per class per wave).
+__global__ __launch_bounds__(256) void k_design_classify(const SeqInfo* seqs, uint32_t nseq, uint32_t* classes) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const bool on = i < nseq && !seqs[i].err;
+    const uint32_t c = on ? design_class(seqs[i].nsym) : 0xFFFFFFFFu;
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (uint32_t k = 0; k < kDesignLists; k++) {
+        const uint64_t m = __ballot(c == k);
+        if (!m) continue;
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&classes[k], (uint32_t)__popcll(m));
+        base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+        if (c == k) classes[kDesignListBase + (size_t)k * nseq + base + __popcll(m & lanes_below(lane))] = i;
+    }
+}
+
+// Designs the table of one sequence with one wave (all lanes call; returns uniformly).  Gws: the
+// design workspace comes from the arena instead of LDS (a compile-time choice, so that the LDS
+// classes address their workspace with LDS instructions rather than generic ones).
 template <class Idx, bool Gws>
-__global__ __launch_bounds__(64) void k_design(SeqInfo* seqs, CoderParams p, uint32_t* scratch, uint64_t cap,
-                                               CoderState* st, uint32_t kmin, uint32_t kmax) {
+__device__ void design_one(SeqInfo* s, const CoderParams& p, uint32_t* scratch, uint64_t cap, CoderState* st,
+                           unsigned char* smem) {
     constexpr bool gws = Gws;
-    extern __shared__ __align__(16) unsigned char smem[];
     __shared__ Ctl ctl;
     __shared__ uint64_t offs[4];
     __shared__ uint64_t bits_sh;
-    SeqInfo* s = &seqs[blockIdx.x];
+    __shared__ uint32_t info_sh[3];
+    __shared__ uint32_t cpl_sh[32];
     if (s->err) return;
     const uint32_t K = s->nsym;
-    if (K < kmin || K > kmax) return;
-    // the largest tables set the length of the design phase: their waves take the SIMD's issue
-    // slots first when they share it with the many small designs running beside them
-    if (K > 1024) __builtin_amdgcn_s_setprio(2);
     const uint32_t nb = s->smax + 1;
     const uint64_t hreg = design::heap_region_bytes<Idx>(K);
     const uint64_t preg = ((uint64_t)sizeof(Idx) * (2ull * K + 2) + 15) & ~uint64_t(15);
@@ -500,8 +527,6 @@ __global__ __launch_bounds__(64) void k_design(SeqInfo* seqs, CoderParams p, uin
     Work<Idx> w{reinterpret_cast<HeapEntry*>(ws), reinterpret_cast<Idx*>(ws + hreg),
                 reinterpret_cast<int*>(ws + hreg + preg)};
     uint32_t info[3];
-    __shared__ uint32_t info_sh[3];
-    __shared__ uint32_t cpl_sh[32];
     TableOut out{scratch + offs[0], scratch + offs[0] + nb, scratch + offs[1], cpl_sh, info_sh};
     WaveEx ex;
     uint64_t bits = 0;
@@ -532,6 +557,22 @@ __global__ __launch_bounds__(64) void k_design(SeqInfo* seqs, CoderParams p, uin
             s->words_off = arena_alloc(st, nwords, cap, s);
             if (s->words_off != ~uint64_t(0)) s->boff_off = arena_alloc(st, nblk ? nblk : 1, cap, s);
         }
+    }
+}
+
+// The tables of one class list, one wave each, the workgroups striding over the list.
+template <class Idx, bool Gws>
+__global__ __launch_bounds__(64) void k_design(SeqInfo* seqs, CoderParams p, uint32_t* scratch, uint64_t cap,
+                                               CoderState* st, const uint32_t* classes, uint32_t nseq, uint32_t cls) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t count = classes[cls];
+    const uint32_t* list = classes + kDesignListBase + (size_t)cls * nseq;
+    // the largest tables set the length of the design phase: their waves take the SIMD's issue
+    // slots first when they share it with the many small designs running beside them
+    if (cls >= 2 && blockIdx.x < count) __builtin_amdgcn_s_setprio(2);
+    for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
+        design_one<Idx, Gws>(&seqs[list[i]], p, scratch, cap, st, smem);
+        __syncthreads();  // LDS state is reused by the next table
     }
 }
 
@@ -809,12 +850,13 @@ __global__ __launch_bounds__(kCoderThreads) void k_gather(const CodeItem* items,
 }  // namespace
 
 void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uint32_t nseq, const CoderParams& p,
-                  int symSize, uint32_t* scratch, uint64_t scratch_cap, CoderState* d_state, CallOut* d_calls,
+                  int symSize, uint32_t* scratch, uint64_t scratch_cap, CoderState* d_state, uint32_t* d_classes,
+                  CallOut* d_calls,
                   uint32_t ncalls, const uint32_t* d_call_first_item, uint32_t* d_out, uint64_t out_cap,
                   uint64_t total_symbols, hipStream_t st, const SideStreams* side, const LayoutLink* link) {
     CCA_REQUIRE(symSize == 2 || symSize == 4, CC_ERR_INVALID_ARG, "symbol size must be 2 or 4");
     uint64_t* item_off = reinterpret_cast<uint64_t*>(scratch + scratch_cap);  // caller reserves nitems+1 u64
-    CCA_LAUNCH("coder_init", 32, k_coder_init, dim3(1), dim3(64), 0, st, d_state);
+    CCA_LAUNCH("coder_init", 32, k_coder_init, dim3(1), dim3(64), 0, st, d_state, d_classes);
     CCA_LAUNCH_CHECK();
     if (nitems == 0) return;
     const uint64_t inb = total_symbols * symSize;
@@ -832,30 +874,32 @@ void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uin
                    d_items, d_seqs, p, scratch);
     }
     CCA_LAUNCH_CHECK();
-    // Design classes by distinct-symbol count (small LDS footprints keep many tables resident).
-    // The classes touch disjoint sequences, so they run concurrently on side streams when given.
+    // Design classes by distinct-symbol count (small LDS footprints keep many tables resident):
+    // the sequences are sorted into per-class lists, and each class runs over its list only, the
+    // classes concurrently on side streams, the largest tables' classes launched first.
+    CCA_LAUNCH("design_classify", 0, k_design_classify, dim3(div_up(nseq, 256u)), dim3(256), 0, st, d_seqs, nseq,
+               d_classes);
+    CCA_LAUNCH_CHECK();
     constexpr int kClasses = 4;
     static_assert(kClasses <= kSideStreams, "one side stream per design class");
-    const uint32_t classes[kClasses] = {256, 1024, 2048, kDesignMid};
+    const uint32_t kmax[kClasses] = {256, 1024, 2048, kDesignMid};
+    const uint32_t grid[kClasses] = {4096, 2048, 1536, 768};  // resident workgroups per class (LDS-bound)
     const char* names[kClasses] = {"huff_design_k256", "huff_design_k1024", "huff_design_k2048",
                                    "huff_design_k4096"};
     if (side) {
         CCA_HIP(hipEventRecord(side->fork, st));
         for (int c = 0; c < kClasses; c++) CCA_HIP(hipStreamWaitEvent(side->s[c], side->fork, 0));
     }
-    uint32_t kmin = 0;
-    for (int c = 0; c < kClasses; c++) {
-        const uint32_t kmax = classes[c];
-        const size_t lds = design::heap_region_bytes<uint16_t>(kmax) + 2 * (2 * kmax + 2) + 16 +
+    for (int c = kClasses - 1; c >= 0; c--) {
+        const size_t lds = design::heap_region_bytes<uint16_t>(kmax[c]) + 2 * (2 * kmax[c] + 2) + 16 +
                            12 * design::kStackRanges + 16;
         hipStream_t sc = side ? side->s[c] : st;
-        CCA_LAUNCH(names[c], 0, (k_design<uint16_t, false>), dim3(nseq), dim3(64), lds, sc, d_seqs, p, scratch,
-                   scratch_cap, d_state, kmin, kmax);
+        CCA_LAUNCH(names[c], 0, (k_design<uint16_t, false>), dim3(std::min(nseq, grid[c])), dim3(64), lds, sc, d_seqs,
+                   p, scratch, scratch_cap, d_state, d_classes, nseq, (uint32_t)c);
         CCA_LAUNCH_CHECK();
-        kmin = kmax + 1;
     }
-    CCA_LAUNCH("huff_design_global", 0, (k_design<uint32_t, true>), dim3(nseq), dim3(64), 0, st, d_seqs, p, scratch,
-               scratch_cap, d_state, kDesignMid + 1, 0xFFFFFFFFu);
+    CCA_LAUNCH("huff_design_global", 0, (k_design<uint32_t, true>), dim3(std::min(nseq, 256u)), dim3(64), 0, st,
+               d_seqs, p, scratch, scratch_cap, d_state, d_classes, nseq, 4u);
     CCA_LAUNCH_CHECK();
     if (side) {
         for (int c = 0; c < kClasses; c++) {

@@ -87,6 +87,7 @@ void encode_prepare(cc_instance* I, DeviceCoder& D, const std::vector<ItemSpec>&
     D.d_call_first.as<uint32_t>(ncalls + 1);
     D.d_seqs.as<SeqInfo>(std::max<uint32_t>(job.nseq, 1));
     D.d_state.as<CoderState>(1);
+    D.d_classes.as<uint32_t>(kDesignListBase + (uint64_t)kDesignLists * std::max<uint32_t>(job.nseq, 1));
     D.d_calls.as<CallOut>(std::max<uint32_t>(ncalls, 1));
     D.h_result.get(sizeof(CoderState) + sizeof(CallOut) * std::max<uint32_t>(ncalls, 1));
     CCA_HIP(hipMemcpyAsync(D.d_items.p, hi, sizeof(CodeItem) * nitems, hipMemcpyHostToDevice, st));
@@ -107,7 +108,8 @@ void encode_launch(DeviceCoder& D, const EncodeJob& job, uint32_t* d_out, uint64
                    const LayoutLink* link) {
     SideStreams ss;
     coder_encode((const CodeItem*)D.d_items.p, job.nitems, (SeqInfo*)D.d_seqs.p, job.nseq, job.p, job.symSize,
-                 (uint32_t*)D.d_scratch.p, job.scratch_cap, (CoderState*)D.d_state.p, (CallOut*)D.d_calls.p,
+                 (uint32_t*)D.d_scratch.p, job.scratch_cap, (CoderState*)D.d_state.p, (uint32_t*)D.d_classes.p,
+                 (CallOut*)D.d_calls.p,
                  job.ncalls, (const uint32_t*)D.d_call_first.p, d_out, out_cap, job.total_syms, st,
                  side_streams(D, ss), link);
 }

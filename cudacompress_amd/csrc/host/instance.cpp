@@ -57,7 +57,7 @@ DeviceGuard::~DeviceGuard() {
 }
 
 void DeviceCoder::release() {
-    DevBuf* devs[] = {&d_items, &d_seqs, &d_scratch, &d_state, &d_calls, &d_call_first, &d_dcalls,
+    DevBuf* devs[] = {&d_items, &d_seqs, &d_scratch, &d_state, &d_classes, &d_calls, &d_call_first, &d_dcalls,
                       &d_ditems, &d_dseqs, &d_dscratch, &d_dstate, &d_dout, &d_waves};
     for (auto* b : devs) b->release();
     h_stage.release();

@@ -44,7 +44,7 @@ constexpr int kDecodeRing = 4;  // decode staging slots (the reference keeps 8, 
 
 // Buffers of the device-resident coder (host/batch.cpp).
 struct DeviceCoder {
-    DevBuf d_items, d_seqs, d_scratch, d_state, d_calls, d_call_first;
+    DevBuf d_items, d_seqs, d_scratch, d_state, d_classes, d_calls, d_call_first;
     PinBuf h_stage, h_result;
     DevBuf d_dcalls, d_ditems, d_dseqs, d_dscratch, d_dstate, d_dout, d_waves;
     PinBuf h_dstage, h_dresult;
