@@ -31,33 +31,99 @@ __device__ __forceinline__ uint32_t find_brick(const uint32_t* tb, uint32_t nb, 
     return lo;
 }
 
+// Brick data is accessed through global-address-space pointers (the descriptors hold generic
+// ones), so loads and stores use global instructions with scalar bases.
+template <class T>
+__device__ __forceinline__ T gld(const T* p, int i) {
+    return ((const __attribute__((address_space(1))) T*)p)[i];
+}
+template <class T>
+__device__ __forceinline__ void gst(T* p, int i, T v) {
+    ((__attribute__((address_space(1))) T*)p)[i] = v;
+}
+
+// Extensions with one reflection, valid when the line is long enough for the window (forward:
+// n >= 5; synthesis: m >= 3); shorter lines use the repeated forms of dwt_ops.hpp.
+__device__ __forceinline__ int mirror1(int j, int n) {
+    j = j < 0 ? -j : j;
+    return j >= n ? 2 * (n - 1) - j : j;
+}
+__device__ __forceinline__ int ext_low1(int g, int m) {
+    g = g < 0 ? -g : g;
+    return g >= m ? 2 * m - 1 - g : g;
+}
+__device__ __forceinline__ int ext_high1(int g, int m) {
+    g = g < 0 ? -g - 1 : g;
+    return g >= m ? 2 * m - 2 - g : g;
+}
+
 template <int AX>
 struct Geo {
-    int n, m, ncols, lx, ly;
+    int n, m, ncols, lx, ly, stride;
     __device__ Geo(int x, int y, int z) : lx(x), ly(y) {
         n = AX == 0 ? x : (AX == 1 ? y : z);
         m = n / 2;
         ncols = AX == 0 ? y * z : (AX == 1 ? x * z : x * y);
+        stride = AX == 0 ? 1 : (AX == 1 ? x : x * y);
     }
-    // element (column c, position p along the axis) of a dense lx*ly*lz brick
-    __device__ long long at(int c, int p) const {
-        if (AX == 0) return (long long)c * lx + p;
-        if (AX == 1) return (long long)(c % lx) + (long long)p * lx + (long long)(c / lx) * lx * ly;
-        return (long long)c + (long long)p * lx * ly;
+    // element offset of position 0 of column c in a dense lx*ly*lz brick (sizes < 2^31 elements)
+    __device__ int base(int c) const {
+        if (AX == 0) return c * lx;
+        if (AX == 1) return (c % lx) + (c / lx) * lx * ly;
+        return c;
     }
 };
 
-// Forward tile: fetch(c, p) -> float, sink(c, p_out, v).
+// Window of a forward tile: every lane issues all of its loads before writing any to LDS.
+// fetch(c, position, offset) -> float.  LONG: one reflection suffices (n >= 5).
+template <int AX, bool LONG, class Fetch>
+__device__ __forceinline__ void fwd_window(const Geo<AX>& g, int c0, int k0, float (*lds)[LW], Fetch fetch) {
+    const int tid = threadIdx.x;
+    constexpr int kLoads = (NC * WF + 255) / 256;
+    float v[kLoads];
+    if (AX == 0) {
+#pragma unroll
+        for (int t = 0; t < kLoads; t++) {
+            const int i = tid + 256 * t;
+            const int col = i / WF, r = i % WF;
+            const int c = c0 + col, p = 2 * k0 - Fwd97::H + r;
+            const bool ok = i < NC * WF && c < g.ncols && p < g.n + Fwd97::H;
+            const int pm = LONG ? mirror1(p, g.n) : mirror(p, g.n);
+            v[t] = ok ? fetch(c, pm, c * g.lx + pm) : 0.0f;
+        }
+#pragma unroll
+        for (int t = 0; t < kLoads; t++) {
+            const int i = tid + 256 * t;
+            if (i < NC * WF) lds[i / WF][i % WF] = v[t];
+        }
+    } else {
+        const int col = tid & 63, c = c0 + col;
+        const bool cv = c < g.ncols;
+        const int cb = cv ? g.base(c) : 0;
+#pragma unroll
+        for (int t = 0; t < kLoads; t++) {
+            const int r = (tid >> 6) + 4 * t, p = 2 * k0 - Fwd97::H + r;
+            const bool ok = r < WF && cv && p < g.n + Fwd97::H;
+            const int pm = LONG ? mirror1(p, g.n) : mirror(p, g.n);
+            v[t] = ok ? fetch(c, pm, cb + pm * g.stride) : 0.0f;
+        }
+#pragma unroll
+        for (int t = 0; t < kLoads; t++) {
+            const int r = (tid >> 6) + 4 * t;
+            if (r < WF) lds[col][r] = v[t];
+        }
+    }
+}
+
+// Forward tile of NC columns x SP pairs: the window (fwd_window), the analysis in LDS, then
+// sink(c, output position, offset, v).
 template <int AX, class Fetch, class Sink>
 __device__ void fwd_tile(const Geo<AX>& g, uint32_t local, float (*lds)[LW], Fetch fetch, Sink sink) {
     const int ctiles = (g.ncols + NC - 1) / NC;
     const int c0 = (int)(local % ctiles) * NC, k0 = (int)(local / ctiles) * SP;
     const int tid = threadIdx.x;
-    for (int i = tid; i < NC * WF; i += 256) {
-        const int col = AX == 0 ? i / WF : i % NC, r = AX == 0 ? i % WF : i / NC;
-        const int c = c0 + col, p = 2 * k0 - Fwd97::H + r;
-        if (c < g.ncols && p < g.n + Fwd97::H) lds[col][r] = fetch(c, mirror(p, g.n));
-    }
+    if (g.n >= 5) fwd_window<AX, true>(g, c0, k0, lds, fetch);
+    else fwd_window<AX, false>(g, c0, k0, lds, fetch);
     __syncthreads();
     const int col = tid & 63;
     float lo[SP / 4], hi[SP / 4];
@@ -75,25 +141,81 @@ __device__ void fwd_tile(const Geo<AX>& g, uint32_t local, float (*lds)[LW], Fet
         lds[col][SP + q] = hi[j];
     }
     __syncthreads();
-    for (int i = tid; i < NC * 2 * SP; i += 256) {
-        const int cc = AX == 0 ? i / (2 * SP) : i % NC, j = AX == 0 ? i % (2 * SP) : i / NC;
-        const int q = j % SP, k = k0 + q, c = c0 + cc;
-        if (c < g.ncols && k < g.m) sink(c, j >= SP ? g.m + k : k, lds[cc][j]);
+    if (AX == 0) {
+        for (int i = tid; i < NC * 2 * SP; i += 256) {
+            const int cc = i / (2 * SP), j = i % (2 * SP);
+            const int q = j % SP, k = k0 + q, c = c0 + cc;
+            const int po = j >= SP ? g.m + k : k;
+            if (c < g.ncols && k < g.m) sink(c, po, c * g.lx + po, lds[cc][j]);
+        }
+    } else {
+        const int c = c0 + col;
+        const int cb = c < g.ncols ? g.base(c) : 0;
+        if (c < g.ncols)
+            for (int j = tid >> 6; j < 2 * SP; j += 4) {
+                const int q = j % SP, k = k0 + q;
+                const int po = j >= SP ? g.m + k : k;
+                if (k < g.m) sink(c, po, cb + po * g.stride, lds[col][j]);
+            }
     }
 }
 
-// Inverse tile: fetch(c, coefficient index along the axis), sink(c, output position, v).
-template <int AX, class Fetch, class Sink>
-__device__ void inv_tile(const Geo<AX>& g, uint32_t local, float (*lds)[LW], Fetch fetch, Sink sink) {
+// Window of an inverse tile: lane-owned entries are loaded raw (32 bits each) in one batch, then
+// converted and stored to LDS.  load(c, coefficient index, offset) -> uint32, conv(c, index, raw)
+// -> float.  LONG: the one-reflection extensions apply (m >= 3).
+template <int AX, bool LONG, class Load, class Conv>
+__device__ __forceinline__ void inv_window(const Geo<AX>& g, int c0, int k0, float (*lds)[LW], Load load, Conv conv) {
+    const int tid = threadIdx.x;
+    constexpr int kLoads = (NC * 2 * NPI + 255) / 256;
+    auto coef = [&](int v) {  // coefficient index of window entry v (interleaved low/high)
+        const int gi = k0 - Inv97::G + (v >> 1);
+        if (v & 1) return g.m + (LONG ? ext_high1(gi, g.m) : ext_high(gi, g.m));
+        return LONG ? ext_low1(gi, g.m) : ext_low(gi, g.m);
+    };
+    uint32_t raw[kLoads];
+    if (AX == 0) {
+#pragma unroll
+        for (int t = 0; t < kLoads; t++) {
+            const int i = tid + 256 * t;
+            const int col = i / (2 * NPI), v = i % (2 * NPI);
+            const int c = c0 + col, gi = k0 - Inv97::G + (v >> 1);
+            const bool ok = i < NC * 2 * NPI && c < g.ncols && gi < g.m + Inv97::G;
+            const int ci = coef(v);
+            raw[t] = ok ? load(c, ci, c * g.lx + ci) : 0u;
+        }
+#pragma unroll
+        for (int t = 0; t < kLoads; t++) {
+            const int i = tid + 256 * t;
+            const int col = i / (2 * NPI), v = i % (2 * NPI);
+            if (i < NC * 2 * NPI) lds[col][v] = conv(c0 + col, coef(v), raw[t]);
+        }
+    } else {
+        const int col = tid & 63, c = c0 + col;
+        const bool cv = c < g.ncols;
+        const int cb = cv ? g.base(c) : 0;
+#pragma unroll
+        for (int t = 0; t < kLoads; t++) {
+            const int v = (tid >> 6) + 4 * t, gi = k0 - Inv97::G + (v >> 1);
+            const bool ok = v < 2 * NPI && cv && gi < g.m + Inv97::G;
+            const int ci = coef(v);
+            raw[t] = ok ? load(c, ci, cb + ci * g.stride) : 0u;
+        }
+#pragma unroll
+        for (int t = 0; t < kLoads; t++) {
+            const int v = (tid >> 6) + 4 * t;
+            if (v < 2 * NPI) lds[col][v] = conv(c, coef(v), raw[t]);
+        }
+    }
+}
+
+// Inverse tile: the window (inv_window), the synthesis in LDS, sink(c, output position, offset, v).
+template <int AX, class Load, class Conv, class Sink>
+__device__ void inv_tile(const Geo<AX>& g, uint32_t local, float (*lds)[LW], Load load, Conv conv, Sink sink) {
     const int ctiles = (g.ncols + NC - 1) / NC;
     const int c0 = (int)(local % ctiles) * NC, k0 = (int)(local / ctiles) * SP;
     const int tid = threadIdx.x;
-    for (int i = tid; i < NC * 2 * NPI; i += 256) {
-        const int col = AX == 0 ? i / (2 * NPI) : i % NC, v = AX == 0 ? i % (2 * NPI) : i / NC;
-        const int q = v >> 1, gi = k0 - Inv97::G + q, c = c0 + col;
-        if (c < g.ncols && gi < g.m + Inv97::G)
-            lds[col][v] = fetch(c, (v & 1) ? g.m + ext_high(gi, g.m) : ext_low(gi, g.m));
-    }
+    if (g.m >= 3) inv_window<AX, true>(g, c0, k0, lds, load, conv);
+    else inv_window<AX, false>(g, c0, k0, lds, load, conv);
     __syncthreads();
     const int col = tid & 63;
     float o[2 * SP / 4];
@@ -107,23 +229,33 @@ __device__ void inv_tile(const Geo<AX>& g, uint32_t local, float (*lds)[LW], Fet
 #pragma unroll
     for (int j = 0; j < 2 * SP / 4; j++) lds[col][(tid >> 6) + 4 * j] = o[j];
     __syncthreads();
-    for (int i = tid; i < NC * 2 * SP; i += 256) {
-        const int cc = AX == 0 ? i / (2 * SP) : i % NC, t = AX == 0 ? i % (2 * SP) : i / NC;
-        const int jo = 2 * k0 + t, c = c0 + cc;
-        if (c < g.ncols && jo < g.n) sink(c, jo, lds[cc][t]);
+    if (AX == 0) {
+        for (int i = tid; i < NC * 2 * SP; i += 256) {
+            const int cc = i / (2 * SP), t = i % (2 * SP);
+            const int jo = 2 * k0 + t, c = c0 + cc;
+            if (c < g.ncols && jo < g.n) sink(c, jo, c * g.lx + jo, lds[cc][t]);
+        }
+    } else {
+        const int c = c0 + col;
+        const int cb = c < g.ncols ? g.base(c) : 0;
+        if (c < g.ncols)
+            for (int t = tid >> 6; t < 2 * SP; t += 4) {
+                const int jo = 2 * k0 + t;
+                if (jo < g.n) sink(c, jo, cb + jo * g.stride, lds[col][t]);
+            }
     }
 }
 
 // band index of a level coefficient and its position inside the dense band
 struct Band {
     int b;
-    long long li;
+    int li;
 };
 __device__ __forceinline__ Band band_of(int x, int y, int z, int hx, int hy, int hz) {
     const int bx = x >= hx, by = y >= hy, bz = z >= hz;
     Band r;
     r.b = bx + 2 * by + 4 * bz;
-    r.li = ((long long)(z - bz * hz) * hy + (y - by * hy)) * hx + (x - bx * hx);
+    r.li = ((z - bz * hz) * hy + (y - by * hy)) * hx + (x - bx * hx);
     return r;
 }
 
@@ -135,8 +267,8 @@ __global__ __launch_bounds__(256) void k_bfwd(const BrickLevel* lv, const uint32
     const Geo<AX> g(L.lx, L.ly, L.lz);
     const float* src = AX == 0 ? L.in : L.a;
     float* dst = AX == 0 ? L.a : L.b;
-    fwd_tile<AX>(g, blockIdx.x - tb[i], lds, [&](int c, int p) { return src[g.at(c, p)]; },
-                 [&](int c, int p, float v) { dst[g.at(c, p)] = v; });
+    fwd_tile<AX>(g, blockIdx.x - tb[i], lds, [&](int, int, int off) { return gld(src, off); },
+                 [&](int, int, int off, float v) { gst(dst, off, v); });
 }
 
 template <class S>
@@ -146,17 +278,17 @@ __global__ __launch_bounds__(256) void k_bfwd_zq(const BrickLevel* lv, const uin
     const BrickLevel L = lv[i];
     const Geo<2> g(L.lx, L.ly, L.lz);
     const int hx = L.lx / 2, hy = L.ly / 2, hz = L.lz / 2;
-    const long long band = (long long)hx * hy * hz;
+    const int band = hx * hy * hz;
     S* hp = static_cast<S*>(L.hp);
     S* lps = static_cast<S*>(L.lpsym);
-    fwd_tile<2>(g, blockIdx.x - tb[i], lds, [&](int c, int p) { return L.b[g.at(c, p)]; },
-                [&](int c, int p, float v) {
+    fwd_tile<2>(g, blockIdx.x - tb[i], lds, [&](int, int, int off) { return gld<float>(L.b, off); },
+                [&](int c, int p, int, float v) {
                     const Band bd = band_of(c % L.lx, c / L.lx, p, hx, hy, hz);
                     if (bd.b == 0) {
-                        if (L.last) lps[bd.li] = (S)symbolize_i(quantize_f(v, L.inv_lp, 0));
-                        else L.lp[bd.li] = v;
+                        if (L.last) gst(lps, bd.li, (S)symbolize_i(quantize_f(v, L.inv_lp, 0)));
+                        else gst(L.lp, (int)bd.li, v);
                     } else {
-                        hp[(bd.b - 1) * band + bd.li] = (S)symbolize_i(quantize_f(v, L.inv_hp, 0));
+                        gst(hp, (bd.b - 1) * band + (int)bd.li, (S)symbolize_i(quantize_f(v, L.inv_hp, 0)));
                     }
                 });
 }
@@ -168,17 +300,23 @@ __global__ __launch_bounds__(256) void k_binv_zs(const BrickLevel* lv, const uin
     const BrickLevel L = lv[i];
     const Geo<2> g(L.lx, L.ly, L.lz);
     const int hx = L.lx / 2, hy = L.ly / 2, hz = L.lz / 2;
-    const long long band = (long long)hx * hy * hz;
+    const int band = hx * hy * hz;
     const S* hp = static_cast<const S*>(L.hp);
     const S* lps = static_cast<const S*>(L.lpsym);
+    // raw entries: the symbol, or the lowpass float's bits (lowpass of a level above the coarsest)
     inv_tile<2>(g, blockIdx.x - tb[i], lds,
-                [&](int c, int p) -> float {
+                [&](int c, int p, int) -> uint32_t {
                     const Band bd = band_of(c % L.lx, c / L.lx, p, hx, hy, hz);
-                    if (bd.b == 0)
-                        return L.last ? unquantize_i(unsymbolize_u((uint32_t)lps[bd.li]), L.step_lp, 0) : L.lp[bd.li];
-                    return unquantize_i(unsymbolize_u((uint32_t)hp[(bd.b - 1) * band + bd.li]), L.step_hp, 0);
+                    if (bd.b == 0 && !L.last) return __float_as_uint(gld<float>(L.lp, bd.li));
+                    const S* sp = bd.b == 0 ? lps : hp + (bd.b - 1) * band;
+                    return (uint32_t)gld(sp, bd.li);
                 },
-                [&](int c, int p, float v) { L.a[g.at(c, p)] = v; });
+                [&](int c, int p, uint32_t raw) -> float {
+                    const Band bd = band_of(c % L.lx, c / L.lx, p, hx, hy, hz);
+                    if (bd.b == 0 && !L.last) return __uint_as_float(raw);
+                    return unquantize_i(unsymbolize_u(raw), bd.b == 0 ? L.step_lp : L.step_hp, 0);
+                },
+                [&](int, int, int off, float v) { gst(L.a, off, v); });
 }
 
 template <int AX>
@@ -189,8 +327,9 @@ __global__ __launch_bounds__(256) void k_binv(const BrickLevel* lv, const uint32
     const Geo<AX> g(L.lx, L.ly, L.lz);
     const float* src = AX == 1 ? L.a : L.b;
     float* dst = AX == 1 ? L.b : L.out;
-    inv_tile<AX>(g, blockIdx.x - tb[i], lds, [&](int c, int p) { return src[g.at(c, p)]; },
-                 [&](int c, int p, float v) { dst[g.at(c, p)] = v; });
+    inv_tile<AX>(g, blockIdx.x - tb[i], lds, [&](int, int, int off) { return __float_as_uint(gld(src, off)); },
+                 [&](int, int, uint32_t raw) { return __uint_as_float(raw); },
+                 [&](int, int, int off, float v) { gst(dst, off, v); });
 }
 
 }  // namespace
