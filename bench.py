@@ -54,6 +54,7 @@ def parse():
     ap.add_argument("--serving-slots", type=int, default=4)
     ap.add_argument("--serving-rates", default="1000,5000,10000,20000,50000", help="arrival rates, bricks/s")
     ap.add_argument("--bricks", type=int, default=2048)
+    ap.add_argument("--groups", type=int, default=0, help="brick groups of eager batches (0: library default)")
     ap.add_argument("--mode", default="plan", choices=["plan", "eager"],
                     help="plan: compression and decompression replayed from hipGraphs (BricksPlan); "
                          "eager: compressBricksFloat + decompressBricksFloat calls")
@@ -129,6 +130,8 @@ def run_bricks(args, torch, cc, lib):
     nvox = sum(b.size for b in bricks)
     out = torch.empty(nvox + 4096 * len(bricks), dtype=torch.int32, device=dev)
     inst = cc.createInstance(0, 7, 128 * 128 * 128 // 8, 128, 14)
+    if args.groups:
+        inst.setBrickGroups(args.groups)
     state = {}
     plan = cc.BricksPlan(inst, srcs, dsts, shapes, steps_q, 2, out) if args.mode == "plan" else None
 

@@ -253,9 +253,17 @@ __global__ __launch_bounds__(64) void k_dec_blocks(const DecSeq* seqs, const uin
     const uint32_t* gsyms = scratch + s.lut_off + (1u << kLut);
     auto symbol_at = [&](int idx) -> uint32_t { return (idx >= 0 && (uint32_t)idx < nsym) ? gsyms[idx] : 0u; };
     if (multi) {
+        // all of the lane's table loads in flight at once, then the LDS stores
         const uint32_t* glut = scratch + s.lut_off;
-        for (int i = lane; i < 96; i += 64) tab[i] = s.tab[i];
-        for (uint32_t e = lane; e < (1u << kLut); e += 64) lut[e] = glut[e];
+        constexpr int kPer = (1 << kLut) / 64;
+        uint32_t lv[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; k++) lv[k] = glut[lane + 64 * k];
+        const int t0 = s.tab[lane], t1 = lane < 32 ? s.tab[64 + lane] : 0;
+#pragma unroll
+        for (int k = 0; k < kPer; k++) lut[lane + 64 * k] = lv[k];
+        tab[lane] = t0;
+        if (lane < 32) tab[64 + lane] = t1;
         __syncthreads();
     }
     const uint32_t single = nsym >= 1 ? symbol_at(0) : 0u;
@@ -265,18 +273,22 @@ __global__ __launch_bounds__(64) void k_dec_blocks(const DecSeq* seqs, const uin
     const uint32_t* words = s.base + s.words_off;
     const uint64_t avail = s.nwords - s.words_off;  // readable words from the codeword start
     auto word = [&](uint64_t i) -> uint32_t { return i < avail ? words[i] : 0u; };
-    // bit window: `cache` holds 64 bits starting at the current word, `pf` the word after them,
-    // loaded one refill ahead so that its latency overlaps the decoding of the cached bits
+    // bit window: `cache` holds 64 bits starting at the current word, pf0..pf3 the four words
+    // after them, each loaded four refills before it is used so that its latency overlaps the
+    // decoding of the bits in between
     uint64_t cache = 0;
-    uint32_t bp = 0, pf = 0;
+    uint32_t bp = 0, pf0 = 0, pf1 = 0, pf2 = 0, pf3 = 0;
     uint64_t nxt = 0;
     if (valid && multi) {
         const uint32_t pos = b == 0 ? 0u : scratch[s.boff_off + b];
         const uint64_t wi = pos >> 5;
         cache = ((uint64_t)word(wi) << 32) | (uint64_t)word(wi + 1);
-        pf = word(wi + 2);
+        pf0 = word(wi + 2);
+        pf1 = word(wi + 3);
+        pf2 = word(wi + 4);
+        pf3 = word(wi + 5);
         bp = pos & 31u;
-        nxt = wi + 3;
+        nxt = wi + 6;
     }
     const uint32_t mlen = s.nlen;
     for (uint32_t r = 0; r < cbs; r += 32) {
@@ -285,8 +297,11 @@ __global__ __launch_bounds__(64) void k_dec_blocks(const DecSeq* seqs, const uin
             uint32_t sym;
             if (multi) {
                 if (bp >= 32) {
-                    cache = (cache << 32) | (uint64_t)pf;
-                    pf = word(nxt);
+                    cache = (cache << 32) | (uint64_t)pf0;
+                    pf0 = pf1;
+                    pf1 = pf2;
+                    pf2 = pf3;
+                    pf3 = word(nxt);
                     nxt++;
                     bp -= 32;
                 }
