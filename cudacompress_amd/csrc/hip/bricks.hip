@@ -160,62 +160,65 @@ __device__ void fwd_tile(const Geo<AX>& g, uint32_t local, float (*lds)[LW], Fet
     }
 }
 
-// Window of an inverse tile: lane-owned entries are loaded raw (32 bits each) in one batch, then
-// converted and stored to LDS.  load(c, coefficient index, offset) -> uint32, conv(c, index, raw)
-// -> float.  LONG: the one-reflection extensions apply (m >= 3).
-template <int AX, bool LONG, class Load, class Conv>
+// Window of an inverse tile: lane-owned entries are loaded raw (32 bits each), BATCH loads in
+// flight at a time, then converted and stored to LDS.  load(c, coefficient index, offset) ->
+// uint32, conv(c, index, raw) -> float.  LONG: the one-reflection extensions apply (m >= 3).
+template <int AX, bool LONG, int BATCH, class Load, class Conv>
 __device__ __forceinline__ void inv_window(const Geo<AX>& g, int c0, int k0, float (*lds)[LW], Load load, Conv conv) {
     const int tid = threadIdx.x;
     constexpr int kLoads = (NC * 2 * NPI + 255) / 256;
+    static_assert(kLoads % BATCH == 0, "batches must divide the window");
     auto coef = [&](int v) {  // coefficient index of window entry v (interleaved low/high)
         const int gi = k0 - Inv97::G + (v >> 1);
         if (v & 1) return g.m + (LONG ? ext_high1(gi, g.m) : ext_high(gi, g.m));
         return LONG ? ext_low1(gi, g.m) : ext_low(gi, g.m);
     };
-    uint32_t raw[kLoads];
-    if (AX == 0) {
+    for (int t0 = 0; t0 < kLoads; t0 += BATCH) {
+        uint32_t raw[BATCH];
+        if (AX == 0) {
 #pragma unroll
-        for (int t = 0; t < kLoads; t++) {
-            const int i = tid + 256 * t;
-            const int col = i / (2 * NPI), v = i % (2 * NPI);
-            const int c = c0 + col, gi = k0 - Inv97::G + (v >> 1);
-            const bool ok = i < NC * 2 * NPI && c < g.ncols && gi < g.m + Inv97::G;
-            const int ci = coef(v);
-            raw[t] = ok ? load(c, ci, c * g.lx + ci) : 0u;
-        }
+            for (int u = 0; u < BATCH; u++) {
+                const int i = tid + 256 * (t0 + u);
+                const int col = i / (2 * NPI), v = i % (2 * NPI);
+                const int c = c0 + col, gi = k0 - Inv97::G + (v >> 1);
+                const bool ok = i < NC * 2 * NPI && c < g.ncols && gi < g.m + Inv97::G;
+                const int ci = coef(v);
+                raw[u] = ok ? load(c, ci, c * g.lx + ci) : 0u;
+            }
 #pragma unroll
-        for (int t = 0; t < kLoads; t++) {
-            const int i = tid + 256 * t;
-            const int col = i / (2 * NPI), v = i % (2 * NPI);
-            if (i < NC * 2 * NPI) lds[col][v] = conv(c0 + col, coef(v), raw[t]);
-        }
-    } else {
-        const int col = tid & 63, c = c0 + col;
-        const bool cv = c < g.ncols;
-        const int cb = cv ? g.base(c) : 0;
+            for (int u = 0; u < BATCH; u++) {
+                const int i = tid + 256 * (t0 + u);
+                const int col = i / (2 * NPI), v = i % (2 * NPI);
+                if (i < NC * 2 * NPI) lds[col][v] = conv(c0 + col, coef(v), raw[u]);
+            }
+        } else {
+            const int col = tid & 63, c = c0 + col;
+            const bool cv = c < g.ncols;
+            const int cb = cv ? g.base(c) : 0;
 #pragma unroll
-        for (int t = 0; t < kLoads; t++) {
-            const int v = (tid >> 6) + 4 * t, gi = k0 - Inv97::G + (v >> 1);
-            const bool ok = v < 2 * NPI && cv && gi < g.m + Inv97::G;
-            const int ci = coef(v);
-            raw[t] = ok ? load(c, ci, cb + ci * g.stride) : 0u;
-        }
+            for (int u = 0; u < BATCH; u++) {
+                const int v = (tid >> 6) + 4 * (t0 + u), gi = k0 - Inv97::G + (v >> 1);
+                const bool ok = v < 2 * NPI && cv && gi < g.m + Inv97::G;
+                const int ci = coef(v);
+                raw[u] = ok ? load(c, ci, cb + ci * g.stride) : 0u;
+            }
 #pragma unroll
-        for (int t = 0; t < kLoads; t++) {
-            const int v = (tid >> 6) + 4 * t;
-            if (v < 2 * NPI) lds[col][v] = conv(c, coef(v), raw[t]);
+            for (int u = 0; u < BATCH; u++) {
+                const int v = (tid >> 6) + 4 * (t0 + u);
+                if (v < 2 * NPI) lds[col][v] = conv(c, coef(v), raw[u]);
+            }
         }
     }
 }
 
 // Inverse tile: the window (inv_window), the synthesis in LDS, sink(c, output position, offset, v).
-template <int AX, class Load, class Conv, class Sink>
+template <int AX, int BATCH, class Load, class Conv, class Sink>
 __device__ void inv_tile(const Geo<AX>& g, uint32_t local, float (*lds)[LW], Load load, Conv conv, Sink sink) {
     const int ctiles = (g.ncols + NC - 1) / NC;
     const int c0 = (int)(local % ctiles) * NC, k0 = (int)(local / ctiles) * SP;
     const int tid = threadIdx.x;
-    if (g.m >= 3) inv_window<AX, true>(g, c0, k0, lds, load, conv);
-    else inv_window<AX, false>(g, c0, k0, lds, load, conv);
+    if (g.m >= 3) inv_window<AX, true, BATCH>(g, c0, k0, lds, load, conv);
+    else inv_window<AX, false, BATCH>(g, c0, k0, lds, load, conv);
     __syncthreads();
     const int col = tid & 63;
     float o[2 * SP / 4];
@@ -304,7 +307,8 @@ __global__ __launch_bounds__(256) void k_binv_zs(const BrickLevel* lv, const uin
     const S* hp = static_cast<const S*>(L.hp);
     const S* lps = static_cast<const S*>(L.lpsym);
     // raw entries: the symbol, or the lowpass float's bits (lowpass of a level above the coarsest)
-    inv_tile<2>(g, blockIdx.x - tb[i], lds,
+    // half the window per batch keeps the register count (and occupancy) of the plain passes
+    inv_tile<2, 9>(g, blockIdx.x - tb[i], lds,
                 [&](int c, int p, int) -> uint32_t {
                     const Band bd = band_of(c % L.lx, c / L.lx, p, hx, hy, hz);
                     if (bd.b == 0 && !L.last) return __float_as_uint(gld<float>(L.lp, bd.li));
@@ -327,7 +331,7 @@ __global__ __launch_bounds__(256) void k_binv(const BrickLevel* lv, const uint32
     const Geo<AX> g(L.lx, L.ly, L.lz);
     const float* src = AX == 1 ? L.a : L.b;
     float* dst = AX == 1 ? L.b : L.out;
-    inv_tile<AX>(g, blockIdx.x - tb[i], lds, [&](int, int, int off) { return __float_as_uint(gld(src, off)); },
+    inv_tile<AX, 18>(g, blockIdx.x - tb[i], lds, [&](int, int, int off) { return __float_as_uint(gld(src, off)); },
                  [&](int, int, uint32_t raw) { return __uint_as_float(raw); },
                  [&](int, int, int off, float v) { gst(dst, off, v); });
 }

@@ -600,17 +600,30 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_pack(const CodeItem* ite
     const bool sglob = ss.smax >= kStatBins;
     const uint32_t* gcode = scratch + ss.code_off;
     const uint32_t* glen = gcode + ss.smax + 1;
-    if (!sglob)
-        for (uint32_t i = threadIdx.x; i <= ss.smax; i += kCoderThreads) {
-            cs[i] = gcode[i];
-            lsh[i] = (unsigned char)glen[i];
+    // tables to LDS, each thread's loads issued four entries at a time
+    auto stage_table = [&](const uint32_t* code, const uint32_t* len, uint32_t nb, uint32_t* dc, unsigned char* dl) {
+        for (uint32_t i0 = threadIdx.x; i0 < nb; i0 += 4 * kCoderThreads) {
+            uint32_t c[4], l[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint32_t i = i0 + k * kCoderThreads;
+                c[k] = i < nb ? code[i] : 0u;
+                l[k] = i < nb ? len[i] : 0u;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint32_t i = i0 + k * kCoderThreads;
+                if (i < nb) {
+                    dc[i] = c[k];
+                    dl[i] = (unsigned char)l[k];
+                }
+            }
         }
+    };
+    if (!sglob) stage_table(gcode, glen, ss.smax + 1, cs, lsh);
     if (rl) {
         const uint32_t* zc = scratch + sz.code_off;
-        for (uint32_t i = threadIdx.x; i <= sz.smax; i += kCoderThreads) {
-            cz[i] = zc[i];
-            lz[i] = (unsigned char)zc[sz.smax + 1 + i];
-        }
+        stage_table(zc, zc + sz.smax + 1, sz.smax + 1, cz, lz);
     }
     for (uint32_t i = threadIdx.x; i < 2 * kWindowWords; i += kCoderThreads) (&win[0][0])[i] = 0;
     __syncthreads();
