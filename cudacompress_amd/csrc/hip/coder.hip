@@ -649,54 +649,95 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_pack(const CodeItem* ite
         long long chunk_max;
         long long prev = wg_excl_max_i64(thread_last_nz(v, p0, n), shl, chunk_max);
         prev = prev > carry ? prev : carry;
-        // counts and bits of this thread's pairs
-        uint32_t np = 0, nbs = 0, nbz = 0;
-        thread_pairs(v, p0, n, rl, prev, rl ? limit : (long long)n, [&](uint32_t s, uint32_t z) {
-            np++;
-            nbs += single ? 0u : (sglob ? glen[s] : (uint32_t)lsh[s]);
-            if (rl) nbz += zsingle ? 0u : (uint32_t)lz[z];
-        });
+        // this thread's pairs, branch-free over its 8 positions: a pair is emitted for every symbol
+        // (plain streams), every nonzero, and every 256th zero of a run that a nonzero ends; its
+        // zero count is the run length before it modulo 256
+        const uint32_t nvalid = p0 >= n ? 0u : (n - p0 >= kPerThread ? kPerThread : (uint32_t)(n - p0));
+        const long long lim_rel = limit - (long long)p0;
+        uint32_t g = (uint32_t)((long long)p0 - prev - 1);
+        uint32_t emit = 0, zc[kPerThread], ls[kPerThread], lzs[kPerThread];
+        uint32_t nbs = 0, nbz = 0;
+#pragma unroll
+        for (int j = 0; j < (int)kPerThread; j++) {
+            const bool nz = v[j] != 0u;
+            const uint32_t z = g & 255u;
+            const bool e = (uint32_t)j < nvalid && (!rl || nz || (z == 255u && (long long)j < lim_rel));
+            g = nz ? 0u : g + 1u;
+            zc[j] = z;
+            emit |= (uint32_t)e << j;
+            uint32_t l = 0, m = 0;
+            if (e && !single) l = sglob ? glen[v[j]] : (uint32_t)lsh[v[j]];
+            if (e && rl && !zsingle) m = lz[z];
+            ls[j] = l;
+            lzs[j] = m;
+            nbs += l;
+            nbz += m;
+        }
+        const uint32_t np = __popc(emit);
         uint64_t tot;
         const uint64_t ex = wg_excl_sum_u64(((uint64_t)np << 40) | ((uint64_t)nbs << 20) | nbz, shq, tot);
-        uint64_t pi = pairs + (ex >> 40);
-        uint64_t os = cbits_s + ((ex >> 20) & 0xFFFFF);
-        uint64_t oz = cbits_z + (ex & 0xFFFFF);
+        const uint64_t os0 = cbits_s + ((ex >> 20) & 0xFFFFF);
+        const uint64_t oz0 = cbits_z + (ex & 0xFFFFF);
         const uint64_t wbs = cbits_s >> 5, wbz = cbits_z >> 5;
-        thread_pairs(v, p0, n, rl, prev, rl ? limit : (long long)n, [&](uint32_t s, uint32_t z) {
-            if (pi >= M) return;  // cannot happen when stats and pack agree; keeps writes in bounds
-            if (pi % cbs == 0) {
-                bs[pi / cbs] = (uint32_t)os;
-                if (rl) bz[pi / cbs] = (uint32_t)oz;
-            }
-            if (!single) {
-                const uint32_t l = sglob ? glen[s] : (uint32_t)lsh[s];
-                const uint32_t c = sglob ? gcode[s] : cs[s];
-                const uint32_t wi = (uint32_t)((os >> 5) - wbs), o = (uint32_t)(os & 31);
-                if (l && wi + 1 < kWindowWords) {
-                    if (o + l <= 32) {
-                        atomicOr(&win[0][wi], c << (32 - o - l));
-                    } else {
-                        atomicOr(&win[0][wi], c >> (o + l - 32));
-                        atomicOr(&win[0][wi + 1], c << (64 - o - l));
+        if (np) {
+            // coding-block starts among this thread's pairs (pair indices < M only: stats and pack
+            // agree on M, the bound keeps the offset writes in their arrays regardless)
+            uint32_t pi = (uint32_t)(pairs + (ex >> 40));
+            uint32_t rem = pi % cbs, blk = pi / cbs;
+            uint32_t os = (uint32_t)os0, oz = (uint32_t)oz0;
+#pragma unroll
+            for (int j = 0; j < (int)kPerThread; j++) {
+                if ((emit >> j) & 1u) {
+                    if (rem == 0 && pi < M) {
+                        bs[blk] = os;
+                        if (rl) bz[blk] = oz;
+                    }
+                    os += ls[j];
+                    oz += lzs[j];
+                    pi++;
+                    if (++rem == cbs) {
+                        rem = 0;
+                        blk++;
                     }
                 }
-                os += l;
+            }
+            // codewords through a 64-bit accumulator, one LDS OR per window word touched
+            if (!single) {
+                uint32_t wi = (uint32_t)((os0 >> 5) - wbs), nacc = (uint32_t)(os0 & 31);
+                uint64_t acc = 0;
+#pragma unroll
+                for (int j = 0; j < (int)kPerThread; j++) {
+                    const uint32_t l = ls[j];
+                    uint32_t c = 0;
+                    if (l) c = sglob ? gcode[v[j]] : cs[v[j]];
+                    acc = (acc << l) | c;
+                    nacc += l;
+                    if (nacc >= 32) {
+                        nacc -= 32;
+                        if (wi < kWindowWords) atomicOr(&win[0][wi], (uint32_t)(acc >> nacc));
+                        wi++;
+                    }
+                }
+                if (nacc && wi < kWindowWords) atomicOr(&win[0][wi], (uint32_t)(acc << (32 - nacc)));
             }
             if (rl && !zsingle) {
-                const uint32_t l = lz[z], c = cz[z];
-                const uint32_t wi = (uint32_t)((oz >> 5) - wbz), o = (uint32_t)(oz & 31);
-                if (l && wi + 1 < kWindowWords) {
-                    if (o + l <= 32) {
-                        atomicOr(&win[1][wi], c << (32 - o - l));
-                    } else {
-                        atomicOr(&win[1][wi], c >> (o + l - 32));
-                        atomicOr(&win[1][wi + 1], c << (64 - o - l));
+                uint32_t wi = (uint32_t)((oz0 >> 5) - wbz), nacc = (uint32_t)(oz0 & 31);
+                uint64_t acc = 0;
+#pragma unroll
+                for (int j = 0; j < (int)kPerThread; j++) {
+                    const uint32_t l = lzs[j];
+                    const uint32_t c = l ? cz[zc[j]] : 0u;
+                    acc = (acc << l) | c;
+                    nacc += l;
+                    if (nacc >= 32) {
+                        nacc -= 32;
+                        if (wi < kWindowWords) atomicOr(&win[1][wi], (uint32_t)(acc >> nacc));
+                        wi++;
                     }
                 }
-                oz += l;
+                if (nacc && wi < kWindowWords) atomicOr(&win[1][wi], (uint32_t)(acc << (32 - nacc)));
             }
-            pi++;
-        });
+        }
         __syncthreads();
         pairs += tot >> 40;
         const uint64_t nend_s = cbits_s + ((tot >> 20) & 0xFFFFF), nend_z = cbits_z + (tot & 0xFFFFF);
