@@ -111,33 +111,48 @@ __device__ __forceinline__ void load8(const S* in, uint64_t p0, uint32_t n, uint
     }
 }
 
-// Pairs of one thread's 8 positions.  prev: position of the last nonzero before them (-1 none).
-// limit: fillers are emitted only at positions < limit (the stream's last nonzero; pass A uses n
-// and removes the trailing run's fillers afterwards).  f(sym, zc) per pair, in order.
+// Pairs of one thread's 8 positions, computed without branches: a pair is emitted for every
+// symbol of a plain stream, and in a run-length stream for every nonzero and for every 256th zero
+// of a run (a filler, symbol 0 with zero count 255) at positions < limit; the pair's zero count is
+// the length of the zero run before it modulo 256.  prev: position of the last nonzero before the
+// thread's positions (-1 none); limit: the stream's last nonzero (pass A passes n and removes the
+// trailing run's fillers afterwards).  Returns the emit mask (bit j: position p0 + j) and the zero
+// counts in zc.
+__device__ __forceinline__ uint32_t thread_flags(const uint32_t (&v)[kPerThread], uint64_t p0, uint32_t n, bool rl,
+                                                 long long prev, long long limit, uint32_t (&zc)[kPerThread]) {
+    const uint32_t nvalid = p0 >= n ? 0u : (n - p0 >= kPerThread ? kPerThread : (uint32_t)(n - p0));
+    const long long lim_rel = limit - (long long)p0;
+    uint32_t g = (uint32_t)((long long)p0 - prev - 1);  // only its low 8 bits matter
+    uint32_t emit = 0;
+#pragma unroll
+    for (int j = 0; j < (int)kPerThread; j++) {
+        const bool nz = v[j] != 0u;
+        const uint32_t z = g & 255u;
+        const bool e = (uint32_t)j < nvalid && (!rl || nz || (z == 255u && (long long)j < lim_rel));
+        g = nz ? 0u : g + 1u;
+        zc[j] = rl ? z : 0u;
+        emit |= (uint32_t)e << j;
+    }
+    return emit;
+}
+
+// f(sym, zc) for each pair of thread_flags, in order.
 template <class F>
 __device__ __forceinline__ void thread_pairs(const uint32_t (&v)[kPerThread], uint64_t p0, uint32_t n, bool rl,
                                              long long prev, long long limit, F f) {
+    uint32_t zc[kPerThread];
+    const uint32_t emit = thread_flags(v, p0, n, rl, prev, limit, zc);
 #pragma unroll
-    for (int j = 0; j < (int)kPerThread; j++) {
-        const long long pos = (long long)p0 + j;
-        if (pos >= (long long)n) break;
-        if (!rl) {
-            f(v[j], 0u);
-        } else if (v[j] != 0u) {
-            f(v[j], (uint32_t)((pos - prev - 1) & 255));
-            prev = pos;
-        } else if (((pos - prev - 1) & 255) == 255 && pos < limit) {
-            f(0u, 255u);
-        }
-    }
+    for (int j = 0; j < (int)kPerThread; j++)
+        if ((emit >> j) & 1u) f(v[j], zc[j]);
 }
 
 __device__ __forceinline__ long long thread_last_nz(const uint32_t (&v)[kPerThread], uint64_t p0, uint32_t n) {
-    long long r = -1;
+    const uint32_t nvalid = p0 >= n ? 0u : (n - p0 >= kPerThread ? kPerThread : (uint32_t)(n - p0));
+    uint32_t m = 0;
 #pragma unroll
-    for (int j = 0; j < (int)kPerThread; j++)
-        if (v[j] != 0u && p0 + j < n) r = (long long)p0 + j;
-    return r;
+    for (int j = 0; j < (int)kPerThread; j++) m |= (uint32_t)(v[j] != 0u && (uint32_t)j < nvalid) << j;
+    return m ? (long long)p0 + (31 - __clz(m)) : -1ll;
 }
 
 // ------------------------------------------------------------------------ kernels
@@ -649,25 +664,16 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_pack(const CodeItem* ite
         long long chunk_max;
         long long prev = wg_excl_max_i64(thread_last_nz(v, p0, n), shl, chunk_max);
         prev = prev > carry ? prev : carry;
-        // this thread's pairs, branch-free over its 8 positions: a pair is emitted for every symbol
-        // (plain streams), every nonzero, and every 256th zero of a run that a nonzero ends; its
-        // zero count is the run length before it modulo 256
-        const uint32_t nvalid = p0 >= n ? 0u : (n - p0 >= kPerThread ? kPerThread : (uint32_t)(n - p0));
-        const long long lim_rel = limit - (long long)p0;
-        uint32_t g = (uint32_t)((long long)p0 - prev - 1);
-        uint32_t emit = 0, zc[kPerThread], ls[kPerThread], lzs[kPerThread];
+        // this thread's pairs; code lengths looked up once for both the scan and the packing
+        uint32_t zc[kPerThread], ls[kPerThread], lzs[kPerThread];
+        const uint32_t emit = thread_flags(v, p0, n, rl, prev, rl ? limit : (long long)n, zc);
         uint32_t nbs = 0, nbz = 0;
 #pragma unroll
         for (int j = 0; j < (int)kPerThread; j++) {
-            const bool nz = v[j] != 0u;
-            const uint32_t z = g & 255u;
-            const bool e = (uint32_t)j < nvalid && (!rl || nz || (z == 255u && (long long)j < lim_rel));
-            g = nz ? 0u : g + 1u;
-            zc[j] = z;
-            emit |= (uint32_t)e << j;
+            const bool e = (emit >> j) & 1u;
             uint32_t l = 0, m = 0;
             if (e && !single) l = sglob ? glen[v[j]] : (uint32_t)lsh[v[j]];
-            if (e && rl && !zsingle) m = lz[z];
+            if (e && rl && !zsingle) m = lz[zc[j]];
             ls[j] = l;
             lzs[j] = m;
             nbs += l;
