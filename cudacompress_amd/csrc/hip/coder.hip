@@ -343,6 +343,23 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_stats_wide(const CodeIte
 }
 
 // wave64 executor of the design code (one wave per workgroup)
+// Six levels of a walk over a ballot on the scalar unit: j <- 2 j + bit j of mr, from j = 1;
+// three scalar instructions a level (shift, bit test into SCC, add with carry).
+__device__ __forceinline__ uint32_t walk6(uint64_t mr) {
+    uint32_t j = 1, t;
+    asm volatile(
+        "s_lshl_b32 %1, %0, 1\n\ts_bitcmp1_b64 %2, %0\n\ts_addc_u32 %0, %1, 0\n\t"
+        "s_lshl_b32 %1, %0, 1\n\ts_bitcmp1_b64 %2, %0\n\ts_addc_u32 %0, %1, 0\n\t"
+        "s_lshl_b32 %1, %0, 1\n\ts_bitcmp1_b64 %2, %0\n\ts_addc_u32 %0, %1, 0\n\t"
+        "s_lshl_b32 %1, %0, 1\n\ts_bitcmp1_b64 %2, %0\n\ts_addc_u32 %0, %1, 0\n\t"
+        "s_lshl_b32 %1, %0, 1\n\ts_bitcmp1_b64 %2, %0\n\ts_addc_u32 %0, %1, 0\n\t"
+        "s_lshl_b32 %1, %0, 1\n\ts_bitcmp1_b64 %2, %0\n\ts_addc_u32 %0, %1, 0"
+        : "+s"(j), "=&s"(t)
+        : "s"(mr)
+        : "scc");
+    return j;
+}
+
 struct WaveEx {
     __device__ int lane() const { return (int)threadIdx.x; }
     __device__ int lanes() const { return 64; }
@@ -362,47 +379,44 @@ struct WaveEx {
     }
     // pop in the form of design::pop_by_walk: per round, lane j < 63 evaluates the child choice
     // of local node j of the six-level subtree below the round's root (one LDS read of both
-    // children's counts); the walk over the ballot of choices runs on the scalar unit; lane k ends
-    // up holding path node k (mypath) and path node k+1 (mynext).  Returns the new top.
+    // children's counts); the walk over the ballot of choices runs on the scalar unit.  With
+    // 1-based heap indices a node's children are 2i and 2i + 1, so the walk's deepest node e
+    // carries the whole round's path: its node at round depth d is e >> (6 - d).  Lane k ends up
+    // holding path node k (mypath) and path node k+1 (mynext).  Returns the new top.
     __device__ HeapEntry heap_pop(HeapEntry* h, int len) const {
         const HeapEntry v = h[len - 1];
         if (len <= 1) return v;
         const int n = len - 1, ln = lane();
         const int lim = (n - 1) / 2, lastint = (n - 2) / 2;
         const int sd = ln < 63 ? 31 - __clz(ln + 1) : 0;  // depth of this lane's local node
-        int mypath = 0, mynext = 0, base = 0, cur = 0, m;
+        int mypath = 0, mynext = 0, base = 0, m;
+        uint32_t c1 = 1;  // 1-based index of the round's root
         for (;;) {
-            const int g = (cur << sd) + ln;
-            bool right = false;
-            if (ln < 63 && g < lim) right = !(h[2 * g + 1].count < h[2 * g + 2].count);
-            const uint64_t mr = __ballot(right);
-            // the walk on the scalar unit, its six local nodes packed 7 bits each; each lane then
-            // takes its own path position (depth dk of the round) and the next one
-            uint32_t j = 0;
-            uint64_t J = 0;
-#pragma unroll
-            for (int d = 0; d < 6; d++) {
-                j = 2 * j + 1 + (uint32_t)((mr >> j) & 1ull);
-                J |= (uint64_t)j << (7 * d);
-            }
+            const int g = (int)((c1 - 1) << sd) + ln;
+            const bool inner = ln < 63 && g < lim;
+            const int gr = inner ? g : 0;  // lanes without a node read the root's children unused
+            const bool ge = !(h[2 * gr + 1].count < h[2 * gr + 2].count);
+            const uint64_t mr = (__ballot(ge) & __ballot(inner)) << 1;  // bit j: choice at 1-based local node j
+            const uint32_t j = walk6(mr);
+            const uint32_t e = (c1 << 6) | (j & 63u);
             const int dk = ln - base;
-            if (dk >= 1 && dk <= 6) mypath = (cur << dk) + (int)((J >> (7 * (dk - 1))) & 127u);
-            if (dk >= 0 && dk <= 5) mynext = (cur << (dk + 1)) + (int)((J >> (7 * dk)) & 127u);
-            const bool real = ln > base && ln <= base + 6 && ((mypath - 1) >> 1) < lim;
-            const int steps = (int)__popcll(__ballot(real));
-            if (steps < 6) {
+            if (dk >= 1 && dk <= 6) mypath = (int)(e >> (6 - dk)) - 1;
+            if (dk >= 0 && dk <= 5) mynext = (int)(e >> (5 - dk)) - 1;
+            // the path is real down to round depth d while its node at depth d - 1 has two
+            // children (index < lim); indices grow along the path, so the real depths are a prefix
+            if ((int)(e >> 1) - 1 >= lim) {
+                const int steps = (int)__popcll(__ballot(ln >= 1 && ln <= 6 && (int)(e >> ln) - 1 < lim));
                 m = base + steps;
-                if ((n & 1) == 0 && __builtin_amdgcn_readlane(mypath, m) == lastint) m++;
+                if ((n & 1) == 0 && (int)(e >> (6 - steps)) - 1 == lastint) m++;
                 break;
             }
             base += 6;
-            cur = __builtin_amdgcn_readlane(mypath, base);
+            c1 = e;
         }
         // lanes 0..m-1 read their successor; v lands at i* = m - #{k in 1..m : v.count < count(p_k)}
-        const HeapEntry pn = ln < m ? h[mynext] : HeapEntry{0u, 0u};
+        const HeapEntry pn = h[ln < m ? mynext : 0];
         const int is = m - (int)__popcll(__ballot(ln < m && v.count < pn.count));
-        if (ln < is) h[mypath] = pn;
-        else if (ln == is) h[mypath] = v;
+        if (ln <= is) h[mypath] = ln < is ? pn : v;
         if (is == 0) return v;
         return HeapEntry{(uint32_t)__builtin_amdgcn_readlane((int)pn.count, 0),
                          (uint32_t)__builtin_amdgcn_readlane((int)pn.node, 0)};
