@@ -388,8 +388,13 @@ struct WaveEx {
         if (len <= 1) return v;
         const int n = len - 1, ln = lane();
         const int lim = (n - 1) / 2, lastint = (n - 2) / 2;
+        // a round ends the pop when its deepest node has no two children, unless that node is the
+        // last internal node of an even-sized heap (its lone left child then comes from one more
+        // round, whose walk goes left at once)
+        const int lim_end = (n & 1) ? lim : lim + 1;
         const int sd = ln < 63 ? 31 - __clz(ln + 1) : 0;  // depth of this lane's local node
         int mypath = 0, mynext = 0, base = 0, m;
+        HeapEntry pn;
         uint32_t c1 = 1;  // 1-based index of the round's root
         for (;;) {
             const int g = (int)((c1 - 1) << sd) + ln;
@@ -402,9 +407,12 @@ struct WaveEx {
             const int dk = ln - base;
             if (dk >= 1 && dk <= 6) mypath = (int)(e >> (6 - dk)) - 1;
             if (dk >= 0 && dk <= 5) mynext = (int)(e >> (5 - dk)) - 1;
-            // the path is real down to round depth d while its node at depth d - 1 has two
-            // children (index < lim); indices grow along the path, so the real depths are a prefix
-            if ((int)(e >> 1) - 1 >= lim) {
+            if ((int)e - 1 >= lim_end) {
+                // every lane reads its successor's entry (no entry changes before the path is
+                // complete), issued ahead of the path's length
+                pn = h[mynext < n ? mynext : 0];
+                // the path is real down to round depth d while its node at depth d - 1 has two
+                // children (index < lim); indices grow along the path, so these depths are a prefix
                 const int steps = (int)__popcll(__ballot(ln >= 1 && ln <= 6 && (int)(e >> ln) - 1 < lim));
                 m = base + steps;
                 if ((n & 1) == 0 && (int)(e >> (6 - steps)) - 1 == lastint) m++;
@@ -413,8 +421,7 @@ struct WaveEx {
             base += 6;
             c1 = e;
         }
-        // lanes 0..m-1 read their successor; v lands at i* = m - #{k in 1..m : v.count < count(p_k)}
-        const HeapEntry pn = h[ln < m ? mynext : 0];
+        // lanes 0..m-1 hold their successor; v lands at i* = m - #{k in 1..m : v.count < count(p_k)}
         const int is = m - (int)__popcll(__ballot(ln < m && v.count < pn.count));
         if (ln <= is) h[mypath] = ln < is ? pn : v;
         if (is == 0) return v;
@@ -427,7 +434,7 @@ struct WaveEx {
         const int T = 31 - __clz(pos + 1);
         const int c = ((pos + 1) >> ln) - 1;
         const bool anc = ln >= 1 && ln <= T;
-        const HeapEntry e = anc ? h[c] : HeapEntry{0u, 0u};
+        const HeapEntry e = h[anc ? c : 0];
         const int j = (int)__popcll(__ballot(anc && v.count < e.count));
         if (ln >= 1 && ln <= j) h[((pos + 1) >> (ln - 1)) - 1] = e;
         if (ln == j) h[c] = v;
