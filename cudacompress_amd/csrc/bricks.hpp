@@ -10,6 +10,8 @@
 // per-brick results are bit-identical.
 #pragma once
 
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace cca {
@@ -23,6 +25,7 @@ struct BrickLevel {
     void* lpsym;       // coarsest level: lowpass symbol stream (hx*hy*hz)
     float* out;        // inverse: this level's output, dense lx*ly*lz
     int lx, ly, lz;
+    int zk;            // z-slices per workgroup of the fused x+y passes
     int last;          // coarsest level
     float inv_hp, step_hp, inv_lp, step_lp;
 };
@@ -37,8 +40,21 @@ inline uint32_t brick_tiles(int ax, int lx, int ly, int lz) {
     return (uint32_t)(((cols + kBrickCols - 1) / kBrickCols) * ((n / 2 + kBrickSP - 1) / kBrickSP));
 }
 
+// Fused x+y passes: a workgroup of kBrickXYThreads holds zk whole z-slices of a brick in LDS
+// (kBrickXYFloats floats), each slice once as rows with their x extensions and once as columns
+// with their y extensions; a level uses them when every brick's slice fits.
+constexpr int kBrickXYThreads = 512;
+constexpr int kBrickXYFloats = 17536;  // one 128 x 128 slice: 128 lines of 128 + 9
+
+// LDS floats of one z-slice in the fused passes (rows of lx + 9, columns of ly + 9; odd strides)
+inline uint32_t brick_xy_floats(int lx, int ly) {
+    const uint64_t r = (uint64_t)ly * (lx + 9), c = (uint64_t)lx * (ly + 9);
+    return (uint32_t)std::min<uint64_t>(std::max(r, c), 0xFFFFFFFFull);
+}
+
 // One level of the batch, forward (X, Y, Z+quantize) or inverse (Z from symbols, Y, X).
-// tiles: [3][nbricks+1] prefix of brick_tiles for axes x, y, z.
+// tiles: [4][nbricks+1] prefix of brick_tiles for axes x, y, z and of the fused x+y
+// workgroups (ceil(lz / zk) per brick); h_tiles_total[3] == 0 selects the separate x and y passes.
 void bricks_forward_level(const BrickLevel* d_levels, const uint32_t* d_tiles, uint32_t nbricks,
                           const uint32_t* h_tiles_total, int symSize, uint64_t level_elems, hipStream_t st);
 void bricks_inverse_level(const BrickLevel* d_levels, const uint32_t* d_tiles, uint32_t nbricks,

@@ -336,6 +336,187 @@ __global__ __launch_bounds__(256) void k_binv(const BrickLevel* lv, const uint32
                  [&](int, int, int off, float v) { gst(dst, off, v); });
 }
 
+// ------------------------------------------------ fused x+y passes over whole z-slices in LDS
+// Workgroup w holds zk slices of one brick: R = zk ly rows (x lines) and C = zk lx columns (y
+// lines), each line with its extensions in LDS (entry 4 + p for window position p), strides lx + 9
+// and ly + 9 (odd: lanes on consecutive lines hit distinct banks).  The filters are Fwd97::pair and
+// Inv97::out over the same windows as the tiled passes, so the results are identical.
+constexpr int XYT = kBrickXYThreads;
+constexpr int XYB = kBrickXYFloats;
+constexpr int NIF = (XYB / 2 + XYT - 1) / XYT;  // forward pairs per thread (R mx < XYB / 2)
+constexpr int NII = (XYB / 2 + XYT - 1) / XYT;  // inverse output pairs per thread (C my < XYB / 2)
+
+// Exact quotient e / d for 0 <= e < 2^16 and 0 < d < 2^16 (one multiply-high; m wraps to 0 for
+// d = 1).
+struct Div {
+    uint32_t m;
+    __device__ explicit Div(int d) : m(0xFFFFFFFFu / (uint32_t)d + 1u) {}
+    __device__ int q(int e) const { return m ? (int)__umulhi((uint32_t)e, m) : e; }
+};
+
+// e = 0 .. E-1 of a dense run of floats, sixteen loads in flight per thread; put(e, v).
+template <class Put>
+__device__ __forceinline__ void xy_load(const float* src, int E, Put put) {
+    constexpr int U = 16;
+    for (int e0 = threadIdx.x; e0 < E; e0 += XYT * U) {
+        float v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int e = e0 + XYT * u;
+            v[u] = e < E ? gld(src, e) : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int e = e0 + XYT * u;
+            if (e < E) put(e, v[u]);
+        }
+    }
+}
+
+// Analysis extensions of `lines` lines of n samples: positions -4..-1 and n..n+3 mirror the line.
+__device__ __forceinline__ void xy_fill_mirror(float* buf, int lines, int stride, int n) {
+    for (int h = threadIdx.x; h < lines * 8; h += XYT) {
+        const int ln = h >> 3, j = h & 7;
+        const int p = j < 4 ? j - 4 : n + j - 4;
+        buf[ln * stride + 4 + p] = buf[ln * stride + 4 + mirror(p, n)];
+    }
+}
+
+// Synthesis extensions of interleaved lines (entry 4 + 2k + h: low (h = 0) or high coefficient k
+// of m): pairs -2, -1, m, m + 1 by ext_low / ext_high.
+__device__ __forceinline__ void xy_fill_inter(float* buf, int lines, int stride, int m) {
+    for (int h = threadIdx.x; h < lines * 8; h += XYT) {
+        const int ln = h >> 3, j = h & 7;
+        const int v = j < 4 ? j - 4 : 2 * m + j - 4;
+        const int k = v >> 1, hi = v & 1;
+        const int ks = hi ? ext_high(k, m) : ext_low(k, m);
+        buf[ln * stride + 4 + v] = buf[ln * stride + 4 + 2 * ks + hi];
+    }
+}
+
+// Forward x then y analysis of zk slices: L.in -> L.b.
+__global__ __launch_bounds__(XYT) void k_bfwd_xy(const BrickLevel* lv, const uint32_t* tb, uint32_t nb) {
+    extern __shared__ float buf[];
+    const uint32_t bi = find_brick(tb, nb, blockIdx.x);
+    const BrickLevel L = lv[bi];
+    const int nx = L.lx, ny = L.ly, mx = nx / 2, my = ny / 2;
+    const int z0 = (int)(blockIdx.x - tb[bi]) * L.zk;
+    const int k = min(L.zk, L.lz - z0);
+    const int RS = nx + 9, CS = ny + 9, R = k * ny, C = k * nx;
+    const float* src = L.in + (size_t)z0 * nx * ny;
+    float* dst = L.b + (size_t)z0 * nx * ny;
+    const int tid = threadIdx.x;
+    const Div dnx(nx), dny(ny), dR(R), dC(C);
+    xy_load(src, R * nx, [&](int e, float v) {
+        const int r = dnx.q(e);
+        buf[r * RS + 4 + (e - r * nx)] = v;
+    });
+    __syncthreads();
+    xy_fill_mirror(buf, R, RS, nx);
+    __syncthreads();
+    // x analysis: pair q of row r, consecutive lanes on consecutive rows
+    float lo[NIF], hi[NIF];
+#pragma unroll
+    for (int j = 0; j < NIF; j++) {
+        const int it = tid + XYT * j;
+        lo[j] = hi[j] = 0.0f;
+        if (it < R * mx) {
+            const int q = dR.q(it), r = it - q * R;
+            Fwd97::pair(&buf[r * RS], q, lo[j], hi[j]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NIF; j++) {
+        const int it = tid + XYT * j;
+        if (it < R * mx) {
+            const int q = dR.q(it), r = it - q * R;
+            const int s = dny.q(r), y = r - s * ny, c = s * nx + q;
+            buf[c * CS + 4 + y] = lo[j];
+            buf[(c + mx) * CS + 4 + y] = hi[j];
+        }
+    }
+    __syncthreads();
+    xy_fill_mirror(buf, C, CS, ny);
+    __syncthreads();
+    // y analysis: pair q of column c, consecutive lanes on consecutive x (coalesced stores)
+    for (int it = tid; it < C * my; it += XYT) {
+        const int q = dC.q(it), c = it - q * C;
+        const int s = dnx.q(c), x = c - s * nx;
+        float l, h;
+        Fwd97::pair(&buf[c * CS], q, l, h);
+        gst(dst, (s * ny + q) * nx + x, l);
+        gst(dst, (s * ny + my + q) * nx + x, h);
+    }
+}
+
+// Inverse y then x synthesis of zk slices: L.a -> L.out.
+__global__ __launch_bounds__(XYT) void k_binv_yx(const BrickLevel* lv, const uint32_t* tb, uint32_t nb) {
+    extern __shared__ float buf[];
+    const uint32_t bi = find_brick(tb, nb, blockIdx.x);
+    const BrickLevel L = lv[bi];
+    const int nx = L.lx, ny = L.ly, mx = nx / 2, my = ny / 2;
+    const int z0 = (int)(blockIdx.x - tb[bi]) * L.zk;
+    const int k = min(L.zk, L.lz - z0);
+    const int RS = nx + 9, CS = ny + 9, R = k * ny, C = k * nx;
+    const float* src = L.a + (size_t)z0 * nx * ny;
+    float* dst = L.out + (size_t)z0 * nx * ny;
+    const int tid = threadIdx.x;
+    const Div dnx(nx), dny(ny), dC(C);
+    // columns, interleaved: row y of a slice holds low (y < my) or high coefficient y - my
+    xy_load(src, R * nx, [&](int e, float v) {
+        const int r = dnx.q(e), x = e - r * nx;
+        const int s = dny.q(r), y = r - s * ny;
+        const int w = y < my ? 2 * y : 2 * (y - my) + 1;
+        buf[(s * nx + x) * CS + 4 + w] = v;
+    });
+    __syncthreads();
+    xy_fill_inter(buf, C, CS, my);
+    __syncthreads();
+    // y synthesis: outputs 2 u and 2 u + 1 of column c from one 10-entry window, consecutive lanes
+    // on consecutive columns
+    float o0[NII], o1[NII];
+#pragma unroll
+    for (int j = 0; j < NII; j++) {
+        const int it = tid + XYT * j;
+        o0[j] = o1[j] = 0.0f;
+        if (it < C * my) {
+            const int u = dC.q(it), c = it - u * C;
+            float w[10];
+#pragma unroll
+            for (int i = 0; i < 10; i++) w[i] = buf[c * CS + 2 * u + i];
+            o0[j] = Inv97::out(w, 0);
+            o1[j] = Inv97::out(w + 1, 1);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NII; j++) {
+        const int it = tid + XYT * j;
+        if (it < C * my) {
+            const int u = dC.q(it), c = it - u * C;
+            const int s = dnx.q(c), x = c - s * nx;
+            const int w = x < mx ? 2 * x : 2 * (x - mx) + 1;
+            const int r = s * ny + 2 * u;
+            buf[r * RS + 4 + w] = o0[j];
+            buf[(r + 1) * RS + 4 + w] = o1[j];
+        }
+    }
+    __syncthreads();
+    xy_fill_inter(buf, R, RS, mx);
+    __syncthreads();
+    // x synthesis: outputs 2 u and 2 u + 1 of row r, plane elements 2 it and 2 it + 1
+    const Div dmx(mx);
+    for (int it = tid; it < R * mx; it += XYT) {
+        const int r = dmx.q(it), u = it - r * mx;
+        float w[10];
+#pragma unroll
+        for (int i = 0; i < 10; i++) w[i] = buf[r * RS + 2 * u + i];
+        gst(dst, 2 * it, Inv97::out(w, 0));
+        gst(dst, 2 * it + 1, Inv97::out(w + 1, 1));
+    }
+}
+
 }  // namespace
 
 void bricks_forward_level(const BrickLevel* d_levels, const uint32_t* d_tiles, uint32_t nbricks,
@@ -343,9 +524,15 @@ void bricks_forward_level(const BrickLevel* d_levels, const uint32_t* d_tiles, u
     const uint32_t* tx = d_tiles;
     const uint32_t* ty = d_tiles + (nbricks + 1);
     const uint32_t* tz = d_tiles + 2 * (nbricks + 1);
-    if (h_tiles_total[0]) CCA_LAUNCH("bricks_fwd_x", level_elems * 8, k_bfwd<0>, dim3(h_tiles_total[0]), dim3(256), 0, st, d_levels, tx, nbricks);
-    CCA_LAUNCH_CHECK();
-    if (h_tiles_total[1]) CCA_LAUNCH("bricks_fwd_y", level_elems * 8, k_bfwd<1>, dim3(h_tiles_total[1]), dim3(256), 0, st, d_levels, ty, nbricks);
+    const uint32_t* txy = d_tiles + 3 * (nbricks + 1);
+    if (h_tiles_total[3]) {
+        CCA_LAUNCH("bricks_fwd_xy", level_elems * 8, k_bfwd_xy, dim3(h_tiles_total[3]), dim3(XYT), XYB * sizeof(float),
+                   st, d_levels, txy, nbricks);
+    } else {
+        if (h_tiles_total[0]) CCA_LAUNCH("bricks_fwd_x", level_elems * 8, k_bfwd<0>, dim3(h_tiles_total[0]), dim3(256), 0, st, d_levels, tx, nbricks);
+        CCA_LAUNCH_CHECK();
+        if (h_tiles_total[1]) CCA_LAUNCH("bricks_fwd_y", level_elems * 8, k_bfwd<1>, dim3(h_tiles_total[1]), dim3(256), 0, st, d_levels, ty, nbricks);
+    }
     CCA_LAUNCH_CHECK();
     const uint64_t zb = level_elems * 4 + level_elems / 8 * 4 + level_elems * 7 / 8 * symSize;
     if (h_tiles_total[2]) {
@@ -370,9 +557,14 @@ void bricks_inverse_level(const BrickLevel* d_levels, const uint32_t* d_tiles, u
             CCA_LAUNCH("bricks_inv_z_dequant", zb, k_binv_zs<uint32_t>, dim3(h_tiles_total[2]), dim3(256), 0, st, d_levels, tz, nbricks);
     }
     CCA_LAUNCH_CHECK();
-    if (h_tiles_total[1]) CCA_LAUNCH("bricks_inv_y", level_elems * 8, k_binv<1>, dim3(h_tiles_total[1]), dim3(256), 0, st, d_levels, ty, nbricks);
-    CCA_LAUNCH_CHECK();
-    if (h_tiles_total[0]) CCA_LAUNCH("bricks_inv_x", level_elems * 8, k_binv<0>, dim3(h_tiles_total[0]), dim3(256), 0, st, d_levels, tx, nbricks);
+    if (h_tiles_total[3]) {
+        CCA_LAUNCH("bricks_inv_yx", level_elems * 8, k_binv_yx, dim3(h_tiles_total[3]), dim3(XYT), XYB * sizeof(float),
+                   st, d_levels, d_tiles + 3 * (nbricks + 1), nbricks);
+    } else {
+        if (h_tiles_total[1]) CCA_LAUNCH("bricks_inv_y", level_elems * 8, k_binv<1>, dim3(h_tiles_total[1]), dim3(256), 0, st, d_levels, ty, nbricks);
+        CCA_LAUNCH_CHECK();
+        if (h_tiles_total[0]) CCA_LAUNCH("bricks_inv_x", level_elems * 8, k_binv<0>, dim3(h_tiles_total[0]), dim3(256), 0, st, d_levels, tx, nbricks);
+    }
     CCA_LAUNCH_CHECK();
 }
 

@@ -96,13 +96,18 @@ void upload_levels(BrickCtx& ctx, BrickJob& job, float* work, unsigned char* sym
     const uint32_t n = (uint32_t)job.bricks.size(), L = job.L;
     const size_t lv_bytes = sizeof(BrickLevel) * (size_t)L * n;
     const size_t tb_off = align_up(std::max<size_t>(lv_bytes, 1), 256);
-    const size_t bytes = tb_off + 4ull * L * 3 * (n + 1);
+    const size_t bytes = tb_off + 4ull * L * 4 * (n + 1);
     unsigned char* h = (unsigned char*)ctx.h_blevels.get(bytes);
     BrickLevel* hl = (BrickLevel*)h;
     uint32_t* ht = (uint32_t*)(h + tb_off);
-    job.tiles_total.assign(3 * L, 0);
+    job.tiles_total.assign(4 * L, 0);
     for (uint32_t l = 0; l < L; l++) {
-        uint32_t acc[3] = {0, 0, 0};
+        // the fused x+y passes serve a level when every brick's z-slice fits a workgroup's LDS
+        bool fused = true;
+        for (uint32_t i = 0; i < n; i++)
+            fused = fused && brick_xy_floats((int)(job.bricks[i].sizeX >> l), (int)(job.bricks[i].sizeY >> l)) <=
+                                 (uint32_t)kBrickXYFloats;
+        uint32_t acc[4] = {0, 0, 0, 0};
         for (uint32_t i = 0; i < n; i++) {
             const cc_brick& b = job.bricks[i];
             const uint64_t N = elems(b);
@@ -124,14 +129,17 @@ void upload_levels(BrickCtx& ctx, BrickJob& job, float* work, unsigned char* sym
             e.inv_hp = 1.0f / e.step_hp;      // Quantize.cu:94
             e.step_lp = q0 / float(1u << (L - 1));
             e.inv_lp = 1.0f / e.step_lp;
-            for (int ax = 0; ax < 3; ax++) {
-                ht[((size_t)l * 3 + ax) * (n + 1) + i] = acc[ax];
-                acc[ax] += brick_tiles(ax, e.lx, e.ly, e.lz);
+            e.zk = fused ? (int)std::min<uint32_t>((uint32_t)e.lz,
+                                                   std::max<uint32_t>(1u, kBrickXYFloats / brick_xy_floats(e.lx, e.ly)))
+                         : 0;
+            for (int ax = 0; ax < 4; ax++) {
+                ht[((size_t)l * 4 + ax) * (n + 1) + i] = acc[ax];
+                acc[ax] += ax < 3 ? brick_tiles(ax, e.lx, e.ly, e.lz) : (fused ? div_up(e.lz, e.zk) : 0u);
             }
         }
-        for (int ax = 0; ax < 3; ax++) {
-            ht[((size_t)l * 3 + ax) * (n + 1) + n] = acc[ax];
-            job.tiles_total[3 * l + ax] = acc[ax];
+        for (int ax = 0; ax < 4; ax++) {
+            ht[((size_t)l * 4 + ax) * (n + 1) + n] = acc[ax];
+            job.tiles_total[4 * l + ax] = acc[ax];
         }
     }
     unsigned char* d = (unsigned char*)ctx.d_blevels.get(bytes);
@@ -177,8 +185,8 @@ void bricks_prepare(cc_instance* I, BrickCtx& ctx, BrickJob& job, bool enc, bool
 void bricks_encode_launch(BrickCtx& ctx, const BrickJob& job, hipStream_t st, const LayoutLink* link) {
     const uint32_t n = (uint32_t)job.bricks.size();
     for (uint32_t l = 0; l < job.L; l++)
-        bricks_forward_level(job.d_levels + (size_t)l * n, job.d_tiles + (size_t)l * 3 * (n + 1), n,
-                             &job.tiles_total[3 * l], job.symSize, job.level_elems[l], st);
+        bricks_forward_level(job.d_levels + (size_t)l * n, job.d_tiles + (size_t)l * 4 * (n + 1), n,
+                             &job.tiles_total[4 * l], job.symSize, job.level_elems[l], st);
     encode_launch(ctx.coder, job.ej, job.d_out, job.out_cap, st, link);
 }
 
@@ -188,8 +196,8 @@ void bricks_decode_launch(BrickCtx& ctx, const BrickJob& job, hipStream_t st, bo
         decode_calls_from_encode(ctx.coder, job.dj, (const CallOut*)ctx.coder.d_calls.p, job.d_out, st);
     decode_launch(ctx.coder, job.dj, st);
     for (int l = (int)job.L - 1; l >= 0; l--)
-        bricks_inverse_level(job.d_levels + (size_t)l * n, job.d_tiles + (size_t)l * 3 * (n + 1), n,
-                             &job.tiles_total[3 * l], job.symSize, job.level_elems[l], st);
+        bricks_inverse_level(job.d_levels + (size_t)l * n, job.d_tiles + (size_t)l * 4 * (n + 1), n,
+                             &job.tiles_total[4 * l], job.symSize, job.level_elems[l], st);
 }
 
 bool bricks_encode_results(const BrickJob& job, const std::vector<CallOut>& res, uint64_t total, uint64_t* offsets,

@@ -22,8 +22,8 @@ struct BrickJob {
     uint64_t work_floats = 0, sym_elems = 0;
     // device tables
     const BrickLevel* d_levels = nullptr;  // [L][n]
-    const uint32_t* d_tiles = nullptr;     // [L][3][n+1]
-    std::vector<uint32_t> tiles_total;     // [L][3]
+    const uint32_t* d_tiles = nullptr;     // [L][4][n+1]
+    std::vector<uint32_t> tiles_total;     // [L][4]
     EncodeJob ej;
     DecodeJob dj;
     uint32_t* d_out = nullptr;

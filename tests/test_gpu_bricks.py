@@ -69,6 +69,37 @@ def test_bricks_equal_single_volume_calls(cc, torch_gpu, field, levels):
     inst.close()
 
 
+def test_bricks_wide_slices_take_separate_passes(cc, torch_gpu):
+    """A level whose bricks include a z-slice too large for the fused x+y passes' LDS (96 x 192)
+    runs the separate x and y passes; the next level (48 x 96) and a batch whose largest slice is
+    exactly the fused budget (128 x 128) run fused.  Every brick equals its single-volume call."""
+    from cudacompress_amd import synthetic as S
+    log("bricks wide slices")
+    torch = torch_gpu
+    field = S.gaussian_field((16, 192, 192), 11)
+    std = float(field.std())
+    cases = [[(8, 96, 192), (16, 32, 48), (8, 16, 16)], [(8, 128, 128), (16, 48, 96), (8, 16, 32)]]
+    inst = cc.createInstance(0, 7, 64 * 64 * 64, 128, 14)
+    for shapes in cases:
+        bricks = [np.ascontiguousarray(field[:z, :y, :x]) for z, y, x in shapes]
+        steps = [0.01 * std, 0.05 * std, 0.2 * std]
+        srcs, out, offs, stat = run_batch(cc, torch, inst, bricks, steps, 2)
+        assert np.all(stat == 0), stat
+        words = out.cpu().numpy().view(np.uint32)
+        dsts = [torch.zeros_like(s) for s in srcs]
+        cc.decompressBricksFloat(inst, dsts, [b.shape for b in bricks], steps, 2, out, offs)
+        torch.cuda.synchronize()
+        for i, b in enumerate(bricks):
+            z, y, x = b.shape
+            want = cc.compressVolumeFloat(inst, srcs[i], x, y, z, 2, steps[i])
+            assert np.array_equal(words[offs[i]:offs[i + 1]], want), f"brick {b.shape}"
+            ref = torch.zeros_like(srcs[i])
+            cc.decompressVolumeFloat(inst, ref, x, y, z, 2, want, steps[i])
+            torch.cuda.synchronize()
+            assert np.array_equal(dsts[i].cpu().numpy(), ref.cpu().numpy()), f"brick {b.shape} reconstruction"
+    inst.close()
+
+
 def test_bricks_symbol32_and_overflow(cc, torch_gpu, field):
     """Symbol32 instance (log2 18) and, on a log2 14 instance, one brick whose symbols overflow:
     only that brick fails."""
