@@ -19,6 +19,7 @@
 namespace cca {
 
 constexpr uint32_t kStatBins = 4096;    // LDS histogram bins of the streaming passes
+constexpr uint32_t kDecLutBits = 10;    // lookup-table bits of the block decoder (codes of up to that length)
 constexpr uint32_t kCoderThreads = 256;
 constexpr uint32_t kPerThread = 8;
 constexpr uint32_t kChunk = kCoderThreads * kPerThread;  // symbols per workgroup step
@@ -112,7 +113,7 @@ struct DecSeq {
     uint64_t words_off;     // codewords (word offset from base)
     uint64_t incs_off;      // u16 increments (word offset from base)
     uint64_t boff_off;      // arena: absolute block offsets
-    uint64_t lut_off;       // arena: 1024-entry lookup table (len << 24 | symbol), then the symbol table
+    uint64_t lut_off;       // arena: 2^kDecLutBits-entry lookup table (len << 24 | symbol), then the symbol table
     void* out;              // decoded values (arena for RLE sequences, the item output otherwise)
     uint32_t nlen, nsym, sb, count;
     uint32_t nwords_cw;     // codeword words

@@ -221,16 +221,27 @@ __device__ void inv_tile(const Geo<AX>& g, uint32_t local, float (*lds)[LW], Loa
     else inv_window<AX, false, BATCH>(g, c0, k0, lds, load, conv);
     __syncthreads();
     const int col = tid & 63;
-    float o[2 * SP / 4];
+    // outputs 2 u and 2 u + 1 from one 10-entry window (n is even: both exist or neither)
+    float o0[SP / 4], o1[SP / 4];
 #pragma unroll
-    for (int j = 0; j < 2 * SP / 4; j++) {
-        const int t = (tid >> 6) + 4 * j;
-        o[j] = 0.0f;
-        if (2 * k0 + t < g.n && c0 + col < g.ncols) o[j] = Inv97::out(&lds[col][t], t & 1);
+    for (int j = 0; j < SP / 4; j++) {
+        const int u = (tid >> 6) + 4 * j;
+        o0[j] = o1[j] = 0.0f;
+        if (2 * (k0 + u) < g.n && c0 + col < g.ncols) {
+            float w[10];
+#pragma unroll
+            for (int i = 0; i < 10; i++) w[i] = lds[col][2 * u + i];
+            o0[j] = Inv97::out(w, 0);
+            o1[j] = Inv97::out(w + 1, 1);
+        }
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < 2 * SP / 4; j++) lds[col][(tid >> 6) + 4 * j] = o[j];
+    for (int j = 0; j < SP / 4; j++) {
+        const int u = (tid >> 6) + 4 * j;
+        lds[col][2 * u] = o0[j];
+        lds[col][2 * u + 1] = o1[j];
+    }
     __syncthreads();
     if (AX == 0) {
         for (int i = tid; i < NC * 2 * SP; i += 256) {

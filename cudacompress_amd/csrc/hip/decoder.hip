@@ -15,7 +15,7 @@
 namespace cca {
 namespace {
 
-constexpr uint32_t kLut = 10;  // lookup-table bits of the block decoder
+constexpr uint32_t kLut = kDecLutBits;
 
 // n <= 32 bits at bit position pos of w[0, nwords); ok cleared when out of range
 __device__ __forceinline__ uint32_t rd_bits(const uint32_t* w, uint64_t nwords, uint64_t pos, uint32_t n, bool& ok) {
@@ -269,59 +269,48 @@ __global__ __launch_bounds__(64) void k_dec_blocks(const DecSeq* seqs, const uin
     const uint32_t single = nsym >= 1 ? symbol_at(0) : 0u;
     const uint32_t b = fb + lane;
     const bool valid = b < nblk;
-    const uint32_t cnt = valid ? min(cbs, s.count - b * cbs) : 0u;
     const uint32_t* words = s.base + s.words_off;
     const uint64_t avail = s.nwords - s.words_off;  // readable words from the codeword start
     auto word = [&](uint64_t i) -> uint32_t { return i < avail ? words[i] : 0u; };
-    // bit window: `cache` holds 64 bits starting at the current word, pf0..pf3 the four words
-    // after them, each loaded four refills before it is used so that its latency overlaps the
-    // decoding of the bits in between
+    // bit window: `cache` holds 64 bits starting at word `nxt - 2`; a refill shifts in the next word
+    // (an L1 hit after the block's first lines).  A separate prefetch ring does not help here:
+    // the load counter is per wave and the lanes refill at different steps, so any later use of a
+    // prefetched word waits for the wave's latest load.
     uint64_t cache = 0;
-    uint32_t bp = 0, pf0 = 0, pf1 = 0, pf2 = 0, pf3 = 0;
+    uint32_t bp = 0;
     uint64_t nxt = 0;
     if (valid && multi) {
         const uint32_t pos = b == 0 ? 0u : scratch[s.boff_off + b];
         const uint64_t wi = pos >> 5;
         cache = ((uint64_t)word(wi) << 32) | (uint64_t)word(wi + 1);
-        pf0 = word(wi + 2);
-        pf1 = word(wi + 3);
-        pf2 = word(wi + 4);
-        pf3 = word(wi + 5);
         bp = pos & 31u;
-        nxt = wi + 6;
+        nxt = wi + 2;
     }
     const uint32_t mlen = s.nlen;
     for (uint32_t r = 0; r < cbs; r += 32) {
-        const uint32_t rend = min(r + 32, cnt);
-        for (uint32_t i = r; i < rend; i++) {
-            uint32_t sym;
+        // every lane steps through the whole chunk; steps past its block's end store nothing
+#pragma unroll 2
+        for (uint32_t i = 0; i < 32; i++) {
+            uint32_t sym = single;
             if (multi) {
                 if (bp >= 32) {
-                    cache = (cache << 32) | (uint64_t)pf0;
-                    pf0 = pf1;
-                    pf1 = pf2;
-                    pf2 = pf3;
-                    pf3 = word(nxt);
+                    cache = (cache << 32) | (uint64_t)word(nxt);
                     nxt++;
                     bp -= 32;
                 }
                 const uint32_t peek = (uint32_t)(cache >> (32 - bp));
                 const uint32_t e = lut[peek >> (32 - kLut)];
-                uint32_t len;
-                if (e) {
-                    len = e >> 24;
-                    sym = e & 0xFFFFFFu;
-                } else {
+                uint32_t len = e >> 24;
+                sym = e & 0xFFFFFFu;
+                if (e == 0u) {
                     len = kLut + 1;
                     while (len < mlen && (int)(peek >> (32 - len)) > tab[64 + len - 1]) len++;
                     const int cw = (int)(peek >> (32 - len));
                     sym = symbol_at(tab[len - 1] + cw - tab[32 + len - 1]);
                 }
                 bp += len;
-            } else {
-                sym = single;
             }
-            stage[lane][i - r] = (S)sym;
+            stage[lane][i] = (S)sym;
         }
         __syncthreads();
         for (uint32_t idx = lane; idx < 64 * 32; idx += 64) {

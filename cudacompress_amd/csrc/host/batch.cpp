@@ -176,7 +176,7 @@ void decode_prepare(cc_instance* I, DeviceCoder& D, const std::vector<ItemSpec>&
         nwaves += q * div_up(nblk, 64);
         total_syms += it.n;
         // block offsets, lookup + symbol table (nsym <= count <= n), RLE decode buffers
-        arena += q * ((uint64_t)nblk + 16 + 1024 + it.n + 17 + (it.rl ? ((uint64_t)it.n * symSize + 3) / 4 + 17 : 0));
+        arena += q * ((uint64_t)nblk + 16 + (1u << kDecLutBits) + it.n + 17 + (it.rl ? ((uint64_t)it.n * symSize + 3) / 4 + 17 : 0));
     }
     job.o_items = align_up(sizeof(DecCall) * std::max<uint32_t>(ncalls, 1), 256);
     job.o_wseq = job.o_items + align_up(sizeof(DecItem) * std::max<uint32_t>(nitems, 1), 256);
