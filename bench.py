@@ -56,7 +56,8 @@ def parse():
     ap.add_argument("--bricks", type=int, default=2048)
     ap.add_argument("--groups", type=int, default=0, help="brick groups of eager batches (0: library default)")
     ap.add_argument("--mode", default="plan", choices=["plan", "eager"],
-                    help="plan: compression and decompression replayed from hipGraphs (BricksPlan); "
+                    help="plan: compression and decompression replayed from hipGraphs with one host "
+                         "synchronisation per step (BricksPlan.run); "
                          "eager: compressBricksFloat + decompressBricksFloat calls")
     ap.add_argument("--plain-sample", type=int, default=256, help="bricks timed through the plain per-brick API")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -137,8 +138,7 @@ def run_bricks(args, torch, cc, lib):
 
     def step():
         if plan is not None:
-            offs, stat = plan.compress()
-            dstat = plan.decompress()
+            offs, stat, dstat = plan.run()
         else:
             offs, stat = cc.compressBricksFloat(inst, srcs, shapes, steps_q, 2, out)
             dstat = cc.decompressBricksFloat(inst, dsts, shapes, steps_q, 2, out, offs)

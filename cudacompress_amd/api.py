@@ -419,7 +419,8 @@ def decompressBricksFloat(inst, dests, shapes, quantSteps, numLevels, words, off
 
 class BricksPlan:
     """A brick batch captured into hipGraphs (cc_bricks_plan_*): compress() and decompress()
-    replay the captured pipelines on the same bricks and output buffer."""
+    replay the captured pipelines on the same bricks and output buffer; run() replays both with
+    one host synchronisation (cc_bricks_plan_run)."""
 
     def __init__(self, inst, sources, dests, shapes, quantSteps, numLevels, out, symbolSize=2):
         self._lib = _capi.load()
@@ -446,6 +447,17 @@ class BricksPlan:
         _check(self._lib.cc_bricks_plan_decompress(self.handle, stat.ctypes.data_as(_capi.P_u32)),
                "cc_bricks_plan_decompress")
         return stat[:self._n]
+
+    def run(self):
+        """Compression then decompression; returns (offsets, compress status, decompress status)."""
+        offs = np.zeros(self._n + 1, dtype=np.uint64)
+        cst = np.zeros(max(self._n, 1), dtype=np.uint32)
+        dst = np.zeros(max(self._n, 1), dtype=np.uint32)
+        st = self._lib.cc_bricks_plan_run(self.handle, offs.ctypes.data_as(_capi.P_u64), cst.ctypes.data_as(_capi.P_u32),
+                                          dst.ctypes.data_as(_capi.P_u32))
+        if st not in (CC_OK, CC_ERR_SYMBOL_OVERFLOW, CC_ERR_CODEWORD_TOO_LONG):
+            _check(st, "cc_bricks_plan_run")
+        return offs, cst[:self._n], dst[:self._n]
 
     def close(self):
         if self.handle:

@@ -134,8 +134,13 @@ struct DecCallOut {
 // kSideStreams non-blocking streams, `fork` / `join` events owned by the caller; a null `side`
 // argument puts everything on st.
 constexpr int kSideStreams = 4;
-// Table-design work lists (device buffer of kDesignListBase + kDesignLists * nseq words).
+// Table-design work lists (device buffer of kDesignListBase + kDesignLists * nseq + 2 * nitems
+// words): counters of the design classes at [0, kDesignLists), of the two pack item lists at
+// kPackCountSmall / kPackCountBig; the class lists from kDesignListBase, then the item lists
+// (items whose tables all belong to the two smallest classes, then the others).
 constexpr uint32_t kDesignLists = 5;
+constexpr uint32_t kPackCountSmall = 5;
+constexpr uint32_t kPackCountBig = 6;
 constexpr uint32_t kDesignListBase = 8;
 struct SideStreams {
     hipStream_t s[kSideStreams];
@@ -150,7 +155,7 @@ struct LayoutLink {
     hipEvent_t wait;
     hipEvent_t signal;
 };
-// d_classes: kDesignListBase + kDesignLists * nseq words of table-design work lists.
+// d_classes: kDesignListBase + kDesignLists * nseq + 2 * nitems words of work lists.
 void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uint32_t nseq, const CoderParams& p,
                   int symSize, uint32_t* scratch, uint64_t scratch_cap, CoderState* d_state, uint32_t* d_classes,
                   CallOut* d_calls,
@@ -163,8 +168,9 @@ void coder_decode(const DecCall* d_calls, uint32_t ncalls, const DecItem* d_item
                   uint32_t nseq, const CoderParams& p, int symSize, uint32_t* scratch, uint64_t scratch_cap,
                   CoderState* d_state, DecCallOut* d_out, const uint32_t* d_wave_seq, const uint32_t* d_wave_first,
                   uint32_t nwaves, uint64_t total_symbols, hipStream_t st);
-// calls[i].words = base + enc[i].word_begin, calls[i].nwords = enc[i].words (the rest unchanged)
-void coder_calls_from_encode(DecCall* d_calls, const CallOut* d_enc, const uint32_t* base, uint32_t ncalls,
-                             hipStream_t st);
+// calls[i].words = base + enc[i].word_begin, calls[i].nwords = enc[i].words (the rest unchanged);
+// an empty range for a failed call or one reaching past cap words
+void coder_calls_from_encode(DecCall* d_calls, const CallOut* d_enc, const uint32_t* base, uint64_t cap,
+                             uint32_t ncalls, hipStream_t st);
 
 }  // namespace cca

@@ -157,7 +157,7 @@ __device__ __forceinline__ long long thread_last_nz(const uint32_t (&v)[kPerThre
 
 // ------------------------------------------------------------------------ kernels
 __global__ void k_coder_init(CoderState* st, uint32_t* classes) {
-    if (threadIdx.x < kDesignLists) classes[threadIdx.x] = 0;
+    if (threadIdx.x < kDesignListBase) classes[threadIdx.x] = 0;
     if (threadIdx.x == 0) {
         st->scratch_used = 0;
         st->out_words = 0;
@@ -531,6 +531,31 @@ __global__ __launch_bounds__(256) void k_design_classify(const SeqInfo* seqs, ui
     }
 }
 
+// One thread per item: items whose tables all belong to the two smallest design classes go to
+// the first pack list (packed as soon as those classes are designed), the others to the second.
+__global__ __launch_bounds__(256) void k_item_classify(const CodeItem* items, uint32_t nitems, const SeqInfo* seqs,
+                                                       uint32_t nseq, uint32_t* classes) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    uint32_t c = 0xFFFFFFFFu;
+    if (i < nitems) {
+        const CodeItem it = items[i];
+        const bool small = design_class(seqs[it.seq].nsym) <= 1u && (!it.rl || design_class(seqs[it.seq + 1].nsym) <= 1u);
+        c = small ? 0u : 1u;
+    }
+    const int lane = threadIdx.x & 63;
+    uint32_t* lists = classes + kDesignListBase + (size_t)kDesignLists * nseq;
+#pragma unroll
+    for (uint32_t k = 0; k < 2; k++) {
+        const uint64_t m = __ballot(c == k);
+        if (!m) continue;
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&classes[k == 0 ? kPackCountSmall : kPackCountBig], (uint32_t)__popcll(m));
+        base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+        if (c == k) lists[(size_t)k * nitems + base + __popcll(m & lanes_below(lane))] = i;
+    }
+}
+
 // Designs the table of one sequence with one wave (all lanes call; returns uniformly).  Gws: the
 // design workspace comes from the arena instead of LDS (a compile-time choice, so that the LDS
 // classes address their workspace with LDS instructions rather than generic ones).
@@ -616,7 +641,8 @@ __global__ __launch_bounds__(64) void k_design(SeqInfo* seqs, CoderParams p, uin
 // the absolute bit offset of every coding block is recorded.
 template <class S>
 __global__ __launch_bounds__(kCoderThreads) void k_code_pack(const CodeItem* items, SeqInfo* seqs, CoderParams p,
-                                                             uint32_t* scratch) {
+                                                             uint32_t* scratch, const uint32_t* list,
+                                                             const uint32_t* count) {
     __shared__ uint32_t cs[kStatBins];
     __shared__ unsigned char lsh[kStatBins];
     __shared__ uint32_t cz[256];
@@ -624,7 +650,8 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_pack(const CodeItem* ite
     __shared__ uint32_t win[2][kWindowWords];
     __shared__ long long shl[4];
     __shared__ uint64_t shq[4];
-    const CodeItem it = items[blockIdx.x];
+    if (blockIdx.x >= *count) return;
+    const CodeItem it = items[list[blockIdx.x]];
     const SeqInfo ss = seqs[it.seq];
     const bool rl = it.rl != 0;
     if (ss.err || (rl && seqs[it.seq + 1].err)) return;
@@ -961,6 +988,9 @@ void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uin
     CCA_LAUNCH("design_classify", 0, k_design_classify, dim3(div_up(nseq, 256u)), dim3(256), 0, st, d_seqs, nseq,
                d_classes);
     CCA_LAUNCH_CHECK();
+    CCA_LAUNCH("item_classify", 0, k_item_classify, dim3(div_up(nitems, 256u)), dim3(256), 0, st, d_items, nitems,
+               d_seqs, nseq, d_classes);
+    CCA_LAUNCH_CHECK();
     constexpr int kClasses = 4;
     static_assert(kClasses <= kSideStreams, "one side stream per design class");
     const uint32_t kmax[kClasses] = {256, 1024, 2048, kDesignMid};
@@ -982,18 +1012,30 @@ void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uin
     CCA_LAUNCH("huff_design_global", 0, (k_design<uint32_t, true>), dim3(std::min(nseq, 256u)), dim3(64), 0, st,
                d_seqs, p, scratch, scratch_cap, d_state, d_classes, nseq, 4u);
     CCA_LAUNCH_CHECK();
+    // items whose tables are all small are packed while the large tables are still being designed:
+    // on the first side stream once the two small classes are done, the others after the join
+    const uint32_t* lists = d_classes + kDesignListBase + (size_t)kDesignLists * nseq;
+    auto pack = [&](hipStream_t sp, const uint32_t* list, const uint32_t* count, uint64_t bytes) {
+        if (symSize == 2)
+            CCA_LAUNCH("code_pack", bytes, k_code_pack<uint16_t>, dim3(nitems), dim3(kCoderThreads), 0, sp, d_items,
+                       d_seqs, p, scratch, list, count);
+        else
+            CCA_LAUNCH("code_pack", bytes, k_code_pack<uint32_t>, dim3(nitems), dim3(kCoderThreads), 0, sp, d_items,
+                       d_seqs, p, scratch, list, count);
+        CCA_LAUNCH_CHECK();
+    };
+    if (side) {
+        CCA_HIP(hipEventRecord(side->join[1], side->s[1]));
+        CCA_HIP(hipStreamWaitEvent(side->s[0], side->join[1], 0));
+    }
+    pack(side ? side->s[0] : st, lists, d_classes + kPackCountSmall, inb);
     if (side) {
         for (int c = 0; c < kClasses; c++) {
             CCA_HIP(hipEventRecord(side->join[c], side->s[c]));
             CCA_HIP(hipStreamWaitEvent(st, side->join[c], 0));
         }
     }
-    if (symSize == 2)
-        CCA_LAUNCH("code_pack", inb, k_code_pack<uint16_t>, dim3(nitems), dim3(kCoderThreads), 0, st, d_items, d_seqs,
-                   p, scratch);
-    else
-        CCA_LAUNCH("code_pack", inb, k_code_pack<uint32_t>, dim3(nitems), dim3(kCoderThreads), 0, st, d_items, d_seqs,
-                   p, scratch);
+    pack(st, lists + nitems, d_classes + kPackCountBig, 0);
     CCA_LAUNCH_CHECK();
     if (link && link->wait) CCA_HIP(hipStreamWaitEvent(st, link->wait, 0));
     CCA_LAUNCH("code_layout", 0, k_layout, dim3(1), dim3(1024), 0, st, d_items, nitems, d_seqs, p, d_state, d_calls,

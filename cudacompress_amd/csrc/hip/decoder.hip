@@ -420,20 +420,25 @@ __global__ void k_dec_finish(const DecCall* calls, uint32_t ncalls, const DecIte
     }
 }
 
-__global__ void k_calls_from_encode(DecCall* calls, const CallOut* enc, const uint32_t* base, uint32_t n) {
+// Calls of a decode from an encode's device-side results; a failed call, or a range outside the
+// output's cap words, decodes from an empty range (its items then report a corrupt stream).
+__global__ void k_calls_from_encode(DecCall* calls, const CallOut* enc, const uint32_t* base, uint32_t n,
+                                    uint64_t cap) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
-    calls[c].words = base + enc[c].word_begin;
-    calls[c].nwords = enc[c].err ? 0 : enc[c].words;
+    const CallOut e = enc[c];
+    const bool ok = !e.err && e.word_begin <= cap && e.words <= cap - e.word_begin;
+    calls[c].words = base + (ok ? e.word_begin : 0u);
+    calls[c].nwords = ok ? e.words : 0u;
 }
 
 }  // namespace
 
-void coder_calls_from_encode(DecCall* d_calls, const CallOut* d_enc, const uint32_t* base, uint32_t ncalls,
-                             hipStream_t st) {
+void coder_calls_from_encode(DecCall* d_calls, const CallOut* d_enc, const uint32_t* base, uint64_t cap,
+                             uint32_t ncalls, hipStream_t st) {
     if (ncalls == 0) return;
     CCA_LAUNCH("dec_calls_from_encode", 0, k_calls_from_encode, dim3(div_up(ncalls, 64)), dim3(64), 0, st, d_calls,
-               d_enc, base, ncalls);
+               d_enc, base, ncalls, cap);
     CCA_LAUNCH_CHECK();
 }
 

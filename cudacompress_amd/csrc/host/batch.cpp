@@ -87,7 +87,8 @@ void encode_prepare(cc_instance* I, DeviceCoder& D, const std::vector<ItemSpec>&
     D.d_call_first.as<uint32_t>(ncalls + 1);
     D.d_seqs.as<SeqInfo>(std::max<uint32_t>(job.nseq, 1));
     D.d_state.as<CoderState>(1);
-    D.d_classes.as<uint32_t>(kDesignListBase + (uint64_t)kDesignLists * std::max<uint32_t>(job.nseq, 1));
+    D.d_classes.as<uint32_t>(kDesignListBase + (uint64_t)kDesignLists * std::max<uint32_t>(job.nseq, 1) +
+                             2ull * std::max<uint32_t>(nitems, 1));
     D.d_calls.as<CallOut>(std::max<uint32_t>(ncalls, 1));
     D.h_result.get(sizeof(CoderState) + sizeof(CallOut) * std::max<uint32_t>(ncalls, 1));
     CCA_HIP(hipMemcpyAsync(D.d_items.p, hi, sizeof(CodeItem) * nitems, hipMemcpyHostToDevice, st));
@@ -245,8 +246,8 @@ void decode_finish(DeviceCoder& D, const DecodeJob& job, std::vector<DecCallOut>
 }
 
 void decode_calls_from_encode(DeviceCoder& D, const DecodeJob& job, const CallOut* d_enc, const uint32_t* base,
-                              hipStream_t st) {
-    coder_calls_from_encode((DecCall*)D.d_dcalls.p, d_enc, base, job.ncalls, st);
+                              uint64_t cap, hipStream_t st) {
+    coder_calls_from_encode((DecCall*)D.d_dcalls.p, d_enc, base, cap, job.ncalls, st);
 }
 
 void device_decode_items(cc_instance* I, const std::vector<ItemSpec>& items, const std::vector<CallWords>& calls,

@@ -52,7 +52,7 @@ void decode_launch(DeviceCoder& D, const DecodeJob& job, hipStream_t st);
 void decode_finish(DeviceCoder& D, const DecodeJob& job, std::vector<DecCallOut>* results, hipStream_t st);
 // Call i of the decode reads base + enc[i].word_begin, enc[i].words words (one kernel).
 void decode_calls_from_encode(DeviceCoder& D, const DecodeJob& job, const CallOut* d_enc, const uint32_t* base,
-                              hipStream_t st);
+                              uint64_t cap, hipStream_t st);
 
 // Eager forms: prepare + launch (+ finish with one synchronisation when `wait`; the encoder
 // retries with a larger arena when it ran out).

@@ -193,7 +193,7 @@ void bricks_encode_launch(BrickCtx& ctx, const BrickJob& job, hipStream_t st, co
 void bricks_decode_launch(BrickCtx& ctx, const BrickJob& job, hipStream_t st, bool from_encode) {
     const uint32_t n = (uint32_t)job.bricks.size();
     if (from_encode)
-        decode_calls_from_encode(ctx.coder, job.dj, (const CallOut*)ctx.coder.d_calls.p, job.d_out, st);
+        decode_calls_from_encode(ctx.coder, job.dj, (const CallOut*)ctx.coder.d_calls.p, job.d_out, job.out_cap, st);
     decode_launch(ctx.coder, job.dj, st);
     for (int l = (int)job.L - 1; l >= 0; l--)
         bricks_inverse_level(job.d_levels + (size_t)l * n, job.d_tiles + (size_t)l * 4 * (n + 1), n,
@@ -465,15 +465,15 @@ cc_bricks_plan* plan_create(cc_instance* I, const cc_brick* bricks, uint32_t n, 
     }
     capture(P->st, &P->g_enc, &P->x_enc, [&] { bricks_encode_launch(ctx, job, P->st); });
     capture(P->st, &P->g_dec, &P->x_dec, [&] { bricks_decode_launch(ctx, job, P->st, true); });
+
     return P.release();
 }
 
-void plan_compress(cc_bricks_plan* P, uint64_t* offsets, uint32_t* status) {
-    DeviceGuard guard(P->inst->device);
+namespace {
+
+// Reads the results of the plan's last compression (synchronises its stream).
+void plan_encode_results(cc_bricks_plan* P, uint64_t* offsets, uint32_t* status) {
     Groups& gr = P->gr;
-    CCA_HIP(hipEventRecord(P->order, P->inst->stream));
-    CCA_HIP(hipStreamWaitEvent(P->st, P->order, 0));
-    CCA_HIP(hipGraphLaunch(P->x_enc, P->st));
     std::vector<CallOut> res;
     const CoderState s = encode_finish(gr.ctx[0]->coder, gr.jobs[0].ej, &res, P->st);
     P->encoded = true;
@@ -484,16 +484,41 @@ void plan_compress(cc_bricks_plan* P, uint64_t* offsets, uint32_t* status) {
     CCA_REQUIRE(ok, CC_ERR_SYMBOL_OVERFLOW, "some bricks failed to encode (see the per-brick status)");
 }
 
+void plan_decode_results(cc_bricks_plan* P, uint32_t* status) {
+    std::vector<DecCallOut> res;
+    decode_finish(P->gr.ctx[0]->coder, P->gr.jobs[0].dj, &res, P->st);
+    CCA_REQUIRE(bricks_decode_results(P->gr.jobs[0], res, status), CC_ERR_CORRUPT_STREAM,
+                "some bricks failed to decode (see the per-brick status)");
+}
+
+}  // namespace
+
+void plan_compress(cc_bricks_plan* P, uint64_t* offsets, uint32_t* status) {
+    DeviceGuard guard(P->inst->device);
+    CCA_HIP(hipEventRecord(P->order, P->inst->stream));
+    CCA_HIP(hipStreamWaitEvent(P->st, P->order, 0));
+    CCA_HIP(hipGraphLaunch(P->x_enc, P->st));
+    plan_encode_results(P, offsets, status);
+}
+
+void plan_run(cc_bricks_plan* P, uint64_t* offsets, uint32_t* cstatus, uint32_t* dstatus) {
+    DeviceGuard guard(P->inst->device);
+    CCA_HIP(hipEventRecord(P->order, P->inst->stream));
+    CCA_HIP(hipStreamWaitEvent(P->st, P->order, 0));
+    CCA_HIP(hipGraphLaunch(P->x_enc, P->st));
+    CCA_HIP(hipGraphLaunch(P->x_dec, P->st));  // stream positions from the device-side results
+    CCA_HIP(hipStreamSynchronize(P->st));
+    plan_encode_results(P, offsets, cstatus);
+    plan_decode_results(P, dstatus);
+}
+
 void plan_decompress(cc_bricks_plan* P, uint32_t* status) {
     DeviceGuard guard(P->inst->device);
     CCA_REQUIRE(P->encoded, CC_ERR_INVALID_ARG, "the plan has not compressed anything yet");
     CCA_HIP(hipEventRecord(P->order, P->inst->stream));
     CCA_HIP(hipStreamWaitEvent(P->st, P->order, 0));
     CCA_HIP(hipGraphLaunch(P->x_dec, P->st));
-    std::vector<DecCallOut> res;
-    decode_finish(P->gr.ctx[0]->coder, P->gr.jobs[0].dj, &res, P->st);
-    CCA_REQUIRE(bricks_decode_results(P->gr.jobs[0], res, status), CC_ERR_CORRUPT_STREAM,
-                "some bricks failed to decode (see the per-brick status)");
+    plan_decode_results(P, status);
 }
 
 void plan_destroy(cc_bricks_plan* P) { delete P; }

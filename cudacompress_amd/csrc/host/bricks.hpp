@@ -66,6 +66,7 @@ cc_bricks_plan* plan_create(cc_instance* I, const cc_brick* bricks, uint32_t n, 
                             uint32_t* d_out, uint64_t cap);
 void plan_compress(cc_bricks_plan* P, uint64_t* offsets, uint32_t* status);
 void plan_decompress(cc_bricks_plan* P, uint32_t* status);
+void plan_run(cc_bricks_plan* P, uint64_t* offsets, uint32_t* cstatus, uint32_t* dstatus);
 void plan_destroy(cc_bricks_plan* P);
 
 }  // namespace cca

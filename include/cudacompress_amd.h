@@ -252,6 +252,12 @@ CC_API cc_status cc_bricks_plan_create(cc_instance* inst, const cc_brick* bricks
 CC_API cc_status cc_bricks_plan_compress(cc_bricks_plan* plan, uint64_t* brickWordOffsets, uint32_t* brickStatus);
 /* Replays the decompression graph (bitstreams of the plan's last compression) into the dst bricks. */
 CC_API cc_status cc_bricks_plan_decompress(cc_bricks_plan* plan, uint32_t* brickStatus);
+/* Replays compression and then decompression with one host synchronisation at the end (the
+ * decompression takes the stream positions from device memory, so nothing waits for the host in
+ * between); fills brickWordOffsets / compressStatus like cc_bricks_plan_compress and
+ * decompressStatus like cc_bricks_plan_decompress.  A compression error is reported first. */
+CC_API cc_status cc_bricks_plan_run(cc_bricks_plan* plan, uint64_t* brickWordOffsets, uint32_t* compressStatus,
+                                    uint32_t* decompressStatus);
 CC_API void cc_bricks_plan_destroy(cc_bricks_plan* plan);
 
 /* ---- Decode serving ----------------------------------------------------------------------- */

@@ -167,6 +167,16 @@ def test_bricks_plan_graph_replays(cc, torch_gpu, field):
         torch.cuda.synchronize()
         for a, b in zip(prec, d2):
             assert np.array_equal(a, b.cpu().numpy()), rep
+        # compression and decompression in one submission: the same words and reconstructions
+        for d in dsts:
+            d.zero_()
+        out.zero_()
+        offs3, cst, dst = plan.run()
+        assert np.all(cst == 0) and np.all(dst == 0)
+        assert np.array_equal(offs3, offs), rep
+        assert np.array_equal(out[: int(offs3[-1])].cpu().numpy().view(np.uint32), pw), rep
+        for a, b in zip(prec, dsts):
+            assert np.array_equal(a, b.cpu().numpy()), rep
     plan.close()
     inst.close()
 
