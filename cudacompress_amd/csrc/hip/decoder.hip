@@ -326,10 +326,12 @@ __global__ __launch_bounds__(64) void k_dec_blocks(const DecSeq* seqs, const uin
     }
 }
 
-// One workgroup per RLE item.  The output is cleared first with 16-byte coalesced stores; then
-// pair i (value v_i after z_i zeros) lands at sum_{j<i}(z_j + 1) + z_i.  Pairs are taken in chunks
-// of 2048 (8 consecutive per thread for the scan); their positions and values are staged in LDS
-// and stored with a 256-pair stride, so one store instruction writes 64 consecutive pairs.
+// One workgroup per RLE item.  Pairs are taken in chunks of 2048 (8 consecutive per thread for
+// the scan); pair i (value v_i after z_i zeros) lands at sum_{j<i}(z_j + 1) + z_i.  Each chunk
+// first clears its own span of output positions with 16-byte coalesced stores (to the end of the
+// output for the last chunk: the trailing zeros), then stores its values there, staged in LDS and
+// issued with a 256-pair stride so that one store instruction writes 64 consecutive pairs.  The
+// span's lines are still in L2 when the values land, so each output line leaves L2 once.
 template <class S>
 __device__ __forceinline__ void clear_symbols(S* out, uint32_t n) {
     const uintptr_t a = reinterpret_cast<uintptr_t>(out);
@@ -358,14 +360,13 @@ __global__ __launch_bounds__(256) void k_dec_rle(const DecItem* items, DecSeq* s
     const S* sym = static_cast<const S*>(ss.out);
     const S* zc = static_cast<const S*>(sz.out);
     S* out = static_cast<S*>(it.out);
-    clear_symbols(out, n);
-    __syncthreads();  // the clearing stores complete before values land on them
     // LDS index swizzle: thread t's 8 consecutive entries and the strided reads are both
     // conflict-free across the 64 banks
     auto sw = [](uint32_t i) { return i ^ ((i >> 6) & 7u); };
     uint64_t carry = 0;
     bool bad = false;
-    for (uint32_t base = 0; base < M; base += kChunk) {
+    for (uint32_t base = 0;; base += kChunk) {
+        const bool last = (uint64_t)base + kChunk >= M;
         uint32_t z[8], v[8], sum = 0;
 #pragma unroll
         for (int k = 0; k < 8; k++) {
@@ -387,7 +388,10 @@ __global__ __launch_bounds__(256) void k_dec_rle(const DecItem* items, DecSeq* s
             sval[e] = v[k];
             pos++;
         }
-        __syncthreads();
+        // the chunk's span of output positions, cleared before its values land
+        const uint64_t s1 = last ? (uint64_t)n : min(carry + tot, (uint64_t)n);
+        if (carry < s1) clear_symbols(out + carry, (uint32_t)(s1 - carry));
+        __syncthreads();  // the clearing stores complete before values land on them
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             const uint32_t idx = k * 256 + threadIdx.x;
@@ -396,6 +400,7 @@ __global__ __launch_bounds__(256) void k_dec_rle(const DecItem* items, DecSeq* s
         }
         __syncthreads();
         carry += tot;
+        if (last) break;
     }
     if (bad) {
         seqs[it.seq].err = kDecCorrupt;
