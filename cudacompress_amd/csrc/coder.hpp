@@ -134,14 +134,16 @@ struct DecCallOut {
 // kSideStreams non-blocking streams, `fork` / `join` events owned by the caller; a null `side`
 // argument puts everything on st.
 constexpr int kSideStreams = 4;
-// Table-design work lists (device buffer of kDesignListBase + kDesignLists * nseq + 2 * nitems
-// words): counters of the design classes at [0, kDesignLists), of the two pack item lists at
-// kPackCountSmall / kPackCountBig; the class lists from kDesignListBase, then the item lists
-// (items whose tables all belong to the two smallest classes, then the others).
+// Table-design work lists (device buffer of kDesignListBase + kDesignLists * nseq +
+// kPackLists * nitems words): counters of the design classes at [0, kDesignLists), of the pack
+// item lists at kPackCountBase + l; the class lists from kDesignListBase, then the item lists
+// (list l < kPackLists - 1: items whose largest table is of design class l; the last list: the
+// others).
 constexpr uint32_t kDesignLists = 5;
-constexpr uint32_t kPackCountSmall = 5;
-constexpr uint32_t kPackCountBig = 6;
+constexpr uint32_t kPackLists = 3;
+constexpr uint32_t kPackCountBase = 5;
 constexpr uint32_t kDesignListBase = 8;
+static_assert(kPackCountBase + kPackLists <= kDesignListBase, "counters before the lists");
 struct SideStreams {
     hipStream_t s[kSideStreams];
     hipEvent_t fork;
@@ -155,7 +157,7 @@ struct LayoutLink {
     hipEvent_t wait;
     hipEvent_t signal;
 };
-// d_classes: kDesignListBase + kDesignLists * nseq + 2 * nitems words of work lists.
+// d_classes: kDesignListBase + kDesignLists * nseq + kPackLists * nitems words of work lists.
 void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uint32_t nseq, const CoderParams& p,
                   int symSize, uint32_t* scratch, uint64_t scratch_cap, CoderState* d_state, uint32_t* d_classes,
                   CallOut* d_calls,

@@ -531,26 +531,28 @@ __global__ __launch_bounds__(256) void k_design_classify(const SeqInfo* seqs, ui
     }
 }
 
-// One thread per item: items whose tables all belong to the two smallest design classes go to
-// the first pack list (packed as soon as those classes are designed), the others to the second.
+// One thread per item: pack list l < kPackLists - 1 takes the items whose largest table is of
+// design class l, the last list the others.  An item's tables may belong to different classes,
+// so list l is packed once every class up to l is designed.
 __global__ __launch_bounds__(256) void k_item_classify(const CodeItem* items, uint32_t nitems, const SeqInfo* seqs,
                                                        uint32_t nseq, uint32_t* classes) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     uint32_t c = 0xFFFFFFFFu;
     if (i < nitems) {
         const CodeItem it = items[i];
-        const bool small = design_class(seqs[it.seq].nsym) <= 1u && (!it.rl || design_class(seqs[it.seq + 1].nsym) <= 1u);
-        c = small ? 0u : 1u;
+        uint32_t k = design_class(seqs[it.seq].nsym);
+        if (it.rl) k = max(k, design_class(seqs[it.seq + 1].nsym));
+        c = min(k, kPackLists - 1);
     }
     const int lane = threadIdx.x & 63;
     uint32_t* lists = classes + kDesignListBase + (size_t)kDesignLists * nseq;
 #pragma unroll
-    for (uint32_t k = 0; k < 2; k++) {
+    for (uint32_t k = 0; k < kPackLists; k++) {
         const uint64_t m = __ballot(c == k);
         if (!m) continue;
         const int leader = __ffsll((unsigned long long)m) - 1;
         uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&classes[k == 0 ? kPackCountSmall : kPackCountBig], (uint32_t)__popcll(m));
+        if (lane == leader) base = atomicAdd(&classes[kPackCountBase + k], (uint32_t)__popcll(m));
         base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
         if (c == k) lists[(size_t)k * nitems + base + __popcll(m & lanes_below(lane))] = i;
     }
@@ -1013,7 +1015,9 @@ void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uin
                d_seqs, p, scratch, scratch_cap, d_state, d_classes, nseq, 4u);
     CCA_LAUNCH_CHECK();
     // items whose tables are all small are packed while the large tables are still being designed:
-    // on the first side stream once the two small classes are done, the others after the join
+    // list l on class l's side stream once classes 0..l are done (class l precedes it on that
+    // stream; the smaller classes through their events), the last list after the join
+    static_assert(kPackLists - 1 <= kClasses, "one side stream per early pack list");
     const uint32_t* lists = d_classes + kDesignListBase + (size_t)kDesignLists * nseq;
     auto pack = [&](hipStream_t sp, const uint32_t* list, const uint32_t* count, uint64_t bytes) {
         if (symSize == 2)
@@ -1024,18 +1028,20 @@ void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uin
                        d_seqs, p, scratch, list, count);
         CCA_LAUNCH_CHECK();
     };
-    if (side) {
-        CCA_HIP(hipEventRecord(side->join[1], side->s[1]));
-        CCA_HIP(hipStreamWaitEvent(side->s[0], side->join[1], 0));
+    for (uint32_t l = 0; l + 1 < kPackLists; l++) {
+        if (side) {
+            for (uint32_t c = 0; c < l; c++) CCA_HIP(hipStreamWaitEvent(side->s[l], side->join[c], 0));
+            CCA_HIP(hipEventRecord(side->join[l], side->s[l]));  // class l done (before its pack)
+        }
+        pack(side ? side->s[l] : st, lists + (size_t)l * nitems, d_classes + kPackCountBase + l, l == 0 ? inb : 0);
     }
-    pack(side ? side->s[0] : st, lists, d_classes + kPackCountSmall, inb);
     if (side) {
         for (int c = 0; c < kClasses; c++) {
             CCA_HIP(hipEventRecord(side->join[c], side->s[c]));
             CCA_HIP(hipStreamWaitEvent(st, side->join[c], 0));
         }
     }
-    pack(st, lists + nitems, d_classes + kPackCountBig, 0);
+    pack(st, lists + (size_t)(kPackLists - 1) * nitems, d_classes + kPackCountBase + kPackLists - 1, 0);
     CCA_LAUNCH_CHECK();
     if (link && link->wait) CCA_HIP(hipStreamWaitEvent(st, link->wait, 0));
     CCA_LAUNCH("code_layout", 0, k_layout, dim3(1), dim3(1024), 0, st, d_items, nitems, d_seqs, p, d_state, d_calls,
