@@ -834,63 +834,94 @@ __device__ __forceinline__ uint64_t seq_words(const SeqInfo& s, const CoderParam
     return (table_bits(s, p) + 31) / 32 + 1 + (s.bits + 31) / 32 + (nblk + 1) / 2;
 }
 
-// Section offsets in call order (one workgroup of 1024 threads).
-__global__ __launch_bounds__(1024) void k_layout(const CodeItem* items, uint32_t nitems, SeqInfo* seqs,
-                                                 CoderParams p, CoderState* st, CallOut* calls, uint32_t ncalls,
-                                                 const uint32_t* call_first, uint64_t* item_off, uint64_t out_cap,
-                                                 const CoderState* prev) {
-    __shared__ uint64_t sh[16];
-    const uint64_t base0 = prev ? prev->out_base + prev->out_words : 0;  // where this job's output starts
-    uint64_t carry = base0;
-    for (uint32_t base = 0; base < nitems; base += 1024) {
-        const uint32_t i = base + threadIdx.x;
-        uint64_t words = 0;
-        if (i < nitems) {
-            const CodeItem it = items[i];
-            const SeqInfo& a = seqs[it.seq];
-            words = it.rl ? 1 + seq_words(a, p) + seq_words(seqs[it.seq + 1], p) : seq_words(a, p);
-        }
-        // block exclusive scan over 16 waves
-        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-        const uint64_t x = wave_incl_scan_u64(words);
-        if (lane == 63) sh[w] = x;
-        __syncthreads();
-        uint64_t b = 0, tot = 0;
-        for (int k = 0; k < 16; k++) {
-            if (k < w) b += sh[k];
-            tot += sh[k];
-        }
-        __syncthreads();
-        const uint64_t off = carry + b + x - words;
-        if (i < nitems) {
-            const CodeItem it = items[i];
-            item_off[i] = off;
-            SeqInfo& a = seqs[it.seq];
-            a.out_off = off + (it.rl ? 1 : 0);
-            if (it.rl) seqs[it.seq + 1].out_off = a.out_off + seq_words(a, p);
-        }
-        carry += tot;
-    }
-    if (threadIdx.x == 0) {
-        item_off[nitems] = carry;
-        st->out_base = base0;
-        st->out_words = carry - base0;
-        if (carry > out_cap) atomicCAS(&st->err, 0u, (uint32_t)kErrOutput);
+// Section offsets in call order, in three launches: the words of every item with each
+// 1024-item block's total, then every block's offsets (its base is the sum of the totals before
+// it), then each call's range and status.
+__device__ __forceinline__ uint64_t item_words(const CodeItem& it, const SeqInfo* seqs, const CoderParams& p) {
+    const SeqInfo& a = seqs[it.seq];
+    return it.rl ? 1 + seq_words(a, p) + seq_words(seqs[it.seq + 1], p) : seq_words(a, p);
+}
+
+// Exclusive scan over the 1024 threads of a workgroup (16 waves); *tot: the workgroup's sum.
+__device__ __forceinline__ uint64_t wg1024_excl_scan(uint64_t x, uint64_t* sh, uint64_t& tot) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t inc = wave_incl_scan_u64(x);
+    if (lane == 63) sh[w] = inc;
+    __syncthreads();
+    uint64_t b = 0;
+    tot = 0;
+    for (int k = 0; k < 16; k++) {
+        if (k < w) b += sh[k];
+        tot += sh[k];
     }
     __syncthreads();
-    for (uint32_t c = threadIdx.x; c < ncalls; c += 1024) {
-        const uint32_t f = call_first[c], l = call_first[c + 1];
-        uint32_t err = 0;
-        for (uint32_t i = f; i < l && !err; i++) {
-            const CodeItem it = items[i];
-            err = seqs[it.seq].err;
-            if (!err && it.rl) err = seqs[it.seq + 1].err;
-        }
-        calls[c].word_begin = item_off[f];
-        calls[c].words = item_off[l] - item_off[f];
-        calls[c].err = err ? err : (item_off[nitems] > out_cap ? (uint32_t)kErrOutput : 0u);
-        calls[c].pad = 0;
+    return b + inc - x;
+}
+
+// Pass 1: item words (kept in item_off for pass 2) and block totals.
+__global__ __launch_bounds__(1024) void k_layout_sizes(const CodeItem* items, uint32_t nitems, const SeqInfo* seqs,
+                                                       CoderParams p, uint64_t* item_off, uint64_t* bsum) {
+    __shared__ uint64_t sh[16];
+    const uint32_t i = blockIdx.x * 1024 + threadIdx.x;
+    uint64_t words = 0;
+    if (i < nitems) {
+        words = item_words(items[i], seqs, p);
+        item_off[i] = words;
     }
+    uint64_t tot;
+    (void)wg1024_excl_scan(words, sh, tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// Pass 2: offsets.  The job's output starts after the previous job's (prev) when chained.
+__global__ __launch_bounds__(1024) void k_layout_offsets(const CodeItem* items, uint32_t nitems, SeqInfo* seqs,
+                                                         CoderParams p, CoderState* st, uint64_t* item_off,
+                                                         const uint64_t* bsum, uint64_t out_cap,
+                                                         const CoderState* prev) {
+    __shared__ uint64_t sh[16];
+    const uint64_t base0 = prev ? prev->out_base + prev->out_words : 0;
+    // sum of the totals of the blocks before this one
+    uint64_t before = 0;
+    for (uint32_t k = threadIdx.x; k < blockIdx.x; k += 1024) before += bsum[k];
+    uint64_t sum_before;
+    (void)wg1024_excl_scan(before, sh, sum_before);
+    const uint32_t i = blockIdx.x * 1024 + threadIdx.x;
+    const uint64_t words = i < nitems ? item_off[i] : 0;
+    uint64_t tot;
+    const uint64_t off = base0 + sum_before + wg1024_excl_scan(words, sh, tot);
+    if (i < nitems) {
+        const CodeItem it = items[i];
+        item_off[i] = off;
+        SeqInfo& a = seqs[it.seq];
+        a.out_off = off + (it.rl ? 1 : 0);
+        if (it.rl) seqs[it.seq + 1].out_off = a.out_off + seq_words(a, p);
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+        const uint64_t end = base0 + sum_before + tot;
+        item_off[nitems] = end;
+        st->out_base = base0;
+        st->out_words = end - base0;
+        if (end > out_cap) atomicCAS(&st->err, 0u, (uint32_t)kErrOutput);
+    }
+}
+
+// Pass 3: one thread per call.
+__global__ __launch_bounds__(256) void k_layout_calls(const CodeItem* items, uint32_t nitems, const SeqInfo* seqs,
+                                                      CallOut* calls, uint32_t ncalls, const uint32_t* call_first,
+                                                      const uint64_t* item_off, uint64_t out_cap) {
+    const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= ncalls) return;
+    const uint32_t f = call_first[c], l = call_first[c + 1];
+    uint32_t err = 0;
+    for (uint32_t i = f; i < l && !err; i++) {
+        const CodeItem it = items[i];
+        err = seqs[it.seq].err;
+        if (!err && it.rl) err = seqs[it.seq + 1].err;
+    }
+    calls[c].word_begin = item_off[f];
+    calls[c].words = item_off[l] - item_off[f];
+    calls[c].err = err ? err : (item_off[nitems] > out_cap ? (uint32_t)kErrOutput : 0u);
+    calls[c].pad = 0;
 }
 
 // 32 header bits starting at bit `pos` of a table header: [nlen:5][cpl x nlen][nsym][sb][symbols].
@@ -965,7 +996,7 @@ void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uin
                   uint32_t ncalls, const uint32_t* d_call_first_item, uint32_t* d_out, uint64_t out_cap,
                   uint64_t total_symbols, hipStream_t st, const SideStreams* side, const LayoutLink* link) {
     CCA_REQUIRE(symSize == 2 || symSize == 4, CC_ERR_INVALID_ARG, "symbol size must be 2 or 4");
-    uint64_t* item_off = reinterpret_cast<uint64_t*>(scratch + scratch_cap);  // caller reserves nitems+1 u64
+    uint64_t* item_off = reinterpret_cast<uint64_t*>(scratch + scratch_cap);  // caller reserves the tail
     CCA_LAUNCH("coder_init", 32, k_coder_init, dim3(1), dim3(64), 0, st, d_state, d_classes);
     CCA_LAUNCH_CHECK();
     if (nitems == 0) return;
@@ -1044,8 +1075,17 @@ void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uin
     pack(st, lists + (size_t)(kPackLists - 1) * nitems, d_classes + kPackCountBase + kPackLists - 1, 0);
     CCA_LAUNCH_CHECK();
     if (link && link->wait) CCA_HIP(hipStreamWaitEvent(st, link->wait, 0));
-    CCA_LAUNCH("code_layout", 0, k_layout, dim3(1), dim3(1024), 0, st, d_items, nitems, d_seqs, p, d_state, d_calls,
-               ncalls, d_call_first_item, item_off, out_cap, link ? link->prev : nullptr);
+    const uint32_t nblocks = div_up(nitems, 1024u);
+    uint64_t* bsum = item_off + nitems + 1;  // caller reserves nitems + 1 + nblocks u64
+    CCA_LAUNCH("code_layout", 0, k_layout_sizes, dim3(nblocks), dim3(1024), 0, st, d_items, nitems, d_seqs, p,
+               item_off, bsum);
+    CCA_LAUNCH_CHECK();
+    CCA_LAUNCH("code_layout", 0, k_layout_offsets, dim3(nblocks), dim3(1024), 0, st, d_items, nitems, d_seqs, p,
+               d_state, item_off, bsum, out_cap, link ? link->prev : nullptr);
+    CCA_LAUNCH_CHECK();
+    if (ncalls)
+        CCA_LAUNCH("code_layout", 0, k_layout_calls, dim3(div_up(ncalls, 256u)), dim3(256), 0, st, d_items, nitems,
+                   d_seqs, d_calls, ncalls, d_call_first_item, item_off, out_cap);
     CCA_LAUNCH_CHECK();
     if (link && link->signal) CCA_HIP(hipEventRecord(link->signal, st));
     CCA_LAUNCH("code_gather", 0, k_gather, dim3(nseq), dim3(kCoderThreads), 0, st, d_items, d_seqs, p, scratch,

@@ -99,7 +99,8 @@ void encode_prepare(cc_instance* I, DeviceCoder& D, const std::vector<ItemSpec>&
 }
 
 void encode_reserve(DeviceCoder& D, EncodeJob& job, uint64_t words) {
-    const uint64_t tail = 2ull * (job.nitems + 1) + 16;  // item offsets (u64) after the arena
+    // item offsets and the layout's block totals (u64) after the arena
+    const uint64_t tail = 2ull * (job.nitems + 1) + 2ull * (div_up(job.nitems, 1024u) + 1) + 16;
     D.d_scratch.as<uint32_t>(words + tail);
     D.scratch_cap = words;
     job.scratch_cap = words;
