@@ -139,22 +139,39 @@ __global__ void k_dec_parse(const DecCall* calls, uint32_t ncalls, const DecItem
 // canonical assignment) and absolute coding-block offsets.
 __global__ __launch_bounds__(256) void k_dec_tables(DecSeq* seqs, CoderParams p, uint32_t* scratch,
                                                     CoderState* st) {
+    constexpr uint32_t kSymPer = (1u << kLut) / 256;  // symbols per thread read up front
     __shared__ uint32_t cpl[32];
     __shared__ uint32_t sh[4];
     __shared__ uint32_t bad;
+    __shared__ int tab_sh[96];
+    __shared__ uint32_t sym_lo[1 << kLut];
     DecSeq& s = seqs[blockIdx.x];
     if (s.err) return;
-    if (threadIdx.x == 0) bad = 0;
-    if (threadIdx.x < 32) {
-        bool ok = true;
-        cpl[threadIdx.x] = threadIdx.x < s.nlen
-                               ? rd_bits(s.base, s.nwords, s.hdr_bit + 5 + (uint64_t)p.count_bits * threadIdx.x,
-                                         p.count_bits, ok)
-                               : 0u;
-        if (!ok) atomicOr(&bad, 1u);
+    // every independent header read is issued before the first barrier: the code-length counts,
+    // the first 2^kLut symbols and the first 1024 block-offset increments
+    bool ok = true;
+    const uint32_t cplv = threadIdx.x < s.nlen
+                              ? rd_bits(s.base, s.nwords, s.hdr_bit + 5 + (uint64_t)p.count_bits * threadIdx.x,
+                                        p.count_bits, ok)
+                              : 0u;
+    uint32_t symv[kSymPer];
+#pragma unroll
+    for (uint32_t j = 0; j < kSymPer; j++) {
+        const uint32_t k = threadIdx.x + 256 * j;
+        symv[j] = k < s.nsym ? rd_bits(s.base, s.nwords, s.sym_bit + (uint64_t)s.sb * k, s.sb, ok) : 0u;
     }
+    const uint32_t nblk = (s.count + p.cbs - 1) / p.cbs;
+    const uint32_t* inc = s.base + s.incs_off;
+    uint32_t incv[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t b = threadIdx.x * 4 + k;
+        incv[k] = b < nblk ? (inc[b >> 1] >> (16 * (b & 1))) & 0xFFFFu : 0u;
+    }
+    if (threadIdx.x == 0) bad = 0;
+    if (threadIdx.x < 32) cpl[threadIdx.x] = cplv;
     __syncthreads();
-    __shared__ int tab_sh[96];
+    if (!ok) atomicOr(&bad, 1u);
     if (threadIdx.x == 0) {
         uint32_t fc[32], fi[32];
         design::first_codes(cpl, (int)s.nlen, fc, fi);
@@ -168,30 +185,31 @@ __global__ __launch_bounds__(256) void k_dec_tables(DecSeq* seqs, CoderParams p,
         }
         if (s.nlen > 0 && total != s.nsym) bad = 1;
     }
-    __syncthreads();
-    if (threadIdx.x < 96) s.tab[threadIdx.x] = tab_sh[threadIdx.x];
-    // symbol table (codeword-index order) and the lookup table for codes of up to kLut bits:
-    // canonical codes of length <= kLut belong to the first 2^kLut symbols
-    __shared__ uint32_t sym_lo[1 << kLut];
-    __shared__ int tb[96];
-    __syncthreads();
+    // symbol table (codeword-index order); canonical codes of length <= kLut belong to the first
+    // 2^kLut symbols, kept in LDS for the lookup table
     uint32_t* gsyms = scratch + s.lut_off + (1u << kLut);
     uint32_t* glut = scratch + s.lut_off;
-    for (uint32_t k = threadIdx.x; k < s.nsym; k += 256) {
-        bool ok = true;
-        const uint32_t v = rd_bits(s.base, s.nwords, s.sym_bit + (uint64_t)s.sb * k, s.sb, ok);
-        if (!ok) atomicOr(&bad, 1u);
-        gsyms[k] = v;
-        if (k < (1u << kLut)) sym_lo[k] = v;
+#pragma unroll
+    for (uint32_t j = 0; j < kSymPer; j++) {
+        const uint32_t k = threadIdx.x + 256 * j;
+        if (k < s.nsym) {
+            gsyms[k] = symv[j];
+            sym_lo[k] = symv[j];
+        }
     }
-    for (int i = threadIdx.x; i < 96; i += 256) tb[i] = tab_sh[i];
+    for (uint32_t k = threadIdx.x + (1u << kLut); k < s.nsym; k += 256) {
+        bool ok2 = true;
+        gsyms[k] = rd_bits(s.base, s.nwords, s.sym_bit + (uint64_t)s.sb * k, s.sb, ok2);
+        if (!ok2) atomicOr(&bad, 1u);
+    }
     __syncthreads();
+    if (threadIdx.x < 96) s.tab[threadIdx.x] = tab_sh[threadIdx.x];
     for (uint32_t e = threadIdx.x; e < (1u << kLut); e += 256) {
         uint32_t entry = 0;
         for (int l = 1; l <= (int)kLut && l <= (int)s.nlen; l++) {
             const int cw = (int)(e >> (kLut - l));
-            if (cw <= tb[64 + l - 1]) {
-                const int idx = tb[l - 1] + cw - tb[32 + l - 1];
+            if (cw <= tab_sh[64 + l - 1]) {
+                const int idx = tab_sh[l - 1] + cw - tab_sh[32 + l - 1];
                 const uint32_t sym = (idx >= 0 && (uint32_t)idx < s.nsym && idx < (1 << kLut)) ? sym_lo[idx] : 0u;
                 entry = ((uint32_t)l << 24) | (sym & 0xFFFFFFu);
                 break;
@@ -200,16 +218,14 @@ __global__ __launch_bounds__(256) void k_dec_tables(DecSeq* seqs, CoderParams p,
         glut[e] = entry;
     }
     // absolute block offsets: running sum of the u16 increments
-    const uint32_t nblk = (s.count + p.cbs - 1) / p.cbs;
     uint32_t* boff = scratch + s.boff_off;
-    const uint32_t* inc = s.base + s.incs_off;
     const uint64_t limit = 32ull * s.nwords_cw;
     uint32_t carry = 0;
     for (uint32_t base = 0; base < nblk; base += 256 * 4) {
         uint32_t v[4], sum = 0;
         for (int k = 0; k < 4; k++) {
             const uint32_t b = base + threadIdx.x * 4 + k;
-            v[k] = b < nblk ? (inc[b >> 1] >> (16 * (b & 1))) & 0xFFFFu : 0u;
+            v[k] = base == 0 ? incv[k] : (b < nblk ? (inc[b >> 1] >> (16 * (b & 1))) & 0xFFFFu : 0u);
             sum += v[k];
         }
         uint32_t tot;
