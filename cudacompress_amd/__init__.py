@@ -27,6 +27,7 @@ from .api import (  # noqa: E402
     Server,
     compressBricksFloat,
     compressVolumeFloat,
+    compressVolumeFloatQuantFirst,
     compressVolumeLossless,
     createInstance,
     decodeDevice,
@@ -34,6 +35,7 @@ from .api import (  # noqa: E402
     decodeRLHuff,
     decompressBricksFloat,
     decompressVolumeFloat,
+    decompressVolumeFloatQuantFirst,
     decompressVolumeLossless,
     destroyInstance,
     encodeDevice,

@@ -57,6 +57,10 @@ SIGNATURES = {
     "cc_get_max_abs": (c_int, [c_vp, c_vp, c_u32, c_vp]),
     "cc_compress_volume_float": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, c_float, c_int, c_int, PP_u32, P_u64]),
     "cc_decompress_volume_float": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, c_vp, c_u64, c_float, c_int, c_int]),
+    "cc_compress_volume_float_quant_first": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, c_float, c_int, PP_u32,
+                                                     P_u64]),
+    "cc_decompress_volume_float_quant_first": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, c_vp, c_u64, c_float,
+                                                       c_int]),
     "cc_compress_volume_lossless": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, PP_u32, P_u64]),
     "cc_decompress_volume_lossless": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, c_vp, c_u64]),
     "cc_compress_bricks_float": (c_int, [c_vp, c_vp, c_u32, c_u32, c_int, c_vp, c_u64, P_u64, P_u32]),

@@ -710,6 +710,33 @@ def decompressVolumeFloat(inst, dpImage, sizeX, sizeY, sizeZ, numLevels, bits, q
                                                    int(doRLEOnlyOnLvl0), inst.symbolSize), "decompressVolumeFloat")
 
 
+def compressVolumeFloatQuantFirst(inst, dpImage, sizeX, sizeY, sizeZ, numLevels, quantizationStep,
+                                  doRLEOnlyOnLvl0=False):
+    """compressVolumeFloatQuantFirst (src/examples/CompressVolume.cpp:651); returns the bitstream
+    words, or None on symbol overflow (the reference returns false)."""
+    lib = _capi.load()
+    words = _capi.P_u32()
+    nbits = ctypes.c_uint64()
+    st = lib.cc_compress_volume_float_quant_first(inst.handle, _ptr(dpImage), sizeX, sizeY, sizeZ, numLevels,
+                                                  quantizationStep, int(doRLEOnlyOnLvl0), ctypes.byref(words),
+                                                  ctypes.byref(nbits))
+    if st == CC_ERR_SYMBOL_OVERFLOW:
+        return None
+    _check(st, "compressVolumeFloatQuantFirst")
+    n = nbits.value // 32
+    return np.ctypeslib.as_array(words, shape=(max(n, 1),))[:n].copy()
+
+
+def decompressVolumeFloatQuantFirst(inst, dpImage, sizeX, sizeY, sizeZ, numLevels, bits, quantizationStep,
+                                    doRLEOnlyOnLvl0=False):
+    """decompressVolumeFloatQuantFirst (src/examples/CompressVolume.cpp:759)."""
+    bits = np.ascontiguousarray(bits, dtype=np.uint32)
+    _check(_capi.load().cc_decompress_volume_float_quant_first(inst.handle, _ptr(dpImage), sizeX, sizeY, sizeZ,
+                                                               numLevels, bits.ctypes.data, bits.size * 32,
+                                                               quantizationStep, int(doRLEOnlyOnLvl0)),
+           "decompressVolumeFloatQuantFirst")
+
+
 def compressVolumeLossless(inst, dpImage, sizeX, sizeY, sizeZ, numLevels):
     """compressVolumeLossless (src/examples/CompressVolume.cpp:234); sizeZ == 1 for 2D fields."""
     lib = _capi.load()

@@ -437,6 +437,26 @@ cc_status cc_decompress_volume_float(cc_instance* inst, float* d_image, uint32_t
                                      doRLEOnlyOnLvl0 != 0, symbolSize);
     });
 }
+cc_status cc_compress_volume_float_quant_first(cc_instance* inst, const float* d_image, uint32_t sizeX, uint32_t sizeY,
+                                               uint32_t sizeZ, uint32_t numLevels, float quantizationStep,
+                                               int doRLEOnlyOnLvl0, const uint32_t** outWords, uint64_t* outBits) {
+    return guarded([&] {
+        CCA_REQUIRE(inst && d_image && outWords && outBits, CC_ERR_INVALID_ARG, "null argument");
+        cca::compress_volume_float_quant_first(inst, d_image, sizeX, sizeY, sizeZ, numLevels, quantizationStep,
+                                               doRLEOnlyOnLvl0 != 0);
+        *outWords = inst->vol_bits.data();
+        *outBits = 32ull * inst->vol_bits.size();
+    });
+}
+cc_status cc_decompress_volume_float_quant_first(cc_instance* inst, float* d_image, uint32_t sizeX, uint32_t sizeY,
+                                                 uint32_t sizeZ, uint32_t numLevels, const uint32_t* bits,
+                                                 uint64_t bitCount, float quantizationStep, int doRLEOnlyOnLvl0) {
+    return guarded([&] {
+        CCA_REQUIRE(inst && d_image && bits, CC_ERR_INVALID_ARG, "null argument");
+        cca::decompress_volume_float_quant_first(inst, d_image, sizeX, sizeY, sizeZ, numLevels, bits, bitCount,
+                                                 quantizationStep, doRLEOnlyOnLvl0 != 0);
+    });
+}
 cc_status cc_compress_volume_lossless(cc_instance* inst, const int16_t* d_image, uint32_t sizeX, uint32_t sizeY,
                                       uint32_t sizeZ, uint32_t numLevels, const uint32_t** outWords,
                                       uint64_t* outBits) {

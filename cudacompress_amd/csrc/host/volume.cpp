@@ -1,13 +1,17 @@
 // Volume codecs built on the library API, restating the caller layouts of
 // src/examples/CompressVolume.cpp (compressVolumeFloat :313-409, decompressVolumeFloat :412-516,
-// compressVolumeLossless :234-277, decompressVolumeLossless :279-309) and, for 2D lossless
-// input, the three-band layout of src/examples/CompressHeightfield.cpp:116-126.
+// compressVolumeLossless :234-277, decompressVolumeLossless :279-309,
+// compressVolumeFloatQuantFirst :651-757, decompressVolumeFloatQuantFirst :759-856) and, for 2D
+// lossless input, the three-band layout of src/examples/CompressHeightfield.cpp:116-126.
 //
 // Differences from the reference callers, none of which changes the bitstream:
 //  * every kernel runs on the instance stream (the reference quantizes on the default stream);
 //  * decode outputs are not cleared first (decodeRLHuff here writes every element);
 //  * 2D lossless input (sizeZ == 1) uses 3 highpass bands per level -- compressVolumeLossless
-//    itself only handles 3D (it would compute a zero blockSizeZ).
+//    itself only handles 3D (it would compute a zero blockSizeZ);
+//  * decompressVolumeFloatQuantFirst starts its inverse transforms from the buffer holding the
+//    coarsest level's coefficients; the reference starts from buffer numLevels % 2, the same
+//    buffer for an even level count (an odd count reads a buffer its decode did not fill).
 #include "volume.hpp"
 
 #include <cstring>
@@ -238,6 +242,103 @@ void decompress_volume_lossless(cc_instance* I, int16_t* image, uint32_t sx, uin
         }
         dwt53_inverse(target, dwt0, dwt1, (int)cx, (int)cy, (int)cz, 0, 0, 0, 0, st);
     }
+}
+
+void compress_volume_float_quant_first(cc_instance* I, const float* image, uint32_t sx, uint32_t sy, uint32_t sz,
+                                       uint32_t levels, float step, bool rle_only_lvl0) {
+    DeviceGuard guard(I->device);
+    hipStream_t st = I->stream;
+    CCA_REQUIRE(I->log2 <= 16, CC_ERR_INVALID_ARG, "QuantFirst volumes use Symbol16 (log2 <= 16)");
+    CCA_REQUIRE(levels >= 1 && sz > 1, CC_ERR_INVALID_ARG, "QuantFirst volumes are 3D with numLevels >= 1");
+    const uint32_t f_all = 1u << levels;
+    CCA_REQUIRE(sx % f_all == 0 && sy % f_all == 0 && sz % f_all == 0, CC_ERR_INVALID_ARG,
+                "sizes must be divisible by 2^numLevels");
+    const uint64_t elems = (uint64_t)sx * sy * sz;
+    const uint32_t nb = 7;
+    int16_t* dwt[2] = {I->d_vol[0].as<int16_t>(elems), I->d_vol[1].as<int16_t>(elems)};
+    const uint64_t per_block_max = elems / 8;
+    uint16_t* sym = I->d_vol[2].as<uint16_t>(nb * per_block_max);
+    std::vector<const void*> streams(nb);
+    for (uint32_t i = 0; i < nb; i++) streams[i] = sym + i * per_block_max;
+    // uniform quantization of the whole volume (CompressVolume.cpp:685-687)
+    quantize_to_short(dwt[0], image, sx, sy, sz, step, sx, sx * sy, 1, st);
+    // every 5/3 level: data path src -> dst -> buf -> dst with src and buf the same buffer, so the
+    // coarsest level's coefficients end in dwt[d], the next finer level's in dwt[1 - d], ...
+    int d = 0;
+    uint32_t f = 1;
+    for (uint32_t level = 0; level < levels; level++) {
+        dwt53_forward(dwt[1 - d], dwt[d], dwt[d], (int)(sx / f), (int)(sy / f), (int)(sz / f), (int)sx, (int)(sx * sy),
+                      (int)sx, (int)(sx * sy), st);
+        d = 1 - d;
+        f *= 2;
+    }
+    std::vector<uint32_t>& bits = I->vol_bits;
+    bits.clear();
+    // lowpass band (CompressVolume.cpp:704-712)
+    symbolize(sym, dwt[d], sx / f, sy / f, sz / f, sx, sx * sy, st);
+    encode_streams(I, false, 2, streams.data(), 1, (uint32_t)(elems / ((uint64_t)f * f * f)), nullptr);
+    append_segments(I, bits);
+    // highpass bands, coarsest level first (CompressVolume.cpp:715-749)
+    for (int level = (int)levels - 1; level >= 0; level--) {
+        const uint32_t bx = sx / f, by = sy / f, bz = sz / f;
+        for (uint32_t b = 1; b <= nb; b++) {
+            const Band bd = band_of(b, true);
+            const uint64_t off = bd.x * bx + (uint64_t)bd.y * by * sx + (uint64_t)bd.z * bz * sx * sy;
+            symbolize(sym + (b - 1) * per_block_max, dwt[d] + off, bx, by, bz, sx, sx * sy, st);
+        }
+        d = 1 - d;
+        encode_streams(I, !(level > 0 && rle_only_lvl0), 2, streams.data(), nb, bx * by * bz, nullptr);
+        append_segments(I, bits);
+        f /= 2;
+    }
+}
+
+void decompress_volume_float_quant_first(cc_instance* I, float* image, uint32_t sx, uint32_t sy, uint32_t sz,
+                                         uint32_t levels, const uint32_t* words, uint64_t nbits, float step,
+                                         bool rle_only_lvl0) {
+    DeviceGuard guard(I->device);
+    hipStream_t st = I->stream;
+    CCA_REQUIRE(I->log2 <= 16, CC_ERR_INVALID_ARG, "QuantFirst volumes use Symbol16 (log2 <= 16)");
+    CCA_REQUIRE(levels >= 1 && sz > 1, CC_ERR_INVALID_ARG, "QuantFirst volumes are 3D with numLevels >= 1");
+    const uint32_t f_all = 1u << levels;
+    CCA_REQUIRE(sx % f_all == 0 && sy % f_all == 0 && sz % f_all == 0, CC_ERR_INVALID_ARG,
+                "sizes must be divisible by 2^numLevels");
+    const uint64_t elems = (uint64_t)sx * sy * sz;
+    const uint32_t nb = 7;
+    int16_t* dwt[2] = {I->d_vol[0].as<int16_t>(elems), I->d_vol[1].as<int16_t>(elems)};
+    const uint64_t per_block_max = elems / 8;
+    uint16_t* sym = I->d_vol[2].as<uint16_t>(nb * per_block_max);
+    std::vector<void*> streams(nb);
+    for (uint32_t i = 0; i < nb; i++) streams[i] = sym + i * per_block_max;
+    uint64_t pos = 0;
+    uint32_t f = f_all;
+    // lowpass band into dwt[0] (CompressVolume.cpp:794-801)
+    decode_streams(I, false, 2, &words, &nbits, &pos, 1, nullptr, streams.data(), 1,
+                   (uint32_t)(elems / ((uint64_t)f * f * f)));
+    unsymbolize(dwt[0], sym, sx / f, sy / f, sz / f, sx, sx * sy, st);
+    // highpass bands, coarsest level first, alternating buffers from dwt[0] (:804-830)
+    int d = 0;
+    for (int level = (int)levels - 1; level >= 0; level--) {
+        const uint32_t bx = sx / f, by = sy / f, bz = sz / f;
+        decode_streams(I, !(level > 0 && rle_only_lvl0), 2, &words, &nbits, &pos, 1, nullptr, streams.data(), nb,
+                       bx * by * bz);
+        f /= 2;
+        for (uint32_t b = 1; b <= nb; b++) {
+            const Band bd = band_of(b, true);
+            const uint64_t off = bd.x * bx + (uint64_t)bd.y * by * sx + (uint64_t)bd.z * bz * sx * sy;
+            unsymbolize(dwt[d] + off, sym + (b - 1) * per_block_max, bx, by, bz, sx, sx * sy, st);
+        }
+        d = 1 - d;
+    }
+    // inverse levels from the coarsest, whose coefficients are in dwt[0] (:833-838)
+    d = 0;
+    for (int level = (int)levels - 1; level >= 0; level--) {
+        const uint32_t fl = 1u << level;
+        dwt53_inverse(dwt[1 - d], dwt[d], dwt[d], (int)(sx / fl), (int)(sy / fl), (int)(sz / fl), (int)sx,
+                      (int)(sx * sy), (int)sx, (int)(sx * sy), st);
+        d = 1 - d;
+    }
+    unquantize_from_short(image, dwt[d], sx, sy, sz, step, sx, sx * sy, 1, st);
 }
 
 }  // namespace cca

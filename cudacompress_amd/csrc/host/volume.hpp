@@ -13,5 +13,10 @@ void compress_volume_lossless(cc_instance* I, const int16_t* image, uint32_t sx,
                               uint32_t levels);
 void decompress_volume_lossless(cc_instance* I, int16_t* image, uint32_t sx, uint32_t sy, uint32_t sz,
                                 uint32_t levels, const uint32_t* words, uint64_t nbits);
+void compress_volume_float_quant_first(cc_instance* I, const float* image, uint32_t sx, uint32_t sy, uint32_t sz,
+                                       uint32_t levels, float step, bool rle_only_lvl0);
+void decompress_volume_float_quant_first(cc_instance* I, float* image, uint32_t sx, uint32_t sy, uint32_t sz,
+                                         uint32_t levels, const uint32_t* words, uint64_t nbits, float step,
+                                         bool rle_only_lvl0);
 
 }  // namespace cca

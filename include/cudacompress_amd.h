@@ -209,6 +209,20 @@ CC_API cc_status cc_decompress_volume_lossless(cc_instance* inst, int16_t* d_ima
                                                uint32_t sizeZ, uint32_t numLevels, const uint32_t* bits,
                                                uint64_t bitCount);
 
+/* compressVolumeFloatQuantFirst, src/examples/CompressVolume.cpp:651-757: uniform quantization to
+ * int16, numLevels 5/3 levels, Symbol16 streams (log2 <= 16).  Output held by the instance. */
+CC_API cc_status cc_compress_volume_float_quant_first(cc_instance* inst, const float* d_image, uint32_t sizeX,
+                                                      uint32_t sizeY, uint32_t sizeZ, uint32_t numLevels,
+                                                      float quantizationStep, int doRLEOnlyOnLvl0,
+                                                      const uint32_t** outWords, uint64_t* outBits);
+/* decompressVolumeFloatQuantFirst, src/examples/CompressVolume.cpp:759-856 (the inverse transforms
+ * start from the buffer holding the coarsest level for any level count; the reference's choice,
+ * buffer numLevels % 2, is that buffer only for an even count). */
+CC_API cc_status cc_decompress_volume_float_quant_first(cc_instance* inst, float* d_image, uint32_t sizeX,
+                                                        uint32_t sizeY, uint32_t sizeZ, uint32_t numLevels,
+                                                        const uint32_t* bits, uint64_t bitCount,
+                                                        float quantizationStep, int doRLEOnlyOnLvl0);
+
 /* ---- Brick batches ------------------------------------------------------------------------ */
 /* No single reference counterpart: a batch is compressVolumeFloat / decompressVolumeFloat
  * (src/examples/CompressVolume.cpp:313-516, doRLEOnlyOnLvl0 = false) applied to many bricks of

@@ -716,6 +716,35 @@ void orc_quantize(const float* data, uint32_t n, float step, int qtype, int symb
     }
 }
 
+// quantize<Q> (Quantize.cuh:76-120) as an int: deadzone truncates data/step, uniform rounds half
+// away from zero (data/step + sign * 0.5, truncated); both saturate like the device conversion.
+static int quantize_int(float v, float inv, int qtype) {
+    volatile float prod = v * inv;
+    if (qtype == 0) return sat_f2i(prod);
+    float sg = v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f);
+    volatile float t = prod + sg * 0.5f;
+    return sat_f2i(t);
+}
+
+// unquantize<Q> (Quantize.cuh:122-147) of an int
+static float unquantize_int(int q, float step, int qtype) {
+    if (qtype == 0) {
+        float sg = q < 0 ? -1.0f : (q > 0 ? 1.0f : 0.0f);
+        volatile float t = (float)q + sg * 0.5f;
+        return t * step;
+    }
+    return (float)q * step;
+}
+
+void orc_quantize_to_short(const float* data, uint32_t n, float step, int qtype, int16_t* quant) {
+    volatile float inv = 1.0f / step;  // the launcher passes 1.0f / quantizationStep (Quantize.cu:236)
+    for (uint32_t i = 0; i < n; i++) quant[i] = (int16_t)(uint16_t)(uint32_t)quantize_int(data[i], inv, qtype);
+}
+
+void orc_unquantize_from_short(const int16_t* quant, uint32_t n, float step, int qtype, float* data) {
+    for (uint32_t i = 0; i < n; i++) data[i] = unquantize_int(quant[i], step, qtype);
+}
+
 void orc_unquantize(const void* symbols, uint32_t n, float step, int qtype, int symbol_size, float* data) {
     for (uint32_t i = 0; i < n; i++) {
         uint32_t s = load_sym(symbols, symbol_size, i);

@@ -97,6 +97,11 @@ void orc_dwt53_inverse(int16_t* data, int sx, int sy, int sz);
 void orc_quantize(const float* data, uint32_t n, float step, int qtype, int symbol_size, void* symbols);
 void orc_unquantize(const void* symbols, uint32_t n, float step, int qtype, int symbol_size, float* data);
 void orc_symbolize(const int16_t* data, uint32_t n, uint16_t* symbols);
+/* quantizeToShort / unquantizeFromShort: src/cudaCompress/util/QuantizeKernels.cui:136-172
+ * (quantize<Q> / unquantize<Q> of src/cudaCompress/util/Quantize.cuh:76-147 with the int result
+ * stored as a short: two's-complement wrap). */
+void orc_quantize_to_short(const float* data, uint32_t n, float step, int qtype, int16_t* quant);
+void orc_unquantize_from_short(const int16_t* quant, uint32_t n, float step, int qtype, float* data);
 void orc_unsymbolize(const uint16_t* symbols, uint32_t n, int16_t* data);
 
 #ifdef __cplusplus

@@ -1,8 +1,9 @@
 """CPU composition of the volume codecs out of oracle primitives (test infrastructure only).
 
 Restates the caller layouts of src/examples/CompressVolume.cpp (compressVolumeFloat :313-409,
-decompressVolumeFloat :412-516, compressVolumeLossless :234-309) and, for 2D lossless fields, the
-band layout of src/examples/CompressHeightfield.cpp:116-126, on top of oracle/liboracle.so.
+decompressVolumeFloat :412-516, compressVolumeLossless :234-309, compressVolumeFloatQuantFirst
+:651-757, decompressVolumeFloatQuantFirst :759-856) and, for 2D lossless fields, the band layout of
+src/examples/CompressHeightfield.cpp:116-126, on top of oracle/liboracle.so.
 Used by tests/ as the checker and by bench.py as the timed CPU baseline.
 """
 import numpy as np
@@ -114,3 +115,48 @@ def decompress_lossless(words, shape, levels, log2=16, cbs=128):
             coef[b] = o.unsymbolize(s).reshape(coef[b].shape)
         cur = o.dwt53_inverse(coef)
     return cur
+
+
+def compress_volume_float_quant_first(vol, levels, step, log2=14, cbs=128, rle_only_lvl0=False):
+    """compressVolumeFloatQuantFirst (CompressVolume.cpp:651-757): uniform quantization to int16,
+    `levels` 5/3 levels, the lowpass band symbolized and Huffman coded, then per level (coarsest
+    first) the 7 symbolized highpass bands RL+Huffman coded (Huffman only above level 0 with
+    rle_only_lvl0)."""
+    vol = np.ascontiguousarray(vol, dtype=np.float32)
+    Z, Y, X = vol.shape
+    coef = o.quantize_to_short(vol, step, qtype=1)
+    for lv in range(levels):
+        s = (slice(0, Z >> lv), slice(0, Y >> lv), slice(0, X >> lv))
+        coef[s] = o.dwt53_forward(np.ascontiguousarray(coef[s]))
+    f = 1 << levels
+    lp = coef[: Z // f, : Y // f, : X // f]
+    w, _ = o.encode([o.symbolize(np.ascontiguousarray(lp).ravel())], rl=False, log2=log2, cbs=cbs)
+    words = [w]
+    for level in range(levels - 1, -1, -1):
+        region = coef[: Z // (f // 2), : Y // (f // 2), : X // (f // 2)]
+        streams = [o.symbolize(np.ascontiguousarray(region[b]).ravel()) for b in _bands3d(region.shape)]
+        w, _ = o.encode(streams, rl=not (level > 0 and rle_only_lvl0), log2=log2, cbs=cbs)
+        words.append(w)
+        f //= 2
+    return np.concatenate(words)
+
+
+def decompress_volume_float_quant_first(words, shape, levels, step, log2=14, cbs=128, rle_only_lvl0=False):
+    """decompressVolumeFloatQuantFirst (CompressVolume.cpp:759-856): the inverse of the above, then
+    unquantizeFromShort (uniform)."""
+    Z, Y, X = shape
+    nbits = words.size * 32
+    coef = np.zeros(shape, dtype=np.int16)
+    f = 1 << levels
+    (lp,), pos = o.decode(words, nbits, 1, (Z // f) * (Y // f) * (X // f), rl=False, log2=log2, cbs=cbs)
+    coef[: Z // f, : Y // f, : X // f] = o.unsymbolize(lp).reshape(Z // f, Y // f, X // f)
+    for level in range(levels - 1, -1, -1):
+        count = (Z // f) * (Y // f) * (X // f)
+        rl = not (level > 0 and rle_only_lvl0)
+        streams, pos = o.decode(words, nbits, 7, count, rl=rl, log2=log2, cbs=cbs, bitpos=pos)
+        f //= 2
+        region = coef[: Z // f, : Y // f, : X // f]
+        for b, st in zip(_bands3d(region.shape), streams):
+            region[b] = o.unsymbolize(st).reshape(region[b].shape)
+        coef[: Z // f, : Y // f, : X // f] = o.dwt53_inverse(np.ascontiguousarray(region))
+    return o.unquantize_from_short(coef, step, qtype=1)

@@ -215,6 +215,22 @@ def unquantize(symbols, step, qtype=0):
     return out
 
 
+def quantize_to_short(data, step, qtype=0):
+    """quantizeToShort (src/cudaCompress/util/QuantizeKernels.cui:136-151): int16 (wrapped)."""
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    out = np.zeros(data.shape, dtype=np.int16)
+    lib().orc_quantize_to_short(_ptr(data), data.size, ctypes.c_float(step), qtype, _ptr(out))
+    return out
+
+
+def unquantize_from_short(quant, step, qtype=0):
+    """unquantizeFromShort (src/cudaCompress/util/QuantizeKernels.cui:154-172)."""
+    quant = np.ascontiguousarray(quant, dtype=np.int16)
+    out = np.zeros(quant.shape, dtype=np.float32)
+    lib().orc_unquantize_from_short(_ptr(quant), quant.size, ctypes.c_float(step), qtype, _ptr(out))
+    return out
+
+
 def symbolize(data):
     data = np.ascontiguousarray(data, dtype=np.int16)
     out = np.zeros(data.shape, dtype=np.uint16)

@@ -1,4 +1,5 @@
-"""GPU parity of the volume codecs (compressVolumeFloat / compressVolumeLossless callers of
+"""GPU parity of the volume codecs (compressVolumeFloat, compressVolumeFloatQuantFirst and
+compressVolumeLossless callers of
 src/examples/CompressVolume.cpp) against the oracle composition in oracle/pipeline.py."""
 import numpy as np
 import pytest
@@ -56,3 +57,52 @@ def test_volume_lossless(cc, torch_gpu, shape, levels):
     torch.cuda.synchronize()
     assert np.array_equal(to_host(out), img)
     inst.close()
+
+
+@pytest.mark.parametrize("shape,levels", [((32, 32, 32), 2), ((64, 48, 32), 2), ((32, 64, 64), 3), ((16, 16, 32), 1)],
+                         ids=["32^3L2", "64x48x32L2", "32x64x64L3", "16x16x32L1"])
+@pytest.mark.parametrize("rle_lvl0", [False, True], ids=["rle_all", "rle_lvl0"])
+def test_volume_float_quant_first(cc, torch_gpu, shape, levels, rle_lvl0):
+    """compressVolumeFloatQuantFirst / decompressVolumeFloatQuantFirst against the oracle: words
+    equal, reconstruction equal (exact: the codec loses only in the quantizer)."""
+    torch = torch_gpu
+    log("volume quant-first", shape, levels, rle_lvl0)
+    vol = S.gaussian_field(shape, 5) * 40.0
+    step = 0.05
+    inst = cc.createInstance(0, 7, int(np.prod(shape)) // 8, 128, 14)
+    src = to_dev(torch, vol)
+    words = cc.compressVolumeFloatQuantFirst(inst, src, shape[2], shape[1], shape[0], levels, step, rle_lvl0)
+    want = P.compress_volume_float_quant_first(vol, levels, step, rle_only_lvl0=rle_lvl0)
+    assert words is not None
+    assert words.size == want.size and np.array_equal(words, want)
+    out = torch.full(shape, 123.0, dtype=torch.float32, device="cuda")
+    cc.decompressVolumeFloatQuantFirst(inst, out, shape[2], shape[1], shape[0], levels, words, step, rle_lvl0)
+    torch.cuda.synchronize()
+    rec = to_host(out)
+    ref = P.decompress_volume_float_quant_first(want, shape, levels, step, rle_only_lvl0=rle_lvl0)
+    log(f"   words={words.size} psnr={S.psnr(vol, rec):.2f}")
+    assert np.array_equal(rec, ref)
+    inst.close()
+
+
+@pytest.mark.parametrize("qtype", [0, 1], ids=["deadzone", "uniform"])
+def test_quantize_to_short(cc, torch_gpu, qtype):
+    """util::quantizeToShort / unquantizeFromShort with source / destination pitches, including
+    values whose int wraps in int16."""
+    from oracle import pyoracle as o
+    torch = torch_gpu
+    log("quantize to short", qtype)
+    z, y, x = 10, 18, 50
+    data = (S.gaussian_field((z, y, x + 6), 4) * 900.0).astype(np.float32)
+    data[0, 0, :4] = [40000.0 * 0.05, -40001.0 * 0.05, 1640.0, -1640.0]
+    step = 0.05
+    src = to_dev(torch, data)
+    q = torch.zeros((z, y, x), dtype=torch.int16, device="cuda")
+    cc.util.quantizeToShort(q, src, x, y, z, step, x + 6, (x + 6) * y, qtype)
+    torch.cuda.synchronize()
+    want = o.quantize_to_short(np.ascontiguousarray(data[..., :x]), step, qtype=qtype)
+    assert np.array_equal(to_host(q), want)
+    rec = torch.zeros((z, y, x + 6), dtype=torch.float32, device="cuda")
+    cc.util.unquantizeFromShort(rec, q, x, y, z, step, x + 6, (x + 6) * y, qtype)
+    torch.cuda.synchronize()
+    assert np.array_equal(to_host(rec)[..., :x], o.unquantize_from_short(want, step, qtype=qtype))

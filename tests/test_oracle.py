@@ -421,3 +421,43 @@ def test_quantizer_closed_forms():
     s = O.symbolize(a)
     assert np.array_equal(np.sort(s), np.arange(65536, dtype=np.uint16))
     assert np.array_equal(O.unsymbolize(s), a)
+
+
+def test_quantize_to_short_closed_forms():
+    """quantizeToShort / unquantizeFromShort: the quantizer's int as a wrapped int16; agrees with
+    the symbol quantizer through the zig-zag map wherever the int fits."""
+    step = np.float32(0.5)
+    v = np.array([0.74, -0.76, 0.25, -0.25, 0.0, 1.5, -1.5, 0.2499, -0.2499], np.float32)
+    assert O.quantize_to_short(v, float(step), qtype=1).tolist() == [1, -2, 1, -1, 0, 3, -3, 0, 0]
+    assert O.quantize_to_short(v, float(step), qtype=0).tolist() == [1, -1, 0, 0, 0, 3, -3, 0, 0]
+    # the int result is stored as a short: 40000 -> -25536, -40001 -> 25535
+    w = np.array([40000.0, -40001.0, 32767.0, -32768.0], np.float32)
+    assert O.quantize_to_short(w, 1.0, qtype=1).tolist() == [-25536, 25535, 32767, -32768]
+    rng = np.random.default_rng(21)
+    x = (rng.standard_normal(20000) * 300).astype(np.float32)
+    for qtype in (0, 1):
+        q = O.quantize_to_short(x, 0.37, qtype=qtype)
+        assert np.array_equal(O.symbolize(q).astype(np.uint32), O.quantize(x, 0.37, qtype=qtype, dtype=np.uint32))
+        r = O.unquantize_from_short(q, 0.37, qtype=qtype)
+        assert np.array_equal(r, O.unquantize(O.symbolize(q), 0.37, qtype=qtype))
+    # uniform reconstruction is the bin centre: q * step
+    q = np.arange(-300, 300, dtype=np.int16)
+    assert np.array_equal(O.unquantize_from_short(q, 0.25, qtype=1), (q.astype(np.float32) * np.float32(0.25)))
+
+
+@pytest.mark.parametrize("levels,rle_only", [(1, False), (2, False), (2, True), (3, True)])
+def test_quant_first_pipeline_reconstructs_the_quantized_volume(levels, rle_only):
+    """The QuantFirst codec loses only in the quantizer (5/3 lifting and the entropy coder are
+    exact), so its round trip equals unquantize(quantize(volume))."""
+    from oracle import pipeline as P
+    rng = np.random.default_rng(levels)
+    vol = np.cumsum(rng.standard_normal((16, 24, 32)), axis=2).astype(np.float32)
+    step = 0.05
+    w = P.compress_volume_float_quant_first(vol, levels, step, rle_only_lvl0=rle_only)
+    got = P.decompress_volume_float_quant_first(w, vol.shape, levels, step, rle_only_lvl0=rle_only)
+    want = O.unquantize_from_short(O.quantize_to_short(vol, step, qtype=1), step, qtype=1)
+    assert np.array_equal(got, want)
+    # the RLE-only-on-level-0 flag changes the bitstream above level 0 only
+    if levels > 1:
+        other = P.compress_volume_float_quant_first(vol, levels, step, rle_only_lvl0=not rle_only)
+        assert not np.array_equal(other, w)
