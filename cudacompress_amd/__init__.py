@@ -36,6 +36,8 @@ from .api import (  # noqa: E402
     decompressBricksFloat,
     decompressVolumeFloat,
     decompressVolumeFloatQuantFirst,
+    decompressVolumeFloatMultiChannel,
+    decompressVolumeFloatQuantFirstMultiChannel,
     decompressVolumeLossless,
     destroyInstance,
     encodeDevice,

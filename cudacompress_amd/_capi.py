@@ -61,6 +61,10 @@ SIGNATURES = {
                                                      P_u64]),
     "cc_decompress_volume_float_quant_first": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, c_vp, c_u64, c_float,
                                                        c_int]),
+    "cc_decompress_volume_float_multichannel": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, c_u32, c_int,
+                                                        c_int]),
+    "cc_decompress_volume_float_quant_first_multichannel": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, c_u32,
+                                                                    c_int]),
     "cc_compress_volume_lossless": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, PP_u32, P_u64]),
     "cc_decompress_volume_lossless": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, c_vp, c_u64]),
     "cc_compress_bricks_float": (c_int, [c_vp, c_vp, c_u32, c_u32, c_int, c_vp, c_u64, P_u64, P_u32]),
@@ -91,6 +95,11 @@ class Brick(ctypes.Structure):
     """struct cc_brick of include/cudacompress_amd.h"""
     _fields_ = [("src", c_vp), ("dst", c_vp), ("sizeX", c_u32), ("sizeY", c_u32), ("sizeZ", c_u32),
                 ("quantStep", c_float)]
+
+
+class VolumeChannel(ctypes.Structure):
+    """struct cc_volume_channel of include/cudacompress_amd.h"""
+    _fields_ = [("d_image", c_vp), ("bits", c_vp), ("bitCount", c_u64), ("quantStep", c_float)]
 
 
 def profile_entries():

@@ -737,6 +737,44 @@ def decompressVolumeFloatQuantFirst(inst, dpImage, sizeX, sizeY, sizeZ, numLevel
            "decompressVolumeFloatQuantFirst")
 
 
+def _channels(channels):
+    """(dpImage, bits, quantStep) triples -> a cc_volume_channel array and the word arrays it points
+    into (kept alive by the caller for the duration of the call)."""
+    keep = [np.ascontiguousarray(b, dtype=np.uint32) for _, b, _ in channels]
+    arr = (_capi.VolumeChannel * len(channels))()
+    for i, ((img, _, q), w) in enumerate(zip(channels, keep)):
+        arr[i] = _capi.VolumeChannel(_ptr(img), w.ctypes.data, w.size * 32, q)
+    return arr, keep
+
+
+def decompressVolumeFloatMultiChannel(inst, channels, sizeX, sizeY, sizeZ, numLevels, doRLEOnlyOnLvl0=False,
+                                      symbolSize=2):
+    """decompressVolumeFloatMultiChannel (src/examples/CompressVolume.cpp:518); `channels` holds
+    (dpImage, bits, quantizationStepLevel0) per channel, as VolumeChannel does
+    (src/examples/CompressVolume.h:36)."""
+    if not channels:
+        raise ValueError("decompressVolumeFloatMultiChannel: no channels")
+    arr, keep = _channels(channels)
+    _check(_capi.load().cc_decompress_volume_float_multichannel(inst.handle, ctypes.addressof(arr), len(channels),
+                                                                sizeX, sizeY, sizeZ, numLevels, int(doRLEOnlyOnLvl0),
+                                                                symbolSize),
+           "decompressVolumeFloatMultiChannel")
+    del keep
+
+
+def decompressVolumeFloatQuantFirstMultiChannel(inst, channels, sizeX, sizeY, sizeZ, numLevels,
+                                                doRLEOnlyOnLvl0=False):
+    """decompressVolumeFloatQuantFirstMultiChannel (src/examples/CompressVolume.cpp:863); `channels`
+    holds (dpImage, bits, quantizationStep) per channel."""
+    if not channels:
+        raise ValueError("decompressVolumeFloatQuantFirstMultiChannel: no channels")
+    arr, keep = _channels(channels)
+    _check(_capi.load().cc_decompress_volume_float_quant_first_multichannel(
+        inst.handle, ctypes.addressof(arr), len(channels), sizeX, sizeY, sizeZ, numLevels, int(doRLEOnlyOnLvl0)),
+        "decompressVolumeFloatQuantFirstMultiChannel")
+    del keep
+
+
 def compressVolumeLossless(inst, dpImage, sizeX, sizeY, sizeZ, numLevels):
     """compressVolumeLossless (src/examples/CompressVolume.cpp:234); sizeZ == 1 for 2D fields."""
     lib = _capi.load()

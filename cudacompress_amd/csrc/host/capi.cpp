@@ -457,6 +457,34 @@ cc_status cc_decompress_volume_float_quant_first(cc_instance* inst, float* d_ima
                                                  quantizationStep, doRLEOnlyOnLvl0 != 0);
     });
 }
+namespace {
+std::vector<cca::VolumeChannelIn> channels_in(const cc_volume_channel* ch, uint32_t n) {
+    std::vector<cca::VolumeChannelIn> v(n);
+    for (uint32_t c = 0; c < n; c++) v[c] = {ch[c].d_image, ch[c].bits, ch[c].bitCount, ch[c].quantStep};
+    return v;
+}
+}  // namespace
+cc_status cc_decompress_volume_float_multichannel(cc_instance* inst, const cc_volume_channel* channels,
+                                                  uint32_t channelCount, uint32_t sizeX, uint32_t sizeY, uint32_t sizeZ,
+                                                  uint32_t numLevels, int doRLEOnlyOnLvl0, int symbolSize) {
+    return guarded([&] {
+        CCA_REQUIRE(inst && channels && channelCount, CC_ERR_INVALID_ARG, "null argument");
+        auto v = channels_in(channels, channelCount);
+        cca::decompress_volume_float_multichannel(inst, v.data(), channelCount, sizeX, sizeY, sizeZ, numLevels,
+                                                  doRLEOnlyOnLvl0 != 0, symbolSize);
+    });
+}
+cc_status cc_decompress_volume_float_quant_first_multichannel(cc_instance* inst, const cc_volume_channel* channels,
+                                                              uint32_t channelCount, uint32_t sizeX, uint32_t sizeY,
+                                                              uint32_t sizeZ, uint32_t numLevels,
+                                                              int doRLEOnlyOnLvl0) {
+    return guarded([&] {
+        CCA_REQUIRE(inst && channels && channelCount, CC_ERR_INVALID_ARG, "null argument");
+        auto v = channels_in(channels, channelCount);
+        cca::decompress_volume_float_quant_first_multichannel(inst, v.data(), channelCount, sizeX, sizeY, sizeZ,
+                                                              numLevels, doRLEOnlyOnLvl0 != 0);
+    });
+}
 cc_status cc_compress_volume_lossless(cc_instance* inst, const int16_t* d_image, uint32_t sizeX, uint32_t sizeY,
                                       uint32_t sizeZ, uint32_t numLevels, const uint32_t** outWords,
                                       uint64_t* outBits) {

@@ -1,8 +1,10 @@
 // Volume codecs built on the library API, restating the caller layouts of
 // src/examples/CompressVolume.cpp (compressVolumeFloat :313-409, decompressVolumeFloat :412-516,
 // compressVolumeLossless :234-277, decompressVolumeLossless :279-309,
-// compressVolumeFloatQuantFirst :651-757, decompressVolumeFloatQuantFirst :759-856) and, for 2D
-// lossless input, the three-band layout of src/examples/CompressHeightfield.cpp:116-126.
+// compressVolumeFloatQuantFirst :651-757, decompressVolumeFloatQuantFirst :759-856, the
+// multi-channel decoders decompressVolumeFloatMultiChannel :518-648 and
+// decompressVolumeFloatQuantFirstMultiChannel :863-985) and, for 2D lossless input, the three-band
+// layout of src/examples/CompressHeightfield.cpp:116-126.
 //
 // Differences from the reference callers, none of which changes the bitstream:
 //  * every kernel runs on the instance stream (the reference quantizes on the default stream);
@@ -11,7 +13,9 @@
 //    itself only handles 3D (it would compute a zero blockSizeZ);
 //  * decompressVolumeFloatQuantFirst starts its inverse transforms from the buffer holding the
 //    coarsest level's coefficients; the reference starts from buffer numLevels % 2, the same
-//    buffer for an even level count (an odd count reads a buffer its decode did not fill).
+//    buffer for an even level count (an odd count reads a buffer its decode did not fill); its
+//    multi-channel form also carries that index from one channel to the next;
+//  * the multi-channel decoders give each channel exactly its single-channel decode.
 #include "volume.hpp"
 
 #include <cstring>
@@ -339,6 +343,126 @@ void decompress_volume_float_quant_first(cc_instance* I, float* image, uint32_t 
         d = 1 - d;
     }
     unquantize_from_short(image, dwt[d], sx, sy, sz, step, sx, sx * sy, 1, st);
+}
+
+namespace {
+
+void check_channels(cc_instance* I, const VolumeChannelIn* ch, uint32_t nch, uint32_t sx, uint32_t sy, uint32_t sz,
+                    uint32_t levels) {
+    CCA_REQUIRE(nch >= 1 && ch, CC_ERR_INVALID_ARG, "no channels");
+    CCA_REQUIRE(levels >= 1 && sz > 1, CC_ERR_INVALID_ARG, "bad volume parameters");
+    const uint32_t f_all = 1u << levels;
+    CCA_REQUIRE(sx % f_all == 0 && sy % f_all == 0 && sz % f_all == 0, CC_ERR_INVALID_ARG,
+                "sizes must be divisible by 2^numLevels");
+    for (uint32_t c = 0; c < nch; c++)
+        CCA_REQUIRE(ch[c].image && ch[c].words, CC_ERR_INVALID_ARG, "null channel buffer");
+}
+
+// Bitstream cursors of the channels; stream s of a call with `per` streams per channel reads
+// channel s / per.
+struct ChannelStreams {
+    std::vector<const uint32_t*> words;
+    std::vector<uint64_t> nbits, pos;
+    std::vector<uint32_t> s2b_lp, s2b_hp;
+    ChannelStreams(const VolumeChannelIn* ch, uint32_t nch, uint32_t per) : nbits(nch), pos(nch, 0) {
+        for (uint32_t c = 0; c < nch; c++) {
+            words.push_back(ch[c].words);
+            nbits[c] = ch[c].nbits;
+            s2b_lp.push_back(c);
+            for (uint32_t b = 0; b < per; b++) s2b_hp.push_back(c);
+        }
+    }
+};
+
+}  // namespace
+
+void decompress_volume_float_multichannel(cc_instance* I, const VolumeChannelIn* ch, uint32_t nch, uint32_t sx,
+                                          uint32_t sy, uint32_t sz, uint32_t levels, bool rle_only_lvl0,
+                                          int symSize) {
+    DeviceGuard guard(I->device);
+    hipStream_t st = I->stream;
+    check_channels(I, ch, nch, sx, sy, sz, levels);
+    const uint64_t elems = (uint64_t)sx * sy * sz;
+    const uint32_t nb = 7, ns = nb * nch;
+    float* dwt[2] = {I->d_vol[0].as<float>(elems), I->d_vol[1].as<float>(elems)};
+    const uint64_t per_block_max = elems / 8;
+    unsigned char* sym = (unsigned char*)I->d_vol[2].get((uint64_t)ns * per_block_max * symSize);
+    std::vector<void*> streams(ns);
+    for (uint32_t i = 0; i < ns; i++) streams[i] = sym + i * per_block_max * symSize;
+    // device table of every channel's 7 band pointers for the fused dequantize + IDWT
+    void** hptr = I->h_vol_ptrs.as<void*>(ns);
+    for (uint32_t i = 0; i < ns; i++) hptr[i] = streams[i];
+    void** dptr = I->d_vol_ptrs.as<void*>(ns);
+    CCA_HIP(hipMemcpyAsync(dptr, hptr, sizeof(void*) * ns, hipMemcpyHostToDevice, st));
+    ChannelStreams cs(ch, nch, nb);
+    uint32_t f = 1u << levels;
+    // every channel's lowpass band in one decode (CompressVolume.cpp:588-599)
+    decode_streams(I, false, symSize, cs.words.data(), cs.nbits.data(), cs.pos.data(), nch, cs.s2b_lp.data(),
+                   streams.data(), nch, (uint32_t)(elems / ((uint64_t)f * f * f)));
+    for (uint32_t c = 0; c < nch; c++)
+        unquantize_from_symbols(ch[c].image, streams[c], symSize, sx / f, sy / f, sz / f, ch[c].step / float(f / 2),
+                                sx, sx * sy, 0, st);
+    // per level every channel's 7 bands in one decode, then each channel's IDWT (:602-641)
+    for (int level = (int)levels - 1; level >= 0; level--) {
+        const uint32_t count = (sx / f) * (sy / f) * (sz / f);
+        decode_streams(I, !(level > 0 && rle_only_lvl0), symSize, cs.words.data(), cs.nbits.data(), cs.pos.data(), nch,
+                       cs.s2b_hp.data(), streams.data(), ns, count);
+        f /= 2;
+        for (uint32_t c = 0; c < nch; c++)
+            dwt97_inverse_from_symbols(ch[c].image, dwt[1], dwt[0], ch[c].image, (const void* const*)(dptr + nb * c),
+                                       symSize, ch[c].step / float(f), (int)(sx / f), (int)(sy / f), (int)(sz / f), 1,
+                                       (int)sx, (int)(sx * sy), (int)sx, (int)(sx * sy), st);
+    }
+    // the pointer table upload must finish before the pinned copy is rewritten
+    CCA_HIP(hipStreamSynchronize(st));
+}
+
+void decompress_volume_float_quant_first_multichannel(cc_instance* I, const VolumeChannelIn* ch, uint32_t nch,
+                                                      uint32_t sx, uint32_t sy, uint32_t sz, uint32_t levels,
+                                                      bool rle_only_lvl0) {
+    DeviceGuard guard(I->device);
+    hipStream_t st = I->stream;
+    CCA_REQUIRE(I->log2 <= 16, CC_ERR_INVALID_ARG, "QuantFirst volumes use Symbol16 (log2 <= 16)");
+    check_channels(I, ch, nch, sx, sy, sz, levels);
+    const uint64_t elems = (uint64_t)sx * sy * sz;
+    const uint32_t nb = 7, ns = nb * nch;
+    // two coefficient buffers per channel (:880-883)
+    int16_t* b0 = I->d_vol[0].as<int16_t>(elems * nch);
+    int16_t* b1 = I->d_vol[1].as<int16_t>(elems * nch);
+    auto buf = [&](uint32_t c, int k) { return (k == 0 ? b0 : b1) + elems * c; };
+    const uint64_t per_block_max = elems / 8;
+    uint16_t* sym = I->d_vol[2].as<uint16_t>((uint64_t)ns * per_block_max);
+    std::vector<void*> streams(ns);
+    for (uint32_t i = 0; i < ns; i++) streams[i] = sym + i * per_block_max;
+    ChannelStreams cs(ch, nch, nb);
+    uint32_t f = 1u << levels;
+    decode_streams(I, false, 2, cs.words.data(), cs.nbits.data(), cs.pos.data(), nch, cs.s2b_lp.data(), streams.data(),
+                   nch, (uint32_t)(elems / ((uint64_t)f * f * f)));
+    for (uint32_t c = 0; c < nch; c++) unsymbolize(buf(c, 0), (const uint16_t*)streams[c], sx / f, sy / f, sz / f, sx, sx * sy, st);
+    int d = 0;
+    for (int level = (int)levels - 1; level >= 0; level--) {
+        const uint32_t bx = sx / f, by = sy / f, bz = sz / f;
+        decode_streams(I, !(level > 0 && rle_only_lvl0), 2, cs.words.data(), cs.nbits.data(), cs.pos.data(), nch,
+                       cs.s2b_hp.data(), streams.data(), ns, bx * by * bz);
+        f /= 2;
+        for (uint32_t c = 0; c < nch; c++)
+            for (uint32_t b = 1; b <= nb; b++) {
+                const Band bd = band_of(b, true);
+                const uint64_t off = bd.x * bx + (uint64_t)bd.y * by * sx + (uint64_t)bd.z * bz * sx * sy;
+                unsymbolize(buf(c, d) + off, (const uint16_t*)streams[nb * c + b - 1], bx, by, bz, sx, sx * sy, st);
+            }
+        d = 1 - d;
+    }
+    for (uint32_t c = 0; c < nch; c++) {
+        int dc = 0;  // each channel's coarsest level is in its first buffer
+        for (int level = (int)levels - 1; level >= 0; level--) {
+            const uint32_t fl = 1u << level;
+            dwt53_inverse(buf(c, 1 - dc), buf(c, dc), buf(c, dc), (int)(sx / fl), (int)(sy / fl), (int)(sz / fl),
+                          (int)sx, (int)(sx * sy), (int)sx, (int)(sx * sy), st);
+            dc = 1 - dc;
+        }
+        unquantize_from_short(ch[c].image, buf(c, dc), sx, sy, sz, ch[c].step, sx, sx * sy, 1, st);
+    }
 }
 
 }  // namespace cca

@@ -222,6 +222,30 @@ CC_API cc_status cc_decompress_volume_float_quant_first(cc_instance* inst, float
                                                         uint32_t sizeY, uint32_t sizeZ, uint32_t numLevels,
                                                         const uint32_t* bits, uint64_t bitCount,
                                                         float quantizationStep, int doRLEOnlyOnLvl0);
+/* One channel of a multi-channel decode: VolumeChannel, src/examples/CompressVolume.h:36-42
+ * (bitCount widened to 64 bits).  quantStep is quantizationStepLevel0 for the float codec and the
+ * uniform step for QuantFirst. */
+typedef struct cc_volume_channel {
+    float* d_image;        /* device, dense sizeX*sizeY*sizeZ output */
+    const uint32_t* bits;  /* host bitstream of this channel */
+    uint64_t bitCount;
+    float quantStep;
+} cc_volume_channel;
+/* decompressVolumeFloatMultiChannel, src/examples/CompressVolume.cpp:518-648: one decode call per
+ * level over every channel's streams; channel c's image equals cc_decompress_volume_float of its
+ * bitstream alone. */
+CC_API cc_status cc_decompress_volume_float_multichannel(cc_instance* inst, const cc_volume_channel* channels,
+                                                         uint32_t channelCount, uint32_t sizeX, uint32_t sizeY,
+                                                         uint32_t sizeZ, uint32_t numLevels, int doRLEOnlyOnLvl0,
+                                                         int symbolSize);
+/* decompressVolumeFloatQuantFirstMultiChannel, src/examples/CompressVolume.cpp:863-985; channel c's
+ * image equals cc_decompress_volume_float_quant_first of its bitstream alone (each channel's
+ * inverse transforms start from the buffer holding its coarsest level). */
+CC_API cc_status cc_decompress_volume_float_quant_first_multichannel(cc_instance* inst,
+                                                                     const cc_volume_channel* channels,
+                                                                     uint32_t channelCount, uint32_t sizeX,
+                                                                     uint32_t sizeY, uint32_t sizeZ,
+                                                                     uint32_t numLevels, int doRLEOnlyOnLvl0);
 
 /* ---- Brick batches ------------------------------------------------------------------------ */
 /* No single reference counterpart: a batch is compressVolumeFloat / decompressVolumeFloat

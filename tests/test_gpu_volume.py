@@ -1,4 +1,5 @@
-"""GPU parity of the volume codecs (compressVolumeFloat, compressVolumeFloatQuantFirst and
+"""GPU parity of the volume codecs (compressVolumeFloat, compressVolumeFloatQuantFirst, their
+multi-channel decoders and
 compressVolumeLossless callers of
 src/examples/CompressVolume.cpp) against the oracle composition in oracle/pipeline.py."""
 import numpy as np
@@ -82,6 +83,52 @@ def test_volume_float_quant_first(cc, torch_gpu, shape, levels, rle_lvl0):
     ref = P.decompress_volume_float_quant_first(want, shape, levels, step, rle_only_lvl0=rle_lvl0)
     log(f"   words={words.size} psnr={S.psnr(vol, rec):.2f}")
     assert np.array_equal(rec, ref)
+    inst.close()
+
+
+@pytest.mark.parametrize("shape,levels", [((32, 32, 32), 2), ((64, 48, 32), 3)], ids=["32^3L2", "64x48x32L3"])
+@pytest.mark.parametrize("rle_lvl0", [False, True], ids=["rle_all", "rle_lvl0"])
+def test_volume_float_multichannel(cc, torch_gpu, shape, levels, rle_lvl0):
+    """decompressVolumeFloatMultiChannel over three channels with their own steps: each channel
+    equals its single-channel decompression exactly and the oracle within 1e-6 relative."""
+    torch = torch_gpu
+    log("volume float multichannel", shape, levels, rle_lvl0)
+    steps = [0.01, 0.03, 0.02]
+    vols = [S.gaussian_field(shape, 11 + c) * (1.0 + c) for c in range(3)]
+    bits = [P.compress_volume_float(v, levels, q, rle_only_lvl0=rle_lvl0) for v, q in zip(vols, steps)]
+    inst = cc.createInstance(0, 7 * len(vols), int(np.prod(shape)) // 8, 128, 14)
+    outs = [torch.full(shape, 123.0, dtype=torch.float32, device="cuda") for _ in vols]
+    cc.decompressVolumeFloatMultiChannel(inst, list(zip(outs, bits, steps)), shape[2], shape[1], shape[0], levels,
+                                         rle_lvl0)
+    one = torch.full(shape, -7.0, dtype=torch.float32, device="cuda")
+    for c, (b, q) in enumerate(zip(bits, steps)):
+        cc.decompressVolumeFloat(inst, one, shape[2], shape[1], shape[0], levels, b, q, rle_lvl0)
+        torch.cuda.synchronize()
+        rec = to_host(outs[c])
+        assert np.array_equal(rec, to_host(one)), f"channel {c}"
+        ref = P.decompress_volume_float(b, shape, levels, q, rle_only_lvl0=rle_lvl0)
+        assert float(np.abs(rec - ref).max()) <= 1e-6 * max(1.0, float(np.abs(ref).max())), f"channel {c}"
+    inst.close()
+
+
+@pytest.mark.parametrize("shape,levels", [((32, 32, 32), 2), ((32, 64, 64), 3)], ids=["32^3L2", "32x64x64L3"])
+@pytest.mark.parametrize("rle_lvl0", [False, True], ids=["rle_all", "rle_lvl0"])
+def test_volume_float_quant_first_multichannel(cc, torch_gpu, shape, levels, rle_lvl0):
+    """decompressVolumeFloatQuantFirstMultiChannel over three channels: each channel equals the
+    oracle's decompression of its bitstream exactly."""
+    torch = torch_gpu
+    log("volume quant-first multichannel", shape, levels, rle_lvl0)
+    steps = [0.05, 0.2, 0.1]
+    vols = [S.gaussian_field(shape, 21 + c) * 40.0 for c in range(3)]
+    bits = [P.compress_volume_float_quant_first(v, levels, q, rle_only_lvl0=rle_lvl0) for v, q in zip(vols, steps)]
+    inst = cc.createInstance(0, 7 * len(vols), int(np.prod(shape)) // 8, 128, 14)
+    outs = [torch.full(shape, 123.0, dtype=torch.float32, device="cuda") for _ in vols]
+    cc.decompressVolumeFloatQuantFirstMultiChannel(inst, list(zip(outs, bits, steps)), shape[2], shape[1], shape[0],
+                                                   levels, rle_lvl0)
+    torch.cuda.synchronize()
+    for c, (b, q) in enumerate(zip(bits, steps)):
+        ref = P.decompress_volume_float_quant_first(b, shape, levels, q, rle_only_lvl0=rle_lvl0)
+        assert np.array_equal(to_host(outs[c]), ref), f"channel {c}"
     inst.close()
 
 
