@@ -37,6 +37,14 @@ from .api import (  # noqa: E402
     decompressVolumeFloat,
     decompressVolumeFloatQuantFirst,
     decompressVolumeFloatMultiChannel,
+    setTimingDetail,
+    getTimings,
+    printTimings,
+    resetTimings,
+    TIMING_DETAIL_NONE,
+    TIMING_DETAIL_LOW,
+    TIMING_DETAIL_MEDIUM,
+    TIMING_DETAIL_HIGH,
     decompressVolumeFloatQuantFirstMultiChannel,
     decompressVolumeLossless,
     destroyInstance,

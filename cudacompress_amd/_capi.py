@@ -65,6 +65,34 @@ SIGNATURES = {
                                                         c_int]),
     "cc_decompress_volume_float_quant_first_multichannel": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, c_u32,
                                                                     c_int]),
+    "cc_dwt_float_forward_from_byte": (c_int, [c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_int,
+                                               c_int, c_int, c_vp]),
+    "cc_dwt_float_inverse_to_byte": (c_int, [c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+                                             c_int, c_vp]),
+    "cc_dwt_float_inverse_from_symbols_to_byte": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_float, c_int, c_int, c_int,
+                                                          c_int, c_int, c_int, c_int, c_int, c_vp]),
+    "cc_dwt_float_forward_lowpass_only_from_byte": (c_int, [c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int, c_int,
+                                                            c_vp]),
+    "cc_dwt_int_forward_lowpass_only": (c_int, [c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int, c_vp]),
+    "cc_float_to_byte_2d": (c_int, [c_vp, c_u32, c_u32, c_vp, c_u32, c_u32, c_u32, c_vp]),
+    "cc_byte_to_float_2d": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, c_u32, c_vp]),
+    "cc_quantize_difference_to_symbols_2d": (c_int, [c_vp, c_vp, c_float, c_vp, c_u32, c_u32, c_u32, c_u32, c_u32,
+                                                     c_int, c_vp]),
+    "cc_quantize_difference_to_symbols_roundtrip_2d": (c_int, [c_vp, c_vp, c_float, c_vp, c_u32, c_u32, c_u32, c_u32,
+                                                               c_u32, c_int, c_vp]),
+    "cc_unquantize_difference_from_symbols_2d": (c_int, [c_vp, c_vp, c_float, c_vp, c_u32, c_u32, c_u32, c_u32,
+                                                         c_u32, c_int, c_vp]),
+    "cc_convert_rgb_to_ycocg": (c_int, [c_vp, c_vp, c_int, c_int, c_vp]),
+    "cc_convert_ycocg_to_rgb": (c_int, [c_vp, c_vp, c_int, c_int, c_vp]),
+    "cc_get_lsb": (c_int, [c_vp, c_vp, c_u32, c_vp]),
+    "cc_remove_lsb": (c_int, [c_vp, c_vp, c_u32, ctypes.c_int16, c_vp]),
+    "cc_append_lsb": (c_int, [c_vp, c_vp, c_u32, ctypes.c_int16, c_vp]),
+    "cc_shift_elems": (c_int, [c_vp, c_u32, ctypes.c_int16, c_vp]),
+    "cc_set_timing_detail": (c_int, [c_vp, c_int]),
+    "cc_get_timings": (c_int, [c_vp, P_u32]),
+    "cc_get_timing_entry": (c_int, [c_vp, c_u32, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(c_float)]),
+    "cc_print_timings": (c_int, [c_vp]),
+    "cc_reset_timings": (c_int, [c_vp]),
     "cc_compress_volume_lossless": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, PP_u32, P_u64]),
     "cc_decompress_volume_lossless": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, c_vp, c_u64]),
     "cc_compress_bricks_float": (c_int, [c_vp, c_vp, c_u32, c_u32, c_int, c_vp, c_u64, P_u64, P_u32]),

@@ -569,7 +569,7 @@ def decodeHuff(inst, bitStream, symbolStreams, symbolCountPerStream):
 
 # ------------------------------------------------------------------------------------ util
 class util:
-    """cudaCompress::util (include/cudaCompress/util/DWT.h, Quantize.h)."""
+    """cudaCompress::util (include/cudaCompress/util/DWT.h, Quantize.h, YCoCg.h, Bits.h)."""
 
     @staticmethod
     def dwtFloat3DForward(dpDest, dpBuffer2, dpBuffer1, dpSource, sizeX, sizeY, sizeZ, srcChannelCount=1,
@@ -681,6 +681,156 @@ class util:
     @staticmethod
     def getMaxAbs(inst, dpImage, elemCount, dpValMax):
         _check(_capi.load().cc_get_max_abs(inst.handle, _ptr(dpImage), elemCount, _ptr(dpValMax)), "getMaxAbs")
+
+    # ---- image path (DWT.h:81-131, Quantize.h:25-61, YCoCg.h) and bit planes (Bits.h) ----
+    @staticmethod
+    def dwtFloat3DForwardFromByte(dpDest, dpBuffer2, dpBuffer1, dpSource, sizeX, sizeY, sizeZ, srcChannelCount=1,
+                                  dstRowPitch=0, dstSlicePitch=0, srcRowPitch=0, srcSlicePitch=0, stream=None):
+        _check(_capi.load().cc_dwt_float_forward_from_byte(
+            _ptr(dpDest), _ptr(dpBuffer2), _ptr(dpBuffer1), _ptr(dpSource), sizeX, sizeY, sizeZ, srcChannelCount,
+            dstRowPitch, dstSlicePitch, srcRowPitch, srcSlicePitch, _stream(stream)), "dwtFloat3DForwardFromByte")
+
+    @staticmethod
+    def dwtFloat2DForwardFromByte(dpDest, dpBuffer, dpSource, sizeX, sizeY, srcChannelCount=1, dstRowPitch=0,
+                                  srcRowPitch=0, stream=None):
+        _check(_capi.load().cc_dwt_float_forward_from_byte(
+            _ptr(dpDest), None, _ptr(dpBuffer), _ptr(dpSource), sizeX, sizeY, 1, srcChannelCount, dstRowPitch, 0,
+            srcRowPitch, 0, _stream(stream)), "dwtFloat2DForwardFromByte")
+
+    @staticmethod
+    def dwtFloat3DInverseToByte(dpDest, dpBuffer2, dpBuffer1, dpSource, sizeX, sizeY, sizeZ, dstChannelCount=1,
+                                dstRowPitch=0, dstSlicePitch=0, srcRowPitch=0, srcSlicePitch=0, stream=None):
+        _check(_capi.load().cc_dwt_float_inverse_to_byte(
+            _ptr(dpDest), _ptr(dpBuffer2), _ptr(dpBuffer1), _ptr(dpSource), sizeX, sizeY, sizeZ, dstChannelCount,
+            dstRowPitch, dstSlicePitch, srcRowPitch, srcSlicePitch, _stream(stream)), "dwtFloat3DInverseToByte")
+
+    @staticmethod
+    def dwtFloat2DInverseToByte(dpDest, dpBuffer, dpSource, sizeX, sizeY, dstChannelCount=1, dstRowPitch=0,
+                                srcRowPitch=0, stream=None):
+        _check(_capi.load().cc_dwt_float_inverse_to_byte(
+            _ptr(dpDest), None, _ptr(dpBuffer), _ptr(dpSource), sizeX, sizeY, 1, dstChannelCount, dstRowPitch, 0,
+            srcRowPitch, 0, _stream(stream)), "dwtFloat2DInverseToByte")
+
+    @staticmethod
+    def dwtFloat3DInverseFromSymbolsToByte(dpDest, dpBuffer2, dpBuffer1, dpLowpass, dppHighpass, quantStep, sizeX,
+                                           sizeY, sizeZ, dstChannelCount=1, dstRowPitch=0, dstSlicePitch=0,
+                                           lowpassRowPitch=0, lowpassSlicePitch=0, stream=None):
+        _check(_capi.load().cc_dwt_float_inverse_from_symbols_to_byte(
+            _ptr(dpDest), _ptr(dpBuffer2), _ptr(dpBuffer1), _ptr(dpLowpass), _ptr(dppHighpass), quantStep, sizeX,
+            sizeY, sizeZ, dstChannelCount, dstRowPitch, dstSlicePitch, lowpassRowPitch, lowpassSlicePitch,
+            _stream(stream)), "dwtFloat3DInverseFromSymbolsToByte")
+
+    @staticmethod
+    def dwtFloat2DInverseFromSymbolsToByte(dpDest, dpBuffer, dpLowpass, dppHighpass, quantStep, sizeX, sizeY,
+                                           dstChannelCount=1, dstRowPitch=0, lowpassRowPitch=0, stream=None):
+        _check(_capi.load().cc_dwt_float_inverse_from_symbols_to_byte(
+            _ptr(dpDest), None, _ptr(dpBuffer), _ptr(dpLowpass), _ptr(dppHighpass), quantStep, sizeX, sizeY, 1,
+            dstChannelCount, dstRowPitch, 0, lowpassRowPitch, 0, _stream(stream)),
+            "dwtFloat2DInverseFromSymbolsToByte")
+
+    @staticmethod
+    def dwtFloat2DForwardLowpassOnlyFromByte(dpDest, dpBuffer, dpSource, sizeX, sizeY, srcChannelCount=1,
+                                             dstRowPitch=0, srcRowPitch=0, stream=None):
+        _check(_capi.load().cc_dwt_float_forward_lowpass_only_from_byte(
+            _ptr(dpDest), _ptr(dpBuffer), _ptr(dpSource), sizeX, sizeY, srcChannelCount, dstRowPitch, srcRowPitch,
+            _stream(stream)), "dwtFloat2DForwardLowpassOnlyFromByte")
+
+    @staticmethod
+    def dwtIntForwardLowpassOnly(dpDest, dpBuffer, dpSource, sizeX, sizeY, dstRowPitch=0, srcRowPitch=0,
+                                 stream=None):
+        _check(_capi.load().cc_dwt_int_forward_lowpass_only(_ptr(dpDest), _ptr(dpBuffer), _ptr(dpSource), sizeX,
+                                                            sizeY, dstRowPitch, srcRowPitch, _stream(stream)),
+               "dwtIntForwardLowpassOnly")
+
+    @staticmethod
+    def floatToByte2D(dpQuant, channelCount, channel, dpData, sizeX, sizeY, rowPitchSrc=0, stream=None):
+        _check(_capi.load().cc_float_to_byte_2d(_ptr(dpQuant), channelCount, channel, _ptr(dpData), sizeX, sizeY,
+                                                rowPitchSrc, _stream(stream)), "floatToByte2D")
+
+    @staticmethod
+    def byteToFloat2D(dpData, dpQuant, channelCount, channel, sizeX, sizeY, rowPitchDst=0, stream=None):
+        _check(_capi.load().cc_byte_to_float_2d(_ptr(dpData), _ptr(dpQuant), channelCount, channel, sizeX, sizeY,
+                                                rowPitchDst, _stream(stream)), "byteToFloat2D")
+
+    @staticmethod
+    def quantizeDifferenceToSymbols2D(dpSymbols, dpData, quantizationStep, dpReference, channelCount, channel, sizeX,
+                                      sizeY, rowPitchSrc=0, quantType=QUANTIZE_DEADZONE, stream=None):
+        _check(_capi.load().cc_quantize_difference_to_symbols_2d(
+            _ptr(dpSymbols), _ptr(dpData), quantizationStep, _ptr(dpReference), channelCount, channel, sizeX, sizeY,
+            rowPitchSrc, quantType, _stream(stream)), "quantizeDifferenceToSymbols2D")
+
+    @staticmethod
+    def quantizeDifferenceToSymbolsRoundtrip2D(dpSymbols, dpData, quantizationStep, dpReference, channelCount,
+                                               channel, sizeX, sizeY, rowPitchSrc=0, quantType=QUANTIZE_DEADZONE,
+                                               stream=None):
+        _check(_capi.load().cc_quantize_difference_to_symbols_roundtrip_2d(
+            _ptr(dpSymbols), _ptr(dpData), quantizationStep, _ptr(dpReference), channelCount, channel, sizeX, sizeY,
+            rowPitchSrc, quantType, _stream(stream)), "quantizeDifferenceToSymbolsRoundtrip2D")
+
+    @staticmethod
+    def unquantizeDifferenceFromSymbols2D(dpData, dpSymbols, quantizationStep, dpReference, channelCount, channel,
+                                          sizeX, sizeY, rowPitchDst=0, quantType=QUANTIZE_DEADZONE, stream=None):
+        _check(_capi.load().cc_unquantize_difference_from_symbols_2d(
+            _ptr(dpData), _ptr(dpSymbols), quantizationStep, _ptr(dpReference), channelCount, channel, sizeX, sizeY,
+            rowPitchDst, quantType, _stream(stream)), "unquantizeDifferenceFromSymbols2D")
+
+    @staticmethod
+    def convertRGBtoYCoCg(dpTarget, dpData, pixelCount, channels=3, stream=None):
+        _check(_capi.load().cc_convert_rgb_to_ycocg(_ptr(dpTarget), _ptr(dpData), pixelCount, channels,
+                                                    _stream(stream)), "convertRGBtoYCoCg")
+
+    @staticmethod
+    def convertYCoCgtoRGB(dpTarget, dpData, pixelCount, channels=3, stream=None):
+        _check(_capi.load().cc_convert_ycocg_to_rgb(_ptr(dpTarget), _ptr(dpData), pixelCount, channels,
+                                                    _stream(stream)), "convertYCoCgtoRGB")
+
+    @staticmethod
+    def getLSB(dpData, dpBits, elemCount, stream=None):
+        _check(_capi.load().cc_get_lsb(_ptr(dpData), _ptr(dpBits), elemCount, _stream(stream)), "getLSB")
+
+    @staticmethod
+    def removeLSB(dpData, dpBits, elemCount, shift=0, stream=None):
+        _check(_capi.load().cc_remove_lsb(_ptr(dpData), _ptr(dpBits), elemCount, shift, _stream(stream)), "removeLSB")
+
+    @staticmethod
+    def appendLSB(dpData, dpBits, elemCount, shift=0, stream=None):
+        _check(_capi.load().cc_append_lsb(_ptr(dpData), _ptr(dpBits), elemCount, shift, _stream(stream)), "appendLSB")
+
+    @staticmethod
+    def shiftElems(dpData, elemCount, shift, stream=None):
+        _check(_capi.load().cc_shift_elems(_ptr(dpData), elemCount, shift, _stream(stream)), "shiftElems")
+
+
+# ------------------------------------------------------------------------------------ timing
+TIMING_DETAIL_NONE, TIMING_DETAIL_LOW, TIMING_DETAIL_MEDIUM, TIMING_DETAIL_HIGH = 0, 1, 2, 3
+
+
+def setTimingDetail(inst, detail):
+    """setTimingDetail (include/cudaCompress/Timing.h:23)."""
+    _check(_capi.load().cc_set_timing_detail(inst.handle, int(detail)), "setTimingDetail")
+
+
+def getTimings(inst):
+    """getTimings (include/cudaCompress/Timing.h:25): (names, times in ms)."""
+    lib = _capi.load()
+    n = ctypes.c_uint32()
+    _check(lib.cc_get_timings(inst.handle, ctypes.byref(n)), "getTimings")
+    names, times = [], []
+    for i in range(n.value):
+        name = ctypes.c_char_p()
+        ms = ctypes.c_float()
+        _check(lib.cc_get_timing_entry(inst.handle, i, ctypes.byref(name), ctypes.byref(ms)), "getTimings")
+        names.append(name.value.decode())
+        times.append(ms.value)
+    return names, times
+
+
+def printTimings(inst):
+    _check(_capi.load().cc_print_timings(inst.handle), "printTimings")
+
+
+def resetTimings(inst):
+    _check(_capi.load().cc_reset_timings(inst.handle), "resetTimings")
 
 
 # ------------------------------------------------------------------------------------ volume codecs

@@ -15,15 +15,32 @@
 #include "../kernels.hpp"
 #include "dwt_ops.hpp"
 
+#include <type_traits>
+
 namespace cca {
 namespace {
 
 constexpr int XP = 64;  // pairs per X-pass chunk (one per lane)
 constexpr int SP = 32;  // pairs per Y/Z-pass chunk
 
+// Byte images enter the first analysis pass as value - 128 and leave the last synthesis pass
+// rounded half up and clamped to [0, 255] (toFloat/fromFloat, src/cudaCompress/util/
+// DWTCommon.cuh:16-34).
+template <class T>
+__device__ __forceinline__ T load_in(T v) { return v; }
+__device__ __forceinline__ float load_in(uint8_t v) { return float(v) - 128.0f; }
+template <class TOut, class T>
+__device__ __forceinline__ TOut store_out(T v) {
+    if constexpr (std::is_same<TOut, uint8_t>::value)
+        return (uint8_t)fminf(fmaxf(v + 128.5f, 0.0f), 255.0f);
+    else
+        return v;
+}
+
 // ---- X pass (contiguous axis): one wave per row ------------------------------------------------
-template <class Op, int CH>
-__global__ __launch_bounds__(256) void fwd_x(typename Op::T* __restrict__ dst, const typename Op::T* __restrict__ src,
+// LO: only the lowpass half is written (the lowpass-only transforms).
+template <class Op, int CH, class TIn = typename Op::T, bool LO = false>
+__global__ __launch_bounds__(256) void fwd_x(typename Op::T* __restrict__ dst, const TIn* __restrict__ src,
                                              int n, int rows, int sy, int dRow, int dSlice, int sRow, int sSlice) {
     using T = typename Op::T;
     constexpr int W = 2 * XP + 2 * Op::H;
@@ -32,14 +49,14 @@ __global__ __launch_bounds__(256) void fwd_x(typename Op::T* __restrict__ dst, c
     const int row = blockIdx.x * 4 + wave;
     const bool active = row < rows;
     const int y = active ? row % sy : 0, z = active ? row / sy : 0;
-    const T* s = src + (long long)z * sSlice + (long long)y * sRow;
+    const TIn* s = src + (long long)z * sSlice + (long long)y * sRow;
     T* d = dst + (long long)z * dSlice + (long long)y * dRow;
     const int m = n / 2;
     for (int k0 = 0; k0 < m; k0 += XP) {
         if (active) {
             for (int i = lane; i < W; i += 64) {
                 int p = 2 * k0 - Op::H + i;
-                if (p < n + Op::H) lds[wave][i] = s[(long long)mirror(p, n) * CH];
+                if (p < n + Op::H) lds[wave][i] = load_in(s[(long long)mirror(p, n) * CH]);
             }
         }
         __syncthreads();
@@ -48,14 +65,14 @@ __global__ __launch_bounds__(256) void fwd_x(typename Op::T* __restrict__ dst, c
             T lo, hi;
             Op::pair(lds[wave], lane, lo, hi);
             d[k] = lo;
-            d[m + k] = hi;
+            if (!LO) d[m + k] = hi;
         }
         __syncthreads();
     }
 }
 
-template <class Op, int CH>
-__global__ __launch_bounds__(256) void inv_x(typename Op::T* __restrict__ dst, const typename Op::T* __restrict__ src,
+template <class Op, int CH, class TOut = typename Op::T>
+__global__ __launch_bounds__(256) void inv_x(TOut* __restrict__ dst, const typename Op::T* __restrict__ src,
                                              int n, int rows, int sy, int dRow, int dSlice, int sRow, int sSlice) {
     using T = typename Op::T;
     constexpr int W = 2 * (XP + 2 * Op::G);
@@ -65,7 +82,7 @@ __global__ __launch_bounds__(256) void inv_x(typename Op::T* __restrict__ dst, c
     const bool active = row < rows;
     const int y = active ? row % sy : 0, z = active ? row / sy : 0;
     const T* s = src + (long long)z * sSlice + (long long)y * sRow;
-    T* d = dst + (long long)z * dSlice + (long long)y * dRow;
+    TOut* d = dst + (long long)z * dSlice + (long long)y * dRow;
     const int m = n / 2;
     for (int k0 = 0; k0 < m; k0 += XP) {
         if (active) {
@@ -84,15 +101,15 @@ __global__ __launch_bounds__(256) void inv_x(typename Op::T* __restrict__ dst, c
                 for (int h = 0; h < 2; h++) {
                     int t = lane + 64 * h;
                     int j = 2 * k0 + t;
-                    if (j < n) d[(long long)j * CH] = Op::out(lds[wave] + t, t & 1);
+                    if (j < n) d[(long long)j * CH] = store_out<TOut>(Op::out(lds[wave] + t, t & 1));
                 }
             } else {
                 int k = k0 + lane;
                 if (k < m) {
                     T e, o;
                     Op::pair_out(lds[wave] + 2 * lane, e, o);
-                    d[(long long)(2 * k) * CH] = e;
-                    d[(long long)(2 * k + 1) * CH] = o;
+                    d[(long long)(2 * k) * CH] = store_out<TOut>(e);
+                    d[(long long)(2 * k + 1) * CH] = store_out<TOut>(o);
                 }
             }
         }
@@ -108,7 +125,7 @@ struct PlainFetch {
     }
 };
 
-template <class Op>
+template <class Op, bool LO = false>
 __global__ __launch_bounds__(256) void fwd_s(typename Op::T* __restrict__ dst, const typename Op::T* __restrict__ src,
                                              int sx, int n, long long aStride, long long oStride) {
     using T = typename Op::T;
@@ -136,7 +153,7 @@ __global__ __launch_bounds__(256) void fwd_s(typename Op::T* __restrict__ dst, c
             T lo, hi;
             Op::pair(w, 0, lo, hi);
             d[k * aStride] = lo;
-            d[(m + k) * aStride] = hi;
+            if (!LO) d[(m + k) * aStride] = hi;
         }
     }
 }
@@ -229,23 +246,128 @@ void check_dims(int sx, int sy, int sz) {
     CCA_REQUIRE(sz <= 65535, CC_ERR_INVALID_ARG, "sizeZ too large for the launch grid");
 }
 
-template <int CH>
-void launch_fwd97_x(float* dst, const float* src, int sx, int sy, int sz, int dRow, int dSlice, int sRow, int sSlice,
+template <int CH, bool LO = false, class TIn>
+void launch_fwd97_x(float* dst, const TIn* src, int sx, int sy, int sz, int dRow, int dSlice, int sRow, int sSlice,
                     hipStream_t st) {
-    CCA_LAUNCH("dwt97_fwd_x", (uint64_t)sx * sy * sz * 8, (fwd_x<Fwd97, CH>), x_grid(sy * sz), dim3(256), 0, st, dst, src, sx, sy * sz, sy, dRow,
-                       dSlice, sRow, sSlice);
+    const uint64_t bytes = (uint64_t)sx * sy * sz * (sizeof(TIn) + (LO ? 2 : 4));
+    CCA_LAUNCH("dwt97_fwd_x", bytes, (fwd_x<Fwd97, CH, TIn, LO>), x_grid(sy * sz), dim3(256), 0, st, dst, src, sx,
+               sy * sz, sy, dRow, dSlice, sRow, sSlice);
 }
-template <int CH>
-void launch_inv97_x(float* dst, const float* src, int sx, int sy, int sz, int dRow, int dSlice, int sRow, int sSlice,
+template <int CH, class TOut>
+void launch_inv97_x(TOut* dst, const float* src, int sx, int sy, int sz, int dRow, int dSlice, int sRow, int sSlice,
                     hipStream_t st) {
-    CCA_LAUNCH("dwt97_inv_x", (uint64_t)sx * sy * sz * 8, (inv_x<Inv97, CH>), x_grid(sy * sz), dim3(256), 0, st, dst, src, sx, sy * sz, sy, dRow,
-                       dSlice, sRow, sSlice);
+    CCA_LAUNCH("dwt97_inv_x", (uint64_t)sx * sy * sz * (4 + sizeof(TOut)), (inv_x<Inv97, CH, TOut>), x_grid(sy * sz),
+               dim3(256), 0, st, dst, src, sx, sy * sz, sy, dRow, dSlice, sRow, sSlice);
 }
+template <bool LO = false, class TIn>
+void launch_fwd97_x_ch(int ch, float* dst, const TIn* src, int sx, int sy, int sz, int dRow, int dSlice, int sRow,
+                       int sSlice, hipStream_t st) {
+    switch (ch) {
+        case 1: launch_fwd97_x<1, LO>(dst, src, sx, sy, sz, dRow, dSlice, sRow, sSlice, st); break;
+        case 2: launch_fwd97_x<2, LO>(dst, src, sx, sy, sz, dRow, dSlice, sRow, sSlice, st); break;
+        case 3: launch_fwd97_x<3, LO>(dst, src, sx, sy, sz, dRow, dSlice, sRow, sSlice, st); break;
+        default: launch_fwd97_x<4, LO>(dst, src, sx, sy, sz, dRow, dSlice, sRow, sSlice, st); break;
+    }
+}
+template <class TOut>
+void launch_inv97_x_ch(int ch, TOut* dst, const float* src, int sx, int sy, int sz, int dRow, int dSlice, int sRow,
+                       int sSlice, hipStream_t st) {
+    switch (ch) {
+        case 1: launch_inv97_x<1>(dst, src, sx, sy, sz, dRow, dSlice, sRow, sSlice, st); break;
+        case 2: launch_inv97_x<2>(dst, src, sx, sy, sz, dRow, dSlice, sRow, sSlice, st); break;
+        case 3: launch_inv97_x<3>(dst, src, sx, sy, sz, dRow, dSlice, sRow, sSlice, st); break;
+        default: launch_inv97_x<4>(dst, src, sx, sy, sz, dRow, dSlice, sRow, sSlice, st); break;
+    }
+}
+
+// dwtFloatForward<T> (src/cudaCompress/util/DWTFloat.cu:85-201) for float and byte sources.
+template <class TIn>
+void dwt97_forward_t(float* dst, float* buf2, float* buf1, const TIn* src, int sx, int sy, int sz, int ch, int dRow,
+                     int dSlice, int sRow, int sSlice, hipStream_t st);
+// dwtFloatInverse<T> (DWTFloat.cu:233-322) for float and byte destinations.
+template <class TOut>
+void dwt97_inverse_t(TOut* dst, float* buf2, float* buf1, const float* src, int sx, int sy, int sz, int ch, int dRow,
+                     int dSlice, int sRow, int sSlice, hipStream_t st);
+// dwtFloatInverseFromSymbols<T> (DWTFloat.cu:326-463) for float and byte destinations.
+template <class TOut>
+void dwt97_inverse_from_symbols_t(TOut* dst, float* buf2, float* buf1, const float* lowpass,
+                                  const void* const* dd_highpass, int symbolSize, float step, int sx, int sy, int sz,
+                                  int ch, int dRow, int dSlice, int lpRow, int lpSlice, hipStream_t st);
 
 }  // namespace
 
 void dwt97_forward(float* dst, float* buf2, float* buf1, const float* src, int sx, int sy, int sz, int ch, int dRow,
                    int dSlice, int sRow, int sSlice, hipStream_t st) {
+    dwt97_forward_t(dst, buf2, buf1, src, sx, sy, sz, ch, dRow, dSlice, sRow, sSlice, st);
+}
+void dwt97_forward_from_byte(float* dst, float* buf2, float* buf1, const uint8_t* src, int sx, int sy, int sz, int ch,
+                             int dRow, int dSlice, int sRow, int sSlice, hipStream_t st) {
+    dwt97_forward_t(dst, buf2, buf1, src, sx, sy, sz, ch, dRow, dSlice, sRow, sSlice, st);
+}
+void dwt97_inverse(float* dst, float* buf2, float* buf1, const float* src, int sx, int sy, int sz, int ch, int dRow,
+                   int dSlice, int sRow, int sSlice, hipStream_t st) {
+    dwt97_inverse_t(dst, buf2, buf1, src, sx, sy, sz, ch, dRow, dSlice, sRow, sSlice, st);
+}
+void dwt97_inverse_to_byte(uint8_t* dst, float* buf2, float* buf1, const float* src, int sx, int sy, int sz, int ch,
+                           int dRow, int dSlice, int sRow, int sSlice, hipStream_t st) {
+    dwt97_inverse_t(dst, buf2, buf1, src, sx, sy, sz, ch, dRow, dSlice, sRow, sSlice, st);
+}
+void dwt97_inverse_from_symbols(float* dst, float* buf2, float* buf1, const float* lowpass,
+                                const void* const* dd_highpass, int symbolSize, float step, int sx, int sy, int sz,
+                                int ch, int dRow, int dSlice, int lpRow, int lpSlice, hipStream_t st) {
+    dwt97_inverse_from_symbols_t(dst, buf2, buf1, lowpass, dd_highpass, symbolSize, step, sx, sy, sz, ch, dRow, dSlice,
+                                 lpRow, lpSlice, st);
+}
+void dwt97_inverse_from_symbols_to_byte(uint8_t* dst, float* buf2, float* buf1, const float* lowpass,
+                                        const void* const* dd_highpass, float step, int sx, int sy, int sz, int ch,
+                                        int dRow, int dSlice, int lpRow, int lpSlice, hipStream_t st) {
+    dwt97_inverse_from_symbols_t(dst, buf2, buf1, lowpass, dd_highpass, 2, step, sx, sy, sz, ch, dRow, dSlice, lpRow,
+                                 lpSlice, st);
+}
+
+// dwtFloat2DForwardLowpassOnlyFromByte (DWTFloat.cu:18-81, :535-552; kernels in
+// src/cudaCompress/util/DWTFloat2DLowpassKernels.cui): the X pass writes only the row lowpass
+// halves to buf (row pitch dRow), the Y pass only the column lowpass halves to dst.  The sums are
+// the same fmaf chains as the full transform, so dst equals its LL band.
+void dwt97_forward_lowpass_only_from_byte(float* dst, float* buf, const uint8_t* src, int sx, int sy, int ch,
+                                          int dRow, int sRow, hipStream_t st) {
+    check_dims(sx, sy, 1);
+    CCA_REQUIRE(ch >= 1 && ch <= 4, CC_ERR_INVALID_ARG, "srcChannelCount must be 1..4");
+    CCA_REQUIRE(dst && buf && src, CC_ERR_INVALID_ARG, "null DWT buffer");
+    if (dRow <= 0) dRow = sx / 2;
+    if (sRow <= 0) sRow = sx * ch;
+    CCA_REQUIRE(dRow >= sx / 2, CC_ERR_INVALID_ARG, "dstRowPitch must be >= sizeX / 2");
+    launch_fwd97_x_ch<true>(ch, buf, src, sx, sy, 1, dRow, 0, sRow, 0, st);
+    CCA_LAUNCH_CHECK();
+    CCA_LAUNCH("dwt97_fwd_y_lowpass", (uint64_t)sx / 2 * sy * 6, (fwd_s<Fwd97, true>), s_grid(sx / 2, sy, 1), dim3(256),
+               0, st, dst, buf, sx / 2, sy, (long long)dRow, 0LL);
+    CCA_LAUNCH_CHECK();
+}
+
+// dwtIntForwardLowpassOnly (src/cudaCompress/util/DWTInt.cu:16-76, kernels in
+// DWTInt2DLowpassKernels.cui): the LL band of the 2D 5/3 transform.  The reference launches
+// only whole 512-column and 64-row tiles; every even size is computed here, identical where the
+// reference computes.
+void dwt53_forward_lowpass_only(int16_t* dst, int16_t* buf, const int16_t* src, int sx, int sy, int dRow, int sRow,
+                                hipStream_t st) {
+    check_dims(sx, sy, 1);
+    CCA_REQUIRE(dst && buf && src, CC_ERR_INVALID_ARG, "null DWT buffer");
+    if (dRow <= 0) dRow = sx / 2;
+    if (sRow <= 0) sRow = sx;
+    CCA_REQUIRE(dRow >= sx / 2, CC_ERR_INVALID_ARG, "dstRowPitch must be >= sizeX / 2");
+    CCA_LAUNCH("dwt53_fwd_x_lowpass", (uint64_t)sx * sy * 3, (fwd_x<Fwd53, 1, int16_t, true>), x_grid(sy), dim3(256), 0,
+               st, buf, src, sx, sy, sy, dRow, 0, sRow, 0);
+    CCA_LAUNCH_CHECK();
+    CCA_LAUNCH("dwt53_fwd_y_lowpass", (uint64_t)sx / 2 * sy * 3, (fwd_s<Fwd53, true>), s_grid(sx / 2, sy, 1), dim3(256), 0,
+               st, dst, buf, sx / 2, sy, (long long)dRow, 0LL);
+    CCA_LAUNCH_CHECK();
+}
+
+namespace {
+
+template <class TIn>
+void dwt97_forward_t(float* dst, float* buf2, float* buf1, const TIn* src, int sx, int sy, int sz, int ch, int dRow,
+                     int dSlice, int sRow, int sSlice, hipStream_t st) {
     check_dims(sx, sy, sz);
     CCA_REQUIRE(ch >= 1 && ch <= 4, CC_ERR_INVALID_ARG, "srcChannelCount must be 1..4");
     if (dRow <= 0) dRow = sx;
@@ -255,12 +377,7 @@ void dwt97_forward(float* dst, float* buf2, float* buf1, const float* src, int s
     const bool do3d = sz > 1;
     CCA_REQUIRE(dst && buf1 && src && (!do3d || buf2), CC_ERR_INVALID_ARG, "null DWT buffer");
     // X: src -> buf1, Y: buf1 -> (3D ? buf2 : dst), Z: buf2 -> dst (DWTFloat.cu:113-201)
-    switch (ch) {
-        case 1: launch_fwd97_x<1>(buf1, src, sx, sy, sz, dRow, dSlice, sRow, sSlice, st); break;
-        case 2: launch_fwd97_x<2>(buf1, src, sx, sy, sz, dRow, dSlice, sRow, sSlice, st); break;
-        case 3: launch_fwd97_x<3>(buf1, src, sx, sy, sz, dRow, dSlice, sRow, sSlice, st); break;
-        default: launch_fwd97_x<4>(buf1, src, sx, sy, sz, dRow, dSlice, sRow, sSlice, st); break;
-    }
+    launch_fwd97_x_ch(ch, buf1, src, sx, sy, sz, dRow, dSlice, sRow, sSlice, st);
     CCA_LAUNCH_CHECK();
     float* ydst = do3d ? buf2 : dst;
     CCA_LAUNCH("dwt97_fwd_y", (uint64_t)sx * sy * sz * 8, (fwd_s<Fwd97>), s_grid(sx, sy, sz), dim3(256), 0, st, ydst, buf1, sx, sy, (long long)dRow,
@@ -273,8 +390,9 @@ void dwt97_forward(float* dst, float* buf2, float* buf1, const float* src, int s
     }
 }
 
-void dwt97_inverse(float* dst, float* buf2, float* buf1, const float* src, int sx, int sy, int sz, int ch, int dRow,
-                   int dSlice, int sRow, int sSlice, hipStream_t st) {
+template <class TOut>
+void dwt97_inverse_t(TOut* dst, float* buf2, float* buf1, const float* src, int sx, int sy, int sz, int ch, int dRow,
+                     int dSlice, int sRow, int sSlice, hipStream_t st) {
     check_dims(sx, sy, sz);
     CCA_REQUIRE(ch >= 1 && ch <= 4, CC_ERR_INVALID_ARG, "dstChannelCount must be 1..4");
     if (dRow <= 0) dRow = sx * ch;
@@ -293,18 +411,14 @@ void dwt97_inverse(float* dst, float* buf2, float* buf1, const float* src, int s
     CCA_LAUNCH("dwt97_inv_y", (uint64_t)sx * sy * sz * 8, (inv_s<Inv97, PlainFetch>), s_grid(sx, sy, sz), dim3(256), 0, st, buf1, ysrc, sx, sy,
                        (long long)sRow, (long long)sSlice, (long long)sRow, (long long)sSlice, PlainFetch{});
     CCA_LAUNCH_CHECK();
-    switch (ch) {
-        case 1: launch_inv97_x<1>(dst, buf1, sx, sy, sz, dRow, dSlice, sRow, sSlice, st); break;
-        case 2: launch_inv97_x<2>(dst, buf1, sx, sy, sz, dRow, dSlice, sRow, sSlice, st); break;
-        case 3: launch_inv97_x<3>(dst, buf1, sx, sy, sz, dRow, dSlice, sRow, sSlice, st); break;
-        default: launch_inv97_x<4>(dst, buf1, sx, sy, sz, dRow, dSlice, sRow, sSlice, st); break;
-    }
+    launch_inv97_x_ch(ch, dst, buf1, sx, sy, sz, dRow, dSlice, sRow, sSlice, st);
     CCA_LAUNCH_CHECK();
 }
 
-void dwt97_inverse_from_symbols(float* dst, float* buf2, float* buf1, const float* lowpass,
-                                const void* const* dd_highpass, int symbolSize, float step, int sx, int sy, int sz,
-                                int ch, int dRow, int dSlice, int lpRow, int lpSlice, hipStream_t st) {
+template <class TOut>
+void dwt97_inverse_from_symbols_t(TOut* dst, float* buf2, float* buf1, const float* lowpass,
+                                  const void* const* dd_highpass, int symbolSize, float step, int sx, int sy, int sz,
+                                  int ch, int dRow, int dSlice, int lpRow, int lpSlice, hipStream_t st) {
     check_dims(sx, sy, sz);
     CCA_REQUIRE(ch >= 1 && ch <= 4, CC_ERR_INVALID_ARG, "dstChannelCount must be 1..4");
     CCA_REQUIRE(symbolSize == 2 || symbolSize == 4, CC_ERR_INVALID_ARG, "symbolSize must be 2 or 4");
@@ -340,14 +454,11 @@ void dwt97_inverse_from_symbols(float* dst, float* buf2, float* buf1, const floa
                            bSlice, bRow, bSlice, PlainFetch{});
         CCA_LAUNCH_CHECK();
     }
-    switch (ch) {
-        case 1: launch_inv97_x<1>(dst, buf1, sx, sy, sz, dRow, dSlice, (int)bRow, (int)bSlice, st); break;
-        case 2: launch_inv97_x<2>(dst, buf1, sx, sy, sz, dRow, dSlice, (int)bRow, (int)bSlice, st); break;
-        case 3: launch_inv97_x<3>(dst, buf1, sx, sy, sz, dRow, dSlice, (int)bRow, (int)bSlice, st); break;
-        default: launch_inv97_x<4>(dst, buf1, sx, sy, sz, dRow, dSlice, (int)bRow, (int)bSlice, st); break;
-    }
+    launch_inv97_x_ch(ch, dst, buf1, sx, sy, sz, dRow, dSlice, (int)bRow, (int)bSlice, st);
     CCA_LAUNCH_CHECK();
 }
+
+}  // namespace
 
 void dwt53_forward(int16_t* dst, int16_t* buf, const int16_t* src, int sx, int sy, int sz, int dRow, int dSlice,
                    int sRow, int sSlice, hipStream_t st) {

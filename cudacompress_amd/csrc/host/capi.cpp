@@ -1,5 +1,7 @@
 // extern "C" entry points declared in include/cudacompress_amd.h.  Every function converts
 // cca::Error / std::bad_alloc into a status code and keeps the text for cc_last_error().
+#include <algorithm>
+#include <cstdio>
 #include <new>
 #include <string>
 
@@ -345,6 +347,168 @@ cc_status cc_dwt_float_inverse_from_symbols(float* d_dst, float* d_buffer2, floa
                                         sizeX, sizeY, sizeZ, dstChannelCount, dstRowPitch, dstSlicePitch,
                                         lowpassRowPitch, lowpassSlicePitch, as_stream(stream));
     });
+}
+cc_status cc_set_timing_detail(cc_instance* inst, int detail) {
+    return guarded([&] {
+        CCA_REQUIRE(inst && detail >= 0 && detail <= 3, CC_ERR_INVALID_ARG, "bad timing detail");
+        inst->timing_detail = detail;
+        if (detail >= 3) cc_profile_enable(1);
+    });
+}
+cc_status cc_get_timings(cc_instance* inst, uint32_t* count) {
+    return guarded([&] {
+        CCA_REQUIRE(inst && count, CC_ERR_INVALID_ARG, "null argument");
+        cca::DeviceGuard guard(inst->device);
+        cca::Timings& t = inst->timings;
+        t.collect();
+        t.view.clear();
+        for (const std::string& n : t.order) t.view.push_back({n, (float)t.ms[n]});
+        if (inst->timing_detail >= 3) {
+            const uint32_t k = cc_profile_collect();
+            for (uint32_t i = 0; i < k; i++) {
+                const char* name = nullptr;
+                uint64_t launches = 0, bytes = 0;
+                double ms = 0.0;
+                if (cc_profile_entry(i, &name, &launches, &ms, &bytes) == CC_OK)
+                    t.view.push_back({std::string("Kernel: ") + name, (float)ms});
+            }
+        }
+        *count = (uint32_t)t.view.size();
+    });
+}
+cc_status cc_get_timing_entry(cc_instance* inst, uint32_t i, const char** name, float* ms) {
+    return guarded([&] {
+        CCA_REQUIRE(inst && name && ms, CC_ERR_INVALID_ARG, "null argument");
+        CCA_REQUIRE(i < inst->timings.view.size(), CC_ERR_INVALID_ARG, "timing index out of range");
+        *name = inst->timings.view[i].first.c_str();
+        *ms = inst->timings.view[i].second;
+    });
+}
+cc_status cc_print_timings(cc_instance* inst) {
+    uint32_t n = 0;
+    cc_status rc = cc_get_timings(inst, &n);
+    if (rc != CC_OK) return rc;
+    size_t w = 0;
+    for (auto& e : inst->timings.view) w = std::max(w, e.first.size());
+    for (auto& e : inst->timings.view) std::printf("%-*s: %7.2f ms\n", (int)w + 1, e.first.c_str(), e.second);
+    return CC_OK;
+}
+cc_status cc_reset_timings(cc_instance* inst) {
+    return guarded([&] {
+        CCA_REQUIRE(inst, CC_ERR_INVALID_ARG, "null argument");
+        cca::DeviceGuard guard(inst->device);
+        inst->timings.reset();
+        inst->timings.view.clear();
+        if (inst->timing_detail >= 3) cc_profile_reset();
+    });
+}
+cc_status cc_dwt_float_forward_from_byte(float* d_dst, float* d_buffer2, float* d_buffer1, const uint8_t* d_src,
+                                         int sizeX, int sizeY, int sizeZ, int srcChannelCount, int dstRowPitch,
+                                         int dstSlicePitch, int srcRowPitch, int srcSlicePitch, void* stream) {
+    return guarded([&] {
+        cca::dwt97_forward_from_byte(d_dst, d_buffer2, d_buffer1, d_src, sizeX, sizeY, sizeZ, srcChannelCount,
+                                     dstRowPitch, dstSlicePitch, srcRowPitch, srcSlicePitch, as_stream(stream));
+    });
+}
+cc_status cc_dwt_float_inverse_to_byte(uint8_t* d_dst, float* d_buffer2, float* d_buffer1, const float* d_src,
+                                       int sizeX, int sizeY, int sizeZ, int dstChannelCount, int dstRowPitch,
+                                       int dstSlicePitch, int srcRowPitch, int srcSlicePitch, void* stream) {
+    return guarded([&] {
+        cca::dwt97_inverse_to_byte(d_dst, d_buffer2, d_buffer1, d_src, sizeX, sizeY, sizeZ, dstChannelCount,
+                                   dstRowPitch, dstSlicePitch, srcRowPitch, srcSlicePitch, as_stream(stream));
+    });
+}
+cc_status cc_dwt_float_inverse_from_symbols_to_byte(uint8_t* d_dst, float* d_buffer2, float* d_buffer1,
+                                                    const float* d_lowpass, const uint16_t* const* dd_highpass,
+                                                    float quantStep, int sizeX, int sizeY, int sizeZ,
+                                                    int dstChannelCount, int dstRowPitch, int dstSlicePitch,
+                                                    int lowpassRowPitch, int lowpassSlicePitch, void* stream) {
+    return guarded([&] {
+        cca::dwt97_inverse_from_symbols_to_byte(d_dst, d_buffer2, d_buffer1, d_lowpass,
+                                                (const void* const*)dd_highpass, quantStep, sizeX, sizeY, sizeZ,
+                                                dstChannelCount, dstRowPitch, dstSlicePitch, lowpassRowPitch,
+                                                lowpassSlicePitch, as_stream(stream));
+    });
+}
+cc_status cc_dwt_float_forward_lowpass_only_from_byte(float* d_dst, float* d_buffer, const uint8_t* d_src, int sizeX,
+                                                      int sizeY, int srcChannelCount, int dstRowPitch, int srcRowPitch,
+                                                      void* stream) {
+    return guarded([&] {
+        cca::dwt97_forward_lowpass_only_from_byte(d_dst, d_buffer, d_src, sizeX, sizeY, srcChannelCount, dstRowPitch,
+                                                  srcRowPitch, as_stream(stream));
+    });
+}
+cc_status cc_dwt_int_forward_lowpass_only(int16_t* d_dst, int16_t* d_buffer, const int16_t* d_src, int sizeX,
+                                          int sizeY, int dstRowPitch, int srcRowPitch, void* stream) {
+    return guarded([&] {
+        cca::dwt53_forward_lowpass_only(d_dst, d_buffer, d_src, sizeX, sizeY, dstRowPitch, srcRowPitch,
+                                        as_stream(stream));
+    });
+}
+cc_status cc_float_to_byte_2d(uint8_t* d_quant, uint32_t channelCount, uint32_t channel, const float* d_data,
+                              uint32_t sizeX, uint32_t sizeY, uint32_t rowPitchSrc, void* stream) {
+    return guarded([&] {
+        cca::float_to_byte_2d(d_quant, channelCount, channel, d_data, sizeX, sizeY, rowPitchSrc, as_stream(stream));
+    });
+}
+cc_status cc_byte_to_float_2d(float* d_data, const uint8_t* d_quant, uint32_t channelCount, uint32_t channel,
+                              uint32_t sizeX, uint32_t sizeY, uint32_t rowPitchDst, void* stream) {
+    return guarded([&] {
+        cca::byte_to_float_2d(d_data, d_quant, channelCount, channel, sizeX, sizeY, rowPitchDst, as_stream(stream));
+    });
+}
+cc_status cc_quantize_difference_to_symbols_2d(uint16_t* d_symbols, const float* d_data, float quantizationStep,
+                                               const uint8_t* d_reference, uint32_t channelCount, uint32_t channel,
+                                               uint32_t sizeX, uint32_t sizeY, uint32_t rowPitchSrc, int quantType,
+                                               void* stream) {
+    return guarded([&] {
+        cca::quantize_difference_2d(d_symbols, const_cast<float*>(d_data), quantizationStep, d_reference, channelCount,
+                                    channel, sizeX, sizeY, rowPitchSrc, quantType, false, as_stream(stream));
+    });
+}
+cc_status cc_quantize_difference_to_symbols_roundtrip_2d(uint16_t* d_symbols, float* d_data, float quantizationStep,
+                                                         const uint8_t* d_reference, uint32_t channelCount,
+                                                         uint32_t channel, uint32_t sizeX, uint32_t sizeY,
+                                                         uint32_t rowPitchSrc, int quantType, void* stream) {
+    return guarded([&] {
+        cca::quantize_difference_2d(d_symbols, d_data, quantizationStep, d_reference, channelCount, channel, sizeX,
+                                    sizeY, rowPitchSrc, quantType, true, as_stream(stream));
+    });
+}
+cc_status cc_unquantize_difference_from_symbols_2d(float* d_data, const uint16_t* d_symbols, float quantizationStep,
+                                                   const uint8_t* d_reference, uint32_t channelCount, uint32_t channel,
+                                                   uint32_t sizeX, uint32_t sizeY, uint32_t rowPitchDst,
+                                                   int quantType, void* stream) {
+    return guarded([&] {
+        cca::unquantize_difference_2d(d_data, d_symbols, quantizationStep, d_reference, channelCount, channel, sizeX,
+                                      sizeY, rowPitchDst, quantType, as_stream(stream));
+    });
+}
+cc_status cc_convert_rgb_to_ycocg(uint8_t* d_target, const uint8_t* d_data, int pixelCount, int channels,
+                                  void* stream) {
+    return guarded([&] {
+        CCA_REQUIRE(pixelCount >= 0, CC_ERR_INVALID_ARG, "negative pixel count");
+        cca::convert_ycocg(d_target, d_data, (uint32_t)pixelCount, channels, true, as_stream(stream));
+    });
+}
+cc_status cc_convert_ycocg_to_rgb(uint8_t* d_target, const uint8_t* d_data, int pixelCount, int channels,
+                                  void* stream) {
+    return guarded([&] {
+        CCA_REQUIRE(pixelCount >= 0, CC_ERR_INVALID_ARG, "negative pixel count");
+        cca::convert_ycocg(d_target, d_data, (uint32_t)pixelCount, channels, false, as_stream(stream));
+    });
+}
+cc_status cc_get_lsb(const int16_t* d_data, uint32_t* d_bits, uint32_t elemCount, void* stream) {
+    return guarded([&] { cca::lsb_get(d_data, d_bits, elemCount, as_stream(stream)); });
+}
+cc_status cc_remove_lsb(int16_t* d_data, uint32_t* d_bits, uint32_t elemCount, int16_t shift, void* stream) {
+    return guarded([&] { cca::lsb_remove(d_data, d_bits, elemCount, shift, as_stream(stream)); });
+}
+cc_status cc_append_lsb(int16_t* d_data, const uint32_t* d_bits, uint32_t elemCount, int16_t shift, void* stream) {
+    return guarded([&] { cca::lsb_append(d_data, d_bits, elemCount, shift, as_stream(stream)); });
+}
+cc_status cc_shift_elems(int16_t* d_data, uint32_t elemCount, int16_t shift, void* stream) {
+    return guarded([&] { cca::shift_elems(d_data, elemCount, shift, as_stream(stream)); });
 }
 cc_status cc_dwt_int_forward(int16_t* d_dst, int16_t* d_buffer, const int16_t* d_src, int sizeX, int sizeY, int sizeZ,
                              int dstRowPitch, int dstSlicePitch, int srcRowPitch, int srcSlicePitch, void* stream) {

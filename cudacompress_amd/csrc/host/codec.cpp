@@ -78,8 +78,11 @@ void encode_streams(cc_instance* I, bool rl, int symSize, const void* const* str
     I->seg_begin.assign(ns, 0);
     I->seg_bits.assign(ns, 0);
     if (ns == 0) return;
+    SectionTimer call(I, 1), stage(I, 2);
+    call(rl ? "Encode RLHuff" : "Encode Huff");
 
     // ---- 1. RLE / maxima --------------------------------------------------------------------
+    stage(rl ? "Encode: Run Length" : "Encode: Maxima");
     const uint64_t cap = align_up(std::max<uint32_t>(n, 1), 64);
     const uint32_t tiles_per = rl ? div_up(n, kRleTile) : 0;
     const uint32_t ntiles = tiles_per * ns;
@@ -117,6 +120,7 @@ void encode_streams(cc_instance* I, bool rl, int symSize, const void* const* str
     std::vector<RleResult> res((RleResult*)I->h_down.p, (RleResult*)I->h_down.p + ns);
 
     // ---- 2. histograms ----------------------------------------------------------------------
+    stage("Encode: Histograms");
     const uint32_t nt = rl ? 2 * ns : ns;
     struct Coded {
         const void* data;
@@ -175,6 +179,7 @@ void encode_streams(cc_instance* I, bool rl, int symSize, const void* const* str
     const uint32_t* hh = (const uint32_t*)I->h_hist.p;
 
     // ---- 3. table design ----------------------------------------------------------------------
+    stage("Encode: Design Huffman Tables");
     std::vector<EncTable> tables(nt);
     std::vector<cc_status> rcs(nt, CC_OK);
     std::vector<uint64_t> bits(nt, 0);
@@ -199,6 +204,7 @@ void encode_streams(cc_instance* I, bool rl, int symSize, const void* const* str
     }
 
     // ---- 4. Huffman encode ------------------------------------------------------------------
+    stage("Encode: Huffman Encode");
     std::vector<uint64_t> tab_off(nt), word_off(nt), offs_off(nt);
     std::vector<uint32_t> nblk(nt), nchk(nt);
     uint64_t tab_total = 0, word_total = 0, offs_total = 0;
@@ -267,6 +273,7 @@ void encode_streams(cc_instance* I, bool rl, int symSize, const void* const* str
     }
 
     // ---- 5. bitstream layout ------------------------------------------------------------------
+    stage("Encode: Bitstream Layout");
     const uint32_t* hw = (const uint32_t*)I->h_words.p;
     std::vector<uint16_t> inc;
     BitWriter w(I->out_words);
@@ -318,6 +325,9 @@ void decode_streams(cc_instance* I, bool rl, int symSize, const uint32_t* const*
     CCA_REQUIRE(outs != nullptr, CC_ERR_INVALID_ARG, "null output array");
     const uint32_t cbs = I->cbs;
     const HeaderFormat fmt = header_format(I->dialect, I->log2);
+    SectionTimer call(I, 1), stage(I, 2);
+    call(rl ? "Decode RLHuff" : "Decode Huff");
+    stage("Decode: Read Tables + Upload");
 
     struct Coded {
         DecTable t;
@@ -443,9 +453,11 @@ void decode_streams(cc_instance* I, bool rl, int symSize, const uint32_t* const*
         total_bits += 32ull * coded[j].nwords;
     }
     for (uint32_t i = 0; i < ns && rl; i++) total_pairs += m[i];
+    stage("Decode: Huffman Decode");
     huffman_decode((const HuffDecStream*)(d + o_desc), nc, (const uint32_t*)(d + o_wmap), nwaves, cbs, symSize,
                    total_syms, total_bits, st);
     if (rl) {
+        stage("Decode: Run Length Decode");
         uint32_t* sums = I->d_dec_sums.as<uint32_t>(std::max<uint32_t>(ntiles, 1));
         rle_decode((const RleDecStream*)(d + o_rdesc), ns, sums, ntiles, (const uint32_t*)(d + o_tmap), n, symSize,
                    total_pairs, (uint64_t)n * ns, st);

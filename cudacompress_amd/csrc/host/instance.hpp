@@ -10,6 +10,7 @@
 
 #include "../coder.hpp"
 #include "../common.hpp"
+#include "timing.hpp"
 
 namespace cca {
 
@@ -106,6 +107,10 @@ struct cc_instance {
     std::vector<std::unique_ptr<cca::BrickCtx>> groups;
     hipEvent_t group_fork = nullptr;
     uint32_t brick_groups = 0;  // groups per brick batch, 0: chosen from the batch size
+
+    // section timing (cc_set_timing_detail)
+    int timing_detail = 0;
+    cca::Timings timings;
 
     ~cc_instance();
 };

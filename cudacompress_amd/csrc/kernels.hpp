@@ -15,8 +15,19 @@ void dwt97_inverse(float* dst, float* buf2, float* buf1, const float* src, int s
 void dwt97_inverse_from_symbols(float* dst, float* buf2, float* buf1, const float* lowpass,
                                 const void* const* dd_highpass, int symbolSize, float step, int sx, int sy, int sz,
                                 int dstCh, int dRow, int dSlice, int lpRow, int lpSlice, hipStream_t st);
+void dwt97_forward_from_byte(float* dst, float* buf2, float* buf1, const uint8_t* src, int sx, int sy, int sz,
+                             int srcCh, int dRow, int dSlice, int sRow, int sSlice, hipStream_t st);
+void dwt97_inverse_to_byte(uint8_t* dst, float* buf2, float* buf1, const float* src, int sx, int sy, int sz, int dstCh,
+                           int dRow, int dSlice, int sRow, int sSlice, hipStream_t st);
+void dwt97_inverse_from_symbols_to_byte(uint8_t* dst, float* buf2, float* buf1, const float* lowpass,
+                                        const void* const* dd_highpass, float step, int sx, int sy, int sz, int dstCh,
+                                        int dRow, int dSlice, int lpRow, int lpSlice, hipStream_t st);
+void dwt97_forward_lowpass_only_from_byte(float* dst, float* buf, const uint8_t* src, int sx, int sy, int srcCh,
+                                          int dRow, int sRow, hipStream_t st);
 void dwt53_forward(int16_t* dst, int16_t* buf, const int16_t* src, int sx, int sy, int sz, int dRow, int dSlice,
                    int sRow, int sSlice, hipStream_t st);
+void dwt53_forward_lowpass_only(int16_t* dst, int16_t* buf, const int16_t* src, int sx, int sy, int dRow, int sRow,
+                                hipStream_t st);
 void dwt53_inverse(int16_t* dst, int16_t* buf, const int16_t* src, int sx, int sy, int sz, int dRow, int dSlice,
                    int sRow, int sSlice, hipStream_t st);
 
@@ -34,6 +45,22 @@ void quantize_to_short(int16_t* q, const float* data, uint32_t sx, uint32_t sy, 
 void unquantize_from_short(float* data, const int16_t* q, uint32_t sx, uint32_t sy, uint32_t sz, float step,
                            uint32_t row, uint32_t slice, int qtype, hipStream_t st);
 void max_abs(const float* data, uint32_t n, float* out, hipStream_t st);
+
+// ---- image path and bit planes (src/cudaCompress/util/Quantize.cu, YCoCg.cu, Bits.cu) ---------
+void float_to_byte_2d(uint8_t* out, uint32_t channelCount, uint32_t channel, const float* data, uint32_t sx,
+                      uint32_t sy, uint32_t row, hipStream_t st);
+void byte_to_float_2d(float* data, const uint8_t* in, uint32_t channelCount, uint32_t channel, uint32_t sx,
+                      uint32_t sy, uint32_t row, hipStream_t st);
+void quantize_difference_2d(uint16_t* sym, float* data, float step, const uint8_t* ref, uint32_t channelCount,
+                            uint32_t channel, uint32_t sx, uint32_t sy, uint32_t row, int qtype, bool roundtrip,
+                            hipStream_t st);
+void unquantize_difference_2d(float* data, const uint16_t* sym, float step, const uint8_t* ref, uint32_t channelCount,
+                              uint32_t channel, uint32_t sx, uint32_t sy, uint32_t row, int qtype, hipStream_t st);
+void convert_ycocg(uint8_t* out, const uint8_t* in, uint32_t pixels, int channels, bool to_ycocg, hipStream_t st);
+void lsb_get(const int16_t* data, uint32_t* bits, uint32_t n, hipStream_t st);
+void lsb_remove(int16_t* data, uint32_t* bits, uint32_t n, int16_t shift, hipStream_t st);
+void lsb_append(int16_t* data, const uint32_t* bits, uint32_t n, int16_t shift, hipStream_t st);
+void shift_elems(int16_t* data, uint32_t n, int16_t shift, hipStream_t st);
 
 // ---- entropy coder kernels -----------------------------------------------------------------
 // Per-stream descriptors live in device memory; see codec.cpp for how they are filled.

@@ -1,6 +1,6 @@
 // Drop-in replacement of include/cudaCompress/util/Quantize.h (m0bl0/cudaCompress): header-only
 // wrappers over cc_quantize_* / cc_symbolize (include/cudacompress_amd.h).  As in the reference,
-// these run on the null stream.  The byte and difference variants are not provided.
+// these run on the null stream (unquantizeDifferenceFromSymbols2D takes a stream).
 #ifndef CUDACOMPRESS_AMD_DROPIN_QUANTIZE_H
 #define CUDACOMPRESS_AMD_DROPIN_QUANTIZE_H
 
@@ -76,6 +76,35 @@ inline void unquantizeFromShort(float* dpData, const short* dpQuant, uint sizeX,
                                 EQuantizeType quantType = QUANTIZE_DEADZONE) {
     cc_unquantize_from_short(dpData, dpQuant, sizeX, sizeY, sizeZ, quantizationStep, rowPitchDst, slicePitchDst,
                              (int)quantType, nullptr);
+}
+inline void floatToByte2D(byte* dpQuant, uint channelCount, uint channel, const float* dpData, uint sizeX, uint sizeY,
+                          uint rowPitchSrc = 0) {
+    cc_float_to_byte_2d(dpQuant, channelCount, channel, dpData, sizeX, sizeY, rowPitchSrc, nullptr);
+}
+inline void byteToFloat2D(float* dpData, const byte* dpQuant, uint channelCount, uint channel, uint sizeX, uint sizeY,
+                          uint rowPitchDst = 0) {
+    cc_byte_to_float_2d(dpData, dpQuant, channelCount, channel, sizeX, sizeY, rowPitchDst, nullptr);
+}
+inline void quantizeDifferenceToSymbols2D(ushort* dpSymbols, const float* dpData, float quantizationStep,
+                                          const byte* dpReference, uint channelCount, uint channel, uint sizeX,
+                                          uint sizeY, uint rowPitchSrc = 0,
+                                          EQuantizeType quantType = QUANTIZE_DEADZONE) {
+    cc_quantize_difference_to_symbols_2d(dpSymbols, dpData, quantizationStep, dpReference, channelCount, channel,
+                                         sizeX, sizeY, rowPitchSrc, (int)quantType, nullptr);
+}
+inline void quantizeDifferenceToSymbolsRoundtrip2D(ushort* dpSymbols, float* dpData, float quantizationStep,
+                                                   const byte* dpReference, uint channelCount, uint channel,
+                                                   uint sizeX, uint sizeY, uint rowPitchSrc = 0,
+                                                   EQuantizeType quantType = QUANTIZE_DEADZONE) {
+    cc_quantize_difference_to_symbols_roundtrip_2d(dpSymbols, dpData, quantizationStep, dpReference, channelCount,
+                                                   channel, sizeX, sizeY, rowPitchSrc, (int)quantType, nullptr);
+}
+inline void unquantizeDifferenceFromSymbols2D(float* dpData, const ushort* dpSymbols, float quantizationStep,
+                                              const byte* dpReference, uint channelCount, uint channel, uint sizeX,
+                                              uint sizeY, uint rowPitchDst = 0,
+                                              EQuantizeType quantType = QUANTIZE_DEADZONE, hipStream_t stream = 0) {
+    cc_unquantize_difference_from_symbols_2d(dpData, dpSymbols, quantizationStep, dpReference, channelCount, channel,
+                                             sizeX, sizeY, rowPitchDst, (int)quantType, (void*)stream);
 }
 inline void getMaxAbs(Instance* pInstance, const float* dpImage, uint elemCount, float* dpValMax) {
     cc_get_max_abs(detail::raw(pInstance), dpImage, elemCount, dpValMax);

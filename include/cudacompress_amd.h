@@ -129,6 +129,19 @@ CC_API cc_status cc_decode_device(cc_instance* inst, int rl, int symbolSize, con
                                   uint64_t wordCount, void* const* d_symbolStreams, uint32_t streamCount,
                                   uint32_t symbolCountPerStream, uint64_t* outWordsConsumed);
 
+/* ---- Timing: include/cudaCompress/Timing.h:15-27 -------------------------------------------- */
+/* setTimingDetail: 0 none, 1 whole coder calls ("Encode RLHuff", ...), 2 also their stages
+ * ("Encode: Histograms", ...), 3 also every kernel launch ("Kernel: <name>"; kernel timing is
+ * process-wide, cc_profile_enable).  Sections are hipEvent pairs on the instance stream. */
+CC_API cc_status cc_set_timing_detail(cc_instance* inst, int detail);
+/* getTimings: waits for the recorded sections and snapshots (name, accumulated ms) pairs, read
+ * with cc_get_timing_entry(i) until the next cc_get_timings call. */
+CC_API cc_status cc_get_timings(cc_instance* inst, uint32_t* count);
+CC_API cc_status cc_get_timing_entry(cc_instance* inst, uint32_t i, const char** name, float* ms);
+/* printTimings / resetTimings */
+CC_API cc_status cc_print_timings(cc_instance* inst);
+CC_API cc_status cc_reset_timings(cc_instance* inst);
+
 /* ---- DWT: include/cudaCompress/util/DWT.h:21-155 ------------------------------------------ */
 
 /* dwtFloat3DForward / dwtFloat2DForward (sizeZ == 1, dpBuffer2 unused), util/DWT.h:21-31.
@@ -156,8 +169,77 @@ CC_API cc_status cc_dwt_int_forward(int16_t* d_dst, int16_t* d_buffer, const int
 CC_API cc_status cc_dwt_int_inverse(int16_t* d_dst, int16_t* d_buffer, const int16_t* d_src,
                                     int sizeX, int sizeY, int sizeZ, int dstRowPitch, int dstSlicePitch,
                                     int srcRowPitch, int srcSlicePitch, void* stream);
+/* dwtFloat{2D,3D}ForwardFromByte, util/DWT.h:81-94: byte samples enter as value - 128. */
+CC_API cc_status cc_dwt_float_forward_from_byte(float* d_dst, float* d_buffer2, float* d_buffer1,
+                                                const uint8_t* d_src, int sizeX, int sizeY, int sizeZ,
+                                                int srcChannelCount, int dstRowPitch, int dstSlicePitch,
+                                                int srcRowPitch, int srcSlicePitch, void* stream);
+/* dwtFloat{2D,3D}InverseToByte, util/DWT.h:96-107: outputs rounded half up, clamped to [0,255]
+ * after adding 128. */
+CC_API cc_status cc_dwt_float_inverse_to_byte(uint8_t* d_dst, float* d_buffer2, float* d_buffer1,
+                                              const float* d_src, int sizeX, int sizeY, int sizeZ,
+                                              int dstChannelCount, int dstRowPitch, int dstSlicePitch,
+                                              int srcRowPitch, int srcSlicePitch, void* stream);
+/* dwtFloat{2D,3D}InverseFromSymbolsToByte (ushort highpass), util/DWT.h:109-122 */
+CC_API cc_status cc_dwt_float_inverse_from_symbols_to_byte(uint8_t* d_dst, float* d_buffer2, float* d_buffer1,
+                                                           const float* d_lowpass,
+                                                           const uint16_t* const* dd_highpass, float quantStep,
+                                                           int sizeX, int sizeY, int sizeZ, int dstChannelCount,
+                                                           int dstRowPitch, int dstSlicePitch, int lowpassRowPitch,
+                                                           int lowpassSlicePitch, void* stream);
+/* dwtFloat2DForwardLowpassOnlyFromByte, util/DWT.h:125-131: the LL band only, (sizeX/2) x
+ * (sizeY/2) at row pitch dstRowPitch (default sizeX/2); d_buffer holds sizeY rows of the same
+ * pitch. */
+CC_API cc_status cc_dwt_float_forward_lowpass_only_from_byte(float* d_dst, float* d_buffer, const uint8_t* d_src,
+                                                             int sizeX, int sizeY, int srcChannelCount,
+                                                             int dstRowPitch, int srcRowPitch, void* stream);
+/* dwtIntForwardLowpassOnly, util/DWT.h:152-156: the LL band of the 2D 5/3 transform (every even
+ * size; the reference computes whole 512 x 64 tiles only). */
+CC_API cc_status cc_dwt_int_forward_lowpass_only(int16_t* d_dst, int16_t* d_buffer, const int16_t* d_src, int sizeX,
+                                                 int sizeY, int dstRowPitch, int srcRowPitch, void* stream);
 
 /* ---- Quantization: include/cudaCompress/util/Quantize.h:29-65 ----------------------------- */
+
+/* floatToByte2D / byteToFloat2D, util/Quantize.h:25-26: channel `channel` of an image of
+ * channelCount interleaved byte channels <-> a float plane of row pitch rowPitch (0: sizeX). */
+CC_API cc_status cc_float_to_byte_2d(uint8_t* d_quant, uint32_t channelCount, uint32_t channel, const float* d_data,
+                                     uint32_t sizeX, uint32_t sizeY, uint32_t rowPitchSrc, void* stream);
+CC_API cc_status cc_byte_to_float_2d(float* d_data, const uint8_t* d_quant, uint32_t channelCount, uint32_t channel,
+                                     uint32_t sizeX, uint32_t sizeY, uint32_t rowPitchDst, void* stream);
+/* quantizeDifferenceToSymbols2D / ...Roundtrip2D / unquantizeDifferenceFromSymbols2D,
+ * util/Quantize.h:58-61: symbols of (data - (reference byte - 128)). */
+CC_API cc_status cc_quantize_difference_to_symbols_2d(uint16_t* d_symbols, const float* d_data, float quantizationStep,
+                                                      const uint8_t* d_reference, uint32_t channelCount,
+                                                      uint32_t channel, uint32_t sizeX, uint32_t sizeY,
+                                                      uint32_t rowPitchSrc, int quantType, void* stream);
+CC_API cc_status cc_quantize_difference_to_symbols_roundtrip_2d(uint16_t* d_symbols, float* d_data,
+                                                                float quantizationStep, const uint8_t* d_reference,
+                                                                uint32_t channelCount, uint32_t channel,
+                                                                uint32_t sizeX, uint32_t sizeY, uint32_t rowPitchSrc,
+                                                                int quantType, void* stream);
+CC_API cc_status cc_unquantize_difference_from_symbols_2d(float* d_data, const uint16_t* d_symbols,
+                                                          float quantizationStep, const uint8_t* d_reference,
+                                                          uint32_t channelCount, uint32_t channel, uint32_t sizeX,
+                                                          uint32_t sizeY, uint32_t rowPitchDst, int quantType,
+                                                          void* stream);
+
+/* ---- Colour: include/cudaCompress/util/YCoCg.h:16-20 ---------------------------------------- */
+/* convertRGBtoYCoCg / convertYCoCgtoRGB on uchar3 (channels 3) or uchar4 (channels 4, the fourth
+ * channel copied) pixels. */
+CC_API cc_status cc_convert_rgb_to_ycocg(uint8_t* d_target, const uint8_t* d_data, int pixelCount, int channels,
+                                         void* stream);
+CC_API cc_status cc_convert_ycocg_to_rgb(uint8_t* d_target, const uint8_t* d_data, int pixelCount, int channels,
+                                         void* stream);
+
+/* ---- Bit planes: include/cudaCompress/util/Bits.h:16-27 ------------------------------------- */
+/* Word w of d_bits holds bit i = least significant bit of element 32*w + i; ceil(elemCount/32)
+ * words (any elemCount; the reference needs multiples of 4096).  cc_remove_lsb: v += shift, the
+ * bit is taken, v = floor(v / 2); d_bits may be NULL.  cc_append_lsb: v = 2*v + bit + shift. */
+CC_API cc_status cc_get_lsb(const int16_t* d_data, uint32_t* d_bits, uint32_t elemCount, void* stream);
+CC_API cc_status cc_remove_lsb(int16_t* d_data, uint32_t* d_bits, uint32_t elemCount, int16_t shift, void* stream);
+CC_API cc_status cc_append_lsb(int16_t* d_data, const uint32_t* d_bits, uint32_t elemCount, int16_t shift,
+                               void* stream);
+CC_API cc_status cc_shift_elems(int16_t* d_data, uint32_t elemCount, int16_t shift, void* stream);
 
 /* symbolize / unsymbolize, util/Quantize.h:29-30 */
 CC_API cc_status cc_symbolize(uint16_t* d_symbols, const int16_t* d_data, uint32_t sizeX, uint32_t sizeY,

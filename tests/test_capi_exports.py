@@ -52,3 +52,17 @@ def test_invalid_arguments_rejected_before_device_use():
     # DWT size checks run on the host
     assert lib.cc_dwt_int_forward(None, None, None, 7, 8, 1, 0, 0, 0, 0, None) == 3
     assert b"even" in lib.cc_last_error()
+
+
+def test_dropin_headers_compile():
+    """The C++ drop-in headers (include/cudaCompress) compile against the C ABI declarations."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None or not os.path.isdir("/opt/rocm/include"):
+        pytest.skip("g++ or the ROCm headers are not available")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([gxx, "-std=c++17", "-fsyntax-only", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(root, "include"),
+                        "-I", "/opt/rocm/include", os.path.join(root, "tests", "cpp", "dropin_headers.cpp")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
