@@ -123,6 +123,15 @@ struct DecSeq {
     int32_t tab[96];        // first | min | max codeword per length
 };
 
+// One sequence of a decode bound to a resident index: the index sequence it copies, the request
+// item it belongs to, and the byte offset of its RLE decode buffer (~0: a plain sequence decoding
+// straight into the item's output).
+struct DecBind {
+    uint32_t idx_seq;
+    uint32_t item;
+    uint64_t rl_off;
+};
+
 struct DecCallOut {
     uint64_t words;         // words consumed by the call
     uint32_t err;
@@ -170,6 +179,19 @@ void coder_decode(const DecCall* d_calls, uint32_t ncalls, const DecItem* d_item
                   uint32_t nseq, const CoderParams& p, int symSize, uint32_t* scratch, uint64_t scratch_cap,
                   CoderState* d_state, DecCallOut* d_out, const uint32_t* d_wave_seq, const uint32_t* d_wave_first,
                   uint32_t nwaves, uint64_t total_symbols, hipStream_t st);
+// Resident decode index: parse + tables of every call once (RLE sequences get no decode buffer),
+// per-call results in d_out.  scratch then holds the tables and block offsets for bound decodes.
+void coder_decode_index(const DecCall* d_calls, uint32_t ncalls, const DecItem* d_items, DecSeq* d_seqs, uint32_t nseq,
+                        const CoderParams& p, int symSize, uint32_t* scratch, uint64_t scratch_cap, CoderState* d_state,
+                        DecCallOut* d_out, hipStream_t st);
+// A decode whose sequences come from a resident index (d_bind, one entry per sequence; call c is
+// index call d_call_brick[c]): bind, block decode, zero-run expansion, per-call status.
+void coder_decode_bound(const DecSeq* d_idx_seqs, const uint32_t* idx_scratch, const DecCallOut* d_idx_out,
+                        const DecBind* d_bind, const uint32_t* d_call_brick, const DecCall* d_calls, uint32_t ncalls,
+                        const DecItem* d_items, uint32_t nitems, DecSeq* d_seqs, uint32_t nseq, const CoderParams& p,
+                        int symSize, unsigned char* rl_arena, CoderState* d_state, DecCallOut* d_out,
+                        const uint32_t* d_wave_seq, const uint32_t* d_wave_first, uint32_t nwaves,
+                        uint64_t total_symbols, hipStream_t st);
 // calls[i].words = base + enc[i].word_begin, calls[i].nwords = enc[i].words (the rest unchanged);
 // an empty range for a failed call or one reaching past cap words
 void coder_calls_from_encode(DecCall* d_calls, const CallOut* d_enc, const uint32_t* base, uint64_t cap,

@@ -54,8 +54,11 @@ __global__ void k_dec_init(CoderState* st) {
 // One thread per call: walks the sections (their sizes are in-band) and records every coded
 // sequence's header, codeword and offset positions; carves the arena for block offsets and for
 // the decoded values of RLE sequences.
+// index_mode: a resident decode index (tables and block offsets only): RLE sequences get no
+// decode buffer, a bound decode supplies one per request (k_dec_bind).
 __global__ void k_dec_parse(const DecCall* calls, uint32_t ncalls, const DecItem* items, DecSeq* seqs, CoderParams p,
-                            int symSize, uint32_t* scratch, uint64_t cap, CoderState* st, DecCallOut* outc) {
+                            int symSize, uint32_t* scratch, uint64_t cap, CoderState* st, DecCallOut* outc,
+                            int index_mode) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncalls) return;
     const DecCall C = calls[c];
@@ -95,7 +98,7 @@ __global__ void k_dec_parse(const DecCall* calls, uint32_t ncalls, const DecItem
             const uint64_t boff = arena_take(st, nblk ? nblk : 1, cap, aok);
             const uint64_t lut = arena_take(st, (1u << kLut) + (uint64_t)nsym + 1, cap, aok);
             void* out = it.out;
-            if (it.rl) {
+            if (it.rl && !index_mode) {
                 const uint64_t o = arena_take(st, ((uint64_t)count * symSize + 3) / 4 + 1, cap, aok);
                 out = scratch + o;
             }
@@ -424,6 +427,23 @@ __global__ __launch_bounds__(256) void k_dec_rle(const DecItem* items, DecSeq* s
     }
 }
 
+// A bound decode's sequences from a resident index: the parsed header, table and block-offset
+// positions are copied, the item and the output are the request's (RLE sequences decode into the
+// request's buffer at rl_off).  Calls start from the index's parse result of their brick.
+__global__ void k_dec_bind(const DecSeq* idx, const DecBind* bind, uint32_t nseq, const DecItem* items,
+                           DecSeq* seqs, unsigned char* rl_arena, const DecCallOut* idx_out,
+                           const uint32_t* call_brick, uint32_t ncalls, DecCallOut* outc) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nseq) {
+        const DecBind b = bind[r];
+        DecSeq s = idx[b.idx_seq];
+        s.item = b.item;
+        s.out = b.rl_off != ~0ull ? (void*)(rl_arena + b.rl_off) : items[b.item].out;
+        seqs[r] = s;
+    }
+    if (r < ncalls) outc[r] = idx_out[call_brick[r]];
+}
+
 // Per-call status: the first error among the call's coded sequences.
 __global__ void k_dec_finish(const DecCall* calls, uint32_t ncalls, const DecItem* items, const DecSeq* seqs,
                              DecCallOut* outc) {
@@ -472,7 +492,7 @@ void coder_decode(const DecCall* d_calls, uint32_t ncalls, const DecItem* d_item
     CCA_LAUNCH_CHECK();
     if (ncalls == 0) return;
     CCA_LAUNCH("dec_parse", 0, k_dec_parse, dim3(div_up(ncalls, 64)), dim3(64), 0, st, d_calls, ncalls, d_items, d_seqs,
-               p, symSize, scratch, scratch_cap, d_state, d_out);
+               p, symSize, scratch, scratch_cap, d_state, d_out, 0);
     CCA_LAUNCH_CHECK();
     if (nseq == 0) return;
     CCA_LAUNCH("dec_tables", 0, k_dec_tables, dim3(nseq), dim3(256), 0, st, d_seqs, p, scratch, d_state);
@@ -492,6 +512,65 @@ void coder_decode(const DecCall* d_calls, uint32_t ncalls, const DecItem* d_item
     else
         CCA_LAUNCH("dec_rle", outb, k_dec_rle<uint32_t>, dim3(nitems), dim3(256), 0, st, d_items, d_seqs, d_state);
     CCA_LAUNCH_CHECK();
+    CCA_LAUNCH("dec_finish", 0, k_dec_finish, dim3(div_up(ncalls, 64)), dim3(64), 0, st, d_calls, ncalls, d_items,
+               d_seqs, d_out);
+    CCA_LAUNCH_CHECK();
+}
+
+namespace {
+template <class S>
+void launch_blocks_rle(const DecItem* d_items, uint32_t nitems, DecSeq* d_seqs, const CoderParams& p,
+                       const uint32_t* scratch, CoderState* d_state, const uint32_t* d_wave_seq,
+                       const uint32_t* d_wave_first, uint32_t nwaves, uint64_t outb, hipStream_t st) {
+    if (nwaves) {
+        CCA_LAUNCH("dec_blocks", outb, k_dec_blocks<S>, dim3(nwaves), dim3(64), 0, st, d_seqs, d_wave_seq, d_wave_first,
+                   p, scratch);
+        CCA_LAUNCH_CHECK();
+    }
+    CCA_LAUNCH("dec_rle", outb, k_dec_rle<S>, dim3(nitems), dim3(256), 0, st, d_items, d_seqs, d_state);
+    CCA_LAUNCH_CHECK();
+}
+}  // namespace
+
+void coder_decode_index(const DecCall* d_calls, uint32_t ncalls, const DecItem* d_items, DecSeq* d_seqs, uint32_t nseq,
+                        const CoderParams& p, int symSize, uint32_t* scratch, uint64_t scratch_cap, CoderState* d_state,
+                        DecCallOut* d_out, hipStream_t st) {
+    CCA_REQUIRE(symSize == 2 || symSize == 4, CC_ERR_INVALID_ARG, "symbol size must be 2 or 4");
+    CCA_LAUNCH("dec_init", 32, k_dec_init, dim3(1), dim3(64), 0, st, d_state);
+    CCA_LAUNCH_CHECK();
+    if (ncalls == 0) return;
+    CCA_LAUNCH("dec_parse", 0, k_dec_parse, dim3(div_up(ncalls, 64)), dim3(64), 0, st, d_calls, ncalls, d_items, d_seqs,
+               p, symSize, scratch, scratch_cap, d_state, d_out, 1);
+    CCA_LAUNCH_CHECK();
+    if (nseq == 0) return;
+    CCA_LAUNCH("dec_tables", 0, k_dec_tables, dim3(nseq), dim3(256), 0, st, d_seqs, p, scratch, d_state);
+    CCA_LAUNCH_CHECK();
+    CCA_LAUNCH("dec_finish", 0, k_dec_finish, dim3(div_up(ncalls, 64)), dim3(64), 0, st, d_calls, ncalls, d_items,
+               d_seqs, d_out);
+    CCA_LAUNCH_CHECK();
+}
+
+void coder_decode_bound(const DecSeq* d_idx_seqs, const uint32_t* idx_scratch, const DecCallOut* d_idx_out,
+                        const DecBind* d_bind, const uint32_t* d_call_brick, const DecCall* d_calls, uint32_t ncalls,
+                        const DecItem* d_items, uint32_t nitems, DecSeq* d_seqs, uint32_t nseq, const CoderParams& p,
+                        int symSize, unsigned char* rl_arena, CoderState* d_state, DecCallOut* d_out,
+                        const uint32_t* d_wave_seq, const uint32_t* d_wave_first, uint32_t nwaves,
+                        uint64_t total_symbols, hipStream_t st) {
+    CCA_REQUIRE(symSize == 2 || symSize == 4, CC_ERR_INVALID_ARG, "symbol size must be 2 or 4");
+    CCA_LAUNCH("dec_init", 32, k_dec_init, dim3(1), dim3(64), 0, st, d_state);
+    CCA_LAUNCH_CHECK();
+    const uint32_t nb = std::max(nseq, ncalls);
+    if (nb == 0) return;
+    CCA_LAUNCH("dec_bind", (uint64_t)nseq * 2 * sizeof(DecSeq), k_dec_bind, dim3(div_up(nb, 64)), dim3(64), 0, st,
+               d_idx_seqs, d_bind, nseq, d_items, d_seqs, rl_arena, d_idx_out, d_call_brick, ncalls, d_out);
+    CCA_LAUNCH_CHECK();
+    const uint64_t outb = total_symbols * symSize;
+    if (symSize == 2)
+        launch_blocks_rle<uint16_t>(d_items, nitems, d_seqs, p, idx_scratch, d_state, d_wave_seq, d_wave_first, nwaves,
+                                    outb, st);
+    else
+        launch_blocks_rle<uint32_t>(d_items, nitems, d_seqs, p, idx_scratch, d_state, d_wave_seq, d_wave_first, nwaves,
+                                    outb, st);
     CCA_LAUNCH("dec_finish", 0, k_dec_finish, dim3(div_up(ncalls, 64)), dim3(64), 0, st, d_calls, ncalls, d_items,
                d_seqs, d_out);
     CCA_LAUNCH_CHECK();

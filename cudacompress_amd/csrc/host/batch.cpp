@@ -246,6 +246,149 @@ void decode_finish(DeviceCoder& D, const DecodeJob& job, std::vector<DecCallOut>
                         (const DecCallOut*)(r + sizeof(CoderState)) + job.ncalls);
 }
 
+void decode_index_build(cc_instance* I, DecodeIndex& X, const std::vector<ItemSpec>& items,
+                        const std::vector<CallWords>& calls, int symSize, hipStream_t st) {
+    check_symbols(I, symSize);
+    const uint32_t ncalls = (uint32_t)calls.size();
+    const uint32_t nitems = (uint32_t)items.size();
+    const uint32_t cbs = I->cbs;
+    X.call_first_seq.assign(ncalls, 0);
+    X.call_nseq.assign(ncalls, 0);
+    uint32_t nseq = 0;
+    uint64_t arena = 64;
+    for (const ItemSpec& it : items) {
+        const uint32_t q = it.rl ? 2 : 1;
+        nseq += q;
+        // block offsets, lookup + symbol table (nsym <= count <= n); no RLE decode buffers
+        arena += q * ((uint64_t)div_up(it.n, cbs) + 16 + (1u << kDecLutBits) + it.n + 17);
+    }
+    const size_t o_items = align_up(sizeof(DecCall) * std::max<uint32_t>(ncalls, 1), 256);
+    const size_t total = o_items + sizeof(DecItem) * std::max<uint32_t>(nitems, 1);
+    unsigned char* h = (unsigned char*)X.h_stage.get(total);
+    DecCall* hc = (DecCall*)h;
+    DecItem* hi = (DecItem*)(h + o_items);
+    for (uint32_t c = 0; c < ncalls; c++) hc[c] = DecCall{calls[c].words, calls[c].nwords, 0, 0};
+    uint32_t seq = 0, call = 0;
+    for (uint32_t i = 0; i < nitems; i++) {
+        const ItemSpec& it = items[i];
+        CCA_REQUIRE(it.call >= call && it.call < ncalls, CC_ERR_INVALID_ARG, "items must be grouped by call in order");
+        if (i == 0 || it.call != call) {
+            hc[it.call].first_item = i;
+            X.call_first_seq[it.call] = seq;
+        }
+        call = it.call;
+        hc[it.call].nitems++;
+        hi[i] = DecItem{nullptr, it.n, it.rl ? 1u : 0u, seq, it.call};
+        seq += it.rl ? 2 : 1;
+        X.call_nseq[it.call] += it.rl ? 2 : 1;
+    }
+    X.nseq = nseq;
+    X.arena = arena;
+    unsigned char* d = (unsigned char*)X.d_calls.get(total);
+    X.d_seqs.as<DecSeq>(std::max<uint32_t>(nseq, 1));
+    X.d_state.as<CoderState>(1);
+    X.d_out.as<DecCallOut>(std::max<uint32_t>(ncalls, 1));
+    X.d_scratch.as<uint32_t>(arena);
+    CCA_HIP(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, st));
+    coder_decode_index((const DecCall*)d, ncalls, (const DecItem*)(d + o_items), (DecSeq*)X.d_seqs.p, nseq,
+                       params_of(I), symSize, (uint32_t*)X.d_scratch.p, arena, (CoderState*)X.d_state.p,
+                       (DecCallOut*)X.d_out.p, st);
+    X.results.resize(ncalls);
+    if (ncalls)
+        CCA_HIP(hipMemcpyAsync(X.results.data(), X.d_out.p, sizeof(DecCallOut) * ncalls, hipMemcpyDeviceToHost, st));
+    CCA_HIP(hipStreamSynchronize(st));
+}
+
+void decode_prepare_bound(cc_instance* I, DeviceCoder& D, const DecodeIndex& X, const std::vector<ItemSpec>& items,
+                          const std::vector<uint32_t>& index_calls, int symSize, BoundJob& job, hipStream_t st) {
+    check_symbols(I, symSize);
+    const uint32_t ncalls = (uint32_t)index_calls.size();
+    const uint32_t nitems = (uint32_t)items.size();
+    const uint32_t cbs = I->cbs;
+    uint32_t nseq = 0, nwaves = 0;
+    uint64_t total_syms = 0, rl_bytes = 0;
+    for (const ItemSpec& it : items) {
+        const uint32_t q = it.rl ? 2 : 1;
+        nseq += q;
+        nwaves += q * div_up(div_up(it.n, cbs), 64);
+        total_syms += it.n;
+    }
+    job.o_items = align_up(sizeof(DecCall) * std::max<uint32_t>(ncalls, 1), 256);
+    job.o_bind = job.o_items + align_up(sizeof(DecItem) * std::max<uint32_t>(nitems, 1), 256);
+    job.o_cbrick = job.o_bind + align_up(sizeof(DecBind) * std::max<uint32_t>(nseq, 1), 256);
+    job.o_wseq = job.o_cbrick + align_up(4ull * std::max<uint32_t>(ncalls, 1), 256);
+    job.o_wfirst = job.o_wseq + align_up(4ull * std::max<uint32_t>(nwaves, 1), 256);
+    const size_t total = job.o_wfirst + 4ull * std::max<uint32_t>(nwaves, 1);
+    unsigned char* h = (unsigned char*)D.h_dstage.get(total);
+    DecCall* hc = (DecCall*)h;
+    DecItem* hi = (DecItem*)(h + job.o_items);
+    DecBind* hb = (DecBind*)(h + job.o_bind);
+    uint32_t* hcb = (uint32_t*)(h + job.o_cbrick);
+    uint32_t* hws = (uint32_t*)(h + job.o_wseq);
+    uint32_t* hwf = (uint32_t*)(h + job.o_wfirst);
+    for (uint32_t c = 0; c < ncalls; c++) {
+        CCA_REQUIRE(index_calls[c] < X.call_first_seq.size(), CC_ERR_INVALID_ARG, "index call out of range");
+        hc[c] = DecCall{nullptr, 0, 0, 0};
+        hcb[c] = index_calls[c];
+    }
+    uint32_t seq = 0, wv = 0, call = 0, local = 0;
+    for (uint32_t i = 0; i < nitems; i++) {
+        const ItemSpec& it = items[i];
+        CCA_REQUIRE(it.call >= call && it.call < ncalls, CC_ERR_INVALID_ARG, "items must be grouped by call in order");
+        if (i == 0 || it.call != call) {
+            hc[it.call].first_item = i;
+            local = 0;
+        }
+        call = it.call;
+        hc[it.call].nitems++;
+        CCA_REQUIRE(it.ptr != nullptr || it.n == 0, CC_ERR_INVALID_ARG, "null output stream");
+        hi[i] = DecItem{it.ptr, it.n, it.rl ? 1u : 0u, seq, it.call};
+        const uint32_t nblk = div_up(it.n, cbs);
+        for (uint32_t q = 0; q < (it.rl ? 2u : 1u); q++) {
+            CCA_REQUIRE(local < X.call_nseq[index_calls[it.call]], CC_ERR_INVALID_ARG,
+                        "request items do not match the index");
+            uint64_t off = ~0ull;
+            if (it.rl) {
+                off = rl_bytes;
+                rl_bytes += align_up((uint64_t)it.n * symSize + 4, 64);
+            }
+            hb[seq + q] = DecBind{X.call_first_seq[index_calls[it.call]] + local, i, off};
+            local++;
+            for (uint32_t b = 0; b < nblk; b += 64) {
+                hws[wv] = seq + q;
+                hwf[wv] = b;
+                wv++;
+            }
+        }
+        seq += it.rl ? 2 : 1;
+    }
+    job.nitems = nitems;
+    job.nseq = nseq;
+    job.ncalls = ncalls;
+    job.nwaves = nwaves;
+    job.total_syms = total_syms;
+    job.rl_bytes = rl_bytes;
+    job.symSize = symSize;
+    job.p = params_of(I);
+    D.d_dcalls.get(total);
+    D.d_dseqs.as<DecSeq>(std::max<uint32_t>(nseq, 1));
+    D.d_dstate.as<CoderState>(1);
+    D.d_dout.as<DecCallOut>(std::max<uint32_t>(ncalls, 1));
+    D.d_dscratch.get(std::max<uint64_t>(rl_bytes, 64));
+    D.h_dresult.get(sizeof(CoderState) + sizeof(DecCallOut) * std::max<uint32_t>(ncalls, 1));
+    CCA_HIP(hipMemcpyAsync(D.d_dcalls.p, h, total, hipMemcpyHostToDevice, st));
+}
+
+void decode_launch_bound(DeviceCoder& D, const DecodeIndex& X, const BoundJob& job, hipStream_t st) {
+    const unsigned char* d = (const unsigned char*)D.d_dcalls.p;
+    coder_decode_bound((const DecSeq*)X.d_seqs.p, (const uint32_t*)X.d_scratch.p, (const DecCallOut*)X.d_out.p,
+                       (const DecBind*)(d + job.o_bind), (const uint32_t*)(d + job.o_cbrick), (const DecCall*)d,
+                       job.ncalls, (const DecItem*)(d + job.o_items), job.nitems, (DecSeq*)D.d_dseqs.p, job.nseq,
+                       job.p, job.symSize, (unsigned char*)D.d_dscratch.p, (CoderState*)D.d_dstate.p,
+                       (DecCallOut*)D.d_dout.p, (const uint32_t*)(d + job.o_wseq),
+                       (const uint32_t*)(d + job.o_wfirst), job.nwaves, job.total_syms, st);
+}
+
 void decode_calls_from_encode(DeviceCoder& D, const DecodeJob& job, const CallOut* d_enc, const uint32_t* base,
                               uint64_t cap, hipStream_t st) {
     coder_calls_from_encode((DecCall*)D.d_dcalls.p, d_enc, base, cap, job.ncalls, st);

@@ -54,6 +54,34 @@ void decode_finish(DeviceCoder& D, const DecodeJob& job, std::vector<DecCallOut>
 void decode_calls_from_encode(DeviceCoder& D, const DecodeJob& job, const CallOut* d_enc, const uint32_t* base,
                               uint64_t cap, hipStream_t st);
 
+// Resident decode index (serving): every call's parsed headers, decode tables and block offsets,
+// built once; a bound decode copies the sequences of the calls it serves and decodes them with no
+// parse and no table build.  Calls of the index are referred to by number.
+struct DecodeIndex {
+    DevBuf d_calls, d_seqs, d_scratch, d_out, d_state;
+    PinBuf h_stage;
+    std::vector<uint32_t> call_first_seq;  // first index sequence of every call
+    std::vector<uint32_t> call_nseq;
+    std::vector<DecCallOut> results;       // parse status of every call
+    uint64_t arena = 0;
+    uint32_t nseq = 0;
+};
+// Builds the index (one synchronisation).  `items` carry shapes only (ptr unused).
+void decode_index_build(cc_instance* I, DecodeIndex& X, const std::vector<ItemSpec>& items,
+                        const std::vector<CallWords>& calls, int symSize, hipStream_t st);
+struct BoundJob {
+    uint32_t nitems = 0, nseq = 0, ncalls = 0, nwaves = 0;
+    uint64_t total_syms = 0, rl_bytes = 0;
+    int symSize = 2;
+    CoderParams p{};
+    size_t o_items = 0, o_bind = 0, o_cbrick = 0, o_wseq = 0, o_wfirst = 0;
+};
+// `items` (grouped by call) are the request's outputs; request call c decodes index call
+// index_calls[c] and must have that call's item shapes.
+void decode_prepare_bound(cc_instance* I, DeviceCoder& D, const DecodeIndex& X, const std::vector<ItemSpec>& items,
+                          const std::vector<uint32_t>& index_calls, int symSize, BoundJob& job, hipStream_t st);
+void decode_launch_bound(DeviceCoder& D, const DecodeIndex& X, const BoundJob& job, hipStream_t st);
+
 // Eager forms: prepare + launch (+ finish with one synchronisation when `wait`; the encoder
 // retries with a larger arena when it ran out).
 uint64_t device_encode_items(cc_instance* I, const std::vector<ItemSpec>& items, uint32_t ncalls, int symSize,

@@ -81,7 +81,9 @@ std::vector<ItemSpec> brick_items(const BrickJob& job, unsigned char* sym) {
     const uint32_t L = job.L;
     for (uint32_t i = 0; i < job.bricks.size(); i++) {
         const uint64_t N = elems(job.bricks[i]);
-        auto ptr = [&](uint32_t k) { return (void*)(sym + (job.sym[i] + stream_off(N, L, k)) * job.symSize); };
+        auto ptr = [&](uint32_t k) {
+            return sym ? (void*)(sym + (job.sym[i] + stream_off(N, L, k)) * job.symSize) : nullptr;
+        };
         items.push_back(ItemSpec{ptr(0), (uint32_t)(N >> (3 * L)), false, i});
         uint32_t k = 1;
         for (int l = (int)L - 1; l >= 0; l--) {
@@ -182,6 +184,31 @@ void bricks_prepare(cc_instance* I, BrickCtx& ctx, BrickJob& job, bool enc, bool
     }
 }
 
+void bricks_prepare_bound(cc_instance* I, BrickCtx& ctx, BrickJob& job, const DecodeIndex& X,
+                          const std::vector<uint32_t>& index_calls, hipStream_t st) {
+    const uint32_t n = (uint32_t)job.bricks.size();
+    CCA_REQUIRE(index_calls.size() == n, CC_ERR_INVALID_ARG, "one index call per brick");
+    check_bricks(job.bricks.data(), n, job.L);
+    for (uint32_t i = 0; i < n; i++)
+        CCA_REQUIRE(job.bricks[i].dst, CC_ERR_INVALID_ARG, "null brick destination");
+    make_layout(job);
+    float* work = ctx.d_bwork.as<float>(std::max<uint64_t>(job.work_floats, 64));
+    unsigned char* sym = (unsigned char*)ctx.d_bsym.get(std::max<uint64_t>(job.sym_elems, 64) * job.symSize);
+    upload_levels(ctx, job, work, sym, st);
+    decode_prepare_bound(I, ctx.coder, X, brick_items(job, sym), index_calls, job.symSize, job.bj, st);
+    job.index = &X;
+}
+
+void bricks_index_build(cc_instance* I, DecodeIndex& X, const std::vector<cc_brick>& bricks, uint32_t L,
+                        int symSize, const std::vector<CallWords>& calls, hipStream_t st) {
+    BrickJob job;
+    job.bricks = bricks;
+    job.L = L;
+    job.symSize = symSize;
+    check_bricks(job.bricks.data(), (uint32_t)bricks.size(), L);
+    decode_index_build(I, X, brick_items(job, nullptr), calls, symSize, st);
+}
+
 void bricks_encode_launch(BrickCtx& ctx, const BrickJob& job, hipStream_t st, const LayoutLink* link) {
     const uint32_t n = (uint32_t)job.bricks.size();
     for (uint32_t l = 0; l < job.L; l++)
@@ -192,9 +219,14 @@ void bricks_encode_launch(BrickCtx& ctx, const BrickJob& job, hipStream_t st, co
 
 void bricks_decode_launch(BrickCtx& ctx, const BrickJob& job, hipStream_t st, bool from_encode) {
     const uint32_t n = (uint32_t)job.bricks.size();
-    if (from_encode)
-        decode_calls_from_encode(ctx.coder, job.dj, (const CallOut*)ctx.coder.d_calls.p, job.d_out, job.out_cap, st);
-    decode_launch(ctx.coder, job.dj, st);
+    if (job.index) {
+        decode_launch_bound(ctx.coder, *job.index, job.bj, st);
+    } else {
+        if (from_encode)
+            decode_calls_from_encode(ctx.coder, job.dj, (const CallOut*)ctx.coder.d_calls.p, job.d_out, job.out_cap,
+                                     st);
+        decode_launch(ctx.coder, job.dj, st);
+    }
     for (int l = (int)job.L - 1; l >= 0; l--)
         bricks_inverse_level(job.d_levels + (size_t)l * n, job.d_tiles + (size_t)l * 4 * (n + 1), n,
                              &job.tiles_total[4 * l], job.symSize, job.level_elems[l], st);

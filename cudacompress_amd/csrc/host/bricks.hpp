@@ -26,6 +26,8 @@ struct BrickJob {
     std::vector<uint32_t> tiles_total;     // [L][4]
     EncodeJob ej;
     DecodeJob dj;
+    BoundJob bj;                  // decode from a resident index (bricks_prepare_bound)
+    const DecodeIndex* index = nullptr;
     uint32_t* d_out = nullptr;
     uint64_t out_cap = 0;
 };
@@ -34,6 +36,13 @@ struct BrickJob {
 // encode staged in the same context
 void bricks_prepare(cc_instance* I, BrickCtx& ctx, BrickJob& job, bool enc, bool dec,
                     const std::vector<CallWords>* calls, hipStream_t st);
+// Decode of bricks whose streams are parsed in a resident index: brick i of the job is index
+// call index_calls[i] (built by bricks_index_build over the same brick shapes and levels).
+void bricks_prepare_bound(cc_instance* I, BrickCtx& ctx, BrickJob& job, const DecodeIndex& X,
+                          const std::vector<uint32_t>& index_calls, hipStream_t st);
+// Resident index of bricks stored at `calls` (one bitstream per brick).
+void bricks_index_build(cc_instance* I, DecodeIndex& X, const std::vector<cc_brick>& bricks, uint32_t L,
+                        int symSize, const std::vector<CallWords>& calls, hipStream_t st);
 bool bricks_decode_results(const BrickJob& job, const std::vector<DecCallOut>& res, uint32_t* status);
 void bricks_encode_launch(BrickCtx& ctx, const BrickJob& job, hipStream_t st, const LayoutLink* link = nullptr);
 void bricks_decode_launch(BrickCtx& ctx, const BrickJob& job, hipStream_t st, bool from_encode);

@@ -6,7 +6,9 @@
 //
 // No reference counterpart (the reference decodes one volume per call).  Each slot owns a brick
 // batch context (host/bricks.cpp), so a request decodes to exactly the values of a batch that
-// holds it alone.
+// holds it alone.  At creation every brick's headers are parsed and its decode tables and block
+// offsets built once into a resident index (the per-call work of src/cudaCompress/Encode.cpp:
+// 522-621); a batch then binds its bricks' index entries and starts at the block decode.
 #include "server.hpp"
 
 #include <algorithm>
@@ -45,6 +47,7 @@ struct cc_server {
     std::vector<uint64_t> begin, end;   // per brick word range
     std::vector<uint32_t> dims;         // per brick x, y, z
     std::vector<float> steps;
+    cca::DecodeIndex index;             // parsed headers, tables and block offsets of every brick
     uint32_t levels = 2;
     int symSize = 2;
     uint64_t slot_budget = 0;
@@ -97,7 +100,7 @@ void reap(cc_server* S, Slot& s, double now) {
 
 void launch(cc_server* S, Slot& s) {
     // admit requests in arrival order while the batch fits the slot's workspace budget
-    std::vector<CallWords> calls;
+    std::vector<uint32_t> ids;
     s.job = BrickJob();
     s.job.L = S->levels;
     s.job.symSize = S->symSize;
@@ -110,20 +113,21 @@ void launch(cc_server* S, Slot& s) {
             const uint32_t b = rq.bricks[j];
             const uint32_t* d = &S->dims[3 * b];
             s.job.bricks.push_back(cc_brick{nullptr, rq.outs[j], d[0], d[1], d[2], S->steps[b]});
-            calls.push_back(CallWords{S->words + S->begin[b], S->end[b] - S->begin[b]});
+            ids.push_back(b);
         }
         s.reqs.push_back(rq.id);
         s.req_nbricks.push_back((uint32_t)rq.bricks.size());
         S->pending.pop_front();
     }
     if (s.reqs.empty()) return;
-    bricks_prepare(S->inst, s.ctx, s.job, false, true, &calls, s.st);
+    bricks_prepare_bound(S->inst, s.ctx, s.job, S->index, ids, s.st);
     bricks_decode_launch(s.ctx, s.job, s.st, false);
     // statuses to the pinned buffer, then the completion event
     DeviceCoder& D = s.ctx.coder;
-    unsigned char* r = (unsigned char*)D.h_dresult.get(sizeof(CoderState) + sizeof(DecCallOut) * s.job.dj.ncalls);
-    CCA_HIP(hipMemcpyAsync(r + sizeof(CoderState), D.d_dout.p, sizeof(DecCallOut) * s.job.dj.ncalls,
-                           hipMemcpyDeviceToHost, s.st));
+    const uint32_t ncalls = s.job.bj.ncalls;  // one call per brick of the batch
+    unsigned char* r = (unsigned char*)D.h_dresult.get(sizeof(CoderState) + sizeof(DecCallOut) * ncalls);
+    CCA_HIP(hipMemcpyAsync(r + sizeof(CoderState), D.d_dout.p, sizeof(DecCallOut) * ncalls, hipMemcpyDeviceToHost,
+                           s.st));
     CCA_HIP(hipEventRecord(s.done, s.st));
     s.busy = true;
 }
@@ -156,6 +160,13 @@ cc_server* server_create(cc_instance* I, const uint32_t* d_words, uint32_t nbric
         CCA_HIP(hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking));
         CCA_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
     }
+    std::vector<cc_brick> shapes(nbricks);
+    std::vector<CallWords> calls(nbricks);
+    for (uint32_t i = 0; i < nbricks; i++) {
+        shapes[i] = cc_brick{nullptr, nullptr, dims[3 * i], dims[3 * i + 1], dims[3 * i + 2], steps[i]};
+        calls[i] = CallWords{d_words + S->begin[i], S->end[i] - S->begin[i]};
+    }
+    bricks_index_build(I, S->index, shapes, levels, symSize, calls, S->slots[0].st);
     return S.release();
 }
 

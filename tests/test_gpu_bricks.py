@@ -273,3 +273,45 @@ def test_server_results_same_alone_and_in_the_mix(cc, torch_gpu, field):
             if i in alone:
                 assert np.array_equal(alone[i][j], mix[i][j]), (i, j)
     inst.close()
+
+
+def test_server_index_reports_broken_bricks(cc, torch_gpu, field):
+    """The serving index parses every brick once at creation: a brick whose stored range is empty
+    or cut short fails the requests that hold it (status: corrupt stream); the other requests
+    decode to the eager batch's values."""
+    log("server index with broken bricks")
+    torch = torch_gpu
+    rng = np.random.default_rng(34)
+    bricks = make_bricks(rng, field, 8)
+    std = float(field.std())
+    steps = [0.005 * std] * len(bricks)
+    inst = cc.createInstance(0, 7, 64 * 64 * 64, 128, 14)
+    srcs, out, offs, stat = run_batch(cc, torch, inst, bricks, steps, 2)
+    assert np.all(stat == 0)
+    shapes = [b.shape for b in bricks]
+    ranges = np.stack([offs[:-1], offs[1:]], axis=1).copy()
+    ranges[2, 1] = ranges[2, 0]                               # empty
+    ranges[5, 1] = ranges[5, 0] + (ranges[5, 1] - ranges[5, 0]) // 2  # truncated
+    reqs = [[0, 1], [2], [3, 5], [4, 6, 7], [1, 2, 3]]
+    srv = cc.Server(inst, out, ranges, shapes, steps, 2, slots=2, workspaceBytes=1 << 30)
+    outs = {i: [torch.full_like(srcs[b], -1.0) for b in r] for i, r in enumerate(reqs)}
+    torch.cuda.synchronize()
+    for i, r in enumerate(reqs):
+        srv.submit(i, r, outs[i], 0.0)
+        srv.poll(0.0)
+    while srv.pending():
+        srv.poll(0.0)
+    done, _ = srv.completed()
+    failed, status = srv.failed()
+    srv.close()
+    torch.cuda.synchronize()
+    assert sorted(done.tolist()) == [0, 3]
+    assert sorted(failed.tolist()) == [1, 2, 4]
+    assert all(int(s) == cc.CC_ERR_CORRUPT_STREAM for s in status)
+    eager = [torch.zeros_like(s) for s in srcs]
+    cc.decompressBricksFloat(inst, eager, shapes, steps, 2, out, offs)
+    torch.cuda.synchronize()
+    for i in (0, 3):
+        for j, b in enumerate(reqs[i]):
+            assert np.array_equal(outs[i][j].cpu().numpy(), eager[b].cpu().numpy()), (i, j)
+    inst.close()
