@@ -436,8 +436,9 @@ def main():
         value, ms, raw, kernels, cfg, cpu, dtype = run_plain(args, torch, cc, lib)
     roof, table, gpu_ms = roofline_of(kernels, args.steps, args.workload, args.traffic_file)
     if roof["kernel"] == "code_pack" and "compressed_bytes" in cfg:
-        # the launch also writes the codewords, whose size is known only after it ran
-        per = roof["algorithmic_bytes_per_launch"] + cfg["compressed_bytes"]
+        # the launches also write the codewords (their share of the step's compressed bytes), whose
+        # size is known only after they ran
+        per = roof["algorithmic_bytes_per_launch"] + cfg["compressed_bytes"] / roof["launches_per_step"]
         roof["algorithmic_bytes_per_launch"] = int(per)
         roof["achieved"] = round(per / (roof["avg_launch_us"] * 1e-6) / 1e9, 2)
         roof["frac"] = round(roof["achieved"] / HBM_PEAK_GBS, 5)
