@@ -7,6 +7,9 @@ std, RLE+Huffman (log2 14, coding block 128).  One step = batched compression of
 one device buffer + batched decompression, inputs and bitstreams resident in HBM.  The same
 bricks are also run through the plain per-brick API (compressVolumeFloat/decompressVolumeFloat,
 host bitstreams) on a sample, and the batched bitstreams are checked against the plain ones.
+Steps are replays of hipGraph plans; three plans of the same bricks (own outputs, bitstreams and
+streams) take the steps in turn, so one step's serial table designs overlap the next step's
+streaming passes (`--inflight`; every plan's words and reconstructions are checked equal).
 
 Serving (on by default after the bricks run, alone with `--workload serving`): a 1024^3 k^-5/3
 field (seed 5, generated on the device), 729 overlapping 128^3 level-0 bricks (stride 120) plus
@@ -34,6 +37,11 @@ import os
 import sys
 import time
 
+# Every plan runs on its own stream with four side streams for the table-design classes; HIP maps
+# streams onto hardware queues (4 by default), and streams sharing a queue serialise.  Sixteen
+# queues let the in-flight plans' streams run concurrently.  Set before HIP initialises.
+os.environ["GPU_MAX_HW_QUEUES"] = "16"
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -55,9 +63,12 @@ def parse():
     ap.add_argument("--serving-rates", default="1000,5000,10000,20000,50000", help="arrival rates, bricks/s")
     ap.add_argument("--bricks", type=int, default=2048)
     ap.add_argument("--groups", type=int, default=0, help="brick groups of eager batches (0: library default)")
+    ap.add_argument("--inflight", type=int, default=3,
+                    help="plan mode: plans (own buffers and streams) whose steps run concurrently; step i "
+                         "launches plan i %% inflight after waiting for that plan's previous step")
     ap.add_argument("--mode", default="plan", choices=["plan", "eager"],
-                    help="plan: compression and decompression replayed from hipGraphs with one host "
-                         "synchronisation per step (BricksPlan.run); "
+                    help="plan: compression and decompression replayed from hipGraphs (BricksPlan.launch, "
+                         "waited for before the plan's next step); "
                          "eager: compressBricksFloat + decompressBricksFloat calls")
     ap.add_argument("--plain-sample", type=int, default=256, help="bricks timed through the plain per-brick API")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -135,14 +146,36 @@ def run_bricks(args, torch, cc, lib):
         inst.setBrickGroups(args.groups)
     state = {}
     plan = cc.BricksPlan(inst, srcs, dsts, shapes, steps_q, 2, out) if args.mode == "plan" else None
+    depth = max(1, args.inflight) if plan is not None else 1
+    # further plans for concurrent steps: the same bricks into their own outputs and bitstreams
+    extra = []
+    for _ in range(depth - 1):
+        d_k = [torch.empty_like(s) for s in srcs]
+        o_k = torch.empty_like(out)
+        extra.append((cc.BricksPlan(inst, srcs, d_k, shapes, steps_q, 2, o_k), d_k, o_k))
+    plans = [plan] + [e[0] for e in extra] if plan is not None else []
+    pending = [False] * depth
+    results = [None] * depth
+    state["i"] = 0
 
     def step():
-        if plan is not None:
-            offs, stat, dstat = plan.run()
+        if plans:
+            k = state["i"] % depth
+            if pending[k]:
+                results[k] = plans[k].wait()
+            plans[k].launch()
+            pending[k] = True
+            state["i"] += 1
         else:
             offs, stat = cc.compressBricksFloat(inst, srcs, shapes, steps_q, 2, out)
             dstat = cc.decompressBricksFloat(inst, dsts, shapes, steps_q, 2, out, offs)
-        state["offs"], state["stat"], state["dstat"] = offs, stat, dstat
+            results[0] = (offs, stat, dstat)
+
+    def drain():
+        for k in range(len(plans)):
+            if pending[k]:
+                results[k] = plans[k].wait()
+                pending[k] = False
 
     if args.copy_calibration:
         a = torch.empty(256 << 20, dtype=torch.uint8, device=dev)
@@ -154,7 +187,17 @@ def run_bricks(args, torch, cc, lib):
         step()
     torch.cuda.synchronize()
     ms, kernels = timed(torch, lib, step, args.steps)
+    drain()
     value = 2.0 * raw_bytes / (ms * 1e-3) / 1e9
+    state["offs"], state["stat"], state["dstat"] = results[0]
+    concurrent_equal = None
+    if extra:
+        # every plan compressed the same bricks: identical words and reconstructions
+        n_w = int(results[0][0][-1])
+        concurrent_equal = all(np.array_equal(r[0], results[0][0]) for r in results[1:]) and all(
+            torch.equal(o_k[:n_w], out[:n_w]) and all(torch.equal(a, b) for a, b in zip(d_k, dsts))
+            for _, d_k, o_k in extra)
+        assert concurrent_equal, "concurrent plans disagree"
     kernel_source = "HIP events around every kernel inside the timed steps"
     if not kernels:  # graph replays carry no per-kernel events: time the same kernels in eager calls
         def eager():
@@ -226,8 +269,9 @@ def run_bricks(args, torch, cc, lib):
                "sample": f"the first {runs} bricks ({done_bytes} raw bytes) compressed and decompressed one by one "
                          f"through the oracle (C++ restatement of the reference path, one host thread), {t_total:.2f} s"}
 
-    if plan is not None:
-        plan.close()
+    for p in plans:
+        p.close()
+    del extra
     inst.close()
     cfg = {"workload": "C4 batched: %d float32 bricks, axes from {16,32,48,64,96,128} (seeded), cut from seeded "
                        "k^-5/3 fields; 2-level 3D CDF 9/7, deadzone step 0.005*std(brick), RLE+Huffman log2 14, "
@@ -235,7 +279,10 @@ def run_bricks(args, torch, cc, lib):
            "bricks": len(bricks), "raw_bytes_per_direction": raw_bytes, "compressed_bytes": compressed,
            "compression_ratio": round(raw_bytes / compressed, 3), "psnr_db": round(float(psnr), 3),
            "worst_brick_psnr_db": round(float(worst), 3),
-           "mode": "batched, " + ("hipGraph plan replays" if args.mode == "plan" else "eager batch calls"),
+           "mode": "batched, " + (("hipGraph plan replays, %d plans in flight (each step one full compression + "
+                                   "decompression of the batch)" % depth) if args.mode == "plan"
+                                  else "eager batch calls"),
+           "concurrent_plans_equal": concurrent_equal,
            "kernel_timing": kernel_source,
            "plain": plain, "batched_vs_plain": round(value / plain["GBps"], 2) if plain else None,
            "bitstreams_equal_plain_sample": equal}

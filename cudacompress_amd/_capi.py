@@ -101,6 +101,8 @@ SIGNATURES = {
     "cc_bricks_plan_compress": (c_int, [c_vp, P_u64, P_u32]),
     "cc_bricks_plan_decompress": (c_int, [c_vp, P_u32]),
     "cc_bricks_plan_run": (c_int, [c_vp, P_u64, P_u32, P_u32]),
+    "cc_bricks_plan_launch": (c_int, [c_vp]),
+    "cc_bricks_plan_wait": (c_int, [c_vp, P_u64, P_u32, P_u32]),
     "cc_bricks_plan_destroy": (None, [c_vp]),
     "cc_server_create": (c_int, [c_vp, c_vp, c_u32, P_u64, P_u32, ctypes.POINTER(c_float), c_u32, c_int, c_u32, c_u64,
                                  ctypes.POINTER(c_vp)]),

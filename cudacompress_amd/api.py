@@ -459,6 +459,21 @@ class BricksPlan:
             _check(st, "cc_bricks_plan_run")
         return offs, cst[:self._n], dst[:self._n]
 
+    def launch(self):
+        """Enqueues compression and decompression on the plan's stream and returns at once."""
+        _check(self._lib.cc_bricks_plan_launch(self.handle), "cc_bricks_plan_launch")
+
+    def wait(self):
+        """Waits for the last launch; returns (offsets, compress status, decompress status)."""
+        offs = np.zeros(self._n + 1, dtype=np.uint64)
+        cst = np.zeros(max(self._n, 1), dtype=np.uint32)
+        dst = np.zeros(max(self._n, 1), dtype=np.uint32)
+        st = self._lib.cc_bricks_plan_wait(self.handle, offs.ctypes.data_as(_capi.P_u64),
+                                           cst.ctypes.data_as(_capi.P_u32), dst.ctypes.data_as(_capi.P_u32))
+        if st not in (CC_OK, CC_ERR_SYMBOL_OVERFLOW, CC_ERR_CODEWORD_TOO_LONG):
+            _check(st, "cc_bricks_plan_wait")
+        return offs, cst[:self._n], dst[:self._n]
+
     def close(self):
         if self.handle:
             self._lib.cc_bricks_plan_destroy(self.handle)

@@ -430,6 +430,7 @@ struct cc_bricks_plan {
     hipGraphExec_t x_enc = nullptr, x_dec = nullptr;
     hipEvent_t order = nullptr;  // orders replays after the instance stream's earlier work
     bool encoded = false;
+    bool launched = false;     // a plan_launch not yet waited for
     ~cc_bricks_plan();
 };
 
@@ -533,15 +534,27 @@ void plan_compress(cc_bricks_plan* P, uint64_t* offsets, uint32_t* status) {
     plan_encode_results(P, offsets, status);
 }
 
-void plan_run(cc_bricks_plan* P, uint64_t* offsets, uint32_t* cstatus, uint32_t* dstatus) {
+void plan_launch(cc_bricks_plan* P) {
     DeviceGuard guard(P->inst->device);
     CCA_HIP(hipEventRecord(P->order, P->inst->stream));
     CCA_HIP(hipStreamWaitEvent(P->st, P->order, 0));
     CCA_HIP(hipGraphLaunch(P->x_enc, P->st));
     CCA_HIP(hipGraphLaunch(P->x_dec, P->st));  // stream positions from the device-side results
+    P->launched = true;
+}
+
+void plan_wait(cc_bricks_plan* P, uint64_t* offsets, uint32_t* cstatus, uint32_t* dstatus) {
+    DeviceGuard guard(P->inst->device);
+    CCA_REQUIRE(P->launched, CC_ERR_INVALID_ARG, "the plan has no launch to wait for");
+    P->launched = false;
     CCA_HIP(hipStreamSynchronize(P->st));
     plan_encode_results(P, offsets, cstatus);
     plan_decode_results(P, dstatus);
+}
+
+void plan_run(cc_bricks_plan* P, uint64_t* offsets, uint32_t* cstatus, uint32_t* dstatus) {
+    plan_launch(P);
+    plan_wait(P, offsets, cstatus, dstatus);
 }
 
 void plan_decompress(cc_bricks_plan* P, uint32_t* status) {

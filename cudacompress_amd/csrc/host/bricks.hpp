@@ -76,6 +76,8 @@ cc_bricks_plan* plan_create(cc_instance* I, const cc_brick* bricks, uint32_t n, 
 void plan_compress(cc_bricks_plan* P, uint64_t* offsets, uint32_t* status);
 void plan_decompress(cc_bricks_plan* P, uint32_t* status);
 void plan_run(cc_bricks_plan* P, uint64_t* offsets, uint32_t* cstatus, uint32_t* dstatus);
+void plan_launch(cc_bricks_plan* P);
+void plan_wait(cc_bricks_plan* P, uint64_t* offsets, uint32_t* cstatus, uint32_t* dstatus);
 void plan_destroy(cc_bricks_plan* P);
 
 }  // namespace cca

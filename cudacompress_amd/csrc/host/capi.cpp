@@ -244,6 +244,21 @@ cc_status cc_bricks_plan_run(cc_bricks_plan* plan, uint64_t* brickWordOffsets, u
     });
 }
 
+cc_status cc_bricks_plan_launch(cc_bricks_plan* plan) {
+    return guarded([&] {
+        CCA_REQUIRE(plan, CC_ERR_INVALID_ARG, "null argument");
+        cca::plan_launch(plan);
+    });
+}
+
+cc_status cc_bricks_plan_wait(cc_bricks_plan* plan, uint64_t* brickWordOffsets, uint32_t* compressStatus,
+                              uint32_t* decompressStatus) {
+    return guarded([&] {
+        CCA_REQUIRE(plan && brickWordOffsets, CC_ERR_INVALID_ARG, "null argument");
+        cca::plan_wait(plan, brickWordOffsets, compressStatus, decompressStatus);
+    });
+}
+
 void cc_bricks_plan_destroy(cc_bricks_plan* plan) { cca::plan_destroy(plan); }
 
 cc_status cc_server_create(cc_instance* inst, const uint32_t* d_words, uint32_t brickCount,

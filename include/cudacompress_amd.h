@@ -378,6 +378,13 @@ CC_API cc_status cc_bricks_plan_decompress(cc_bricks_plan* plan, uint32_t* brick
  * decompressStatus like cc_bricks_plan_decompress.  A compression error is reported first. */
 CC_API cc_status cc_bricks_plan_run(cc_bricks_plan* plan, uint64_t* brickWordOffsets, uint32_t* compressStatus,
                                     uint32_t* decompressStatus);
+/* cc_bricks_plan_run split in two: _launch enqueues compression and decompression on the plan's
+ * stream and returns at once; _wait synchronises that stream and fills the results of the last
+ * launch.  Plans have their own streams and buffers, so launches of different plans run
+ * concurrently (one plan's serial table designs overlap another's streaming passes). */
+CC_API cc_status cc_bricks_plan_launch(cc_bricks_plan* plan);
+CC_API cc_status cc_bricks_plan_wait(cc_bricks_plan* plan, uint64_t* brickWordOffsets, uint32_t* compressStatus,
+                                     uint32_t* decompressStatus);
 CC_API void cc_bricks_plan_destroy(cc_bricks_plan* plan);
 
 /* ---- Decode serving ----------------------------------------------------------------------- */

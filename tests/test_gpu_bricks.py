@@ -177,6 +177,23 @@ def test_bricks_plan_graph_replays(cc, torch_gpu, field):
         assert np.array_equal(out[: int(offs3[-1])].cpu().numpy().view(np.uint32), pw), rep
         for a, b in zip(prec, dsts):
             assert np.array_equal(a, b.cpu().numpy()), rep
+    # a second plan of the same bricks, both launched before either is waited for
+    dsts_b = [torch.zeros_like(s) for s in srcs]
+    out_b = torch.zeros_like(out)
+    plan_b = cc.BricksPlan(inst, srcs, dsts_b, shapes, steps, 2, out_b)
+    torch.cuda.synchronize()
+    for _ in range(2):
+        plan.launch()
+        plan_b.launch()
+        ra, rb = plan.wait(), plan_b.wait()
+        for x, y in zip(ra, rb):
+            assert np.array_equal(x, y)
+        assert np.all(ra[1] == 0) and np.all(ra[2] == 0)
+        nw = int(ra[0][-1])
+        assert torch.equal(out[:nw], out_b[:nw])
+        for a, b in zip(dsts, dsts_b):
+            assert torch.equal(a, b)
+    plan_b.close()
     plan.close()
     inst.close()
 
