@@ -52,6 +52,20 @@ def test_invalid_arguments_rejected_before_device_use():
     # DWT size checks run on the host
     assert lib.cc_dwt_int_forward(None, None, None, 7, 8, 1, 0, 0, 0, 0, None) == 3
     assert b"even" in lib.cc_last_error()
+    # image path, bit planes, lowpass-only transforms, multi-channel volumes, timing, plans
+    assert lib.cc_dwt_int_forward_lowpass_only(None, None, None, 7, 8, 0, 0, None) == 3
+    assert lib.cc_dwt_float_forward_lowpass_only_from_byte(None, None, None, 6, 8, 5, 0, 0, None) == 3
+    assert lib.cc_float_to_byte_2d(None, 3, 3, None, 8, 8, 0, None) == 3
+    assert b"channel" in lib.cc_last_error()
+    assert lib.cc_quantize_difference_to_symbols_2d(None, None, 1.0, None, 1, 0, 8, 8, 0, 2, None) == 3
+    assert lib.cc_convert_rgb_to_ycocg(None, None, 10, 5, None) == 3
+    assert lib.cc_convert_ycocg_to_rgb(None, None, -1, 3, None) == 3
+    assert lib.cc_get_lsb(None, None, 100, None) == 3
+    assert lib.cc_get_lsb(None, None, 0, None) == 0  # nothing to do
+    assert lib.cc_append_lsb(None, None, 100, 0, None) == 3
+    assert lib.cc_decompress_volume_float_multichannel(None, None, 0, 8, 8, 8, 1, 0, 2) == 3
+    assert lib.cc_set_timing_detail(None, 1) == 3
+    assert lib.cc_bricks_plan_launch(None) == 3
 
 
 def test_dropin_headers_compile():
