@@ -351,6 +351,22 @@ __global__ __launch_bounds__(64) void k_dec_blocks(const DecSeq* seqs, const uin
 // output for the last chunk: the trailing zeros), then stores its values there, staged in LDS and
 // issued with a 256-pair stride so that one store instruction writes 64 consecutive pairs.  The
 // span's lines are still in L2 when the values land, so each output line leaves L2 once.
+// Eight consecutive symbols from a 16-byte aligned address.
+__device__ __forceinline__ void load8_aligned(const uint16_t* p, uint32_t (&o)[8]) {
+    const uint4 w = *reinterpret_cast<const uint4*>(p);
+    const uint32_t a[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        o[2 * k] = a[k] & 0xFFFFu;
+        o[2 * k + 1] = a[k] >> 16;
+    }
+}
+__device__ __forceinline__ void load8_aligned(const uint32_t* p, uint32_t (&o)[8]) {
+    const uint4 w0 = reinterpret_cast<const uint4*>(p)[0], w1 = reinterpret_cast<const uint4*>(p)[1];
+    o[0] = w0.x; o[1] = w0.y; o[2] = w0.z; o[3] = w0.w;
+    o[4] = w1.x; o[5] = w1.y; o[6] = w1.z; o[7] = w1.w;
+}
+
 template <class S>
 __device__ __forceinline__ void clear_symbols(S* out, uint32_t n) {
     const uintptr_t a = reinterpret_cast<uintptr_t>(out);
@@ -384,15 +400,26 @@ __global__ __launch_bounds__(256) void k_dec_rle(const DecItem* items, DecSeq* s
     auto sw = [](uint32_t i) { return i ^ ((i >> 6) & 7u); };
     uint64_t carry = 0;
     bool bad = false;
+    // the decode buffers are 64-byte aligned arena ranges: a thread's 8 pairs are two 16-byte
+    // (Symbol16) or four (Symbol32) loads
+    const bool vec = ((reinterpret_cast<uintptr_t>(sym) | reinterpret_cast<uintptr_t>(zc)) & 15u) == 0;
     for (uint32_t base = 0;; base += kChunk) {
         const bool last = (uint64_t)base + kChunk >= M;
         uint32_t z[8], v[8], sum = 0;
+        const uint32_t i0 = base + threadIdx.x * 8;
+        if (vec && i0 + 8 <= M) {
+            load8_aligned(zc + i0, z);
+            load8_aligned(sym + i0, v);
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const uint32_t i = base + threadIdx.x * 8 + k;
-            z[k] = i < M ? (uint32_t)zc[i] : 0u;
-            v[k] = i < M ? (uint32_t)sym[i] : 0u;
-            sum += i < M ? z[k] + 1 : 0u;
+            for (int k = 0; k < 8; k++) sum += z[k] + 1;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const uint32_t i = i0 + k;
+                z[k] = i < M ? (uint32_t)zc[i] : 0u;
+                v[k] = i < M ? (uint32_t)sym[i] : 0u;
+                sum += i < M ? z[k] + 1 : 0u;
+            }
         }
         uint32_t tot;
         const uint32_t ex = block_excl_scan<256>(sum, sh, tot);
