@@ -207,10 +207,12 @@ __global__ __launch_bounds__(256) void k_dec_tables(DecSeq* seqs, CoderParams p,
     }
     __syncthreads();
     if (threadIdx.x < 96) s.tab[threadIdx.x] = tab_sh[threadIdx.x];
-    for (uint32_t e = threadIdx.x; e < (1u << kLut); e += 256) {
+    // the table is indexed by the next lb = min(longest code, kLut) bits: 2^lb entries
+    const uint32_t lb = s.nlen < kLut ? s.nlen : kLut;
+    for (uint32_t e = threadIdx.x; e < (1u << lb); e += 256) {
         uint32_t entry = 0;
-        for (int l = 1; l <= (int)kLut && l <= (int)s.nlen; l++) {
-            const int cw = (int)(e >> (kLut - l));
+        for (int l = 1; l <= (int)lb; l++) {
+            const int cw = (int)(e >> (lb - l));
             if (cw <= tab_sh[64 + l - 1]) {
                 const int idx = tab_sh[l - 1] + cw - tab_sh[32 + l - 1];
                 const uint32_t sym = (idx >= 0 && (uint32_t)idx < s.nsym && idx < (1 << kLut)) ? sym_lo[idx] : 0u;
@@ -271,16 +273,20 @@ __global__ __launch_bounds__(64) void k_dec_blocks(const DecSeq* seqs, const uin
     const uint32_t nsym = s.nsym;
     const uint32_t* gsyms = scratch + s.lut_off + (1u << kLut);
     auto symbol_at = [&](int idx) -> uint32_t { return (idx >= 0 && (uint32_t)idx < nsym) ? gsyms[idx] : 0u; };
+    // lookup table: 2^lb entries, indexed by the next lb bits (lb = min(longest code, kLut))
+    const uint32_t lb = s.nlen < kLut ? s.nlen : kLut;
     if (multi) {
         // all of the lane's table loads in flight at once, then the LDS stores
         const uint32_t* glut = scratch + s.lut_off;
         constexpr int kPer = (1 << kLut) / 64;
+        const uint32_t nl = 1u << lb;
         uint32_t lv[kPer];
 #pragma unroll
-        for (int k = 0; k < kPer; k++) lv[k] = glut[lane + 64 * k];
+        for (int k = 0; k < kPer; k++) lv[k] = (uint32_t)(lane + 64 * k) < nl ? glut[lane + 64 * k] : 0u;
         const int t0 = s.tab[lane], t1 = lane < 32 ? s.tab[64 + lane] : 0;
 #pragma unroll
-        for (int k = 0; k < kPer; k++) lut[lane + 64 * k] = lv[k];
+        for (int k = 0; k < kPer; k++)
+            if ((uint32_t)(lane + 64 * k) < nl) lut[lane + 64 * k] = lv[k];
         tab[lane] = t0;
         if (lane < 32) tab[64 + lane] = t1;
         __syncthreads();
@@ -318,7 +324,7 @@ __global__ __launch_bounds__(64) void k_dec_blocks(const DecSeq* seqs, const uin
                     bp -= 32;
                 }
                 const uint32_t peek = (uint32_t)(cache >> (32 - bp));
-                const uint32_t e = lut[peek >> (32 - kLut)];
+                const uint32_t e = lut[peek >> (32 - lb)];
                 uint32_t len = e >> 24;
                 sym = e & 0xFFFFFFu;
                 if (e == 0u) {
