@@ -314,7 +314,7 @@ __global__ __launch_bounds__(64) void k_dec_blocks(const DecSeq* seqs, const uin
     const uint32_t mlen = s.nlen;
     for (uint32_t r = 0; r < cbs; r += 32) {
         // every lane steps through the whole chunk; steps past its block's end store nothing
-#pragma unroll 4
+#pragma unroll 2
         for (uint32_t i = 0; i < 32; i++) {
             uint32_t sym = single;
             if (multi) {
