@@ -367,7 +367,15 @@ cc_status cc_set_timing_detail(cc_instance* inst, int detail) {
     return guarded([&] {
         CCA_REQUIRE(inst && detail >= 0 && detail <= 3, CC_ERR_INVALID_ARG, "bad timing detail");
         inst->timing_detail = detail;
-        if (detail >= 3) cc_profile_enable(1);
+        // the per-kernel timer is process-wide: switched on for level 3, off again when the
+        // instance that switched it on leaves level 3
+        if (detail >= 3 && !inst->timing_kernels) {
+            cc_profile_enable(1);
+            inst->timing_kernels = true;
+        } else if (detail < 3 && inst->timing_kernels) {
+            cc_profile_enable(0);
+            inst->timing_kernels = false;
+        }
     });
 }
 cc_status cc_get_timings(cc_instance* inst, uint32_t* count) {

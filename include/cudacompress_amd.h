@@ -132,7 +132,8 @@ CC_API cc_status cc_decode_device(cc_instance* inst, int rl, int symbolSize, con
 /* ---- Timing: include/cudaCompress/Timing.h:15-27 -------------------------------------------- */
 /* setTimingDetail: 0 none, 1 whole coder calls ("Encode RLHuff", ...), 2 also their stages
  * ("Encode: Histograms", ...), 3 also every kernel launch ("Kernel: <name>"; kernel timing is
- * process-wide, cc_profile_enable).  Sections are hipEvent pairs on the instance stream. */
+ * process-wide, cc_profile_enable: switched on by the instance that enters level 3 and off when it
+ * leaves it).  Sections are hipEvent pairs on the instance stream. */
 CC_API cc_status cc_set_timing_detail(cc_instance* inst, int detail);
 /* getTimings: waits for the recorded sections and snapshots (name, accumulated ms) pairs, read
  * with cc_get_timing_entry(i) until the next cc_get_timings call. */

@@ -277,6 +277,13 @@ def test_timing_sections(cc, torch_gpu):
         assert np.array_equal(to_host(outs[i]), to_host(streams[i]))
     cc.resetTimings(inst)
     assert cc.getTimings(inst) == ([], [])
+    # level HIGH adds the kernels of the calls
+    cc.setTimingDetail(inst, cc.TIMING_DETAIL_HIGH)
+    assert cc.encodeRLHuff(inst, cc.BitStream(), streams, n)
+    torch.cuda.synchronize()
+    names, times = cc.getTimings(inst)
+    assert "Encode RLHuff" in names and any(a.startswith("Kernel: ") for a in names)
+    cc.resetTimings(inst)
     cc.setTimingDetail(inst, cc.TIMING_DETAIL_NONE)
     assert cc.encodeRLHuff(inst, cc.BitStream(), streams, n)
     assert cc.getTimings(inst) == ([], [])
