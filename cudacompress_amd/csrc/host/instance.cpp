@@ -105,6 +105,7 @@ cc_instance::~cc_instance() {
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
     else (void)hipDeviceSynchronize();
+    if (timing_kernels) cc_profile_enable(0);  // this instance switched the kernel timer on
     for (auto& e : dec_event)
         if (e) (void)hipEventDestroy(e);
     cca::DevBuf* devs[] = {&d_rle_sym, &d_rle_zc, &d_desc, &d_tiles, &d_map, &d_map2, &d_results, &d_hist,
