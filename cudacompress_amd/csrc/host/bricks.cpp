@@ -567,5 +567,6 @@ void plan_decompress(cc_bricks_plan* P, uint32_t* status) {
 }
 
 void plan_destroy(cc_bricks_plan* P) { delete P; }
+const void* plan_owner(const cc_bricks_plan* P) { return P ? P->inst : nullptr; }
 
 }  // namespace cca

@@ -79,5 +79,7 @@ void plan_run(cc_bricks_plan* P, uint64_t* offsets, uint32_t* cstatus, uint32_t*
 void plan_launch(cc_bricks_plan* P);
 void plan_wait(cc_bricks_plan* P, uint64_t* offsets, uint32_t* cstatus, uint32_t* dstatus);
 void plan_destroy(cc_bricks_plan* P);
+// the instance a plan belongs to (its launches are that instance's in the kernel timing)
+const void* plan_owner(const cc_bricks_plan* P);
 
 }  // namespace cca

@@ -41,12 +41,19 @@ cc_status guarded(F&& f) {
     }
 }
 
+// guarded() with the launches of the call tagged as the instance's (its kernel timing entries).
+template <class F>
+cc_status guarded_for(const void* owner, F&& f) {
+    cca::ProfileOwner tag(owner);
+    return guarded(std::forward<F>(f));
+}
+
 hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 cc_status encode_entry(cc_instance* inst, bool rl, int symbolSize, const void* const* streams, uint32_t ns,
                        uint32_t n, const uint32_t* startBits, const uint32_t** outWords,
                        const uint64_t** outSegBegin, const uint64_t** outSegBits) {
-    return guarded([&] {
+    return guarded_for(inst, [&] {
         CCA_REQUIRE(inst && outWords && outSegBegin && outSegBits, CC_ERR_INVALID_ARG, "null argument");
         cca::encode_streams(inst, rl, symbolSize, streams, ns, n, startBits);
         *outWords = inst->out_words.data();
@@ -57,7 +64,7 @@ cc_status encode_entry(cc_instance* inst, bool rl, int symbolSize, const void* c
 
 cc_status decode_entry(cc_instance* inst, bool rl, int symbolSize, const uint32_t* const* words, const uint64_t* bits,
                        uint64_t* pos, uint32_t nbs, const uint32_t* s2b, void* const* outs, uint32_t ns, uint32_t n) {
-    return guarded([&] {
+    return guarded_for(inst, [&] {
         CCA_REQUIRE(inst, CC_ERR_INVALID_ARG, "null instance");
         cca::decode_streams(inst, rl, symbolSize, words, bits, pos, nbs, s2b, outs, ns, n);
     });
@@ -145,14 +152,14 @@ void cc_set_instance_stream(cc_instance* inst, void* stream) {
 }
 void* cc_get_instance_stream(const cc_instance* inst) { return inst ? (void*)inst->stream : nullptr; }
 cc_status cc_set_instance_dialect(cc_instance* inst, int dialect) {
-    return guarded([&] {
+    return guarded_for(inst, [&] {
         CCA_REQUIRE(inst && (dialect == CC_DIALECT_GPU || dialect == CC_DIALECT_CPU), CC_ERR_INVALID_ARG,
                     "bad dialect");
         inst->dialect = dialect;
     });
 }
 cc_status cc_set_instance_brick_groups(cc_instance* inst, uint32_t groups) {
-    return guarded([&] {
+    return guarded_for(inst, [&] {
         CCA_REQUIRE(inst && groups <= 64, CC_ERR_INVALID_ARG, "brick groups must be in [0, 64]");
         inst->brick_groups = groups;
     });
@@ -161,7 +168,7 @@ cc_status cc_set_instance_brick_groups(cc_instance* inst, uint32_t groups) {
 cc_status cc_encode_device(cc_instance* inst, int rl, int symbolSize, const void* const* d_symbolStreams,
                            uint32_t streamCount, uint32_t symbolCountPerStream, uint32_t* d_out,
                            uint64_t outCapacityWords, uint64_t* outWords) {
-    return guarded([&] {
+    return guarded_for(inst, [&] {
         CCA_REQUIRE(inst && outWords && (d_out || outCapacityWords == 0), CC_ERR_INVALID_ARG, "null argument");
         CCA_REQUIRE(streamCount == 0 || d_symbolStreams, CC_ERR_INVALID_ARG, "null stream array");
         std::vector<const void*> streams(d_symbolStreams, d_symbolStreams + streamCount);
@@ -179,7 +186,7 @@ cc_status cc_encode_device(cc_instance* inst, int rl, int symbolSize, const void
 cc_status cc_decode_device(cc_instance* inst, int rl, int symbolSize, const uint32_t* d_words, uint64_t wordCount,
                            void* const* d_symbolStreams, uint32_t streamCount, uint32_t symbolCountPerStream,
                            uint64_t* outWordsConsumed) {
-    return guarded([&] {
+    return guarded_for(inst, [&] {
         CCA_REQUIRE(inst && outWordsConsumed, CC_ERR_INVALID_ARG, "null argument");
         CCA_REQUIRE(streamCount == 0 || d_symbolStreams, CC_ERR_INVALID_ARG, "null stream array");
         std::vector<void*> outs(d_symbolStreams, d_symbolStreams + streamCount);
@@ -197,7 +204,7 @@ cc_status cc_decode_device(cc_instance* inst, int rl, int symbolSize, const uint
 cc_status cc_compress_bricks_float(cc_instance* inst, const cc_brick* bricks, uint32_t count, uint32_t numLevels,
                                    int symbolSize, uint32_t* d_out, uint64_t outCapacityWords,
                                    uint64_t* brickWordOffsets, uint32_t* brickStatus) {
-    return guarded([&] {
+    return guarded_for(inst, [&] {
         CCA_REQUIRE(inst && brickWordOffsets && (d_out || outCapacityWords == 0), CC_ERR_INVALID_ARG, "null argument");
         cca::compress_bricks(inst, bricks, count, numLevels, symbolSize, d_out, outCapacityWords, brickWordOffsets,
                              brickStatus);
@@ -207,7 +214,7 @@ cc_status cc_compress_bricks_float(cc_instance* inst, const cc_brick* bricks, ui
 cc_status cc_decompress_bricks_float(cc_instance* inst, const cc_brick* bricks, uint32_t count, uint32_t numLevels,
                                      int symbolSize, const uint32_t* d_in, const uint64_t* brickWordOffsets,
                                      uint32_t* brickStatus) {
-    return guarded([&] {
+    return guarded_for(inst, [&] {
         CCA_REQUIRE(inst && brickWordOffsets && (d_in || count == 0), CC_ERR_INVALID_ARG, "null argument");
         cca::decompress_bricks(inst, bricks, count, numLevels, symbolSize, d_in, brickWordOffsets, brickStatus);
     });
@@ -215,7 +222,7 @@ cc_status cc_decompress_bricks_float(cc_instance* inst, const cc_brick* bricks, 
 
 cc_status cc_bricks_plan_create(cc_instance* inst, const cc_brick* bricks, uint32_t count, uint32_t numLevels,
                                 int symbolSize, uint32_t* d_out, uint64_t outCapacityWords, cc_bricks_plan** plan) {
-    return guarded([&] {
+    return guarded_for(inst, [&] {
         CCA_REQUIRE(inst && plan && d_out, CC_ERR_INVALID_ARG, "null argument");
         *plan = nullptr;
         *plan = cca::plan_create(inst, bricks, count, numLevels, symbolSize, d_out, outCapacityWords);
@@ -223,14 +230,14 @@ cc_status cc_bricks_plan_create(cc_instance* inst, const cc_brick* bricks, uint3
 }
 
 cc_status cc_bricks_plan_compress(cc_bricks_plan* plan, uint64_t* brickWordOffsets, uint32_t* brickStatus) {
-    return guarded([&] {
+    return guarded_for(cca::plan_owner(plan), [&] {
         CCA_REQUIRE(plan && brickWordOffsets, CC_ERR_INVALID_ARG, "null argument");
         cca::plan_compress(plan, brickWordOffsets, brickStatus);
     });
 }
 
 cc_status cc_bricks_plan_decompress(cc_bricks_plan* plan, uint32_t* brickStatus) {
-    return guarded([&] {
+    return guarded_for(cca::plan_owner(plan), [&] {
         CCA_REQUIRE(plan, CC_ERR_INVALID_ARG, "null plan");
         cca::plan_decompress(plan, brickStatus);
     });
@@ -238,14 +245,14 @@ cc_status cc_bricks_plan_decompress(cc_bricks_plan* plan, uint32_t* brickStatus)
 
 cc_status cc_bricks_plan_run(cc_bricks_plan* plan, uint64_t* brickWordOffsets, uint32_t* compressStatus,
                              uint32_t* decompressStatus) {
-    return guarded([&] {
+    return guarded_for(cca::plan_owner(plan), [&] {
         CCA_REQUIRE(plan && brickWordOffsets, CC_ERR_INVALID_ARG, "null argument");
         cca::plan_run(plan, brickWordOffsets, compressStatus, decompressStatus);
     });
 }
 
 cc_status cc_bricks_plan_launch(cc_bricks_plan* plan) {
-    return guarded([&] {
+    return guarded_for(cca::plan_owner(plan), [&] {
         CCA_REQUIRE(plan, CC_ERR_INVALID_ARG, "null argument");
         cca::plan_launch(plan);
     });
@@ -253,7 +260,7 @@ cc_status cc_bricks_plan_launch(cc_bricks_plan* plan) {
 
 cc_status cc_bricks_plan_wait(cc_bricks_plan* plan, uint64_t* brickWordOffsets, uint32_t* compressStatus,
                               uint32_t* decompressStatus) {
-    return guarded([&] {
+    return guarded_for(cca::plan_owner(plan), [&] {
         CCA_REQUIRE(plan && brickWordOffsets, CC_ERR_INVALID_ARG, "null argument");
         cca::plan_wait(plan, brickWordOffsets, compressStatus, decompressStatus);
     });
@@ -265,7 +272,7 @@ cc_status cc_server_create(cc_instance* inst, const uint32_t* d_words, uint32_t 
                            const uint64_t* brickWordRanges, const uint32_t* brickSizes, const float* quantSteps,
                            uint32_t numLevels, int symbolSize, uint32_t slots, uint64_t workspaceBytes,
                            cc_server** out) {
-    return guarded([&] {
+    return guarded_for(inst, [&] {
         CCA_REQUIRE(inst && out, CC_ERR_INVALID_ARG, "null argument");
         *out = nullptr;
         *out = cca::server_create(inst, d_words, brickCount, brickWordRanges, brickSizes, quantSteps, numLevels,
@@ -275,14 +282,14 @@ cc_status cc_server_create(cc_instance* inst, const uint32_t* d_words, uint32_t 
 
 cc_status cc_server_submit(cc_server* server, uint64_t requestId, const uint32_t* brickIds, uint32_t count,
                            float* const* d_outputs, double arrivalTime) {
-    return guarded([&] {
+    return guarded_for(cca::server_owner(server), [&] {
         CCA_REQUIRE(server, CC_ERR_INVALID_ARG, "null server");
         cca::server_submit(server, requestId, brickIds, count, d_outputs, arrivalTime);
     });
 }
 
 cc_status cc_server_poll(cc_server* server, double now, uint32_t* batchesLaunched) {
-    return guarded([&] {
+    return guarded_for(cca::server_owner(server), [&] {
         CCA_REQUIRE(server, CC_ERR_INVALID_ARG, "null server");
         const uint32_t n = cca::server_poll(server, now);
         if (batchesLaunched) *batchesLaunched = n;
@@ -301,7 +308,7 @@ uint32_t cc_server_failed(cc_server* server, uint64_t* requestIds, uint32_t* sta
 
 cc_status cc_server_run(cc_server* server, uint32_t requestCount, const double* arrivals, const uint32_t* counts,
                         const uint32_t* brickIds, float* const* d_outputs, double* latencies, double* elapsed) {
-    return guarded([&] {
+    return guarded_for(cca::server_owner(server), [&] {
         CCA_REQUIRE(server, CC_ERR_INVALID_ARG, "null server");
         cca::server_run(server, requestCount, arrivals, counts, brickIds, d_outputs, latencies, elapsed);
     });
@@ -364,43 +371,29 @@ cc_status cc_dwt_float_inverse_from_symbols(float* d_dst, float* d_buffer2, floa
     });
 }
 cc_status cc_set_timing_detail(cc_instance* inst, int detail) {
-    return guarded([&] {
+    return guarded_for(inst, [&] {
         CCA_REQUIRE(inst && detail >= 0 && detail <= 3, CC_ERR_INVALID_ARG, "bad timing detail");
         inst->timing_detail = detail;
-        // the per-kernel timer is process-wide: switched on for level 3, off again when the
-        // instance that switched it on leaves level 3
-        if (detail >= 3 && !inst->timing_kernels) {
-            cc_profile_enable(1);
-            inst->timing_kernels = true;
-        } else if (detail < 3 && inst->timing_kernels) {
-            cc_profile_enable(0);
-            inst->timing_kernels = false;
-        }
+        // level 3: this instance's kernel launches are timed into its own records, whatever other
+        // instances or the process-wide switch do
+        cca::profile_vote(inst, detail >= 3);
     });
 }
 cc_status cc_get_timings(cc_instance* inst, uint32_t* count) {
-    return guarded([&] {
+    return guarded_for(inst, [&] {
         CCA_REQUIRE(inst && count, CC_ERR_INVALID_ARG, "null argument");
         cca::DeviceGuard guard(inst->device);
         cca::Timings& t = inst->timings;
         t.collect();
         t.view.clear();
         for (const std::string& n : t.order) t.view.push_back({n, (float)t.ms[n]});
-        if (inst->timing_detail >= 3) {
-            const uint32_t k = cc_profile_collect();
-            for (uint32_t i = 0; i < k; i++) {
-                const char* name = nullptr;
-                uint64_t launches = 0, bytes = 0;
-                double ms = 0.0;
-                if (cc_profile_entry(i, &name, &launches, &ms, &bytes) == CC_OK)
-                    t.view.push_back({std::string("Kernel: ") + name, (float)ms});
-            }
-        }
+        if (inst->timing_detail >= 3)
+            for (auto& e : cca::profile_owner_entries(inst)) t.view.push_back({"Kernel: " + e.first, (float)e.second});
         *count = (uint32_t)t.view.size();
     });
 }
 cc_status cc_get_timing_entry(cc_instance* inst, uint32_t i, const char** name, float* ms) {
-    return guarded([&] {
+    return guarded_for(inst, [&] {
         CCA_REQUIRE(inst && name && ms, CC_ERR_INVALID_ARG, "null argument");
         CCA_REQUIRE(i < inst->timings.view.size(), CC_ERR_INVALID_ARG, "timing index out of range");
         *name = inst->timings.view[i].first.c_str();
@@ -417,12 +410,12 @@ cc_status cc_print_timings(cc_instance* inst) {
     return CC_OK;
 }
 cc_status cc_reset_timings(cc_instance* inst) {
-    return guarded([&] {
+    return guarded_for(inst, [&] {
         CCA_REQUIRE(inst, CC_ERR_INVALID_ARG, "null argument");
         cca::DeviceGuard guard(inst->device);
         inst->timings.reset();
         inst->timings.view.clear();
-        if (inst->timing_detail >= 3) cc_profile_reset();
+        cca::profile_owner_reset(inst);
     });
 }
 cc_status cc_dwt_float_forward_from_byte(float* d_dst, float* d_buffer2, float* d_buffer1, const uint8_t* d_src,
@@ -597,7 +590,7 @@ cc_status cc_unquantize_from_short(float* d_data, const int16_t* d_quant, uint32
     });
 }
 cc_status cc_get_max_abs(cc_instance* inst, const float* d_image, uint32_t elemCount, float* d_valMax) {
-    return guarded([&] {
+    return guarded_for(inst, [&] {
         CCA_REQUIRE(inst && d_valMax, CC_ERR_INVALID_ARG, "null argument");
         cca::DeviceGuard g(inst->device);
         cca::max_abs(d_image, elemCount, d_valMax, inst->stream);
@@ -607,7 +600,7 @@ cc_status cc_get_max_abs(cc_instance* inst, const float* d_image, uint32_t elemC
 cc_status cc_compress_volume_float(cc_instance* inst, const float* d_image, uint32_t sizeX, uint32_t sizeY,
                                    uint32_t sizeZ, uint32_t numLevels, float quantStepLevel0, int doRLEOnlyOnLvl0,
                                    int symbolSize, const uint32_t** outWords, uint64_t* outBits) {
-    return guarded([&] {
+    return guarded_for(inst, [&] {
         CCA_REQUIRE(inst && outWords && outBits, CC_ERR_INVALID_ARG, "null argument");
         cca::compress_volume_float(inst, d_image, sizeX, sizeY, sizeZ, numLevels, quantStepLevel0,
                                    doRLEOnlyOnLvl0 != 0, symbolSize);
@@ -618,7 +611,7 @@ cc_status cc_compress_volume_float(cc_instance* inst, const float* d_image, uint
 cc_status cc_decompress_volume_float(cc_instance* inst, float* d_image, uint32_t sizeX, uint32_t sizeY, uint32_t sizeZ,
                                      uint32_t numLevels, const uint32_t* bits, uint64_t bitCount,
                                      float quantStepLevel0, int doRLEOnlyOnLvl0, int symbolSize) {
-    return guarded([&] {
+    return guarded_for(inst, [&] {
         CCA_REQUIRE(inst && bits, CC_ERR_INVALID_ARG, "null argument");
         cca::decompress_volume_float(inst, d_image, sizeX, sizeY, sizeZ, numLevels, bits, bitCount, quantStepLevel0,
                                      doRLEOnlyOnLvl0 != 0, symbolSize);
@@ -627,7 +620,7 @@ cc_status cc_decompress_volume_float(cc_instance* inst, float* d_image, uint32_t
 cc_status cc_compress_volume_float_quant_first(cc_instance* inst, const float* d_image, uint32_t sizeX, uint32_t sizeY,
                                                uint32_t sizeZ, uint32_t numLevels, float quantizationStep,
                                                int doRLEOnlyOnLvl0, const uint32_t** outWords, uint64_t* outBits) {
-    return guarded([&] {
+    return guarded_for(inst, [&] {
         CCA_REQUIRE(inst && d_image && outWords && outBits, CC_ERR_INVALID_ARG, "null argument");
         cca::compress_volume_float_quant_first(inst, d_image, sizeX, sizeY, sizeZ, numLevels, quantizationStep,
                                                doRLEOnlyOnLvl0 != 0);
@@ -638,7 +631,7 @@ cc_status cc_compress_volume_float_quant_first(cc_instance* inst, const float* d
 cc_status cc_decompress_volume_float_quant_first(cc_instance* inst, float* d_image, uint32_t sizeX, uint32_t sizeY,
                                                  uint32_t sizeZ, uint32_t numLevels, const uint32_t* bits,
                                                  uint64_t bitCount, float quantizationStep, int doRLEOnlyOnLvl0) {
-    return guarded([&] {
+    return guarded_for(inst, [&] {
         CCA_REQUIRE(inst && d_image && bits, CC_ERR_INVALID_ARG, "null argument");
         cca::decompress_volume_float_quant_first(inst, d_image, sizeX, sizeY, sizeZ, numLevels, bits, bitCount,
                                                  quantizationStep, doRLEOnlyOnLvl0 != 0);
@@ -654,7 +647,7 @@ std::vector<cca::VolumeChannelIn> channels_in(const cc_volume_channel* ch, uint3
 cc_status cc_decompress_volume_float_multichannel(cc_instance* inst, const cc_volume_channel* channels,
                                                   uint32_t channelCount, uint32_t sizeX, uint32_t sizeY, uint32_t sizeZ,
                                                   uint32_t numLevels, int doRLEOnlyOnLvl0, int symbolSize) {
-    return guarded([&] {
+    return guarded_for(inst, [&] {
         CCA_REQUIRE(inst && channels && channelCount, CC_ERR_INVALID_ARG, "null argument");
         auto v = channels_in(channels, channelCount);
         cca::decompress_volume_float_multichannel(inst, v.data(), channelCount, sizeX, sizeY, sizeZ, numLevels,
@@ -665,7 +658,7 @@ cc_status cc_decompress_volume_float_quant_first_multichannel(cc_instance* inst,
                                                               uint32_t channelCount, uint32_t sizeX, uint32_t sizeY,
                                                               uint32_t sizeZ, uint32_t numLevels,
                                                               int doRLEOnlyOnLvl0) {
-    return guarded([&] {
+    return guarded_for(inst, [&] {
         CCA_REQUIRE(inst && channels && channelCount, CC_ERR_INVALID_ARG, "null argument");
         auto v = channels_in(channels, channelCount);
         cca::decompress_volume_float_quant_first_multichannel(inst, v.data(), channelCount, sizeX, sizeY, sizeZ,
@@ -675,7 +668,7 @@ cc_status cc_decompress_volume_float_quant_first_multichannel(cc_instance* inst,
 cc_status cc_compress_volume_lossless(cc_instance* inst, const int16_t* d_image, uint32_t sizeX, uint32_t sizeY,
                                       uint32_t sizeZ, uint32_t numLevels, const uint32_t** outWords,
                                       uint64_t* outBits) {
-    return guarded([&] {
+    return guarded_for(inst, [&] {
         CCA_REQUIRE(inst && outWords && outBits, CC_ERR_INVALID_ARG, "null argument");
         cca::compress_volume_lossless(inst, d_image, sizeX, sizeY, sizeZ, numLevels);
         *outWords = inst->vol_bits.data();
@@ -684,7 +677,7 @@ cc_status cc_compress_volume_lossless(cc_instance* inst, const int16_t* d_image,
 }
 cc_status cc_decompress_volume_lossless(cc_instance* inst, int16_t* d_image, uint32_t sizeX, uint32_t sizeY,
                                         uint32_t sizeZ, uint32_t numLevels, const uint32_t* bits, uint64_t bitCount) {
-    return guarded([&] {
+    return guarded_for(inst, [&] {
         CCA_REQUIRE(inst && bits, CC_ERR_INVALID_ARG, "null argument");
         cca::decompress_volume_lossless(inst, d_image, sizeX, sizeY, sizeZ, numLevels, bits, bitCount);
     });

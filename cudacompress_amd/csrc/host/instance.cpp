@@ -1,6 +1,8 @@
 // Device/host buffer helpers and instance teardown.
 #include "instance.hpp"
 
+#include "../profile.hpp"
+
 namespace cca {
 
 void* DevBuf::get(size_t bytes) {
@@ -105,7 +107,7 @@ cc_instance::~cc_instance() {
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
     else (void)hipDeviceSynchronize();
-    if (timing_kernels) cc_profile_enable(0);  // this instance switched the kernel timer on
+    cca::profile_owner_forget(this);  // its kernel timing vote and records
     for (auto& e : dec_event)
         if (e) (void)hipEventDestroy(e);
     cca::DevBuf* devs[] = {&d_rle_sym, &d_rle_zc, &d_desc, &d_tiles, &d_map, &d_map2, &d_results, &d_hist,

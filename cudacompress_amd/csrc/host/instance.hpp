@@ -110,7 +110,6 @@ struct cc_instance {
 
     // section timing (cc_set_timing_detail)
     int timing_detail = 0;
-    bool timing_kernels = false;  // this instance switched the process-wide kernel timer on
     cca::Timings timings;
 
     ~cc_instance();

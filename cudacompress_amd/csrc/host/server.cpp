@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <chrono>
 #include <deque>
+#include <new>
 
 #include "bricks.hpp"
 
@@ -99,36 +100,53 @@ void reap(cc_server* S, Slot& s, double now) {
 }
 
 void launch(cc_server* S, Slot& s) {
-    // admit requests in arrival order while the batch fits the slot's workspace budget
+    // admit requests in arrival order while the batch fits the slot's workspace budget; the batch
+    // is built aside and becomes the slot's only once it is on the device
     std::vector<uint32_t> ids;
-    s.job = BrickJob();
-    s.job.L = S->levels;
-    s.job.symSize = S->symSize;
+    std::vector<uint64_t> reqs;
+    std::vector<uint32_t> nbricks;
+    BrickJob job;
+    job.L = S->levels;
+    job.symSize = S->symSize;
     uint64_t ws = 0;
     while (!S->pending.empty()) {
         Request& rq = S->pending.front();
-        if (!s.reqs.empty() && ws + rq.workspace > S->slot_budget) break;
+        if (!reqs.empty() && ws + rq.workspace > S->slot_budget) break;
         ws += rq.workspace;
         for (size_t j = 0; j < rq.bricks.size(); j++) {
             const uint32_t b = rq.bricks[j];
             const uint32_t* d = &S->dims[3 * b];
-            s.job.bricks.push_back(cc_brick{nullptr, rq.outs[j], d[0], d[1], d[2], S->steps[b]});
+            job.bricks.push_back(cc_brick{nullptr, rq.outs[j], d[0], d[1], d[2], S->steps[b]});
             ids.push_back(b);
         }
-        s.reqs.push_back(rq.id);
-        s.req_nbricks.push_back((uint32_t)rq.bricks.size());
+        reqs.push_back(rq.id);
+        nbricks.push_back((uint32_t)rq.bricks.size());
         S->pending.pop_front();
     }
-    if (s.reqs.empty()) return;
-    bricks_prepare_bound(S->inst, s.ctx, s.job, S->index, ids, s.st);
-    bricks_decode_launch(s.ctx, s.job, s.st, false);
-    // statuses to the pinned buffer, then the completion event
-    DeviceCoder& D = s.ctx.coder;
-    const uint32_t ncalls = s.job.bj.ncalls;  // one call per brick of the batch
-    unsigned char* r = (unsigned char*)D.h_dresult.get(sizeof(CoderState) + sizeof(DecCallOut) * ncalls);
-    CCA_HIP(hipMemcpyAsync(r + sizeof(CoderState), D.d_dout.p, sizeof(DecCallOut) * ncalls, hipMemcpyDeviceToHost,
-                           s.st));
-    CCA_HIP(hipEventRecord(s.done, s.st));
+    if (reqs.empty()) return;
+    try {
+        s.job = std::move(job);
+        bricks_prepare_bound(S->inst, s.ctx, s.job, S->index, ids, s.st);
+        bricks_decode_launch(s.ctx, s.job, s.st, false);
+        // statuses to the pinned buffer, then the completion event
+        DeviceCoder& D = s.ctx.coder;
+        const uint32_t ncalls = s.job.bj.ncalls;  // one call per brick of the batch
+        unsigned char* r = (unsigned char*)D.h_dresult.get(sizeof(CoderState) + sizeof(DecCallOut) * ncalls);
+        CCA_HIP(hipMemcpyAsync(r + sizeof(CoderState), D.d_dout.p, sizeof(DecCallOut) * ncalls,
+                               hipMemcpyDeviceToHost, s.st));
+        CCA_HIP(hipEventRecord(s.done, s.st));
+    } catch (const Error& e) {
+        // the admitted requests fail with the launch's status; the slot stays free and empty
+        for (uint64_t id : reqs) S->failed.emplace_back(id, (uint32_t)e.status);
+        s.job = BrickJob();
+        throw;
+    } catch (const std::bad_alloc&) {
+        for (uint64_t id : reqs) S->failed.emplace_back(id, (uint32_t)CC_ERR_OUT_OF_MEMORY);
+        s.job = BrickJob();
+        throw;
+    }
+    s.reqs = std::move(reqs);
+    s.req_nbricks = std::move(nbricks);
     s.busy = true;
 }
 
@@ -201,6 +219,7 @@ uint32_t server_poll(cc_server* S, double now) {
 }
 
 void server_destroy(cc_server* S) { delete S; }
+const void* server_owner(const cc_server* S) { return S ? S->inst : nullptr; }
 
 uint32_t server_busy(const cc_server* S) {
     uint32_t b = 0;

@@ -17,6 +17,8 @@ void server_submit(cc_server* S, uint64_t id, const uint32_t* bricks, uint32_t c
 // reaps finished batches (completion time `now`) and launches pending requests on free slots
 uint32_t server_poll(cc_server* S, double now);
 void server_destroy(cc_server* S);
+// the instance a server belongs to (its launches are that instance's in the kernel timing)
+const void* server_owner(const cc_server* S);
 // slots in flight + requests waiting
 uint32_t server_busy(const cc_server* S);
 uint32_t server_take_completed(cc_server* S, uint64_t* ids, double* times, uint32_t max);

@@ -160,3 +160,56 @@ def decompress_volume_float_quant_first(words, shape, levels, step, log2=14, cbs
             region[b] = o.unsymbolize(st).reshape(region[b].shape)
         coef[: Z // f, : Y // f, : X // f] = o.dwt53_inverse(np.ascontiguousarray(region))
     return o.unquantize_from_short(coef, step, qtype=1)
+
+
+# ---------------------------------------------------------------------------- precision reports
+def dwt97_levels(vol, levels, dtype=np.float32):
+    """`levels` levels of the 9/7 analysis in `dtype` with the in-place layout of
+    compressVolumeFloat (CompressVolume.cpp:342-352): level l transforms the leading
+    (Z, Y, X) >> l corner."""
+    coef = np.array(vol, dtype=dtype, order="C", copy=True)
+    for lv in range(levels):
+        s = tuple(slice(0, n >> lv) for n in coef.shape)
+        coef[s] = o.dwt97_forward(coef[s], dtype=dtype)
+    return coef
+
+
+def band_steps(shape, levels, q0):
+    """(slices, step) of every quantized band in bitstream order: the lowpass band at
+    q0 / 2^(levels-1), then per level (coarsest first) the 7 highpass bands at q0 / 2^level
+    (CompressVolume.cpp:355-386)."""
+    Z, Y, X = shape
+    f = 1 << levels
+    out = [((slice(0, Z // f), slice(0, Y // f), slice(0, X // f)), float(_step(q0, f // 2)))]
+    for level in range(levels - 1, -1, -1):
+        region = (Z // (f // 2), Y // (f // 2), X // (f // 2))
+        q = float(_step(q0, f // 2))
+        out += [(b, q) for b in _bands3d(region)]
+        f //= 2
+    return out
+
+
+def precision_report(vol, levels, q0, boundary_tol=1e-5):
+    """FP32-vs-FP64 error of the multi-level 9/7 analysis and the quantizer's sensitivity to it.
+
+    dwt_max_abs / dwt_rel_l2: max |c32 - c64| and ||c32 - c64|| / ||c64|| over all coefficients;
+    near_boundary: coefficients whose FP64 value lies within boundary_tol (in units of the band's
+    step) of a deadzone bin edge |c| / step = k >= 1 (Quantize.cuh:76-120 truncates);
+    symbols_differ_fp64: quantized symbols of the FP32 coefficients that differ from those of the
+    FP64 coefficients quantized the same way."""
+    c32 = dwt97_levels(vol, levels, np.float32)
+    c64 = dwt97_levels(vol, levels, np.float64)
+    d = c32.astype(np.float64) - c64
+    near = 0
+    differ = 0
+    for sl, q in band_steps(vol.shape, levels, q0):
+        a64 = np.abs(c64[sl]).ravel() / np.float64(np.float32(q))
+        k = np.rint(a64)
+        near += int(np.count_nonzero((k >= 1) & (np.abs(a64 - k) < boundary_tol)))
+        s32 = o.quantize(np.ascontiguousarray(c32[sl]).ravel(), q, dtype=np.uint32)
+        t64 = np.trunc(c64[sl].ravel() / np.float64(np.float32(q))).astype(np.int64)
+        s64 = np.where(t64 >= 0, 2 * t64, -2 * t64 - 1).astype(np.uint32)
+        differ += int(np.count_nonzero(s32 != s64))
+    return {"dwt_max_abs": float(np.abs(d).max()),
+            "dwt_rel_l2": float(np.sqrt((d * d).sum() / max((c64 * c64).sum(), 1e-300))),
+            "near_boundary": near, "symbols_differ_fp64": differ, "coefficients": int(c64.size)}

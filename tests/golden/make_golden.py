@@ -13,6 +13,10 @@ header fields hold them):
   - runLengthZeroesEncodeCPU pairs of stream 0 (src/CPU/RunLengthCPU.cpp:8-32);
   - HuffmanTableCPU::design of stream 0 (src/CPU/HuffmanTableCPU.cpp:22-105).
 Only data (inputs and outputs) is written; no reference source is stored.
+
+It also writes tests/golden/bitstream_trace_ref.txt.gz: the output of oracle/bitstream_trace.cpp
+built against the reference's own include/cudaCompress/BitStream.h (oracle/_ref/bitstream_trace),
+a seeded sequence of bit stream writes and reads with every returned value, position and word.
 """
 import json
 import os
@@ -27,6 +31,7 @@ sys.path.insert(0, ROOT)
 from oracle import pyoracle as O  # noqa: E402
 
 OUT = os.path.join(HERE, "cpu_coder.npz")
+TRACE_OUT = os.path.join(HERE, "bitstream_trace_ref.txt.gz")
 
 
 def laplace_symbols(rng, n, scale, zero_frac):
@@ -81,6 +86,18 @@ def main():
     arrays["meta_json"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
     np.savez_compressed(OUT, **arrays)
     print(f"wrote {OUT}: {len(meta)} cases, {os.path.getsize(OUT)} bytes")
+    write_trace()
+
+
+def write_trace():
+    import gzip
+    import subprocess
+    exe = os.path.join(ROOT, "oracle", "_ref", "bitstream_trace")
+    text = subprocess.run([exe], check=True, capture_output=True).stdout
+    with gzip.GzipFile(TRACE_OUT, "wb", mtime=0) as f:
+        f.write(text)
+    lines = len(text.splitlines())
+    print(f"wrote {TRACE_OUT}: {lines} lines")
 
 
 if __name__ == "__main__":

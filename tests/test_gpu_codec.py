@@ -1,5 +1,7 @@
 """GPU parity of the entropy coder (encodeRLHuff/encodeHuff/decodeRLHuff/decodeHuff) against the
 oracle (GPU dialect) and the reference's own CPU coder (CPU dialect), through the C ABI."""
+import zlib
+
 import numpy as np
 import pytest
 
@@ -66,7 +68,7 @@ CASES = [
 @pytest.mark.parametrize("rl", [True, False], ids=["rlhuff", "huff"])
 def test_codec_matches_oracle(cc, torch_gpu, inst14, name, gen, rl):
     log("case", name, "rl", rl)
-    rng = np.random.default_rng(hash(name) & 0xFFFF)
+    rng = np.random.default_rng(zlib.crc32(name.encode()) & 0xFFFF)
     check_roundtrip(cc, torch_gpu, inst14, gen(rng), rl, 14, 128)
 
 

@@ -1,5 +1,7 @@
 """GPU parity of the device-resident coder (cc_encode_device: RLE, histograms, table design,
 packing and layout on the GPU) against the oracle's bitstreams, through the C ABI."""
+import zlib
+
 import numpy as np
 import pytest
 
@@ -65,7 +67,7 @@ def inst14(cc, torch_gpu):
 @pytest.mark.parametrize("rl", [True, False], ids=["rlhuff", "huff"])
 def test_device_coder_matches_oracle(cc, torch_gpu, inst14, name, gen, rl):
     log("device coder case", name, "rl", rl)
-    rng = np.random.default_rng(hash(name) & 0xFFFF)
+    rng = np.random.default_rng(zlib.crc32(name.encode()) & 0xFFFF)
     check(cc, torch_gpu, inst14, gen(rng), rl, 14, 128)
 
 
@@ -135,7 +137,7 @@ def device_decode(cc, torch, inst, words, ns, n, dtype, rl, avail=None):
 def test_device_decoder_roundtrip(cc, torch_gpu, inst14, name, gen, rl):
     """The device decoder reads the oracle's bitstream back to the original symbols."""
     log("device decoder case", name, "rl", rl)
-    rng = np.random.default_rng(hash(name) & 0xFFFF)
+    rng = np.random.default_rng(zlib.crc32(name.encode()) & 0xFFFF)
     streams = gen(rng)
     words, bits = o.encode(streams, rl=rl, log2=14, cbs=128)
     dec, used = device_decode(cc, torch_gpu, inst14, words, len(streams), int(streams[0].size), streams[0].dtype, rl)

@@ -288,3 +288,51 @@ def test_timing_sections(cc, torch_gpu):
     assert cc.encodeRLHuff(inst, cc.BitStream(), streams, n)
     assert cc.getTimings(inst) == ([], [])
     inst.close()
+
+
+def test_timing_kernels_per_instance(cc, torch_gpu):
+    """Two instances at TIMING_DETAIL_HIGH keep their own kernel entries: one instance's calls do
+    not show up in the other's timings, lowering or destroying one leaves the other's kernel timing
+    on, resetting one leaves the other's entries, and the process-wide profiler is a separate vote."""
+    torch = torch_gpu
+    log("timing: two instances")
+    rng = np.random.default_rng(23)
+    n = 1 << 13
+    streams = [to_dev(torch, (rng.integers(0, 5, n) * (rng.random(n) < 0.4)).astype(np.uint16)) for _ in range(2)]
+    a = cc.createInstance(0, 2, n, 128, 14)
+    b = cc.createInstance(0, 2, n, 128, 14)
+    cc.setTimingDetail(a, cc.TIMING_DETAIL_HIGH)
+    cc.setTimingDetail(b, cc.TIMING_DETAIL_HIGH)
+
+    def kernels(inst):
+        names, _ = cc.getTimings(inst)
+        return [x for x in names if x.startswith("Kernel: ")]
+
+    assert cc.encodeRLHuff(a, cc.BitStream(), streams, n)
+    torch.cuda.synchronize()
+    assert kernels(a) and not kernels(b)          # b made no launches
+    assert cc.encodeHuff(b, cc.BitStream(), streams, n)
+    torch.cuda.synchronize()
+    kb = kernels(b)
+    assert kb
+    cc.setTimingDetail(a, cc.TIMING_DETAIL_NONE)  # a leaves level 3: b keeps timing kernels
+    cc.resetTimings(b)
+    assert kernels(b) == []
+    assert cc.encodeHuff(b, cc.BitStream(), streams, n)
+    torch.cuda.synchronize()
+    assert kernels(b)
+    cc.setTimingDetail(a, cc.TIMING_DETAIL_HIGH)
+    cc.resetTimings(a)                            # a's reset leaves b's entries
+    assert kernels(b)
+    lib = cc._capi.load()
+    lib.cc_profile_reset()
+    lib.cc_profile_enable(1)                      # the process-wide switch is its own vote
+    a.close()                                     # destroying a leaves b's timing and the switch on
+    assert cc.encodeHuff(b, cc.BitStream(), streams, n)
+    torch.cuda.synchronize()
+    assert kernels(b)
+    assert lib.cc_profile_collect() > 0
+    lib.cc_profile_enable(0)
+    lib.cc_profile_reset()
+    cc.setTimingDetail(b, cc.TIMING_DETAIL_NONE)
+    b.close()
