@@ -72,6 +72,9 @@ def parse():
                          "eager: compressBricksFloat + decompressBricksFloat calls")
     ap.add_argument("--plain-sample", type=int, default=256, help="bricks timed through the plain per-brick API")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=8.0, help="length of each timed CPU-baseline leg")
+    ap.add_argument("--no-configs", action="store_true", help="skip the C1/C2/C3 single-volume records")
+    ap.add_argument("--config-reps", type=int, default=3, help="timed round trips per single-volume config")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic.json"))
     ap.add_argument("--copy-calibration", action="store_true",
@@ -127,6 +130,226 @@ def roofline_of(kernels, steps, workload, traffic_file):
     table = [{"kernel": k[0], "launches_per_step": round(k[1] / steps, 2), "ms_per_step": round(k[2] / steps, 4),
               "GBps": round(k[3] / max(k[2], 1e-9) / 1e6, 1)} for k in kernels[:14]]
     return roof, table, sum(k[2] for k in kernels) / steps
+
+
+def pipeline_bytes(N, M, NH, C, levels, b=4, s=2, dims=3):
+    """Algorithmic bytes of one compression + one decompression of the 9/7 pipeline: each stage
+    reads its input once and writes its output once.  N coefficients (b bytes), symbols of s bytes,
+    M zero-run pairs, NH symbols in Huffman-only streams, C compressed bytes, g = sum over levels
+    of 2^(-dims*l) (the DWT touches the leading corner of every level)."""
+    g = sum(2.0 ** (-dims * l) for l in range(levels))
+    dwt = 2 * b * N * g
+    quant = (b + s) * N
+    rle = s * (N - NH) + 2 * s * M
+    hist = s * (2 * M + NH)
+    huff = s * (2 * M + NH) + C
+    enc = dwt + quant + rle + hist + huff
+    dec = huff + rle + quant + dwt  # Huffman decode, zero-run expansion, dequantize, inverse DWT
+    return {"encode": int(enc), "decode": int(dec), "total": int(enc + dec)}
+
+
+def band_layout(shape, levels, q0):
+    """(slices, step) of every quantized band of compressVolumeFloat in bitstream order: the lowpass
+    band at q0 / 2^(levels-1), then per level (coarsest first) the 7 highpass bands (band b at
+    x = b % 2, y = b / 2 % 2, z = b / 4) at q0 / 2^level."""
+    Z, Y, X = shape
+    f = 1 << levels
+    out = [((slice(0, Z // f), slice(0, Y // f), slice(0, X // f)), float(np.float32(q0) / np.float32(f // 2)))]
+    for level in range(levels - 1, -1, -1):
+        z, y, x = Z // (f // 2) // 2, Y // (f // 2) // 2, X // (f // 2) // 2
+        q = float(np.float32(q0) / np.float32(f // 2))
+        for bnd in range(1, 8):
+            ox, oy, oz = bnd % 2, bnd // 2 % 2, bnd // 4
+            out.append(((slice(oz * z, (oz + 1) * z), slice(oy * y, (oy + 1) * y), slice(ox * x, (ox + 1) * x)), q))
+        f //= 2
+    return out
+
+
+def fp64_report(torch, cc, vol, levels, q0):
+    """FP32-vs-FP64 error of the multi-level 9/7 analysis on the device: the FP32 levels
+    (util.dwtFloat3DForward) against the FP64 twin (util.dwtDouble3DForward), and per band the
+    coefficients whose FP64 value lies within 1e-5 (in steps) of a deadzone bin edge |c|/step = k
+    >= 1, and the symbols an FP64 quantizer would give differently."""
+    c32 = vol.clone()
+    c64 = vol.double()
+    Z, Y, X = vol.shape
+    for lv in range(levels):
+        sl = (slice(0, Z >> lv), slice(0, Y >> lv), slice(0, X >> lv))
+        src = c32[sl].contiguous()
+        dst, b1, b2 = torch.empty_like(src), torch.empty_like(src), torch.empty_like(src)
+        cc.util.dwtFloat3DForward(dst, b2, b1, src, X >> lv, Y >> lv, Z >> lv)
+        c32[sl] = dst
+        src = c64[sl].contiguous()
+        dst, buf = torch.empty_like(src), torch.empty_like(src)
+        cc.util.dwtDouble3DForward(dst, buf, src, X >> lv, Y >> lv, Z >> lv)
+        c64[sl] = dst
+    d = c32.double() - c64
+    near = differ = 0
+    for sl, q in band_layout(vol.shape, levels, q0):
+        a64 = c64[sl].abs() / q
+        k = torch.round(a64)
+        near += int(((k >= 1) & ((a64 - k).abs() < 1e-5)).sum())
+        inv = np.float32(1.0) / np.float32(q)   # the quantizer's fp32 product with 1/step, truncated
+        q32 = torch.trunc(c32[sl] * float(inv))
+        q64 = torch.trunc(c64[sl] / q)
+        differ += int((q32.double() != q64).sum())
+    torch.cuda.synchronize()
+    return {"dwt_max_abs": float(d.abs().max()), "dwt_rel_l2": float(d.norm() / c64.norm()),
+            "near_boundary_1e-5": near, "symbols_differ_fp64": differ, "coefficients": int(vol.numel())}
+
+
+def run_configs(args, torch, cc):
+    """The single-volume configurations through the plain per-call API, inputs resident on the
+    device: C1 (128^3, 2-level 9/7, step 0.01), C2 (4096^2 lossless 5/3, log2 16) and C3 (3 x 256^3
+    velocity, 3-level 9/7, steps {0.1, 0.01, 0.001} x std, log2 16).  Raw GB/s = (raw bytes
+    compressed + decompressed) / time of compress + decompress calls (host bitstreams)."""
+    from cudacompress_amd import synthetic as S
+    dev = torch.device("cuda:0")
+    reps = max(1, args.config_reps)
+
+    def timed_pair(compress, decompress):
+        w = compress()
+        decompress(w)  # warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            w = compress()
+            decompress(w)
+        torch.cuda.synchronize()
+        return w, (time.perf_counter() - t0) / reps
+
+    out = {}
+    # C1
+    vol = S.gaussian_field((128, 128, 128), 1)
+    src = torch.from_numpy(vol).to(dev)
+    rec = torch.empty_like(src)
+    inst = cc.createInstance(0, 7, 64 ** 3, 128, 14)
+    w, t = timed_pair(lambda: cc.compressVolumeFloat(inst, src, 128, 128, 128, 2, 0.01),
+                      lambda w: cc.decompressVolumeFloat(inst, rec, 128, 128, 128, 2, w, 0.01))
+    out["C1"] = {"workload": "128^3 k^-5/3 field (seed 1), 2-level 9/7, step 0.01, log2 14, plain API",
+                 "GBps": round(2 * vol.nbytes / t / 1e9, 3), "ms_per_roundtrip": round(t * 1e3, 3),
+                 "compression_ratio": round(vol.nbytes / (4 * w.size), 3),
+                 "psnr_db": round(S.psnr(vol, rec.cpu().numpy()), 3),
+                 "fp32_vs_fp64": fp64_report(torch, cc, src, 2, 0.01), "input_sha256": S.sha256(vol)[:16]}
+    inst.close()
+    # C2
+    img = S.fbm_heightfield(4096, 0.8, 2)
+    src = torch.from_numpy(img).to(dev)
+    rec = torch.empty_like(src)
+    inst = cc.createInstance(0, 3, 2048 * 2048, 128, 16)
+    w, t = timed_pair(lambda: cc.compressVolumeLossless(inst, src, 4096, 4096, 1, 3),
+                      lambda w: cc.decompressVolumeLossless(inst, rec, 4096, 4096, 1, 3, w))
+    out["C2"] = {"workload": "4096^2 uint16 fBm (H 0.8, seed 2) as int16, 3-level 2D 5/3, log2 16, plain API",
+                 "GBps": round(2 * img.nbytes / t / 1e9, 3), "ms_per_roundtrip": round(t * 1e3, 3),
+                 "compression_ratio": round(img.nbytes / (4 * w.size), 3),
+                 "lossless_exact": bool(torch.equal(rec, src)), "input_sha256": S.sha256(img)[:16]}
+    inst.close()
+    del src, rec
+    # C3
+    u = S.velocity_field(256, 3)
+    comps = [torch.from_numpy(np.ascontiguousarray(u[c])).to(dev) for c in range(3)]
+    recs = [torch.empty_like(c) for c in comps]
+    stds = [float(u[c].std()) for c in range(3)]
+    lo, hi = float(u.min()), float(u.max())
+    inst = cc.createInstance(0, 7, 128 ** 3, 128, 16)
+    rows = []
+    for rel in (0.1, 0.01, 0.001):
+        qs = [float(np.float32(rel * s)) for s in stds]
+        words = [None] * 3
+
+        def compress():
+            for c in range(3):
+                words[c] = cc.compressVolumeFloat(inst, comps[c], 256, 256, 256, 3, qs[c])
+            return words
+
+        def decompress(ws):
+            for c in range(3):
+                cc.decompressVolumeFloat(inst, recs[c], 256, 256, 256, 3, ws[c], qs[c])
+        ws, t = timed_pair(compress, decompress)
+        se = sum(float(((r.double() - c.double()) ** 2).sum()) for r, c in zip(recs, comps))
+        rows.append({"step_x_std": rel, "GBps": round(2 * u.nbytes / t / 1e9, 3),
+                     "compression_ratio": round(u.nbytes / (4 * sum(x.size for x in ws)), 3),
+                     "psnr_db": round(float(20 * np.log10((hi - lo) / np.sqrt(se / u.size))), 3),
+                     "fp32_vs_fp64_u_x": fp64_report(torch, cc, comps[0], 3, qs[0])})
+    inst.close()
+    out["C3"] = {"workload": "3 x 256^3 divergence-free velocity (seed 3), 3-level 9/7 per component, "
+                             "steps {0.1, 0.01, 0.001} x std, log2 16, plain API", "steps": rows}
+    del comps, recs
+    return out
+
+
+def cpu_baseline_bricks(bricks, steps_q, seconds):
+    """The oracle (C++ restatement of the reference path) on the first bricks of the batch: one
+    host thread, then every host core (a thread pool over bricks; the oracle releases the GIL in
+    its C calls); and the reference's own src/CPU entropy coder (oracle/_ref) against the oracle's
+    coder on the same symbol streams, one thread."""
+    import concurrent.futures as cf
+    from oracle import pipeline as P
+    from oracle import pyoracle as o
+
+    def one(i):
+        b = bricks[i]
+        ow = P.compress_volume_float(b, 2, steps_q[i])
+        P.decompress_volume_float(ow, b.shape, 2, steps_q[i])
+        return b.nbytes
+
+    t0 = time.perf_counter()
+    runs, done = 0, 0
+    while time.perf_counter() - t0 < seconds and runs < len(bricks):
+        done += one(runs)
+        runs += 1
+    t1 = time.perf_counter() - t0
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cores = max(1, min(cores, 16))
+    t0 = time.perf_counter()
+    done_p, runs_p = 0, 0
+    with cf.ThreadPoolExecutor(cores) as ex:
+        futs = []
+        i = 0
+        while i < len(bricks) and (time.perf_counter() - t0 < seconds):
+            batch = [ex.submit(one, j) for j in range(i, min(i + cores, len(bricks)))]
+            i += len(batch)
+            done_p += sum(f.result() for f in batch)
+            runs_p += len(batch)
+    tp = time.perf_counter() - t0
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    res = {"value": round(2.0 * done_p / tp / 1e9, 5), "unit": "GB/s", "cores": cores, "kind": "port",
+           "cpu_model": model, "host_cpus": os.cpu_count(),
+           "sample": f"bricks 0..{runs_p - 1} ({done_p} raw bytes) compressed and decompressed through the oracle "
+                     f"(C++ restatement of the reference path), {cores} threads over bricks, {tp:.2f} s",
+           "single_thread": {"value": round(2.0 * done / t1 / 1e9, 5), "cores": 1,
+                             "sample": f"bricks 0..{runs - 1} ({done} raw bytes) one by one, {t1:.2f} s"}}
+    if o.ref_available():
+        # the entropy half: the reference's own CPU coder and the oracle's coder on the same streams
+        k = max(1, min(runs, 64))
+        calls = [P.volume_float_calls(bricks[i], 2, steps_q[i]) for i in range(k)]
+        raw = sum(bricks[i].nbytes for i in range(k))
+        t0 = time.perf_counter()
+        for cl in calls:
+            for rl, st in cl:
+                w, nb = o.ref_encode(st, rl=rl, cbs=128)
+                o.ref_decode(w, nb, len(st), st[0].size, rl=rl, cbs=128)
+        tr = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for cl in calls:
+            for rl, st in cl:
+                w, nb = o.encode(st, rl=rl, log2=14, cbs=128)
+                o.decode(w, nb, len(st), st[0].size, rl=rl, log2=14, cbs=128)
+        to = time.perf_counter() - t0
+        res["entropy_coder"] = {
+            "reference_GBps": round(2.0 * raw / tr / 1e9, 5), "oracle_GBps": round(2.0 * raw / to / 1e9, 5),
+            "kind": "reference", "cores": 1,
+            "sample": f"the symbol streams of bricks 0..{k - 1} ({raw} raw bytes) coded and decoded by the "
+                      f"reference's src/CPU coder (CPU dialect) and by the oracle's coder (GPU dialect), one thread"}
+    return res
 
 
 # ------------------------------------------------------------------------------------ bricks
@@ -256,20 +479,12 @@ def run_bricks(args, torch, cc, lib):
 
     cpu = None
     if not args.no_cpu_baseline:
-        from oracle import pipeline as P
-        runs, t_total, done_bytes = 0, 0.0, 0
-        while t_total < 10.0 and runs < len(bricks):
-            b = bricks[runs]
-            t0 = time.perf_counter()
-            ow = P.compress_volume_float(b, 2, steps_q[runs])
-            P.decompress_volume_float(ow, b.shape, 2, steps_q[runs])
-            t_total += time.perf_counter() - t0
-            done_bytes += b.nbytes
-            runs += 1
-        cpu = {"value": round(2.0 * done_bytes / t_total / 1e9, 5), "unit": "GB/s", "cores": 1, "kind": "port",
-               "sample": f"the first {runs} bricks ({done_bytes} raw bytes) compressed and decompressed one by one "
-                         f"through the oracle (C++ restatement of the reference path, one host thread), {t_total:.2f} s"}
+        cpu = cpu_baseline_bricks(bricks, steps_q, args.cpu_seconds)
 
+    coding = None
+    if plans:
+        coding = plans[0].codingStats()
+        coding["pipeline_bytes_per_step"] = pipeline_bytes(coding["N"], coding["M"], coding["N_H"], coding["C"], 2)
     for p in plans:
         p.close()
     del extra
@@ -286,7 +501,7 @@ def run_bricks(args, torch, cc, lib):
            "concurrent_plans_equal": concurrent_equal,
            "kernel_timing": kernel_source,
            "plain": plain, "batched_vs_plain": round(value / plain["GBps"], 2) if plain else None,
-           "bitstreams_equal_plain_sample": equal}
+           "bitstreams_equal_plain_sample": equal, "coding": coding}
     return value, ms, raw_bytes, kernels, cfg, cpu, "f32"
 
 
@@ -491,6 +706,16 @@ def main():
         roof["achieved"] = round(per / (roof["avg_launch_us"] * 1e-6) / 1e9, 2)
         roof["frac"] = round(roof["achieved"] / HBM_PEAK_GBS, 5)
     cfg["gpu_kernel_ms_per_step"] = round(gpu_ms, 4)
+    if cfg.get("coding"):
+        pb = cfg["coding"]["pipeline_bytes_per_step"]["total"]
+        ach = pb / (ms * 1e-3) / 1e9
+        roof["pipeline"] = {"algorithmic_bytes_per_step": pb, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
+                            "formula": "per stage one read of its input + one write of its output: DWT 2bNg, "
+                                       "quantize (b+s)N, RLE s(N-N_H)+2sM, histogram s(2M+N_H), Huffman "
+                                       "s(2M+N_H)+C; decode mirrors all but the histogram (b=4, s=2)"}
+    if args.workload == "bricks" and not args.no_configs:
+        cfg["configs"] = run_configs(args, torch, cc)
     res = {
         "metric": METRIC, "value": round(value, 3), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",

@@ -42,6 +42,7 @@ SIGNATURES = {
     "cc_decode_rlhuff": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp, c_u32, c_vp, c_vp, c_u32, c_u32]),
     "cc_decode_huff": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp, c_u32, c_vp, c_vp, c_u32, c_u32]),
     "cc_dwt_float_forward": (c_int, [c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_vp]),
+    "cc_dwt_double_forward": (c_int, [c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_vp]),
     "cc_dwt_float_inverse": (c_int, [c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_vp]),
     "cc_dwt_float_inverse_from_symbols": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_float, c_int, c_int, c_int,
                                                    c_int, c_int, c_int, c_int, c_int, c_vp]),
@@ -104,6 +105,7 @@ SIGNATURES = {
     "cc_bricks_plan_launch": (c_int, [c_vp]),
     "cc_bricks_plan_wait": (c_int, [c_vp, P_u64, P_u32, P_u32]),
     "cc_bricks_plan_destroy": (None, [c_vp]),
+    "cc_bricks_plan_coding_stats": (c_int, [c_vp, P_u64]),
     "cc_server_create": (c_int, [c_vp, c_vp, c_u32, P_u64, P_u32, ctypes.POINTER(c_float), c_u32, c_int, c_u32, c_u64,
                                  ctypes.POINTER(c_vp)]),
     "cc_server_submit": (c_int, [c_vp, c_u64, P_u32, c_u32, ctypes.POINTER(c_vp), ctypes.c_double]),

@@ -474,6 +474,14 @@ class BricksPlan:
             _check(st, "cc_bricks_plan_wait")
         return offs, cst[:self._n], dst[:self._n]
 
+    def codingStats(self):
+        """Counts of the last compression (cc_bricks_plan_coding_stats): dict with N (symbols),
+        M (zero-run pairs), N_H (Huffman-only symbols) and C (compressed bytes)."""
+        s = np.zeros(4, dtype=np.uint64)
+        _check(self._lib.cc_bricks_plan_coding_stats(self.handle, s.ctypes.data_as(_capi.P_u64)),
+               "cc_bricks_plan_coding_stats")
+        return {"N": int(s[0]), "M": int(s[1]), "N_H": int(s[2]), "C": int(s[3]) * 4}
+
     def close(self):
         if self.handle:
             self._lib.cc_bricks_plan_destroy(self.handle)
@@ -592,6 +600,13 @@ class util:
         _check(_capi.load().cc_dwt_float_forward(_ptr(dpDest), _ptr(dpBuffer2), _ptr(dpBuffer1), _ptr(dpSource),
                                                  sizeX, sizeY, sizeZ, srcChannelCount, dstRowPitch, dstSlicePitch,
                                                  srcRowPitch, srcSlicePitch, _stream(stream)), "dwtFloat3DForward")
+
+    @staticmethod
+    def dwtDouble3DForward(dpDest, dpBuffer, dpSource, sizeX, sizeY, sizeZ, rowPitch=0, slicePitch=0, stream=None):
+        """One 9/7 analysis level in float64 (cc_dwt_double_forward; no reference counterpart): the
+        FP64 twin of dwtFloat3DForward / dwtFloat2DForward (sizeZ 1) that states the FP32 error."""
+        _check(_capi.load().cc_dwt_double_forward(_ptr(dpDest), _ptr(dpBuffer), _ptr(dpSource), sizeX, sizeY, sizeZ,
+                                                  rowPitch, slicePitch, _stream(stream)), "dwtDouble3DForward")
 
     @staticmethod
     def dwtFloat2DForward(dpDest, dpBuffer, dpSource, sizeX, sizeY, srcChannelCount=1, dstRowPitch=0, srcRowPitch=0,

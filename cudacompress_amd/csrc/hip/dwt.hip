@@ -325,6 +325,33 @@ void dwt97_inverse_from_symbols_to_byte(uint8_t* dst, float* buf2, float* buf1, 
                                  lpSlice, st);
 }
 
+// One level of the 9/7 analysis at double precision (no reference counterpart; the FP64 twin of
+// dwtFloat3DForward / dwtFloat2DForward used to state the FP32 transform's error).  dst and src
+// share the row and slice pitches and must not overlap.  3D: X src -> dst, Y dst -> buf,
+// Z buf -> dst; 2D: X src -> buf, Y buf -> dst.
+void dwt97_forward_f64(double* dst, double* buf, const double* src, int sx, int sy, int sz, int row, int slice,
+                       hipStream_t st) {
+    check_dims(sx, sy, sz);
+    CCA_REQUIRE(dst && buf && src, CC_ERR_INVALID_ARG, "null DWT buffer");
+    if (row <= 0) row = sx;
+    if (slice <= 0) slice = sy * row;
+    CCA_REQUIRE(row >= sx && slice >= sy * row, CC_ERR_INVALID_ARG, "pitches smaller than the level");
+    const bool do3d = sz > 1;
+    const uint64_t bytes = (uint64_t)sx * sy * sz * 16;
+    double* xdst = do3d ? dst : buf;
+    CCA_LAUNCH("dwt97d_fwd_x", bytes, (fwd_x<Fwd97d, 1>), x_grid(sy * sz), dim3(256), 0, st, xdst, src, sx, sy * sz,
+               sy, row, slice, row, slice);
+    CCA_LAUNCH_CHECK();
+    CCA_LAUNCH("dwt97d_fwd_y", bytes, (fwd_s<Fwd97d>), s_grid(sx, sy, sz), dim3(256), 0, st, do3d ? buf : dst, xdst,
+               sx, sy, (long long)row, (long long)slice);
+    CCA_LAUNCH_CHECK();
+    if (do3d) {
+        CCA_LAUNCH("dwt97d_fwd_z", bytes, (fwd_s<Fwd97d>), s_grid(sx, sz, sy), dim3(256), 0, st, dst, buf, sx, sz,
+                   (long long)slice, (long long)row);
+        CCA_LAUNCH_CHECK();
+    }
+}
+
 // dwtFloat2DForwardLowpassOnlyFromByte (DWTFloat.cu:18-81, :535-552; kernels in
 // src/cudaCompress/util/DWTFloat2DLowpassKernels.cui): the X pass writes only the row lowpass
 // halves to buf (row pitch dRow), the Y pass only the column lowpass halves to dst.  The sums are

@@ -76,6 +76,28 @@ struct Fwd97 {
         hi = s;
     }
 };
+// FP64 twin of Fwd97 (no reference counterpart): the decimal filter coefficients of
+// DWTFloatFilters.cuh:42-49 at double precision, fused multiply-adds in the same filter order.
+// It measures the error of the FP32 transform; nothing on the coding path uses it.
+__constant__ double c_aLd[9] = {0.0267488, -0.0168641, -0.0782233, 0.2668641, 0.6029490,
+                                0.2668641, -0.0782233, -0.0168641, 0.0267488};
+__constant__ double c_aHd[9] = {0.0, 0.0456359, -0.0287718, -0.2956359, 0.5575435,
+                                -0.2956359, -0.0287718, 0.0456359, 0.0};
+struct Fwd97d {
+    using T = double;
+    static constexpr int H = 4;
+    __device__ static void pair(const T* w, int q, T& lo, T& hi) {
+        const T* x = w + 2 * q;
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < 9; i++) s = __builtin_fma(c_aLd[i], x[i], s);
+        lo = s;
+        s = 0.0;
+#pragma unroll
+        for (int i = 0; i < 9; i++) s = __builtin_fma(c_aHd[i], x[i + 1], s);
+        hi = s;
+    }
+};
 struct Inv97 {
     using T = float;
     static constexpr int G = 2;

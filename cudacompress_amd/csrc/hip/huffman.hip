@@ -2,8 +2,10 @@
 //
 // Histogram (src/cudaCompress/Histogram.cu:110-289): exact per-stream counts.  Each workgroup
 // privatises up to 16384 bins in LDS (64 KB), counts a 32768-symbol chunk with LDS atomics and
-// flushes non-zero bins with one global atomicAdd each; symbols >= 16384 go straight to global
-// atomics.
+// flushes non-zero bins with one global atomicAdd each.  Alphabets of up to 2^16 bins (log2 16)
+// use 16-bit counters, two per LDS word (128 KB of CDNA4's 160 KB), over chunks of fewer than
+// 2^16 symbols so that a counter cannot carry into its neighbour; only larger alphabets (Symbol32)
+// send the symbols beyond the LDS bins to global atomics.
 //
 // Encode (src/cudaCompress/Huffman.cu:73-187, HuffmanKernels.cui:36-190): the bitstream is the
 // concatenation of codewords MSB-first in 32-bit words, the same bits a sequential writeBits
@@ -28,6 +30,9 @@ namespace {
 
 constexpr int kHistLds = 16384;
 constexpr uint32_t kHistChunk = 32768;
+constexpr uint32_t kHistWideBins = 65536;            // 16-bit counters, two per LDS word
+constexpr uint32_t kHistWideChunk = 60 * 1024;       // < 2^16 symbols per workgroup
+constexpr uint32_t kHistWideThreads = 1024;
 constexpr int kPackSyms = 2048;  // symbols per pack workgroup (8 per thread)
 constexpr int kLutBits = 10;
 
@@ -53,6 +58,33 @@ __global__ __launch_bounds__(256) void k_hist(const HistStream* __restrict__ str
     for (uint32_t i = threadIdx.x; i < nl; i += 256) {
         uint32_t c = h[i];
         if (c) atomicAdd(&d.hist[i], c);
+    }
+    if (d.quirk && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&d.hist[65535], 1u);
+}
+
+// Alphabets of up to 2^16 bins: bin v counts in half (v & 1) of LDS word v >> 1.  A chunk holds
+// fewer than 2^16 symbols, so no half overflows into the other.
+template <typename S>
+__global__ __launch_bounds__(kHistWideThreads) void k_hist_wide(const HistStream* __restrict__ streams) {
+    const HistStream d = streams[blockIdx.y];
+    const uint32_t start = blockIdx.x * kHistWideChunk;
+    if (start >= d.n && !(blockIdx.x == 0 && d.quirk)) return;
+    extern __shared__ uint32_t h2[];
+    const uint32_t nl = min(d.nbins, kHistWideBins);
+    const uint32_t nw = (nl + 1) / 2;
+    for (uint32_t i = threadIdx.x; i < nw; i += kHistWideThreads) h2[i] = 0;
+    __syncthreads();
+    const uint32_t end = min(start + kHistWideChunk, d.n);
+    for (uint32_t i = start + threadIdx.x; i < end; i += kHistWideThreads) {
+        const uint32_t v = ld<S>(d.data, i);
+        if (v < nl) atomicAdd(&h2[v >> 1], (v & 1u) ? 65536u : 1u);
+        else if (v < d.nbins) atomicAdd(&d.hist[v], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nw; i += kHistWideThreads) {
+        const uint32_t w = h2[i];
+        if (w & 0xFFFFu) atomicAdd(&d.hist[2 * i], w & 0xFFFFu);
+        if (w >> 16) atomicAdd(&d.hist[2 * i + 1], w >> 16);
     }
     if (d.quirk && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&d.hist[65535], 1u);
 }
@@ -249,11 +281,24 @@ void histogram(const HistStream* d_streams, const HistStream* h_streams, uint32_
                hipStream_t st) {
     if (nstreams == 0) return;
     CCA_REQUIRE(nstreams <= 65535u, CC_ERR_INVALID_ARG, "too many streams for one launch");
+    uint32_t maxbins = 0;
+    for (uint32_t i = 0; i < nstreams; i++) maxbins = std::max(maxbins, h_streams[i].nbins);
+    const bool wide = maxbins > (uint32_t)kHistLds;
+    const uint32_t chunk = wide ? kHistWideChunk : kHistChunk;
     uint32_t maxchunks = 1;
-    for (uint32_t i = 0; i < nstreams; i++) maxchunks = std::max(maxchunks, div_up(h_streams[i].n, kHistChunk));
+    for (uint32_t i = 0; i < nstreams; i++) maxchunks = std::max(maxchunks, div_up(h_streams[i].n, chunk));
     dim3 grid(maxchunks, nstreams);
     uint64_t bytes = 0;
     for (uint32_t i = 0; i < nstreams; i++) bytes += (uint64_t)h_streams[i].n * symSize + 4ull * h_streams[i].nbins;
+    if (wide) {
+        const size_t lds = (std::min(maxbins, kHistWideBins) + 1) / 2 * sizeof(uint32_t);
+        if (symSize == 2)
+            CCA_LAUNCH("histogram", bytes, k_hist_wide<uint16_t>, grid, dim3(kHistWideThreads), lds, st, d_streams);
+        else
+            CCA_LAUNCH("histogram", bytes, k_hist_wide<uint32_t>, grid, dim3(kHistWideThreads), lds, st, d_streams);
+        CCA_LAUNCH_CHECK();
+        return;
+    }
     if (symSize == 2) CCA_LAUNCH("histogram", bytes, k_hist<uint16_t>, grid, dim3(256), 0, st, d_streams);
     else CCA_LAUNCH("histogram", bytes, k_hist<uint32_t>, grid, dim3(256), 0, st, d_streams);
     CCA_LAUNCH_CHECK();

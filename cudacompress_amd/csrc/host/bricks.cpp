@@ -566,6 +566,30 @@ void plan_decompress(cc_bricks_plan* P, uint32_t* status) {
     plan_decode_results(P, status);
 }
 
+void plan_coding_stats(cc_bricks_plan* P, uint64_t* out) {
+    DeviceGuard guard(P->inst->device);
+    CCA_REQUIRE(P->encoded, CC_ERR_INVALID_ARG, "the plan has not compressed anything yet");
+    CCA_HIP(hipStreamSynchronize(P->st));
+    const BrickJob& job = P->gr.jobs[0];
+    const DeviceCoder& D = P->gr.ctx[0]->coder;
+    std::vector<CodeItem> items(job.ej.nitems);
+    std::vector<SeqInfo> seqs(job.ej.nseq);
+    CCA_HIP(hipMemcpy(items.data(), D.d_items.p, items.size() * sizeof(CodeItem), hipMemcpyDeviceToHost));
+    CCA_HIP(hipMemcpy(seqs.data(), D.d_seqs.p, seqs.size() * sizeof(SeqInfo), hipMemcpyDeviceToHost));
+    uint64_t n = 0, m = 0, nh = 0;
+    for (const CodeItem& it : items) {
+        n += it.n;
+        if (it.rl) m += seqs[it.seq].count;  // pairs: symbols and zero counts are one pair each
+        else nh += it.n;
+    }
+    CoderState s{};
+    CCA_HIP(hipMemcpy(&s, D.d_state.p, sizeof(CoderState), hipMemcpyDeviceToHost));
+    out[0] = n;
+    out[1] = m;
+    out[2] = nh;
+    out[3] = s.out_words;
+}
+
 void plan_destroy(cc_bricks_plan* P) { delete P; }
 const void* plan_owner(const cc_bricks_plan* P) { return P ? P->inst : nullptr; }
 

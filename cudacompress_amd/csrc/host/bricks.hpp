@@ -79,6 +79,9 @@ void plan_run(cc_bricks_plan* P, uint64_t* offsets, uint32_t* cstatus, uint32_t*
 void plan_launch(cc_bricks_plan* P);
 void plan_wait(cc_bricks_plan* P, uint64_t* offsets, uint32_t* cstatus, uint32_t* dstatus);
 void plan_destroy(cc_bricks_plan* P);
+// out[4]: symbols coded (N), zero-run pairs of the RLE streams (M), symbols of the Huffman-only
+// streams (N_H), compressed words (C / 4) of the plan's last compression
+void plan_coding_stats(cc_bricks_plan* P, uint64_t* out);
 // the instance a plan belongs to (its launches are that instance's in the kernel timing)
 const void* plan_owner(const cc_bricks_plan* P);
 

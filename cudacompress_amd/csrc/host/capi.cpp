@@ -268,6 +268,13 @@ cc_status cc_bricks_plan_wait(cc_bricks_plan* plan, uint64_t* brickWordOffsets, 
 
 void cc_bricks_plan_destroy(cc_bricks_plan* plan) { cca::plan_destroy(plan); }
 
+cc_status cc_bricks_plan_coding_stats(cc_bricks_plan* plan, uint64_t* stats) {
+    return guarded_for(cca::plan_owner(plan), [&] {
+        CCA_REQUIRE(plan && stats, CC_ERR_INVALID_ARG, "null argument");
+        cca::plan_coding_stats(plan, stats);
+    });
+}
+
 cc_status cc_server_create(cc_instance* inst, const uint32_t* d_words, uint32_t brickCount,
                            const uint64_t* brickWordRanges, const uint32_t* brickSizes, const float* quantSteps,
                            uint32_t numLevels, int symbolSize, uint32_t slots, uint64_t workspaceBytes,
@@ -349,6 +356,13 @@ cc_status cc_dwt_float_forward(float* d_dst, float* d_buffer2, float* d_buffer1,
     return guarded([&] {
         cca::dwt97_forward(d_dst, d_buffer2, d_buffer1, d_src, sizeX, sizeY, sizeZ, srcChannelCount, dstRowPitch,
                            dstSlicePitch, srcRowPitch, srcSlicePitch, as_stream(stream));
+    });
+}
+cc_status cc_dwt_double_forward(double* d_dst, double* d_buffer, const double* d_src, int sizeX, int sizeY, int sizeZ,
+                                int rowPitch, int slicePitch, void* stream) {
+    return guarded([&] {
+        CCA_REQUIRE(d_dst != d_src, CC_ERR_INVALID_ARG, "d_dst and d_src must not overlap");
+        cca::dwt97_forward_f64(d_dst, d_buffer, d_src, sizeX, sizeY, sizeZ, rowPitch, slicePitch, as_stream(stream));
     });
 }
 cc_status cc_dwt_float_inverse(float* d_dst, float* d_buffer2, float* d_buffer1, const float* d_src, int sizeX,

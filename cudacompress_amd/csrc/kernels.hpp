@@ -8,6 +8,9 @@
 namespace cca {
 
 // ---- DWT (src/cudaCompress/util/DWTFloat.cu, DWTInt.cu) -------------------------------------
+// one 9/7 analysis level in FP64 (no reference counterpart; measures the FP32 transform's error)
+void dwt97_forward_f64(double* dst, double* buf, const double* src, int sx, int sy, int sz, int row, int slice,
+                       hipStream_t st);
 void dwt97_forward(float* dst, float* buf2, float* buf1, const float* src, int sx, int sy, int sz, int srcCh,
                    int dRow, int dSlice, int sRow, int sSlice, hipStream_t st);
 void dwt97_inverse(float* dst, float* buf2, float* buf1, const float* src, int sx, int sy, int sz, int dstCh,

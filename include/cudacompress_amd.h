@@ -152,6 +152,13 @@ CC_API cc_status cc_dwt_float_forward(float* d_dst, float* d_buffer2, float* d_b
                                       int dstRowPitch, int dstSlicePitch, int srcRowPitch, int srcSlicePitch,
                                       void* stream);
 /* dwtFloat3DInverse / dwtFloat2DInverse, util/DWT.h:35-45 */
+/* No reference counterpart: one level of the CDF 9/7 analysis of dwtFloat3DForward /
+ * dwtFloat2DForward (src/cudaCompress/util/DWTFloat.cu:85-202) in double precision, with the
+ * filter coefficients as decimal doubles, to state the error of the fp32 transform against an
+ * fp64 one.  d_dst and d_src share rowPitch and slicePitch (elements; 0: dense) and must not
+ * overlap; d_buffer holds sizeX*sizeY*sizeZ at the same pitches.  sizeZ = 1: 2D. */
+CC_API cc_status cc_dwt_double_forward(double* d_dst, double* d_buffer, const double* d_src, int sizeX, int sizeY,
+                                       int sizeZ, int rowPitch, int slicePitch, void* stream);
 CC_API cc_status cc_dwt_float_inverse(float* d_dst, float* d_buffer2, float* d_buffer1, const float* d_src,
                                       int sizeX, int sizeY, int sizeZ, int dstChannelCount,
                                       int dstRowPitch, int dstSlicePitch, int srcRowPitch, int srcSlicePitch,
@@ -387,6 +394,11 @@ CC_API cc_status cc_bricks_plan_launch(cc_bricks_plan* plan);
 CC_API cc_status cc_bricks_plan_wait(cc_bricks_plan* plan, uint64_t* brickWordOffsets, uint32_t* compressStatus,
                                      uint32_t* decompressStatus);
 CC_API void cc_bricks_plan_destroy(cc_bricks_plan* plan);
+/* Counts of the plan's last compression (no reference counterpart), from which the algorithmic
+ * bytes of every stage follow: stats[0] symbols coded (N), stats[1] zero-run pairs of the
+ * run-length streams (M; each pair is one Huffman symbol and one zero count), stats[2] symbols of
+ * the Huffman-only streams (N_H), stats[3] compressed words (C / 4).  Synchronises the plan. */
+CC_API cc_status cc_bricks_plan_coding_stats(cc_bricks_plan* plan, uint64_t* stats);
 
 /* ---- Decode serving ----------------------------------------------------------------------- */
 /* A request loop over compressed bricks resident in device memory (no reference counterpart).

@@ -213,3 +213,17 @@ def precision_report(vol, levels, q0, boundary_tol=1e-5):
     return {"dwt_max_abs": float(np.abs(d).max()),
             "dwt_rel_l2": float(np.sqrt((d * d).sum() / max((c64 * c64).sum(), 1e-300))),
             "near_boundary": near, "symbols_differ_fp64": differ, "coefficients": int(c64.size)}
+
+
+def volume_float_calls(vol, levels, q0, dtype=np.uint16):
+    """The coder calls of compressVolumeFloat (CompressVolume.cpp:355-400) as (rl, streams): the
+    lowpass band Huffman coded, then per level (coarsest first) the 7 highpass bands run-length +
+    Huffman coded.  The entropy half of the pipeline, for timing coders on the same symbols."""
+    coef = dwt97_levels(vol, levels, np.float32)
+    calls = []
+    bs = band_steps(vol.shape, levels, q0)
+    calls.append((False, [o.quantize(np.ascontiguousarray(coef[bs[0][0]]).ravel(), bs[0][1], dtype=dtype)]))
+    for lv in range(levels):
+        part = bs[1 + 7 * lv: 8 + 7 * lv]
+        calls.append((True, [o.quantize(np.ascontiguousarray(coef[sl]).ravel(), q, dtype=dtype) for sl, q in part]))
+    return calls

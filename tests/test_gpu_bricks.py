@@ -193,6 +193,23 @@ def test_bricks_plan_graph_replays(cc, torch_gpu, field):
         assert torch.equal(out[:nw], out_b[:nw])
         for a, b in zip(dsts, dsts_b):
             assert torch.equal(a, b)
+    # coding statistics of the last compression against the oracle's streams
+    st = plan.codingStats()
+    from oracle import pyoracle as o
+    n = m = nh = 0
+    for s_, q in zip(srcs, steps):  # the sources as last compressed (changed in the third replay)
+        b = s_.cpu().numpy()
+        coef = P.dwt97_levels(b, 2)
+        for k, (sl, qb) in enumerate(P.band_steps(b.shape, 2, q)):
+            syms = o.quantize(np.ascontiguousarray(coef[sl]).ravel(), qb)
+            n += syms.size
+            if k == 0:
+                nh += syms.size           # the lowpass band: Huffman only
+            else:
+                m += o.rle_encode(syms)[0].size
+    log(f"   stats {st}; oracle N={n} M={m} N_H={nh}")
+    assert (st["N"], st["M"], st["N_H"]) == (n, m, nh)
+    assert st["C"] == 4 * int(ra[0][-1])
     plan_b.close()
     plan.close()
     inst.close()

@@ -250,3 +250,35 @@ def test_max_abs(cc, torch_gpu):
     torch.cuda.synchronize()
     assert float(out.item()) == float(np.abs(data).max())
     inst.close()
+
+
+@pytest.mark.parametrize("shape", [(64, 48, 32), (16, 16, 128), (96, 64)], ids=["64x48x32", "16x16x128", "2d96x64"])
+def test_dwt97_forward_f64(cc, torch_gpu, shape):
+    """The FP64 twin of the 9/7 analysis (cc_dwt_double_forward) against the oracle's FP64
+    transform: both sum the decimal coefficients with fused multiply-adds in filter order, so
+    they agree to rounding (1e-12 relative); pitched levels use the leading corner of a volume."""
+    torch = torch_gpu
+    log("dwt97 f64", shape)
+    vol = S.gaussian_field(shape, 7).astype(np.float64)
+    dims = (shape[-1], shape[-2], shape[0] if len(shape) == 3 else 1)
+    src = torch.from_numpy(vol).cuda()
+    dst = torch.zeros_like(src)
+    buf = torch.zeros_like(src)
+    cc.util.dwtDouble3DForward(dst, buf, src, *dims)
+    torch.cuda.synchronize()
+    want = o.dwt97_forward(vol, dtype=np.float64)
+    err = float(np.abs(to_host(dst) - want).max())
+    log(f"   max|err|={err:.3e}")
+    assert err <= 1e-12 * max(1.0, float(np.abs(want).max()))
+    # second level on the leading corner, pitched like the float transform's levels
+    half = tuple(n // 2 for n in shape)
+    corner = np.ascontiguousarray(want[tuple(slice(0, h) for h in half)])
+    row = shape[-1]
+    slab = shape[-1] * shape[-2] if len(shape) == 3 else 0
+    d2 = torch.zeros_like(src)
+    cc.util.dwtDouble3DForward(d2, buf, torch.from_numpy(want).cuda(), dims[0] // 2, dims[1] // 2,
+                               max(1, dims[2] // 2) if len(shape) == 3 else 1, row, slab)
+    torch.cuda.synchronize()
+    got = to_host(d2)[tuple(slice(0, h) for h in half)]
+    want2 = o.dwt97_forward(corner, dtype=np.float64)
+    assert float(np.abs(got - want2).max()) <= 1e-12 * max(1.0, float(np.abs(want2).max()))
