@@ -30,6 +30,8 @@ from .api import (  # noqa: E402
     compressVolumeFloatQuantFirst,
     compressVolumeLossless,
     createInstance,
+    defaultTuning,
+    loadTuning,
     decodeDevice,
     decodeHuff,
     decodeRLHuff,

@@ -106,6 +106,10 @@ SIGNATURES = {
     "cc_bricks_plan_wait": (c_int, [c_vp, P_u64, P_u32, P_u32]),
     "cc_bricks_plan_destroy": (None, [c_vp]),
     "cc_bricks_plan_coding_stats": (c_int, [c_vp, P_u64]),
+    "cc_default_tuning": (c_int, [c_vp]),
+    "cc_load_tuning": (c_int, [ctypes.c_char_p, c_vp]),
+    "cc_set_instance_tuning": (c_int, [c_vp, c_vp]),
+    "cc_get_instance_tuning": (c_int, [c_vp, c_vp]),
     "cc_server_create": (c_int, [c_vp, c_vp, c_u32, P_u64, P_u32, ctypes.POINTER(c_float), c_u32, c_int, c_u32, c_u64,
                                  ctypes.POINTER(c_vp)]),
     "cc_server_submit": (c_int, [c_vp, c_u64, P_u32, c_u32, ctypes.POINTER(c_vp), ctypes.c_double]),
@@ -121,6 +125,11 @@ SIGNATURES = {
     "cc_profile_collect": (c_u32, []),
     "cc_profile_entry": (c_int, [c_u32, ctypes.POINTER(ctypes.c_char_p), P_u64, ctypes.POINTER(ctypes.c_double), P_u64]),
 }
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("designGrid", c_u32 * 4), ("fusedXY", c_u32), ("xySlicesMax", c_u32), ("codingBlockSize", c_u32),
+                ("reserved", c_u32)]
 
 
 class Brick(ctypes.Structure):

@@ -62,6 +62,32 @@ def _elem_size(t):
 
 
 # ------------------------------------------------------------------------------------ Instance
+def tuning_to_dict(t):
+    return {"designGrid": [int(g) for g in t.designGrid], "fusedXY": int(t.fusedXY), "xySlicesMax": int(t.xySlicesMax),
+            "codingBlockSize": int(t.codingBlockSize)}
+
+
+def tuning_from_dict(d):
+    t = _capi.Tuning()
+    for i, g in enumerate(d["designGrid"]):
+        t.designGrid[i] = int(g)
+    t.fusedXY, t.xySlicesMax, t.codingBlockSize = int(d["fusedXY"]), int(d["xySlicesMax"]), int(d["codingBlockSize"])
+    return t
+
+
+def loadTuning(path=None):
+    """A tuning table parsed by the library (cc_load_tuning); None: the library's own table."""
+    t = _capi.Tuning()
+    _check(_capi.load().cc_load_tuning(None if path is None else str(path).encode(), ctypes.byref(t)), "cc_load_tuning")
+    return tuning_to_dict(t)
+
+
+def defaultTuning():
+    t = _capi.Tuning()
+    _check(_capi.load().cc_default_tuning(ctypes.byref(t)), "cc_default_tuning")
+    return tuning_to_dict(t)
+
+
 class Instance:
     """cudaCompress::Instance (include/cudaCompress/Instance.h:14-45)."""
 
@@ -123,6 +149,17 @@ class Instance:
 
     def setDialect(self, dialect):
         _check(_capi.load().cc_set_instance_dialect(self._h, dialect), "setDialect")
+
+    def getTuning(self):
+        """The kernel shapes brick batches and plans of this instance use (cc_get_instance_tuning)."""
+        t = _capi.Tuning()
+        _check(_capi.load().cc_get_instance_tuning(self.handle, ctypes.byref(t)), "cc_get_instance_tuning")
+        return tuning_to_dict(t)
+
+    def setTuning(self, tuning=None):
+        """Sets the kernel shapes (a dict as getTuning returns; None: the library's tuning table)."""
+        t = None if tuning is None else ctypes.byref(tuning_from_dict(tuning))
+        _check(_capi.load().cc_set_instance_tuning(self.handle, t), "cc_set_instance_tuning")
 
     def setBrickGroups(self, groups):
         """Concurrent groups of a brick batch (0: chosen from the batch size)."""

@@ -172,7 +172,7 @@ void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uin
                   CallOut* d_calls,
                   uint32_t ncalls, const uint32_t* d_call_first_item, uint32_t* d_out, uint64_t out_cap,
                   uint64_t total_symbols, hipStream_t st, const SideStreams* side,
-                  const LayoutLink* link = nullptr);
+                  const LayoutLink* link = nullptr, const uint32_t* design_grid = nullptr);
 // wave_seq: for every decode wave the coded sequence it serves (upper bound of the block count:
 // RLE sequences have at most n pairs); wave_first: that wave's first block.
 void coder_decode(const DecCall* d_calls, uint32_t ncalls, const DecItem* d_items, uint32_t nitems, DecSeq* d_seqs,

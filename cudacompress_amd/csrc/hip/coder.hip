@@ -994,7 +994,8 @@ void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uin
                   int symSize, uint32_t* scratch, uint64_t scratch_cap, CoderState* d_state, uint32_t* d_classes,
                   CallOut* d_calls,
                   uint32_t ncalls, const uint32_t* d_call_first_item, uint32_t* d_out, uint64_t out_cap,
-                  uint64_t total_symbols, hipStream_t st, const SideStreams* side, const LayoutLink* link) {
+                  uint64_t total_symbols, hipStream_t st, const SideStreams* side, const LayoutLink* link,
+                  const uint32_t* design_grid) {
     CCA_REQUIRE(symSize == 2 || symSize == 4, CC_ERR_INVALID_ARG, "symbol size must be 2 or 4");
     uint64_t* item_off = reinterpret_cast<uint64_t*>(scratch + scratch_cap);  // caller reserves the tail
     CCA_LAUNCH("coder_init", 32, k_coder_init, dim3(1), dim3(64), 0, st, d_state, d_classes);
@@ -1027,7 +1028,9 @@ void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uin
     constexpr int kClasses = 4;
     static_assert(kClasses <= kSideStreams, "one side stream per design class");
     const uint32_t kmax[kClasses] = {256, 1024, 2048, kDesignMid};
-    const uint32_t grid[kClasses] = {4096, 2048, 1536, 768};  // resident workgroups per class (LDS-bound)
+    uint32_t grid[kClasses] = {4096, 2048, 1536, 768};  // resident workgroups per class (LDS-bound)
+    if (design_grid)
+        for (int c = 0; c < kClasses; c++) grid[c] = std::max(1u, design_grid[c]);
     const char* names[kClasses] = {"huff_design_k256", "huff_design_k1024", "huff_design_k2048",
                                    "huff_design_k4096"};
     if (side) {

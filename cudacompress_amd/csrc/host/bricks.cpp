@@ -94,7 +94,8 @@ std::vector<ItemSpec> brick_items(const BrickJob& job, unsigned char* sym) {
     return items;
 }
 
-void upload_levels(BrickCtx& ctx, BrickJob& job, float* work, unsigned char* sym, hipStream_t st) {
+void upload_levels(BrickCtx& ctx, BrickJob& job, float* work, unsigned char* sym, const cc_tuning& tune,
+                   hipStream_t st) {
     const uint32_t n = (uint32_t)job.bricks.size(), L = job.L;
     const size_t lv_bytes = sizeof(BrickLevel) * (size_t)L * n;
     const size_t tb_off = align_up(std::max<size_t>(lv_bytes, 1), 256);
@@ -105,7 +106,8 @@ void upload_levels(BrickCtx& ctx, BrickJob& job, float* work, unsigned char* sym
     job.tiles_total.assign(4 * L, 0);
     for (uint32_t l = 0; l < L; l++) {
         // the fused x+y passes serve a level when every brick's z-slice fits a workgroup's LDS
-        bool fused = true;
+        // (and the tuning keeps them on)
+        bool fused = tune.fusedXY != 0;
         for (uint32_t i = 0; i < n; i++)
             fused = fused && brick_xy_floats((int)(job.bricks[i].sizeX >> l), (int)(job.bricks[i].sizeY >> l)) <=
                                  (uint32_t)kBrickXYFloats;
@@ -131,9 +133,9 @@ void upload_levels(BrickCtx& ctx, BrickJob& job, float* work, unsigned char* sym
             e.inv_hp = 1.0f / e.step_hp;      // Quantize.cu:94
             e.step_lp = q0 / float(1u << (L - 1));
             e.inv_lp = 1.0f / e.step_lp;
-            e.zk = fused ? (int)std::min<uint32_t>((uint32_t)e.lz,
-                                                   std::max<uint32_t>(1u, kBrickXYFloats / brick_xy_floats(e.lx, e.ly)))
-                         : 0;
+            uint32_t zk = std::min<uint32_t>((uint32_t)e.lz, std::max<uint32_t>(1u, kBrickXYFloats / brick_xy_floats(e.lx, e.ly)));
+            if (tune.xySlicesMax) zk = std::min(zk, tune.xySlicesMax);
+            e.zk = fused ? (int)zk : 0;
             for (int ax = 0; ax < 4; ax++) {
                 ht[((size_t)l * 4 + ax) * (n + 1) + i] = acc[ax];
                 acc[ax] += ax < 3 ? brick_tiles(ax, e.lx, e.ly, e.lz) : (fused ? div_up(e.lz, e.zk) : 0u);
@@ -170,7 +172,7 @@ void bricks_prepare(cc_instance* I, BrickCtx& ctx, BrickJob& job, bool enc, bool
     make_layout(job);
     float* work = ctx.d_bwork.as<float>(std::max<uint64_t>(job.work_floats, 64));
     unsigned char* sym = (unsigned char*)ctx.d_bsym.get(std::max<uint64_t>(job.sym_elems, 64) * job.symSize);
-    upload_levels(ctx, job, work, sym, st);
+    upload_levels(ctx, job, work, sym, I->tuning, st);
     const std::vector<ItemSpec> items = brick_items(job, sym);
     if (enc) encode_prepare(I, ctx.coder, items, n, job.symSize, job.ej, st);
     if (dec) {
@@ -194,7 +196,7 @@ void bricks_prepare_bound(cc_instance* I, BrickCtx& ctx, BrickJob& job, const De
     make_layout(job);
     float* work = ctx.d_bwork.as<float>(std::max<uint64_t>(job.work_floats, 64));
     unsigned char* sym = (unsigned char*)ctx.d_bsym.get(std::max<uint64_t>(job.sym_elems, 64) * job.symSize);
-    upload_levels(ctx, job, work, sym, st);
+    upload_levels(ctx, job, work, sym, I->tuning, st);
     decode_prepare_bound(I, ctx.coder, X, brick_items(job, sym), index_calls, job.symSize, job.bj, st);
     job.index = &X;
 }

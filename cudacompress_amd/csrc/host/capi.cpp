@@ -110,6 +110,7 @@ cc_status cc_create_instance(int device, uint32_t streamCountMax, uint32_t elemC
         I->elem_count_per_stream_max = elemCountPerStreamMax;
         I->cbs = codingBlockSize;
         I->log2 = log2HuffmanDistinctSymbolCountMax;
+        I->tuning = cca::library_tuning();
         try {
             cca::DeviceGuard g(device);
             for (auto& e : I->dec_event) {
@@ -382,6 +383,31 @@ cc_status cc_dwt_float_inverse_from_symbols(float* d_dst, float* d_buffer2, floa
         cca::dwt97_inverse_from_symbols(d_dst, d_buffer2, d_buffer1, d_lowpass, dd_highpass, symbolSize, quantStep,
                                         sizeX, sizeY, sizeZ, dstChannelCount, dstRowPitch, dstSlicePitch,
                                         lowpassRowPitch, lowpassSlicePitch, as_stream(stream));
+    });
+}
+cc_status cc_default_tuning(cc_tuning* out) {
+    return guarded([&] {
+        CCA_REQUIRE(out, CC_ERR_INVALID_ARG, "null argument");
+        *out = cca::default_tuning();
+    });
+}
+cc_status cc_load_tuning(const char* path, cc_tuning* out) {
+    return guarded([&] {
+        CCA_REQUIRE(out, CC_ERR_INVALID_ARG, "null argument");
+        *out = path ? cca::load_tuning_file(path) : cca::library_tuning();
+    });
+}
+cc_status cc_set_instance_tuning(cc_instance* inst, const cc_tuning* tuning) {
+    return guarded([&] {
+        CCA_REQUIRE(inst, CC_ERR_INVALID_ARG, "null instance");
+        if (tuning) cca::validate_tuning(*tuning);
+        inst->tuning = tuning ? *tuning : cca::library_tuning();
+    });
+}
+cc_status cc_get_instance_tuning(const cc_instance* inst, cc_tuning* out) {
+    return guarded([&] {
+        CCA_REQUIRE(inst && out, CC_ERR_INVALID_ARG, "null argument");
+        *out = inst->tuning;
     });
 }
 cc_status cc_set_timing_detail(cc_instance* inst, int detail) {

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "../coder.hpp"
@@ -68,6 +69,12 @@ struct BrickCtx {
     void release();
 };
 
+// Tuning of the kernel shapes (host/tuning.cpp).
+cc_tuning default_tuning();
+cc_tuning load_tuning_file(const std::string& path);  // defaults for keys the file lacks or a missing file
+const cc_tuning& library_tuning();                    // the library's table, read once
+void validate_tuning(const cc_tuning& t);
+
 }  // namespace cca
 
 struct cc_instance {
@@ -107,6 +114,7 @@ struct cc_instance {
     std::vector<std::unique_ptr<cca::BrickCtx>> groups;
     hipEvent_t group_fork = nullptr;
     uint32_t brick_groups = 0;  // groups per brick batch, 0: chosen from the batch size
+    cc_tuning tuning{};         // kernel shapes of brick batches and the device coder (cc_set_instance_tuning)
 
     // section timing (cc_set_timing_detail)
     int timing_detail = 0;

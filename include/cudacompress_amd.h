@@ -434,6 +434,34 @@ CC_API cc_status cc_server_run(cc_server* server, uint32_t requestCount, const d
                                const uint32_t* brickIds, float* const* d_outputs, double* latencies, double* elapsed);
 CC_API void cc_server_destroy(cc_server* server);
 
+/* ---- Tuning (no reference counterpart) ---------------------------------------------------- */
+/* Kernel-shape parameters of the brick batches and the device coder, chosen by the project's
+ * autotuner (cudacompress_amd/autotune.py) and kept in a tuning table.  They replace fixed
+ * choices of the reference (the 32x4-thread DWT blocks of src/cudaCompress/util/DWTFloat.cu:92-100,
+ * the default coding block size of src/cudaCompress/InstanceImpl.cpp:65-69).  None of them changes
+ * a bitstream except codingBlockSize, which the table only recommends: an instance's coding block
+ * size is fixed at creation. */
+typedef struct cc_tuning {
+    uint32_t designGrid[4];   /* resident workgroups of the LDS table-design classes (tables of at most
+                                 256, 1024, 2048, 4096 distinct symbols) */
+    uint32_t fusedXY;         /* 1: brick levels whose z-slices fit the LDS use the fused x+y passes;
+                                 0: separate x and y passes everywhere */
+    uint32_t xySlicesMax;     /* most z-slices one fused x+y workgroup holds (0: as many as fit) */
+    uint32_t codingBlockSize; /* recommended coding block size (32..256) */
+    uint32_t reserved;
+} cc_tuning;
+/* The built-in defaults. */
+CC_API cc_status cc_default_tuning(cc_tuning* out);
+/* Parses a tuning table ("key value..." lines: design_grid g0 g1 g2 g3, fused_xy b, xy_slices_max k,
+ * coding_block_size c; '#' comments).  path NULL: the library's table, $CUDACOMPRESS_AMD_TUNING or
+ * else tuning/gfx950.txt beside the library's lib/ directory; keys it lacks keep their defaults, and
+ * a missing file gives the defaults with CC_OK. */
+CC_API cc_status cc_load_tuning(const char* path, cc_tuning* out);
+/* An instance starts with the library's table; brick batches and plans created afterwards use the
+ * instance's tuning.  NULL restores the table. */
+CC_API cc_status cc_set_instance_tuning(cc_instance* inst, const cc_tuning* tuning);
+CC_API cc_status cc_get_instance_tuning(const cc_instance* inst, cc_tuning* out);
+
 /* ---- Kernel timing ------------------------------------------------------------------------ */
 /* No reference counterpart (the reference times host-side sections with CUDA events,
  * src/cudaCompress/util/CudaTimer.cpp:65-169).  When enabled, every kernel launch of this library

@@ -349,3 +349,54 @@ def test_server_index_reports_broken_bricks(cc, torch_gpu, field):
         for j, b in enumerate(reqs[i]):
             assert np.array_equal(outs[i][j].cpu().numpy(), eager[b].cpu().numpy()), (i, j)
     inst.close()
+
+
+def test_tuned_shapes_keep_words_and_values(cc, torch_gpu, field):
+    """Kernel-shape tuning (design grids, separate or fused x+y passes, z-slices per fused
+    workgroup) changes no word and no reconstructed value; a coding block size from the table
+    changes the words but not the values."""
+    log("tuning: shapes keep the words")
+    torch = torch_gpu
+    rng = np.random.default_rng(29)
+    bricks = make_bricks(rng, field, 16, (16, 32, 64))
+    std = float(field.std())
+    steps = [0.005 * std] * len(bricks)
+    shapes = [b.shape for b in bricks]
+    srcs = [torch.from_numpy(b).cuda() for b in bricks]
+    total = sum(b.size for b in bricks)
+    base = cc.defaultTuning()
+    variants = [dict(base), dict(base, designGrid=[64, 64, 64, 64]), dict(base, fusedXY=0),
+                dict(base, xySlicesMax=1), dict(base, xySlicesMax=2, designGrid=[8192, 512, 256, 128])]
+    ref = None
+    for t in variants:
+        inst = cc.createInstance(0, 7, 64 ** 3, 128, 14)
+        inst.setTuning(t)
+        assert inst.getTuning() == t
+        out = torch.zeros(total + 4096 * len(bricks), dtype=torch.int32, device="cuda")
+        dsts = [torch.zeros_like(s) for s in srcs]
+        plan = cc.BricksPlan(inst, srcs, dsts, shapes, steps, 2, out)
+        offs, cst, dst = plan.run()
+        assert np.all(cst == 0) and np.all(dst == 0)
+        got = (out[: int(offs[-1])].cpu().numpy().copy(), [d.cpu().numpy() for d in dsts])
+        if ref is None:
+            ref = got
+        assert np.array_equal(got[0], ref[0]), t
+        for a, b in zip(got[1], ref[1]):
+            assert np.array_equal(a, b), t
+        plan.close()
+        inst.setTuning(None)
+        inst.close()
+    inst = cc.createInstance(0, 7, 64 ** 3, 64, 14)
+    out = torch.zeros(total + 4096 * len(bricks), dtype=torch.int32, device="cuda")
+    dsts = [torch.zeros_like(s) for s in srcs]
+    offs, stat = cc.compressBricksFloat(inst, srcs, shapes, steps, 2, out)
+    assert np.all(stat == 0)
+    cc.decompressBricksFloat(inst, dsts, shapes, steps, 2, out, offs)
+    torch.cuda.synchronize()
+    for i, b in enumerate(bricks):
+        z, y, x = b.shape
+        w = cc.compressVolumeFloat(inst, srcs[i], x, y, z, 2, steps[i])
+        assert np.array_equal(out[int(offs[i]):int(offs[i + 1])].cpu().numpy().view(np.uint32), w)
+    for a, b in zip(dsts, ref[1]):
+        assert np.array_equal(a.cpu().numpy(), b)
+    inst.close()
