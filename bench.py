@@ -132,6 +132,10 @@ def roofline_of(kernels, steps, workload, traffic_file):
     return roof, table, sum(k[2] for k in kernels) / steps
 
 
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def pipeline_bytes(N, M, NH, C, levels, b=4, s=2, dims=3):
     """Algorithmic bytes of one compression + one decompression of the 9/7 pipeline: each stage
     reads its input once and writes its output once.  N coefficients (b bytes), symbols of s bytes,
@@ -357,6 +361,7 @@ def run_bricks(args, torch, cc, lib):
     from cudacompress_amd import synthetic as S
     dev = torch.device("cuda:0")
     bricks = S.brick_set(args.bricks)
+    progress(f"bricks: {len(bricks)} generated")
     shapes = [b.shape for b in bricks]
     raw_bytes = int(sum(b.nbytes for b in bricks))
     steps_q = [0.005 * float(b.std()) for b in bricks]
@@ -409,8 +414,10 @@ def run_bricks(args, torch, cc, lib):
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    progress("bricks: plans ready, timing")
     ms, kernels = timed(torch, lib, step, args.steps)
     drain()
+    progress(f"bricks: {ms:.3f} ms per step")
     value = 2.0 * raw_bytes / (ms * 1e-3) / 1e9
     state["offs"], state["stat"], state["dstat"] = results[0]
     concurrent_equal = None
@@ -449,6 +456,7 @@ def run_bricks(args, torch, cc, lib):
     plain = None
     equal = None
     if not args.no_plain:
+        progress("bricks: plain per-brick sample")
         words = out[: int(offs[-1])].cpu().numpy().view(np.uint32)
         k = min(args.plain_sample, len(bricks))
         pin = cc.createInstance(0, 7, 128 * 128 * 128 // 8, 128, 14)
@@ -479,6 +487,7 @@ def run_bricks(args, torch, cc, lib):
 
     cpu = None
     if not args.no_cpu_baseline:
+        progress("bricks: CPU baseline legs")
         cpu = cpu_baseline_bricks(bricks, steps_q, args.cpu_seconds)
 
     coding = None
@@ -715,6 +724,7 @@ def main():
                                        "quantize (b+s)N, RLE s(N-N_H)+2sM, histogram s(2M+N_H), Huffman "
                                        "s(2M+N_H)+C; decode mirrors all but the histogram (b=4, s=2)"}
     if args.workload == "bricks" and not args.no_configs:
+        progress("configs C1-C3")
         cfg["configs"] = run_configs(args, torch, cc)
     res = {
         "metric": METRIC, "value": round(value, 3), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
