@@ -33,6 +33,7 @@ from .api import (  # noqa: E402
     defaultTuning,
     loadTuning,
     decodeDevice,
+    decodeDeviceNoAlloc,
     decodeHuff,
     decodeRLHuff,
     decompressBricksFloat,

@@ -39,6 +39,7 @@ SIGNATURES = {
     "cc_encode_huff": (c_int, [c_vp, c_int, c_vp, c_u32, c_u32, c_vp, PP_u32, PP_u64, PP_u64]),
     "cc_encode_device": (c_int, [c_vp, c_int, c_int, c_vp, c_u32, c_u32, c_vp, c_u64, P_u64]),
     "cc_decode_device": (c_int, [c_vp, c_int, c_int, c_vp, c_u64, c_vp, c_u32, c_u32, P_u64]),
+    "cc_decode_device_noalloc": (c_int, [c_vp, c_int, c_int, c_vp, c_u64, c_vp, c_u32, c_u32, P_u64]),
     "cc_decode_rlhuff": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp, c_u32, c_vp, c_vp, c_u32, c_u32]),
     "cc_decode_huff": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp, c_u32, c_vp, c_vp, c_u32, c_u32]),
     "cc_dwt_float_forward": (c_int, [c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_vp]),

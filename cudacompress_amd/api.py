@@ -414,6 +414,21 @@ def decodeDevice(inst, rl, words, symbolStreams, symbolCountPerStream, wordCount
     return int(nw.value)
 
 
+def decodeDeviceNoAlloc(inst, rl, words, symbolStreams, symbolCountPerStream, wordCount=None):
+    """decodeDevice with no allocation: the decode kernels' tables and arena live in the library's
+    static per-device scratch array (32 MiB; calls on one device must not overlap).  Raises
+    CCError (status CC_ERR_INVALID_ARG) for a call too large for that array."""
+    lib = _capi.load()
+    ns = len(symbolStreams)
+    ptrs = (ctypes.c_void_p * max(ns, 1))(*[_ptr(s) for s in symbolStreams])
+    nw = ctypes.c_uint64()
+    avail = wordCount if wordCount is not None else words.numel() * words.element_size() // 4
+    st = lib.cc_decode_device_noalloc(inst.handle, 1 if rl else 0, inst.symbolSize, _ptr(words), avail, ptrs, ns,
+                                      symbolCountPerStream, ctypes.byref(nw))
+    _check(st, "cc_decode_device_noalloc")
+    return int(nw.value)
+
+
 def _bricks(srcs, dsts, shapes, steps):
     n = len(shapes)
     arr = (_capi.Brick * max(n, 1))()

@@ -23,11 +23,6 @@ uint64_t arena_estimate(const std::vector<ItemSpec>& items, uint32_t log2, uint3
     return words;
 }
 
-CoderParams params_of(const cc_instance* I) {
-    const HeaderFormat fmt = header_format(I->dialect, I->log2);
-    return CoderParams{I->cbs, I->log2, (uint32_t)I->dialect, fmt.count_bits, fmt.symbits_bits, 0};
-}
-
 // Side streams of a coder context (created on first use, on the current device).
 const SideStreams* side_streams(DeviceCoder& D, SideStreams& out) {
     if (!D.fork) {
@@ -45,14 +40,19 @@ const SideStreams* side_streams(DeviceCoder& D, SideStreams& out) {
     return &out;
 }
 
+}  // namespace
+
+CoderParams params_of(const cc_instance* I) {
+    const HeaderFormat fmt = header_format(I->dialect, I->log2);
+    return CoderParams{I->cbs, I->log2, (uint32_t)I->dialect, fmt.count_bits, fmt.symbits_bits, 0};
+}
+
 void check_symbols(const cc_instance* I, int symSize) {
     const bool longSymbols = I->log2 > 16;
     CCA_REQUIRE(symSize == (longSymbols ? 4 : 2), CC_ERR_INVALID_ARG,
                 "symbol size does not match the instance (log2HuffmanDistinctSymbolCountMax > 16 uses Symbol32)");
     CCA_REQUIRE(I->dialect == CC_DIALECT_GPU || symSize == 2, CC_ERR_INVALID_ARG, "the CPU dialect is Symbol16 only");
 }
-
-}  // namespace
 
 // ------------------------------------------------------------------------------------- encode
 void encode_prepare(cc_instance* I, DeviceCoder& D, const std::vector<ItemSpec>& items, uint32_t ncalls, int symSize,

@@ -18,6 +18,12 @@ struct ItemSpec {
     bool rl;
     uint32_t call;
 };
+
+// Header-format parameters of the instance's coder (coding block size, alphabet, dialect).
+CoderParams params_of(const cc_instance* I);
+// Throws CC_ERR_INVALID_ARG when symSize does not fit the instance (Symbol32 iff log2 > 16; the
+// CPU dialect is Symbol16 only).
+void check_symbols(const cc_instance* I, int symSize);
 struct CallWords {
     const uint32_t* words;  // device bitstream of a decode call
     uint64_t nwords;
@@ -109,5 +115,11 @@ struct DecodeCall {
 };
 void device_decode(cc_instance* I, const std::vector<DecodeCall>& calls, const std::vector<void*>& outs, int symSize,
                    std::vector<DecCallOut>* results, bool wait);
+
+// device_decode of one call of nstreams equal streams with no allocation: every buffer beyond
+// the bitstream and the outputs is carved from a per-device static array (hip/noalloc.hip).
+// Synchronises the instance stream; returns the call's word count and sets *err (DecErr).
+uint64_t decode_device_noalloc(cc_instance* I, bool rl, int symSize, const uint32_t* d_words, uint64_t wordCount,
+                               void* const* outs, uint32_t nstreams, uint32_t n, uint32_t* err);
 
 }  // namespace cca

@@ -202,6 +202,20 @@ cc_status cc_decode_device(cc_instance* inst, int rl, int symbolSize, const uint
     });
 }
 
+cc_status cc_decode_device_noalloc(cc_instance* inst, int rl, int symbolSize, const uint32_t* d_words,
+                                   uint64_t wordCount, void* const* d_symbolStreams, uint32_t streamCount,
+                                   uint32_t symbolCountPerStream, uint64_t* outWordsConsumed) {
+    return guarded_for(inst, [&] {
+        CCA_REQUIRE(inst && outWordsConsumed, CC_ERR_INVALID_ARG, "null argument");
+        CCA_REQUIRE(streamCount == 0 || d_symbolStreams, CC_ERR_INVALID_ARG, "null stream array");
+        uint32_t err = 0;
+        *outWordsConsumed = cca::decode_device_noalloc(inst, rl != 0, symbolSize, d_words, wordCount, d_symbolStreams,
+                                                       streamCount, symbolCountPerStream, &err);
+        CCA_REQUIRE(err != cca::kDecCorrupt, CC_ERR_CORRUPT_STREAM, "malformed bitstream");
+        CCA_REQUIRE(err == 0, CC_ERR_OUT_OF_MEMORY, "no-allocation decode scratch exhausted");
+    });
+}
+
 cc_status cc_compress_bricks_float(cc_instance* inst, const cc_brick* bricks, uint32_t count, uint32_t numLevels,
                                    int symbolSize, uint32_t* d_out, uint64_t outCapacityWords,
                                    uint64_t* brickWordOffsets, uint32_t* brickStatus) {

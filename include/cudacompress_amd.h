@@ -129,6 +129,20 @@ CC_API cc_status cc_decode_device(cc_instance* inst, int rl, int symbolSize, con
                                   uint64_t wordCount, void* const* d_symbolStreams, uint32_t streamCount,
                                   uint32_t symbolCountPerStream, uint64_t* outWordsConsumed);
 
+/* cc_decode_device with no allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (serving loops and other code that must not call hipMalloc).
+ * Same arguments, results and errors as cc_decode_device, but every buffer the decode kernels
+ * need beyond the bitstream and the outputs (call, item and wave tables, sequence records, job
+ * state, decode arena) lives in one static device array of 32 MiB per device (held by each device
+ * the library runs on until the process ends); nothing is allocated, on the device or pinned on
+ * the host.  At most 128 streams per call; a call whose tables and arena exceed the array returns
+ * CC_ERR_INVALID_ARG before any launch.  Every call on a device shares that device's array: calls
+ * that overlap on one device race.  With kernel timing on (timing detail 3 or cc_profile_enable)
+ * the timer creates HIP events. */
+CC_API cc_status cc_decode_device_noalloc(cc_instance* inst, int rl, int symbolSize, const uint32_t* d_words,
+                                          uint64_t wordCount, void* const* d_symbolStreams, uint32_t streamCount,
+                                          uint32_t symbolCountPerStream, uint64_t* outWordsConsumed);
+
 /* ---- Timing: include/cudaCompress/Timing.h:15-27 -------------------------------------------- */
 /* setTimingDetail: 0 none, 1 whole coder calls ("Encode RLHuff", ...), 2 also their stages
  * ("Encode: Histograms", ...), 3 also every kernel launch ("Kernel: <name>"; kernel timing is
