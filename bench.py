@@ -62,10 +62,15 @@ def parse():
     ap.add_argument("--serving-slots", type=int, default=4)
     ap.add_argument("--serving-rates", default="1000,5000,10000,20000,50000", help="arrival rates, bricks/s")
     ap.add_argument("--bricks", type=int, default=2048)
-    ap.add_argument("--groups", type=int, default=0, help="brick groups of eager batches (0: library default)")
+    ap.add_argument("--groups", type=int, default=0,
+                    help="brick groups of a batch: eager batches and plans (a plan runs its groups staggered, "
+                         "each captured on its own); 0: library default (one group)")
     ap.add_argument("--inflight", type=int, default=3,
                     help="plan mode: plans (own buffers and streams) whose steps run concurrently; step i "
                          "launches plan i %% inflight after waiting for that plan's previous step")
+    ap.add_argument("--single-plan-groups", type=int, default=2,
+                    help="after the timed steps, also time one plan alone with its bricks in this many staggered "
+                         "groups (0: skip)")
     ap.add_argument("--mode", default="plan", choices=["plan", "eager"],
                     help="plan: compression and decompression replayed from hipGraphs (BricksPlan.launch, "
                          "waited for before the plan's next step); "
@@ -430,7 +435,8 @@ def run_bricks(args, torch, cc, lib):
             for k in ran)
         assert concurrent_equal, "concurrent plans disagree"
     kernel_source = "HIP events around every kernel inside the timed steps"
-    if not kernels:  # graph replays carry no per-kernel events: time the same kernels in eager calls
+    if not any(k[0] == "code_pack" for k in kernels):
+        # graph replays carry no per-kernel events: time the same kernels in eager calls
         def eager():
             o, _ = cc.compressBricksFloat(inst, srcs, shapes, steps_q, 2, out)
             cc.decompressBricksFloat(inst, dsts, shapes, steps_q, 2, out, o)
@@ -485,6 +491,39 @@ def run_bricks(args, torch, cc, lib):
         pin.close()
         del ref
 
+    single = None
+    if plans and (depth > 1 or args.groups <= 1) and args.single_plan_groups > 0:
+        # one plan alone (one copy of every buffer), its bricks in staggered groups
+        progress(f"bricks: one plan, {args.single_plan_groups} groups")
+        for p in plans[1:]:
+            p.close()
+        inst1 = cc.createInstance(0, 7, 128 * 128 * 128 // 8, 128, 14)
+        inst1.setBrickGroups(args.single_plan_groups)
+        d1 = [torch.empty_like(s) for s in srcs]
+        o1 = torch.empty_like(out)
+        p1 = cc.BricksPlan(inst1, srcs, d1, shapes, steps_q, 2, o1)
+        for _ in range(args.warmup):
+            p1.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            r1 = p1.run()
+        torch.cuda.synchronize()
+        ms1 = (time.perf_counter() - t0) * 1e3 / args.steps
+        n_w = int(state["offs"][-1])
+        same = bool(np.array_equal(r1[0], state["offs"]) and torch.equal(o1[:n_w], out[:n_w])
+                    and all(torch.equal(a, b) for a, b in zip(d1, dsts)))
+        assert same, "the single grouped plan disagrees with the batch"
+        single = {"GBps": round(2.0 * raw_bytes / (ms1 * 1e-3) / 1e9, 3), "ms_per_step": round(ms1, 4),
+                  "plans_in_flight": 1, "brick_groups": args.single_plan_groups,
+                  "device_buffers": "one copy of the outputs, bitstreams and work buffers (+ group bitstream staging)",
+                  "words_and_reconstructions_equal_batch": same}
+        progress(f"bricks: one plan {ms1:.3f} ms per step")
+        p1.close()
+        inst1.close()
+        del d1, o1
+        plans = plans[:1]
+
     cpu = None
     if not args.no_cpu_baseline:
         progress("bricks: CPU baseline legs")
@@ -504,11 +543,14 @@ def run_bricks(args, torch, cc, lib):
            "bricks": len(bricks), "raw_bytes_per_direction": raw_bytes, "compressed_bytes": compressed,
            "compression_ratio": round(raw_bytes / compressed, 3), "psnr_db": round(float(psnr), 3),
            "worst_brick_psnr_db": round(float(worst), 3),
-           "mode": "batched, " + (("hipGraph plan replays, %d plans in flight (each step one full compression + "
+           "mode": "batched, " + (("hipGraph plan replays, %d plan(s) in flight (each step one full compression + "
                                    "decompression of the batch)" % depth) if args.mode == "plan"
                                   else "eager batch calls"),
+           "brick_groups": max(1, args.groups),
+           "device_buffers": "%d copies of the outputs, bitstreams and work buffers" % depth,
            "concurrent_plans_equal": concurrent_equal,
            "kernel_timing": kernel_source,
+           "single_plan": single,
            "plain": plain, "batched_vs_plain": round(value / plain["GBps"], 2) if plain else None,
            "bitstreams_equal_plain_sample": equal, "coding": coding}
     return value, ms, raw_bytes, kernels, cfg, cpu, "f32"

@@ -166,6 +166,15 @@ struct LayoutLink {
     hipEvent_t wait;
     hipEvent_t signal;
 };
+// One separately laid out job of a concatenation: its bitstream words and its state (word count
+// out_base + out_words).
+struct ConcatGroup {
+    const uint32_t* words;
+    const CoderState* st;
+};
+// Copies the jobs' bitstreams back to back into dst (capacity dst_cap words), in group order.
+void coder_concat(const ConcatGroup* d_groups, uint32_t ngroups, uint32_t* dst, uint64_t dst_cap, uint64_t bytes,
+                  hipStream_t st);
 // d_classes: kDesignListBase + kDesignLists * nseq + kPackLists * nitems words of work lists.
 void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uint32_t nseq, const CoderParams& p,
                   int symSize, uint32_t* scratch, uint64_t scratch_cap, CoderState* d_state, uint32_t* d_classes,

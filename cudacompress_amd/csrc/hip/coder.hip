@@ -988,7 +988,28 @@ __global__ __launch_bounds__(kCoderThreads) void k_gather(const CodeItem* items,
     }
 }
 
+// Concatenation of separately laid out jobs (the groups of a plan): group blockIdx.y's words
+// [0, n_g) land at the sum of the earlier groups' word counts; every group's count is its job
+// state's out_base + out_words.  Words past dst_cap are not written.
+__global__ __launch_bounds__(256) void k_concat_groups(const ConcatGroup* groups, uint32_t* dst, uint64_t dst_cap) {
+    const uint32_t g = blockIdx.y;
+    uint64_t prefix = 0;
+    for (uint32_t h = 0; h < g; h++) prefix += groups[h].st->out_base + groups[h].st->out_words;
+    const uint64_t n = groups[g].st->out_base + groups[g].st->out_words;
+    const uint64_t end = prefix + n < dst_cap ? n : (dst_cap > prefix ? dst_cap - prefix : 0);
+    const uint32_t* src = groups[g].words;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < end; i += (uint64_t)gridDim.x * 256)
+        dst[prefix + i] = src[i];
+}
+
 }  // namespace
+
+void coder_concat(const ConcatGroup* d_groups, uint32_t ngroups, uint32_t* dst, uint64_t dst_cap, uint64_t bytes,
+                  hipStream_t st) {
+    if (ngroups == 0) return;
+    CCA_LAUNCH("code_concat", bytes, k_concat_groups, dim3(1024, ngroups), dim3(256), 0, st, d_groups, dst, dst_cap);
+    CCA_LAUNCH_CHECK();
+}
 
 void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uint32_t nseq, const CoderParams& p,
                   int symSize, uint32_t* scratch, uint64_t scratch_cap, CoderState* d_state, uint32_t* d_classes,

@@ -211,11 +211,15 @@ void bricks_index_build(cc_instance* I, DecodeIndex& X, const std::vector<cc_bri
     decode_index_build(I, X, brick_items(job, nullptr), calls, symSize, st);
 }
 
-void bricks_encode_launch(BrickCtx& ctx, const BrickJob& job, hipStream_t st, const LayoutLink* link) {
+void bricks_forward_launch(const BrickJob& job, hipStream_t st) {
     const uint32_t n = (uint32_t)job.bricks.size();
     for (uint32_t l = 0; l < job.L; l++)
         bricks_forward_level(job.d_levels + (size_t)l * n, job.d_tiles + (size_t)l * 4 * (n + 1), n,
                              &job.tiles_total[4 * l], job.symSize, job.level_elems[l], st);
+}
+
+void bricks_encode_launch(BrickCtx& ctx, const BrickJob& job, hipStream_t st, const LayoutLink* link) {
+    bricks_forward_launch(job, st);
     encode_launch(ctx.coder, job.ej, job.d_out, job.out_cap, st, link);
 }
 
@@ -423,13 +427,34 @@ void decompress_bricks(cc_instance* I, const cc_brick* bricks, uint32_t n, uint3
 }  // namespace cca
 
 // ------------------------------------------------------------------------------------- plans
+// A plan captures its batch once and replays it.  With one group (the default) the compression
+// and the decompression are two hipGraphs on the group's stream.  With several groups
+// (cc_set_instance_brick_groups), each group -- a contiguous range of bricks with its own context
+// and stream -- is captured on its own as three graphs (forward passes, coder, decode); a replay
+// starts group g's forward passes once group g-1's are done, so the groups run staggered: one
+// group's latency-bound table designs overlap the next group's bandwidth-bound passes and the
+// previous group's decode.  Each group lays its bitstream out from word 0 of its own staging
+// region and decodes it from there; once every group has been coded, one kernel concatenates the
+// regions into the caller's output, where the words are exactly those of the one-group plan.
 struct cc_bricks_plan {
     cc_instance* inst = nullptr;
     std::vector<std::unique_ptr<cca::BrickCtx>> pool;  // one context (and stream) per group
     cca::Groups gr;
-    hipStream_t st = nullptr;  // the group context's stream: capture origin and replay stream
+    // one group: the group context's stream (capture origin and replay stream); several groups:
+    // the plan's own stream (ordering, concatenation, results)
+    hipStream_t st = nullptr;
+    bool own_stream = false;
     hipGraph_t g_enc = nullptr, g_dec = nullptr;
     hipGraphExec_t x_enc = nullptr, x_dec = nullptr;
+    // several groups: per group its forward, coder and decode graphs and the events between them
+    std::vector<hipGraph_t> graphs;
+    std::vector<hipGraphExec_t> gx_fwd, gx_code, gx_dec;
+    std::vector<hipEvent_t> ev_fwd, ev_code, ev_dec;
+    cca::DevBuf d_stage;   // the groups' bitstreams, group g from word stage_off[g]
+    cca::DevBuf d_concat;  // ConcatGroup records of the concatenation kernel
+    std::vector<uint64_t> stage_off, stage_cap;
+    uint32_t* d_out = nullptr;
+    uint64_t out_cap = 0;
     hipEvent_t order = nullptr;  // orders replays after the instance stream's earlier work
     bool encoded = false;
     bool launched = false;     // a plan_launch not yet waited for
@@ -441,12 +466,25 @@ cc_bricks_plan::~cc_bricks_plan() {
     (void)hipGetDevice(&prev);
     if (inst) (void)hipSetDevice(inst->device);
     if (st) (void)hipStreamSynchronize(st);
+    for (auto* c : gr.ctx)
+        if (c && c->st) (void)hipStreamSynchronize(c->st);
     if (x_enc) (void)hipGraphExecDestroy(x_enc);
     if (x_dec) (void)hipGraphExecDestroy(x_dec);
     if (g_enc) (void)hipGraphDestroy(g_enc);
     if (g_dec) (void)hipGraphDestroy(g_dec);
+    for (auto* v : {&gx_fwd, &gx_code, &gx_dec})
+        for (hipGraphExec_t x : *v)
+            if (x) (void)hipGraphExecDestroy(x);
+    for (hipGraph_t g : graphs)
+        if (g) (void)hipGraphDestroy(g);
+    for (auto* v : {&ev_fwd, &ev_code, &ev_dec})
+        for (hipEvent_t e : *v)
+            if (e) (void)hipEventDestroy(e);
     if (order) (void)hipEventDestroy(order);
-    for (auto& c : pool) c->release();  // also destroys the context stream `st` points to
+    d_stage.release();
+    d_concat.release();
+    if (own_stream && st) (void)hipStreamDestroy(st);
+    for (auto& c : pool) c->release();  // also destroys the group streams
     if (prev >= 0) (void)hipSetDevice(prev);
 }
 
@@ -466,6 +504,101 @@ void capture(hipStream_t st, hipGraph_t* g, hipGraphExec_t* x, const std::functi
     CCA_HIP(hipStreamEndCapture(st, g));
     CCA_HIP(hipGraphInstantiate(x, *g, nullptr, nullptr, 0));
 }
+
+bool grouped(const cc_bricks_plan* P) { return P->gr.ctx.size() > 1; }
+
+// Several groups: an eager grouped compression (chained layouts into the caller's output) sizes
+// every group's arena and bitstream; then each group gets a staging region with headroom and is
+// captured as forward, coder and decode graphs.
+void plan_create_groups(cc_bricks_plan* P, cc_instance* I) {
+    Groups& gr = P->gr;
+    const size_t ng = gr.ctx.size();
+    CCA_HIP(hipStreamCreateWithFlags(&P->st, hipStreamNonBlocking));
+    P->own_stream = true;
+    CCA_HIP(hipEventRecord(P->order, I->stream));
+    CCA_HIP(hipStreamWaitEvent(P->st, P->order, 0));
+    for (size_t g = 0; g < ng; g++) {
+        CCA_HIP(hipStreamWaitEvent(gr.ctx[g]->st, P->order, 0));
+        bricks_prepare(I, *gr.ctx[g], gr.jobs[g], true, true, nullptr, gr.ctx[g]->st);
+    }
+    const uint32_t n = gr.first.back();
+    std::vector<uint64_t> offs(n + 1);
+    bool ok = false;
+    hipEvent_t fork = nullptr;
+    CCA_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    P->ev_fwd.push_back(fork);  // destroyed with the plan
+    gr.encode_run(P->st, fork, offs.data(), nullptr, &ok, 4);
+    P->stage_off.assign(ng, 0);
+    P->stage_cap.assign(ng, 0);
+    uint64_t total = 0;
+    for (size_t g = 0; g < ng; g++) {
+        const uint64_t w = offs[gr.first[g + 1]] - offs[gr.first[g]];
+        P->stage_off[g] = total;
+        P->stage_cap[g] = w + w / 4 + 4096;
+        total += align_up(P->stage_cap[g], 64);
+    }
+    uint32_t* stage = P->d_stage.as<uint32_t>(total);
+    std::vector<ConcatGroup> cg(ng);
+    for (size_t g = 0; g < ng; g++) {
+        gr.jobs[g].d_out = stage + P->stage_off[g];
+        gr.jobs[g].out_cap = P->stage_cap[g];
+        cg[g] = ConcatGroup{gr.jobs[g].d_out, (const CoderState*)gr.ctx[g]->coder.d_state.p};
+    }
+    CCA_HIP(hipMemcpy(P->d_concat.get(sizeof(ConcatGroup) * ng), cg.data(), sizeof(ConcatGroup) * ng,
+                      hipMemcpyHostToDevice));
+    P->ev_fwd.pop_back();
+    CCA_HIP(hipEventDestroy(fork));
+    P->gx_fwd.assign(ng, nullptr);
+    P->gx_code.assign(ng, nullptr);
+    P->gx_dec.assign(ng, nullptr);
+    P->ev_fwd.assign(ng, nullptr);
+    P->ev_code.assign(ng, nullptr);
+    P->ev_dec.assign(ng, nullptr);
+    for (size_t g = 0; g < ng; g++) {
+        for (auto* v : {&P->ev_fwd, &P->ev_code, &P->ev_dec})
+            CCA_HIP(hipEventCreateWithFlags(&(*v)[g], hipEventDisableTiming));
+        BrickCtx& ctx = *gr.ctx[g];
+        const BrickJob& job = gr.jobs[g];
+        hipGraph_t a = nullptr, b = nullptr, c = nullptr;
+        capture(ctx.st, &a, &P->gx_fwd[g], [&] { bricks_forward_launch(job, ctx.st); });
+        P->graphs.push_back(a);
+        capture(ctx.st, &b, &P->gx_code[g],
+                [&] { encode_launch(ctx.coder, job.ej, job.d_out, job.out_cap, ctx.st, nullptr); });
+        P->graphs.push_back(b);
+        capture(ctx.st, &c, &P->gx_dec[g], [&] { bricks_decode_launch(ctx, job, ctx.st, true); });
+        P->graphs.push_back(c);
+    }
+}
+
+// Several groups: replays the groups' graphs, staggered by their forward passes; the plan stream
+// waits for every group's coder, concatenates, and (with decode) waits for every group's decode.
+void launch_groups(cc_bricks_plan* P, bool enc, bool dec) {
+    Groups& gr = P->gr;
+    const size_t ng = gr.ctx.size();
+    CCA_HIP(hipEventRecord(P->order, P->inst->stream));
+    CCA_HIP(hipStreamWaitEvent(P->st, P->order, 0));
+    for (size_t g = 0; g < ng; g++) {
+        const hipStream_t s = gr.ctx[g]->st;
+        CCA_HIP(hipStreamWaitEvent(s, P->order, 0));
+        if (enc) {
+            if (g > 0) CCA_HIP(hipStreamWaitEvent(s, P->ev_fwd[g - 1], 0));
+            CCA_HIP(hipGraphLaunch(P->gx_fwd[g], s));
+            CCA_HIP(hipEventRecord(P->ev_fwd[g], s));
+            CCA_HIP(hipGraphLaunch(P->gx_code[g], s));
+            CCA_HIP(hipEventRecord(P->ev_code[g], s));
+        }
+        if (dec) {
+            CCA_HIP(hipGraphLaunch(P->gx_dec[g], s));
+            CCA_HIP(hipEventRecord(P->ev_dec[g], s));
+        }
+    }
+    if (enc) {
+        for (size_t g = 0; g < ng; g++) CCA_HIP(hipStreamWaitEvent(P->st, P->ev_code[g], 0));
+        coder_concat((const ConcatGroup*)P->d_concat.p, (uint32_t)ng, P->d_out, P->out_cap, 0, P->st);
+    }
+    if (dec)
+        for (size_t g = 0; g < ng; g++) CCA_HIP(hipStreamWaitEvent(P->st, P->ev_dec[g], 0));
+}
 }  // namespace
 
 cc_bricks_plan* plan_create(cc_instance* I, const cc_brick* bricks, uint32_t n, uint32_t L, int symSize,
@@ -475,11 +608,17 @@ cc_bricks_plan* plan_create(cc_instance* I, const cc_brick* bricks, uint32_t n, 
     CCA_REQUIRE(n > 0, CC_ERR_INVALID_ARG, "a plan needs at least one brick");
     std::unique_ptr<cc_bricks_plan> P(new cc_bricks_plan());
     P->inst = I;
+    P->d_out = d_out;
+    P->out_cap = cap;
     CCA_HIP(hipEventCreateWithFlags(&P->order, hipEventDisableTiming));
-    // A plan captures one group on its own stream (the capture origin), with the table-design
-    // classes forked to that context's side streams; brick groups are not captured.
     Groups& gr = P->gr;
-    gr.setup(I, bricks, n, L, symSize, d_out, cap, 1, P->pool);
+    gr.setup(I, bricks, n, L, symSize, d_out, cap, groups_for(I, bricks, n), P->pool);
+    if (grouped(P.get())) {
+        plan_create_groups(P.get(), I);
+        return P.release();
+    }
+    // One group: captured on the group context's stream (the capture origin), with the
+    // table-design classes forked to that context's side streams.
     BrickCtx& ctx = *gr.ctx[0];
     BrickJob& job = gr.jobs[0];
     P->st = ctx.st;
@@ -506,42 +645,68 @@ cc_bricks_plan* plan_create(cc_instance* I, const cc_brick* bricks, uint32_t n, 
 
 namespace {
 
-// Reads the results of the plan's last compression (synchronises its stream).
+// Reads the results of the plan's last compression (synchronises its streams).
 void plan_encode_results(cc_bricks_plan* P, uint64_t* offsets, uint32_t* status) {
     Groups& gr = P->gr;
-    std::vector<CallOut> res;
-    const CoderState s = encode_finish(gr.ctx[0]->coder, gr.jobs[0].ej, &res, P->st);
     P->encoded = true;
-    CCA_REQUIRE(s.err != kErrScratch, CC_ERR_OUT_OF_MEMORY, "plan arena too small for this data; create a new plan");
-    const uint64_t total = s.out_base + s.out_words;
-    CCA_REQUIRE(total <= gr.jobs[0].out_cap, CC_ERR_INVALID_ARG, "output buffer too small for the compressed bricks");
-    const bool ok = bricks_encode_results(gr.jobs[0], res, total, offsets, status);
+    const size_t ng = gr.ctx.size();
+    uint64_t prefix = 0;
+    bool ok = true;
+    for (size_t g = 0; g < ng; g++) {
+        std::vector<CallOut> res;
+        const hipStream_t s = grouped(P) ? gr.ctx[g]->st : P->st;
+        const CoderState cs = encode_finish(gr.ctx[g]->coder, gr.jobs[g].ej, &res, s);
+        CCA_REQUIRE(cs.err != kErrScratch, CC_ERR_OUT_OF_MEMORY, "plan arena too small for this data; create a new plan");
+        const uint64_t total = cs.out_base + cs.out_words;
+        CCA_REQUIRE(total <= gr.jobs[g].out_cap, CC_ERR_INVALID_ARG,
+                    grouped(P) ? "plan staging too small for this data; create a new plan"
+                               : "output buffer too small for the compressed bricks");
+        const uint32_t f = gr.first[g], nb = gr.first[g + 1] - f;
+        std::vector<uint64_t> o(nb + 1);
+        ok = bricks_encode_results(gr.jobs[g], res, total, o.data(), status ? status + f : nullptr) && ok;
+        for (uint32_t k = 0; k <= nb; k++) offsets[f + k] = prefix + o[k];
+        prefix += total;
+    }
+    CCA_REQUIRE(prefix <= P->out_cap, CC_ERR_INVALID_ARG, "output buffer too small for the compressed bricks");
     CCA_REQUIRE(ok, CC_ERR_SYMBOL_OVERFLOW, "some bricks failed to encode (see the per-brick status)");
 }
 
 void plan_decode_results(cc_bricks_plan* P, uint32_t* status) {
-    std::vector<DecCallOut> res;
-    decode_finish(P->gr.ctx[0]->coder, P->gr.jobs[0].dj, &res, P->st);
-    CCA_REQUIRE(bricks_decode_results(P->gr.jobs[0], res, status), CC_ERR_CORRUPT_STREAM,
-                "some bricks failed to decode (see the per-brick status)");
+    Groups& gr = P->gr;
+    bool ok = true;
+    for (size_t g = 0; g < gr.ctx.size(); g++) {
+        std::vector<DecCallOut> res;
+        decode_finish(gr.ctx[g]->coder, gr.jobs[g].dj, &res, grouped(P) ? gr.ctx[g]->st : P->st);
+        ok = bricks_decode_results(gr.jobs[g], res, status ? status + gr.first[g] : nullptr) && ok;
+    }
+    CCA_REQUIRE(ok, CC_ERR_CORRUPT_STREAM, "some bricks failed to decode (see the per-brick status)");
 }
 
 }  // namespace
 
 void plan_compress(cc_bricks_plan* P, uint64_t* offsets, uint32_t* status) {
     DeviceGuard guard(P->inst->device);
-    CCA_HIP(hipEventRecord(P->order, P->inst->stream));
-    CCA_HIP(hipStreamWaitEvent(P->st, P->order, 0));
-    CCA_HIP(hipGraphLaunch(P->x_enc, P->st));
+    if (grouped(P)) {
+        launch_groups(P, true, false);
+        CCA_HIP(hipStreamSynchronize(P->st));
+    } else {
+        CCA_HIP(hipEventRecord(P->order, P->inst->stream));
+        CCA_HIP(hipStreamWaitEvent(P->st, P->order, 0));
+        CCA_HIP(hipGraphLaunch(P->x_enc, P->st));
+    }
     plan_encode_results(P, offsets, status);
 }
 
 void plan_launch(cc_bricks_plan* P) {
     DeviceGuard guard(P->inst->device);
-    CCA_HIP(hipEventRecord(P->order, P->inst->stream));
-    CCA_HIP(hipStreamWaitEvent(P->st, P->order, 0));
-    CCA_HIP(hipGraphLaunch(P->x_enc, P->st));
-    CCA_HIP(hipGraphLaunch(P->x_dec, P->st));  // stream positions from the device-side results
+    if (grouped(P)) {
+        launch_groups(P, true, true);
+    } else {
+        CCA_HIP(hipEventRecord(P->order, P->inst->stream));
+        CCA_HIP(hipStreamWaitEvent(P->st, P->order, 0));
+        CCA_HIP(hipGraphLaunch(P->x_enc, P->st));
+        CCA_HIP(hipGraphLaunch(P->x_dec, P->st));  // stream positions from the device-side results
+    }
     P->launched = true;
 }
 
@@ -562,9 +727,14 @@ void plan_run(cc_bricks_plan* P, uint64_t* offsets, uint32_t* cstatus, uint32_t*
 void plan_decompress(cc_bricks_plan* P, uint32_t* status) {
     DeviceGuard guard(P->inst->device);
     CCA_REQUIRE(P->encoded, CC_ERR_INVALID_ARG, "the plan has not compressed anything yet");
-    CCA_HIP(hipEventRecord(P->order, P->inst->stream));
-    CCA_HIP(hipStreamWaitEvent(P->st, P->order, 0));
-    CCA_HIP(hipGraphLaunch(P->x_dec, P->st));
+    if (grouped(P)) {
+        launch_groups(P, false, true);
+    } else {
+        CCA_HIP(hipEventRecord(P->order, P->inst->stream));
+        CCA_HIP(hipStreamWaitEvent(P->st, P->order, 0));
+        CCA_HIP(hipGraphLaunch(P->x_dec, P->st));
+    }
+    CCA_HIP(hipStreamSynchronize(P->st));
     plan_decode_results(P, status);
 }
 
@@ -572,24 +742,28 @@ void plan_coding_stats(cc_bricks_plan* P, uint64_t* out) {
     DeviceGuard guard(P->inst->device);
     CCA_REQUIRE(P->encoded, CC_ERR_INVALID_ARG, "the plan has not compressed anything yet");
     CCA_HIP(hipStreamSynchronize(P->st));
-    const BrickJob& job = P->gr.jobs[0];
-    const DeviceCoder& D = P->gr.ctx[0]->coder;
-    std::vector<CodeItem> items(job.ej.nitems);
-    std::vector<SeqInfo> seqs(job.ej.nseq);
-    CCA_HIP(hipMemcpy(items.data(), D.d_items.p, items.size() * sizeof(CodeItem), hipMemcpyDeviceToHost));
-    CCA_HIP(hipMemcpy(seqs.data(), D.d_seqs.p, seqs.size() * sizeof(SeqInfo), hipMemcpyDeviceToHost));
-    uint64_t n = 0, m = 0, nh = 0;
-    for (const CodeItem& it : items) {
-        n += it.n;
-        if (it.rl) m += seqs[it.seq].count;  // pairs: symbols and zero counts are one pair each
-        else nh += it.n;
+    uint64_t n = 0, m = 0, nh = 0, words = 0;
+    for (size_t g = 0; g < P->gr.ctx.size(); g++) {
+        const BrickJob& job = P->gr.jobs[g];
+        const DeviceCoder& D = P->gr.ctx[g]->coder;
+        if (grouped(P)) CCA_HIP(hipStreamSynchronize(P->gr.ctx[g]->st));
+        std::vector<CodeItem> items(job.ej.nitems);
+        std::vector<SeqInfo> seqs(job.ej.nseq);
+        CCA_HIP(hipMemcpy(items.data(), D.d_items.p, items.size() * sizeof(CodeItem), hipMemcpyDeviceToHost));
+        CCA_HIP(hipMemcpy(seqs.data(), D.d_seqs.p, seqs.size() * sizeof(SeqInfo), hipMemcpyDeviceToHost));
+        for (const CodeItem& it : items) {
+            n += it.n;
+            if (it.rl) m += seqs[it.seq].count;  // pairs: symbols and zero counts are one pair each
+            else nh += it.n;
+        }
+        CoderState s{};
+        CCA_HIP(hipMemcpy(&s, D.d_state.p, sizeof(CoderState), hipMemcpyDeviceToHost));
+        words += s.out_base + s.out_words;
     }
-    CoderState s{};
-    CCA_HIP(hipMemcpy(&s, D.d_state.p, sizeof(CoderState), hipMemcpyDeviceToHost));
     out[0] = n;
     out[1] = m;
     out[2] = nh;
-    out[3] = s.out_words;
+    out[3] = words;
 }
 
 void plan_destroy(cc_bricks_plan* P) { delete P; }

@@ -44,6 +44,8 @@ void bricks_prepare_bound(cc_instance* I, BrickCtx& ctx, BrickJob& job, const De
 void bricks_index_build(cc_instance* I, DecodeIndex& X, const std::vector<cc_brick>& bricks, uint32_t L,
                         int symSize, const std::vector<CallWords>& calls, hipStream_t st);
 bool bricks_decode_results(const BrickJob& job, const std::vector<DecCallOut>& res, uint32_t* status);
+// forward transform and quantization of every brick (the first part of bricks_encode_launch)
+void bricks_forward_launch(const BrickJob& job, hipStream_t st);
 void bricks_encode_launch(BrickCtx& ctx, const BrickJob& job, hipStream_t st, const LayoutLink* link = nullptr);
 void bricks_decode_launch(BrickCtx& ctx, const BrickJob& job, hipStream_t st, bool from_encode);
 

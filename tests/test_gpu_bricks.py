@@ -218,8 +218,9 @@ def test_bricks_plan_graph_replays(cc, torch_gpu, field):
 @pytest.mark.parametrize("groups", [2, 3, 7])
 def test_grouped_batches_equal_single_batch(cc, torch_gpu, field, groups):
     """A batch run as concurrent groups writes the single batch's words at the same offsets and
-    reconstructs the same values; a plan made on the same instance replays the same (plans capture
-    one group)."""
+    reconstructs the same values; a plan made on the same instance (each group captured on its own,
+    staggered, bitstreams concatenated on the device) replays the same, through separate
+    compress / decompress replays and through one launch of both."""
     log("bricks groups", groups)
     torch = torch_gpu
     rng = np.random.default_rng(40 + groups)
@@ -256,6 +257,17 @@ def test_grouped_batches_equal_single_batch(cc, torch_gpu, field, groups):
         assert np.array_equal(out1[:n].cpu().numpy(), out3[:n].cpu().numpy())
         for a, b in zip(d1, d3):
             assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+    for _ in range(2):
+        out3.fill_(-1)
+        for d in d3:
+            d.zero_()
+        offs4, cst, dst = plan.run()
+        assert np.all(cst == 0) and np.all(dst == 0) and np.array_equal(offs1, offs4)
+        assert np.array_equal(out1[:n].cpu().numpy(), out3[:n].cpu().numpy())
+        for a, b in zip(d1, d3):
+            assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+    st = plan.codingStats()
+    assert st["C"] == 4 * n
     plan.close()
     inst.close()
 
