@@ -68,6 +68,9 @@ def parse():
     ap.add_argument("--inflight", type=int, default=3,
                     help="plan mode: plans (own buffers and streams) whose steps run concurrently; step i "
                          "launches plan i %% inflight after waiting for that plan's previous step")
+    ap.add_argument("--tune", default="",
+                    help="JSON object of kernel-shape tuning keys applied over the library's table for the brick "
+                         "batches (e.g. '{\"laneDesignK\": 128}')")
     ap.add_argument("--single-plan-groups", type=int, default=2,
                     help="after the timed steps, also time one plan alone with its bricks in this many staggered "
                          "groups (0: skip)")
@@ -375,6 +378,8 @@ def run_bricks(args, torch, cc, lib):
     nvox = sum(b.size for b in bricks)
     out = torch.empty(nvox + 4096 * len(bricks), dtype=torch.int32, device=dev)
     inst = cc.createInstance(0, 7, 128 * 128 * 128 // 8, 128, 14)
+    tuning = dict(inst.getTuning(), **(json.loads(args.tune) if args.tune else {}))
+    inst.setTuning(tuning)
     if args.groups:
         inst.setBrickGroups(args.groups)
     state = {}
@@ -498,6 +503,7 @@ def run_bricks(args, torch, cc, lib):
         for p in plans[1:]:
             p.close()
         inst1 = cc.createInstance(0, 7, 128 * 128 * 128 // 8, 128, 14)
+        inst1.setTuning(tuning)
         inst1.setBrickGroups(args.single_plan_groups)
         d1 = [torch.empty_like(s) for s in srcs]
         o1 = torch.empty_like(out)
@@ -546,7 +552,7 @@ def run_bricks(args, torch, cc, lib):
            "mode": "batched, " + (("hipGraph plan replays, %d plan(s) in flight (each step one full compression + "
                                    "decompression of the batch)" % depth) if args.mode == "plan"
                                   else "eager batch calls"),
-           "brick_groups": max(1, args.groups),
+           "brick_groups": max(1, args.groups), "tuning": tuning,
            "device_buffers": "%d copies of the outputs, bitstreams and work buffers" % depth,
            "concurrent_plans_equal": concurrent_equal,
            "kernel_timing": kernel_source,

@@ -130,7 +130,7 @@ SIGNATURES = {
 
 class Tuning(ctypes.Structure):
     _fields_ = [("designGrid", c_u32 * 4), ("fusedXY", c_u32), ("xySlicesMax", c_u32), ("codingBlockSize", c_u32),
-                ("reserved", c_u32)]
+                ("laneDesignK", c_u32)]
 
 
 class Brick(ctypes.Structure):

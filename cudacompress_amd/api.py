@@ -64,7 +64,7 @@ def _elem_size(t):
 # ------------------------------------------------------------------------------------ Instance
 def tuning_to_dict(t):
     return {"designGrid": [int(g) for g in t.designGrid], "fusedXY": int(t.fusedXY), "xySlicesMax": int(t.xySlicesMax),
-            "codingBlockSize": int(t.codingBlockSize)}
+            "codingBlockSize": int(t.codingBlockSize), "laneDesignK": int(t.laneDesignK)}
 
 
 def tuning_from_dict(d):
@@ -72,6 +72,7 @@ def tuning_from_dict(d):
     for i, g in enumerate(d["designGrid"]):
         t.designGrid[i] = int(g)
     t.fusedXY, t.xySlicesMax, t.codingBlockSize = int(d["fusedXY"]), int(d["xySlicesMax"]), int(d["codingBlockSize"])
+    t.laneDesignK = int(d.get("laneDesignK", 0))
     return t
 
 

@@ -148,10 +148,11 @@ constexpr int kSideStreams = 4;
 // item lists at kPackCountBase + l; the class lists from kDesignListBase, then the item lists
 // (list l < kPackLists - 1: items whose largest table is of design class l; the last list: the
 // others).
-constexpr uint32_t kDesignLists = 5;
+constexpr uint32_t kDesignLists = 6;   // four LDS classes, the arena class, the per-thread class
+constexpr uint32_t kLaneClass = 5;     // tables designed one per thread (packed with class 0)
 constexpr uint32_t kPackLists = 3;
-constexpr uint32_t kPackCountBase = 5;
-constexpr uint32_t kDesignListBase = 8;
+constexpr uint32_t kPackCountBase = 6;
+constexpr uint32_t kDesignListBase = 12;
 static_assert(kPackCountBase + kPackLists <= kDesignListBase, "counters before the lists");
 struct SideStreams {
     hipStream_t s[kSideStreams];
@@ -181,7 +182,7 @@ void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uin
                   CallOut* d_calls,
                   uint32_t ncalls, const uint32_t* d_call_first_item, uint32_t* d_out, uint64_t out_cap,
                   uint64_t total_symbols, hipStream_t st, const SideStreams* side,
-                  const LayoutLink* link = nullptr, const uint32_t* design_grid = nullptr);
+                  const LayoutLink* link = nullptr, const uint32_t* design_grid = nullptr, uint32_t lane_k = 0);
 // wave_seq: for every decode wave the coded sequence it serves (upper bound of the block count:
 // RLE sequences have at most n pairs); wave_first: that wave's first block.
 void coder_decode(const DecCall* d_calls, uint32_t ncalls, const DecItem* d_items, uint32_t nitems, DecSeq* d_seqs,

@@ -167,9 +167,32 @@ __global__ void k_coder_init(CoderState* st, uint32_t* classes) {
     }
 }
 
+// Words of a histogram of nb bins in the arena and, when the table may be designed one per thread
+// (k < kLaneMaxK + 1 used symbols), its leaf list after it: (symbol, count) pairs in symbol order
+// from the next 4-word boundary.
+__device__ __forceinline__ uint64_t leaves_at(uint32_t nb) { return ((uint64_t)nb + 3) & ~uint64_t(3); }
+__device__ __forceinline__ uint64_t hist_words(uint32_t nb, uint32_t k, bool leaves) {
+    return leaves && k <= design::kLaneMaxK ? leaves_at(nb) + 2ull * k : (uint64_t)nb;
+}
+
+// The used bins of h[0, nb) (nb <= kStatBins) as (symbol, count) pairs in symbol order; every
+// thread of the workgroup calls.
+__device__ void write_leaves(const uint32_t* h, uint32_t nb, uint2* out, uint64_t* sh) {
+    constexpr uint32_t per = kStatBins / kCoderThreads;
+    const uint32_t lo = threadIdx.x * per;
+    uint32_t c = 0;
+    for (uint32_t j = 0; j < per; j++) c += (lo + j < nb && h[lo + j]) ? 1u : 0u;
+    uint64_t tot;
+    uint64_t r = wg_excl_sum_u64(c, sh, tot);
+    for (uint32_t j = 0; j < per; j++) {
+        const uint32_t b = lo + j;
+        if (b < nb && h[b]) out[r++] = make_uint2(b, h[b]);
+    }
+}
+
 // Pass A: pair counts, histograms (LDS), maxima; tables of the LDS class get their histogram
-// copied to the arena; streams with symbols >= kStatBins are marked wide and get a zeroed
-// global histogram for k_code_stats_wide.
+// copied to the arena (with the leaf list of small tables); streams with symbols >= kStatBins are
+// marked wide and get a zeroed global histogram for k_code_stats_wide.
 template <class S>
 __global__ __launch_bounds__(kCoderThreads) void k_code_stats(const CodeItem* items, SeqInfo* seqs, CoderParams p,
                                                               uint32_t* scratch, uint64_t cap, CoderState* st) {
@@ -178,6 +201,7 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_stats(const CodeItem* it
     __shared__ long long shl[4];
     __shared__ uint32_t shu[4];
     __shared__ uint64_t sh_off[2];
+    __shared__ uint64_t shq[4];
     const CodeItem it = items[blockIdx.x];
     const S* in = static_cast<const S*>(it.in);
     const bool rl = it.rl != 0;
@@ -268,7 +292,7 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_stats(const CodeItem* it
         ss->bits = 0;
         ss->out_off = (uint64_t)(carry + 1);  // last nonzero + 1, read by the pack pass
         if (p.dialect == CC_DIALECT_GPU && (uint64_t)ss->smax + 1 > lim) ss->err = kErrOverflow;
-        sh_off[0] = ss->err ? ~uint64_t(0) : arena_alloc(st, (uint64_t)ss->smax + 1, cap, ss);
+        sh_off[0] = ss->err ? ~uint64_t(0) : arena_alloc(st, hist_words(ss->smax + 1, ks, !wide), cap, ss);
         if (sz) {
             sz->count = M;
             sz->smax = M ? zmax : 0u;
@@ -281,7 +305,7 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_stats(const CodeItem* it
             sz->bits = 0;
             sz->out_off = 0;
             if (p.dialect == CC_DIALECT_GPU && (uint64_t)sz->smax + 1 > lim) sz->err = kErrOverflow;
-            sh_off[1] = sz->err ? ~uint64_t(0) : arena_alloc(st, (uint64_t)sz->smax + 1, cap, sz);
+            sh_off[1] = sz->err ? ~uint64_t(0) : arena_alloc(st, hist_words(sz->smax + 1, kz, true), cap, sz);
         }
     }
     __syncthreads();
@@ -291,11 +315,18 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_stats(const CodeItem* it
         for (uint32_t i = threadIdx.x; i < nb; i += kCoderThreads) h[i] = wide ? 0u : hs[i];
         if (threadIdx.x == 0) ss->hist_off = sh_off[0];
     }
+    // the uniform branches below (sh_off is shared) keep every thread at the scans' barriers
+    if (sh_off[0] != ~uint64_t(0) && !wide && ks <= design::kLaneMaxK) {
+        const uint32_t nb = (M ? smax : 0u) + 1;
+        write_leaves(hs, nb, reinterpret_cast<uint2*>(scratch + sh_off[0] + leaves_at(nb)), shq);
+    }
     if (rl && sh_off[1] != ~uint64_t(0)) {
         uint32_t* h = scratch + sh_off[1];
         const uint32_t nb = (M ? zmax : 0u) + 1;
         for (uint32_t i = threadIdx.x; i < nb; i += kCoderThreads) h[i] = hz[i];
         if (threadIdx.x == 0) sz->hist_off = sh_off[1];
+        if (kz <= design::kLaneMaxK)
+            write_leaves(hz, nb, reinterpret_cast<uint2*>(scratch + sh_off[1] + leaves_at(nb)), shq);
     }
 }
 
@@ -508,16 +539,22 @@ struct WaveEx {
 // Design class of a sequence by distinct-symbol count: four LDS classes and one whose workspace
 // lives in the arena.  Class lists: counters at classes[0, kDesignLists), list c at
 // classes + kDesignListBase + c * nseq.
-__device__ __forceinline__ uint32_t design_class(uint32_t K) {
+// Tables of at most lane_k symbols below symbol kStatBins (so that k_code_stats wrote their leaf
+// list) whose sequence is shorter than 2^23 go to the per-thread class; the others to the wave
+// classes by distinct-symbol count.
+__device__ __forceinline__ uint32_t design_class(const SeqInfo& s, uint32_t lane_k) {
+    const uint32_t K = s.nsym;
+    if (K <= lane_k && s.smax < kStatBins && s.count < design::kLaneMaxCount - 1) return kLaneClass;
     return K <= 256 ? 0u : K <= 1024 ? 1u : K <= 2048 ? 2u : K <= kDesignMid ? 3u : 4u;
 }
 
 // One thread per sequence: appends every sequence still to be designed to its class list (one
 // atomic per class per wave).
-__global__ __launch_bounds__(256) void k_design_classify(const SeqInfo* seqs, uint32_t nseq, uint32_t* classes) {
+__global__ __launch_bounds__(256) void k_design_classify(const SeqInfo* seqs, uint32_t nseq, uint32_t* classes,
+                                                         uint32_t lane_k) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     const bool on = i < nseq && !seqs[i].err;
-    const uint32_t c = on ? design_class(seqs[i].nsym) : 0xFFFFFFFFu;
+    const uint32_t c = on ? design_class(seqs[i], lane_k) : 0xFFFFFFFFu;
     const int lane = threadIdx.x & 63;
 #pragma unroll
     for (uint32_t k = 0; k < kDesignLists; k++) {
@@ -535,13 +572,18 @@ __global__ __launch_bounds__(256) void k_design_classify(const SeqInfo* seqs, ui
 // design class l, the last list the others.  An item's tables may belong to different classes,
 // so list l is packed once every class up to l is designed.
 __global__ __launch_bounds__(256) void k_item_classify(const CodeItem* items, uint32_t nitems, const SeqInfo* seqs,
-                                                       uint32_t nseq, uint32_t* classes) {
+                                                       uint32_t nseq, uint32_t* classes, uint32_t lane_k) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     uint32_t c = 0xFFFFFFFFu;
     if (i < nitems) {
         const CodeItem it = items[i];
-        uint32_t k = design_class(seqs[it.seq].nsym);
-        if (it.rl) k = max(k, design_class(seqs[it.seq + 1].nsym));
+        // the per-thread class is designed on class 0's stream, before it
+        auto pack_class = [&](const SeqInfo& s) {
+            const uint32_t c = design_class(s, lane_k);
+            return c == kLaneClass ? 0u : c;
+        };
+        uint32_t k = pack_class(seqs[it.seq]);
+        if (it.rl) k = max(k, pack_class(seqs[it.seq + 1]));
         c = min(k, kPackLists - 1);
     }
     const int lane = threadIdx.x & 63;
@@ -636,6 +678,102 @@ __global__ __launch_bounds__(64) void k_design(SeqInfo* seqs, CoderParams p, uin
     for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
         design_one<Idx, Gws>(&seqs[list[i]], p, scratch, cap, st, smem);
         __syncthreads();  // LDS state is reused by the next table
+    }
+}
+
+// Arena words of one allocation as arena_alloc rounds them.
+__device__ __forceinline__ uint64_t arena_words(uint64_t w) { return (w + 15) & ~uint64_t(15); }
+
+// Arena ranges for the threads of a workgroup with one atomic: every thread calls with its own
+// word count (0: none) and gets its offset; a range past the capacity is the caller's to reject.
+template <int T>
+__device__ uint64_t wg_arena_alloc(CoderState* st, uint64_t words, uint64_t* sh) {
+    sh[threadIdx.x] = words;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t acc = 0;
+        for (int k = 0; k < T; k++) {
+            const uint64_t w = sh[k];
+            sh[k] = acc;
+            acc += w;
+        }
+        const uint64_t base = acc ? atomicAdd(&st->scratch_used, (unsigned long long)acc) : 0ull;
+        for (int k = 0; k < T; k++) sh[k] += base;
+    }
+    __syncthreads();
+    const uint64_t off = sh[threadIdx.x];
+    __syncthreads();
+    return off;
+}
+
+// The per-thread class: one table per thread (design::design_table_lane), kLaneThreads tables per
+// workgroup, the heaps and parent links interleaved across the threads in LDS (8 LaneK bytes per
+// table: tables of at most LaneK symbols).
+constexpr int kLaneThreads = 32;
+template <int LaneK>
+__global__ __launch_bounds__(kLaneThreads) void k_design_lanes(SeqInfo* seqs, CoderParams p, uint32_t* scratch,
+                                                               uint64_t cap, CoderState* st, const uint32_t* classes,
+                                                               uint32_t nseq) {
+    static_assert(LaneK <= (int)design::kLaneMaxK, "packed node indices");
+    __shared__ uint32_t heap_sh[LaneK * kLaneThreads];
+    __shared__ uint16_t par_sh[2 * LaneK * kLaneThreads];
+    __shared__ uint64_t alloc_sh[kLaneThreads];
+    const uint32_t count = classes[kLaneClass];
+    const uint32_t* list = classes + kDesignListBase + (size_t)kLaneClass * nseq;
+    const int t = threadIdx.x;
+    for (uint32_t i0 = blockIdx.x * kLaneThreads; i0 < count; i0 += gridDim.x * kLaneThreads) {
+        const uint32_t i = i0 + t;
+        SeqInfo* s = i < count ? &seqs[list[i]] : nullptr;
+        const bool on = s != nullptr && s->err == 0;
+        const uint32_t K = on ? s->nsym : 0u, nb = on ? s->smax + 1 : 0u;
+        // code and len (2 nb words), then the symbol list (K words), as design_one allocates them
+        const uint64_t w0 = on ? arena_words(2ull * nb) + arena_words(K ? K : 1) : 0ull;
+        const uint64_t off0 = wg_arena_alloc<kLaneThreads>(st, w0, alloc_sh);
+        bool ok = on;
+        if (on && off0 + w0 > cap) {
+            ok = false;
+            s->err = kErrScratch;
+            atomicCAS(&st->err, 0u, (uint32_t)kErrScratch);
+        }
+        uint64_t bits = 0;
+        uint32_t info[3] = {0u, 0u, 0u}, cpl[32];
+        if (ok) {
+            const uint64_t code_off = off0, syms_off = off0 + arena_words(2ull * nb);
+            // the leaf list k_code_stats wrote after the histogram
+            const uint2* leaves = reinterpret_cast<const uint2*>(scratch + s->hist_off + leaves_at(nb));
+            TableOut out{scratch + code_off, scratch + code_off + nb, scratch + syms_off, cpl, info};
+            design::design_table_lane([&](int r) { return leaves[r]; }, (int)K, (int)p.dialect, s->quirk,
+                                      design::Strided<uint32_t>{heap_sh + t, kLaneThreads},
+                                      design::Strided<uint16_t>{par_sh + t, kLaneThreads}, out, &bits);
+            for (int l = 0; l < 32; l++) s->cpl[l] = (uint32_t)l < info[0] ? cpl[l] : 0u;
+            s->code_off = code_off;
+            s->syms_off = syms_off;
+            s->nlen = info[0];
+            s->bits = bits;
+            if (info[2]) {
+                ok = false;
+                s->err = info[2];
+                atomicCAS(&st->err, 0u, info[2]);
+            } else if (bits >= (1ull << 32)) {
+                ok = false;
+                s->err = kErrOutput;
+                atomicCAS(&st->err, 0u, (uint32_t)kErrOutput);
+            }
+        }
+        // codewords (one word of slack) and the coding-block offsets
+        const uint64_t nwords = (bits + 31) / 32 + 1;
+        const uint64_t nblk = ok ? (s->count + p.cbs - 1) / p.cbs : 0ull;
+        const uint64_t w1 = ok ? arena_words(nwords) + arena_words(nblk ? nblk : 1) : 0ull;
+        const uint64_t off1 = wg_arena_alloc<kLaneThreads>(st, w1, alloc_sh);
+        if (ok) {
+            if (off1 + w1 > cap) {
+                s->err = kErrScratch;
+                atomicCAS(&st->err, 0u, (uint32_t)kErrScratch);
+            } else {
+                s->words_off = off1;
+                s->boff_off = off1 + arena_words(nwords);
+            }
+        }
     }
 }
 
@@ -1016,7 +1154,7 @@ void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uin
                   CallOut* d_calls,
                   uint32_t ncalls, const uint32_t* d_call_first_item, uint32_t* d_out, uint64_t out_cap,
                   uint64_t total_symbols, hipStream_t st, const SideStreams* side, const LayoutLink* link,
-                  const uint32_t* design_grid) {
+                  const uint32_t* design_grid, uint32_t lane_k) {
     CCA_REQUIRE(symSize == 2 || symSize == 4, CC_ERR_INVALID_ARG, "symbol size must be 2 or 4");
     uint64_t* item_off = reinterpret_cast<uint64_t*>(scratch + scratch_cap);  // caller reserves the tail
     CCA_LAUNCH("coder_init", 32, k_coder_init, dim3(1), dim3(64), 0, st, d_state, d_classes);
@@ -1041,10 +1179,10 @@ void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uin
     // the sequences are sorted into per-class lists, and each class runs over its list only, the
     // classes concurrently on side streams, the largest tables' classes launched first.
     CCA_LAUNCH("design_classify", 0, k_design_classify, dim3(div_up(nseq, 256u)), dim3(256), 0, st, d_seqs, nseq,
-               d_classes);
+               d_classes, lane_k);
     CCA_LAUNCH_CHECK();
     CCA_LAUNCH("item_classify", 0, k_item_classify, dim3(div_up(nitems, 256u)), dim3(256), 0, st, d_items, nitems,
-               d_seqs, nseq, d_classes);
+               d_seqs, nseq, d_classes, lane_k);
     CCA_LAUNCH_CHECK();
     constexpr int kClasses = 4;
     static_assert(kClasses <= kSideStreams, "one side stream per design class");
@@ -1059,9 +1197,21 @@ void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uin
         for (int c = 0; c < kClasses; c++) CCA_HIP(hipStreamWaitEvent(side->s[c], side->fork, 0));
     }
     for (int c = kClasses - 1; c >= 0; c--) {
+        hipStream_t sc = side ? side->s[c] : st;
+        if (c == 0 && lane_k) {
+            const dim3 lg(std::min(div_up(nseq, (uint32_t)kLaneThreads), grid[0]));
+#define CCA_LANES(K)                                                                                              \
+    CCA_LAUNCH("huff_design_lanes", 0, k_design_lanes<K>, lg, dim3(kLaneThreads), 0, sc, d_seqs, p, scratch,    \
+               scratch_cap, d_state, d_classes, nseq)
+            if (lane_k <= 32) CCA_LANES(32);
+            else if (lane_k <= 64) CCA_LANES(64);
+            else if (lane_k <= 128) CCA_LANES(128);
+            else CCA_LANES(256);
+#undef CCA_LANES
+            CCA_LAUNCH_CHECK();
+        }
         const size_t lds = design::heap_region_bytes<uint16_t>(kmax[c]) + 2 * (2 * kmax[c] + 2) + 16 +
                            12 * design::kStackRanges + 16;
-        hipStream_t sc = side ? side->s[c] : st;
         CCA_LAUNCH(names[c], 0, (k_design<uint16_t, false>), dim3(std::min(nseq, grid[c])), dim3(64), lds, sc, d_seqs,
                    p, scratch, scratch_cap, d_state, d_classes, nseq, (uint32_t)c);
         CCA_LAUNCH_CHECK();

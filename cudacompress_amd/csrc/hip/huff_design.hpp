@@ -55,8 +55,10 @@ struct DepthLess {
     CCA_HD bool operator()(uint32_t a, uint32_t b) const { return (a >> 24) < (b >> 24); }
 };
 
-template <class T, class Less>
-CCA_HD void push_heap_at(T* h, int hole, int top, T v, Less less) {
+// The heap and sort routines take any array-like H (a pointer, or the strided per-thread view of
+// the one-table-per-thread design) with elements of type T.
+template <class H, class T, class Less>
+CCA_HD void push_heap_at(H h, int hole, int top, T v, Less less) {
     int parent = (hole - 1) / 2;
     while (hole > top && less(h[parent], v)) {
         h[hole] = h[parent];
@@ -66,8 +68,8 @@ CCA_HD void push_heap_at(T* h, int hole, int top, T v, Less less) {
     h[hole] = v;
 }
 
-template <class T, class Less>
-CCA_HD void adjust_heap(T* h, int hole, int len, T v, Less less) {
+template <class H, class T, class Less>
+CCA_HD void adjust_heap(H h, int hole, int len, T v, Less less) {
     const int top = hole;
     int child = hole;
     while (child < (len - 1) / 2) {
@@ -84,55 +86,55 @@ CCA_HD void adjust_heap(T* h, int hole, int len, T v, Less less) {
     push_heap_at(h, hole, top, v, less);
 }
 
-template <class T, class Less>
-CCA_HD void make_heap(T* h, int len, Less less) {
+template <class H, class Less>
+CCA_HD void make_heap(H h, int len, Less less) {
     if (len < 2) return;
     for (int parent = (len - 2) / 2;; parent--) {
-        T v = h[parent];
+        auto v = h[parent];
         adjust_heap(h, parent, len, v, less);
         if (parent == 0) return;
     }
 }
 
 // std::pop_heap over h[0, len): the top moves to h[len-1].
-template <class T, class Less>
-CCA_HD void pop_heap(T* h, int len, Less less) {
+template <class H, class Less>
+CCA_HD void pop_heap(H h, int len, Less less) {
     if (len > 1) {
-        T v = h[len - 1];
+        auto v = h[len - 1];
         h[len - 1] = h[0];
         adjust_heap(h, 0, len - 1, v, less);
     }
 }
 
 // std::push_heap over h[0, len) whose new element is h[len-1].
-template <class T, class Less>
-CCA_HD void push_heap(T* h, int len, Less less) {
-    T v = h[len - 1];
+template <class H, class Less>
+CCA_HD void push_heap(H h, int len, Less less) {
+    auto v = h[len - 1];
     push_heap_at(h, len - 1, 0, v, less);
 }
 
 // std::partial_sort(first, last, last): heap select (make_heap) then sort_heap.
-template <class T, class Less>
-CCA_HD void heap_sort(T* a, int len, Less less) {
+template <class H, class Less>
+CCA_HD void heap_sort(H a, int len, Less less) {
     make_heap(a, len, less);
     while (len > 1) {
         len--;
-        T v = a[len];
+        auto v = a[len];
         a[len] = a[0];
         adjust_heap(a, 0, len, v, less);
     }
 }
 
 // ---------------------------------------------------------------- sort (stl_algo.h semantics)
-template <class T>
-CCA_HD void swap_elems(T* a, int i, int j) {
-    T t = a[i];
+template <class H>
+CCA_HD void swap_elems(H a, int i, int j) {
+    auto t = a[i];
     a[i] = a[j];
     a[j] = t;
 }
 
-template <class T, class Less>
-CCA_HD void move_median_to_first(T* a, int result, int x, int y, int z, Less less) {
+template <class H, class Less>
+CCA_HD void move_median_to_first(H a, int result, int x, int y, int z, Less less) {
     if (less(a[x], a[y])) {
         if (less(a[y], a[z])) swap_elems(a, result, y);
         else if (less(a[x], a[z])) swap_elems(a, result, z);
@@ -146,8 +148,8 @@ CCA_HD void move_median_to_first(T* a, int result, int x, int y, int z, Less les
     }
 }
 
-template <class T, class Less>
-CCA_HD int unguarded_partition(T* a, int first, int last, int pivot, Less less) {
+template <class H, class Less>
+CCA_HD int unguarded_partition(H a, int first, int last, int pivot, Less less) {
     for (;;) {
         while (less(a[first], a[pivot])) first++;
         last--;
@@ -159,8 +161,8 @@ CCA_HD int unguarded_partition(T* a, int first, int last, int pivot, Less less) 
 }
 
 // __unguarded_partition_pivot: median of (first+1, mid, last-1) moved to first.
-template <class T, class Less>
-CCA_HD void place_pivot(T* a, int first, int last, Less less) {
+template <class H, class Less>
+CCA_HD void place_pivot(H a, int first, int last, Less less) {
     const int mid = first + (last - first) / 2;
     move_median_to_first(a, first, first + 1, mid, last - 1, less);
 }
@@ -173,8 +175,8 @@ CCA_HD int floor_log2(uint32_t n) {
 
 // The serial introsort loop (without the final insertion sort) over [first, last) with the given
 // remaining depth.  stack: room for 2*floor(log2 n)+2 ranges (3 ints each).
-template <class T, class Less>
-CCA_HD void introsort_loop(T* a, int first, int last, int depth, Less less, int* stack) {
+template <class H, class Less>
+CCA_HD void introsort_loop(H a, int first, int last, int depth, Less less, int* stack) {
     int sp = 0;
     for (;;) {
         while (last - first > kSortThreshold) {
@@ -423,6 +425,136 @@ CCA_HD void design_table(Ex& ex, const uint32_t* hist, uint32_t nbins, int diale
         out.info[2] = 0;
         *bits = b;
     }
+}
+
+// ---------------------------------------------------------------- one table per thread
+// Tables of at most kLaneMaxK symbols whose sequence has fewer than kLaneMaxCount symbols are
+// designed by one thread each, with the serial steps above (the same moves as the reference's
+// libstdc++ calls): the heap holds packed 32-bit entries count << kLaneNodeBits | node (a count
+// never exceeds the sequence length, a node index stays below 2 kLaneMaxK), the parent links are
+// 16-bit node indices, and both arrays are strided views, which the device lays out interleaved
+// across the threads of a workgroup (element j of thread t at j * threads + t) so that threads at
+// the same step of their tables touch distinct LDS banks.
+constexpr uint32_t kLaneMaxK = 256;
+constexpr int kLaneNodeBits = 9;
+constexpr uint32_t kLaneMaxCount = 1u << (32 - kLaneNodeBits);
+static_assert(2 * kLaneMaxK <= (1u << kLaneNodeBits), "node indices fit the packed entry");
+
+template <class T>
+struct Strided {
+    T* base;
+    int stride;
+    CCA_HD T& operator[](int j) const { return base[j * stride]; }
+    CCA_HD Strided operator+(int j) const { return Strided{base + j * stride, stride}; }
+};
+
+// QueueLess on packed entries: the counts decide, ties are equal.
+struct PackedQueueLess {
+    CCA_HD bool operator()(uint32_t a, uint32_t b) const { return (b >> kLaneNodeBits) < (a >> kLaneNodeBits); }
+};
+
+// Designs one table (same results as design_table) from its K leaves: leaf(r) is the (symbol,
+// count) of the r-th used symbol in symbol order (K <= kLaneMaxK, counts summing below
+// kLaneMaxCount); heap has room for kLaneMaxK entries, parent for 2 kLaneMaxK; out.cpl receives
+// 32 entries.  Leaves are read in batches of kLaneBatch so that their loads overlap.
+constexpr int kLaneBatch = 8;
+template <class LeafF>
+CCA_HD void design_table_lane(LeafF leaf, int K, int dialect, uint32_t quirk, Strided<uint32_t> heap,
+                              Strided<uint16_t> parent, TableOut out, uint64_t* bits) {
+    constexpr uint32_t nodemask = (1u << kLaneNodeBits) - 1;
+    for (int l = 0; l < 32; l++) out.cpl[l] = 0;
+    *bits = 0;
+    out.info[0] = 0;
+    out.info[1] = (uint32_t)K;
+    out.info[2] = 0;
+    if (K == 0) return;
+    // (1) leaves in symbol order
+    for (int r0 = 0; r0 < K; r0 += kLaneBatch) {
+        uint32_t c[kLaneBatch];
+        for (int j = 0; j < kLaneBatch; j++) c[j] = r0 + j < K ? leaf(r0 + j).y : 0u;
+        for (int j = 0; j < kLaneBatch; j++)
+            if (r0 + j < K) heap[r0 + j] = (c[j] << kLaneNodeBits) | (uint32_t)(r0 + j);
+    }
+    // (2) merge loop: pop, pop, push of the merged node
+    const PackedQueueLess qless;
+    if (dialect == kDialectGpu) {
+        make_heap(heap, K, qless);
+    } else {
+        for (int i = 1; i < K; i++) push_heap(heap, i + 1, qless);
+    }
+    int size = K, next = K;
+    while (size > 1) {
+        const uint32_t a = heap[0];
+        pop_heap(heap, size, qless);
+        size--;
+        const uint32_t b = heap[0];
+        pop_heap(heap, size, qless);
+        size--;
+        parent[(int)(a & nodemask)] = (uint16_t)next;
+        parent[(int)(b & nodemask)] = (uint16_t)next;
+        heap[size] = (((a >> kLaneNodeBits) + (b >> kLaneNodeBits)) << kLaneNodeBits) | (uint32_t)next;
+        size++;
+        push_heap(heap, size, qless);
+        next++;
+    }
+    // (3) depths: every node's parent has a larger index, so one pass from the root down turns
+    // the parent links into depths
+    const int root = next - 1;
+    parent[root] = 0;
+    for (int x = root - 1; x >= 0; x--) parent[x] = (uint16_t)(parent[(int)parent[x]] + 1);
+    // (4) leaves packed as depth << 24 | rank in symbol order, in the (now free) heap; the sort
+    // compares depths only, so carrying the rank instead of the symbol moves nothing differently
+    int deepest = 0;
+    for (int r = 0; r < K; r++) {
+        const int d = parent[r];
+        deepest = d > deepest ? d : deepest;
+        heap[r] = ((uint32_t)(d > kMaxDepth ? kMaxDepth : d) << 24) | (uint32_t)r;
+    }
+    if (deepest > kMaxDepth) {
+        out.info[1] = 0;
+        out.info[2] = 5;
+        return;
+    }
+    if (deepest == 0) {  // a single leaf: no codeword bits
+        const uint32_t s0 = leaf(0).x;
+        out.syms[0] = s0;
+        out.code[s0] = 0;
+        out.len[s0] = 0;
+        return;
+    }
+    // (5) std::sort by depth: the introsort loop, then its final insertion sort as a stable
+    // counting sort by depth; (6) canonical codes
+    int stack[3 * 20];  // pending ranges <= 2 floor(log2 K) + 2 <= 18 for K <= 256
+    introsort_loop(heap, 0, K, 2 * floor_log2((uint32_t)K), DepthLess(), stack);
+    uint32_t fi[32], fc[32], seen[32];
+    for (int i = 0; i < K; i++) out.cpl[(heap[i] >> 24) - 1]++;
+    first_codes(out.cpl, 31, fc, fi);
+    for (int l = 0; l < 32; l++) seen[l] = 0;
+    uint64_t b = 0;
+    for (int i0 = 0; i0 < K; i0 += kLaneBatch) {
+        uint32_t e[kLaneBatch], sym[kLaneBatch], cnt[kLaneBatch];
+        for (int j = 0; j < kLaneBatch; j++) e[j] = i0 + j < K ? heap[i0 + j] : 0u;
+        for (int j = 0; j < kLaneBatch; j++) {
+            if (i0 + j < K) {
+                const auto lv = leaf((int)(e[j] & 0xFFFFFFu));
+                sym[j] = lv.x;
+                cnt[j] = lv.y;
+            } else {
+                sym[j] = cnt[j] = 0u;
+            }
+        }
+        for (int j = 0; j < kLaneBatch; j++) {
+            if (i0 + j >= K) break;
+            const uint32_t d = e[j] >> 24, s = sym[j];
+            const uint32_t rank = seen[d - 1]++;
+            out.syms[fi[d - 1] + rank] = s;
+            out.code[s] = fc[d - 1] + rank;
+            out.len[s] = d;
+            b += (uint64_t)(cnt[j] - ((quirk && s == 65535u) ? 1u : 0u)) * d;
+        }
+    }
+    out.info[0] = (uint32_t)deepest;
+    *bits = b;
 }
 
 // ---------------------------------------------------------------- heap operations by path

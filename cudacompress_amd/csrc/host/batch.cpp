@@ -84,6 +84,7 @@ void encode_prepare(cc_instance* I, DeviceCoder& D, const std::vector<ItemSpec>&
     job.symSize = symSize;
     job.p = params_of(I);
     for (int c = 0; c < 4; c++) job.design_grid[c] = I->tuning.designGrid[c];
+    job.lane_k = I->tuning.laneDesignK;
     D.d_items.as<CodeItem>(std::max<uint32_t>(nitems, 1));
     D.d_call_first.as<uint32_t>(ncalls + 1);
     D.d_seqs.as<SeqInfo>(std::max<uint32_t>(job.nseq, 1));
@@ -114,7 +115,7 @@ void encode_launch(DeviceCoder& D, const EncodeJob& job, uint32_t* d_out, uint64
                  (uint32_t*)D.d_scratch.p, job.scratch_cap, (CoderState*)D.d_state.p, (uint32_t*)D.d_classes.p,
                  (CallOut*)D.d_calls.p,
                  job.ncalls, (const uint32_t*)D.d_call_first.p, d_out, out_cap, job.total_syms, st,
-                 side_streams(D, ss), link, job.design_grid);
+                 side_streams(D, ss), link, job.design_grid, job.lane_k);
 }
 
 CoderState encode_finish(DeviceCoder& D, const EncodeJob& job, std::vector<CallOut>* results, hipStream_t st) {

@@ -36,6 +36,7 @@ struct EncodeJob {
     CoderParams p{};
     uint64_t scratch_cap = 0;  // arena words
     uint32_t design_grid[4] = {4096, 2048, 1536, 768};  // resident workgroups per design class
+    uint32_t lane_k = 0;  // tables of at most this many symbols are designed one per thread (0: none)
 };
 void encode_prepare(cc_instance* I, DeviceCoder& D, const std::vector<ItemSpec>& items, uint32_t ncalls, int symSize,
                     EncodeJob& job, hipStream_t st);

@@ -25,6 +25,7 @@ cc_tuning default_tuning() {
     t.fusedXY = 1;
     t.xySlicesMax = 0;
     t.codingBlockSize = 128;
+    t.laneDesignK = 0;
     return t;
 }
 
@@ -53,6 +54,9 @@ void check(const cc_tuning& t) {
     const uint32_t c = t.codingBlockSize;
     CCA_REQUIRE(c >= 32 && c <= 256 && (c & (c - 1)) == 0, CC_ERR_INVALID_ARG,
                 "codingBlockSize must be a power of two in [32, 256]");
+    const uint32_t k = t.laneDesignK;
+    CCA_REQUIRE(k == 0 || k == 32 || k == 64 || k == 128 || k == 256, CC_ERR_INVALID_ARG,
+                "laneDesignK must be 0, 32, 64, 128 or 256");
 }
 
 }  // namespace
@@ -80,6 +84,8 @@ cc_tuning load_tuning_file(const std::string& path) {
             ok = static_cast<bool>(in >> t.xySlicesMax);
         } else if (key == "coding_block_size") {
             ok = static_cast<bool>(in >> t.codingBlockSize);
+        } else if (key == "lane_design_k") {
+            ok = static_cast<bool>(in >> t.laneDesignK);
         }  // other keys (measurements, provenance) are ignored
         if (!ok) {
             std::fclose(f);

@@ -462,12 +462,13 @@ typedef struct cc_tuning {
                                  0: separate x and y passes everywhere */
     uint32_t xySlicesMax;     /* most z-slices one fused x+y workgroup holds (0: as many as fit) */
     uint32_t codingBlockSize; /* recommended coding block size (32..256) */
-    uint32_t reserved;
+    uint32_t laneDesignK;     /* tables of at most this many distinct symbols (0, 32, 64, 128 or 256;
+                                 0: none) are designed one per thread, the others one per wave */
 } cc_tuning;
 /* The built-in defaults. */
 CC_API cc_status cc_default_tuning(cc_tuning* out);
 /* Parses a tuning table ("key value..." lines: design_grid g0 g1 g2 g3, fused_xy b, xy_slices_max k,
- * coding_block_size c; '#' comments).  path NULL: the library's table, $CUDACOMPRESS_AMD_TUNING or
+ * coding_block_size c, lane_design_k k; '#' comments).  path NULL: the library's table, $CUDACOMPRESS_AMD_TUNING or
  * else tuning/gfx950.txt beside the library's lib/ directory; keys it lacks keep their defaults, and
  * a missing file gives the defaults with CC_OK. */
 CC_API cc_status cc_load_tuning(const char* path, cc_tuning* out);

@@ -3,6 +3,7 @@
 // on tie-heavy random histograms and adversarial sort inputs.  Built and run by
 // tests/test_design_host.py; prints "OK <cases>" on success.
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <queue>
@@ -91,7 +92,22 @@ bool check_one(const std::vector<uint32_t>& hist, int dialect, uint32_t quirk, c
     std::vector<uint32_t> code(nb, 0xDEADBEEFu), len(nb, 0xDEADBEEFu), syms(K + 1), cpl(32), info(3);
     TableOut out{code.data(), len.data(), syms.data(), cpl.data(), info.data()};
     uint64_t bits = 0;
-    if (form == 2) {
+    // the one-table-per-thread form serves tables of <= kLaneMaxK symbols with small counts
+    uint64_t total = 0;
+    for (uint32_t h : hist) total += h;
+    if (form == 3 && (K > kLaneMaxK || total >= kLaneMaxCount)) form = 0;
+    std::vector<uint32_t> lheap(3 * kLaneMaxK + 3, 0xA5A5A5A5u);
+    std::vector<uint16_t> lpar(3 * 2 * kLaneMaxK + 3, 0xA5A5u);
+    if (form == 3) {
+        // the leaves (symbol, count) in symbol order; stride 3 (the device interleaves the threads
+        // of a workgroup), offset 1
+        struct U2 { uint32_t x, y; };
+        std::vector<U2> leaves;
+        for (uint32_t s = 0; s < nb; s++)
+            if (hist[s]) leaves.push_back(U2{s, hist[s]});
+        design_table_lane([&](int r) { return leaves[r]; }, (int)K, dialect, quirk,
+                          Strided<uint32_t>{lheap.data() + 1, 3}, Strided<uint16_t>{lpar.data() + 1, 3}, out, &bits);
+    } else if (form == 2) {
         WalkEx ex;
         design_table(ex, hist.data(), nb, dialect, quirk, w, &ctl, out, &bits);
     } else if (form == 1) {
@@ -159,7 +175,7 @@ int main(int argc, char** argv) {
                                                 {0, 7, 0, 7, 0, 7}, {1, 2, 4, 8, 16, 32, 64, 128}};
     for (auto& h : fixed)
         for (int d = 0; d < 2; d++) {
-            for (int pf : {0, 1, 2}) {
+            for (int pf : {0, 1, 2, 3}) {
                 ok &= check_one<uint16_t>(h, d, 0, "fixed16", pf);
                 ok &= check_one<uint32_t>(h, d, 0, "fixed32", pf);
                 cases += 2;
@@ -170,8 +186,8 @@ int main(int argc, char** argv) {
         std::vector<uint32_t> h{1, 1};
         while (h.size() < 40) h.push_back(h[h.size() - 1] + h[h.size() - 2]);
         ok &= check_one<uint32_t>(h, 0, 0, "fib", 0) && check_one<uint32_t>(h, 0, 0, "fib", 1) &&
-              check_one<uint32_t>(h, 0, 0, "fib", 2);
-        cases += 3;
+              check_one<uint32_t>(h, 0, 0, "fib", 2) && check_one<uint32_t>(h, 0, 0, "fib", 3);
+        cases += 4;
     }
     for (int t = 0; t < trials && ok; t++) {
         const int shape = t % 6;
@@ -199,6 +215,34 @@ int main(int argc, char** argv) {
         for (uint32_t x : h) K += x ? 1 : 0;
         if (K <= 32768) ok &= check_one<uint16_t>(h, dialect, quirk, "rand16", (t + 1) % 3);
         cases += 2;
+    }
+    // small tables (the one-table-per-thread form): up to kLaneMaxK symbols over wide bin ranges,
+    // tie-heavy and skewed counts, both dialects, the histogram pad quirk
+    for (int t = 0; t < 6 * trials && ok; t++) {
+        const uint32_t K = 1 + (uint32_t)(rng() % kLaneMaxK);
+        const uint32_t nb = K + (uint32_t)(rng() % (t % 3 == 0 ? 4000 : 300));
+        std::vector<uint32_t> h(nb, 0);
+        std::vector<uint32_t> pos(nb);
+        for (uint32_t i = 0; i < nb; i++) pos[i] = i;
+        std::shuffle(pos.begin(), pos.end(), rng);
+        for (uint32_t i = 0; i < K; i++) {
+            switch (t % 5) {
+                case 0: h[pos[i]] = 1 + (uint32_t)(rng() % 3); break;
+                case 1: h[pos[i]] = 1 + (uint32_t)(rng() % 2000); break;
+                case 2: h[pos[i]] = 1u << (rng() % 14); break;
+                case 3: h[pos[i]] = (uint32_t)(1 + std::exp((double)(rng() % 1000) / 100.0)); break;
+                default: h[pos[i]] = i < 8 ? 100000 + (uint32_t)(rng() % 9) : 1 + (uint32_t)(rng() % 4); break;
+            }
+        }
+        uint32_t quirk = 0;
+        if (t % 40 == 0) {
+            h.resize(65536, 0);
+            h[65535] += 1;
+            quirk = 1;
+        }
+        const int dialect = (int)(rng() % 2);
+        ok &= check_one<uint16_t>(h, dialect, quirk, "lane", 3);
+        cases++;
     }
     // sorts: tie-heavy, sorted, reversed, organ pipe, median-of-3 killers
     for (int t = 0; t < 200 && ok; t++) {

@@ -365,8 +365,8 @@ def test_server_index_reports_broken_bricks(cc, torch_gpu, field):
 
 def test_tuned_shapes_keep_words_and_values(cc, torch_gpu, field):
     """Kernel-shape tuning (design grids, separate or fused x+y passes, z-slices per fused
-    workgroup) changes no word and no reconstructed value; a coding block size from the table
-    changes the words but not the values."""
+    workgroup, tables designed one per thread up to 32..256 symbols) changes no word and no
+    reconstructed value; a coding block size from the table changes the words but not the values."""
     log("tuning: shapes keep the words")
     torch = torch_gpu
     rng = np.random.default_rng(29)
@@ -379,6 +379,8 @@ def test_tuned_shapes_keep_words_and_values(cc, torch_gpu, field):
     base = cc.defaultTuning()
     variants = [dict(base), dict(base, designGrid=[64, 64, 64, 64]), dict(base, fusedXY=0),
                 dict(base, xySlicesMax=1), dict(base, xySlicesMax=2, designGrid=[8192, 512, 256, 128])]
+    variants += [dict(base, laneDesignK=k) for k in (0, 32, 64, 128, 256) if k != base["laneDesignK"]]
+    variants += [dict(base, laneDesignK=256, designGrid=[3, 64, 64, 64])]
     ref = None
     for t in variants:
         inst = cc.createInstance(0, 7, 64 ** 3, 128, 14)

@@ -71,6 +71,23 @@ def test_device_coder_matches_oracle(cc, torch_gpu, inst14, name, gen, rl):
     check(cc, torch_gpu, inst14, gen(rng), rl, 14, 128)
 
 
+@pytest.fixture(scope="module", params=[32, 256], ids=["lanes32", "lanes256"])
+def inst_lanes(cc, torch_gpu, request):
+    inst = cc.createInstance(0, 8, 1 << 20, 128, 14)
+    inst.setTuning(dict(cc.defaultTuning(), laneDesignK=request.param))
+    yield inst
+    inst.close()
+
+
+@pytest.mark.parametrize("name,gen", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("rl", [True, False], ids=["rlhuff", "huff"])
+def test_device_coder_tables_per_thread_match_oracle(cc, torch_gpu, inst_lanes, name, gen, rl):
+    """Tables designed one per thread (laneDesignK): the same words as the oracle."""
+    log("device coder, per-thread tables, case", name, "rl", rl)
+    rng = np.random.default_rng(zlib.crc32(name.encode()) & 0xFFFF)
+    check(cc, torch_gpu, inst_lanes, gen(rng), rl, 14, 128)
+
+
 @pytest.mark.parametrize("cbs", [32, 256])
 def test_device_coder_block_sizes(cc, torch_gpu, cbs):
     log("device coder cbs", cbs)

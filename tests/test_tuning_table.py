@@ -9,15 +9,16 @@ def test_defaults_and_partial_table(tmp_path):
     d = cc.defaultTuning()
     assert d["codingBlockSize"] == 128 and d["fusedXY"] == 1 and len(d["designGrid"]) == 4
     p = tmp_path / "t.txt"
-    p.write_text("# comment\ndesign_grid 1 2 3 4   # trailing comment\ncoding_block_size 64\nmeasured 123.4\n")
+    p.write_text("# comment\ndesign_grid 1 2 3 4   # trailing comment\ncoding_block_size 64\nmeasured 123.4\n"
+                 "lane_design_k 128\n")
     t = cc.loadTuning(p)
-    assert t["designGrid"] == [1, 2, 3, 4] and t["codingBlockSize"] == 64
+    assert t["designGrid"] == [1, 2, 3, 4] and t["codingBlockSize"] == 64 and t["laneDesignK"] == 128
     assert t["fusedXY"] == d["fusedXY"] and t["xySlicesMax"] == d["xySlicesMax"]
     assert cc.loadTuning(tmp_path / "missing.txt") == d
 
 
 @pytest.mark.parametrize("text", ["design_grid 1 2\n", "fused_xy 2\n", "coding_block_size 100\n",
-                                  "design_grid 0 1 1 1\n", "xy_slices_max x\n"])
+                                  "design_grid 0 1 1 1\n", "xy_slices_max x\n", "lane_design_k 100\n"])
 def test_bad_tables_are_refused(tmp_path, text):
     p = tmp_path / "bad.txt"
     p.write_text(text)
