@@ -53,7 +53,7 @@ struct SeqInfo {
     uint64_t hist_off;   // scratch word offsets ...
     uint64_t code_off;   // code[smax+1] then len[smax+1]
     uint64_t syms_off;   // symbols in codeword-index order
-    uint64_t words_off;  // codewords
+    uint64_t last_nz1;   // position of the stream's last nonzero + 1 (0: none), set by the stats pass
     uint64_t boff_off;   // absolute bit offset of every coding block
     uint64_t out_off;    // output word offset of the table header
     uint32_t cpl[32];    // codewords per length
@@ -143,17 +143,12 @@ struct DecCallOut {
 // kSideStreams non-blocking streams, `fork` / `join` events owned by the caller; a null `side`
 // argument puts everything on st.
 constexpr int kSideStreams = 4;
-// Table-design work lists (device buffer of kDesignListBase + kDesignLists * nseq +
-// kPackLists * nitems words): counters of the design classes at [0, kDesignLists), of the pack
-// item lists at kPackCountBase + l; the class lists from kDesignListBase, then the item lists
-// (list l < kPackLists - 1: items whose largest table is of design class l; the last list: the
-// others).
+// Table-design work lists (device buffer of kDesignListBase + kDesignLists * nseq words): the
+// counters of the design classes at [0, kDesignLists), class list c from kDesignListBase + c * nseq.
 constexpr uint32_t kDesignLists = 6;   // four LDS classes, the arena class, the per-thread class
-constexpr uint32_t kLaneClass = 5;     // tables designed one per thread (packed with class 0)
-constexpr uint32_t kPackLists = 3;
-constexpr uint32_t kPackCountBase = 6;
-constexpr uint32_t kDesignListBase = 12;
-static_assert(kPackCountBase + kPackLists <= kDesignListBase, "counters before the lists");
+constexpr uint32_t kLaneClass = 5;     // tables designed one per thread (on class 0's stream)
+constexpr uint32_t kDesignListBase = 8;
+static_assert(kDesignLists <= kDesignListBase, "counters before the lists");
 struct SideStreams {
     hipStream_t s[kSideStreams];
     hipEvent_t fork;
@@ -176,7 +171,7 @@ struct ConcatGroup {
 // Copies the jobs' bitstreams back to back into dst (capacity dst_cap words), in group order.
 void coder_concat(const ConcatGroup* d_groups, uint32_t ngroups, uint32_t* dst, uint64_t dst_cap, uint64_t bytes,
                   hipStream_t st);
-// d_classes: kDesignListBase + kDesignLists * nseq + kPackLists * nitems words of work lists.
+// d_classes: kDesignListBase + kDesignLists * nseq words of work lists.
 void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uint32_t nseq, const CoderParams& p,
                   int symSize, uint32_t* scratch, uint64_t scratch_cap, CoderState* d_state, uint32_t* d_classes,
                   CallOut* d_calls,

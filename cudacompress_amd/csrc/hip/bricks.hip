@@ -285,6 +285,27 @@ __global__ __launch_bounds__(256) void k_bfwd(const BrickLevel* lv, const uint32
                  [&](int, int, int off, float v) { gst(dst, off, v); });
 }
 
+// A thread of a z tile serves one column (x, y) of the brick level (the tile's column c0 + lane):
+// its x and y octant bits and its offset inside a band are fixed, so only z varies per element.
+struct ZColumn {
+    int bxy;   // bx + 2 by
+    int lixy;  // (y - by hy) hx + (x - bx hx)
+    int hz, hxy;
+    __device__ ZColumn(const Geo<2>& g, uint32_t local, int hx, int hy, int hz_) : hz(hz_), hxy(hx * hy) {
+        const int ctiles = (g.ncols + NC - 1) / NC;
+        const int c = (int)(local % (uint32_t)ctiles) * NC + (int)(threadIdx.x & 63);
+        const int x = c % g.lx, y = c / g.lx;
+        const int bx = x >= hx, by = y >= hy;
+        bxy = bx + 2 * by;
+        lixy = (y - by * hy) * hx + (x - bx * hx);
+    }
+    // band_of(x, y, z) of this column
+    __device__ Band at(int z) const {
+        const int bz = z >= hz;
+        return Band{bxy + 4 * bz, (z - bz * hz) * hxy + lixy};
+    }
+};
+
 template <class S>
 __global__ __launch_bounds__(256) void k_bfwd_zq(const BrickLevel* lv, const uint32_t* tb, uint32_t nb) {
     __shared__ float lds[NC][LW];
@@ -295,9 +316,11 @@ __global__ __launch_bounds__(256) void k_bfwd_zq(const BrickLevel* lv, const uin
     const int band = hx * hy * hz;
     S* hp = static_cast<S*>(L.hp);
     S* lps = static_cast<S*>(L.lpsym);
-    fwd_tile<2>(g, blockIdx.x - tb[i], lds, [&](int, int, int off) { return gld<float>(L.b, off); },
-                [&](int c, int p, int, float v) {
-                    const Band bd = band_of(c % L.lx, c / L.lx, p, hx, hy, hz);
+    const uint32_t local = blockIdx.x - tb[i];
+    const ZColumn zc(g, local, hx, hy, hz);
+    fwd_tile<2>(g, local, lds, [&](int, int, int off) { return gld<float>(L.b, off); },
+                [&](int, int p, int, float v) {
+                    const Band bd = zc.at(p);
                     if (bd.b == 0) {
                         if (L.last) gst(lps, bd.li, (S)symbolize_i(quantize_f(v, L.inv_lp, 0)));
                         else gst(L.lp, (int)bd.li, v);
@@ -317,19 +340,21 @@ __global__ __launch_bounds__(256) void k_binv_zs(const BrickLevel* lv, const uin
     const int band = hx * hy * hz;
     const S* hp = static_cast<const S*>(L.hp);
     const S* lps = static_cast<const S*>(L.lpsym);
+    const uint32_t local = blockIdx.x - tb[i];
+    const ZColumn zc(g, local, hx, hy, hz);
     // raw entries: the symbol, or the lowpass float's bits (lowpass of a level above the coarsest)
     // half the window per batch keeps the register count (and occupancy) of the plain passes
-    inv_tile<2, 9>(g, blockIdx.x - tb[i], lds,
-                [&](int c, int p, int) -> uint32_t {
-                    const Band bd = band_of(c % L.lx, c / L.lx, p, hx, hy, hz);
+    inv_tile<2, 9>(g, local, lds,
+                [&](int, int p, int) -> uint32_t {
+                    const Band bd = zc.at(p);
                     if (bd.b == 0 && !L.last) return __float_as_uint(gld<float>(L.lp, bd.li));
                     const S* sp = bd.b == 0 ? lps : hp + (bd.b - 1) * band;
                     return (uint32_t)gld(sp, bd.li);
                 },
-                [&](int c, int p, uint32_t raw) -> float {
-                    const Band bd = band_of(c % L.lx, c / L.lx, p, hx, hy, hz);
-                    if (bd.b == 0 && !L.last) return __uint_as_float(raw);
-                    return unquantize_i(unsymbolize_u(raw), bd.b == 0 ? L.step_lp : L.step_hp, 0);
+                [&](int, int p, uint32_t raw) -> float {
+                    const int bz = p >= hz;
+                    if (zc.bxy + 4 * bz == 0 && !L.last) return __uint_as_float(raw);
+                    return unquantize_i(unsymbolize_u(raw), zc.bxy + 4 * bz == 0 ? L.step_lp : L.step_hp, 0);
                 },
                 [&](int, int, int off, float v) { gst(L.a, off, v); });
 }

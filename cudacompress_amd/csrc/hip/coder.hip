@@ -290,7 +290,8 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_stats(const CodeItem* it
         ss->wide = wide;
         ss->item = blockIdx.x;
         ss->bits = 0;
-        ss->out_off = (uint64_t)(carry + 1);  // last nonzero + 1, read by the pack pass
+        ss->last_nz1 = (uint64_t)(carry + 1);  // last nonzero + 1, read by the pack pass
+        ss->out_off = 0;
         if (p.dialect == CC_DIALECT_GPU && (uint64_t)ss->smax + 1 > lim) ss->err = kErrOverflow;
         sh_off[0] = ss->err ? ~uint64_t(0) : arena_alloc(st, hist_words(ss->smax + 1, ks, !wide), cap, ss);
         if (sz) {
@@ -304,6 +305,7 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_stats(const CodeItem* it
             sz->item = blockIdx.x;
             sz->bits = 0;
             sz->out_off = 0;
+            sz->last_nz1 = 0;
             if (p.dialect == CC_DIALECT_GPU && (uint64_t)sz->smax + 1 > lim) sz->err = kErrOverflow;
             sh_off[1] = sz->err ? ~uint64_t(0) : arena_alloc(st, hist_words(sz->smax + 1, kz, true), cap, sz);
         }
@@ -344,7 +346,7 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_stats_wide(const CodeIte
     const bool rl = it.rl != 0;
     const uint32_t n = it.n;
     uint32_t* h = scratch + ss->hist_off;
-    const long long last_nz = (long long)ss->out_off - 1;
+    const long long last_nz = (long long)ss->last_nz1 - 1;
     long long carry = -1;
     for (uint64_t base = 0; base < n; base += kChunk) {
         const uint64_t p0 = base + (uint64_t)threadIdx.x * kPerThread;
@@ -568,38 +570,6 @@ __global__ __launch_bounds__(256) void k_design_classify(const SeqInfo* seqs, ui
     }
 }
 
-// One thread per item: pack list l < kPackLists - 1 takes the items whose largest table is of
-// design class l, the last list the others.  An item's tables may belong to different classes,
-// so list l is packed once every class up to l is designed.
-__global__ __launch_bounds__(256) void k_item_classify(const CodeItem* items, uint32_t nitems, const SeqInfo* seqs,
-                                                       uint32_t nseq, uint32_t* classes, uint32_t lane_k) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    uint32_t c = 0xFFFFFFFFu;
-    if (i < nitems) {
-        const CodeItem it = items[i];
-        // the per-thread class is designed on class 0's stream, before it
-        auto pack_class = [&](const SeqInfo& s) {
-            const uint32_t c = design_class(s, lane_k);
-            return c == kLaneClass ? 0u : c;
-        };
-        uint32_t k = pack_class(seqs[it.seq]);
-        if (it.rl) k = max(k, pack_class(seqs[it.seq + 1]));
-        c = min(k, kPackLists - 1);
-    }
-    const int lane = threadIdx.x & 63;
-    uint32_t* lists = classes + kDesignListBase + (size_t)kDesignLists * nseq;
-#pragma unroll
-    for (uint32_t k = 0; k < kPackLists; k++) {
-        const uint64_t m = __ballot(c == k);
-        if (!m) continue;
-        const int leader = __ffsll((unsigned long long)m) - 1;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&classes[kPackCountBase + k], (uint32_t)__popcll(m));
-        base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
-        if (c == k) lists[(size_t)k * nitems + base + __popcll(m & lanes_below(lane))] = i;
-    }
-}
-
 // Designs the table of one sequence with one wave (all lanes call; returns uniformly).  Gws: the
 // design workspace comes from the arena instead of LDS (a compile-time choice, so that the LDS
 // classes address their workspace with LDS instructions rather than generic ones).
@@ -657,10 +627,9 @@ __device__ void design_one(SeqInfo* s, const CoderParams& p, uint32_t* scratch, 
             s->err = kErrOutput;
             atomicCAS(&st->err, 0u, (uint32_t)kErrOutput);
         } else {
-            const uint64_t nwords = (bits_sh + 31) / 32 + 1;
+            // the coding-block offsets (the codewords go straight to the output)
             const uint64_t nblk = (s->count + p.cbs - 1) / p.cbs;
-            s->words_off = arena_alloc(st, nwords, cap, s);
-            if (s->words_off != ~uint64_t(0)) s->boff_off = arena_alloc(st, nblk ? nblk : 1, cap, s);
+            s->boff_off = arena_alloc(st, nblk ? nblk : 1, cap, s);
         }
     }
 }
@@ -715,8 +684,10 @@ __global__ __launch_bounds__(kLaneThreads) void k_design_lanes(SeqInfo* seqs, Co
                                                                uint64_t cap, CoderState* st, const uint32_t* classes,
                                                                uint32_t nseq) {
     static_assert(LaneK <= (int)design::kLaneMaxK, "packed node indices");
+    constexpr int kStack = design::lane_stack_ints(LaneK);
     __shared__ uint32_t heap_sh[LaneK * kLaneThreads];
     __shared__ uint16_t par_sh[2 * LaneK * kLaneThreads];
+    __shared__ int stack_sh[kStack * kLaneThreads];
     __shared__ uint64_t alloc_sh[kLaneThreads];
     const uint32_t count = classes[kLaneClass];
     const uint32_t* list = classes + kDesignListBase + (size_t)kLaneClass * nseq;
@@ -736,16 +707,16 @@ __global__ __launch_bounds__(kLaneThreads) void k_design_lanes(SeqInfo* seqs, Co
             atomicCAS(&st->err, 0u, (uint32_t)kErrScratch);
         }
         uint64_t bits = 0;
-        uint32_t info[3] = {0u, 0u, 0u}, cpl[32];
+        uint32_t info[3] = {0u, 0u, 0u};
         if (ok) {
             const uint64_t code_off = off0, syms_off = off0 + arena_words(2ull * nb);
             // the leaf list k_code_stats wrote after the histogram
             const uint2* leaves = reinterpret_cast<const uint2*>(scratch + s->hist_off + leaves_at(nb));
-            TableOut out{scratch + code_off, scratch + code_off + nb, scratch + syms_off, cpl, info};
+            TableOut out{scratch + code_off, scratch + code_off + nb, scratch + syms_off, s->cpl, info};
             design::design_table_lane([&](int r) { return leaves[r]; }, (int)K, (int)p.dialect, s->quirk,
                                       design::Strided<uint32_t>{heap_sh + t, kLaneThreads},
-                                      design::Strided<uint16_t>{par_sh + t, kLaneThreads}, out, &bits);
-            for (int l = 0; l < 32; l++) s->cpl[l] = (uint32_t)l < info[0] ? cpl[l] : 0u;
+                                      design::Strided<uint16_t>{par_sh + t, kLaneThreads},
+                                      design::Strided<int>{stack_sh + t, kLaneThreads}, out, &bits);
             s->code_off = code_off;
             s->syms_off = syms_off;
             s->nlen = info[0];
@@ -760,206 +731,18 @@ __global__ __launch_bounds__(kLaneThreads) void k_design_lanes(SeqInfo* seqs, Co
                 atomicCAS(&st->err, 0u, (uint32_t)kErrOutput);
             }
         }
-        // codewords (one word of slack) and the coding-block offsets
-        const uint64_t nwords = (bits + 31) / 32 + 1;
+        // the coding-block offsets (the codewords go straight to the output)
         const uint64_t nblk = ok ? (s->count + p.cbs - 1) / p.cbs : 0ull;
-        const uint64_t w1 = ok ? arena_words(nwords) + arena_words(nblk ? nblk : 1) : 0ull;
+        const uint64_t w1 = ok ? arena_words(nblk ? nblk : 1) : 0ull;
         const uint64_t off1 = wg_arena_alloc<kLaneThreads>(st, w1, alloc_sh);
         if (ok) {
             if (off1 + w1 > cap) {
                 s->err = kErrScratch;
                 atomicCAS(&st->err, 0u, (uint32_t)kErrScratch);
             } else {
-                s->words_off = off1;
-                s->boff_off = off1 + arena_words(nwords);
+                s->boff_off = off1;
             }
         }
-    }
-}
-
-// Pass B: pairs recomputed, codewords packed into an LDS window and flushed as whole words;
-// the absolute bit offset of every coding block is recorded.
-template <class S>
-__global__ __launch_bounds__(kCoderThreads) void k_code_pack(const CodeItem* items, SeqInfo* seqs, CoderParams p,
-                                                             uint32_t* scratch, const uint32_t* list,
-                                                             const uint32_t* count) {
-    __shared__ uint32_t cs[kStatBins];
-    __shared__ unsigned char lsh[kStatBins];
-    __shared__ uint32_t cz[256];
-    __shared__ unsigned char lz[256];
-    __shared__ uint32_t win[2][kWindowWords];
-    __shared__ long long shl[4];
-    __shared__ uint64_t shq[4];
-    if (blockIdx.x >= *count) return;
-    const CodeItem it = items[list[blockIdx.x]];
-    const SeqInfo ss = seqs[it.seq];
-    const bool rl = it.rl != 0;
-    if (ss.err || (rl && seqs[it.seq + 1].err)) return;
-    const SeqInfo sz = rl ? seqs[it.seq + 1] : ss;
-    const uint32_t M = ss.count;
-    if (M == 0) return;
-    const S* in = static_cast<const S*>(it.in);
-    const uint32_t n = it.n;
-    const bool sglob = ss.smax >= kStatBins;
-    const uint32_t* gcode = scratch + ss.code_off;
-    const uint32_t* glen = gcode + ss.smax + 1;
-    // tables to LDS, each thread's loads issued four entries at a time
-    auto stage_table = [&](const uint32_t* code, const uint32_t* len, uint32_t nb, uint32_t* dc, unsigned char* dl) {
-        for (uint32_t i0 = threadIdx.x; i0 < nb; i0 += 4 * kCoderThreads) {
-            uint32_t c[4], l[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const uint32_t i = i0 + k * kCoderThreads;
-                c[k] = i < nb ? code[i] : 0u;
-                l[k] = i < nb ? len[i] : 0u;
-            }
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const uint32_t i = i0 + k * kCoderThreads;
-                if (i < nb) {
-                    dc[i] = c[k];
-                    dl[i] = (unsigned char)l[k];
-                }
-            }
-        }
-    };
-    if (!sglob) stage_table(gcode, glen, ss.smax + 1, cs, lsh);
-    if (rl) {
-        const uint32_t* zc = scratch + sz.code_off;
-        stage_table(zc, zc + sz.smax + 1, sz.smax + 1, cz, lz);
-    }
-    for (uint32_t i = threadIdx.x; i < 2 * kWindowWords; i += kCoderThreads) (&win[0][0])[i] = 0;
-    __syncthreads();
-    const bool single = ss.nlen == 0;  // one distinct symbol: no codeword bits
-    const bool zsingle = rl && sz.nlen == 0;
-    uint32_t* ws = scratch + ss.words_off;
-    uint32_t* wz = scratch + sz.words_off;
-    uint32_t* bs = scratch + ss.boff_off;
-    uint32_t* bz = scratch + sz.boff_off;
-    const uint64_t nws = (ss.bits + 31) / 32 + 1, nwz = (sz.bits + 31) / 32 + 1;  // allocated words
-    const long long limit = (long long)ss.out_off - 1;  // last nonzero
-    long long carry = -1;
-    uint64_t pairs = 0, cbits_s = 0, cbits_z = 0;
-    const uint32_t cbs = p.cbs;
-    uint32_t vn[kPerThread];
-    load8(in, (uint64_t)threadIdx.x * kPerThread, n, vn);
-    for (uint64_t base = 0; base < n && pairs < M; base += kChunk) {
-        const uint64_t p0 = base + (uint64_t)threadIdx.x * kPerThread;
-        uint32_t v[kPerThread];
-#pragma unroll
-        for (int j = 0; j < (int)kPerThread; j++) v[j] = vn[j];
-        if (base + kChunk < n) load8(in, p0 + kChunk, n, vn);  // next step's symbols in flight
-        long long chunk_max;
-        long long prev = wg_excl_max_i64(thread_last_nz(v, p0, n), shl, chunk_max);
-        prev = prev > carry ? prev : carry;
-        // this thread's pairs; code lengths looked up once for both the scan and the packing
-        uint32_t zc[kPerThread], ls[kPerThread], lzs[kPerThread];
-        const uint32_t emit = thread_flags(v, p0, n, rl, prev, rl ? limit : (long long)n, zc);
-        uint32_t nbs = 0, nbz = 0;
-#pragma unroll
-        for (int j = 0; j < (int)kPerThread; j++) {
-            const bool e = (emit >> j) & 1u;
-            uint32_t l = 0, m = 0;
-            if (e && !single) l = sglob ? glen[v[j]] : (uint32_t)lsh[v[j]];
-            if (e && rl && !zsingle) m = lz[zc[j]];
-            ls[j] = l;
-            lzs[j] = m;
-            nbs += l;
-            nbz += m;
-        }
-        const uint32_t np = __popc(emit);
-        uint64_t tot;
-        const uint64_t ex = wg_excl_sum_u64(((uint64_t)np << 40) | ((uint64_t)nbs << 20) | nbz, shq, tot);
-        const uint64_t os0 = cbits_s + ((ex >> 20) & 0xFFFFF);
-        const uint64_t oz0 = cbits_z + (ex & 0xFFFFF);
-        const uint64_t wbs = cbits_s >> 5, wbz = cbits_z >> 5;
-        if (np) {
-            // coding-block starts among this thread's pairs (pair indices < M only: stats and pack
-            // agree on M, the bound keeps the offset writes in their arrays regardless)
-            uint32_t pi = (uint32_t)(pairs + (ex >> 40));
-            uint32_t rem = pi % cbs, blk = pi / cbs;
-            uint32_t os = (uint32_t)os0, oz = (uint32_t)oz0;
-#pragma unroll
-            for (int j = 0; j < (int)kPerThread; j++) {
-                if ((emit >> j) & 1u) {
-                    if (rem == 0 && pi < M) {
-                        bs[blk] = os;
-                        if (rl) bz[blk] = oz;
-                    }
-                    os += ls[j];
-                    oz += lzs[j];
-                    pi++;
-                    if (++rem == cbs) {
-                        rem = 0;
-                        blk++;
-                    }
-                }
-            }
-            // codewords through a 64-bit accumulator, one LDS OR per window word touched
-            if (!single) {
-                uint32_t wi = (uint32_t)((os0 >> 5) - wbs), nacc = (uint32_t)(os0 & 31);
-                uint64_t acc = 0;
-#pragma unroll
-                for (int j = 0; j < (int)kPerThread; j++) {
-                    const uint32_t l = ls[j];
-                    uint32_t c = 0;
-                    if (l) c = sglob ? gcode[v[j]] : cs[v[j]];
-                    acc = (acc << l) | c;
-                    nacc += l;
-                    if (nacc >= 32) {
-                        nacc -= 32;
-                        if (wi < kWindowWords) atomicOr(&win[0][wi], (uint32_t)(acc >> nacc));
-                        wi++;
-                    }
-                }
-                if (nacc && wi < kWindowWords) atomicOr(&win[0][wi], (uint32_t)(acc << (32 - nacc)));
-            }
-            if (rl && !zsingle) {
-                uint32_t wi = (uint32_t)((oz0 >> 5) - wbz), nacc = (uint32_t)(oz0 & 31);
-                uint64_t acc = 0;
-#pragma unroll
-                for (int j = 0; j < (int)kPerThread; j++) {
-                    const uint32_t l = lzs[j];
-                    const uint32_t c = l ? cz[zc[j]] : 0u;
-                    acc = (acc << l) | c;
-                    nacc += l;
-                    if (nacc >= 32) {
-                        nacc -= 32;
-                        if (wi < kWindowWords) atomicOr(&win[1][wi], (uint32_t)(acc >> nacc));
-                        wi++;
-                    }
-                }
-                if (nacc && wi < kWindowWords) atomicOr(&win[1][wi], (uint32_t)(acc << (32 - nacc)));
-            }
-        }
-        __syncthreads();
-        pairs += tot >> 40;
-        const uint64_t nend_s = cbits_s + ((tot >> 20) & 0xFFFFF), nend_z = cbits_z + (tot & 0xFFFFF);
-        // flush whole words, keep the partial last word at window[0]
-        const uint32_t full_s = (uint32_t)((nend_s >> 5) - wbs), full_z = (uint32_t)((nend_z >> 5) - wbz);
-        for (uint32_t i = threadIdx.x; i < full_s; i += kCoderThreads)
-            if (wbs + i < nws) ws[wbs + i] = win[0][i];
-        if (rl)
-            for (uint32_t i = threadIdx.x; i < full_z; i += kCoderThreads)
-                if (wbz + i < nwz) wz[wbz + i] = win[1][i];
-        __syncthreads();
-        const uint32_t keep_s = win[0][full_s], keep_z = win[1][full_z];
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i <= full_s + 1 && i < kWindowWords; i += kCoderThreads) win[0][i] = 0;
-        for (uint32_t i = threadIdx.x; i <= full_z + 1 && i < kWindowWords; i += kCoderThreads) win[1][i] = 0;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            win[0][0] = keep_s;
-            win[1][0] = keep_z;
-        }
-        __syncthreads();
-        cbits_s = nend_s;
-        cbits_z = nend_z;
-        carry = chunk_max > carry ? chunk_max : carry;
-    }
-    if (threadIdx.x == 0) {
-        if ((cbits_s & 31) && (cbits_s >> 5) < nws) ws[cbits_s >> 5] = win[0][0];
-        if (rl && (cbits_z & 31) && (cbits_z >> 5) < nwz) wz[cbits_z >> 5] = win[1][0];
     }
 }
 
@@ -970,6 +753,261 @@ __device__ __forceinline__ uint64_t table_bits(const SeqInfo& s, const CoderPara
 __device__ __forceinline__ uint64_t seq_words(const SeqInfo& s, const CoderParams& p) {
     const uint64_t nblk = (s.count + p.cbs - 1) / p.cbs;
     return (table_bits(s, p) + 31) / 32 + 1 + (s.bits + 31) / 32 + (nblk + 1) / 2;
+}
+
+// 32 header bits starting at bit `pos` of a table header: [nlen:5][cpl x nlen][nsym][sb][symbols].
+__device__ uint32_t header_word(const SeqInfo& s, const uint32_t* cpl, const CoderParams& p, const uint32_t* syms,
+                                uint64_t pos) {
+    const uint32_t cb = p.count_bits, sbf = p.symbits_bits;
+    const uint32_t sb = req_bits(s.smax);
+    const uint64_t fixed = 5ull + (uint64_t)cb * (s.nlen + 1) + sbf;
+    uint32_t out = 0;
+    auto put = [&](uint64_t fpos, uint32_t width, uint32_t val) {  // field at fpos..fpos+width
+        if (width == 0 || fpos + width <= pos || fpos >= pos + 32) return;
+        if (width < 32) val &= (1u << width) - 1u;
+        // align the field's MSB to bit position (fpos - pos) counted from the word's MSB
+        const long long sh = (long long)(pos + 32) - (long long)(fpos + width);  // left shift amount
+        if (sh >= 0) out |= (sh >= 32) ? 0u : (val << sh);
+        else out |= (-sh >= 32) ? 0u : (val >> (-sh));
+    };
+    if (pos < fixed) {
+        put(0, 5, s.nlen);
+        for (uint32_t l = 0; l < s.nlen; l++) put(5 + (uint64_t)cb * l, cb, cpl[l]);
+        put(5 + (uint64_t)cb * s.nlen, cb, s.nsym);
+        put(5 + (uint64_t)cb * (s.nlen + 1), sbf, sb);
+    }
+    if (sb && s.nsym) {
+        const uint64_t lo = pos > fixed ? (pos - fixed) / sb : 0;
+        for (uint64_t k = lo; k < s.nsym; k++) {
+            const uint64_t fp = fixed + k * sb;
+            if (fp >= pos + 32) break;
+            put(fp, sb, syms[k]);
+        }
+    }
+    return out;
+}
+
+// Pass B: pairs recomputed, codewords packed into an LDS window and flushed as whole words
+// straight to their place in the output section (laid out by k_layout_* from the designed bit
+// counts): each workgroup also writes its item's zero-run count, table headers and bit counts, and
+// after the pairs its packInc16 coding-block offsets (src/cudaCompress/PackInc.cu:210-220).  The
+// absolute bit offset of every coding block is kept in the arena for the increments.
+template <class S>
+__global__ __launch_bounds__(kCoderThreads) void k_code_pack(const CodeItem* items, SeqInfo* seqs, CoderParams p,
+                                                             uint32_t* scratch, const CoderState* st, uint32_t* out,
+                                                             uint64_t out_cap) {
+    __shared__ uint32_t cs[kStatBins];
+    __shared__ unsigned char lsh[kStatBins];
+    __shared__ uint32_t cz[256];
+    __shared__ unsigned char lz[256];
+    __shared__ uint32_t win[2][kWindowWords];
+    __shared__ long long shl[4];
+    __shared__ uint64_t shq[4];
+    if (st->out_base + st->out_words > out_cap) return;  // the host reports the short output
+    const CodeItem it = items[blockIdx.x];
+    const SeqInfo ss = seqs[it.seq];
+    const bool rl = it.rl != 0;
+    if (ss.err || (rl && seqs[it.seq + 1].err)) return;
+    const SeqInfo sz = rl ? seqs[it.seq + 1] : ss;
+    const uint32_t M = ss.count;
+    // section heads: [M] (RLE items), then per sequence its table header and bit count
+    const uint32_t tws = (uint32_t)((table_bits(ss, p) + 31) / 32), twz = (uint32_t)((table_bits(sz, p) + 31) / 32);
+    {
+        const uint32_t* syms_s = scratch + ss.syms_off;
+        uint32_t* hs = out + ss.out_off;
+        for (uint32_t w = threadIdx.x; w < tws; w += kCoderThreads) hs[w] = header_word(ss, ss.cpl, p, syms_s, 32ull * w);
+        if (threadIdx.x == 0) {
+            hs[tws] = (uint32_t)ss.bits;
+            if (rl) out[ss.out_off - 1] = M;  // aligned u32 M
+        }
+        if (rl) {
+            const uint32_t* syms_z = scratch + sz.syms_off;
+            uint32_t* hz = out + sz.out_off;
+            for (uint32_t w = threadIdx.x; w < twz; w += kCoderThreads)
+                hz[w] = header_word(sz, sz.cpl, p, syms_z, 32ull * w);
+            if (threadIdx.x == 0) hz[twz] = (uint32_t)sz.bits;
+        }
+    }
+    const uint64_t nws = (ss.bits + 31) / 32, nwz = (sz.bits + 31) / 32;  // codeword words of each
+    uint32_t* ws = out + ss.out_off + tws + 1;
+    uint32_t* wz = out + sz.out_off + twz + 1;
+    uint32_t* bs = scratch + ss.boff_off;
+    uint32_t* bz = scratch + sz.boff_off;
+    const uint32_t cbs = p.cbs;
+    if (M > 0) {
+        const S* in = static_cast<const S*>(it.in);
+        const uint32_t n = it.n;
+        const bool sglob = ss.smax >= kStatBins;
+        const uint32_t* gcode = scratch + ss.code_off;
+        const uint32_t* glen = gcode + ss.smax + 1;
+        // tables to LDS, each thread's loads issued four entries at a time
+        auto stage_table = [&](const uint32_t* code, const uint32_t* len, uint32_t nb, uint32_t* dc, unsigned char* dl) {
+            for (uint32_t i0 = threadIdx.x; i0 < nb; i0 += 4 * kCoderThreads) {
+                uint32_t c[4], l[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint32_t i = i0 + k * kCoderThreads;
+                    c[k] = i < nb ? code[i] : 0u;
+                    l[k] = i < nb ? len[i] : 0u;
+                }
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint32_t i = i0 + k * kCoderThreads;
+                    if (i < nb) {
+                        dc[i] = c[k];
+                        dl[i] = (unsigned char)l[k];
+                    }
+                }
+            }
+        };
+        if (!sglob) stage_table(gcode, glen, ss.smax + 1, cs, lsh);
+        if (rl) {
+            const uint32_t* zc = scratch + sz.code_off;
+            stage_table(zc, zc + sz.smax + 1, sz.smax + 1, cz, lz);
+        }
+        for (uint32_t i = threadIdx.x; i < 2 * kWindowWords; i += kCoderThreads) (&win[0][0])[i] = 0;
+        __syncthreads();
+        const bool single = ss.nlen == 0;  // one distinct symbol: no codeword bits
+        const bool zsingle = rl && sz.nlen == 0;
+        const long long limit = (long long)ss.last_nz1 - 1;  // last nonzero
+        long long carry = -1;
+        uint64_t pairs = 0, cbits_s = 0, cbits_z = 0;
+        uint32_t vn[kPerThread];
+        load8(in, (uint64_t)threadIdx.x * kPerThread, n, vn);
+        for (uint64_t base = 0; base < n && pairs < M; base += kChunk) {
+            const uint64_t p0 = base + (uint64_t)threadIdx.x * kPerThread;
+            uint32_t v[kPerThread];
+#pragma unroll
+            for (int j = 0; j < (int)kPerThread; j++) v[j] = vn[j];
+            if (base + kChunk < n) load8(in, p0 + kChunk, n, vn);  // next step's symbols in flight
+            long long chunk_max;
+            long long prev = wg_excl_max_i64(thread_last_nz(v, p0, n), shl, chunk_max);
+            prev = prev > carry ? prev : carry;
+            // this thread's pairs; code lengths looked up once for both the scan and the packing
+            uint32_t zc[kPerThread], ls[kPerThread], lzs[kPerThread];
+            const uint32_t emit = thread_flags(v, p0, n, rl, prev, rl ? limit : (long long)n, zc);
+            uint32_t nbs = 0, nbz = 0;
+#pragma unroll
+            for (int j = 0; j < (int)kPerThread; j++) {
+                const bool e = (emit >> j) & 1u;
+                uint32_t l = 0, m = 0;
+                if (e && !single) l = sglob ? glen[v[j]] : (uint32_t)lsh[v[j]];
+                if (e && rl && !zsingle) m = lz[zc[j]];
+                ls[j] = l;
+                lzs[j] = m;
+                nbs += l;
+                nbz += m;
+            }
+            const uint32_t np = __popc(emit);
+            uint64_t tot;
+            const uint64_t ex = wg_excl_sum_u64(((uint64_t)np << 40) | ((uint64_t)nbs << 20) | nbz, shq, tot);
+            const uint64_t os0 = cbits_s + ((ex >> 20) & 0xFFFFF);
+            const uint64_t oz0 = cbits_z + (ex & 0xFFFFF);
+            const uint64_t wbs = cbits_s >> 5, wbz = cbits_z >> 5;
+            if (np) {
+                // coding-block starts among this thread's pairs (pair indices < M only: stats and
+                // pack agree on M, the bound keeps the offset writes in their arrays regardless)
+                uint32_t pi = (uint32_t)(pairs + (ex >> 40));
+                uint32_t rem = pi % cbs, blk = pi / cbs;
+                uint32_t os = (uint32_t)os0, oz = (uint32_t)oz0;
+#pragma unroll
+                for (int j = 0; j < (int)kPerThread; j++) {
+                    if ((emit >> j) & 1u) {
+                        if (rem == 0 && pi < M) {
+                            bs[blk] = os;
+                            if (rl) bz[blk] = oz;
+                        }
+                        os += ls[j];
+                        oz += lzs[j];
+                        pi++;
+                        if (++rem == cbs) {
+                            rem = 0;
+                            blk++;
+                        }
+                    }
+                }
+                // codewords through a 64-bit accumulator, one LDS OR per window word touched
+                if (!single) {
+                    uint32_t wi = (uint32_t)((os0 >> 5) - wbs), nacc = (uint32_t)(os0 & 31);
+                    uint64_t acc = 0;
+#pragma unroll
+                    for (int j = 0; j < (int)kPerThread; j++) {
+                        const uint32_t l = ls[j];
+                        uint32_t c = 0;
+                        if (l) c = sglob ? gcode[v[j]] : cs[v[j]];
+                        acc = (acc << l) | c;
+                        nacc += l;
+                        if (nacc >= 32) {
+                            nacc -= 32;
+                            if (wi < kWindowWords) atomicOr(&win[0][wi], (uint32_t)(acc >> nacc));
+                            wi++;
+                        }
+                    }
+                    if (nacc && wi < kWindowWords) atomicOr(&win[0][wi], (uint32_t)(acc << (32 - nacc)));
+                }
+                if (rl && !zsingle) {
+                    uint32_t wi = (uint32_t)((oz0 >> 5) - wbz), nacc = (uint32_t)(oz0 & 31);
+                    uint64_t acc = 0;
+#pragma unroll
+                    for (int j = 0; j < (int)kPerThread; j++) {
+                        const uint32_t l = lzs[j];
+                        const uint32_t c = l ? cz[zc[j]] : 0u;
+                        acc = (acc << l) | c;
+                        nacc += l;
+                        if (nacc >= 32) {
+                            nacc -= 32;
+                            if (wi < kWindowWords) atomicOr(&win[1][wi], (uint32_t)(acc >> nacc));
+                            wi++;
+                        }
+                    }
+                    if (nacc && wi < kWindowWords) atomicOr(&win[1][wi], (uint32_t)(acc << (32 - nacc)));
+                }
+            }
+            __syncthreads();
+            pairs += tot >> 40;
+            const uint64_t nend_s = cbits_s + ((tot >> 20) & 0xFFFFF), nend_z = cbits_z + (tot & 0xFFFFF);
+            // flush whole words, keep the partial last word at window[0]
+            const uint32_t full_s = (uint32_t)((nend_s >> 5) - wbs), full_z = (uint32_t)((nend_z >> 5) - wbz);
+            for (uint32_t i = threadIdx.x; i < full_s; i += kCoderThreads)
+                if (wbs + i < nws) ws[wbs + i] = win[0][i];
+            if (rl)
+                for (uint32_t i = threadIdx.x; i < full_z; i += kCoderThreads)
+                    if (wbz + i < nwz) wz[wbz + i] = win[1][i];
+            __syncthreads();
+            const uint32_t keep_s = win[0][full_s], keep_z = win[1][full_z];
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i <= full_s + 1 && i < kWindowWords; i += kCoderThreads) win[0][i] = 0;
+            for (uint32_t i = threadIdx.x; i <= full_z + 1 && i < kWindowWords; i += kCoderThreads) win[1][i] = 0;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                win[0][0] = keep_s;
+                win[1][0] = keep_z;
+            }
+            __syncthreads();
+            cbits_s = nend_s;
+            cbits_z = nend_z;
+            carry = chunk_max > carry ? chunk_max : carry;
+        }
+        if (threadIdx.x == 0) {
+            if ((cbits_s & 31) && (cbits_s >> 5) < nws) ws[cbits_s >> 5] = win[0][0];
+            if (rl && (cbits_z & 31) && (cbits_z >> 5) < nwz) wz[cbits_z >> 5] = win[1][0];
+        }
+    }
+    // coding-block offsets as packInc16 increments, from the absolute offsets the pairs recorded
+    __threadfence_block();
+    __syncthreads();
+    for (int q = 0; q < (rl ? 2 : 1); q++) {
+        const SeqInfo& s = q ? sz : ss;
+        const uint32_t* bo = q ? bz : bs;
+        uint32_t* o = out + s.out_off + (q ? twz : tws) + 1 + (q ? nwz : nws);
+        const uint32_t nblk = (s.count + cbs - 1) / cbs;
+        for (uint32_t w = threadIdx.x; w < (nblk + 1) / 2; w += kCoderThreads) {
+            const uint32_t b0 = 2 * w, b1 = 2 * w + 1;
+            const uint32_t i0 = b0 == 0 ? bo[0] : bo[b0] - bo[b0 - 1];
+            const uint32_t i1 = b1 < nblk ? bo[b1] - bo[b1 - 1] : 0u;
+            o[w] = (i0 & 0xFFFFu) | (i1 << 16);
+        }
+    }
 }
 
 // Section offsets in call order, in three launches: the words of every item with each
@@ -1062,70 +1100,6 @@ __global__ __launch_bounds__(256) void k_layout_calls(const CodeItem* items, uin
     calls[c].pad = 0;
 }
 
-// 32 header bits starting at bit `pos` of a table header: [nlen:5][cpl x nlen][nsym][sb][symbols].
-__device__ uint32_t header_word(const SeqInfo& s, const uint32_t* cpl, const CoderParams& p, const uint32_t* syms,
-                                uint64_t pos) {
-    const uint32_t cb = p.count_bits, sbf = p.symbits_bits;
-    const uint32_t sb = req_bits(s.smax);
-    const uint64_t fixed = 5ull + (uint64_t)cb * (s.nlen + 1) + sbf;
-    uint32_t out = 0;
-    auto put = [&](uint64_t fpos, uint32_t width, uint32_t val) {  // field at fpos..fpos+width
-        if (width == 0 || fpos + width <= pos || fpos >= pos + 32) return;
-        if (width < 32) val &= (1u << width) - 1u;
-        // align the field's MSB to bit position (fpos - pos) counted from the word's MSB
-        const long long sh = (long long)(pos + 32) - (long long)(fpos + width);  // left shift amount
-        if (sh >= 0) out |= (sh >= 32) ? 0u : (val << sh);
-        else out |= (-sh >= 32) ? 0u : (val >> (-sh));
-    };
-    if (pos < fixed) {
-        put(0, 5, s.nlen);
-        for (uint32_t l = 0; l < s.nlen; l++) put(5 + (uint64_t)cb * l, cb, cpl[l]);
-        put(5 + (uint64_t)cb * s.nlen, cb, s.nsym);
-        put(5 + (uint64_t)cb * (s.nlen + 1), sbf, sb);
-    }
-    if (sb && s.nsym) {
-        const uint64_t lo = pos > fixed ? (pos - fixed) / sb : 0;
-        for (uint64_t k = lo; k < s.nsym; k++) {
-            const uint64_t fp = fixed + k * sb;
-            if (fp >= pos + 32) break;
-            put(fp, sb, syms[k]);
-        }
-    }
-    return out;
-}
-
-__global__ __launch_bounds__(kCoderThreads) void k_gather(const CodeItem* items, const SeqInfo* seqs, CoderParams p,
-                                                          const uint32_t* scratch, const CoderState* st,
-                                                          uint32_t* out, uint64_t out_cap) {
-    const SeqInfo& s = seqs[blockIdx.x];
-    const CodeItem it = items[s.item];
-    if (st->out_base + st->out_words > out_cap) return;
-    if (s.err) return;
-    if (it.rl && (seqs[it.seq].err || seqs[it.seq + 1].err)) return;
-    const bool is_first = it.seq == blockIdx.x;
-    if (it.rl && is_first && threadIdx.x == 0) out[s.out_off - 1] = s.count;  // aligned u32 M
-    const uint64_t tb = table_bits(s, p);
-    const uint32_t tw = (uint32_t)((tb + 31) / 32);
-    const uint32_t* syms = scratch + s.syms_off;
-    uint32_t* o = out + s.out_off;
-    for (uint32_t w = threadIdx.x; w < tw; w += kCoderThreads) o[w] = header_word(s, s.cpl, p, syms, 32ull * w);
-    o += tw;
-    if (threadIdx.x == 0) o[0] = (uint32_t)s.bits;
-    o += 1;
-    const uint64_t nw = (s.bits + 31) / 32;
-    const uint32_t* src = scratch + s.words_off;
-    for (uint64_t w = threadIdx.x; w < nw; w += kCoderThreads) o[w] = src[w];
-    o += nw;
-    const uint32_t nblk = (s.count + p.cbs - 1) / p.cbs;
-    const uint32_t* bo = scratch + s.boff_off;
-    for (uint32_t w = threadIdx.x; w < (nblk + 1) / 2; w += kCoderThreads) {
-        const uint32_t b0 = 2 * w, b1 = 2 * w + 1;
-        const uint32_t i0 = b0 == 0 ? bo[0] : bo[b0] - bo[b0 - 1];  // packInc16, PackInc.cu:210-220
-        const uint32_t i1 = b1 < nblk ? bo[b1] - bo[b1 - 1] : 0u;
-        o[w] = (i0 & 0xFFFFu) | (i1 << 16);
-    }
-}
-
 // Concatenation of separately laid out jobs (the groups of a plan): group blockIdx.y's words
 // [0, n_g) land at the sum of the earlier groups' word counts; every group's count is its job
 // state's out_base + out_words.  Words past dst_cap are not written.
@@ -1181,9 +1155,6 @@ void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uin
     CCA_LAUNCH("design_classify", 0, k_design_classify, dim3(div_up(nseq, 256u)), dim3(256), 0, st, d_seqs, nseq,
                d_classes, lane_k);
     CCA_LAUNCH_CHECK();
-    CCA_LAUNCH("item_classify", 0, k_item_classify, dim3(div_up(nitems, 256u)), dim3(256), 0, st, d_items, nitems,
-               d_seqs, nseq, d_classes, lane_k);
-    CCA_LAUNCH_CHECK();
     constexpr int kClasses = 4;
     static_assert(kClasses <= kSideStreams, "one side stream per design class");
     const uint32_t kmax[kClasses] = {256, 1024, 2048, kDesignMid};
@@ -1219,35 +1190,14 @@ void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uin
     CCA_LAUNCH("huff_design_global", 0, (k_design<uint32_t, true>), dim3(std::min(nseq, 256u)), dim3(64), 0, st,
                d_seqs, p, scratch, scratch_cap, d_state, d_classes, nseq, 4u);
     CCA_LAUNCH_CHECK();
-    // items whose tables are all small are packed while the large tables are still being designed:
-    // list l on class l's side stream once classes 0..l are done (class l precedes it on that
-    // stream; the smaller classes through their events), the last list after the join
-    static_assert(kPackLists - 1 <= kClasses, "one side stream per early pack list");
-    const uint32_t* lists = d_classes + kDesignListBase + (size_t)kDesignLists * nseq;
-    auto pack = [&](hipStream_t sp, const uint32_t* list, const uint32_t* count, uint64_t bytes) {
-        if (symSize == 2)
-            CCA_LAUNCH("code_pack", bytes, k_code_pack<uint16_t>, dim3(nitems), dim3(kCoderThreads), 0, sp, d_items,
-                       d_seqs, p, scratch, list, count);
-        else
-            CCA_LAUNCH("code_pack", bytes, k_code_pack<uint32_t>, dim3(nitems), dim3(kCoderThreads), 0, sp, d_items,
-                       d_seqs, p, scratch, list, count);
-        CCA_LAUNCH_CHECK();
-    };
-    for (uint32_t l = 0; l + 1 < kPackLists; l++) {
-        if (side) {
-            for (uint32_t c = 0; c < l; c++) CCA_HIP(hipStreamWaitEvent(side->s[l], side->join[c], 0));
-            CCA_HIP(hipEventRecord(side->join[l], side->s[l]));  // class l done (before its pack)
-        }
-        pack(side ? side->s[l] : st, lists + (size_t)l * nitems, d_classes + kPackCountBase + l, l == 0 ? inb : 0);
-    }
     if (side) {
         for (int c = 0; c < kClasses; c++) {
             CCA_HIP(hipEventRecord(side->join[c], side->s[c]));
             CCA_HIP(hipStreamWaitEvent(st, side->join[c], 0));
         }
     }
-    pack(st, lists + (size_t)(kPackLists - 1) * nitems, d_classes + kPackCountBase + kPackLists - 1, 0);
-    CCA_LAUNCH_CHECK();
+    // every table designed: the sections' sizes are known, so the layout comes before the pack,
+    // which writes each section in place
     if (link && link->wait) CCA_HIP(hipStreamWaitEvent(st, link->wait, 0));
     const uint32_t nblocks = div_up(nitems, 1024u);
     uint64_t* bsum = item_off + nitems + 1;  // caller reserves nitems + 1 + nblocks u64
@@ -1262,8 +1212,12 @@ void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uin
                    d_seqs, d_calls, ncalls, d_call_first_item, item_off, out_cap);
     CCA_LAUNCH_CHECK();
     if (link && link->signal) CCA_HIP(hipEventRecord(link->signal, st));
-    CCA_LAUNCH("code_gather", 0, k_gather, dim3(nseq), dim3(kCoderThreads), 0, st, d_items, d_seqs, p, scratch,
-               d_state, d_out, out_cap);
+    if (symSize == 2)
+        CCA_LAUNCH("code_pack", inb, k_code_pack<uint16_t>, dim3(nitems), dim3(kCoderThreads), 0, st, d_items, d_seqs,
+                   p, scratch, d_state, d_out, out_cap);
+    else
+        CCA_LAUNCH("code_pack", inb, k_code_pack<uint32_t>, dim3(nitems), dim3(kCoderThreads), 0, st, d_items, d_seqs,
+                   p, scratch, d_state, d_out, out_cap);
     CCA_LAUNCH_CHECK();
 }
 

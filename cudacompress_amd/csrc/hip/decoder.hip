@@ -312,30 +312,62 @@ __global__ __launch_bounds__(64) void k_dec_blocks(const DecSeq* seqs, const uin
         nxt = wi + 2;
     }
     const uint32_t mlen = s.nlen;
+    // one codeword of length 1 (then it is "0" and every other codeword starts with a 1): a run of
+    // k zero bits is k copies of its symbol, counted at once (the zero-count sequences, where a
+    // zero count is nearly every pair, decode a few steps per 32 symbols instead of 32)
+    const bool run0 = multi && s.tab[64] == 0 && s.tab[32] == 0;
+    const uint32_t sym0 = run0 ? symbol_at(s.tab[0]) : 0u;
+    auto decode_one = [&](uint32_t peek, uint32_t& len) -> uint32_t {
+        const uint32_t e = lut[peek >> (32 - lb)];
+        len = e >> 24;
+        uint32_t sym = e & 0xFFFFFFu;
+        if (e == 0u) {
+            len = kLut + 1;
+            while (len < mlen && (int)(peek >> (32 - len)) > tab[64 + len - 1]) len++;
+            const int cw = (int)(peek >> (32 - len));
+            sym = symbol_at(tab[len - 1] + cw - tab[32 + len - 1]);
+        }
+        return sym;
+    };
     for (uint32_t r = 0; r < cbs; r += 32) {
         // every lane steps through the whole chunk; steps past its block's end store nothing
-#pragma unroll 2
-        for (uint32_t i = 0; i < 32; i++) {
-            uint32_t sym = single;
-            if (multi) {
+        if (run0) {
+            for (uint32_t i = 0; i < 32; i++) stage[lane][i] = (S)sym0;
+            for (uint32_t i = 0; i < 32;) {
                 if (bp >= 32) {
                     cache = (cache << 32) | (uint64_t)word(nxt);
                     nxt++;
                     bp -= 32;
                 }
                 const uint32_t peek = (uint32_t)(cache >> (32 - bp));
-                const uint32_t e = lut[peek >> (32 - lb)];
-                uint32_t len = e >> 24;
-                sym = e & 0xFFFFFFu;
-                if (e == 0u) {
-                    len = kLut + 1;
-                    while (len < mlen && (int)(peek >> (32 - len)) > tab[64 + len - 1]) len++;
-                    const int cw = (int)(peek >> (32 - len));
-                    sym = symbol_at(tab[len - 1] + cw - tab[32 + len - 1]);
+                if ((peek >> 31) == 0u) {
+                    const uint32_t k = min(peek ? (uint32_t)__clz(peek) : 32u, 32u - i);
+                    i += k;
+                    bp += k;
+                } else {
+                    uint32_t len;
+                    stage[lane][i] = (S)decode_one(peek, len);
+                    i++;
+                    bp += len;
                 }
-                bp += len;
             }
-            stage[lane][i] = (S)sym;
+        } else {
+#pragma unroll 2
+            for (uint32_t i = 0; i < 32; i++) {
+                uint32_t sym = single;
+                if (multi) {
+                    if (bp >= 32) {
+                        cache = (cache << 32) | (uint64_t)word(nxt);
+                        nxt++;
+                        bp -= 32;
+                    }
+                    const uint32_t peek = (uint32_t)(cache >> (32 - bp));
+                    uint32_t len;
+                    sym = decode_one(peek, len);
+                    bp += len;
+                }
+                stage[lane][i] = (S)sym;
+            }
         }
         __syncthreads();
         for (uint32_t idx = lane; idx < 64 * 32; idx += 64) {

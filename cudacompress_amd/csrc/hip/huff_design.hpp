@@ -175,8 +175,8 @@ CCA_HD int floor_log2(uint32_t n) {
 
 // The serial introsort loop (without the final insertion sort) over [first, last) with the given
 // remaining depth.  stack: room for 2*floor(log2 n)+2 ranges (3 ints each).
-template <class H, class Less>
-CCA_HD void introsort_loop(H a, int first, int last, int depth, Less less, int* stack) {
+template <class H, class Less, class Stk>
+CCA_HD void introsort_loop(H a, int first, int last, int depth, Less less, Stk stack) {
     int sp = 0;
     for (;;) {
         while (last - first > kSortThreshold) {
@@ -455,12 +455,19 @@ struct PackedQueueLess {
 
 // Designs one table (same results as design_table) from its K leaves: leaf(r) is the (symbol,
 // count) of the r-th used symbol in symbol order (K <= kLaneMaxK, counts summing below
-// kLaneMaxCount); heap has room for kLaneMaxK entries, parent for 2 kLaneMaxK; out.cpl receives
-// 32 entries.  Leaves are read in batches of kLaneBatch so that their loads overlap.
+// kLaneMaxCount); heap has room for kLaneMaxK entries, parent for 2 kLaneMaxK, stack for
+// lane_stack_ints(K) ints; out.cpl receives 32 entries.  Leaves are read in batches of kLaneBatch
+// so that their loads overlap; no array is indexed by a per-thread value other than these.
 constexpr int kLaneBatch = 8;
-template <class LeafF>
+// ints of the introsort stack for tables of at most k symbols: pending ranges <= 2 floor(log2 k) + 2
+CCA_HD constexpr int lane_stack_ints(uint32_t k) {
+    int l = 0;
+    while ((2u << l) <= k) l++;
+    return 3 * (2 * l + 2);
+}
+template <class LeafF, class Stk>
 CCA_HD void design_table_lane(LeafF leaf, int K, int dialect, uint32_t quirk, Strided<uint32_t> heap,
-                              Strided<uint16_t> parent, TableOut out, uint64_t* bits) {
+                              Strided<uint16_t> parent, Stk stack, TableOut out, uint64_t* bits) {
     constexpr uint32_t nodemask = (1u << kLaneNodeBits) - 1;
     for (int l = 0; l < 32; l++) out.cpl[l] = 0;
     *bits = 0;
@@ -522,15 +529,19 @@ CCA_HD void design_table_lane(LeafF leaf, int K, int dialect, uint32_t quirk, St
         out.len[s0] = 0;
         return;
     }
-    // (5) std::sort by depth: the introsort loop, then its final insertion sort as a stable
-    // counting sort by depth; (6) canonical codes
-    int stack[3 * 20];  // pending ranges <= 2 floor(log2 K) + 2 <= 18 for K <= 256
+    // (5) std::sort by depth: the introsort loop, then its final insertion sort (a stable
+    // insertion sort gives the same order: elements only move past strictly deeper ones)
     introsort_loop(heap, 0, K, 2 * floor_log2((uint32_t)K), DepthLess(), stack);
-    uint32_t fi[32], fc[32], seen[32];
-    for (int i = 0; i < K; i++) out.cpl[(heap[i] >> 24) - 1]++;
-    first_codes(out.cpl, 31, fc, fi);
-    for (int l = 0; l < 32; l++) seen[l] = 0;
+    for (int i = 1; i < K; i++) {
+        const uint32_t v = heap[i];
+        int j = i;
+        for (; j > 0 && (heap[j - 1] >> 24) > (v >> 24); j--) heap[j] = heap[j - 1];
+        heap[j] = v;
+    }
+    // (6) canonical codes in sorted order, c_0 = 0, c_i = (c_{i-1} + 1) << (l_i - l_{i-1})
+    // (HuffmanTable.cu:553-580), and the count per length as runs of equal depth
     uint64_t b = 0;
+    uint32_t code = 0, prev = 0, run = 0;
     for (int i0 = 0; i0 < K; i0 += kLaneBatch) {
         uint32_t e[kLaneBatch], sym[kLaneBatch], cnt[kLaneBatch];
         for (int j = 0; j < kLaneBatch; j++) e[j] = i0 + j < K ? heap[i0 + j] : 0u;
@@ -546,13 +557,20 @@ CCA_HD void design_table_lane(LeafF leaf, int K, int dialect, uint32_t quirk, St
         for (int j = 0; j < kLaneBatch; j++) {
             if (i0 + j >= K) break;
             const uint32_t d = e[j] >> 24, s = sym[j];
-            const uint32_t rank = seen[d - 1]++;
-            out.syms[fi[d - 1] + rank] = s;
-            out.code[s] = fc[d - 1] + rank;
+            if (i0 + j > 0) code = (code + 1) << (d - prev);
+            if (d != prev && run) {
+                out.cpl[prev - 1] = run;
+                run = 0;
+            }
+            run++;
+            prev = d;
+            out.syms[i0 + j] = s;
+            out.code[s] = code;
             out.len[s] = d;
             b += (uint64_t)(cnt[j] - ((quirk && s == 65535u) ? 1u : 0u)) * d;
         }
     }
+    out.cpl[prev - 1] = run;
     out.info[0] = (uint32_t)deepest;
     *bits = b;
 }

@@ -89,8 +89,7 @@ void encode_prepare(cc_instance* I, DeviceCoder& D, const std::vector<ItemSpec>&
     D.d_call_first.as<uint32_t>(ncalls + 1);
     D.d_seqs.as<SeqInfo>(std::max<uint32_t>(job.nseq, 1));
     D.d_state.as<CoderState>(1);
-    D.d_classes.as<uint32_t>(kDesignListBase + (uint64_t)kDesignLists * std::max<uint32_t>(job.nseq, 1) +
-                             (uint64_t)kPackLists * std::max<uint32_t>(nitems, 1));
+    D.d_classes.as<uint32_t>(kDesignListBase + (uint64_t)kDesignLists * std::max<uint32_t>(job.nseq, 1));
     D.d_calls.as<CallOut>(std::max<uint32_t>(ncalls, 1));
     D.h_result.get(sizeof(CoderState) + sizeof(CallOut) * std::max<uint32_t>(ncalls, 1));
     CCA_HIP(hipMemcpyAsync(D.d_items.p, hi, sizeof(CodeItem) * nitems, hipMemcpyHostToDevice, st));

@@ -105,8 +105,10 @@ bool check_one(const std::vector<uint32_t>& hist, int dialect, uint32_t quirk, c
         std::vector<U2> leaves;
         for (uint32_t s = 0; s < nb; s++)
             if (hist[s]) leaves.push_back(U2{s, hist[s]});
+        std::vector<int> lstack(3 * lane_stack_ints(kLaneMaxK) + 3, -7);
         design_table_lane([&](int r) { return leaves[r]; }, (int)K, dialect, quirk,
-                          Strided<uint32_t>{lheap.data() + 1, 3}, Strided<uint16_t>{lpar.data() + 1, 3}, out, &bits);
+                          Strided<uint32_t>{lheap.data() + 1, 3}, Strided<uint16_t>{lpar.data() + 1, 3},
+                          Strided<int>{lstack.data() + 1, 3}, out, &bits);
     } else if (form == 2) {
         WalkEx ex;
         design_table(ex, hist.data(), nb, dialect, quirk, w, &ctl, out, &bits);
