@@ -1,8 +1,9 @@
 // Decode serving loop: compressed bricks stay resident in device memory; requests (lists of
-// brick ids with caller-owned output bricks) are admitted as they arrive, batched by a
-// decode-workspace budget into one brick batch per free slot, and launched asynchronously on that
-// slot's stream, so batches of different slots overlap on the device.  Completion is observed
-// with an event per slot; a request completes when its batch does.
+// brick ids with caller-owned output bricks) are admitted brick by brick as they arrive, batched
+// by a decode-workspace budget into one brick batch per free slot, and launched asynchronously on
+// that slot's stream, so batches of different slots overlap on the device and a large request
+// spreads over several slots.  Completion is observed with an event per slot; a request completes
+// when the batch holding its last brick does.
 //
 // No reference counterpart (the reference decodes one volume per call).  Each slot owns a brick
 // batch context (host/bricks.cpp), so a request decodes to exactly the values of a batch that
@@ -20,11 +21,18 @@
 
 namespace {
 
+// A request's bricks are admitted one by one: they may land in batches of different slots, and
+// the request completes when its last brick's batch does.
 struct Request {
     uint64_t id;
-    std::vector<uint32_t> bricks;
-    std::vector<float*> outs;
-    double arrival;
+    uint32_t remaining;  // bricks not yet decoded
+    uint32_t err;        // first decoder error among its bricks
+};
+
+struct WorkItem {        // one brick of a request, waiting for a slot
+    uint32_t req;        // index into the server's request table
+    uint32_t brick;
+    float* out;
     uint64_t workspace;
 };
 
@@ -34,8 +42,7 @@ struct Slot {
     hipStream_t st = nullptr;
     hipEvent_t done = nullptr;
     bool busy = false;
-    std::vector<uint64_t> reqs;
-    std::vector<uint32_t> req_nbricks;
+    std::vector<uint32_t> brick_req;  // request of every brick of the batch, in batch order
 };
 
 uint64_t elems3(const uint32_t* d) { return (uint64_t)d[0] * d[1] * d[2]; }
@@ -53,7 +60,10 @@ struct cc_server {
     int symSize = 2;
     uint64_t slot_budget = 0;
     std::vector<Slot> slots;
-    std::deque<Request> pending;
+    std::deque<WorkItem> pending;       // bricks in arrival order
+    std::vector<Request> reqs;          // request table (entries reused through free_reqs)
+    std::vector<uint32_t> free_reqs;
+    uint32_t live_reqs = 0;             // submitted and not yet finished
     std::vector<std::pair<uint64_t, double>> completed;
     std::vector<std::pair<uint64_t, uint32_t>> failed;
     ~cc_server();
@@ -79,6 +89,17 @@ namespace {
 // decode workspace of one brick: work buffers (2 N + N/7 floats), symbols, coder arena
 uint64_t brick_workspace(uint64_t N, int symSize) { return N * 4 * 2 + N * 4 / 7 + N * symSize + N * 6 + 64 * 1024; }
 
+// One brick of request r finished (err: its decoder status).
+void finish_brick(cc_server* S, uint32_t r, uint32_t err, double now) {
+    Request& q = S->reqs[r];
+    if (err && !q.err) q.err = err;
+    if (--q.remaining) return;
+    if (q.err) S->failed.emplace_back(q.id, q.err);
+    else S->completed.emplace_back(q.id, now);
+    S->free_reqs.push_back(r);
+    S->live_reqs--;
+}
+
 void reap(cc_server* S, Slot& s, double now) {
     if (!s.busy) return;
     const hipError_t q = hipEventQuery(s.done);
@@ -86,44 +107,30 @@ void reap(cc_server* S, Slot& s, double now) {
     CCA_HIP(q);
     // per-brick statuses were copied to the pinned result buffer before the event
     const DecCallOut* r = (const DecCallOut*)((const unsigned char*)s.ctx.coder.h_dresult.p + sizeof(CoderState));
-    uint32_t k = 0;
-    for (size_t i = 0; i < s.reqs.size(); i++) {
-        uint32_t err = 0;
-        for (uint32_t j = 0; j < s.req_nbricks[i]; j++, k++)
-            if (r[k].err && !err) err = r[k].err;
-        if (err) S->failed.emplace_back(s.reqs[i], err);
-        else S->completed.emplace_back(s.reqs[i], now);
-    }
+    for (size_t k = 0; k < s.brick_req.size(); k++) finish_brick(S, s.brick_req[k], r[k].err, now);
     s.busy = false;
-    s.reqs.clear();
-    s.req_nbricks.clear();
+    s.brick_req.clear();
 }
 
-void launch(cc_server* S, Slot& s) {
-    // admit requests in arrival order while the batch fits the slot's workspace budget; the batch
-    // is built aside and becomes the slot's only once it is on the device
-    std::vector<uint32_t> ids;
-    std::vector<uint64_t> reqs;
-    std::vector<uint32_t> nbricks;
+void launch(cc_server* S, Slot& s, double now) {
+    // admit bricks in arrival order while the batch fits the slot's workspace budget (at least
+    // one); the batch is built aside and becomes the slot's only once it is on the device
+    std::vector<uint32_t> ids, breq;
     BrickJob job;
     job.L = S->levels;
     job.symSize = S->symSize;
     uint64_t ws = 0;
     while (!S->pending.empty()) {
-        Request& rq = S->pending.front();
-        if (!reqs.empty() && ws + rq.workspace > S->slot_budget) break;
-        ws += rq.workspace;
-        for (size_t j = 0; j < rq.bricks.size(); j++) {
-            const uint32_t b = rq.bricks[j];
-            const uint32_t* d = &S->dims[3 * b];
-            job.bricks.push_back(cc_brick{nullptr, rq.outs[j], d[0], d[1], d[2], S->steps[b]});
-            ids.push_back(b);
-        }
-        reqs.push_back(rq.id);
-        nbricks.push_back((uint32_t)rq.bricks.size());
+        const WorkItem& w = S->pending.front();
+        if (!ids.empty() && ws + w.workspace > S->slot_budget) break;
+        ws += w.workspace;
+        const uint32_t* d = &S->dims[3 * w.brick];
+        job.bricks.push_back(cc_brick{nullptr, w.out, d[0], d[1], d[2], S->steps[w.brick]});
+        ids.push_back(w.brick);
+        breq.push_back(w.req);
         S->pending.pop_front();
     }
-    if (reqs.empty()) return;
+    if (ids.empty()) return;
     try {
         s.job = std::move(job);
         bricks_prepare_bound(S->inst, s.ctx, s.job, S->index, ids, s.st);
@@ -136,17 +143,16 @@ void launch(cc_server* S, Slot& s) {
                                hipMemcpyDeviceToHost, s.st));
         CCA_HIP(hipEventRecord(s.done, s.st));
     } catch (const Error& e) {
-        // the admitted requests fail with the launch's status; the slot stays free and empty
-        for (uint64_t id : reqs) S->failed.emplace_back(id, (uint32_t)e.status);
+        // the admitted bricks fail their requests with the launch's status; the slot stays free
+        for (uint32_t r : breq) finish_brick(S, r, (uint32_t)e.status, now);
         s.job = BrickJob();
         throw;
     } catch (const std::bad_alloc&) {
-        for (uint64_t id : reqs) S->failed.emplace_back(id, (uint32_t)CC_ERR_OUT_OF_MEMORY);
+        for (uint32_t r : breq) finish_brick(S, r, (uint32_t)CC_ERR_OUT_OF_MEMORY, now);
         s.job = BrickJob();
         throw;
     }
-    s.reqs = std::move(reqs);
-    s.req_nbricks = std::move(nbricks);
+    s.brick_req = std::move(breq);
     s.busy = true;
 }
 
@@ -190,19 +196,24 @@ cc_server* server_create(cc_instance* I, const uint32_t* d_words, uint32_t nbric
 
 void server_submit(cc_server* S, uint64_t id, const uint32_t* bricks, uint32_t count, float* const* outs,
                    double arrival) {
+    (void)arrival;
     CCA_REQUIRE(count >= 1 && bricks && outs, CC_ERR_INVALID_ARG, "a request names at least one brick");
-    Request r;
-    r.id = id;
-    r.arrival = arrival;
-    r.workspace = 0;
     for (uint32_t i = 0; i < count; i++) {
         CCA_REQUIRE(bricks[i] < S->begin.size(), CC_ERR_INVALID_ARG, "brick id out of range");
         CCA_REQUIRE(outs[i], CC_ERR_INVALID_ARG, "null output brick");
-        r.bricks.push_back(bricks[i]);
-        r.outs.push_back(outs[i]);
-        r.workspace += brick_workspace(elems3(&S->dims[3 * bricks[i]]), S->symSize);
     }
-    S->pending.push_back(std::move(r));
+    uint32_t r;
+    if (!S->free_reqs.empty()) {
+        r = S->free_reqs.back();
+        S->free_reqs.pop_back();
+    } else {
+        r = (uint32_t)S->reqs.size();
+        S->reqs.emplace_back();
+    }
+    S->reqs[r] = Request{id, count, 0u};
+    S->live_reqs++;
+    for (uint32_t i = 0; i < count; i++)
+        S->pending.push_back(WorkItem{r, bricks[i], outs[i], brick_workspace(elems3(&S->dims[3 * bricks[i]]), S->symSize)});
 }
 
 uint32_t server_poll(cc_server* S, double now) {
@@ -212,7 +223,7 @@ uint32_t server_poll(cc_server* S, double now) {
     for (Slot& s : S->slots) {
         if (S->pending.empty()) break;
         if (s.busy) continue;
-        launch(S, s);
+        launch(S, s, now);
         launched++;
     }
     return launched;
@@ -224,7 +235,7 @@ const void* server_owner(const cc_server* S) { return S ? S->inst : nullptr; }
 uint32_t server_busy(const cc_server* S) {
     uint32_t b = 0;
     for (const Slot& s : S->slots) b += s.busy ? 1 : 0;
-    return b + (uint32_t)S->pending.size();
+    return b + S->live_reqs;
 }
 
 uint32_t server_take_completed(cc_server* S, uint64_t* ids, double* times, uint32_t max) {

@@ -419,11 +419,12 @@ CC_API cc_status cc_bricks_plan_coding_stats(cc_bricks_plan* plan, uint64_t* sta
  * Brick i is the bitstream d_words[brickWordRanges[2i], brickWordRanges[2i+1]) of a brick of
  * brickSizes[3i..3i+2] (x, y, z) compressed with quantSteps[i] and numLevels (as written by
  * cc_compress_bricks_float).  Requests name bricks and caller-owned device output bricks;
- * cc_server_poll launches every pending request that fits on a free slot -- one batch per slot,
- * up to workspaceBytes / slots of decode workspace -- asynchronously on that slot's stream, and
- * records the requests of finished batches as completed at time `now` (the caller's clock).
- * Batches of different slots run concurrently.  Requests decode to the same values whatever they
- * are batched with.  A server is driven from one host thread. */
+ * cc_server_poll admits pending bricks in arrival order onto free slots -- one batch per slot, up
+ * to workspaceBytes / slots of decode workspace (at least one brick) -- asynchronously on that
+ * slot's stream, and records a request as completed at time `now` (the caller's clock) once the
+ * batches holding all of its bricks have finished.  Batches of different slots run concurrently;
+ * a request's bricks may be spread over several.  Requests decode to the same values whatever
+ * they are batched with.  A server is driven from one host thread. */
 typedef struct cc_server cc_server;
 CC_API cc_status cc_server_create(cc_instance* inst, const uint32_t* d_words, uint32_t brickCount,
                                   const uint64_t* brickWordRanges, const uint32_t* brickSizes,
@@ -432,7 +433,7 @@ CC_API cc_status cc_server_create(cc_instance* inst, const uint32_t* d_words, ui
 CC_API cc_status cc_server_submit(cc_server* server, uint64_t requestId, const uint32_t* brickIds, uint32_t count,
                                   float* const* d_outputs, double arrivalTime);
 CC_API cc_status cc_server_poll(cc_server* server, double now, uint32_t* batchesLaunched);
-/* batches in flight plus requests waiting */
+/* batches in flight plus requests not yet finished */
 CC_API uint32_t cc_server_pending(const cc_server* server);
 /* moves up to `max` completed requests (id, completion time) out; returns how many */
 CC_API uint32_t cc_server_completed(cc_server* server, uint64_t* requestIds, double* times, uint32_t max);
