@@ -180,10 +180,20 @@ void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uin
                   const LayoutLink* link = nullptr, const uint32_t* design_grid = nullptr, uint32_t lane_k = 0);
 // wave_seq: for every decode wave the coded sequence it serves (upper bound of the block count:
 // RLE sequences have at most n pairs); wave_first: that wave's first block.
+// Zero-run expansion in chunks of kRleChunk pairs, every chunk its own workgroup: item i of a
+// decode job has rle_chunks(n_i, rl_i) chunks, numbered from item_first[i] (nitems + 1 entries,
+// item_first[nitems] = nchunks); sums receives one word per chunk.
+constexpr uint32_t kRleChunk = 2048;
+inline uint32_t rle_chunks(uint32_t n, bool rl) { return rl ? (n + kRleChunk - 1) / kRleChunk : 0u; }
+struct RleChunks {
+    const uint32_t* item_first;
+    uint32_t nchunks;
+    uint32_t* sums;
+};
 void coder_decode(const DecCall* d_calls, uint32_t ncalls, const DecItem* d_items, uint32_t nitems, DecSeq* d_seqs,
                   uint32_t nseq, const CoderParams& p, int symSize, uint32_t* scratch, uint64_t scratch_cap,
                   CoderState* d_state, DecCallOut* d_out, const uint32_t* d_wave_seq, const uint32_t* d_wave_first,
-                  uint32_t nwaves, uint64_t total_symbols, hipStream_t st);
+                  uint32_t nwaves, uint64_t total_symbols, const RleChunks& rc, hipStream_t st);
 // Resident decode index: parse + tables of every call once (RLE sequences get no decode buffer),
 // per-call results in d_out.  scratch then holds the tables and block offsets for bound decodes.
 void coder_decode_index(const DecCall* d_calls, uint32_t ncalls, const DecItem* d_items, DecSeq* d_seqs, uint32_t nseq,
@@ -196,7 +206,7 @@ void coder_decode_bound(const DecSeq* d_idx_seqs, const uint32_t* idx_scratch, c
                         const DecItem* d_items, uint32_t nitems, DecSeq* d_seqs, uint32_t nseq, const CoderParams& p,
                         int symSize, unsigned char* rl_arena, CoderState* d_state, DecCallOut* d_out,
                         const uint32_t* d_wave_seq, const uint32_t* d_wave_first, uint32_t nwaves,
-                        uint64_t total_symbols, hipStream_t st);
+                        uint64_t total_symbols, const RleChunks& rc, hipStream_t st);
 // calls[i].words = base + enc[i].word_begin, calls[i].nwords = enc[i].words (the rest unchanged);
 // an empty range for a failed call or one reaching past cap words
 void coder_calls_from_encode(DecCall* d_calls, const CallOut* d_enc, const uint32_t* base, uint64_t cap,

@@ -418,73 +418,122 @@ __device__ __forceinline__ void clear_symbols(S* out, uint32_t n) {
     for (uint32_t i = head + nv * per + threadIdx.x; i < n; i += blockDim.x) out[i] = (S)0;
 }
 
+__device__ __forceinline__ uint32_t chunk_item(const uint32_t* first, uint32_t nitems, uint32_t b) {
+    uint32_t lo = 0, hi = nitems;  // first[lo] <= b < first[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (first[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// The first pass of the zero-run expansion: chunk b's output span, the sum of (zero count + 1)
+// over its pairs (0 past the sequence's pairs).
 template <class S>
-__global__ __launch_bounds__(256) void k_dec_rle(const DecItem* items, DecSeq* seqs, CoderState* st) {
-    constexpr uint32_t kChunk = 256 * 8;
+__global__ __launch_bounds__(256) void k_dec_rle_sums(const DecItem* items, uint32_t nitems, const DecSeq* seqs,
+                                                      RleChunks rc) {
     __shared__ uint32_t sh[4];
-    __shared__ uint32_t spos[kChunk];
-    __shared__ uint32_t sval[kChunk];
-    const DecItem it = items[blockIdx.x];
-    if (!it.rl) return;
+    const uint32_t b = blockIdx.x;
+    const uint32_t i = chunk_item(rc.item_first, nitems, b);
+    const DecItem it = items[i];
+    const DecSeq& ss = seqs[it.seq];
+    const DecSeq& sz = seqs[it.seq + 1];
+    const uint32_t M = (ss.err || sz.err) ? 0u : ss.count;
+    const S* zc = static_cast<const S*>(sz.out);
+    const uint32_t i0 = (b - rc.item_first[i]) * kRleChunk + threadIdx.x * 8;
+    uint32_t z[8], sum = 0;
+    if (((reinterpret_cast<uintptr_t>(zc) & 15u) == 0) && i0 + 8 <= M) {
+        load8_aligned(zc + i0, z);
+#pragma unroll
+        for (int k = 0; k < 8; k++) sum += z[k] + 1;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; k++) sum += i0 + k < M ? (uint32_t)zc[i0 + k] + 1 : 0u;
+    }
+    uint32_t tot;
+    (void)block_excl_scan<256>(sum, sh, tot);
+    if (threadIdx.x == 0) rc.sums[b] = tot;
+}
+
+// One workgroup per chunk of kRleChunk pairs.  Pair i (value v_i after z_i zeros) lands at
+// sum_{j<i}(z_j + 1) + z_i: the chunk starts at the sum of the earlier chunks' spans and scans
+// its own pairs (8 consecutive per thread).  It clears its span of output positions with 16-byte
+// coalesced stores (to the end of the output for the sequence's last chunk: the trailing zeros),
+// then stores its values there, staged in LDS and issued with a 256-pair stride so that one store
+// instruction writes 64 consecutive pairs; the span's lines are still in L2 when the values land,
+// so each output line leaves L2 once.
+template <class S>
+__global__ __launch_bounds__(256) void k_dec_rle(const DecItem* items, uint32_t nitems, DecSeq* seqs, RleChunks rc,
+                                                 CoderState* st) {
+    __shared__ uint32_t sh[4];
+    __shared__ uint64_t shs[4];
+    __shared__ uint32_t spos[kRleChunk];
+    __shared__ uint32_t sval[kRleChunk];
+    const uint32_t b = blockIdx.x;
+    const uint32_t ii = chunk_item(rc.item_first, nitems, b);
+    const DecItem it = items[ii];
     const DecSeq& ss = seqs[it.seq];
     const DecSeq& sz = seqs[it.seq + 1];
     if (ss.err || sz.err) return;
     const uint32_t M = ss.count, n = it.n;
+    const uint32_t c = b - rc.item_first[ii];
+    const uint32_t clast = M == 0 ? 0u : (M - 1) / kRleChunk;
+    if (c > clast) return;
+    const uint32_t base = c * kRleChunk;
+    const bool last = c == clast;
+    // the chunk's first output position: the earlier chunks' spans
+    uint64_t before = 0;
+    for (uint32_t k = threadIdx.x; k < c; k += 256) before += rc.sums[rc.item_first[ii] + k];
+    const uint64_t carry = block_sum_u64<256>(before, shs);
     const S* sym = static_cast<const S*>(ss.out);
     const S* zc = static_cast<const S*>(sz.out);
     S* out = static_cast<S*>(it.out);
     // LDS index swizzle: thread t's 8 consecutive entries and the strided reads are both
     // conflict-free across the 64 banks
     auto sw = [](uint32_t i) { return i ^ ((i >> 6) & 7u); };
-    uint64_t carry = 0;
     bool bad = false;
-    // the decode buffers are 64-byte aligned arena ranges: a thread's 8 pairs are two 16-byte
-    // (Symbol16) or four (Symbol32) loads
+    // the decode buffers are 64-byte aligned: a thread's 8 pairs are two 16-byte (Symbol16) or
+    // four (Symbol32) loads
     const bool vec = ((reinterpret_cast<uintptr_t>(sym) | reinterpret_cast<uintptr_t>(zc)) & 15u) == 0;
-    for (uint32_t base = 0;; base += kChunk) {
-        const bool last = (uint64_t)base + kChunk >= M;
-        uint32_t z[8], v[8], sum = 0;
-        const uint32_t i0 = base + threadIdx.x * 8;
-        if (vec && i0 + 8 <= M) {
-            load8_aligned(zc + i0, z);
-            load8_aligned(sym + i0, v);
+    uint32_t z[8], v[8], sum = 0;
+    const uint32_t i0 = base + threadIdx.x * 8;
+    if (vec && i0 + 8 <= M) {
+        load8_aligned(zc + i0, z);
+        load8_aligned(sym + i0, v);
 #pragma unroll
-            for (int k = 0; k < 8; k++) sum += z[k] + 1;
-        } else {
-#pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const uint32_t i = i0 + k;
-                z[k] = i < M ? (uint32_t)zc[i] : 0u;
-                v[k] = i < M ? (uint32_t)sym[i] : 0u;
-                sum += i < M ? z[k] + 1 : 0u;
-            }
-        }
-        uint32_t tot;
-        const uint32_t ex = block_excl_scan<256>(sum, sh, tot);
-        uint64_t pos = carry + ex;
+        for (int k = 0; k < 8; k++) sum += z[k] + 1;
+    } else {
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            pos += z[k];
-            const uint32_t e = sw(threadIdx.x * 8 + k);
-            const bool inb = pos < n;
-            bad |= !inb && base + threadIdx.x * 8 + k < M;
-            spos[e] = inb ? (uint32_t)pos : 0xFFFFFFFFu;
-            sval[e] = v[k];
-            pos++;
+            const uint32_t i = i0 + k;
+            z[k] = i < M ? (uint32_t)zc[i] : 0u;
+            v[k] = i < M ? (uint32_t)sym[i] : 0u;
+            sum += i < M ? z[k] + 1 : 0u;
         }
-        // the chunk's span of output positions, cleared before its values land
-        const uint64_t s1 = last ? (uint64_t)n : min(carry + tot, (uint64_t)n);
-        if (carry < s1) clear_symbols(out + carry, (uint32_t)(s1 - carry));
-        __syncthreads();  // the clearing stores complete before values land on them
+    }
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan<256>(sum, sh, tot);
+    uint64_t pos = carry + ex;
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const uint32_t idx = k * 256 + threadIdx.x;
-            const uint32_t q = spos[sw(idx)];
-            if (base + idx < M && q != 0xFFFFFFFFu) out[q] = (S)sval[sw(idx)];
-        }
-        __syncthreads();
-        carry += tot;
-        if (last) break;
+    for (int k = 0; k < 8; k++) {
+        pos += z[k];
+        const uint32_t e = sw(threadIdx.x * 8 + k);
+        const bool inb = pos < n;
+        bad |= !inb && base + threadIdx.x * 8 + k < M;
+        spos[e] = inb ? (uint32_t)pos : 0xFFFFFFFFu;
+        sval[e] = v[k];
+        pos++;
+    }
+    // the chunk's span of output positions, cleared before its values land
+    const uint64_t s1 = last ? (uint64_t)n : min(carry + tot, (uint64_t)n);
+    if (carry < s1) clear_symbols(out + carry, (uint32_t)(s1 - carry));
+    __syncthreads();  // the clearing stores complete before values land on them
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const uint32_t idx = k * 256 + threadIdx.x;
+        const uint32_t q = spos[sw(idx)];
+        if (base + idx < M && q != 0xFFFFFFFFu) out[q] = (S)sval[sw(idx)];
     }
     if (bad) {
         seqs[it.seq].err = kDecCorrupt;
@@ -548,10 +597,23 @@ void coder_calls_from_encode(DecCall* d_calls, const CallOut* d_enc, const uint3
     CCA_LAUNCH_CHECK();
 }
 
+namespace {
+// The zero-run expansion: chunk spans, then the chunks.
+template <class S>
+void launch_rle(const DecItem* d_items, uint32_t nitems, DecSeq* d_seqs, const RleChunks& rc, CoderState* d_state,
+                uint64_t outb, hipStream_t st) {
+    if (rc.nchunks == 0) return;
+    CCA_LAUNCH("dec_rle_sums", 0, k_dec_rle_sums<S>, dim3(rc.nchunks), dim3(256), 0, st, d_items, nitems, d_seqs, rc);
+    CCA_LAUNCH_CHECK();
+    CCA_LAUNCH("dec_rle", outb, k_dec_rle<S>, dim3(rc.nchunks), dim3(256), 0, st, d_items, nitems, d_seqs, rc, d_state);
+    CCA_LAUNCH_CHECK();
+}
+}  // namespace
+
 void coder_decode(const DecCall* d_calls, uint32_t ncalls, const DecItem* d_items, uint32_t nitems, DecSeq* d_seqs,
                   uint32_t nseq, const CoderParams& p, int symSize, uint32_t* scratch, uint64_t scratch_cap,
                   CoderState* d_state, DecCallOut* d_out, const uint32_t* d_wave_seq, const uint32_t* d_wave_first,
-                  uint32_t nwaves, uint64_t total_symbols, hipStream_t st) {
+                  uint32_t nwaves, uint64_t total_symbols, const RleChunks& rc, hipStream_t st) {
     CCA_REQUIRE(symSize == 2 || symSize == 4, CC_ERR_INVALID_ARG, "symbol size must be 2 or 4");
     CCA_LAUNCH("dec_init", 32, k_dec_init, dim3(1), dim3(64), 0, st, d_state);
     CCA_LAUNCH_CHECK();
@@ -572,11 +634,8 @@ void coder_decode(const DecCall* d_calls, uint32_t ncalls, const DecItem* d_item
                        d_wave_first, p, scratch);
         CCA_LAUNCH_CHECK();
     }
-    if (symSize == 2)
-        CCA_LAUNCH("dec_rle", outb, k_dec_rle<uint16_t>, dim3(nitems), dim3(256), 0, st, d_items, d_seqs, d_state);
-    else
-        CCA_LAUNCH("dec_rle", outb, k_dec_rle<uint32_t>, dim3(nitems), dim3(256), 0, st, d_items, d_seqs, d_state);
-    CCA_LAUNCH_CHECK();
+    if (symSize == 2) launch_rle<uint16_t>(d_items, nitems, d_seqs, rc, d_state, outb, st);
+    else launch_rle<uint32_t>(d_items, nitems, d_seqs, rc, d_state, outb, st);
     CCA_LAUNCH("dec_finish", 0, k_dec_finish, dim3(div_up(ncalls, 64)), dim3(64), 0, st, d_calls, ncalls, d_items,
                d_seqs, d_out);
     CCA_LAUNCH_CHECK();
@@ -586,14 +645,14 @@ namespace {
 template <class S>
 void launch_blocks_rle(const DecItem* d_items, uint32_t nitems, DecSeq* d_seqs, const CoderParams& p,
                        const uint32_t* scratch, CoderState* d_state, const uint32_t* d_wave_seq,
-                       const uint32_t* d_wave_first, uint32_t nwaves, uint64_t outb, hipStream_t st) {
+                       const uint32_t* d_wave_first, uint32_t nwaves, uint64_t outb, const RleChunks& rc,
+                       hipStream_t st) {
     if (nwaves) {
         CCA_LAUNCH("dec_blocks", outb, k_dec_blocks<S>, dim3(nwaves), dim3(64), 0, st, d_seqs, d_wave_seq, d_wave_first,
                    p, scratch);
         CCA_LAUNCH_CHECK();
     }
-    CCA_LAUNCH("dec_rle", outb, k_dec_rle<S>, dim3(nitems), dim3(256), 0, st, d_items, d_seqs, d_state);
-    CCA_LAUNCH_CHECK();
+    launch_rle<S>(d_items, nitems, d_seqs, rc, d_state, outb, st);
 }
 }  // namespace
 
@@ -620,7 +679,7 @@ void coder_decode_bound(const DecSeq* d_idx_seqs, const uint32_t* idx_scratch, c
                         const DecItem* d_items, uint32_t nitems, DecSeq* d_seqs, uint32_t nseq, const CoderParams& p,
                         int symSize, unsigned char* rl_arena, CoderState* d_state, DecCallOut* d_out,
                         const uint32_t* d_wave_seq, const uint32_t* d_wave_first, uint32_t nwaves,
-                        uint64_t total_symbols, hipStream_t st) {
+                        uint64_t total_symbols, const RleChunks& rc, hipStream_t st) {
     CCA_REQUIRE(symSize == 2 || symSize == 4, CC_ERR_INVALID_ARG, "symbol size must be 2 or 4");
     CCA_LAUNCH("dec_init", 32, k_dec_init, dim3(1), dim3(64), 0, st, d_state);
     CCA_LAUNCH_CHECK();
@@ -632,10 +691,10 @@ void coder_decode_bound(const DecSeq* d_idx_seqs, const uint32_t* idx_scratch, c
     const uint64_t outb = total_symbols * symSize;
     if (symSize == 2)
         launch_blocks_rle<uint16_t>(d_items, nitems, d_seqs, p, idx_scratch, d_state, d_wave_seq, d_wave_first, nwaves,
-                                    outb, st);
+                                    outb, rc, st);
     else
         launch_blocks_rle<uint32_t>(d_items, nitems, d_seqs, p, idx_scratch, d_state, d_wave_seq, d_wave_first, nwaves,
-                                    outb, st);
+                                    outb, rc, st);
     CCA_LAUNCH("dec_finish", 0, k_dec_finish, dim3(div_up(ncalls, 64)), dim3(64), 0, st, d_calls, ncalls, d_items,
                d_seqs, d_out);
     CCA_LAUNCH_CHECK();

@@ -22,21 +22,24 @@ namespace {
 struct NoAllocArgs {
     const uint32_t* words;
     uint64_t nwords;
-    uint32_t n, nstreams, rl, waves_per_seq;
+    uint32_t n, nstreams, rl, waves_per_seq, chunks_per_item;
     DecCall* calls;
     DecItem* items;
     uint32_t* wave_seq;
     uint32_t* wave_first;
+    uint32_t* item_first;
     void* outs[kNoAllocMaxStreams];
 };
 
 // Writes the job tables of one call of nstreams equal items: item s decodes sequence s*q (+1
-// for its zero counts, q = 2 for RLE items), each sequence's blocks in waves of 64.
+// for its zero counts, q = 2 for RLE items), each sequence's blocks in waves of 64, its zero-run
+// expansion chunks from s * chunks_per_item.
 __global__ void k_noalloc_setup(NoAllocArgs a) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t q = a.rl ? 2u : 1u;
     if (t == 0) a.calls[0] = DecCall{a.words, a.nwords, 0, a.nstreams};
     if (t < a.nstreams) a.items[t] = DecItem{a.outs[t], a.n, a.rl, t * q, 0};
+    if (t <= a.nstreams) a.item_first[t] = t * a.chunks_per_item;
     const uint32_t nwaves = a.nstreams * q * a.waves_per_seq;
     if (t < nwaves) {
         a.wave_seq[t] = t / a.waves_per_seq;
@@ -93,7 +96,11 @@ uint64_t decode_device_noalloc(cc_instance* I, bool rl, int symSize, const uint3
     const uint64_t o_state = o_seqs + align_up(sizeof(DecSeq) * 2 * kNoAllocMaxStreams, 256);
     const uint64_t o_wseq = o_state + 256;  // CoderState, then the call's DecCallOut
     const uint64_t o_wfirst = o_wseq + align_up(4 * std::max<uint64_t>(nwaves, 1), 256);
-    const uint64_t o_arena = o_wfirst + align_up(4 * std::max<uint64_t>(nwaves, 1), 256);
+    const uint32_t cpi = rle_chunks(n, rl);
+    const uint64_t nchunks = (uint64_t)cpi * nstreams;
+    const uint64_t o_ifirst = o_wfirst + align_up(4 * std::max<uint64_t>(nwaves, 1), 256);
+    const uint64_t o_sums = o_ifirst + align_up(4 * ((uint64_t)nstreams + 1), 256);
+    const uint64_t o_arena = o_sums + align_up(4 * std::max<uint64_t>(nchunks, 1), 256);
     CCA_REQUIRE(o_arena + 4 * arena_words <= kNoAllocScratchBytes, CC_ERR_INVALID_ARG,
                 "call too large for the no-allocation decode scratch (32 MiB per device)");
     static_assert(sizeof(CoderState) + sizeof(DecCallOut) <= 256, "state and call result share 256 bytes");
@@ -105,12 +112,14 @@ uint64_t decode_device_noalloc(cc_instance* I, bool rl, int symSize, const uint3
     a.nstreams = nstreams;
     a.rl = rl ? 1u : 0u;
     a.waves_per_seq = waves_per_seq;
+    a.chunks_per_item = cpi;
     a.calls = reinterpret_cast<DecCall*>(base);
     a.items = reinterpret_cast<DecItem*>(base + o_items);
     a.wave_seq = reinterpret_cast<uint32_t*>(base + o_wseq);
     a.wave_first = reinterpret_cast<uint32_t*>(base + o_wfirst);
+    a.item_first = reinterpret_cast<uint32_t*>(base + o_ifirst);
     for (uint32_t s = 0; s < nstreams; s++) a.outs[s] = outs[s];
-    const uint64_t setup_threads = std::max<uint64_t>({nwaves, (uint64_t)nstreams, 1});
+    const uint64_t setup_threads = std::max<uint64_t>({nwaves, (uint64_t)nstreams + 1, 1});
     CCA_LAUNCH("noalloc_setup", 0, k_noalloc_setup, dim3(div_up(setup_threads, 256)), dim3(256), 0, st, a);
     CCA_LAUNCH_CHECK();
 
@@ -118,7 +127,8 @@ uint64_t decode_device_noalloc(cc_instance* I, bool rl, int symSize, const uint3
     DecCallOut* d_out = reinterpret_cast<DecCallOut*>(base + o_state + sizeof(CoderState));
     coder_decode(a.calls, 1, a.items, nstreams, reinterpret_cast<DecSeq*>(base + o_seqs), nseq, p, symSize,
                  reinterpret_cast<uint32_t*>(base + o_arena), arena_words, d_state, d_out, a.wave_seq, a.wave_first,
-                 (uint32_t)nwaves, (uint64_t)nstreams * n, st);
+                 (uint32_t)nwaves, (uint64_t)nstreams * n,
+                 RleChunks{a.item_first, (uint32_t)nchunks, reinterpret_cast<uint32_t*>(base + o_sums)}, st);
     struct {
         CoderState s;
         DecCallOut o;

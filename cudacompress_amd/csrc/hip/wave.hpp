@@ -94,4 +94,19 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* scratc
     return base + incl - v;
 }
 
+// Sum over a workgroup of NT threads (NT multiple of 64), every thread gets it; NT/64 words of
+// LDS scratch; contains barriers, so every thread must call it.
+template <int NT>
+__device__ __forceinline__ uint64_t block_sum_u64(uint64_t v, uint64_t* scratch) {
+    constexpr int NW = NT / 64;
+    const uint64_t w = wave_sum_u64(v);
+    if (lane_id() == 0) scratch[threadIdx.x >> 6] = w;
+    __syncthreads();
+    uint64_t tot = 0;
+#pragma unroll
+    for (int k = 0; k < NW; k++) tot += scratch[k];
+    __syncthreads();
+    return tot;
+}
+
 }  // namespace cca

@@ -164,6 +164,20 @@ uint64_t device_encode(cc_instance* I, const std::vector<CoderCall>& calls, cons
 }
 
 // ------------------------------------------------------------------------------------- decode
+namespace {
+// first[i]: the first zero-run expansion chunk of item i (nitems + 1 entries); returns the total.
+uint32_t chunk_table(const std::vector<ItemSpec>& items, uint32_t* first) {
+    uint64_t k = 0;
+    for (size_t i = 0; i < items.size(); i++) {
+        first[i] = (uint32_t)k;
+        k += rle_chunks(items[i].n, items[i].rl);
+    }
+    CCA_REQUIRE(k < (1ull << 31), CC_ERR_INVALID_ARG, "too many symbols in one decode job");
+    first[items.size()] = (uint32_t)k;
+    return (uint32_t)k;
+}
+}  // namespace
+
 void decode_prepare(cc_instance* I, DeviceCoder& D, const std::vector<ItemSpec>& items,
                     const std::vector<CallWords>& calls, int symSize, DecodeJob& job, hipStream_t st) {
     check_symbols(I, symSize);
@@ -184,12 +198,15 @@ void decode_prepare(cc_instance* I, DeviceCoder& D, const std::vector<ItemSpec>&
     job.o_items = align_up(sizeof(DecCall) * std::max<uint32_t>(ncalls, 1), 256);
     job.o_wseq = job.o_items + align_up(sizeof(DecItem) * std::max<uint32_t>(nitems, 1), 256);
     job.o_wfirst = job.o_wseq + align_up(4ull * std::max<uint32_t>(nwaves, 1), 256);
-    const size_t total = job.o_wfirst + 4ull * std::max<uint32_t>(nwaves, 1);
+    job.o_ifirst = job.o_wfirst + align_up(4ull * std::max<uint32_t>(nwaves, 1), 256);
+    const size_t total = job.o_ifirst + 4ull * (nitems + 1);
     unsigned char* h = (unsigned char*)D.h_dstage.get(total);
     DecCall* hc = (DecCall*)h;
     DecItem* hi = (DecItem*)(h + job.o_items);
     uint32_t* hws = (uint32_t*)(h + job.o_wseq);
     uint32_t* hwf = (uint32_t*)(h + job.o_wfirst);
+    job.nchunks = chunk_table(items, (uint32_t*)(h + job.o_ifirst));
+    job.o_sums = align_up(total, 256);
     for (uint32_t c = 0; c < ncalls; c++) hc[c] = DecCall{calls[c].words, calls[c].nwords, 0, 0};
     uint32_t seq = 0, wv = 0, call = 0;
     for (uint32_t i = 0; i < nitems; i++) {
@@ -217,7 +234,7 @@ void decode_prepare(cc_instance* I, DeviceCoder& D, const std::vector<ItemSpec>&
     job.arena = arena;
     job.symSize = symSize;
     job.p = params_of(I);
-    D.d_dcalls.get(total);
+    D.d_dcalls.get(job.o_sums + 4ull * std::max<uint32_t>(job.nchunks, 1));
     D.d_dseqs.as<DecSeq>(std::max<uint32_t>(nseq, 1));
     D.d_dstate.as<CoderState>(1);
     D.d_dout.as<DecCallOut>(std::max<uint32_t>(ncalls, 1));
@@ -231,7 +248,10 @@ void decode_launch(DeviceCoder& D, const DecodeJob& job, hipStream_t st) {
     coder_decode((const DecCall*)d, job.ncalls, (const DecItem*)(d + job.o_items), job.nitems, (DecSeq*)D.d_dseqs.p,
                  job.nseq, job.p, job.symSize, (uint32_t*)D.d_dscratch.p, job.arena, (CoderState*)D.d_dstate.p,
                  (DecCallOut*)D.d_dout.p, (const uint32_t*)(d + job.o_wseq), (const uint32_t*)(d + job.o_wfirst),
-                 job.nwaves, job.total_syms, st);
+                 job.nwaves, job.total_syms,
+                 RleChunks{(const uint32_t*)(d + job.o_ifirst), job.nchunks,
+                           (uint32_t*)((unsigned char*)D.d_dcalls.p + job.o_sums)},
+                 st);
 }
 
 void decode_finish(DeviceCoder& D, const DecodeJob& job, std::vector<DecCallOut>* results, hipStream_t st) {
@@ -319,8 +339,11 @@ void decode_prepare_bound(cc_instance* I, DeviceCoder& D, const DecodeIndex& X, 
     job.o_cbrick = job.o_bind + align_up(sizeof(DecBind) * std::max<uint32_t>(nseq, 1), 256);
     job.o_wseq = job.o_cbrick + align_up(4ull * std::max<uint32_t>(ncalls, 1), 256);
     job.o_wfirst = job.o_wseq + align_up(4ull * std::max<uint32_t>(nwaves, 1), 256);
-    const size_t total = job.o_wfirst + 4ull * std::max<uint32_t>(nwaves, 1);
+    job.o_ifirst = job.o_wfirst + align_up(4ull * std::max<uint32_t>(nwaves, 1), 256);
+    const size_t total = job.o_ifirst + 4ull * (nitems + 1);
     unsigned char* h = (unsigned char*)D.h_dstage.get(total);
+    job.nchunks = chunk_table(items, (uint32_t*)(h + job.o_ifirst));
+    job.o_sums = align_up(total, 256);
     DecCall* hc = (DecCall*)h;
     DecItem* hi = (DecItem*)(h + job.o_items);
     DecBind* hb = (DecBind*)(h + job.o_bind);
@@ -371,7 +394,7 @@ void decode_prepare_bound(cc_instance* I, DeviceCoder& D, const DecodeIndex& X, 
     job.rl_bytes = rl_bytes;
     job.symSize = symSize;
     job.p = params_of(I);
-    D.d_dcalls.get(total);
+    D.d_dcalls.get(job.o_sums + 4ull * std::max<uint32_t>(job.nchunks, 1));
     D.d_dseqs.as<DecSeq>(std::max<uint32_t>(nseq, 1));
     D.d_dstate.as<CoderState>(1);
     D.d_dout.as<DecCallOut>(std::max<uint32_t>(ncalls, 1));
@@ -387,7 +410,10 @@ void decode_launch_bound(DeviceCoder& D, const DecodeIndex& X, const BoundJob& j
                        job.ncalls, (const DecItem*)(d + job.o_items), job.nitems, (DecSeq*)D.d_dseqs.p, job.nseq,
                        job.p, job.symSize, (unsigned char*)D.d_dscratch.p, (CoderState*)D.d_dstate.p,
                        (DecCallOut*)D.d_dout.p, (const uint32_t*)(d + job.o_wseq),
-                       (const uint32_t*)(d + job.o_wfirst), job.nwaves, job.total_syms, st);
+                       (const uint32_t*)(d + job.o_wfirst), job.nwaves, job.total_syms,
+                       RleChunks{(const uint32_t*)(d + job.o_ifirst), job.nchunks,
+                                 (uint32_t*)((unsigned char*)D.d_dcalls.p + job.o_sums)},
+                       st);
 }
 
 void decode_calls_from_encode(DeviceCoder& D, const DecodeJob& job, const CallOut* d_enc, const uint32_t* base,

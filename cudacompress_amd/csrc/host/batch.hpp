@@ -52,6 +52,9 @@ struct DecodeJob {
     int symSize = 2;
     CoderParams p{};
     size_t o_items = 0, o_wseq = 0, o_wfirst = 0;
+    // zero-run expansion chunks: item_first table (uploaded) and chunk sums (device only)
+    uint32_t nchunks = 0;
+    size_t o_ifirst = 0, o_sums = 0;
 };
 // Calls may carry null words when decode_calls_from_encode fills them on the device.
 void decode_prepare(cc_instance* I, DeviceCoder& D, const std::vector<ItemSpec>& items,
@@ -83,6 +86,8 @@ struct BoundJob {
     int symSize = 2;
     CoderParams p{};
     size_t o_items = 0, o_bind = 0, o_cbrick = 0, o_wseq = 0, o_wfirst = 0;
+    uint32_t nchunks = 0;
+    size_t o_ifirst = 0, o_sums = 0;
 };
 // `items` (grouped by call) are the request's outputs; request call c decodes index call
 // index_calls[c] and must have that call's item shapes.
