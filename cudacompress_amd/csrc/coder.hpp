@@ -20,6 +20,13 @@ namespace cca {
 
 constexpr uint32_t kStatBins = 4096;    // LDS histogram bins of the streaming passes
 constexpr uint32_t kDecLutBits = 10;    // lookup-table bits of the block decoder (codes of up to that length)
+constexpr uint32_t kDecGroupBlocks = 256;  // coding blocks per block-decoder workgroup of a long sequence
+constexpr uint32_t kDecWaveBlocks = 64;    // a sequence of at most this many blocks: one wave
+// Block-decoder entries of a sequence of (at most) nblk coding blocks.
+inline bool dec_long(uint32_t nblk) { return nblk > kDecWaveBlocks; }
+inline uint32_t dec_entries(uint32_t nblk) {
+    return dec_long(nblk) ? (nblk + kDecGroupBlocks - 1) / kDecGroupBlocks : (nblk ? 1u : 0u);
+}
 constexpr uint32_t kCoderThreads = 256;
 constexpr uint32_t kPerThread = 8;
 constexpr uint32_t kChunk = kCoderThreads * kPerThread;  // symbols per workgroup step
@@ -87,8 +94,9 @@ struct CallOut {              // per coder call
 // A decode job: calls, each a word-aligned bitstream segment in device memory holding the items
 // of one encodeRLHuff/encodeHuff call in order.  parse (one thread per call: the section sizes
 // are in-band, so a call is walked sequentially) -> tables (one workgroup per coded sequence:
-// first/min/max per length, absolute block offsets) -> blocks (one wave per 64 coding blocks,
-// LDS lookup table) -> rle (one workgroup per RLE item: zero runs expanded into the output).
+// first/min/max per length, absolute block offsets) -> blocks (four waves per 256 coding blocks,
+// one LDS lookup table; the zero counts' chunk spans) -> rle (one workgroup per chunk of 2048
+// pairs: zero runs expanded into the output).
 struct DecItem {
     void* out;           // n decoded symbols
     uint32_t n;
@@ -178,8 +186,10 @@ void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uin
                   uint32_t ncalls, const uint32_t* d_call_first_item, uint32_t* d_out, uint64_t out_cap,
                   uint64_t total_symbols, hipStream_t st, const SideStreams* side,
                   const LayoutLink* link = nullptr, const uint32_t* design_grid = nullptr, uint32_t lane_k = 0);
-// wave_seq: for every decode wave the coded sequence it serves (upper bound of the block count:
-// RLE sequences have at most n pairs); wave_first: that wave's first block.
+// wave_seq: for every block-decoder workgroup the coded sequence it serves; wave_first: that
+// workgroup's first block.  The first nlong entries are workgroups of kDecGroupBlocks blocks of
+// the long sequences (dec_long of the block-count bound: RLE sequences have at most n pairs), the
+// rest one wave for each short sequence (first block 0).
 // Zero-run expansion in chunks of kRleChunk pairs, every chunk its own workgroup: item i of a
 // decode job has rle_chunks(n_i, rl_i) chunks, numbered from item_first[i] (nitems + 1 entries,
 // item_first[nitems] = nchunks); sums receives one word per chunk.
@@ -193,7 +203,8 @@ struct RleChunks {
 void coder_decode(const DecCall* d_calls, uint32_t ncalls, const DecItem* d_items, uint32_t nitems, DecSeq* d_seqs,
                   uint32_t nseq, const CoderParams& p, int symSize, uint32_t* scratch, uint64_t scratch_cap,
                   CoderState* d_state, DecCallOut* d_out, const uint32_t* d_wave_seq, const uint32_t* d_wave_first,
-                  uint32_t nwaves, uint64_t total_symbols, const RleChunks& rc, hipStream_t st);
+                  uint32_t nwaves, uint32_t nlong, uint64_t total_symbols, const RleChunks& rc,
+                  hipStream_t st);
 // Resident decode index: parse + tables of every call once (RLE sequences get no decode buffer),
 // per-call results in d_out.  scratch then holds the tables and block offsets for bound decodes.
 void coder_decode_index(const DecCall* d_calls, uint32_t ncalls, const DecItem* d_items, DecSeq* d_seqs, uint32_t nseq,
@@ -206,7 +217,7 @@ void coder_decode_bound(const DecSeq* d_idx_seqs, const uint32_t* idx_scratch, c
                         const DecItem* d_items, uint32_t nitems, DecSeq* d_seqs, uint32_t nseq, const CoderParams& p,
                         int symSize, unsigned char* rl_arena, CoderState* d_state, DecCallOut* d_out,
                         const uint32_t* d_wave_seq, const uint32_t* d_wave_first, uint32_t nwaves,
-                        uint64_t total_symbols, const RleChunks& rc, hipStream_t st);
+                        uint32_t nlong, uint64_t total_symbols, const RleChunks& rc, hipStream_t st);
 // calls[i].words = base + enc[i].word_begin, calls[i].nwords = enc[i].words (the rest unchanged);
 // an empty range for a failed call or one reaching past cap words
 void coder_calls_from_encode(DecCall* d_calls, const CallOut* d_enc, const uint32_t* base, uint64_t cap,

@@ -7,6 +7,8 @@
 // src/cudaCompress/HuffmanKernels.cui:197-371, and the zero-run expansion of
 // src/cudaCompress/RunLengthKernels.cui:57-76.  Every read is bounded by the call's word count,
 // so a corrupt stream yields an error status, never an out-of-bounds access.
+#include <type_traits>
+
 #include "../coder.hpp"
 #include "../profile.hpp"
 #include "huff_design.hpp"
@@ -253,21 +255,47 @@ __global__ __launch_bounds__(256) void k_dec_tables(DecSeq* seqs, CoderParams p,
     }
 }
 
-// One wave per 64 coding blocks of a sequence.
+// Block decoder: one thread per coding block, W waves per workgroup (W = 4: 256 blocks of one
+// long sequence sharing one LDS copy of its lookup table, which then leaves L2 once per 256
+// blocks; W = 1: a short sequence's blocks, so that a sequence of a few blocks does not hold a
+// four-wave workgroup's LDS).  A lane decodes its block kDecRound symbols per round and stores
+// them itself (16 bytes per 8 16-bit symbols); the waves of a workgroup meet only at the table load.
+// The kernel is bound by instruction issue and latency, so a code costs a handful of vector
+// instructions: the bit window is two words {hi, lo} read with one funnel shift (`alignbit`, the
+// next code starting q bits above lo's lowest bit); codewords reach a lane through an LDS copy of
+// the kWinWords words from the 16-byte boundary at or below its next word, loaded at the start of
+// every round with three 16-byte loads and one wait (a round takes at most one word per code), so
+// refilling the bit window is a branch-free select; the round is unrolled, its symbols collect in
+// registers at fixed positions; and the zero-count sums run only in the instance for the zero
+// counts of RLE items.
+constexpr uint32_t kDecRound = 8;
+constexpr uint32_t kWinWords = 12;  // >= kDecRound + 3 (the aligned start is up to 3 words early), 16-byte rows
+// row of a lane's run-path symbols in LDS: an odd number of words, so lanes writing the same
+// position use distinct banks
 template <class S>
-__global__ __launch_bounds__(64) void k_dec_blocks(const DecSeq* seqs, const uint32_t* wave_seq,
-                                                   const uint32_t* wave_first, CoderParams p,
-                                                   const uint32_t* scratch) {
+constexpr uint32_t run_row_words() { return sizeof(S) == 2 ? kDecRound / 2 + 1 : kDecRound + 1; }
+
+template <class S, int W>
+__global__ __launch_bounds__(64 * W) void k_dec_blocks(const DecSeq* seqs, const uint32_t* wave_seq,
+                                                       const uint32_t* wave_first, CoderParams p,
+                                                       const uint32_t* scratch, const DecItem* items, RleChunks rc) {
+    constexpr uint32_t RW = run_row_words<S>();
+    constexpr uint32_t NW = kDecRound * sizeof(S) / 4;  // words of a round's symbols
     __shared__ uint32_t lut[1 << kLut];
     __shared__ int tab[96];
-    __shared__ S stage[64][33];  // odd row length: the per-lane column writes spread over the banks
-    const DecSeq& s = seqs[wave_seq[blockIdx.x]];
+    __shared__ S run_all[W][64 * RW * 4 / sizeof(S)];
+    __shared__ __align__(16) uint32_t win_all[W][64 * kWinWords];
+    // the workgroups of one sequence (consecutive in the list) on one XCD: its tables leave HBM once
+    const uint32_t wv = xcd_order(blockIdx.x, gridDim.x);
+    const uint32_t sq = wave_seq[wv];
+    const DecSeq& s = seqs[sq];
     if (s.err) return;
     const uint32_t cbs = p.cbs;
-    const uint32_t nblk = (s.count + cbs - 1) / cbs;
-    const uint32_t fb = wave_first[blockIdx.x];
-    if (fb >= nblk) return;
-    const int lane = threadIdx.x;
+    const uint32_t count = s.count;
+    const uint32_t nblk = (count + cbs - 1) / cbs;
+    const uint32_t fb0 = wave_first[wv];
+    if (fb0 >= nblk) return;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     S* out = static_cast<S*>(s.out);
     const bool multi = s.nlen > 0;
     const uint32_t nsym = s.nsym;
@@ -276,119 +304,195 @@ __global__ __launch_bounds__(64) void k_dec_blocks(const DecSeq* seqs, const uin
     // lookup table: 2^lb entries, indexed by the next lb bits (lb = min(longest code, kLut))
     const uint32_t lb = s.nlen < kLut ? s.nlen : kLut;
     if (multi) {
-        // all of the lane's table loads in flight at once, then the LDS stores
+        // all of the thread's table loads in flight at once, then the LDS stores
         const uint32_t* glut = scratch + s.lut_off;
-        constexpr int kPer = (1 << kLut) / 64;
+        constexpr int kPer = (1 << kLut) / (64 * W);
         const uint32_t nl = 1u << lb;
         uint32_t lv[kPer];
 #pragma unroll
-        for (int k = 0; k < kPer; k++) lv[k] = (uint32_t)(lane + 64 * k) < nl ? glut[lane + 64 * k] : 0u;
-        const int t0 = s.tab[lane], t1 = lane < 32 ? s.tab[64 + lane] : 0;
+        for (int k = 0; k < kPer; k++) lv[k] = (uint32_t)(t + 64 * W * k) < nl ? glut[t + 64 * W * k] : 0u;
+        int tv[(96 + 64 * W - 1) / (64 * W)];
+#pragma unroll
+        for (int k = 0; k < (96 + 64 * W - 1) / (64 * W); k++) tv[k] = t + 64 * W * k < 96 ? s.tab[t + 64 * W * k] : 0;
 #pragma unroll
         for (int k = 0; k < kPer; k++)
-            if ((uint32_t)(lane + 64 * k) < nl) lut[lane + 64 * k] = lv[k];
-        tab[lane] = t0;
-        if (lane < 32) tab[64 + lane] = t1;
-        __syncthreads();
+            if ((uint32_t)(t + 64 * W * k) < nl) lut[t + 64 * W * k] = lv[k];
+#pragma unroll
+        for (int k = 0; k < (96 + 64 * W - 1) / (64 * W); k++)
+            if (t + 64 * W * k < 96) tab[t + 64 * W * k] = tv[k];
     }
+    __syncthreads();
+    const uint32_t fb = fb0 + 64u * (uint32_t)wave;
+    if (fb >= nblk) return;  // no block for this wave
+    S* run_row = &run_all[wave][lane * RW * 4 / sizeof(S)];
+    // the run row and the codeword copy are also accessed as words / 16-byte vectors
+    typedef uint32_t __attribute__((may_alias)) u32a;
+    typedef uint32_t u32x4a __attribute__((ext_vector_type(4), may_alias));
+    uint32_t* win = &win_all[wave][lane * kWinWords];
     const uint32_t single = nsym >= 1 ? symbol_at(0) : 0u;
     const uint32_t b = fb + lane;
     const bool valid = b < nblk;
+    // the zero counts of an RLE item also give the zero-run expansion its chunk spans: the sum of
+    // (count + 1) over each chunk of kRleChunk pairs (kRleChunk / cbs whole blocks, a power of two
+    // dividing the wave's 64 blocks)
+    const DecItem zit = rc.nchunks ? items[s.item] : DecItem{};
+    const bool zsum = rc.nchunks != 0 && zit.rl && zit.seq + 1 == sq;
     const uint32_t* words = s.base + s.words_off;
     const uint64_t avail = s.nwords - s.words_off;  // readable words from the codeword start
-    auto word = [&](uint64_t i) -> uint32_t { return i < avail ? words[i] : 0u; };
-    // bit window: `cache` holds 64 bits starting at word `nxt - 2`; a refill shifts in the next word
-    // (an L1 hit after the block's first lines).  A separate prefetch ring does not help here:
-    // the load counter is per wave and the lanes refill at different steps, so any later use of a
-    // prefetched word waits for the wave's latest load.
-    uint64_t cache = 0;
-    uint32_t bp = 0;
-    uint64_t nxt = 0;
+    auto word = [&](uint32_t i) -> uint32_t { return i < avail ? words[i] : 0u; };
+    // window {hi, lo} (the next code: alignbit(hi, lo, q), 32 + q bits unread, q in [0, 31] after a
+    // refill), word nxt next; the LDS copy holds words [w0, w0 + kWinWords) of the block's stream
+    uint32_t hi = 0, lo = 0, nxt = 0;
+    int q = 0, w0 = 0;
     if (valid && multi) {
         const uint32_t pos = b == 0 ? 0u : scratch[s.boff_off + b];
-        const uint64_t wi = pos >> 5;
-        cache = ((uint64_t)word(wi) << 32) | (uint64_t)word(wi + 1);
-        bp = pos & 31u;
-        nxt = wi + 2;
+        const uint32_t wi = pos >> 5, b0 = pos & 31u;
+        if (b0 == 0) {
+            lo = word(wi);
+            nxt = wi + 1;
+        } else {
+            hi = word(wi);
+            lo = word(wi + 1);
+            nxt = wi + 2;
+            q = 32 - (int)b0;
+        }
     }
+    // words from the 16-byte boundary at or below word nxt into the LDS copy.  The call's buffer
+    // holds words [-words_off, avail); a start below it stays in the 16-byte block of the buffer's
+    // first word.  Where the copy would reach past the buffer's end, words are read one by one.
+    const uint32_t* cend = s.base + s.nwords;
+    auto load_window = [&]() {
+        const uint32_t* gp = reinterpret_cast<const uint32_t*>(reinterpret_cast<uintptr_t>(words + nxt) & ~uintptr_t(15));
+        w0 = (int)nxt - (int)(words + nxt - gp);
+        const bool fast = !(valid && multi) || gp + kWinWords <= cend;
+        if (__all(fast)) {
+            if (valid && multi) {
+                // a global-space pointer: the loads cannot alias the LDS stores, so all of them
+                // are issued before the first store waits
+                using G4 = const __attribute__((address_space(1))) u32x4a*;
+                const G4 g4 = (G4)(const void*)gp;
+                u32x4a v[kWinWords / 4];
+#pragma unroll
+                for (uint32_t k = 0; k < kWinWords / 4; k++) v[k] = g4[k];
+#pragma unroll
+                for (uint32_t k = 0; k < kWinWords / 4; k++) reinterpret_cast<u32x4a*>(win)[k] = v[k];
+            }
+        } else if (valid && multi) {
+#pragma unroll
+            for (uint32_t k = 0; k < kWinWords; k++) {
+                const int i = w0 + (int)k;
+                win[k] = (i >= 0 && (uint64_t)i < avail) ? words[i] : 0u;
+            }
+        }
+    };
+    auto refill = [&]() {
+        const uint32_t w = win[nxt - (uint32_t)w0];
+        const bool need = q < 0;
+        hi = need ? lo : hi;
+        lo = need ? w : lo;
+        nxt += need ? 1u : 0u;
+        q &= 31;
+    };
+    const uint32_t sh = 32 - lb;
     const uint32_t mlen = s.nlen;
+    // a code longer than the lookup table: the per-length first/min codes; (len << 24) | symbol
+    auto slow = [&](uint32_t pk) -> uint32_t {
+        uint32_t len = kLut + 1;
+        while (len < mlen && (int)(pk >> (32 - len)) > tab[64 + len - 1]) len++;
+        const int cw = (int)(pk >> (32 - len));
+        return (len << 24) | symbol_at(tab[len - 1] + cw - tab[32 + len - 1]);
+    };
     // one codeword of length 1 (then it is "0" and every other codeword starts with a 1): a run of
-    // k zero bits is k copies of its symbol, counted at once (the zero-count sequences, where a
-    // zero count is nearly every pair, decode a few steps per 32 symbols instead of 32)
+    // k zero bits is k copies of its symbol, taken at once (the zero-count sequences, where a zero
+    // count is nearly every pair, decode a few steps per round instead of kDecRound)
     const bool run0 = multi && s.tab[64] == 0 && s.tab[32] == 0;
     const uint32_t sym0 = run0 ? symbol_at(s.tab[0]) : 0u;
-    auto decode_one = [&](uint32_t peek, uint32_t& len) -> uint32_t {
-        const uint32_t e = lut[peek >> (32 - lb)];
-        len = e >> 24;
-        uint32_t sym = e & 0xFFFFFFu;
-        if (e == 0u) {
-            len = kLut + 1;
-            while (len < mlen && (int)(peek >> (32 - len)) > tab[64 + len - 1]) len++;
-            const int cw = (int)(peek >> (32 - len));
-            sym = symbol_at(tab[len - 1] + cw - tab[32 + len - 1]);
-        }
-        return sym;
-    };
-    for (uint32_t r = 0; r < cbs; r += 32) {
-        // every lane steps through the whole chunk; steps past its block's end store nothing
-        if (run0) {
-            for (uint32_t i = 0; i < 32; i++) stage[lane][i] = (S)sym0;
-            for (uint32_t i = 0; i < 32;) {
-                if (bp >= 32) {
-                    cache = (cache << 32) | (uint64_t)word(nxt);
-                    nxt++;
-                    bp -= 32;
-                }
-                const uint32_t peek = (uint32_t)(cache >> (32 - bp));
-                if ((peek >> 31) == 0u) {
-                    const uint32_t k = min(peek ? (uint32_t)__clz(peek) : 32u, 32u - i);
-                    i += k;
-                    bp += k;
-                } else {
-                    uint32_t len;
-                    stage[lane][i] = (S)decode_one(peek, len);
-                    i++;
-                    bp += len;
-                }
-            }
+    const uint32_t bend_l = valid ? min((b + 1) * cbs, count) : 0u;  // this lane's block end
+    // a lane's round leaves as 16-byte stores when the output is 16-byte aligned (round starts are
+    // then aligned: cbs and kDecRound are multiples of 8), symbol by symbol otherwise and at the
+    // sequence's end
+    const bool wide = (reinterpret_cast<uintptr_t>(out) & 15u) == 0;
+    auto store_round = [&](uint32_t p0, uint32_t vlen, const uint32_t (&v)[NW]) {
+        if (wide && vlen == kDecRound) {
+#pragma unroll
+            for (uint32_t k = 0; k < NW; k += 4)
+                *reinterpret_cast<uint4*>(out + p0 + k * 4 / sizeof(S)) = make_uint4(v[k], v[k + 1], v[k + 2], v[k + 3]);
         } else {
-#pragma unroll 2
-            for (uint32_t i = 0; i < 32; i++) {
-                uint32_t sym = single;
-                if (multi) {
-                    if (bp >= 32) {
-                        cache = (cache << 32) | (uint64_t)word(nxt);
-                        nxt++;
-                        bp -= 32;
+#pragma unroll
+            for (uint32_t i = 0; i < kDecRound; i++)
+                if (i < vlen) out[p0 + i] = sizeof(S) == 2 ? (S)(v[i >> 1] >> (16 * (i & 1))) : (S)v[i];
+        }
+    };
+    // symbol i of a round into the round's words (fixed positions: the round is unrolled)
+    auto put = [](uint32_t (&v)[NW], uint32_t i, uint32_t sym) {
+        if (sizeof(S) == 2) v[i >> 1] = (i & 1) ? (v[i >> 1] | (sym << 16)) : (sym & 0xFFFFu);
+        else v[i] = sym;
+    };
+    auto rounds = [&](auto zflag) -> uint32_t {
+        constexpr bool Z = decltype(zflag)::value;
+        uint32_t zs = 0;
+        for (uint32_t r = 0; r < cbs; r += kDecRound) {
+            // every lane steps through the whole round; steps past its block's end store nothing
+            const uint32_t p0 = b * cbs + r;
+            const uint32_t vlen = bend_l > p0 ? min(bend_l - p0, kDecRound) : 0u;  // symbols of the block here
+            uint32_t v[NW];
+            if (!multi) {
+#pragma unroll
+                for (uint32_t i = 0; i < kDecRound; i++) put(v, i, single);
+                if constexpr (Z) zs += (single + 1) * vlen;
+            } else {
+                load_window();
+                if (run0) {
+                    // positions taken at a lane's own pace: its LDS row, prefilled with the run symbol
+                    const uint32_t fill = sizeof(S) == 2 ? (sym0 & 0xFFFFu) * 0x10001u : sym0;
+#pragma unroll
+                    for (uint32_t k = 0; k < NW; k++) reinterpret_cast<u32a*>(run_row)[k] = fill;
+                    uint32_t i = 0;
+                    while (i < kDecRound) {
+                        refill();
+                        const uint32_t pk = __builtin_amdgcn_alignbit(hi, lo, (uint32_t)q);
+                        const bool run = (pk >> 31) == 0u;
+                        uint32_t e = lut[pk >> sh];
+                        if (!run && e == 0u) e = slow(pk);
+                        const uint32_t k = min(pk ? (uint32_t)__clz(pk) : 32u, kDecRound - i);
+                        const uint32_t sym = e & 0xFFFFFFu;
+                        if (!run) run_row[i] = (S)sym;
+                        if constexpr (Z)
+                            zs += run ? (sym0 + 1) * (min(i + k, vlen) - min(i, vlen)) : (i < vlen ? sym + 1 : 0u);
+                        q -= (int)(run ? k : (e >> 24));
+                        i += run ? k : 1u;
                     }
-                    const uint32_t peek = (uint32_t)(cache >> (32 - bp));
-                    uint32_t len;
-                    sym = decode_one(peek, len);
-                    bp += len;
+#pragma unroll
+                    for (uint32_t k = 0; k < NW; k++) v[k] = reinterpret_cast<const u32a*>(run_row)[k];
+                } else {
+#pragma unroll
+                    for (uint32_t i = 0; i < kDecRound; i++) {
+                        refill();
+                        const uint32_t pk = __builtin_amdgcn_alignbit(hi, lo, (uint32_t)q);
+                        uint32_t e = lut[pk >> sh];
+                        if (e == 0u) e = slow(pk);
+                        q -= (int)(e >> 24);
+                        const uint32_t sym = e & 0xFFFFFFu;
+                        put(v, i, sym);
+                        if constexpr (Z) zs += i < vlen ? sym + 1 : 0u;
+                    }
                 }
-                stage[lane][i] = (S)sym;
             }
+            store_round(p0, vlen, v);
         }
-        __syncthreads();
-        for (uint32_t idx = lane; idx < 64 * 32; idx += 64) {
-            const uint32_t blk = idx >> 5, k = idx & 31u;
-            const uint32_t gb = fb + blk;
-            if (gb < nblk) {
-                const uint32_t pos = gb * cbs + r + k;
-                const uint32_t bend = min((gb + 1) * cbs, s.count);
-                if (pos < bend) out[pos] = stage[blk][k];
-            }
-        }
-        __syncthreads();
+        return zs;
+    };
+    if (zsum) {
+        uint32_t zs = rounds(std::true_type{});
+        const uint32_t bpc = kRleChunk / cbs;
+        for (uint32_t o = 1; o < bpc; o <<= 1) zs += (uint32_t)__shfl_xor((int)zs, (int)o);
+        const uint32_t c = b / bpc;
+        if ((lane & (bpc - 1)) == 0 && c < (zit.n + kRleChunk - 1) / kRleChunk) rc.sums[rc.item_first[s.item] + c] = zs;
+    } else {
+        (void)rounds(std::false_type{});
     }
 }
 
-// One workgroup per RLE item.  Pairs are taken in chunks of 2048 (8 consecutive per thread for
-// the scan); pair i (value v_i after z_i zeros) lands at sum_{j<i}(z_j + 1) + z_i.  Each chunk
-// first clears its own span of output positions with 16-byte coalesced stores (to the end of the
-// output for the last chunk: the trailing zeros), then stores its values there, staged in LDS and
-// issued with a 256-pair stride so that one store instruction writes 64 consecutive pairs.  The
-// span's lines are still in L2 when the values land, so each output line leaves L2 once.
 // Eight consecutive symbols from a 16-byte aligned address.
 __device__ __forceinline__ void load8_aligned(const uint16_t* p, uint32_t (&o)[8]) {
     const uint4 w = *reinterpret_cast<const uint4*>(p);
@@ -418,6 +522,7 @@ __device__ __forceinline__ void clear_symbols(S* out, uint32_t n) {
     for (uint32_t i = head + nv * per + threadIdx.x; i < n; i += blockDim.x) out[i] = (S)0;
 }
 
+// The item of zero-run expansion chunk b: the last i with first[i] <= b (first: nitems + 1 entries).
 __device__ __forceinline__ uint32_t chunk_item(const uint32_t* first, uint32_t nitems, uint32_t b) {
     uint32_t lo = 0, hi = nitems;  // first[lo] <= b < first[hi]
     while (hi - lo > 1) {
@@ -428,36 +533,9 @@ __device__ __forceinline__ uint32_t chunk_item(const uint32_t* first, uint32_t n
     return lo;
 }
 
-// The first pass of the zero-run expansion: chunk b's output span, the sum of (zero count + 1)
-// over its pairs (0 past the sequence's pairs).
-template <class S>
-__global__ __launch_bounds__(256) void k_dec_rle_sums(const DecItem* items, uint32_t nitems, const DecSeq* seqs,
-                                                      RleChunks rc) {
-    __shared__ uint32_t sh[4];
-    const uint32_t b = blockIdx.x;
-    const uint32_t i = chunk_item(rc.item_first, nitems, b);
-    const DecItem it = items[i];
-    const DecSeq& ss = seqs[it.seq];
-    const DecSeq& sz = seqs[it.seq + 1];
-    const uint32_t M = (ss.err || sz.err) ? 0u : ss.count;
-    const S* zc = static_cast<const S*>(sz.out);
-    const uint32_t i0 = (b - rc.item_first[i]) * kRleChunk + threadIdx.x * 8;
-    uint32_t z[8], sum = 0;
-    if (((reinterpret_cast<uintptr_t>(zc) & 15u) == 0) && i0 + 8 <= M) {
-        load8_aligned(zc + i0, z);
-#pragma unroll
-        for (int k = 0; k < 8; k++) sum += z[k] + 1;
-    } else {
-#pragma unroll
-        for (int k = 0; k < 8; k++) sum += i0 + k < M ? (uint32_t)zc[i0 + k] + 1 : 0u;
-    }
-    uint32_t tot;
-    (void)block_excl_scan<256>(sum, sh, tot);
-    if (threadIdx.x == 0) rc.sums[b] = tot;
-}
-
 // One workgroup per chunk of kRleChunk pairs.  Pair i (value v_i after z_i zeros) lands at
-// sum_{j<i}(z_j + 1) + z_i: the chunk starts at the sum of the earlier chunks' spans and scans
+// sum_{j<i}(z_j + 1) + z_i: the chunk starts at the sum of the earlier chunks' spans (each written
+// by the block decode of the item's zero counts) and scans
 // its own pairs (8 consecutive per thread).  It clears its span of output positions with 16-byte
 // coalesced stores (to the end of the output for the sequence's last chunk: the trailing zeros),
 // then stores its values there, staged in LDS and issued with a 256-pair stride so that one store
@@ -546,8 +624,14 @@ __global__ __launch_bounds__(256) void k_dec_rle(const DecItem* items, uint32_t 
 // request's buffer at rl_off).  Calls start from the index's parse result of their brick.
 __global__ void k_dec_bind(const DecSeq* idx, const DecBind* bind, uint32_t nseq, const DecItem* items,
                            DecSeq* seqs, unsigned char* rl_arena, const DecCallOut* idx_out,
-                           const uint32_t* call_brick, uint32_t ncalls, DecCallOut* outc) {
+                           const uint32_t* call_brick, uint32_t ncalls, DecCallOut* outc, CoderState* st) {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r == 0) {
+        st->scratch_used = 0;
+        st->out_words = 0;
+        st->err = 0;
+        st->pad = 0;
+    }
     if (r < nseq) {
         const DecBind b = bind[r];
         DecSeq s = idx[b.idx_seq];
@@ -558,18 +642,25 @@ __global__ void k_dec_bind(const DecSeq* idx, const DecBind* bind, uint32_t nseq
     if (r < ncalls) outc[r] = idx_out[call_brick[r]];
 }
 
-// Per-call status: the first error among the call's coded sequences.
-__global__ void k_dec_finish(const DecCall* calls, uint32_t ncalls, const DecItem* items, const DecSeq* seqs,
-                             DecCallOut* outc) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+// Per-call status: the first error among the call's coded sequences, one wave per call, its
+// lanes over the call's items.
+__global__ __launch_bounds__(64) void k_dec_finish(const DecCall* calls, uint32_t ncalls, const DecItem* items,
+                                                   const DecSeq* seqs, DecCallOut* outc) {
+    const uint32_t c = blockIdx.x;
     if (c >= ncalls || outc[c].err) return;
     const DecCall C = calls[c];
-    for (uint32_t k = C.first_item; k < C.first_item + C.nitems; k++) {
-        const DecItem it = items[k];
-        uint32_t e = seqs[it.seq].err;
-        if (!e && it.rl) e = seqs[it.seq + 1].err;
-        if (e) {
-            outc[c].err = e;
+    for (uint32_t k0 = 0; k0 < C.nitems; k0 += 64) {
+        const uint32_t k = k0 + threadIdx.x;
+        uint32_t e = 0;
+        if (k < C.nitems) {
+            const DecItem it = items[C.first_item + k];
+            e = seqs[it.seq].err;
+            if (!e && it.rl) e = seqs[it.seq + 1].err;
+        }
+        const uint64_t m = __ballot(e != 0);
+        if (m) {
+            const int first = __ffsll((unsigned long long)m) - 1;
+            if ((int)threadIdx.x == first) outc[c].err = e;
             return;
         }
     }
@@ -598,13 +689,28 @@ void coder_calls_from_encode(DecCall* d_calls, const CallOut* d_enc, const uint3
 }
 
 namespace {
-// The zero-run expansion: chunk spans, then the chunks.
+// The block decode: the long sequences' four-wave workgroups, then the short ones' single waves.
+template <class S>
+void launch_blocks(DecSeq* d_seqs, const uint32_t* d_wave_seq, const uint32_t* d_wave_first, uint32_t nwaves,
+                   uint32_t nlong, const CoderParams& p, const uint32_t* scratch, const DecItem* d_items,
+                   const RleChunks& rc, uint64_t outb, hipStream_t st) {
+    if (nlong) {
+        CCA_LAUNCH("dec_blocks", outb, (k_dec_blocks<S, 4>), dim3(nlong), dim3(256), 0, st, d_seqs, d_wave_seq,
+                   d_wave_first, p, scratch, d_items, rc);
+        CCA_LAUNCH_CHECK();
+    }
+    if (nwaves > nlong) {
+        CCA_LAUNCH("dec_blocks_short", 0, (k_dec_blocks<S, 1>), dim3(nwaves - nlong), dim3(64), 0, st, d_seqs,
+                   d_wave_seq + nlong, d_wave_first + nlong, p, scratch, d_items, rc);
+        CCA_LAUNCH_CHECK();
+    }
+}
+
+// The zero-run expansion (its chunk spans come from the block decode of the zero counts).
 template <class S>
 void launch_rle(const DecItem* d_items, uint32_t nitems, DecSeq* d_seqs, const RleChunks& rc, CoderState* d_state,
                 uint64_t outb, hipStream_t st) {
     if (rc.nchunks == 0) return;
-    CCA_LAUNCH("dec_rle_sums", 0, k_dec_rle_sums<S>, dim3(rc.nchunks), dim3(256), 0, st, d_items, nitems, d_seqs, rc);
-    CCA_LAUNCH_CHECK();
     CCA_LAUNCH("dec_rle", outb, k_dec_rle<S>, dim3(rc.nchunks), dim3(256), 0, st, d_items, nitems, d_seqs, rc, d_state);
     CCA_LAUNCH_CHECK();
 }
@@ -613,7 +719,8 @@ void launch_rle(const DecItem* d_items, uint32_t nitems, DecSeq* d_seqs, const R
 void coder_decode(const DecCall* d_calls, uint32_t ncalls, const DecItem* d_items, uint32_t nitems, DecSeq* d_seqs,
                   uint32_t nseq, const CoderParams& p, int symSize, uint32_t* scratch, uint64_t scratch_cap,
                   CoderState* d_state, DecCallOut* d_out, const uint32_t* d_wave_seq, const uint32_t* d_wave_first,
-                  uint32_t nwaves, uint64_t total_symbols, const RleChunks& rc, hipStream_t st) {
+                  uint32_t nwaves, uint32_t nlong, uint64_t total_symbols, const RleChunks& rc,
+                  hipStream_t st) {
     CCA_REQUIRE(symSize == 2 || symSize == 4, CC_ERR_INVALID_ARG, "symbol size must be 2 or 4");
     CCA_LAUNCH("dec_init", 32, k_dec_init, dim3(1), dim3(64), 0, st, d_state);
     CCA_LAUNCH_CHECK();
@@ -625,33 +732,26 @@ void coder_decode(const DecCall* d_calls, uint32_t ncalls, const DecItem* d_item
     CCA_LAUNCH("dec_tables", 0, k_dec_tables, dim3(nseq), dim3(256), 0, st, d_seqs, p, scratch, d_state);
     CCA_LAUNCH_CHECK();
     const uint64_t outb = total_symbols * symSize;
-    if (nwaves) {
-        if (symSize == 2)
-            CCA_LAUNCH("dec_blocks", outb, k_dec_blocks<uint16_t>, dim3(nwaves), dim3(64), 0, st, d_seqs, d_wave_seq,
-                       d_wave_first, p, scratch);
-        else
-            CCA_LAUNCH("dec_blocks", outb, k_dec_blocks<uint32_t>, dim3(nwaves), dim3(64), 0, st, d_seqs, d_wave_seq,
-                       d_wave_first, p, scratch);
-        CCA_LAUNCH_CHECK();
-    }
+    if (symSize == 2)
+        launch_blocks<uint16_t>(d_seqs, d_wave_seq, d_wave_first, nwaves, nlong, p, scratch, d_items, rc, outb, st);
+    else
+        launch_blocks<uint32_t>(d_seqs, d_wave_seq, d_wave_first, nwaves, nlong, p, scratch, d_items, rc, outb, st);
     if (symSize == 2) launch_rle<uint16_t>(d_items, nitems, d_seqs, rc, d_state, outb, st);
     else launch_rle<uint32_t>(d_items, nitems, d_seqs, rc, d_state, outb, st);
-    CCA_LAUNCH("dec_finish", 0, k_dec_finish, dim3(div_up(ncalls, 64)), dim3(64), 0, st, d_calls, ncalls, d_items,
-               d_seqs, d_out);
-    CCA_LAUNCH_CHECK();
+    if (ncalls) {
+        CCA_LAUNCH("dec_finish", 0, k_dec_finish, dim3(ncalls), dim3(64), 0, st, d_calls, ncalls, d_items, d_seqs,
+                   d_out);
+        CCA_LAUNCH_CHECK();
+    }
 }
 
 namespace {
 template <class S>
 void launch_blocks_rle(const DecItem* d_items, uint32_t nitems, DecSeq* d_seqs, const CoderParams& p,
                        const uint32_t* scratch, CoderState* d_state, const uint32_t* d_wave_seq,
-                       const uint32_t* d_wave_first, uint32_t nwaves, uint64_t outb, const RleChunks& rc,
-                       hipStream_t st) {
-    if (nwaves) {
-        CCA_LAUNCH("dec_blocks", outb, k_dec_blocks<S>, dim3(nwaves), dim3(64), 0, st, d_seqs, d_wave_seq, d_wave_first,
-                   p, scratch);
-        CCA_LAUNCH_CHECK();
-    }
+                       const uint32_t* d_wave_first, uint32_t nwaves, uint32_t nlong, uint64_t outb,
+                       const RleChunks& rc, hipStream_t st) {
+    launch_blocks<S>(d_seqs, d_wave_seq, d_wave_first, nwaves, nlong, p, scratch, d_items, rc, outb, st);
     launch_rle<S>(d_items, nitems, d_seqs, rc, d_state, outb, st);
 }
 }  // namespace
@@ -669,9 +769,11 @@ void coder_decode_index(const DecCall* d_calls, uint32_t ncalls, const DecItem* 
     if (nseq == 0) return;
     CCA_LAUNCH("dec_tables", 0, k_dec_tables, dim3(nseq), dim3(256), 0, st, d_seqs, p, scratch, d_state);
     CCA_LAUNCH_CHECK();
-    CCA_LAUNCH("dec_finish", 0, k_dec_finish, dim3(div_up(ncalls, 64)), dim3(64), 0, st, d_calls, ncalls, d_items,
-               d_seqs, d_out);
-    CCA_LAUNCH_CHECK();
+    if (ncalls) {
+        CCA_LAUNCH("dec_finish", 0, k_dec_finish, dim3(ncalls), dim3(64), 0, st, d_calls, ncalls, d_items, d_seqs,
+                   d_out);
+        CCA_LAUNCH_CHECK();
+    }
 }
 
 void coder_decode_bound(const DecSeq* d_idx_seqs, const uint32_t* idx_scratch, const DecCallOut* d_idx_out,
@@ -679,25 +781,30 @@ void coder_decode_bound(const DecSeq* d_idx_seqs, const uint32_t* idx_scratch, c
                         const DecItem* d_items, uint32_t nitems, DecSeq* d_seqs, uint32_t nseq, const CoderParams& p,
                         int symSize, unsigned char* rl_arena, CoderState* d_state, DecCallOut* d_out,
                         const uint32_t* d_wave_seq, const uint32_t* d_wave_first, uint32_t nwaves,
-                        uint64_t total_symbols, const RleChunks& rc, hipStream_t st) {
+                        uint32_t nlong, uint64_t total_symbols, const RleChunks& rc, hipStream_t st) {
     CCA_REQUIRE(symSize == 2 || symSize == 4, CC_ERR_INVALID_ARG, "symbol size must be 2 or 4");
-    CCA_LAUNCH("dec_init", 32, k_dec_init, dim3(1), dim3(64), 0, st, d_state);
-    CCA_LAUNCH_CHECK();
     const uint32_t nb = std::max(nseq, ncalls);
-    if (nb == 0) return;
+    if (nb == 0) {
+        CCA_LAUNCH("dec_init", 32, k_dec_init, dim3(1), dim3(64), 0, st, d_state);
+        CCA_LAUNCH_CHECK();
+        return;
+    }
+    // the job state is reset by the bind (its first thread)
     CCA_LAUNCH("dec_bind", (uint64_t)nseq * 2 * sizeof(DecSeq), k_dec_bind, dim3(div_up(nb, 64)), dim3(64), 0, st,
-               d_idx_seqs, d_bind, nseq, d_items, d_seqs, rl_arena, d_idx_out, d_call_brick, ncalls, d_out);
+               d_idx_seqs, d_bind, nseq, d_items, d_seqs, rl_arena, d_idx_out, d_call_brick, ncalls, d_out, d_state);
     CCA_LAUNCH_CHECK();
     const uint64_t outb = total_symbols * symSize;
     if (symSize == 2)
         launch_blocks_rle<uint16_t>(d_items, nitems, d_seqs, p, idx_scratch, d_state, d_wave_seq, d_wave_first, nwaves,
-                                    outb, rc, st);
+                                    nlong, outb, rc, st);
     else
         launch_blocks_rle<uint32_t>(d_items, nitems, d_seqs, p, idx_scratch, d_state, d_wave_seq, d_wave_first, nwaves,
-                                    outb, rc, st);
-    CCA_LAUNCH("dec_finish", 0, k_dec_finish, dim3(div_up(ncalls, 64)), dim3(64), 0, st, d_calls, ncalls, d_items,
-               d_seqs, d_out);
-    CCA_LAUNCH_CHECK();
+                                    nlong, outb, rc, st);
+    if (ncalls) {
+        CCA_LAUNCH("dec_finish", 0, k_dec_finish, dim3(ncalls), dim3(64), 0, st, d_calls, ncalls, d_items, d_seqs,
+                   d_out);
+        CCA_LAUNCH_CHECK();
+    }
 }
 
 }  // namespace cca

@@ -32,7 +32,8 @@ struct NoAllocArgs {
 };
 
 // Writes the job tables of one call of nstreams equal items: item s decodes sequence s*q (+1
-// for its zero counts, q = 2 for RLE items), each sequence's blocks in waves of 64, its zero-run
+// for its zero counts, q = 2 for RLE items), each sequence's blocks in workgroups of
+// kDecGroupBlocks (a short sequence: one wave), its zero-run
 // expansion chunks from s * chunks_per_item.
 __global__ void k_noalloc_setup(NoAllocArgs a) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -43,7 +44,7 @@ __global__ void k_noalloc_setup(NoAllocArgs a) {
     const uint32_t nwaves = a.nstreams * q * a.waves_per_seq;
     if (t < nwaves) {
         a.wave_seq[t] = t / a.waves_per_seq;
-        a.wave_first[t] = (t % a.waves_per_seq) * 64;
+        a.wave_first[t] = (t % a.waves_per_seq) * kDecGroupBlocks;  // one entry (first block 0) when short
     }
 }
 
@@ -83,7 +84,8 @@ uint64_t decode_device_noalloc(cc_instance* I, bool rl, int symSize, const uint3
     const CoderParams p = params_of(I);
     const uint32_t q = rl ? 2u : 1u;
     const uint32_t nblk = div_up(n, p.cbs);
-    const uint32_t waves_per_seq = div_up(nblk, 64);
+    const uint32_t waves_per_seq = dec_entries(nblk);
+    const bool lng = dec_long(nblk);
     const uint32_t nseq = nstreams * q;
     const uint64_t nwaves = (uint64_t)nseq * waves_per_seq;
     // the arena bound of the allocating decode (block offsets, lookup and symbol tables, the
@@ -127,7 +129,7 @@ uint64_t decode_device_noalloc(cc_instance* I, bool rl, int symSize, const uint3
     DecCallOut* d_out = reinterpret_cast<DecCallOut*>(base + o_state + sizeof(CoderState));
     coder_decode(a.calls, 1, a.items, nstreams, reinterpret_cast<DecSeq*>(base + o_seqs), nseq, p, symSize,
                  reinterpret_cast<uint32_t*>(base + o_arena), arena_words, d_state, d_out, a.wave_seq, a.wave_first,
-                 (uint32_t)nwaves, (uint64_t)nstreams * n,
+                 (uint32_t)nwaves, lng ? (uint32_t)nwaves : 0u, (uint64_t)nstreams * n,
                  RleChunks{a.item_first, (uint32_t)nchunks, reinterpret_cast<uint32_t*>(base + o_sums)}, st);
     struct {
         CoderState s;

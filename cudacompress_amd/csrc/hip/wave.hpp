@@ -94,6 +94,17 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* scratc
     return base + incl - v;
 }
 
+// The dispatcher hands workgroups to the 8 XCDs round robin (workgroup b runs on XCD b % 8), and
+// every XCD has its own L2.  xcd_order(b, n) renumbers the n workgroups of a launch so that each
+// XCD runs one contiguous range of logical workgroups: neighbours in logical order, which read the
+// same tables or lines, then share an L2 instead of fetching them once per XCD.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t n) {
+    const uint32_t x = b % kXcds, k = b / kXcds;
+    const uint32_t q = n / kXcds, r = n % kXcds;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // Sum over a workgroup of NT threads (NT multiple of 64), every thread gets it; NT/64 words of
 // LDS scratch; contains barriers, so every thread must call it.
 template <int NT>

@@ -165,6 +165,22 @@ uint64_t device_encode(cc_instance* I, const std::vector<CoderCall>& calls, cons
 
 // ------------------------------------------------------------------------------------- decode
 namespace {
+// Block-decoder entries of sequence `seq` of at most nblk blocks: workgroups of kDecGroupBlocks
+// blocks at wl (long sequences), one wave at ws (short ones).
+void add_entries(uint32_t seq, uint32_t nblk, uint32_t* hws, uint32_t* hwf, uint32_t& wl, uint32_t& ws) {
+    if (dec_long(nblk)) {
+        for (uint32_t b = 0; b < nblk; b += kDecGroupBlocks) {
+            hws[wl] = seq;
+            hwf[wl] = b;
+            wl++;
+        }
+    } else if (nblk) {
+        hws[ws] = seq;
+        hwf[ws] = 0;
+        ws++;
+    }
+}
+
 // first[i]: the first zero-run expansion chunk of item i (nitems + 1 entries); returns the total.
 uint32_t chunk_table(const std::vector<ItemSpec>& items, uint32_t* first) {
     uint64_t k = 0;
@@ -184,17 +200,18 @@ void decode_prepare(cc_instance* I, DeviceCoder& D, const std::vector<ItemSpec>&
     const uint32_t ncalls = (uint32_t)calls.size();
     const uint32_t nitems = (uint32_t)items.size();
     const uint32_t cbs = I->cbs;
-    uint32_t nseq = 0, nwaves = 0;
+    uint32_t nseq = 0, nlong = 0, nshort = 0;
     uint64_t total_syms = 0, arena = 64;
     for (const ItemSpec& it : items) {
         const uint32_t q = it.rl ? 2 : 1;
         const uint32_t nblk = div_up(it.n, cbs);
         nseq += q;
-        nwaves += q * div_up(nblk, 64);
+        (dec_long(nblk) ? nlong : nshort) += q * dec_entries(nblk);
         total_syms += it.n;
         // block offsets, lookup + symbol table (nsym <= count <= n), RLE decode buffers
         arena += q * ((uint64_t)nblk + 16 + (1u << kDecLutBits) + it.n + 17 + (it.rl ? ((uint64_t)it.n * symSize + 3) / 4 + 17 : 0));
     }
+    const uint32_t nwaves = nlong + nshort;
     job.o_items = align_up(sizeof(DecCall) * std::max<uint32_t>(ncalls, 1), 256);
     job.o_wseq = job.o_items + align_up(sizeof(DecItem) * std::max<uint32_t>(nitems, 1), 256);
     job.o_wfirst = job.o_wseq + align_up(4ull * std::max<uint32_t>(nwaves, 1), 256);
@@ -208,7 +225,7 @@ void decode_prepare(cc_instance* I, DeviceCoder& D, const std::vector<ItemSpec>&
     job.nchunks = chunk_table(items, (uint32_t*)(h + job.o_ifirst));
     job.o_sums = align_up(total, 256);
     for (uint32_t c = 0; c < ncalls; c++) hc[c] = DecCall{calls[c].words, calls[c].nwords, 0, 0};
-    uint32_t seq = 0, wv = 0, call = 0;
+    uint32_t seq = 0, wl = 0, ws = nlong, call = 0;
     for (uint32_t i = 0; i < nitems; i++) {
         const ItemSpec& it = items[i];
         CCA_REQUIRE(it.call >= call && it.call < ncalls, CC_ERR_INVALID_ARG, "items must be grouped by call in order");
@@ -219,17 +236,14 @@ void decode_prepare(cc_instance* I, DeviceCoder& D, const std::vector<ItemSpec>&
         hi[i] = DecItem{it.ptr, it.n, it.rl ? 1u : 0u, seq, it.call};
         const uint32_t nblk = div_up(it.n, cbs);
         for (uint32_t q = 0; q < (it.rl ? 2u : 1u); q++)
-            for (uint32_t b = 0; b < nblk; b += 64) {
-                hws[wv] = seq + q;
-                hwf[wv] = b;
-                wv++;
-            }
+            add_entries(seq + q, nblk, hws, hwf, wl, ws);
         seq += it.rl ? 2 : 1;
     }
     job.nitems = nitems;
     job.nseq = nseq;
     job.ncalls = ncalls;
     job.nwaves = nwaves;
+    job.nlong = nlong;
     job.total_syms = total_syms;
     job.arena = arena;
     job.symSize = symSize;
@@ -248,7 +262,7 @@ void decode_launch(DeviceCoder& D, const DecodeJob& job, hipStream_t st) {
     coder_decode((const DecCall*)d, job.ncalls, (const DecItem*)(d + job.o_items), job.nitems, (DecSeq*)D.d_dseqs.p,
                  job.nseq, job.p, job.symSize, (uint32_t*)D.d_dscratch.p, job.arena, (CoderState*)D.d_dstate.p,
                  (DecCallOut*)D.d_dout.p, (const uint32_t*)(d + job.o_wseq), (const uint32_t*)(d + job.o_wfirst),
-                 job.nwaves, job.total_syms,
+                 job.nwaves, job.nlong, job.total_syms,
                  RleChunks{(const uint32_t*)(d + job.o_ifirst), job.nchunks,
                            (uint32_t*)((unsigned char*)D.d_dcalls.p + job.o_sums)},
                  st);
@@ -326,14 +340,16 @@ void decode_prepare_bound(cc_instance* I, DeviceCoder& D, const DecodeIndex& X, 
     const uint32_t ncalls = (uint32_t)index_calls.size();
     const uint32_t nitems = (uint32_t)items.size();
     const uint32_t cbs = I->cbs;
-    uint32_t nseq = 0, nwaves = 0;
+    uint32_t nseq = 0, nlong = 0, nshort = 0;
     uint64_t total_syms = 0, rl_bytes = 0;
     for (const ItemSpec& it : items) {
         const uint32_t q = it.rl ? 2 : 1;
+        const uint32_t nblk = div_up(it.n, cbs);
         nseq += q;
-        nwaves += q * div_up(div_up(it.n, cbs), 64);
+        (dec_long(nblk) ? nlong : nshort) += q * dec_entries(nblk);
         total_syms += it.n;
     }
+    const uint32_t nwaves = nlong + nshort;
     job.o_items = align_up(sizeof(DecCall) * std::max<uint32_t>(ncalls, 1), 256);
     job.o_bind = job.o_items + align_up(sizeof(DecItem) * std::max<uint32_t>(nitems, 1), 256);
     job.o_cbrick = job.o_bind + align_up(sizeof(DecBind) * std::max<uint32_t>(nseq, 1), 256);
@@ -355,7 +371,7 @@ void decode_prepare_bound(cc_instance* I, DeviceCoder& D, const DecodeIndex& X, 
         hc[c] = DecCall{nullptr, 0, 0, 0};
         hcb[c] = index_calls[c];
     }
-    uint32_t seq = 0, wv = 0, call = 0, local = 0;
+    uint32_t seq = 0, wl = 0, ws = nlong, call = 0, local = 0;
     for (uint32_t i = 0; i < nitems; i++) {
         const ItemSpec& it = items[i];
         CCA_REQUIRE(it.call >= call && it.call < ncalls, CC_ERR_INVALID_ARG, "items must be grouped by call in order");
@@ -378,11 +394,7 @@ void decode_prepare_bound(cc_instance* I, DeviceCoder& D, const DecodeIndex& X, 
             }
             hb[seq + q] = DecBind{X.call_first_seq[index_calls[it.call]] + local, i, off};
             local++;
-            for (uint32_t b = 0; b < nblk; b += 64) {
-                hws[wv] = seq + q;
-                hwf[wv] = b;
-                wv++;
-            }
+            add_entries(seq + q, nblk, hws, hwf, wl, ws);
         }
         seq += it.rl ? 2 : 1;
     }
@@ -390,6 +402,7 @@ void decode_prepare_bound(cc_instance* I, DeviceCoder& D, const DecodeIndex& X, 
     job.nseq = nseq;
     job.ncalls = ncalls;
     job.nwaves = nwaves;
+    job.nlong = nlong;
     job.total_syms = total_syms;
     job.rl_bytes = rl_bytes;
     job.symSize = symSize;
@@ -410,7 +423,7 @@ void decode_launch_bound(DeviceCoder& D, const DecodeIndex& X, const BoundJob& j
                        job.ncalls, (const DecItem*)(d + job.o_items), job.nitems, (DecSeq*)D.d_dseqs.p, job.nseq,
                        job.p, job.symSize, (unsigned char*)D.d_dscratch.p, (CoderState*)D.d_dstate.p,
                        (DecCallOut*)D.d_dout.p, (const uint32_t*)(d + job.o_wseq),
-                       (const uint32_t*)(d + job.o_wfirst), job.nwaves, job.total_syms,
+                       (const uint32_t*)(d + job.o_wfirst), job.nwaves, job.nlong, job.total_syms,
                        RleChunks{(const uint32_t*)(d + job.o_ifirst), job.nchunks,
                                  (uint32_t*)((unsigned char*)D.d_dcalls.p + job.o_sums)},
                        st);

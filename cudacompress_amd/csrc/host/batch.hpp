@@ -47,7 +47,7 @@ void encode_launch(DeviceCoder& D, const EncodeJob& job, uint32_t* d_out, uint64
 CoderState encode_finish(DeviceCoder& D, const EncodeJob& job, std::vector<CallOut>* results, hipStream_t st);
 
 struct DecodeJob {
-    uint32_t nitems = 0, nseq = 0, ncalls = 0, nwaves = 0;
+    uint32_t nitems = 0, nseq = 0, ncalls = 0, nwaves = 0, nlong = 0;
     uint64_t total_syms = 0, arena = 0;
     int symSize = 2;
     CoderParams p{};
@@ -81,7 +81,7 @@ struct DecodeIndex {
 void decode_index_build(cc_instance* I, DecodeIndex& X, const std::vector<ItemSpec>& items,
                         const std::vector<CallWords>& calls, int symSize, hipStream_t st);
 struct BoundJob {
-    uint32_t nitems = 0, nseq = 0, ncalls = 0, nwaves = 0;
+    uint32_t nitems = 0, nseq = 0, ncalls = 0, nwaves = 0, nlong = 0;
     uint64_t total_syms = 0, rl_bytes = 0;
     int symSize = 2;
     CoderParams p{};
