@@ -258,28 +258,35 @@ __global__ __launch_bounds__(256) void k_dec_tables(DecSeq* seqs, CoderParams p,
 // Block decoder: one thread per coding block, W waves per workgroup (W = 4: 256 blocks of one
 // long sequence sharing one LDS copy of its lookup table, which then leaves L2 once per 256
 // blocks; W = 1: a short sequence's blocks, so that a sequence of a few blocks does not hold a
-// four-wave workgroup's LDS).  A lane decodes its block kDecRound symbols per round and stores
-// them itself (16 bytes per 8 16-bit symbols); the waves of a workgroup meet only at the table load.
+// four-wave workgroup's LDS).  A lane decodes its block R symbols per round (16 in the long
+// sequences' workgroups: fewer rounds, so fewer copies and waits per block; 8 in the short ones'
+// waves: less LDS per wave) and stores them itself (16 bytes per 8 16-bit symbols); the waves of a
+// workgroup meet only at the table load.
 // The kernel is bound by instruction issue and latency, so a code costs a handful of vector
 // instructions: the bit window is two words {hi, lo} read with one funnel shift (`alignbit`, the
 // next code starting q bits above lo's lowest bit); codewords reach a lane through an LDS copy of
-// the kWinWords words from the 16-byte boundary at or below its next word, loaded at the start of
-// every round with three 16-byte loads and one wait (a round takes at most one word per code), so
+// the R + 4 words from the 16-byte boundary at or below its next word, loaded at the start of
+// every round with 16-byte loads and one wait (a round takes at most one word per code), so
 // refilling the bit window is a branch-free select; the round is unrolled, its symbols collect in
 // registers at fixed positions; and the zero-count sums run only in the instance for the zero
 // counts of RLE items.
-constexpr uint32_t kDecRound = 8;
-constexpr uint32_t kWinWords = 12;  // >= kDecRound + 3 (the aligned start is up to 3 words early), 16-byte rows
+// Symbols per round R: 16 for the long sequences' workgroups, 8 for the short ones' waves (which
+// then hold less LDS: the copy is R + 4 words, >= R + 3 since it starts up to 3 words early).
+template <uint32_t R>
+constexpr uint32_t win_words() { return R + 4; }
 // row of a lane's run-path symbols in LDS: an odd number of words, so lanes writing the same
 // position use distinct banks
-template <class S>
-constexpr uint32_t run_row_words() { return sizeof(S) == 2 ? kDecRound / 2 + 1 : kDecRound + 1; }
+template <class S, uint32_t R>
+constexpr uint32_t run_row_words() { return sizeof(S) == 2 ? R / 2 + 1 : R + 1; }
 
-template <class S, int W>
+template <class S, int W, uint32_t R>
 __global__ __launch_bounds__(64 * W) void k_dec_blocks(const DecSeq* seqs, const uint32_t* wave_seq,
                                                        const uint32_t* wave_first, CoderParams p,
                                                        const uint32_t* scratch, const DecItem* items, RleChunks rc) {
-    constexpr uint32_t RW = run_row_words<S>();
+    constexpr uint32_t kDecRound = R;
+    constexpr uint32_t kWinWords = win_words<R>();
+    static_assert(R % 8 == 0 && kWinWords % 4 == 0, "16-byte stores and copies");
+    constexpr uint32_t RW = run_row_words<S, R>();
     constexpr uint32_t NW = kDecRound * sizeof(S) / 4;  // words of a round's symbols
     __shared__ uint32_t lut[1 << kLut];
     __shared__ int tab[96];
@@ -695,12 +702,12 @@ void launch_blocks(DecSeq* d_seqs, const uint32_t* d_wave_seq, const uint32_t* d
                    uint32_t nlong, const CoderParams& p, const uint32_t* scratch, const DecItem* d_items,
                    const RleChunks& rc, uint64_t outb, hipStream_t st) {
     if (nlong) {
-        CCA_LAUNCH("dec_blocks", outb, (k_dec_blocks<S, 4>), dim3(nlong), dim3(256), 0, st, d_seqs, d_wave_seq,
+        CCA_LAUNCH("dec_blocks", outb, (k_dec_blocks<S, 4, 16>), dim3(nlong), dim3(256), 0, st, d_seqs, d_wave_seq,
                    d_wave_first, p, scratch, d_items, rc);
         CCA_LAUNCH_CHECK();
     }
     if (nwaves > nlong) {
-        CCA_LAUNCH("dec_blocks_short", 0, (k_dec_blocks<S, 1>), dim3(nwaves - nlong), dim3(64), 0, st, d_seqs,
+        CCA_LAUNCH("dec_blocks_short", 0, (k_dec_blocks<S, 1, 8>), dim3(nwaves - nlong), dim3(64), 0, st, d_seqs,
                    d_wave_seq + nlong, d_wave_first + nlong, p, scratch, d_items, rc);
         CCA_LAUNCH_CHECK();
     }
