@@ -815,7 +815,10 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_pack(const CodeItem* ite
     {
         const uint32_t* syms_s = scratch + ss.syms_off;
         uint32_t* hs = out + ss.out_off;
-        for (uint32_t w = threadIdx.x; w < tws; w += kCoderThreads) hs[w] = header_word(ss, ss.cpl, p, syms_s, 32ull * w);
+        // the header from the global record: a local record whose address is taken lives in
+        // scratch memory, and so would every field the pack reads from it
+        const SeqInfo& gs = seqs[it.seq];
+        for (uint32_t w = threadIdx.x; w < tws; w += kCoderThreads) hs[w] = header_word(gs, gs.cpl, p, syms_s, 32ull * w);
         if (threadIdx.x == 0) {
             hs[tws] = (uint32_t)ss.bits;
             if (rl) out[ss.out_off - 1] = M;  // aligned u32 M
@@ -823,8 +826,9 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_pack(const CodeItem* ite
         if (rl) {
             const uint32_t* syms_z = scratch + sz.syms_off;
             uint32_t* hz = out + sz.out_off;
+            const SeqInfo& gz = seqs[it.seq + 1];
             for (uint32_t w = threadIdx.x; w < twz; w += kCoderThreads)
-                hz[w] = header_word(sz, sz.cpl, p, syms_z, 32ull * w);
+                hz[w] = header_word(gz, gz.cpl, p, syms_z, 32ull * w);
             if (threadIdx.x == 0) hz[twz] = (uint32_t)sz.bits;
         }
     }

@@ -628,25 +628,39 @@ __global__ __launch_bounds__(256) void k_dec_rle(const DecItem* items, uint32_t 
 
 // A bound decode's sequences from a resident index: the parsed header, table and block-offset
 // positions are copied, the item and the output are the request's (RLE sequences decode into the
-// request's buffer at rl_off).  Calls start from the index's parse result of their brick.
+// request's buffer at rl_off).  Calls start from the index's parse result of their brick.  One
+// thread per word of a record (a whole record per thread would sit in scratch memory).
+constexpr uint32_t kSeqWords = sizeof(DecSeq) / 4;
+static_assert(sizeof(DecSeq) % 8 == 0, "records of whole 64-bit words");
 __global__ void k_dec_bind(const DecSeq* idx, const DecBind* bind, uint32_t nseq, const DecItem* items,
                            DecSeq* seqs, unsigned char* rl_arena, const DecCallOut* idx_out,
                            const uint32_t* call_brick, uint32_t ncalls, DecCallOut* outc, CoderState* st) {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r == 0) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) {
         st->scratch_used = 0;
         st->out_words = 0;
         st->err = 0;
         st->pad = 0;
     }
+    if (t < ncalls) outc[t] = idx_out[call_brick[t]];
+    const uint64_t r = t / kSeqWords;
+    const uint32_t w = (uint32_t)(t % kSeqWords);
     if (r < nseq) {
+        constexpr uint32_t kItem = offsetof(DecSeq, item) / 4, kOut = offsetof(DecSeq, out) / 4;
+        static_assert(offsetof(DecSeq, out) % 8 == 0, "the output pointer is one 64-bit word");
         const DecBind b = bind[r];
-        DecSeq s = idx[b.idx_seq];
-        s.item = b.item;
-        s.out = b.rl_off != ~0ull ? (void*)(rl_arena + b.rl_off) : items[b.item].out;
-        seqs[r] = s;
+        uint32_t v;
+        if (w == kItem) {
+            v = b.item;
+        } else if (w == kOut || w == kOut + 1) {
+            const uint64_t o = reinterpret_cast<uint64_t>(b.rl_off != ~0ull ? (void*)(rl_arena + b.rl_off)
+                                                                               : items[b.item].out);
+            v = w == kOut ? (uint32_t)o : (uint32_t)(o >> 32);
+        } else {
+            v = reinterpret_cast<const uint32_t*>(idx + b.idx_seq)[w];
+        }
+        reinterpret_cast<uint32_t*>(seqs + r)[w] = v;
     }
-    if (r < ncalls) outc[r] = idx_out[call_brick[r]];
 }
 
 // Per-call status: the first error among the call's coded sequences, one wave per call, its
@@ -790,14 +804,15 @@ void coder_decode_bound(const DecSeq* d_idx_seqs, const uint32_t* idx_scratch, c
                         const uint32_t* d_wave_seq, const uint32_t* d_wave_first, uint32_t nwaves,
                         uint32_t nlong, uint64_t total_symbols, const RleChunks& rc, hipStream_t st) {
     CCA_REQUIRE(symSize == 2 || symSize == 4, CC_ERR_INVALID_ARG, "symbol size must be 2 or 4");
-    const uint32_t nb = std::max(nseq, ncalls);
+    const uint64_t nb = std::max<uint64_t>((uint64_t)nseq * kSeqWords, ncalls);
     if (nb == 0) {
         CCA_LAUNCH("dec_init", 32, k_dec_init, dim3(1), dim3(64), 0, st, d_state);
         CCA_LAUNCH_CHECK();
         return;
     }
     // the job state is reset by the bind (its first thread)
-    CCA_LAUNCH("dec_bind", (uint64_t)nseq * 2 * sizeof(DecSeq), k_dec_bind, dim3(div_up(nb, 64)), dim3(64), 0, st,
+    CCA_LAUNCH("dec_bind", (uint64_t)nseq * 2 * sizeof(DecSeq), k_dec_bind, dim3(div_up(nb, 256)),
+               dim3(256), 0, st,
                d_idx_seqs, d_bind, nseq, d_items, d_seqs, rl_arena, d_idx_out, d_call_brick, ncalls, d_out, d_state);
     CCA_LAUNCH_CHECK();
     const uint64_t outb = total_symbols * symSize;
