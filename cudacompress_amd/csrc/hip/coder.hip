@@ -909,26 +909,24 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_pack(const CodeItem* ite
             const uint64_t oz0 = cbits_z + (ex & 0xFFFFF);
             const uint64_t wbs = cbits_s >> 5, wbz = cbits_z >> 5;
             if (np) {
-                // coding-block starts among this thread's pairs (pair indices < M only: stats and
-                // pack agree on M, the bound keeps the offset writes in their arrays regardless)
-                uint32_t pi = (uint32_t)(pairs + (ex >> 40));
-                uint32_t rem = pi % cbs, blk = pi / cbs;
-                uint32_t os = (uint32_t)os0, oz = (uint32_t)oz0;
+                // the coding-block start among this thread's pairs: a thread emits at most
+                // kPerThread < cbs pairs (cbs a power of two), so at most one, its k-th pair (pair
+                // indices < M only: stats and pack agree on M, the bound keeps the offset writes in
+                // their arrays regardless); its bit offsets add the lengths of the k pairs before it
+                const uint32_t pi0 = (uint32_t)(pairs + (ex >> 40));
+                const uint32_t k = (cbs - (pi0 & (cbs - 1))) & (cbs - 1);
+                if (k < np && pi0 + k < M) {
+                    uint32_t os = (uint32_t)os0, oz = (uint32_t)oz0, e = 0;
 #pragma unroll
-                for (int j = 0; j < (int)kPerThread; j++) {
-                    if ((emit >> j) & 1u) {
-                        if (rem == 0 && pi < M) {
-                            bs[blk] = os;
-                            if (rl) bz[blk] = oz;
-                        }
-                        os += ls[j];
-                        oz += lzs[j];
-                        pi++;
-                        if (++rem == cbs) {
-                            rem = 0;
-                            blk++;
-                        }
+                    for (int j = 0; j < (int)kPerThread; j++) {
+                        const bool before = ((emit >> j) & 1u) && e < k;
+                        os += before ? ls[j] : 0u;
+                        oz += before ? lzs[j] : 0u;
+                        e += (emit >> j) & 1u;
                     }
+                    const uint32_t blk = (pi0 + k) >> (31 - __clz(cbs));
+                    bs[blk] = os;
+                    if (rl) bz[blk] = oz;
                 }
                 // codewords through a 64-bit accumulator, one LDS OR per window word touched
                 if (!single) {
