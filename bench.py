@@ -3,8 +3,9 @@
 Default workload (`bricks`, the batched configuration the headline metric compares with plain
 calls): 2048 float32 bricks, each axis drawn from {16,32,48,64,96,128} (seeded), cut from seeded
 k^-5/3 Gaussian random fields; 2-level 3D CDF 9/7, deadzone quantization at 0.005 x the brick's
-std, RLE+Huffman (log2 14, coding block 128).  One step = batched compression of every brick into
-one device buffer + batched decompression, inputs and bitstreams resident in HBM.  The same
+std, RLE+Huffman (log2 14, the coding block size of the library's tuning table).  One step =
+batched compression of every brick into one device buffer + batched decompression, inputs and
+bitstreams resident in HBM.  The same
 bricks are also run through the plain per-brick API (compressVolumeFloat/decompressVolumeFloat,
 host bitstreams) on a sample, and the batched bitstreams are checked against the plain ones.
 Steps are replays of hipGraph plans; three plans of the same bricks (own outputs, bitstreams and
@@ -378,8 +379,10 @@ def run_bricks(args, torch, cc, lib):
     dsts = [torch.empty_like(s) for s in srcs]
     nvox = sum(b.size for b in bricks)
     out = torch.empty(nvox + 4096 * len(bricks), dtype=torch.int32, device=dev)
-    inst = cc.createInstance(0, 7, 128 * 128 * 128 // 8, 128, 14)
-    tuning = dict(inst.getTuning(), **(json.loads(args.tune) if args.tune else {}))
+    # kernel shapes and coding block size from the library's tuning table (the autotuner's choice)
+    tuning = dict(cc.loadTuning(), **(json.loads(args.tune) if args.tune else {}))
+    cbs = int(tuning["codingBlockSize"])
+    inst = cc.createInstance(0, 7, 128 * 128 * 128 // 8, cbs, 14)
     inst.setTuning(tuning)
     if args.groups:
         inst.setBrickGroups(args.groups)
@@ -471,7 +474,7 @@ def run_bricks(args, torch, cc, lib):
         progress("bricks: plain per-brick sample")
         words = out[: int(offs[-1])].cpu().numpy().view(np.uint32)
         k = min(args.plain_sample, len(bricks))
-        pin = cc.createInstance(0, 7, 128 * 128 * 128 // 8, 128, 14)
+        pin = cc.createInstance(0, 7, 128 * 128 * 128 // 8, cbs, 14)
         ref = torch.empty_like(srcs[0])
         sample_bytes = sum(bricks[i].nbytes for i in range(k))
         for i in range(min(4, k)):  # warm the plain path
@@ -503,7 +506,7 @@ def run_bricks(args, torch, cc, lib):
         progress(f"bricks: one plan, {args.single_plan_groups} groups")
         for p in plans[1:]:
             p.close()
-        inst1 = cc.createInstance(0, 7, 128 * 128 * 128 // 8, 128, 14)
+        inst1 = cc.createInstance(0, 7, 128 * 128 * 128 // 8, cbs, 14)
         inst1.setTuning(tuning)
         inst1.setBrickGroups(args.single_plan_groups)
         d1 = [torch.empty_like(s) for s in srcs]
@@ -546,7 +549,8 @@ def run_bricks(args, torch, cc, lib):
     inst.close()
     cfg = {"workload": "C4 batched: %d float32 bricks, axes from {16,32,48,64,96,128} (seeded), cut from seeded "
                        "k^-5/3 fields; 2-level 3D CDF 9/7, deadzone step 0.005*std(brick), RLE+Huffman log2 14, "
-                       "coding block 128; one batched compress + one batched decompress per step" % len(bricks),
+                       "coding block %d (the tuning table's); one batched compress + one batched decompress per "
+                       "step" % (len(bricks), cbs),
            "bricks": len(bricks), "raw_bytes_per_direction": raw_bytes, "compressed_bytes": compressed,
            "compression_ratio": round(raw_bytes / compressed, 3), "psnr_db": round(float(psnr), 3),
            "worst_brick_psnr_db": round(float(worst), 3),

@@ -409,6 +409,46 @@ __device__ __forceinline__ void xy_load(const float* src, int E, Put put) {
     }
 }
 
+// Raw buffer accesses (AMD buffer resources: base, size in bytes; an access past the size reads 0
+// and a store there is dropped): 32-bit offsets and no bounds branch per element.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
+}
+constexpr uint32_t kBufOut = 0x40000000u;  // an offset past every buffer here
+
+// The R rows of nx floats of a slab (row r at element r nx of src), put(r, x, v).  Lanes run along
+// x (W, the power of two at or above nx, lanes per row, so that a row needs no division), XYT / W
+// rows per step and sixteen loads in flight per thread; raw buffer loads, so rows past R and
+// lanes past nx read 0 without a branch.  Rows wider than XYT take the element-linear form.
+template <class Put>
+__device__ __forceinline__ void xy_load_rows(const float* src, int nx, int R, Put put) {
+    const int lw = nx <= 1 ? 0 : 32 - __clz(nx - 1);
+    if ((1 << lw) > XYT) {
+        const Div dnx(nx);
+        xy_load(src, R * nx, [&](int e, float v) {
+            const int r = dnx.q(e);
+            put(r, e - r * nx, v);
+        });
+        return;
+    }
+    const int x = (int)threadIdx.x & ((1 << lw) - 1), r0 = (int)threadIdx.x >> lw, rs = XYT >> lw;
+    const __amdgpu_buffer_rsrc_t rsrc = buffer_of(src, (uint32_t)(R * nx) * 4u);
+    const uint32_t xo = x < nx ? (uint32_t)x * 4u : kBufOut;
+    constexpr int U = 16;
+    for (int rb = r0; rb < R; rb += rs * U) {
+        float v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            v[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                                 rsrc, (int)(xo + (uint32_t)((rb + rs * u) * nx) * 4u), 0, 0));
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int r = rb + rs * u;
+            if (r < R && x < nx) put(r, x, v[u]);
+        }
+    }
+}
+
 // Analysis extensions of `lines` lines of n samples: positions -4..-1 and n..n+3 mirror the line.
 __device__ __forceinline__ void xy_fill_mirror(float* buf, int lines, int stride, int n) {
     for (int h = threadIdx.x; h < lines * 8; h += XYT) {
@@ -443,10 +483,7 @@ __global__ __launch_bounds__(XYT) void k_bfwd_xy(const BrickLevel* lv, const uin
     float* dst = L.b + (size_t)z0 * nx * ny;
     const int tid = threadIdx.x;
     const Div dnx(nx), dny(ny), dR(R), dC(C);
-    xy_load(src, R * nx, [&](int e, float v) {
-        const int r = dnx.q(e);
-        buf[r * RS + 4 + (e - r * nx)] = v;
-    });
+    xy_load_rows(src, nx, R, [&](int r, int x, float v) { buf[r * RS + 4 + x] = v; });
     __syncthreads();
     xy_fill_mirror(buf, R, RS, nx);
     __syncthreads();
@@ -500,8 +537,7 @@ __global__ __launch_bounds__(XYT) void k_binv_yx(const BrickLevel* lv, const uin
     const int tid = threadIdx.x;
     const Div dnx(nx), dny(ny), dC(C);
     // columns, interleaved: row y of a slice holds low (y < my) or high coefficient y - my
-    xy_load(src, R * nx, [&](int e, float v) {
-        const int r = dnx.q(e), x = e - r * nx;
+    xy_load_rows(src, nx, R, [&](int r, int x, float v) {
         const int s = dny.q(r), y = r - s * ny;
         const int w = y < my ? 2 * y : 2 * (y - my) + 1;
         buf[(s * nx + x) * CS + 4 + w] = v;
@@ -543,13 +579,17 @@ __global__ __launch_bounds__(XYT) void k_binv_yx(const BrickLevel* lv, const uin
     __syncthreads();
     // x synthesis: outputs 2 u and 2 u + 1 of row r, plane elements 2 it and 2 it + 1
     const Div dmx(mx);
+    const __amdgpu_buffer_rsrc_t rdst = buffer_of(dst, (uint32_t)(R * nx) * 4u);
     for (int it = tid; it < R * mx; it += XYT) {
         const int r = dmx.q(it), u = it - r * mx;
         float w[10];
 #pragma unroll
         for (int i = 0; i < 10; i++) w[i] = buf[r * RS + 2 * u + i];
-        gst(dst, 2 * it, Inv97::out(w, 0));
-        gst(dst, 2 * it + 1, Inv97::out(w + 1, 1));
+        const float a = Inv97::out(w, 0), b = Inv97::out(w + 1, 1);
+        // outputs 2 it and 2 it + 1 are adjacent: one 8-byte store
+        typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+        const u32x2 ab = {__float_as_uint(a), __float_as_uint(b)};
+        __builtin_amdgcn_raw_buffer_store_b64(ab, rdst, (int)((uint32_t)it * 8u), 0, 0);
     }
 }
 
