@@ -260,19 +260,6 @@ __device__ void inv_tile(const Geo<AX>& g, uint32_t local, float (*lds)[LW], Loa
     }
 }
 
-// band index of a level coefficient and its position inside the dense band
-struct Band {
-    int b;
-    int li;
-};
-__device__ __forceinline__ Band band_of(int x, int y, int z, int hx, int hy, int hz) {
-    const int bx = x >= hx, by = y >= hy, bz = z >= hz;
-    Band r;
-    r.b = bx + 2 * by + 4 * bz;
-    r.li = ((z - bz * hz) * hy + (y - by * hy)) * hx + (x - bx * hx);
-    return r;
-}
-
 template <int AX>
 __global__ __launch_bounds__(256) void k_bfwd(const BrickLevel* lv, const uint32_t* tb, uint32_t nb) {
     __shared__ float lds[NC][LW];
@@ -285,78 +272,200 @@ __global__ __launch_bounds__(256) void k_bfwd(const BrickLevel* lv, const uint32
                  [&](int, int, int off, float v) { gst(dst, off, v); });
 }
 
-// A thread of a z tile serves one column (x, y) of the brick level (the tile's column c0 + lane):
-// its x and y octant bits and its offset inside a band are fixed, so only z varies per element.
+// Z passes: one thread per column (x, y) of the brick level and per segment of SP output pairs
+// along z.  The 10-sample filter window streams through registers (no LDS, no barrier); a
+// segment reads its z range once plus the filter halo.  A workgroup is one wave of NC columns,
+// numbered band-major (the columns of octant quarter bx + 2 by of the xy plane, then y, then x
+// inside it), so the lanes of a wave read and write consecutive elements of one band (and, for a
+// level whose quarter holds a multiple of NC columns, all serve one quarter: the lowpass quarter's
+// loads and stores differ from the highpass ones).  Tiles (NC columns x SP pairs) are those of
+// brick_tiles along z.  The window and the fmaf order are those of the tiled passes
+// (Fwd97::pair over positions 2q - 4 .. 2q + 5, Inv97::out over interleaved pairs u - 2 .. u + 2),
+// so the results are identical.
+constexpr int ZU = 8;  // output pairs per batch of loads (2 ZU loads in flight per thread)
+
+// The column of a z-pass thread.
 struct ZColumn {
-    int bxy;   // bx + 2 by
-    int lixy;  // (y - by hy) hx + (x - bx hx)
-    int hz, hxy;
-    __device__ ZColumn(const Geo<2>& g, uint32_t local, int hx, int hy, int hz_) : hz(hz_), hxy(hx * hy) {
+    int k0, k1;     // output pairs [k0, k1) of the segment
+    int q;          // quarter bx + 2 by (-1: past the level's columns)
+    uint32_t r;     // (y - by hy) hx + (x - bx hx): offset inside a band plane
+    uint32_t xy;    // y lx + x: offset inside a dense plane
+    __device__ ZColumn(const Geo<2>& g, uint32_t local, int hx, int hy) {
         const int ctiles = (g.ncols + NC - 1) / NC;
-        const int c = (int)(local % (uint32_t)ctiles) * NC + (int)(threadIdx.x & 63);
-        const int x = c % g.lx, y = c / g.lx;
-        const int bx = x >= hx, by = y >= hy;
-        bxy = bx + 2 * by;
-        lixy = (y - by * hy) * hx + (x - bx * hx);
-    }
-    // band_of(x, y, z) of this column
-    __device__ Band at(int z) const {
-        const int bz = z >= hz;
-        return Band{bxy + 4 * bz, (z - bz * hz) * hxy + lixy};
+        const int c = (int)(local % (uint32_t)ctiles) * NC + (int)threadIdx.x;
+        k0 = (int)(local / (uint32_t)ctiles) * SP;
+        k1 = min(k0 + SP, g.m);
+        const int hxy = hx * hy;
+        q = c < g.ncols ? c / hxy : -1;
+        const int rr = c - q * hxy, yq = rr / hx, xq = rr - yq * hx;
+        r = (uint32_t)rr;
+        xy = (uint32_t)(((q >> 1) * hy + yq) * g.lx + (q & 1) * hx + xq);
     }
 };
 
-template <class S>
-__global__ __launch_bounds__(256) void k_bfwd_zq(const BrickLevel* lv, const uint32_t* tb, uint32_t nb) {
-    __shared__ float lds[NC][LW];
-    const uint32_t i = find_brick(tb, nb, blockIdx.x);
-    const BrickLevel L = lv[i];
-    const Geo<2> g(L.lx, L.ly, L.lz);
-    const int hx = L.lx / 2, hy = L.ly / 2, hz = L.lz / 2;
-    const int band = hx * hy * hz;
-    S* hp = static_cast<S*>(L.hp);
-    S* lps = static_cast<S*>(L.lpsym);
-    const uint32_t local = blockIdx.x - tb[i];
-    const ZColumn zc(g, local, hx, hy, hz);
-    fwd_tile<2>(g, local, lds, [&](int, int, int off) { return gld<float>(L.b, off); },
-                [&](int, int p, int, float v) {
-                    const Band bd = zc.at(p);
-                    if (bd.b == 0) {
-                        if (L.last) gst(lps, bd.li, (S)symbolize_i(quantize_f(v, L.inv_lp, 0)));
-                        else gst(L.lp, (int)bd.li, v);
-                    } else {
-                        gst(hp, (bd.b - 1) * band + (int)bd.li, (S)symbolize_i(quantize_f(v, L.inv_hp, 0)));
-                    }
-                });
+// Extensions of a pair or position index, uniform across the wave: one reflection when the line
+// is long enough (LONG), else the repeated forms.
+template <bool LONG>
+__device__ __forceinline__ int zmirror(int j, int n) { return LONG ? mirror1(j, n) : mirror(j, n); }
+template <bool LONG>
+__device__ __forceinline__ int zext_low(int g, int m) { return LONG ? ext_low1(g, m) : ext_low(g, m); }
+template <bool LONG>
+__device__ __forceinline__ int zext_high(int g, int m) { return LONG ? ext_high1(g, m) : ext_high(g, m); }
+
+// element i of the plane at p (p uniform, i per lane)
+template <class T>
+__device__ __forceinline__ T zld(const T* p, uint32_t i) {
+    return ((const __attribute__((address_space(1))) T*)p)[i];
+}
+template <class T>
+__device__ __forceinline__ void zst(T* p, uint32_t i, T v) {
+    ((__attribute__((address_space(1))) T*)p)[i] = v;
 }
 
+// Forward analysis of one column segment: positions from src (plane stride nc), low outputs to
+// lo (floats when LOF, else symbols of lo_sym), high outputs to the symbols of hi (band plane
+// stride hxy).
+template <bool LOF, bool LONG, class S>
+__device__ __forceinline__ void zfwd_column(const ZColumn& zc, const float* src, int n, int nc, float* lo_f,
+                                            S* lo_sym, uint32_t ilo, float inv_lo, S* hi, uint32_t ihi,
+                                            float inv_hi, int hxy) {
+    auto put = [&](int k, float lo, float h) {
+        if (LOF) zst(lo_f + k * hxy, ilo, lo);
+        else zst(lo_sym + k * hxy, ilo, (S)symbolize_i(quantize_f(lo, inv_lo, 0)));
+        zst(hi + k * hxy, ihi, (S)symbolize_i(quantize_f(h, inv_hi, 0)));
+    };
+    float w[10];  // positions 2k - 4 .. 2k + 5 of the current pair k
+#pragma unroll
+    for (int j = 0; j < 8; j++) w[j] = zld(src + zmirror<LONG>(2 * zc.k0 - 4 + j, n) * nc, zc.xy);
+    int k = zc.k0;
+    for (; k + ZU <= zc.k1; k += ZU) {
+        float v[2 * ZU];
+#pragma unroll
+        for (int j = 0; j < 2 * ZU; j++) v[j] = zld(src + zmirror<LONG>(2 * k + 4 + j, n) * nc, zc.xy);
+#pragma unroll
+        for (int u = 0; u < ZU; u++) {
+            w[8] = v[2 * u];
+            w[9] = v[2 * u + 1];
+            float lo, h;
+            Fwd97::pair(w, 0, lo, h);
+            put(k + u, lo, h);
+#pragma unroll
+            for (int t = 0; t < 8; t++) w[t] = w[t + 2];
+        }
+    }
+    for (; k < zc.k1; k++) {
+        w[8] = zld(src + zmirror<LONG>(2 * k + 4, n) * nc, zc.xy);
+        w[9] = zld(src + zmirror<LONG>(2 * k + 5, n) * nc, zc.xy);
+        float lo, h;
+        Fwd97::pair(w, 0, lo, h);
+        put(k, lo, h);
+#pragma unroll
+        for (int t = 0; t < 8; t++) w[t] = w[t + 2];
+    }
+}
+
+// Forward z analysis with the quantizer: L.b -> lowpass octant (L.lp, or the symbol stream on the
+// coarsest level) and the highpass symbol bands.
 template <class S>
-__global__ __launch_bounds__(256) void k_binv_zs(const BrickLevel* lv, const uint32_t* tb, uint32_t nb) {
-    __shared__ float lds[NC][LW];
+__global__ __launch_bounds__(NC) void k_bfwd_zq(const BrickLevel* lv, const uint32_t* tb, uint32_t nb) {
     const uint32_t i = find_brick(tb, nb, blockIdx.x);
     const BrickLevel L = lv[i];
     const Geo<2> g(L.lx, L.ly, L.lz);
-    const int hx = L.lx / 2, hy = L.ly / 2, hz = L.lz / 2;
-    const int band = hx * hy * hz;
+    const int hx = L.lx / 2, hy = L.ly / 2, hxy = hx * hy;
+    const uint32_t band = (uint32_t)hxy * (uint32_t)g.m;
+    const ZColumn zc(g, blockIdx.x - tb[i], hx, hy);
+    if (zc.q < 0) return;
+    S* hp = static_cast<S*>(L.hp);
+    const uint32_t ihi = (uint32_t)(zc.q + 3) * band + zc.r;
+    if (g.n >= 5) {
+        if (zc.q != 0) zfwd_column<false, true>(zc, L.b, g.n, g.ncols, nullptr, hp, (zc.q - 1) * band + zc.r, L.inv_hp, hp, ihi, L.inv_hp, hxy);
+        else if (L.last) zfwd_column<false, true>(zc, L.b, g.n, g.ncols, nullptr, static_cast<S*>(L.lpsym), zc.r, L.inv_lp, hp, ihi, L.inv_hp, hxy);
+        else zfwd_column<true, true, S>(zc, L.b, g.n, g.ncols, L.lp, nullptr, zc.r, 0.0f, hp, ihi, L.inv_hp, hxy);
+    } else {
+        if (zc.q != 0) zfwd_column<false, false>(zc, L.b, g.n, g.ncols, nullptr, hp, (zc.q - 1) * band + zc.r, L.inv_hp, hp, ihi, L.inv_hp, hxy);
+        else if (L.last) zfwd_column<false, false>(zc, L.b, g.n, g.ncols, nullptr, static_cast<S*>(L.lpsym), zc.r, L.inv_lp, hp, ihi, L.inv_hp, hxy);
+        else zfwd_column<true, false, S>(zc, L.b, g.n, g.ncols, L.lp, nullptr, zc.r, 0.0f, hp, ihi, L.inv_hp, hxy);
+    }
+}
+
+// Inverse synthesis of one column segment: low coefficients from lo (floats when LOF, else
+// symbols of lo_sym dequantized with step_lo), high ones from the symbols of hi (band plane
+// stride hxy), outputs to dst (plane stride nc).
+template <bool LOF, bool LONG, class S>
+__device__ __forceinline__ void zinv_column(const ZColumn& zc, const float* lo_f, const S* lo_sym, uint32_t ilo,
+                                            float step_lo, const S* hi, uint32_t ihi, float step_hi, int hxy,
+                                            int m, float* dst, int nc) {
+    auto raw_lo = [&](int k) -> uint32_t {
+        const int e = zext_low<LONG>(k, m) * hxy;
+        return LOF ? __float_as_uint(zld(lo_f + e, ilo)) : (uint32_t)zld(lo_sym + e, ilo);
+    };
+    auto raw_hi = [&](int k) -> uint32_t { return (uint32_t)zld(hi + zext_high<LONG>(k, m) * hxy, ihi); };
+    auto val_lo = [&](uint32_t r) { return LOF ? __uint_as_float(r) : unquantize_i(unsymbolize_u(r), step_lo, 0); };
+    auto val_hi = [&](uint32_t r) { return unquantize_i(unsymbolize_u(r), step_hi, 0); };
+    float w[10];  // interleaved pairs u - 2 .. u + 2 of the current output pair u
+    {
+        uint32_t r[8];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            r[2 * j] = raw_lo(zc.k0 - 2 + j);
+            r[2 * j + 1] = raw_hi(zc.k0 - 2 + j);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            w[2 * j] = val_lo(r[2 * j]);
+            w[2 * j + 1] = val_hi(r[2 * j + 1]);
+        }
+    }
+    auto emit = [&](int u) {
+        zst(dst + 2 * u * nc, zc.xy, Inv97::out(w, 0));
+        zst(dst + (2 * u + 1) * nc, zc.xy, Inv97::out(w + 1, 1));
+#pragma unroll
+        for (int t = 0; t < 8; t++) w[t] = w[t + 2];
+    };
+    int u = zc.k0;
+    for (; u + ZU <= zc.k1; u += ZU) {
+        uint32_t r[2 * ZU];
+#pragma unroll
+        for (int j = 0; j < ZU; j++) {
+            r[2 * j] = raw_lo(u + 2 + j);
+            r[2 * j + 1] = raw_hi(u + 2 + j);
+        }
+#pragma unroll
+        for (int j = 0; j < ZU; j++) {
+            w[8] = val_lo(r[2 * j]);
+            w[9] = val_hi(r[2 * j + 1]);
+            emit(u + j);
+        }
+    }
+    for (; u < zc.k1; u++) {
+        w[8] = val_lo(raw_lo(u + 2));
+        w[9] = val_hi(raw_hi(u + 2));
+        emit(u);
+    }
+}
+
+// Inverse z synthesis from the symbols (dequantized on the fly) and the lowpass octant (floats of
+// the level above, or symbols on the coarsest level): -> L.a.
+template <class S>
+__global__ __launch_bounds__(NC) void k_binv_zs(const BrickLevel* lv, const uint32_t* tb, uint32_t nb) {
+    const uint32_t i = find_brick(tb, nb, blockIdx.x);
+    const BrickLevel L = lv[i];
+    const Geo<2> g(L.lx, L.ly, L.lz);
+    const int hx = L.lx / 2, hy = L.ly / 2, hxy = hx * hy;
+    const uint32_t band = (uint32_t)hxy * (uint32_t)g.m;
+    const ZColumn zc(g, blockIdx.x - tb[i], hx, hy);
+    if (zc.q < 0) return;
     const S* hp = static_cast<const S*>(L.hp);
-    const S* lps = static_cast<const S*>(L.lpsym);
-    const uint32_t local = blockIdx.x - tb[i];
-    const ZColumn zc(g, local, hx, hy, hz);
-    // raw entries: the symbol, or the lowpass float's bits (lowpass of a level above the coarsest)
-    // half the window per batch keeps the register count (and occupancy) of the plain passes
-    inv_tile<2, 9>(g, local, lds,
-                [&](int, int p, int) -> uint32_t {
-                    const Band bd = zc.at(p);
-                    if (bd.b == 0 && !L.last) return __float_as_uint(gld<float>(L.lp, bd.li));
-                    const S* sp = bd.b == 0 ? lps : hp + (bd.b - 1) * band;
-                    return (uint32_t)gld(sp, bd.li);
-                },
-                [&](int, int p, uint32_t raw) -> float {
-                    const int bz = p >= hz;
-                    if (zc.bxy + 4 * bz == 0 && !L.last) return __uint_as_float(raw);
-                    return unquantize_i(unsymbolize_u(raw), zc.bxy + 4 * bz == 0 ? L.step_lp : L.step_hp, 0);
-                },
-                [&](int, int, int off, float v) { gst(L.a, off, v); });
+    const uint32_t ihi = (uint32_t)(zc.q + 3) * band + zc.r;
+    if (g.m >= 3) {
+        if (zc.q != 0) zinv_column<false, true>(zc, nullptr, hp, (zc.q - 1) * band + zc.r, L.step_hp, hp, ihi, L.step_hp, hxy, g.m, L.a, g.ncols);
+        else if (L.last) zinv_column<false, true>(zc, nullptr, static_cast<const S*>(L.lpsym), zc.r, L.step_lp, hp, ihi, L.step_hp, hxy, g.m, L.a, g.ncols);
+        else zinv_column<true, true, S>(zc, L.lp, nullptr, zc.r, 0.0f, hp, ihi, L.step_hp, hxy, g.m, L.a, g.ncols);
+    } else {
+        if (zc.q != 0) zinv_column<false, false>(zc, nullptr, hp, (zc.q - 1) * band + zc.r, L.step_hp, hp, ihi, L.step_hp, hxy, g.m, L.a, g.ncols);
+        else if (L.last) zinv_column<false, false>(zc, nullptr, static_cast<const S*>(L.lpsym), zc.r, L.step_lp, hp, ihi, L.step_hp, hxy, g.m, L.a, g.ncols);
+        else zinv_column<true, false, S>(zc, L.lp, nullptr, zc.r, 0.0f, hp, ihi, L.step_hp, hxy, g.m, L.a, g.ncols);
+    }
 }
 
 template <int AX>
@@ -613,9 +722,9 @@ void bricks_forward_level(const BrickLevel* d_levels, const uint32_t* d_tiles, u
     const uint64_t zb = level_elems * 4 + level_elems / 8 * 4 + level_elems * 7 / 8 * symSize;
     if (h_tiles_total[2]) {
         if (symSize == 2)
-            CCA_LAUNCH("bricks_fwd_z_quant", zb, k_bfwd_zq<uint16_t>, dim3(h_tiles_total[2]), dim3(256), 0, st, d_levels, tz, nbricks);
+            CCA_LAUNCH("bricks_fwd_z_quant", zb, k_bfwd_zq<uint16_t>, dim3(h_tiles_total[2]), dim3(NC), 0, st, d_levels, tz, nbricks);
         else
-            CCA_LAUNCH("bricks_fwd_z_quant", zb, k_bfwd_zq<uint32_t>, dim3(h_tiles_total[2]), dim3(256), 0, st, d_levels, tz, nbricks);
+            CCA_LAUNCH("bricks_fwd_z_quant", zb, k_bfwd_zq<uint32_t>, dim3(h_tiles_total[2]), dim3(NC), 0, st, d_levels, tz, nbricks);
     }
     CCA_LAUNCH_CHECK();
 }
@@ -628,9 +737,9 @@ void bricks_inverse_level(const BrickLevel* d_levels, const uint32_t* d_tiles, u
     const uint64_t zb = level_elems * 4 + level_elems / 8 * 4 + level_elems * 7 / 8 * symSize;
     if (h_tiles_total[2]) {
         if (symSize == 2)
-            CCA_LAUNCH("bricks_inv_z_dequant", zb, k_binv_zs<uint16_t>, dim3(h_tiles_total[2]), dim3(256), 0, st, d_levels, tz, nbricks);
+            CCA_LAUNCH("bricks_inv_z_dequant", zb, k_binv_zs<uint16_t>, dim3(h_tiles_total[2]), dim3(NC), 0, st, d_levels, tz, nbricks);
         else
-            CCA_LAUNCH("bricks_inv_z_dequant", zb, k_binv_zs<uint32_t>, dim3(h_tiles_total[2]), dim3(256), 0, st, d_levels, tz, nbricks);
+            CCA_LAUNCH("bricks_inv_z_dequant", zb, k_binv_zs<uint32_t>, dim3(h_tiles_total[2]), dim3(NC), 0, st, d_levels, tz, nbricks);
     }
     CCA_LAUNCH_CHECK();
     if (h_tiles_total[3]) {
