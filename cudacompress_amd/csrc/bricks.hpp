@@ -40,21 +40,23 @@ inline uint32_t brick_tiles(int ax, int lx, int ly, int lz) {
     return (uint32_t)(((cols + kBrickCols - 1) / kBrickCols) * ((n / 2 + kBrickSP - 1) / kBrickSP));
 }
 
-// Fused x+y passes: a workgroup of kBrickXYThreads holds zk whole z-slices of a brick in LDS
-// (kBrickXYFloats floats), each slice once as rows with their x extensions and once as columns
-// with their y extensions; a level uses them when every brick's slice fits.
-constexpr int kBrickXYThreads = 512;
-constexpr int kBrickXYFloats = 17536;  // one 128 x 128 slice: 128 lines of 128 + 9
-
-// LDS floats of one z-slice in the fused passes (rows of lx + 9, columns of ly + 9; odd strides)
-inline uint32_t brick_xy_floats(int lx, int ly) {
-    const uint64_t r = (uint64_t)ly * (lx + 9), c = (uint64_t)lx * (ly + 9);
-    return (uint32_t)std::min<uint64_t>(std::max(r, c), 0xFFFFFFFFull);
+// Fused x+y passes, streaming along y: a workgroup of kBrickXYThreads threads serves zk z-slices
+// of a brick (thread k of a slice: x pair k, i.e. the columns k and lx/2 + k of the x-filtered
+// plane) and a segment of kBrickXYSeg output pairs along y.  Each step brings two rows through
+// LDS for the x filter while the y filter windows stay in registers.  A level uses them when
+// every brick's rows have at least 8 samples and its x pairs fit a workgroup.
+constexpr int kBrickXYThreads = 256;
+constexpr int kBrickXYSeg = 32;
+constexpr int kBrickXYRow = 1024;  // LDS floats of one staged row of every slice (zk (lx + 8) <= 1024)
+inline bool brick_xy_fits(int lx) { return lx >= 8 && lx / 2 <= kBrickXYThreads; }
+inline uint32_t brick_xy_slices(int lx) { return (uint32_t)(kBrickXYThreads / (lx / 2)); }
+inline uint32_t brick_xy_tiles(int ly, int lz, int zk) {
+    return div_up((uint64_t)lz, (uint64_t)zk) * div_up((uint64_t)(ly / 2), (uint64_t)kBrickXYSeg);
 }
 
 // One level of the batch, forward (X, Y, Z+quantize) or inverse (Z from symbols, Y, X).
 // tiles: [4][nbricks+1] prefix of brick_tiles for axes x, y, z and of the fused x+y
-// workgroups (ceil(lz / zk) per brick); h_tiles_total[3] == 0 selects the separate x and y passes.
+// workgroups (brick_xy_tiles per brick); h_tiles_total[3] == 0 selects the separate x and y passes.
 void bricks_forward_level(const BrickLevel* d_levels, const uint32_t* d_tiles, uint32_t nbricks,
                           const uint32_t* h_tiles_total, int symSize, uint64_t level_elems, hipStream_t st);
 void bricks_inverse_level(const BrickLevel* d_levels, const uint32_t* d_tiles, uint32_t nbricks,

@@ -481,225 +481,255 @@ __global__ __launch_bounds__(256) void k_binv(const BrickLevel* lv, const uint32
                  [&](int, int, int off, float v) { gst(dst, off, v); });
 }
 
-// ------------------------------------------------ fused x+y passes over whole z-slices in LDS
-// Workgroup w holds zk slices of one brick: R = zk ly rows (x lines) and C = zk lx columns (y
-// lines), each line with its extensions in LDS (entry 4 + p for window position p), strides lx + 9
-// and ly + 9 (odd: lanes on consecutive lines hit distinct banks).  The filters are Fwd97::pair and
-// Inv97::out over the same windows as the tiled passes, so the results are identical.
+// ------------------------------------------------ fused x+y passes, streaming along y
+// Workgroup: brick bi, slices z0 .. z0 + zk - 1 (thread t: slice t / mx, x pair k = t % mx) and
+// output pairs [k0, k1) along y.  A step stages two rows of every slice in LDS (entry 4 + p for
+// position p, with the extensions of the x filter written by the threads holding their sources),
+// one barrier, then each thread runs the x filter of its pair on both rows.  The y filter windows
+// (10 samples of the columns k and mx + k of the x-filtered plane, or of the inverse's input)
+// stay in registers, as in the z passes.  Filters and windows are those of the tiled passes
+// (Fwd97::pair, Inv97::out over the same extended samples), so the results are identical.
+// LDS: two buffers (alternate steps, so one barrier per step suffices) of two rows.
 constexpr int XYT = kBrickXYThreads;
-constexpr int XYB = kBrickXYFloats;
-constexpr int NIF = (XYB / 2 + XYT - 1) / XYT;  // forward pairs per thread (R mx < XYB / 2)
-constexpr int NII = (XYB / 2 + XYT - 1) / XYT;  // inverse output pairs per thread (C my < XYB / 2)
+constexpr int XRS = kBrickXYRow;
+constexpr int XU = 4;  // steps per batch of loads
 
-// Exact quotient e / d for 0 <= e < 2^16 and 0 < d < 2^16 (one multiply-high; m wraps to 0 for
-// d = 1).
-struct Div {
-    uint32_t m;
-    __device__ explicit Div(int d) : m(0xFFFFFFFFu / (uint32_t)d + 1u) {}
-    __device__ int q(int e) const { return m ? (int)__umulhi((uint32_t)e, m) : e; }
+// 8-byte LDS accesses of the staged rows (a float array): a may_alias vector type, so the accesses
+// keep their order against the array's float accesses.
+typedef float xyf2 __attribute__((ext_vector_type(2), may_alias));
+__device__ __forceinline__ void lds_put2(float* p, float2 v) { *(xyf2*)p = xyf2{v.x, v.y}; }
+__device__ __forceinline__ void lds_get2(float* w, const float* p) {
+    const xyf2 v = *(const xyf2*)p;
+    w[0] = v.x;
+    w[1] = v.y;
+}
+
+struct XYTile {
+    int nx, ny, mx, my;
+    int zk;          // slices of the workgroup
+    int k0, k1;      // output pairs along y
+    int s, k;        // thread: slice, x pair
+    bool act;        // s < zk
+    size_t plane0;   // element offset of the thread's slice (z0 + s) in the level
+    int row0;        // LDS offset of the thread's slice in a staged row
+    __device__ XYTile(const BrickLevel& L, uint32_t local) {
+        nx = L.lx;
+        ny = L.ly;
+        mx = nx / 2;
+        my = ny / 2;
+        const int zg = (L.lz + L.zk - 1) / L.zk;
+        const int z0 = (int)(local % (uint32_t)zg) * L.zk;
+        k0 = (int)(local / (uint32_t)zg) * kBrickXYSeg;
+        k1 = min(k0 + kBrickXYSeg, my);
+        zk = min(L.zk, L.lz - z0);
+        s = (int)threadIdx.x / mx;
+        k = (int)threadIdx.x - s * mx;
+        act = s < zk;
+        plane0 = (size_t)(z0 + (act ? s : 0)) * (size_t)(nx * ny);
+        row0 = s * (nx + 8);
+    }
 };
 
-// e = 0 .. E-1 of a dense run of floats, sixteen loads in flight per thread; put(e, v).
-template <class Put>
-__device__ __forceinline__ void xy_load(const float* src, int E, Put put) {
-    constexpr int U = 16;
-    for (int e0 = threadIdx.x; e0 < E; e0 += XYT * U) {
-        float v[U];
+// Forward: stage rows a (v0 = positions 2k, 2k + 1) and a + 1 (v1) into buffer b, then the x
+// analysis of pair k of both rows: lo[r] (column k), hi[r] (column mx + k).
+__device__ __forceinline__ void xy_fwd_ingest(float* st, const XYTile& t, int b, float2 v0, float2 v1, float lo[2],
+                                              float hi[2]) {
+    float* r0 = st + (2 * b) * XRS + t.row0;
+    float* r1 = r0 + XRS;
+    if (t.act) {
+        lds_put2(r0 + 4 + 2 * t.k, v0);
+        lds_put2(r1 + 4 + 2 * t.k, v1);
+        if (t.k < 3 || t.k >= t.mx - 3) {
+            // whole-sample symmetric extensions: positions -4 .. -1 and nx .. nx + 3
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int e = e0 + XYT * u;
-            v[u] = e < E ? gld(src, e) : 0.0f;
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int e = e0 + XYT * u;
-            if (e < E) put(e, v[u]);
-        }
-    }
-}
-
-// Raw buffer accesses (AMD buffer resources: base, size in bytes; an access past the size reads 0
-// and a store there is dropped): 32-bit offsets and no bounds branch per element.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const void* p, uint32_t bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
-}
-constexpr uint32_t kBufOut = 0x40000000u;  // an offset past every buffer here
-
-// The R rows of nx floats of a slab (row r at element r nx of src), put(r, x, v).  Lanes run along
-// x (W, the power of two at or above nx, lanes per row, so that a row needs no division), XYT / W
-// rows per step and sixteen loads in flight per thread; raw buffer loads, so rows past R and
-// lanes past nx read 0 without a branch.  Rows wider than XYT take the element-linear form.
-template <class Put>
-__device__ __forceinline__ void xy_load_rows(const float* src, int nx, int R, Put put) {
-    const int lw = nx <= 1 ? 0 : 32 - __clz(nx - 1);
-    if ((1 << lw) > XYT) {
-        const Div dnx(nx);
-        xy_load(src, R * nx, [&](int e, float v) {
-            const int r = dnx.q(e);
-            put(r, e - r * nx, v);
-        });
-        return;
-    }
-    const int x = (int)threadIdx.x & ((1 << lw) - 1), r0 = (int)threadIdx.x >> lw, rs = XYT >> lw;
-    const __amdgpu_buffer_rsrc_t rsrc = buffer_of(src, (uint32_t)(R * nx) * 4u);
-    const uint32_t xo = x < nx ? (uint32_t)x * 4u : kBufOut;
-    constexpr int U = 16;
-    for (int rb = r0; rb < R; rb += rs * U) {
-        float v[U];
-#pragma unroll
-        for (int u = 0; u < U; u++)
-            v[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
-                                                 rsrc, (int)(xo + (uint32_t)((rb + rs * u) * nx) * 4u), 0, 0));
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int r = rb + rs * u;
-            if (r < R && x < nx) put(r, x, v[u]);
-        }
-    }
-}
-
-// Analysis extensions of `lines` lines of n samples: positions -4..-1 and n..n+3 mirror the line.
-__device__ __forceinline__ void xy_fill_mirror(float* buf, int lines, int stride, int n) {
-    for (int h = threadIdx.x; h < lines * 8; h += XYT) {
-        const int ln = h >> 3, j = h & 7;
-        const int p = j < 4 ? j - 4 : n + j - 4;
-        buf[ln * stride + 4 + p] = buf[ln * stride + 4 + mirror(p, n)];
-    }
-}
-
-// Synthesis extensions of interleaved lines (entry 4 + 2k + h: low (h = 0) or high coefficient k
-// of m): pairs -2, -1, m, m + 1 by ext_low / ext_high.
-__device__ __forceinline__ void xy_fill_inter(float* buf, int lines, int stride, int m) {
-    for (int h = threadIdx.x; h < lines * 8; h += XYT) {
-        const int ln = h >> 3, j = h & 7;
-        const int v = j < 4 ? j - 4 : 2 * m + j - 4;
-        const int k = v >> 1, hi = v & 1;
-        const int ks = hi ? ext_high(k, m) : ext_low(k, m);
-        buf[ln * stride + 4 + v] = buf[ln * stride + 4 + 2 * ks + hi];
-    }
-}
-
-// Forward x then y analysis of zk slices: L.in -> L.b.
-__global__ __launch_bounds__(XYT) void k_bfwd_xy(const BrickLevel* lv, const uint32_t* tb, uint32_t nb) {
-    extern __shared__ float buf[];
-    const uint32_t bi = find_brick(tb, nb, blockIdx.x);
-    const BrickLevel L = lv[bi];
-    const int nx = L.lx, ny = L.ly, mx = nx / 2, my = ny / 2;
-    const int z0 = (int)(blockIdx.x - tb[bi]) * L.zk;
-    const int k = min(L.zk, L.lz - z0);
-    const int RS = nx + 9, CS = ny + 9, R = k * ny, C = k * nx;
-    const float* src = L.in + (size_t)z0 * nx * ny;
-    float* dst = L.b + (size_t)z0 * nx * ny;
-    const int tid = threadIdx.x;
-    const Div dnx(nx), dny(ny), dR(R), dC(C);
-    xy_load_rows(src, nx, R, [&](int r, int x, float v) { buf[r * RS + 4 + x] = v; });
-    __syncthreads();
-    xy_fill_mirror(buf, R, RS, nx);
-    __syncthreads();
-    // x analysis: pair q of row r, consecutive lanes on consecutive rows
-    float lo[NIF], hi[NIF];
-#pragma unroll
-    for (int j = 0; j < NIF; j++) {
-        const int it = tid + XYT * j;
-        lo[j] = hi[j] = 0.0f;
-        if (it < R * mx) {
-            const int q = dR.q(it), r = it - q * R;
-            Fwd97::pair(&buf[r * RS], q, lo[j], hi[j]);
+            for (int h = 0; h < 8; h++) {
+                const int p = h < 4 ? h - 4 : t.nx + h - 4;
+                const int x = mirror1(p, t.nx);
+                if (x == 2 * t.k) {
+                    r0[4 + p] = v0.x;
+                    r1[4 + p] = v1.x;
+                } else if (x == 2 * t.k + 1) {
+                    r0[4 + p] = v0.y;
+                    r1[4 + p] = v1.y;
+                }
+            }
         }
     }
     __syncthreads();
-#pragma unroll
-    for (int j = 0; j < NIF; j++) {
-        const int it = tid + XYT * j;
-        if (it < R * mx) {
-            const int q = dR.q(it), r = it - q * R;
-            const int s = dny.q(r), y = r - s * ny, c = s * nx + q;
-            buf[c * CS + 4 + y] = lo[j];
-            buf[(c + mx) * CS + 4 + y] = hi[j];
-        }
-    }
-    __syncthreads();
-    xy_fill_mirror(buf, C, CS, ny);
-    __syncthreads();
-    // y analysis: pair q of column c, consecutive lanes on consecutive x (coalesced stores)
-    for (int it = tid; it < C * my; it += XYT) {
-        const int q = dC.q(it), c = it - q * C;
-        const int s = dnx.q(c), x = c - s * nx;
-        float l, h;
-        Fwd97::pair(&buf[c * CS], q, l, h);
-        gst(dst, (s * ny + q) * nx + x, l);
-        gst(dst, (s * ny + my + q) * nx + x, h);
-    }
-}
-
-// Inverse y then x synthesis of zk slices: L.a -> L.out.
-__global__ __launch_bounds__(XYT) void k_binv_yx(const BrickLevel* lv, const uint32_t* tb, uint32_t nb) {
-    extern __shared__ float buf[];
-    const uint32_t bi = find_brick(tb, nb, blockIdx.x);
-    const BrickLevel L = lv[bi];
-    const int nx = L.lx, ny = L.ly, mx = nx / 2, my = ny / 2;
-    const int z0 = (int)(blockIdx.x - tb[bi]) * L.zk;
-    const int k = min(L.zk, L.lz - z0);
-    const int RS = nx + 9, CS = ny + 9, R = k * ny, C = k * nx;
-    const float* src = L.a + (size_t)z0 * nx * ny;
-    float* dst = L.out + (size_t)z0 * nx * ny;
-    const int tid = threadIdx.x;
-    const Div dnx(nx), dny(ny), dC(C);
-    // columns, interleaved: row y of a slice holds low (y < my) or high coefficient y - my
-    xy_load_rows(src, nx, R, [&](int r, int x, float v) {
-        const int s = dny.q(r), y = r - s * ny;
-        const int w = y < my ? 2 * y : 2 * (y - my) + 1;
-        buf[(s * nx + x) * CS + 4 + w] = v;
-    });
-    __syncthreads();
-    xy_fill_inter(buf, C, CS, my);
-    __syncthreads();
-    // y synthesis: outputs 2 u and 2 u + 1 of column c from one 10-entry window, consecutive lanes
-    // on consecutive columns
-    float o0[NII], o1[NII];
-#pragma unroll
-    for (int j = 0; j < NII; j++) {
-        const int it = tid + XYT * j;
-        o0[j] = o1[j] = 0.0f;
-        if (it < C * my) {
-            const int u = dC.q(it), c = it - u * C;
-            float w[10];
-#pragma unroll
-            for (int i = 0; i < 10; i++) w[i] = buf[c * CS + 2 * u + i];
-            o0[j] = Inv97::out(w, 0);
-            o1[j] = Inv97::out(w + 1, 1);
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < NII; j++) {
-        const int it = tid + XYT * j;
-        if (it < C * my) {
-            const int u = dC.q(it), c = it - u * C;
-            const int s = dnx.q(c), x = c - s * nx;
-            const int w = x < mx ? 2 * x : 2 * (x - mx) + 1;
-            const int r = s * ny + 2 * u;
-            buf[r * RS + 4 + w] = o0[j];
-            buf[(r + 1) * RS + 4 + w] = o1[j];
-        }
-    }
-    __syncthreads();
-    xy_fill_inter(buf, R, RS, mx);
-    __syncthreads();
-    // x synthesis: outputs 2 u and 2 u + 1 of row r, plane elements 2 it and 2 it + 1
-    const Div dmx(mx);
-    const __amdgpu_buffer_rsrc_t rdst = buffer_of(dst, (uint32_t)(R * nx) * 4u);
-    for (int it = tid; it < R * mx; it += XYT) {
-        const int r = dmx.q(it), u = it - r * mx;
+    if (t.act) {
         float w[10];
 #pragma unroll
-        for (int i = 0; i < 10; i++) w[i] = buf[r * RS + 2 * u + i];
-        const float a = Inv97::out(w, 0), b = Inv97::out(w + 1, 1);
-        // outputs 2 it and 2 it + 1 are adjacent: one 8-byte store
-        typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-        const u32x2 ab = {__float_as_uint(a), __float_as_uint(b)};
-        __builtin_amdgcn_raw_buffer_store_b64(ab, rdst, (int)((uint32_t)it * 8u), 0, 0);
+        for (int j = 0; j < 5; j++) lds_get2(w + 2 * j, r0 + 2 * t.k + 2 * j);
+        Fwd97::pair(w, 0, lo[0], hi[0]);
+#pragma unroll
+        for (int j = 0; j < 5; j++) lds_get2(w + 2 * j, r1 + 2 * t.k + 2 * j);
+        Fwd97::pair(w, 0, lo[1], hi[1]);
     }
+}
+
+// Forward x then y analysis: L.in -> L.b.
+__global__ __launch_bounds__(XYT) void k_bfwd_xy(const BrickLevel* lv, const uint32_t* tb, uint32_t nb) {
+    __shared__ float st[4 * XRS];
+    const uint32_t bi = find_brick(tb, nb, blockIdx.x);
+    const BrickLevel L = lv[bi];
+    const XYTile t(L, blockIdx.x - tb[bi]);
+    const int nx = t.nx, ny = t.ny, my = t.my;
+    const float* src = L.in + t.plane0 + 2 * t.k;
+    float* dst = L.b + t.plane0 + t.k;
+    auto row = [&](int p) -> float2 {  // positions 2k, 2k + 1 of row mirror(p)
+        const float* q = src + mirror(p, ny) * nx;
+        return make_float2(gld(q, 0), gld(q, 1));
+    };
+    float wl[10], wh[10];  // y windows of the columns k and mx + k: rows 2q - 4 .. 2q + 5
+    int b = 0;
+    {
+        float2 v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = row(2 * t.k0 - 4 + j);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            float lo[2], hi[2];
+            xy_fwd_ingest(st, t, b, v[2 * j], v[2 * j + 1], lo, hi);
+            b ^= 1;
+            wl[2 * j] = lo[0];
+            wl[2 * j + 1] = lo[1];
+            wh[2 * j] = hi[0];
+            wh[2 * j + 1] = hi[1];
+        }
+    }
+    auto step = [&](int q, float2 v0, float2 v1) {
+        float lo[2], hi[2];
+        xy_fwd_ingest(st, t, b, v0, v1, lo, hi);
+        b ^= 1;
+        wl[8] = lo[0];
+        wl[9] = lo[1];
+        wh[8] = hi[0];
+        wh[9] = hi[1];
+        if (t.act) {
+            float l0, h0, l1, h1;
+            Fwd97::pair(wl, 0, l0, h0);
+            Fwd97::pair(wh, 0, l1, h1);
+            gst(dst, q * nx, l0);
+            gst(dst, q * nx + t.mx, l1);
+            gst(dst, (my + q) * nx, h0);
+            gst(dst, (my + q) * nx + t.mx, h1);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            wl[i] = wl[i + 2];
+            wh[i] = wh[i + 2];
+        }
+    };
+    int q = t.k0;
+    for (; q + XU <= t.k1; q += XU) {
+        float2 v[2 * XU];
+#pragma unroll
+        for (int j = 0; j < 2 * XU; j++) v[j] = row(2 * q + 4 + j);
+#pragma unroll
+        for (int u = 0; u < XU; u++) step(q + u, v[2 * u], v[2 * u + 1]);
+    }
+    for (; q < t.k1; q++) step(q, row(2 * q + 4), row(2 * q + 5));
+}
+
+// Inverse: stage the y-synthesized rows 2u (a0: low and high x coefficient k) and 2u + 1 (a1) into
+// buffer b in interleaved order, then the x synthesis of output pair k of both rows to dst rows.
+__device__ __forceinline__ void xy_inv_ingest(float* st, const XYTile& t, int b, float2 a0, float2 a1, float* d0,
+                                              float* d1) {
+    float* r0 = st + (2 * b) * XRS + t.row0;
+    float* r1 = r0 + XRS;
+    if (t.act) {
+        lds_put2(r0 + 4 + 2 * t.k, a0);
+        lds_put2(r1 + 4 + 2 * t.k, a1);
+        if (t.k < 3 || t.k >= t.mx - 3) {
+            // pairs -2, -1, mx, mx + 1: low coefficient ext_low, high coefficient ext_high
+#pragma unroll
+            for (int h = 0; h < 4; h++) {
+                const int g = h < 2 ? h - 2 : t.mx + h - 2;
+                if (ext_low1(g, t.mx) == t.k) {
+                    r0[4 + 2 * g] = a0.x;
+                    r1[4 + 2 * g] = a1.x;
+                }
+                if (ext_high1(g, t.mx) == t.k) {
+                    r0[4 + 2 * g + 1] = a0.y;
+                    r1[4 + 2 * g + 1] = a1.y;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (t.act) {
+        float w[10];
+#pragma unroll
+        for (int j = 0; j < 5; j++) lds_get2(w + 2 * j, r0 + 2 * t.k + 2 * j);
+        gst(d0, 0, Inv97::out(w, 0));
+        gst(d0, 1, Inv97::out(w + 1, 1));
+#pragma unroll
+        for (int j = 0; j < 5; j++) lds_get2(w + 2 * j, r1 + 2 * t.k + 2 * j);
+        gst(d1, 0, Inv97::out(w, 0));
+        gst(d1, 1, Inv97::out(w + 1, 1));
+    }
+}
+
+// Inverse y then x synthesis: L.a -> L.out.
+__global__ __launch_bounds__(XYT) void k_binv_yx(const BrickLevel* lv, const uint32_t* tb, uint32_t nb) {
+    __shared__ float st[4 * XRS];
+    const uint32_t bi = find_brick(tb, nb, blockIdx.x);
+    const BrickLevel L = lv[bi];
+    const XYTile t(L, blockIdx.x - tb[bi]);
+    const int nx = t.nx, my = t.my;
+    const float* src = L.a + t.plane0 + t.k;
+    float* dst = L.out + t.plane0 + 2 * t.k;
+    // low (row ext_low(g)) and high (row my + ext_high(g)) y coefficient g of the columns k, mx + k
+    auto lo_row = [&](int g) -> float2 {
+        const float* q = src + ext_low(g, my) * nx;
+        return make_float2(gld(q, 0), gld(q, t.mx));
+    };
+    auto hi_row = [&](int g) -> float2 {
+        const float* q = src + (my + ext_high(g, my)) * nx;
+        return make_float2(gld(q, 0), gld(q, t.mx));
+    };
+    float wl[10], wh[10];  // interleaved y pairs u - 2 .. u + 2 of the columns k and mx + k
+    {
+        float2 v[8];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            v[2 * j] = lo_row(t.k0 - 2 + j);
+            v[2 * j + 1] = hi_row(t.k0 - 2 + j);
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            wl[j] = v[j].x;
+            wh[j] = v[j].y;
+        }
+    }
+    int b = 0;
+    auto step = [&](int u, float2 vl, float2 vh) {
+        wl[8] = vl.x;
+        wl[9] = vh.x;
+        wh[8] = vl.y;
+        wh[9] = vh.y;
+        const float2 a0 = make_float2(Inv97::out(wl, 0), Inv97::out(wh, 0));
+        const float2 a1 = make_float2(Inv97::out(wl + 1, 1), Inv97::out(wh + 1, 1));
+        xy_inv_ingest(st, t, b, a0, a1, dst + 2 * u * nx, dst + (2 * u + 1) * nx);
+        b ^= 1;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            wl[i] = wl[i + 2];
+            wh[i] = wh[i + 2];
+        }
+    };
+    int u = t.k0;
+    for (; u + XU <= t.k1; u += XU) {
+        float2 v[2 * XU];
+#pragma unroll
+        for (int j = 0; j < XU; j++) {
+            v[2 * j] = lo_row(u + 2 + j);
+            v[2 * j + 1] = hi_row(u + 2 + j);
+        }
+#pragma unroll
+        for (int j = 0; j < XU; j++) step(u + j, v[2 * j], v[2 * j + 1]);
+    }
+    for (; u < t.k1; u++) step(u, lo_row(u + 2), hi_row(u + 2));
 }
 
 }  // namespace
@@ -711,7 +741,7 @@ void bricks_forward_level(const BrickLevel* d_levels, const uint32_t* d_tiles, u
     const uint32_t* tz = d_tiles + 2 * (nbricks + 1);
     const uint32_t* txy = d_tiles + 3 * (nbricks + 1);
     if (h_tiles_total[3]) {
-        CCA_LAUNCH("bricks_fwd_xy", level_elems * 8, k_bfwd_xy, dim3(h_tiles_total[3]), dim3(XYT), XYB * sizeof(float),
+        CCA_LAUNCH("bricks_fwd_xy", level_elems * 8, k_bfwd_xy, dim3(h_tiles_total[3]), dim3(XYT), 0,
                    st, d_levels, txy, nbricks);
     } else {
         if (h_tiles_total[0]) CCA_LAUNCH("bricks_fwd_x", level_elems * 8, k_bfwd<0>, dim3(h_tiles_total[0]), dim3(256), 0, st, d_levels, tx, nbricks);
@@ -743,7 +773,7 @@ void bricks_inverse_level(const BrickLevel* d_levels, const uint32_t* d_tiles, u
     }
     CCA_LAUNCH_CHECK();
     if (h_tiles_total[3]) {
-        CCA_LAUNCH("bricks_inv_yx", level_elems * 8, k_binv_yx, dim3(h_tiles_total[3]), dim3(XYT), XYB * sizeof(float),
+        CCA_LAUNCH("bricks_inv_yx", level_elems * 8, k_binv_yx, dim3(h_tiles_total[3]), dim3(XYT), 0,
                    st, d_levels, d_tiles + 3 * (nbricks + 1), nbricks);
     } else {
         if (h_tiles_total[1]) CCA_LAUNCH("bricks_inv_y", level_elems * 8, k_binv<1>, dim3(h_tiles_total[1]), dim3(256), 0, st, d_levels, ty, nbricks);

@@ -105,12 +105,10 @@ void upload_levels(BrickCtx& ctx, BrickJob& job, float* work, unsigned char* sym
     uint32_t* ht = (uint32_t*)(h + tb_off);
     job.tiles_total.assign(4 * L, 0);
     for (uint32_t l = 0; l < L; l++) {
-        // the fused x+y passes serve a level when every brick's z-slice fits a workgroup's LDS
-        // (and the tuning keeps them on)
+        // the fused x+y passes serve a level when every brick's rows fit them (and the tuning
+        // keeps them on)
         bool fused = tune.fusedXY != 0;
-        for (uint32_t i = 0; i < n; i++)
-            fused = fused && brick_xy_floats((int)(job.bricks[i].sizeX >> l), (int)(job.bricks[i].sizeY >> l)) <=
-                                 (uint32_t)kBrickXYFloats;
+        for (uint32_t i = 0; i < n; i++) fused = fused && brick_xy_fits((int)(job.bricks[i].sizeX >> l));
         uint32_t acc[4] = {0, 0, 0, 0};
         for (uint32_t i = 0; i < n; i++) {
             const cc_brick& b = job.bricks[i];
@@ -133,12 +131,12 @@ void upload_levels(BrickCtx& ctx, BrickJob& job, float* work, unsigned char* sym
             e.inv_hp = 1.0f / e.step_hp;      // Quantize.cu:94
             e.step_lp = q0 / float(1u << (L - 1));
             e.inv_lp = 1.0f / e.step_lp;
-            uint32_t zk = std::min<uint32_t>((uint32_t)e.lz, std::max<uint32_t>(1u, kBrickXYFloats / brick_xy_floats(e.lx, e.ly)));
-            if (tune.xySlicesMax) zk = std::min(zk, tune.xySlicesMax);
-            e.zk = fused ? (int)zk : 0;
+            uint32_t zk = fused ? std::min<uint32_t>((uint32_t)e.lz, brick_xy_slices(e.lx)) : 0u;
+            if (fused && tune.xySlicesMax) zk = std::min(zk, tune.xySlicesMax);
+            e.zk = (int)zk;
             for (int ax = 0; ax < 4; ax++) {
                 ht[((size_t)l * 4 + ax) * (n + 1) + i] = acc[ax];
-                acc[ax] += ax < 3 ? brick_tiles(ax, e.lx, e.ly, e.lz) : (fused ? div_up(e.lz, e.zk) : 0u);
+                acc[ax] += ax < 3 ? brick_tiles(ax, e.lx, e.ly, e.lz) : (fused ? brick_xy_tiles(e.ly, e.lz, e.zk) : 0u);
             }
         }
         for (int ax = 0; ax < 4; ax++) {

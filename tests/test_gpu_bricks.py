@@ -37,12 +37,12 @@ def field():
     return S.gaussian_field((96, 96, 96), 4)
 
 
-@pytest.mark.parametrize("levels", [1, 2, 3])
-def test_bricks_equal_single_volume_calls(cc, torch_gpu, field, levels):
-    log("bricks levels", levels)
+# (levels, brick axes): the last case has odd half-sizes on level 1 (x pairs 5, 7, 9, 11)
+@pytest.mark.parametrize("levels,axes", [(1, AXES), (2, AXES), (3, (16, 32, 48)), (2, (20, 28, 36, 44))])
+def test_bricks_equal_single_volume_calls(cc, torch_gpu, field, levels, axes):
+    log("bricks levels", levels, "axes", axes)
     torch = torch_gpu
-    rng = np.random.default_rng(levels)
-    axes = AXES if levels < 3 else (16, 32, 48)
+    rng = np.random.default_rng(levels + len(axes) * (axes[0] % 16))
     bricks = make_bricks(rng, field, 24, axes)
     std = float(field.std())
     steps = [float(rng.choice([0.005, 0.05, 0.5])) * std for _ in bricks]
