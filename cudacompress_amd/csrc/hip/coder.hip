@@ -24,7 +24,11 @@ using design::TableOut;
 using design::Work;
 
 constexpr uint32_t kDesignMid = 4096;
-constexpr uint32_t kWindowWords = kChunk + 4;  // LDS packing window (2048 pairs x 31 bits < 2048 words)
+// LDS packing window of each coded sequence, a ring indexed by output word mod kWindowWords: a chunk
+// of kChunk pairs of at most 31 bits spans at most 1985 words plus the partial word before it.
+constexpr uint32_t kWindowWords = 2048;
+static_assert(kWindowWords >= (kChunk * 31 + 31) / 32 + 2 && (kWindowWords & (kWindowWords - 1)) == 0,
+              "window ring");
 
 // ------------------------------------------------------------------------ helpers
 // Exclusive scans over a 256-thread workgroup.  sh: 4 entries of LDS.  All threads must call.
@@ -60,6 +64,55 @@ __device__ long long wg_excl_max_i64(long long v, long long* sh, long long& tota
     __syncthreads();
     total = tot;
     return base > ex ? base : ex;
+}
+// wg_excl_max_i64 without the closing barrier: sh must not be written again before the next
+// barrier (callers alternate two arrays).
+__device__ __forceinline__ long long wg_excl_max_i64_open(long long v, long long* sh, long long& total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const long long x = wave_incl_max_i64(v);
+    const long long ex = wave_shift_up_i64(x, -1);
+    if (lane == 63) sh[w] = x;
+    __syncthreads();
+    long long base = -1, tot = -1;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const long long s = sh[i];
+        if (i < w) base = base > s ? base : s;
+        tot = tot > s ? tot : s;
+    }
+    total = tot;
+    return base > ex ? base : ex;
+}
+// Exclusive sum of a and exclusive max of b (identity -1) over a 256-thread workgroup with one
+// barrier: sha and shb (4 entries each) must not be written again before the next barrier.
+__device__ __forceinline__ void wg_excl_sum_max(uint64_t a, long long b, uint64_t* sha, long long* shb,
+                                                uint64_t& ex_a, uint64_t& tot_a, long long& ex_b, long long& tot_b) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t xa = wave_incl_scan_u64(a);
+    const long long xb = wave_incl_max_i64(b);
+    const long long eb = wave_shift_up_i64(xb, -1);
+    if (lane == 63) {
+        sha[w] = xa;
+        shb[w] = xb;
+    }
+    __syncthreads();
+    uint64_t base_a = 0, ta = 0;
+    long long base_b = -1, tb = -1;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const uint64_t sa = sha[i];
+        const long long sb = shb[i];
+        if (i < w) {
+            base_a += sa;
+            base_b = base_b > sb ? base_b : sb;
+        }
+        ta += sa;
+        tb = tb > sb ? tb : sb;
+    }
+    ex_a = base_a + xa - a;
+    tot_a = ta;
+    ex_b = base_b > eb ? base_b : eb;
+    tot_b = tb;
 }
 __device__ uint32_t wg_sum_u32(uint32_t v, uint32_t* sh) {
     v = wave_sum(v);
@@ -198,7 +251,7 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_stats(const CodeItem* it
                                                               uint32_t* scratch, uint64_t cap, CoderState* st) {
     __shared__ uint32_t hs[kStatBins];
     __shared__ uint32_t hz[256];
-    __shared__ long long shl[4];
+    __shared__ long long shl[2][4];  // alternate chunks: one barrier per chunk
     __shared__ uint32_t shu[4];
     __shared__ uint64_t sh_off[2];
     __shared__ uint64_t shq[4];
@@ -222,14 +275,15 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_stats(const CodeItem* it
     };
     uint32_t vn[kPerThread];
     load8(in, (uint64_t)threadIdx.x * kPerThread, n, vn);
-    for (uint64_t base = 0; base < n; base += kChunk) {
+    uint32_t par = 0;
+    for (uint64_t base = 0; base < n; base += kChunk, par ^= 1) {
         const uint64_t p0 = base + (uint64_t)threadIdx.x * kPerThread;
         uint32_t v[kPerThread];
 #pragma unroll
         for (int j = 0; j < (int)kPerThread; j++) v[j] = vn[j];
         if (base + kChunk < n) load8(in, p0 + kChunk, n, vn);  // next step's symbols in flight
         long long chunk_max;
-        long long prev = wg_excl_max_i64(thread_last_nz(v, p0, n), shl, chunk_max);
+        long long prev = wg_excl_max_i64_open(thread_last_nz(v, p0, n), shl[par], chunk_max);
         prev = prev > carry ? prev : carry;
         thread_pairs(v, p0, n, rl, prev, (long long)n, [&](uint32_t s, uint32_t z) {
             cnt++;
@@ -870,23 +924,25 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_pack(const CodeItem* ite
             stage_table(zc, zc + sz.smax + 1, sz.smax + 1, cz, lz);
         }
         for (uint32_t i = threadIdx.x; i < 2 * kWindowWords; i += kCoderThreads) (&win[0][0])[i] = 0;
-        __syncthreads();
         const bool single = ss.nlen == 0;  // one distinct symbol: no codeword bits
         const bool zsingle = rl && sz.nlen == 0;
         const long long limit = (long long)ss.last_nz1 - 1;  // last nonzero
-        long long carry = -1;
+        constexpr uint32_t WM = kWindowWords - 1;
+        // One barrier per chunk besides the one before the flush: the exclusive max of the last
+        // nonzeros (zero-run lengths) of the next chunk rides on this chunk's pair/bit-count scan,
+        // so the symbols run two chunks ahead (v: this chunk, vn: the next, vnn: in flight).
+        uint32_t v[kPerThread], vn[kPerThread];
+        load8(in, (uint64_t)threadIdx.x * kPerThread, n, v);
+        load8(in, (uint64_t)threadIdx.x * kPerThread + kChunk, n, vn);
+        long long prevx, chunk_max;  // this chunk: exclusive max of the last nonzeros, its total
+        prevx = wg_excl_max_i64(thread_last_nz(v, (uint64_t)threadIdx.x * kPerThread, n), shl, chunk_max);
+        long long carry = -1;  // last nonzero before this chunk
         uint64_t pairs = 0, cbits_s = 0, cbits_z = 0;
-        uint32_t vn[kPerThread];
-        load8(in, (uint64_t)threadIdx.x * kPerThread, n, vn);
         for (uint64_t base = 0; base < n && pairs < M; base += kChunk) {
             const uint64_t p0 = base + (uint64_t)threadIdx.x * kPerThread;
-            uint32_t v[kPerThread];
-#pragma unroll
-            for (int j = 0; j < (int)kPerThread; j++) v[j] = vn[j];
-            if (base + kChunk < n) load8(in, p0 + kChunk, n, vn);  // next step's symbols in flight
-            long long chunk_max;
-            long long prev = wg_excl_max_i64(thread_last_nz(v, p0, n), shl, chunk_max);
-            prev = prev > carry ? prev : carry;
+            uint32_t vnn[kPerThread];
+            if (base + 2 * kChunk < n) load8(in, p0 + 2 * kChunk, n, vnn);
+            const long long prev = prevx > carry ? prevx : carry;
             // this thread's pairs; code lengths looked up once for both the scan and the packing
             uint32_t zc[kPerThread], ls[kPerThread], lzs[kPerThread];
             const uint32_t emit = thread_flags(v, p0, n, rl, prev, rl ? limit : (long long)n, zc);
@@ -903,11 +959,13 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_pack(const CodeItem* ite
                 nbz += m;
             }
             const uint32_t np = __popc(emit);
-            uint64_t tot;
-            const uint64_t ex = wg_excl_sum_u64(((uint64_t)np << 40) | ((uint64_t)nbs << 20) | nbz, shq, tot);
+            uint64_t ex, tot;
+            long long prevx_n, chunk_max_n;
+            const long long lnz_n = base + kChunk < n ? thread_last_nz(vn, p0 + kChunk, n) : -1ll;
+            wg_excl_sum_max(((uint64_t)np << 40) | ((uint64_t)nbs << 20) | nbz, lnz_n, shq, shl, ex, tot, prevx_n,
+                            chunk_max_n);
             const uint64_t os0 = cbits_s + ((ex >> 20) & 0xFFFFF);
             const uint64_t oz0 = cbits_z + (ex & 0xFFFFF);
-            const uint64_t wbs = cbits_s >> 5, wbz = cbits_z >> 5;
             if (np) {
                 // the coding-block start among this thread's pairs: a thread emits at most
                 // kPerThread < cbs pairs (cbs a power of two), so at most one, its k-th pair (pair
@@ -930,7 +988,7 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_pack(const CodeItem* ite
                 }
                 // codewords through a 64-bit accumulator, one LDS OR per window word touched
                 if (!single) {
-                    uint32_t wi = (uint32_t)((os0 >> 5) - wbs), nacc = (uint32_t)(os0 & 31);
+                    uint32_t wi = (uint32_t)(os0 >> 5), nacc = (uint32_t)(os0 & 31);
                     uint64_t acc = 0;
 #pragma unroll
                     for (int j = 0; j < (int)kPerThread; j++) {
@@ -941,14 +999,14 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_pack(const CodeItem* ite
                         nacc += l;
                         if (nacc >= 32) {
                             nacc -= 32;
-                            if (wi < kWindowWords) atomicOr(&win[0][wi], (uint32_t)(acc >> nacc));
+                            atomicOr(&win[0][wi & WM], (uint32_t)(acc >> nacc));
                             wi++;
                         }
                     }
-                    if (nacc && wi < kWindowWords) atomicOr(&win[0][wi], (uint32_t)(acc << (32 - nacc)));
+                    if (nacc) atomicOr(&win[0][wi & WM], (uint32_t)(acc << (32 - nacc)));
                 }
                 if (rl && !zsingle) {
-                    uint32_t wi = (uint32_t)((oz0 >> 5) - wbz), nacc = (uint32_t)(oz0 & 31);
+                    uint32_t wi = (uint32_t)(oz0 >> 5), nacc = (uint32_t)(oz0 & 31);
                     uint64_t acc = 0;
 #pragma unroll
                     for (int j = 0; j < (int)kPerThread; j++) {
@@ -958,41 +1016,45 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_pack(const CodeItem* ite
                         nacc += l;
                         if (nacc >= 32) {
                             nacc -= 32;
-                            if (wi < kWindowWords) atomicOr(&win[1][wi], (uint32_t)(acc >> nacc));
+                            atomicOr(&win[1][wi & WM], (uint32_t)(acc >> nacc));
                             wi++;
                         }
                     }
-                    if (nacc && wi < kWindowWords) atomicOr(&win[1][wi], (uint32_t)(acc << (32 - nacc)));
+                    if (nacc) atomicOr(&win[1][wi & WM], (uint32_t)(acc << (32 - nacc)));
                 }
             }
             __syncthreads();
             pairs += tot >> 40;
             const uint64_t nend_s = cbits_s + ((tot >> 20) & 0xFFFFF), nend_z = cbits_z + (tot & 0xFFFFF);
-            // flush whole words, keep the partial last word at window[0]
+            // flush the whole words and clear their slots; the partial last word stays in the ring
+            // (the next chunk's barrier orders these clears before its ORs)
+            const uint64_t wbs = cbits_s >> 5, wbz = cbits_z >> 5;
             const uint32_t full_s = (uint32_t)((nend_s >> 5) - wbs), full_z = (uint32_t)((nend_z >> 5) - wbz);
-            for (uint32_t i = threadIdx.x; i < full_s; i += kCoderThreads)
-                if (wbs + i < nws) ws[wbs + i] = win[0][i];
-            if (rl)
-                for (uint32_t i = threadIdx.x; i < full_z; i += kCoderThreads)
-                    if (wbz + i < nwz) wz[wbz + i] = win[1][i];
-            __syncthreads();
-            const uint32_t keep_s = win[0][full_s], keep_z = win[1][full_z];
-            __syncthreads();
-            for (uint32_t i = threadIdx.x; i <= full_s + 1 && i < kWindowWords; i += kCoderThreads) win[0][i] = 0;
-            for (uint32_t i = threadIdx.x; i <= full_z + 1 && i < kWindowWords; i += kCoderThreads) win[1][i] = 0;
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                win[0][0] = keep_s;
-                win[1][0] = keep_z;
+            for (uint32_t i = threadIdx.x; i < full_s; i += kCoderThreads) {
+                const uint32_t r = (uint32_t)(wbs + i) & WM;
+                if (wbs + i < nws) ws[wbs + i] = win[0][r];
+                win[0][r] = 0;
             }
-            __syncthreads();
+            if (rl)
+                for (uint32_t i = threadIdx.x; i < full_z; i += kCoderThreads) {
+                    const uint32_t r = (uint32_t)(wbz + i) & WM;
+                    if (wbz + i < nwz) wz[wbz + i] = win[1][r];
+                    win[1][r] = 0;
+                }
             cbits_s = nend_s;
             cbits_z = nend_z;
             carry = chunk_max > carry ? chunk_max : carry;
+            prevx = prevx_n;
+            chunk_max = chunk_max_n;
+#pragma unroll
+            for (int j = 0; j < (int)kPerThread; j++) {
+                v[j] = vn[j];
+                vn[j] = vnn[j];
+            }
         }
-        if (threadIdx.x == 0) {
-            if ((cbits_s & 31) && (cbits_s >> 5) < nws) ws[cbits_s >> 5] = win[0][0];
-            if (rl && (cbits_z & 31) && (cbits_z >> 5) < nwz) wz[cbits_z >> 5] = win[1][0];
+        if (threadIdx.x == 0) {  // the partial last words (their ORs precede the last barrier)
+            if ((cbits_s & 31) && (cbits_s >> 5) < nws) ws[cbits_s >> 5] = win[0][(uint32_t)(cbits_s >> 5) & WM];
+            if (rl && (cbits_z & 31) && (cbits_z >> 5) < nwz) wz[cbits_z >> 5] = win[1][(uint32_t)(cbits_z >> 5) & WM];
         }
     }
     // coding-block offsets as packInc16 increments, from the absolute offsets the pairs recorded
