@@ -10,6 +10,8 @@
 // nothing).  This is the pair sequence of src/cudaCompress/RunLengthKernels.cui:5-44 (fillers
 // for runs >= 256, src/CPU/RunLengthCPU.cpp:8-32), with each filler attributed to the zero that
 // completes it, so a 2048-symbol step of a workgroup never yields more than 2048 pairs.
+#include <type_traits>
+
 #include "../coder.hpp"
 #include "../profile.hpp"
 #include "huff_design.hpp"
@@ -208,6 +210,83 @@ __device__ __forceinline__ long long thread_last_nz(const uint32_t (&v)[kPerThre
     return m ? (long long)p0 + (31 - __clz(m)) : -1ll;
 }
 
+// Chunk-relative forms for the chunk loops of the stats and pack passes.  A chunk of kChunk
+// positions starts at `base`; thread t holds positions base + t8 + j (t8 = kPerThread t), nvalid
+// of them inside the stream.  A position enters the pair rule only relative to the chunk or
+// through its low 8 bits (the zero-run length modulo 256), so per-thread arithmetic is 32-bit and
+// the 64-bit stream state (base, the last nonzero before the chunk) stays uniform.
+__device__ __forceinline__ uint32_t chunk_nvalid(uint64_t base, uint32_t n, uint32_t t8) {
+    const uint64_t rem = (uint64_t)n - base;
+    const uint32_t r = rem > (uint64_t)kChunk ? kChunk : (uint32_t)rem;
+    return r > t8 ? (r - t8 < kPerThread ? r - t8 : kPerThread) : 0u;
+}
+// the thread's last nonzero, chunk-relative (-1: none)
+__device__ __forceinline__ int thread_last_nz_rel(const uint32_t (&v)[kPerThread], uint32_t t8, uint32_t nvalid) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int j = 0; j < (int)kPerThread; j++) m |= (uint32_t)(v[j] != 0u && (uint32_t)j < nvalid) << j;
+    return m ? (int)(t8 + 31u - (uint32_t)__clz(m)) : -1;
+}
+// thread_flags with g0 = (first position - last nonzero before it - 1) mod 2^32 and
+// lim = limit - first position (clamped; only lim > j matters)
+__device__ __forceinline__ uint32_t thread_flags_rel(const uint32_t (&v)[kPerThread], uint32_t nvalid, bool rl,
+                                                     uint32_t g0, int lim, uint32_t (&zc)[kPerThread]) {
+    uint32_t g = g0, emit = 0;
+#pragma unroll
+    for (int j = 0; j < (int)kPerThread; j++) {
+        const bool nz = v[j] != 0u;
+        const uint32_t z = g & 255u;
+        const bool e = (uint32_t)j < nvalid && (!rl || nz || (z == 255u && j < lim));
+        g = nz ? 0u : g + 1u;
+        zc[j] = rl ? z : 0u;
+        emit |= (uint32_t)e << j;
+    }
+    return emit;
+}
+// Uniform per-chunk terms: gc = (base - carry - 1) mod 2^32 for the last nonzero `carry` before
+// the chunk (-1: none), limc = limit - base clamped to [-1, kChunk + kPerThread].
+__device__ __forceinline__ uint32_t chunk_gc(uint64_t base, long long carry) {
+    return (uint32_t)(base - (uint64_t)(carry + 1));
+}
+__device__ __forceinline__ int chunk_lim(uint64_t base, long long limit) {
+    const long long d = limit - (long long)base;
+    return d < -1 ? -1 : (d > (long long)(kChunk + kPerThread) ? (int)(kChunk + kPerThread) : (int)d);
+}
+// g0 of a thread from its exclusive chunk-relative last nonzero prel (-1: before the chunk)
+__device__ __forceinline__ uint32_t thread_g0(int prel, uint32_t t8, uint32_t gc) {
+    return prel >= 0 ? t8 - (uint32_t)prel - 1u : gc + t8;
+}
+
+// Exclusive last nonzero of the chunk's threads (chunk-relative positions grow with the thread
+// index, -1: none): in a wave a ballot and one lane read (the nearest lower lane holding a
+// nonzero), across the waves the entries of the waves before.  No barrier here: the caller writes
+// sh[wave] (the wave's last, `wlast`) before its barrier and combines after it (wg_last_combine).
+__device__ __forceinline__ int wave_excl_last(int b, int& wlast) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t m = __ballot(b >= 0);
+    const uint64_t below = m & ((1ull << lane) - 1ull);
+    const int got = __shfl(b, below ? 63 - __clzll(below) : 0);
+    wlast = m ? __builtin_amdgcn_readlane(b, 63 - __clzll(m)) : -1;
+    return below ? got : -1;
+}
+__device__ __forceinline__ void wg_last_combine(int ex, const int* sh, int& excl, int& total) {
+    const int w = threadIdx.x >> 6;
+    int base = -1, tot = -1;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int s = sh[i];
+        if (i < w) base = base > s ? base : s;
+        tot = tot > s ? tot : s;
+    }
+    excl = ex >= 0 ? ex : base;
+    total = __builtin_amdgcn_readfirstlane(tot);
+}
+// a uniform 64-bit value held in vector registers, moved to scalar ones
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+    return ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(v >> 32)) << 32) |
+           (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+}
+
 // ------------------------------------------------------------------------ kernels
 __global__ void k_coder_init(CoderState* st, uint32_t* classes) {
     if (threadIdx.x < kDesignListBase) classes[threadIdx.x] = 0;
@@ -251,7 +330,7 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_stats(const CodeItem* it
                                                               uint32_t* scratch, uint64_t cap, CoderState* st) {
     __shared__ uint32_t hs[kStatBins];
     __shared__ uint32_t hz[256];
-    __shared__ long long shl[2][4];  // alternate chunks: one barrier per chunk
+    __shared__ int shl[2][4];  // alternate chunks: one barrier per chunk
     __shared__ uint32_t shu[4];
     __shared__ uint64_t sh_off[2];
     __shared__ uint64_t shq[4];
@@ -273,19 +352,29 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_stats(const CodeItem* it
             else wide = 1;
         }
     };
+    const uint32_t t8 = threadIdx.x * kPerThread;
     uint32_t vn[kPerThread];
-    load8(in, (uint64_t)threadIdx.x * kPerThread, n, vn);
+    load8(in, (uint64_t)t8, n, vn);
     uint32_t par = 0;
     for (uint64_t base = 0; base < n; base += kChunk, par ^= 1) {
-        const uint64_t p0 = base + (uint64_t)threadIdx.x * kPerThread;
         uint32_t v[kPerThread];
 #pragma unroll
         for (int j = 0; j < (int)kPerThread; j++) v[j] = vn[j];
-        if (base + kChunk < n) load8(in, p0 + kChunk, n, vn);  // next step's symbols in flight
-        long long chunk_max;
-        long long prev = wg_excl_max_i64_open(thread_last_nz(v, p0, n), shl[par], chunk_max);
-        prev = prev > carry ? prev : carry;
-        thread_pairs(v, p0, n, rl, prev, (long long)n, [&](uint32_t s, uint32_t z) {
+        if (base + kChunk < n) load8(in, base + kChunk + t8, n, vn);  // next step's symbols in flight
+        const uint32_t nvalid = chunk_nvalid(base, n, t8);
+        int wl;
+        const int ex = wave_excl_last(thread_last_nz_rel(v, t8, nvalid), wl);
+        if ((threadIdx.x & 63) == 0) shl[par][threadIdx.x >> 6] = wl;
+        __syncthreads();
+        int prel, ctot;
+        wg_last_combine(ex, shl[par], prel, ctot);
+        uint32_t zc[kPerThread];
+        const uint32_t emit = thread_flags_rel(v, nvalid, rl, thread_g0(prel, t8, chunk_gc(base, carry)),
+                                               chunk_lim(base, (long long)n) - (int)t8, zc);
+#pragma unroll
+        for (int j = 0; j < (int)kPerThread; j++) {
+            if (!((emit >> j) & 1u)) continue;
+            const uint32_t s = v[j], z = zc[j];
             cnt++;
             vmax = max(vmax, s);
             if (s == run_sym) {
@@ -299,8 +388,8 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_stats(const CodeItem* it
                 if (z == 0) zc0++;
                 else atomicAdd(&hz[z], 1u);
             }
-        });
-        carry = chunk_max > carry ? chunk_max : carry;
+        }
+        if (ctot >= 0) carry = (long long)base + ctot;
     }
     flush_run();
     zc0 = wave_sum(zc0);
@@ -855,7 +944,7 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_pack(const CodeItem* ite
     __shared__ uint32_t cz[256];
     __shared__ unsigned char lz[256];
     __shared__ uint32_t win[2][kWindowWords];
-    __shared__ long long shl[4];
+    __shared__ int shl[4];
     __shared__ uint64_t shq[4];
     if (st->out_base + st->out_words > out_cap) return;  // the host reports the short output
     const CodeItem it = items[blockIdx.x];
@@ -926,136 +1015,183 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_pack(const CodeItem* ite
         for (uint32_t i = threadIdx.x; i < 2 * kWindowWords; i += kCoderThreads) (&win[0][0])[i] = 0;
         const bool single = ss.nlen == 0;  // one distinct symbol: no codeword bits
         const bool zsingle = rl && sz.nlen == 0;
-        const long long limit = (long long)ss.last_nz1 - 1;  // last nonzero
+        const long long limit = rl ? (long long)ss.last_nz1 - 1 : (long long)n;  // last nonzero
         constexpr uint32_t WM = kWindowWords - 1;
-        // One barrier per chunk besides the one before the flush: the exclusive max of the last
-        // nonzeros (zero-run lengths) of the next chunk rides on this chunk's pair/bit-count scan,
-        // so the symbols run two chunks ahead (v: this chunk, vn: the next, vnn: in flight).
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        const uint32_t t8 = threadIdx.x * kPerThread;
+        // One barrier per chunk besides the one before the flush: the exclusive last nonzeros of
+        // the next chunk (zero-run lengths) ride on this chunk's pair/bit-count scan, so the
+        // symbols run two chunks ahead (v: this chunk, vn: the next, vnn: in flight).
         uint32_t v[kPerThread], vn[kPerThread];
-        load8(in, (uint64_t)threadIdx.x * kPerThread, n, v);
-        load8(in, (uint64_t)threadIdx.x * kPerThread + kChunk, n, vn);
-        long long prevx, chunk_max;  // this chunk: exclusive max of the last nonzeros, its total
-        prevx = wg_excl_max_i64(thread_last_nz(v, (uint64_t)threadIdx.x * kPerThread, n), shl, chunk_max);
-        long long carry = -1;  // last nonzero before this chunk
-        uint64_t pairs = 0, cbits_s = 0, cbits_z = 0;
-        for (uint64_t base = 0; base < n && pairs < M; base += kChunk) {
-            const uint64_t p0 = base + (uint64_t)threadIdx.x * kPerThread;
-            uint32_t vnn[kPerThread];
-            if (base + 2 * kChunk < n) load8(in, p0 + 2 * kChunk, n, vnn);
-            const long long prev = prevx > carry ? prevx : carry;
-            // this thread's pairs; code lengths looked up once for both the scan and the packing
-            uint32_t zc[kPerThread], ls[kPerThread], lzs[kPerThread];
-            const uint32_t emit = thread_flags(v, p0, n, rl, prev, rl ? limit : (long long)n, zc);
-            uint32_t nbs = 0, nbz = 0;
-#pragma unroll
-            for (int j = 0; j < (int)kPerThread; j++) {
-                const bool e = (emit >> j) & 1u;
-                uint32_t l = 0, m = 0;
-                if (e && !single) l = sglob ? glen[v[j]] : (uint32_t)lsh[v[j]];
-                if (e && rl && !zsingle) m = lz[zc[j]];
-                ls[j] = l;
-                lzs[j] = m;
-                nbs += l;
-                nbz += m;
-            }
-            const uint32_t np = __popc(emit);
-            uint64_t ex, tot;
-            long long prevx_n, chunk_max_n;
-            const long long lnz_n = base + kChunk < n ? thread_last_nz(vn, p0 + kChunk, n) : -1ll;
-            wg_excl_sum_max(((uint64_t)np << 40) | ((uint64_t)nbs << 20) | nbz, lnz_n, shq, shl, ex, tot, prevx_n,
-                            chunk_max_n);
-            const uint64_t os0 = cbits_s + ((ex >> 20) & 0xFFFFF);
-            const uint64_t oz0 = cbits_z + (ex & 0xFFFFF);
-            if (np) {
-                // the coding-block start among this thread's pairs: a thread emits at most
-                // kPerThread < cbs pairs (cbs a power of two), so at most one, its k-th pair (pair
-                // indices < M only: stats and pack agree on M, the bound keeps the offset writes in
-                // their arrays regardless); its bit offsets add the lengths of the k pairs before it
-                const uint32_t pi0 = (uint32_t)(pairs + (ex >> 40));
-                const uint32_t k = (cbs - (pi0 & (cbs - 1))) & (cbs - 1);
-                if (k < np && pi0 + k < M) {
-                    uint32_t os = (uint32_t)os0, oz = (uint32_t)oz0, e = 0;
-#pragma unroll
-                    for (int j = 0; j < (int)kPerThread; j++) {
-                        const bool before = ((emit >> j) & 1u) && e < k;
-                        os += before ? ls[j] : 0u;
-                        oz += before ? lzs[j] : 0u;
-                        e += (emit >> j) & 1u;
-                    }
-                    const uint32_t blk = (pi0 + k) >> (31 - __clz(cbs));
-                    bs[blk] = os;
-                    if (rl) bz[blk] = oz;
-                }
-                // codewords through a 64-bit accumulator, one LDS OR per window word touched
-                if (!single) {
-                    uint32_t wi = (uint32_t)(os0 >> 5), nacc = (uint32_t)(os0 & 31);
-                    uint64_t acc = 0;
-#pragma unroll
-                    for (int j = 0; j < (int)kPerThread; j++) {
-                        const uint32_t l = ls[j];
-                        uint32_t c = 0;
-                        if (l) c = sglob ? gcode[v[j]] : cs[v[j]];
-                        acc = (acc << l) | c;
-                        nacc += l;
-                        if (nacc >= 32) {
-                            nacc -= 32;
-                            atomicOr(&win[0][wi & WM], (uint32_t)(acc >> nacc));
-                            wi++;
-                        }
-                    }
-                    if (nacc) atomicOr(&win[0][wi & WM], (uint32_t)(acc << (32 - nacc)));
-                }
-                if (rl && !zsingle) {
-                    uint32_t wi = (uint32_t)(oz0 >> 5), nacc = (uint32_t)(oz0 & 31);
-                    uint64_t acc = 0;
-#pragma unroll
-                    for (int j = 0; j < (int)kPerThread; j++) {
-                        const uint32_t l = lzs[j];
-                        const uint32_t c = l ? cz[zc[j]] : 0u;
-                        acc = (acc << l) | c;
-                        nacc += l;
-                        if (nacc >= 32) {
-                            nacc -= 32;
-                            atomicOr(&win[1][wi & WM], (uint32_t)(acc >> nacc));
-                            wi++;
-                        }
-                    }
-                    if (nacc) atomicOr(&win[1][wi & WM], (uint32_t)(acc << (32 - nacc)));
-                }
-            }
+        load8(in, (uint64_t)t8, n, v);
+        load8(in, (uint64_t)t8 + kChunk, n, vn);
+        int prel, ctot;  // this chunk: the exclusive chunk-relative last nonzero, the chunk's last
+        {
+            int wl;
+            const int ex = wave_excl_last(thread_last_nz_rel(v, t8, chunk_nvalid(0, n, t8)), wl);
+            if (lane == 0) shl[wv] = wl;
             __syncthreads();
-            pairs += tot >> 40;
-            const uint64_t nend_s = cbits_s + ((tot >> 20) & 0xFFFFF), nend_z = cbits_z + (tot & 0xFFFFF);
-            // flush the whole words and clear their slots; the partial last word stays in the ring
-            // (the next chunk's barrier orders these clears before its ORs)
-            const uint64_t wbs = cbits_s >> 5, wbz = cbits_z >> 5;
-            const uint32_t full_s = (uint32_t)((nend_s >> 5) - wbs), full_z = (uint32_t)((nend_z >> 5) - wbz);
-            for (uint32_t i = threadIdx.x; i < full_s; i += kCoderThreads) {
-                const uint32_t r = (uint32_t)(wbs + i) & WM;
-                if (wbs + i < nws) ws[wbs + i] = win[0][r];
-                win[0][r] = 0;
-            }
-            if (rl)
-                for (uint32_t i = threadIdx.x; i < full_z; i += kCoderThreads) {
-                    const uint32_t r = (uint32_t)(wbz + i) & WM;
-                    if (wbz + i < nwz) wz[wbz + i] = win[1][r];
-                    win[1][r] = 0;
-                }
-            cbits_s = nend_s;
-            cbits_z = nend_z;
-            carry = chunk_max > carry ? chunk_max : carry;
-            prevx = prevx_n;
-            chunk_max = chunk_max_n;
+            wg_last_combine(ex, shl, prel, ctot);
+            __syncthreads();  // shl is written again by the first chunk's scan
+        }
+        // the chunk loop specialised on where the code table lives (LDS, or global memory for
+        // symbols >= kStatBins)
+        auto chunks = [&](auto sg) {
+            constexpr bool SG = decltype(sg)::value;
+            long long carry = -1;  // last nonzero before this chunk
+            uint64_t pairs = 0, cbits_s = 0, cbits_z = 0;
+            for (uint64_t base = 0; base < n && pairs < M; base += kChunk) {
+                uint32_t vnn[kPerThread];
+                if (base + 2 * kChunk < n) load8(in, base + 2 * kChunk + t8, n, vnn);
+                const uint32_t nvalid = chunk_nvalid(base, n, t8);
+                // this thread's pairs; code lengths looked up once for both the scan and the packing
+                uint32_t zc[kPerThread], ls[kPerThread], lzs[kPerThread];
+                const uint32_t emit = thread_flags_rel(v, nvalid, rl, thread_g0(prel, t8, chunk_gc(base, carry)),
+                                                       chunk_lim(base, limit) - (int)t8, zc);
+                uint32_t nbs = 0, nbz = 0;
+                // table reads issued together, unconditionally (symbols and zero counts index inside
+                // the tables; a length is then zeroed where no pair is emitted)
 #pragma unroll
-            for (int j = 0; j < (int)kPerThread; j++) {
-                v[j] = vn[j];
-                vn[j] = vnn[j];
+                for (int j = 0; j < (int)kPerThread; j++) {
+                    if (SG) ls[j] = (((emit >> j) & 1u) && !single) ? glen[v[j]] : 0u;
+                    else ls[j] = lsh[v[j]];
+                    lzs[j] = lz[zc[j]];
+                }
+#pragma unroll
+                for (int j = 0; j < (int)kPerThread; j++) {
+                    const bool e = (emit >> j) & 1u;
+                    ls[j] = (e && !single) ? ls[j] : 0u;
+                    lzs[j] = (e && rl && !zsingle) ? lzs[j] : 0u;
+                    nbs += ls[j];
+                    nbz += lzs[j];
+                }
+                const uint32_t np = __popc(emit);
+                // pair/bit-count scan, and the next chunk's last nonzeros, over one barrier
+                const bool more = base + kChunk < n;
+                int wl;
+                const int exn =
+                    wave_excl_last(more ? thread_last_nz_rel(vn, t8, chunk_nvalid(base + kChunk, n, t8)) : -1, wl);
+                const uint64_t a = ((uint64_t)np << 40) | ((uint64_t)nbs << 20) | nbz;
+                const uint64_t xa = wave_incl_scan_u64(a);
+                if (lane == 63) shq[wv] = xa;
+                if (lane == 0) shl[wv] = wl;
+                __syncthreads();
+                uint64_t ba = 0, ta = 0;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const uint64_t sa = shq[i];
+                    if (i < wv) ba += sa;
+                    ta += sa;
+                }
+                const uint64_t ex = ba + xa - a;
+                const uint64_t tot = uniform_u64(ta);
+                int prel_n, ctot_n;
+                wg_last_combine(exn, shl, prel_n, ctot_n);
+                const uint32_t os0 = (uint32_t)cbits_s + (uint32_t)((ex >> 20) & 0xFFFFF);  // mod 2^32
+                const uint32_t oz0 = (uint32_t)cbits_z + (uint32_t)(ex & 0xFFFFF);
+                if (np) {
+                    // the coding-block start among this thread's pairs: a thread emits at most
+                    // kPerThread < cbs pairs (cbs a power of two), so at most one, its k-th pair (pair
+                    // indices < M only: stats and pack agree on M, the bound keeps the offset writes in
+                    // their arrays regardless); its bit offsets add the lengths of the k pairs before it
+                    const uint32_t pi0 = (uint32_t)pairs + (uint32_t)(ex >> 40);
+                    const uint32_t k = (cbs - (pi0 & (cbs - 1))) & (cbs - 1);
+                    if (k < np && pi0 + k < M) {
+                        uint32_t os = os0, oz = oz0, e = 0;
+#pragma unroll
+                        for (int j = 0; j < (int)kPerThread; j++) {
+                            const bool before = ((emit >> j) & 1u) && e < k;
+                            os += before ? ls[j] : 0u;
+                            oz += before ? lzs[j] : 0u;
+                            e += (emit >> j) & 1u;
+                        }
+                        const uint32_t blk = (pi0 + k) >> (31 - __clz(cbs));
+                        bs[blk] = os;
+                        if (rl) bz[blk] = oz;
+                    }
+                    // codewords through a 64-bit accumulator, one LDS OR per window word touched
+                    // (ring slot: output word mod kWindowWords)
+                    if (!single) {
+                        uint32_t wi = os0 >> 5, nacc = os0 & 31;
+                        uint64_t acc = 0;
+                        uint32_t cw[kPerThread];
+#pragma unroll
+                        for (int j = 0; j < (int)kPerThread; j++) {
+                            if (SG) cw[j] = ls[j] ? gcode[v[j]] : 0u;
+                            else cw[j] = cs[v[j]];
+                        }
+#pragma unroll
+                        for (int j = 0; j < (int)kPerThread; j++) {
+                            const uint32_t l = ls[j];
+                            const uint32_t c = l ? cw[j] : 0u;
+                            acc = (acc << l) | c;
+                            nacc += l;
+                            if (nacc >= 32) {
+                                nacc -= 32;
+                                atomicOr(&win[0][wi & WM], (uint32_t)(acc >> nacc));
+                                wi++;
+                            }
+                        }
+                        if (nacc) atomicOr(&win[0][wi & WM], (uint32_t)(acc << (32 - nacc)));
+                    }
+                    if (rl && !zsingle) {
+                        uint32_t wi = oz0 >> 5, nacc = oz0 & 31;
+                        uint64_t acc = 0;
+                        uint32_t cw[kPerThread];
+#pragma unroll
+                        for (int j = 0; j < (int)kPerThread; j++) cw[j] = cz[zc[j]];
+#pragma unroll
+                        for (int j = 0; j < (int)kPerThread; j++) {
+                            const uint32_t l = lzs[j];
+                            const uint32_t c = l ? cw[j] : 0u;
+                            acc = (acc << l) | c;
+                            nacc += l;
+                            if (nacc >= 32) {
+                                nacc -= 32;
+                                atomicOr(&win[1][wi & WM], (uint32_t)(acc >> nacc));
+                                wi++;
+                            }
+                        }
+                        if (nacc) atomicOr(&win[1][wi & WM], (uint32_t)(acc << (32 - nacc)));
+                    }
+                }
+                __syncthreads();
+                pairs += tot >> 40;
+                const uint64_t nend_s = cbits_s + ((tot >> 20) & 0xFFFFF), nend_z = cbits_z + (tot & 0xFFFFF);
+                // flush the whole words and clear their slots; the partial last word stays in the ring
+                // (the next chunk's barrier orders these clears before its ORs)
+                const uint64_t wbs = cbits_s >> 5, wbz = cbits_z >> 5;
+                const uint32_t full_s = (uint32_t)((nend_s >> 5) - wbs), full_z = (uint32_t)((nend_z >> 5) - wbz);
+                for (uint32_t i = threadIdx.x; i < full_s; i += kCoderThreads) {
+                    const uint32_t r = (uint32_t)(wbs + i) & WM;
+                    if (wbs + i < nws) ws[wbs + i] = win[0][r];
+                    win[0][r] = 0;
+                }
+                if (rl)
+                    for (uint32_t i = threadIdx.x; i < full_z; i += kCoderThreads) {
+                        const uint32_t r = (uint32_t)(wbz + i) & WM;
+                        if (wbz + i < nwz) wz[wbz + i] = win[1][r];
+                        win[1][r] = 0;
+                    }
+                cbits_s = nend_s;
+                cbits_z = nend_z;
+                if (ctot >= 0) carry = (long long)base + ctot;
+                prel = prel_n;
+                ctot = ctot_n;
+#pragma unroll
+                for (int j = 0; j < (int)kPerThread; j++) {
+                    v[j] = vn[j];
+                    vn[j] = vnn[j];
+                }
             }
-        }
-        if (threadIdx.x == 0) {  // the partial last words (their ORs precede the last barrier)
-            if ((cbits_s & 31) && (cbits_s >> 5) < nws) ws[cbits_s >> 5] = win[0][(uint32_t)(cbits_s >> 5) & WM];
-            if (rl && (cbits_z & 31) && (cbits_z >> 5) < nwz) wz[cbits_z >> 5] = win[1][(uint32_t)(cbits_z >> 5) & WM];
-        }
+            if (threadIdx.x == 0) {  // the partial last words (their ORs precede the last barrier)
+                if ((cbits_s & 31) && (cbits_s >> 5) < nws) ws[cbits_s >> 5] = win[0][(uint32_t)(cbits_s >> 5) & WM];
+                if (rl && (cbits_z & 31) && (cbits_z >> 5) < nwz)
+                    wz[cbits_z >> 5] = win[1][(uint32_t)(cbits_z >> 5) & WM];
+            }
+        };
+        if (sglob) chunks(std::true_type{});
+        else chunks(std::false_type{});
     }
     // coding-block offsets as packInc16 increments, from the absolute offsets the pairs recorded
     __threadfence_block();
