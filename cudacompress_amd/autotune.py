@@ -2,8 +2,8 @@
 
 The brick pipeline has shape choices that do not change a bitstream but do change the time of a
 batch: the largest tables designed one per thread instead of one per wave, the resident workgroups
-of the four LDS table-design classes, whether levels whose z-slices fit the LDS run the fused x+y
-passes, how many z-slices one fused workgroup holds, and -- the one choice that does change the
+of the four LDS table-design classes, whether levels whose rows fit them run the fused x+y
+passes, how many z-slices one fused workgroup serves at most, and -- the one choice that does change the
 words -- the coding block size.  The reference fixes all of them
 (32x4-thread DWT blocks, src/cudaCompress/util/DWTFloat.cu:92-100; coding block 128,
 src/cudaCompress/InstanceImpl.cpp:65-69).  This module times candidates on the device with hipGraph
@@ -12,7 +12,7 @@ plans of a brick batch and keeps the fastest, by coordinate descent from the cur
   1. tables designed one per thread: up to 0, 32, 64, 128 symbols;
   2. each design class's grid: x1/2 and x2 of the current value;
   3. fused x+y passes on / off;
-  4. z-slices per fused workgroup: all that fit, 1, 2, 4;
+  4. z-slices per fused workgroup: all whose x pairs fit its threads, at most 1, 2, 4;
   5. coding block size 32, 64, 128, 256 (a new instance each).
 
 Every candidate must give the words of the starting configuration at the same coding block size

@@ -459,9 +459,10 @@ CC_API void cc_server_destroy(cc_server* server);
 typedef struct cc_tuning {
     uint32_t designGrid[4];   /* resident workgroups of the LDS table-design classes (tables of at most
                                  256, 1024, 2048, 4096 distinct symbols) */
-    uint32_t fusedXY;         /* 1: brick levels whose z-slices fit the LDS use the fused x+y passes;
-                                 0: separate x and y passes everywhere */
-    uint32_t xySlicesMax;     /* most z-slices one fused x+y workgroup holds (0: as many as fit) */
+    uint32_t fusedXY;         /* 1: brick levels whose rows have 8..512 samples use the fused x+y
+                                 passes; 0: separate x and y passes everywhere */
+    uint32_t xySlicesMax;     /* most z-slices one fused x+y workgroup serves (0: as many as its
+                                 threads cover, one thread per x pair) */
     uint32_t codingBlockSize; /* recommended coding block size (32..256) */
     uint32_t laneDesignK;     /* tables of at most this many distinct symbols (0, 32, 64, 128 or 256;
                                  0: none) are designed one per thread, the others one per wave */
