@@ -10,6 +10,7 @@
 #include "../profile.hpp"
 #include "dwt_ops.hpp"
 #include "quant_ops.hpp"
+#include "wave.hpp"
 
 namespace cca {
 namespace {
@@ -282,6 +283,8 @@ __global__ __launch_bounds__(256) void k_bfwd(const BrickLevel* lv, const uint32
 // brick_tiles along z.  The window and the fmaf order are those of the tiled passes
 // (Fwd97::pair over positions 2q - 4 .. 2q + 5, Inv97::out over interleaved pairs u - 2 .. u + 2),
 // so the results are identical.
+// Workgroups are renumbered XCD by XCD (xcd_order): the waves of the four quarters of the same rows
+// share their lines of the dense plane, and neighbours in logical order then share an L2.
 constexpr int ZU = 8;  // output pairs per batch of loads (2 ZU loads in flight per thread)
 
 // The column of a z-pass thread.
@@ -368,12 +371,13 @@ __device__ __forceinline__ void zfwd_column(const ZColumn& zc, const float* src,
 // coarsest level) and the highpass symbol bands.
 template <class S>
 __global__ __launch_bounds__(NC) void k_bfwd_zq(const BrickLevel* lv, const uint32_t* tb, uint32_t nb) {
-    const uint32_t i = find_brick(tb, nb, blockIdx.x);
+    const uint32_t w = xcd_order(blockIdx.x, gridDim.x);
+    const uint32_t i = find_brick(tb, nb, w);
     const BrickLevel L = lv[i];
     const Geo<2> g(L.lx, L.ly, L.lz);
     const int hx = L.lx / 2, hy = L.ly / 2, hxy = hx * hy;
     const uint32_t band = (uint32_t)hxy * (uint32_t)g.m;
-    const ZColumn zc(g, blockIdx.x - tb[i], hx, hy);
+    const ZColumn zc(g, w - tb[i], hx, hy);
     if (zc.q < 0) return;
     S* hp = static_cast<S*>(L.hp);
     const uint32_t ihi = (uint32_t)(zc.q + 3) * band + zc.r;
@@ -448,12 +452,13 @@ __device__ __forceinline__ void zinv_column(const ZColumn& zc, const float* lo_f
 // the level above, or symbols on the coarsest level): -> L.a.
 template <class S>
 __global__ __launch_bounds__(NC) void k_binv_zs(const BrickLevel* lv, const uint32_t* tb, uint32_t nb) {
-    const uint32_t i = find_brick(tb, nb, blockIdx.x);
+    const uint32_t w = xcd_order(blockIdx.x, gridDim.x);
+    const uint32_t i = find_brick(tb, nb, w);
     const BrickLevel L = lv[i];
     const Geo<2> g(L.lx, L.ly, L.lz);
     const int hx = L.lx / 2, hy = L.ly / 2, hxy = hx * hy;
     const uint32_t band = (uint32_t)hxy * (uint32_t)g.m;
-    const ZColumn zc(g, blockIdx.x - tb[i], hx, hy);
+    const ZColumn zc(g, w - tb[i], hx, hy);
     if (zc.q < 0) return;
     const S* hp = static_cast<const S*>(L.hp);
     const uint32_t ihi = (uint32_t)(zc.q + 3) * band + zc.r;
