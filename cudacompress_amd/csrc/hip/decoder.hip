@@ -53,42 +53,104 @@ __global__ void k_dec_init(CoderState* st) {
     }
 }
 
-// One thread per call: walks the sections (their sizes are in-band) and records every coded
-// sequence's header, codeword and offset positions; carves the arena for block offsets and for
-// the decoded values of RLE sequences.
-// index_mode: a resident decode index (tables and block offsets only): RLE sequences get no
-// decode buffer, a bound decode supplies one per request (k_dec_bind).
-__global__ void k_dec_parse(const DecCall* calls, uint32_t ncalls, const DecItem* items, DecSeq* seqs, CoderParams p,
-                            int symSize, uint32_t* scratch, uint64_t cap, CoderState* st, DecCallOut* outc,
-                            int index_mode) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per call.  The sections of a call are walked in order (their sizes are in-band); each
+// section's head is read as one 64-word window by the wave's lanes (lane i: word i of the window),
+// and the fields come out of the window with lane reads, so a section costs one coalesced load (two
+// when its bit-count word lies past the window) instead of a chain of dependent word reads.  The
+// call's arena ranges are taken after the walk with one atomic per 64 sequences: lane k holds the
+// k-th sequence's range sizes, a wave scan places them (block offsets, lookup and symbol table,
+// and the decoded values of an RLE sequence).  index_mode: a resident decode index (tables and
+// block offsets only): RLE sequences get no decode buffer, a bound decode supplies one per request
+// (k_dec_bind).
+struct ParseWin {
+    uint64_t base;  // stream word of lane 0
+    uint32_t v;     // this lane's word (0 past the stream)
+};
+__device__ __forceinline__ void parse_load(ParseWin& W, const DecCall& C, uint64_t base) {
+    const uint64_t i = base + (uint64_t)threadIdx.x;
+    W.base = base;
+    W.v = i < C.nwords ? C.words[i] : 0u;
+}
+// stream word i (i < C.nwords): from the window, else from memory
+__device__ __forceinline__ uint32_t parse_word(const ParseWin& W, const DecCall& C, uint64_t i) {
+    const uint64_t d = i - W.base;
+    if (i >= W.base && d < 64) return (uint32_t)__builtin_amdgcn_readlane((int)W.v, (int)d);
+    return C.words[i];
+}
+// rd_bits over the window
+__device__ __forceinline__ uint32_t parse_bits(const ParseWin& W, const DecCall& C, uint64_t pos, uint32_t n,
+                                               bool& ok) {
+    if (n == 0) return 0;
+    if (((pos + n - 1) >> 5) >= C.nwords) {
+        ok = false;
+        return 0;
+    }
+    const uint64_t wi = pos >> 5;
+    const uint32_t o = (uint32_t)(pos & 31);
+    uint64_t win = (uint64_t)parse_word(W, C, wi) << 32;
+    if (o + n > 32) win |= parse_word(W, C, wi + 1);
+    return (uint32_t)((win << o) >> (64 - n));
+}
+__device__ __forceinline__ uint64_t arena_words16(uint64_t words) { return (words + 15) & ~uint64_t(15); }
+
+__global__ __launch_bounds__(64) void k_dec_parse(const DecCall* calls, uint32_t ncalls, const DecItem* items,
+                                                  DecSeq* seqs, CoderParams p, int symSize, uint32_t* scratch,
+                                                  uint64_t cap, CoderState* st, DecCallOut* outc, int index_mode) {
+    const uint32_t c = blockIdx.x;
     if (c >= ncalls) return;
+    const int lane = threadIdx.x;
     const DecCall C = calls[c];
     const uint32_t cb = p.count_bits, sbf = p.symbits_bits;
     uint64_t w = 0;
     uint32_t err = 0;
-    uint32_t i = C.first_item;
-    for (; i < C.first_item + C.nitems && !err; i++) {
+    ParseWin W;
+    // the batch of sequences waiting for their arena ranges: lane k holds the k-th one's sizes
+    uint32_t kb = 0, my_seq = 0;
+    uint64_t my_boff = 0, my_lut = 0, my_out = 0;
+    auto place = [&]() {
+        const uint64_t mine = (uint32_t)lane < kb ? my_boff + my_lut + my_out : 0ull;
+        const uint64_t incl = wave_incl_scan_u64(mine);
+        const uint64_t total = dpp::last_lane64(incl);
+        uint64_t base = 0;
+        if (lane == 0) base = atomicAdd(&st->scratch_used, (unsigned long long)total);
+        base = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(base >> 32), 0) << 32) |
+               (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)base, 0);
+        if (base + total > cap) {
+            err = kDecScratch;
+            if (lane == 0) atomicCAS(&st->err, 0u, (uint32_t)kDecScratch);
+        } else if ((uint32_t)lane < kb) {
+            const uint64_t o = base + incl - mine;
+            DecSeq& s = seqs[my_seq];
+            s.boff_off = o;
+            s.lut_off = o + my_boff;
+            if (my_out) s.out = scratch + o + my_boff + my_lut;
+        }
+        kb = 0;
+    };
+    for (uint32_t i = C.first_item; i < C.first_item + C.nitems && !err; i++) {
         const DecItem it = items[i];
         uint32_t count = it.n;
         if (it.rl) {
             if (w >= C.nwords) { err = kDecCorrupt; break; }
-            count = C.words[w++];
+            parse_load(W, C, w);
+            count = parse_word(W, C, w);
+            w++;
             if (count > it.n) { err = kDecCorrupt; break; }
         }
         for (uint32_t q = 0; q < (it.rl ? 2u : 1u); q++) {
-            DecSeq& s = seqs[it.seq + q];
             bool ok = true;
             const uint64_t hb = w * 32;
-            const uint32_t nlen = rd_bits(C.words, C.nwords, hb, 5, ok);
-            const uint32_t nsym = rd_bits(C.words, C.nwords, hb + 5 + (uint64_t)cb * nlen, cb, ok);
-            uint32_t sb = rd_bits(C.words, C.nwords, hb + 5 + (uint64_t)cb * (nlen + 1), sbf, ok);
+            if (q == 1 || !it.rl) parse_load(W, C, w);  // the first sequence of an RLE item: loaded at its count word
+            const uint32_t nlen = parse_bits(W, C, hb, 5, ok);
+            const uint32_t nsym = parse_bits(W, C, hb + 5 + (uint64_t)cb * nlen, cb, ok);
+            uint32_t sb = parse_bits(W, C, hb + 5 + (uint64_t)cb * (nlen + 1), sbf, ok);
             if (p.dialect == CC_DIALECT_GPU && sbf == 4 && sb == 0 && nsym > 1) sb = 16;
             const uint64_t sym_bit = hb + 5 + (uint64_t)cb * (nlen + 1) + sbf;
             const uint64_t tb = 5 + (uint64_t)cb * (nlen + 1) + sbf + (uint64_t)sb * nsym;
             w = (hb + tb + 31) / 32;
             if (!ok || sb > 32 || w >= C.nwords || (count > 0 && nsym == 0) || nsym > count) { err = kDecCorrupt; break; }
-            const uint32_t bits = C.words[w++];
+            const uint32_t bits = parse_word(W, C, w);
+            w++;
             const uint64_t words_off = w;
             const uint64_t nwc = ((uint64_t)bits + 31) / 32;
             w += nwc;
@@ -96,35 +158,39 @@ __global__ void k_dec_parse(const DecCall* calls, uint32_t ncalls, const DecItem
             const uint64_t nblk = ((uint64_t)count + p.cbs - 1) / p.cbs;
             w += (nblk + 1) / 2;
             if (w > C.nwords) { err = kDecCorrupt; break; }
-            bool aok = true;
-            const uint64_t boff = arena_take(st, nblk ? nblk : 1, cap, aok);
-            const uint64_t lut = arena_take(st, (1u << kLut) + (uint64_t)nsym + 1, cap, aok);
-            void* out = it.out;
-            if (it.rl && !index_mode) {
-                const uint64_t o = arena_take(st, ((uint64_t)count * symSize + 3) / 4 + 1, cap, aok);
-                out = scratch + o;
+            const uint32_t sq = it.seq + q;
+            if (lane == 0) {
+                DecSeq& s = seqs[sq];
+                s.base = C.words;
+                s.nwords = C.nwords;
+                s.hdr_bit = hb;
+                s.sym_bit = sym_bit;
+                s.words_off = words_off;
+                s.incs_off = incs_off;
+                if (!it.rl || index_mode) s.out = it.out;  // else the arena buffer, placed below
+                s.nlen = nlen;
+                s.nsym = nsym;
+                s.sb = sb;
+                s.count = count;
+                s.nwords_cw = (uint32_t)nwc;
+                s.err = 0;
+                s.item = i;
             }
-            if (!aok) { err = kDecScratch; break; }
-            s.base = C.words;
-            s.nwords = C.nwords;
-            s.hdr_bit = hb;
-            s.sym_bit = sym_bit;
-            s.words_off = words_off;
-            s.incs_off = incs_off;
-            s.boff_off = boff;
-            s.lut_off = lut;
-            s.out = out;
-            s.nlen = nlen;
-            s.nsym = nsym;
-            s.sb = sb;
-            s.count = count;
-            s.nwords_cw = (uint32_t)nwc;
-            s.err = 0;
-            s.item = i;
+            if ((uint32_t)lane == kb) {
+                my_seq = sq;
+                my_boff = arena_words16(nblk ? nblk : 1);
+                my_lut = arena_words16((1u << kLut) + (uint64_t)nsym + 1);
+                my_out = it.rl && !index_mode ? arena_words16(((uint64_t)count * symSize + 3) / 4 + 1) : 0ull;
+            }
+            if (++kb == 64) {
+                place();
+                if (err) break;
+            }
         }
     }
+    if (!err && kb) place();
     if (err) {  // mark every sequence of the call failed
-        for (uint32_t k = C.first_item; k < C.first_item + C.nitems; k++) {
+        for (uint32_t k = C.first_item + lane; k < C.first_item + C.nitems; k += 64) {
             const DecItem it = items[k];
             seqs[it.seq].err = err;
             seqs[it.seq].item = k;
@@ -133,11 +199,13 @@ __global__ void k_dec_parse(const DecCall* calls, uint32_t ncalls, const DecItem
                 seqs[it.seq + 1].item = k;
             }
         }
-        atomicCAS(&st->err, 0u, err);
+        if (lane == 0) atomicCAS(&st->err, 0u, err);
     }
-    outc[c].words = w;
-    outc[c].err = err;
-    outc[c].pad = 0;
+    if (lane == 0) {
+        outc[c].words = w;
+        outc[c].err = err;
+        outc[c].pad = 0;
+    }
 }
 
 // One workgroup per coded sequence: first/min/max codeword per length (closed form of the
@@ -746,7 +814,7 @@ void coder_decode(const DecCall* d_calls, uint32_t ncalls, const DecItem* d_item
     CCA_LAUNCH("dec_init", 32, k_dec_init, dim3(1), dim3(64), 0, st, d_state);
     CCA_LAUNCH_CHECK();
     if (ncalls == 0) return;
-    CCA_LAUNCH("dec_parse", 0, k_dec_parse, dim3(div_up(ncalls, 64)), dim3(64), 0, st, d_calls, ncalls, d_items, d_seqs,
+    CCA_LAUNCH("dec_parse", 0, k_dec_parse, dim3(ncalls), dim3(64), 0, st, d_calls, ncalls, d_items, d_seqs,
                p, symSize, scratch, scratch_cap, d_state, d_out, 0);
     CCA_LAUNCH_CHECK();
     if (nseq == 0) return;
@@ -784,7 +852,7 @@ void coder_decode_index(const DecCall* d_calls, uint32_t ncalls, const DecItem* 
     CCA_LAUNCH("dec_init", 32, k_dec_init, dim3(1), dim3(64), 0, st, d_state);
     CCA_LAUNCH_CHECK();
     if (ncalls == 0) return;
-    CCA_LAUNCH("dec_parse", 0, k_dec_parse, dim3(div_up(ncalls, 64)), dim3(64), 0, st, d_calls, ncalls, d_items, d_seqs,
+    CCA_LAUNCH("dec_parse", 0, k_dec_parse, dim3(ncalls), dim3(64), 0, st, d_calls, ncalls, d_items, d_seqs,
                p, symSize, scratch, scratch_cap, d_state, d_out, 1);
     CCA_LAUNCH_CHECK();
     if (nseq == 0) return;
