@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving run after the bricks run")
     ap.add_argument("--serving-n", type=int, default=1024, help="side of the serving field")
     ap.add_argument("--serving-slots", type=int, default=16)
+    ap.add_argument("--serving-cbs", type=int, default=0,
+                    help="coding block size of the served bricks (0: the tuning table's)")
     ap.add_argument("--serving-requests", type=int, default=1000, help="requests replayed per arrival rate")
     ap.add_argument("--serving-rates", default="1000,5000,10000,20000,50000", help="arrival rates, bricks/s")
     ap.add_argument("--bricks", type=int, default=2048)
@@ -581,7 +583,8 @@ def run_serving(args, torch, cc, lib):
     shapes = [tuple(b.shape) for b in bricks]
     steps = [0.005 * std0] * n0 + [0.005 * std1] * len(lod)
     nvox = sum(b.numel() for b in bricks)
-    inst = cc.createInstance(0, 7, 128 * 128 * 128 // 8, 128, 14)
+    cbs = args.serving_cbs or int(cc.loadTuning()["codingBlockSize"])
+    inst = cc.createInstance(0, 7, 128 * 128 * 128 // 8, cbs, 14)
     out = torch.empty(nvox, dtype=torch.int32, device=dev)
     # compressed in batches of 243 bricks (at most ~5.1e8 voxels per batch) into one word buffer
     offs = np.zeros(len(bricks) + 1, dtype=np.uint64)
@@ -654,8 +657,8 @@ def run_serving(args, torch, cc, lib):
     return {"workload": "C5 serving: %d^3 k^-5/3 field (seed 5), %d level-0 128^3 bricks (stride 120, last origin "
                         "clamped) + %d 64^3 LOD bricks of the 2x box-downsampled field, 2-level 9/7, step "
                         "0.005*std per level, resident in HBM; Poisson arrivals, requests of U{1..64} bricks, "
-                        "each brick LOD with p=1/2; %d slots, 32 GB decode-workspace budget"
-                        % (args.serving_n, n0, n0, args.serving_slots),
+                        "each brick LOD with p=1/2; coding block %d; %d slots, 32 GB decode-workspace budget"
+                        % (args.serving_n, n0, n0, cbs, args.serving_slots),
             "latency": "request arrival to completion of the batch holding its last brick, observed by the host loop",
             "admission": "brick by brick in arrival order, batches up to the slot's share of the workspace budget",
             "compressed_bytes": compressed, "raw_bytes": 4 * nvox, "compression_ratio": round(4 * nvox / compressed, 3),
