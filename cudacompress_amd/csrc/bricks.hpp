@@ -31,13 +31,15 @@ struct BrickLevel {
 };
 
 constexpr int kBrickSP = 32;  // pairs per tile along the pass axis
+constexpr int kBrickZP = 32;  // output pairs per column segment of the streaming z passes
 constexpr int kBrickCols = 64;
 
 // tiles of one brick for a pass along axis ax (0 x, 1 y, 2 z)
 inline uint32_t brick_tiles(int ax, int lx, int ly, int lz) {
     const int n = ax == 0 ? lx : (ax == 1 ? ly : lz);
     const uint64_t cols = ax == 0 ? (uint64_t)ly * lz : (ax == 1 ? (uint64_t)lx * lz : (uint64_t)lx * ly);
-    return (uint32_t)(((cols + kBrickCols - 1) / kBrickCols) * ((n / 2 + kBrickSP - 1) / kBrickSP));
+    const int sp = ax == 2 ? kBrickZP : kBrickSP;
+    return (uint32_t)(((cols + kBrickCols - 1) / kBrickCols) * ((n / 2 + sp - 1) / sp));
 }
 
 // Fused x+y passes, streaming along y: a workgroup of kBrickXYThreads threads serves zk z-slices

@@ -273,16 +273,16 @@ __global__ __launch_bounds__(256) void k_bfwd(const BrickLevel* lv, const uint32
                  [&](int, int, int off, float v) { gst(dst, off, v); });
 }
 
-// Z passes: one thread per column (x, y) of the brick level and per segment of SP output pairs
-// along z.  The 10-sample filter window streams through registers (no LDS, no barrier); a
+// Z passes: one thread per column (x, y) of the brick level and per segment of kBrickZP output
+// pairs along z.  The 10-sample filter window streams through registers (no LDS, no barrier); a
 // segment reads its z range once plus the filter halo.  A workgroup is one wave of NC columns,
 // numbered band-major (the columns of octant quarter bx + 2 by of the xy plane, then y, then x
 // inside it), so the lanes of a wave read and write consecutive elements of one band (and, for a
 // level whose quarter holds a multiple of NC columns, all serve one quarter: the lowpass quarter's
-// loads and stores differ from the highpass ones).  Tiles (NC columns x SP pairs) are those of
-// brick_tiles along z.  The window and the fmaf order are those of the tiled passes
-// (Fwd97::pair over positions 2q - 4 .. 2q + 5, Inv97::out over interleaved pairs u - 2 .. u + 2),
-// so the results are identical.
+// loads and stores differ from the highpass ones).  Tiles (NC columns x kBrickZP pairs) are those
+// of brick_tiles along z.  The window and the fmaf order are those of the tiled passes (Fwd97::pair
+// over positions 2q - 4 .. 2q + 5, Inv97::out over interleaved pairs u - 2 .. u + 2), so the
+// results are identical.
 // Workgroups are renumbered XCD by XCD (xcd_order): the waves of the four quarters of the same rows
 // share their lines of the dense plane, and neighbours in logical order then share an L2.
 constexpr int ZU = 8;  // output pairs per batch of loads (2 ZU loads in flight per thread)
@@ -296,8 +296,8 @@ struct ZColumn {
     __device__ ZColumn(const Geo<2>& g, uint32_t local, int hx, int hy) {
         const int ctiles = (g.ncols + NC - 1) / NC;
         const int c = (int)(local % (uint32_t)ctiles) * NC + (int)threadIdx.x;
-        k0 = (int)(local / (uint32_t)ctiles) * SP;
-        k1 = min(k0 + SP, g.m);
+        k0 = (int)(local / (uint32_t)ctiles) * kBrickZP;
+        k1 = min(k0 + kBrickZP, g.m);
         const int hxy = hx * hy;
         q = c < g.ncols ? c / hxy : -1;
         const int rr = c - q * hxy, yq = rr / hx, xq = rr - yq * hx;
