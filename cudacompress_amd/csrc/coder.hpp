@@ -92,8 +92,8 @@ struct CallOut {              // per coder call
 
 // ---- decoder ---------------------------------------------------------------------------------
 // A decode job: calls, each a word-aligned bitstream segment in device memory holding the items
-// of one encodeRLHuff/encodeHuff call in order.  parse (one thread per call: the section sizes
-// are in-band, so a call is walked sequentially) -> tables (one workgroup per coded sequence:
+// of one encodeRLHuff/encodeHuff call in order.  parse (one wave per call: the section sizes
+// are in-band, so a call is walked sequentially, each section head read as one window) -> tables (one workgroup per coded sequence:
 // first/min/max per length, absolute block offsets) -> blocks (four waves per 256 coding blocks,
 // one LDS lookup table; the zero counts' chunk spans) -> rle (one workgroup per chunk of 2048
 // pairs: zero runs expanded into the output).
