@@ -100,6 +100,43 @@ void finish_brick(cc_server* S, uint32_t r, uint32_t err, double now) {
     S->live_reqs--;
 }
 
+// Every slot's buffers sized up front for the largest batches admission can form: the slot's
+// budget filled with copies of the largest brick, and with copies of the smallest (the most
+// bricks; at most kWarmBricks).  A buffer that grew while serving would free its old allocation,
+// and hipFree waits for the whole device, every other slot's batch with it.  Best effort: a warm
+// batch that does not fit the device leaves the slot to grow on demand.
+constexpr uint64_t kWarmBricks = 4096;
+void warm_slots(cc_server* S) {
+    const uint32_t n = (uint32_t)S->begin.size();
+    if (n == 0) return;
+    uint32_t lo = 0, hi = 0;
+    for (uint32_t i = 1; i < n; i++) {
+        const uint64_t e = elems3(&S->dims[3 * i]);
+        if (e < elems3(&S->dims[3 * lo])) lo = i;
+        if (e > elems3(&S->dims[3 * hi])) hi = i;
+    }
+    for (const uint32_t pick : {hi, lo}) {
+        const uint32_t* d = &S->dims[3 * pick];
+        const uint64_t w = brick_workspace(elems3(d), S->symSize);
+        const uint32_t k = (uint32_t)std::min<uint64_t>(kWarmBricks, std::max<uint64_t>(1, S->slot_budget / w));
+        const std::vector<uint32_t> ids(k, pick);
+        for (Slot& s : S->slots) {
+            BrickJob job;
+            job.L = S->levels;
+            job.symSize = S->symSize;
+            // destinations are recorded, never written: nothing is launched
+            job.bricks.assign(k, cc_brick{nullptr, reinterpret_cast<float*>(const_cast<uint32_t*>(S->words)), d[0], d[1],
+                                          d[2], S->steps[pick]});
+            try {
+                bricks_prepare_bound(S->inst, s.ctx, job, S->index, ids, s.st);
+            } catch (const Error& e) {
+                if (e.status != CC_ERR_OUT_OF_MEMORY) throw;
+            }
+            CCA_HIP(hipStreamSynchronize(s.st));
+        }
+    }
+}
+
 void reap(cc_server* S, Slot& s, double now) {
     if (!s.busy) return;
     const hipError_t q = hipEventQuery(s.done);
@@ -191,6 +228,7 @@ cc_server* server_create(cc_instance* I, const uint32_t* d_words, uint32_t nbric
         calls[i] = CallWords{d_words + S->begin[i], S->end[i] - S->begin[i]};
     }
     bricks_index_build(I, S->index, shapes, levels, symSize, calls, S->slots[0].st);
+    warm_slots(S.get());
     return S.release();
 }
 
