@@ -503,57 +503,77 @@ __global__ __launch_bounds__(64 * W) void k_dec_blocks(const DecSeq* seqs, const
         if (sizeof(S) == 2) v[i >> 1] = (i & 1) ? (v[i >> 1] | (sym << 16)) : (sym & 0xFFFFu);
         else v[i] = sym;
     };
-    auto rounds = [&](auto zflag) -> uint32_t {
+    // one round: the symbols of positions [r, r + kDecRound) of the lane's block into v; every
+    // lane steps through the whole round, steps past its block's end are not stored
+    auto one_round = [&](uint32_t r, uint32_t (&v)[NW], uint32_t& zs, auto zflag) {
         constexpr bool Z = decltype(zflag)::value;
-        uint32_t zs = 0;
-        for (uint32_t r = 0; r < cbs; r += kDecRound) {
-            // every lane steps through the whole round; steps past its block's end store nothing
-            const uint32_t p0 = b * cbs + r;
-            const uint32_t vlen = bend_l > p0 ? min(bend_l - p0, kDecRound) : 0u;  // symbols of the block here
-            uint32_t v[NW];
-            if (!multi) {
+        const uint32_t p0 = b * cbs + r;
+        const uint32_t vlen = bend_l > p0 ? min(bend_l - p0, kDecRound) : 0u;  // symbols of the block here
+        if (!multi) {
 #pragma unroll
-                for (uint32_t i = 0; i < kDecRound; i++) put(v, i, single);
-                if constexpr (Z) zs += (single + 1) * vlen;
+            for (uint32_t i = 0; i < kDecRound; i++) put(v, i, single);
+            if constexpr (Z) zs += (single + 1) * vlen;
+        } else {
+            load_window();
+            if (run0) {
+                // positions taken at a lane's own pace: its LDS row, prefilled with the run symbol
+                const uint32_t fill = sizeof(S) == 2 ? (sym0 & 0xFFFFu) * 0x10001u : sym0;
+#pragma unroll
+                for (uint32_t k = 0; k < NW; k++) reinterpret_cast<u32a*>(run_row)[k] = fill;
+                uint32_t i = 0;
+                while (i < kDecRound) {
+                    refill();
+                    const uint32_t pk = __builtin_amdgcn_alignbit(hi, lo, (uint32_t)q);
+                    const bool run = (pk >> 31) == 0u;
+                    uint32_t e = lut[pk >> sh];
+                    if (!run && e == 0u) e = slow(pk);
+                    const uint32_t k = min(pk ? (uint32_t)__clz(pk) : 32u, kDecRound - i);
+                    const uint32_t sym = e & 0xFFFFFFu;
+                    if (!run) run_row[i] = (S)sym;
+                    if constexpr (Z)
+                        zs += run ? (sym0 + 1) * (min(i + k, vlen) - min(i, vlen)) : (i < vlen ? sym + 1 : 0u);
+                    q -= (int)(run ? k : (e >> 24));
+                    i += run ? k : 1u;
+                }
+#pragma unroll
+                for (uint32_t k = 0; k < NW; k++) v[k] = reinterpret_cast<const u32a*>(run_row)[k];
             } else {
-                load_window();
-                if (run0) {
-                    // positions taken at a lane's own pace: its LDS row, prefilled with the run symbol
-                    const uint32_t fill = sizeof(S) == 2 ? (sym0 & 0xFFFFu) * 0x10001u : sym0;
 #pragma unroll
-                    for (uint32_t k = 0; k < NW; k++) reinterpret_cast<u32a*>(run_row)[k] = fill;
-                    uint32_t i = 0;
-                    while (i < kDecRound) {
-                        refill();
-                        const uint32_t pk = __builtin_amdgcn_alignbit(hi, lo, (uint32_t)q);
-                        const bool run = (pk >> 31) == 0u;
-                        uint32_t e = lut[pk >> sh];
-                        if (!run && e == 0u) e = slow(pk);
-                        const uint32_t k = min(pk ? (uint32_t)__clz(pk) : 32u, kDecRound - i);
-                        const uint32_t sym = e & 0xFFFFFFu;
-                        if (!run) run_row[i] = (S)sym;
-                        if constexpr (Z)
-                            zs += run ? (sym0 + 1) * (min(i + k, vlen) - min(i, vlen)) : (i < vlen ? sym + 1 : 0u);
-                        q -= (int)(run ? k : (e >> 24));
-                        i += run ? k : 1u;
-                    }
-#pragma unroll
-                    for (uint32_t k = 0; k < NW; k++) v[k] = reinterpret_cast<const u32a*>(run_row)[k];
-                } else {
-#pragma unroll
-                    for (uint32_t i = 0; i < kDecRound; i++) {
-                        refill();
-                        const uint32_t pk = __builtin_amdgcn_alignbit(hi, lo, (uint32_t)q);
-                        uint32_t e = lut[pk >> sh];
-                        if (e == 0u) e = slow(pk);
-                        q -= (int)(e >> 24);
-                        const uint32_t sym = e & 0xFFFFFFu;
-                        put(v, i, sym);
-                        if constexpr (Z) zs += i < vlen ? sym + 1 : 0u;
-                    }
+                for (uint32_t i = 0; i < kDecRound; i++) {
+                    refill();
+                    const uint32_t pk = __builtin_amdgcn_alignbit(hi, lo, (uint32_t)q);
+                    uint32_t e = lut[pk >> sh];
+                    if (e == 0u) e = slow(pk);
+                    q -= (int)(e >> 24);
+                    const uint32_t sym = e & 0xFFFFFFu;
+                    put(v, i, sym);
+                    if constexpr (Z) zs += i < vlen ? sym + 1 : 0u;
                 }
             }
-            store_round(p0, vlen, v);
+        }
+    };
+    auto round_len = [&](uint32_t r) {
+        const uint32_t p0 = b * cbs + r;
+        return bend_l > p0 ? min(bend_l - p0, kDecRound) : 0u;
+    };
+    // A 64-symbol coding block is held in registers for all its rounds and leaves as back-to-back
+    // stores, so that its output lines are whole when they leave L2 (a round at a time, a lane's
+    // partial lines were written out 1.6 times over); other sizes store round by round.
+    constexpr uint32_t kHeld = 64 / kDecRound;
+    auto rounds = [&](auto zflag) -> uint32_t {
+        uint32_t zs = 0;
+        if (cbs == kHeld * kDecRound) {
+            uint32_t v[kHeld][NW];
+#pragma unroll
+            for (uint32_t h = 0; h < kHeld; h++) one_round(h * kDecRound, v[h], zs, zflag);
+#pragma unroll
+            for (uint32_t h = 0; h < kHeld; h++) store_round(b * cbs + h * kDecRound, round_len(h * kDecRound), v[h]);
+        } else {
+            for (uint32_t r = 0; r < cbs; r += kDecRound) {
+                uint32_t v[NW];
+                one_round(r, v, zs, zflag);
+                store_round(b * cbs + r, round_len(r), v);
+            }
         }
         return zs;
     };
