@@ -179,13 +179,16 @@ struct ConcatGroup {
 // Copies the jobs' bitstreams back to back into dst (capacity dst_cap words), in group order.
 void coder_concat(const ConcatGroup* d_groups, uint32_t ngroups, uint32_t* dst, uint64_t dst_cap, uint64_t bytes,
                   hipStream_t st);
-// d_classes: kDesignListBase + kDesignLists * nseq words of work lists.
+// d_classes: kDesignListBase + kDesignLists * nseq words of work lists.  d_item_order: workgroup
+// k of the stats and pack passes (one per item, each a serial loop over its chunks) serves item
+// d_item_order[k] (null: item k); largest items first, so the longest loops start first.
 void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uint32_t nseq, const CoderParams& p,
                   int symSize, uint32_t* scratch, uint64_t scratch_cap, CoderState* d_state, uint32_t* d_classes,
                   CallOut* d_calls,
                   uint32_t ncalls, const uint32_t* d_call_first_item, uint32_t* d_out, uint64_t out_cap,
                   uint64_t total_symbols, hipStream_t st, const SideStreams* side,
-                  const LayoutLink* link = nullptr, const uint32_t* design_grid = nullptr, uint32_t lane_k = 0);
+                  const LayoutLink* link = nullptr, const uint32_t* design_grid = nullptr, uint32_t lane_k = 0,
+                  const uint32_t* d_item_order = nullptr);
 // wave_seq: for every block-decoder workgroup the coded sequence it serves; wave_first: that
 // workgroup's first block.  The first nlong entries are workgroups of kDecGroupBlocks blocks of
 // the long sequences (dec_long of the block-count bound: RLE sequences have at most n pairs), the

@@ -327,14 +327,16 @@ __device__ void write_leaves(const uint32_t* h, uint32_t nb, uint2* out, uint64_
 // marked wide and get a zeroed global histogram for k_code_stats_wide.
 template <class S>
 __global__ __launch_bounds__(kCoderThreads) void k_code_stats(const CodeItem* items, SeqInfo* seqs, CoderParams p,
-                                                              uint32_t* scratch, uint64_t cap, CoderState* st) {
+                                                              uint32_t* scratch, uint64_t cap, CoderState* st,
+                                                              const uint32_t* order) {
     __shared__ uint32_t hs[kStatBins];
     __shared__ uint32_t hz[256];
     __shared__ int shl[2][4];  // alternate chunks: one barrier per chunk
     __shared__ uint32_t shu[4];
     __shared__ uint64_t sh_off[2];
     __shared__ uint64_t shq[4];
-    const CodeItem it = items[blockIdx.x];
+    const uint32_t ii = order ? order[blockIdx.x] : blockIdx.x;
+    const CodeItem it = items[ii];
     const S* in = static_cast<const S*>(it.in);
     const bool rl = it.rl != 0;
     const uint32_t n = it.n;
@@ -431,7 +433,7 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_stats(const CodeItem* it
         ss->err = kErrNone;
         ss->quirk = 0;
         ss->wide = wide;
-        ss->item = blockIdx.x;
+        ss->item = ii;
         ss->bits = 0;
         ss->last_nz1 = (uint64_t)(carry + 1);  // last nonzero + 1, read by the pack pass
         ss->out_off = 0;
@@ -445,7 +447,7 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_stats(const CodeItem* it
             sz->err = kErrNone;
             sz->quirk = 0;
             sz->wide = 0;
-            sz->item = blockIdx.x;
+            sz->item = ii;
             sz->bits = 0;
             sz->out_off = 0;
             sz->last_nz1 = 0;
@@ -938,7 +940,7 @@ __device__ uint32_t header_word(const SeqInfo& s, const uint32_t* cpl, const Cod
 template <class S>
 __global__ __launch_bounds__(kCoderThreads) void k_code_pack(const CodeItem* items, SeqInfo* seqs, CoderParams p,
                                                              uint32_t* scratch, const CoderState* st, uint32_t* out,
-                                                             uint64_t out_cap) {
+                                                             uint64_t out_cap, const uint32_t* order) {
     __shared__ uint32_t cs[kStatBins];
     __shared__ unsigned char lsh[kStatBins];
     __shared__ uint32_t cz[256];
@@ -947,7 +949,7 @@ __global__ __launch_bounds__(kCoderThreads) void k_code_pack(const CodeItem* ite
     __shared__ int shl[4];
     __shared__ uint64_t shq[4];
     if (st->out_base + st->out_words > out_cap) return;  // the host reports the short output
-    const CodeItem it = items[blockIdx.x];
+    const CodeItem it = items[order ? order[blockIdx.x] : blockIdx.x];
     const SeqInfo ss = seqs[it.seq];
     const bool rl = it.rl != 0;
     if (ss.err || (rl && seqs[it.seq + 1].err)) return;
@@ -1328,7 +1330,7 @@ void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uin
                   CallOut* d_calls,
                   uint32_t ncalls, const uint32_t* d_call_first_item, uint32_t* d_out, uint64_t out_cap,
                   uint64_t total_symbols, hipStream_t st, const SideStreams* side, const LayoutLink* link,
-                  const uint32_t* design_grid, uint32_t lane_k) {
+                  const uint32_t* design_grid, uint32_t lane_k, const uint32_t* d_item_order) {
     CCA_REQUIRE(symSize == 2 || symSize == 4, CC_ERR_INVALID_ARG, "symbol size must be 2 or 4");
     uint64_t* item_off = reinterpret_cast<uint64_t*>(scratch + scratch_cap);  // caller reserves the tail
     CCA_LAUNCH("coder_init", 32, k_coder_init, dim3(1), dim3(64), 0, st, d_state, d_classes);
@@ -1337,13 +1339,13 @@ void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uin
     const uint64_t inb = total_symbols * symSize;
     if (symSize == 2) {
         CCA_LAUNCH("code_stats", inb, k_code_stats<uint16_t>, dim3(nitems), dim3(kCoderThreads), 0, st, d_items, d_seqs,
-                   p, scratch, scratch_cap, d_state);
+                   p, scratch, scratch_cap, d_state, d_item_order);
         CCA_LAUNCH_CHECK();
         CCA_LAUNCH("code_stats_wide", 0, k_code_stats_wide<uint16_t>, dim3(nitems), dim3(kCoderThreads), 0, st,
                    d_items, d_seqs, p, scratch);
     } else {
         CCA_LAUNCH("code_stats", inb, k_code_stats<uint32_t>, dim3(nitems), dim3(kCoderThreads), 0, st, d_items, d_seqs,
-                   p, scratch, scratch_cap, d_state);
+                   p, scratch, scratch_cap, d_state, d_item_order);
         CCA_LAUNCH_CHECK();
         CCA_LAUNCH("code_stats_wide", 0, k_code_stats_wide<uint32_t>, dim3(nitems), dim3(kCoderThreads), 0, st,
                    d_items, d_seqs, p, scratch);
@@ -1414,10 +1416,10 @@ void coder_encode(const CodeItem* d_items, uint32_t nitems, SeqInfo* d_seqs, uin
     if (link && link->signal) CCA_HIP(hipEventRecord(link->signal, st));
     if (symSize == 2)
         CCA_LAUNCH("code_pack", inb, k_code_pack<uint16_t>, dim3(nitems), dim3(kCoderThreads), 0, st, d_items, d_seqs,
-                   p, scratch, d_state, d_out, out_cap);
+                   p, scratch, d_state, d_out, out_cap, d_item_order);
     else
         CCA_LAUNCH("code_pack", inb, k_code_pack<uint32_t>, dim3(nitems), dim3(kCoderThreads), 0, st, d_items, d_seqs,
-                   p, scratch, d_state, d_out, out_cap);
+                   p, scratch, d_state, d_out, out_cap, d_item_order);
     CCA_LAUNCH_CHECK();
 }
 

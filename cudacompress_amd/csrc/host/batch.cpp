@@ -60,7 +60,7 @@ void encode_prepare(cc_instance* I, DeviceCoder& D, const std::vector<ItemSpec>&
     check_symbols(I, symSize);
     const uint32_t nitems = (uint32_t)items.size();
     const size_t o_first = align_up(sizeof(CodeItem) * std::max<uint32_t>(nitems, 1), 256);
-    const size_t stage_bytes = o_first + sizeof(uint32_t) * (ncalls + 1);
+    const size_t stage_bytes = o_first + sizeof(uint32_t) * (ncalls + 1 + nitems);
     unsigned char* h = (unsigned char*)D.h_stage.get(stage_bytes);
     CodeItem* hi = (CodeItem*)h;
     uint32_t* hf = (uint32_t*)(h + o_first);
@@ -77,6 +77,10 @@ void encode_prepare(cc_instance* I, DeviceCoder& D, const std::vector<ItemSpec>&
         total_syms += it.n;
     }
     while (call < ncalls) hf[++call] = nitems;
+    // the stats and pack passes take the items largest first (their serial chunk loops start first)
+    uint32_t* ho = hf + ncalls + 1;
+    for (uint32_t i = 0; i < nitems; i++) ho[i] = i;
+    std::stable_sort(ho, ho + nitems, [&](uint32_t a, uint32_t b) { return items[a].n > items[b].n; });
     job.nitems = nitems;
     job.nseq = seq;
     job.ncalls = ncalls;
@@ -86,14 +90,14 @@ void encode_prepare(cc_instance* I, DeviceCoder& D, const std::vector<ItemSpec>&
     for (int c = 0; c < 4; c++) job.design_grid[c] = I->tuning.designGrid[c];
     job.lane_k = I->tuning.laneDesignK;
     D.d_items.as<CodeItem>(std::max<uint32_t>(nitems, 1));
-    D.d_call_first.as<uint32_t>(ncalls + 1);
+    D.d_call_first.as<uint32_t>(ncalls + 1 + std::max<uint32_t>(nitems, 1));
     D.d_seqs.as<SeqInfo>(std::max<uint32_t>(job.nseq, 1));
     D.d_state.as<CoderState>(1);
     D.d_classes.as<uint32_t>(kDesignListBase + (uint64_t)kDesignLists * std::max<uint32_t>(job.nseq, 1));
     D.d_calls.as<CallOut>(std::max<uint32_t>(ncalls, 1));
     D.h_result.get(sizeof(CoderState) + sizeof(CallOut) * std::max<uint32_t>(ncalls, 1));
     CCA_HIP(hipMemcpyAsync(D.d_items.p, hi, sizeof(CodeItem) * nitems, hipMemcpyHostToDevice, st));
-    CCA_HIP(hipMemcpyAsync(D.d_call_first.p, hf, sizeof(uint32_t) * (ncalls + 1), hipMemcpyHostToDevice, st));
+    CCA_HIP(hipMemcpyAsync(D.d_call_first.p, hf, sizeof(uint32_t) * (ncalls + 1 + nitems), hipMemcpyHostToDevice, st));
     encode_reserve(D, job, std::max<uint64_t>(D.scratch_cap, arena_estimate(items, I->log2, I->cbs)));
     SideStreams ss;
     side_streams(D, ss);
@@ -114,7 +118,8 @@ void encode_launch(DeviceCoder& D, const EncodeJob& job, uint32_t* d_out, uint64
                  (uint32_t*)D.d_scratch.p, job.scratch_cap, (CoderState*)D.d_state.p, (uint32_t*)D.d_classes.p,
                  (CallOut*)D.d_calls.p,
                  job.ncalls, (const uint32_t*)D.d_call_first.p, d_out, out_cap, job.total_syms, st,
-                 side_streams(D, ss), link, job.design_grid, job.lane_k);
+                 side_streams(D, ss), link, job.design_grid, job.lane_k,
+                 (const uint32_t*)D.d_call_first.p + job.ncalls + 1);
 }
 
 CoderState encode_finish(DeviceCoder& D, const EncodeJob& job, std::vector<CallOut>* results, hipStream_t st) {
