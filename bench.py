@@ -773,6 +773,18 @@ def main():
         roof["algorithmic_bytes_per_launch"] = int(per)
         roof["achieved"] = round(per / (roof["avg_launch_us"] * 1e-6) / 1e9, 2)
         roof["frac"] = round(roof["achieved"] / HBM_PEAK_GBS, 5)
+    if roof["kernel"] == "dec_blocks" and cfg.get("coding"):
+        # the block decode reads the codewords (and block offsets) and writes every decoded pair of
+        # the RLE sequences (symbols and zero counts) and the plain sequences' symbols, 16-bit each:
+        # C + 2 (2M + N_H) per step, over the time of both of its launches (long and short sequences)
+        cd = cfg["coding"]
+        per = (cd["C"] + 2 * (2 * cd["M"] + cd["N_H"])) / roof["launches_per_step"]
+        short_ms = sum(k[2] for k in kernels if k[0] == "dec_blocks_short")
+        t_us = roof["avg_launch_us"] + short_ms * 1e3 / (roof["launches_per_step"] * args.steps)
+        roof["algorithmic_bytes_per_launch"] = int(per)
+        roof["avg_launch_us_with_short"] = round(t_us, 3)
+        roof["achieved"] = round(per / (t_us * 1e-6) / 1e9, 2)
+        roof["frac"] = round(roof["achieved"] / HBM_PEAK_GBS, 5)
     cfg["gpu_kernel_ms_per_step"] = round(gpu_ms, 4)
     if cfg.get("coding"):
         pb = cfg["coding"]["pipeline_bytes_per_step"]["total"]
