@@ -326,10 +326,10 @@ __global__ __launch_bounds__(256) void k_dec_tables(DecSeq* seqs, CoderParams p,
 // Block decoder: one thread per coding block, W waves per workgroup (W = 4: 256 blocks of one
 // long sequence sharing one LDS copy of its lookup table, which then leaves L2 once per 256
 // blocks; W = 1: a short sequence's blocks, so that a sequence of a few blocks does not hold a
-// four-wave workgroup's LDS).  A lane decodes its block R symbols per round (16 in the long
-// sequences' workgroups: fewer rounds, so fewer copies and waits per block; 8 in the short ones'
-// waves: less LDS per wave) and stores them itself (16 bytes per 8 16-bit symbols); the waves of a
-// workgroup meet only at the table load.
+// four-wave workgroup's LDS).  A lane decodes its block R symbols per round (8 in both kinds:
+// less LDS per wave, so more waves hide the latency of the serial lookups; a 64-symbol block is
+// held in registers across its rounds and stored at once) and stores them itself (16 bytes per 8
+// 16-bit symbols); the waves of a workgroup meet only at the table load.
 // The kernel is bound by instruction issue and latency, so a code costs a handful of vector
 // instructions: the bit window is two words {hi, lo} read with one funnel shift (`alignbit`, the
 // next code starting q bits above lo's lowest bit); codewords reach a lane through an LDS copy of
@@ -804,7 +804,7 @@ void launch_blocks(DecSeq* d_seqs, const uint32_t* d_wave_seq, const uint32_t* d
                    uint32_t nlong, const CoderParams& p, const uint32_t* scratch, const DecItem* d_items,
                    const RleChunks& rc, uint64_t outb, hipStream_t st) {
     if (nlong) {
-        CCA_LAUNCH("dec_blocks", outb, (k_dec_blocks<S, 4, 16>), dim3(nlong), dim3(256), 0, st, d_seqs, d_wave_seq,
+        CCA_LAUNCH("dec_blocks", outb, (k_dec_blocks<S, 4, 8>), dim3(nlong), dim3(256), 0, st, d_seqs, d_wave_seq,
                    d_wave_first, p, scratch, d_items, rc);
         CCA_LAUNCH_CHECK();
     }
