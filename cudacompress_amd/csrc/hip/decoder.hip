@@ -348,7 +348,8 @@ template <class S, uint32_t R>
 constexpr uint32_t run_row_words() { return sizeof(S) == 2 ? R / 2 + 1 : R + 1; }
 
 template <class S, int W, uint32_t R>
-__global__ __launch_bounds__(64 * W) void k_dec_blocks(const DecSeq* seqs, const uint32_t* wave_seq,
+// (at most 72 vector registers: seven waves per SIMD, as the long workgroups' LDS allows)
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(7))) void k_dec_blocks(const DecSeq* seqs, const uint32_t* wave_seq,
                                                        const uint32_t* wave_first, CoderParams p,
                                                        const uint32_t* scratch, const DecItem* items, RleChunks rc) {
     constexpr uint32_t kDecRound = R;
